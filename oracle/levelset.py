@@ -1,0 +1,227 @@
+"""Level-set oracle (FP64 numpy) — TEST INFRASTRUCTURE ONLY, never imported by the product.
+
+Restates, from the reference src/petch/threed.py:
+  init_sdf      make_trench_3d            threed.py:32-59   (skfmm replaced by the exact distance;
+                                                              skfmm's own output: parity unpinned)
+  advect_3d     advect_3d                 threed.py:1248-1259
+  advect_pinned advect_3d_gpu / CPU loop  threed.py:1295-1307, 1761-1763
+  eik_solve     _eik_solve                threed.py:1421-1452
+  eik_init      _eik_init                 threed.py:1455-1479
+  eik_jacobi    _eik_jacobi               threed.py:1482-1512
+  reinit_fsm    reinit_fsm                threed.py:1515-1537
+  depth3d / max_depth_3d                  threed.py:1798-1869
+"""
+import numpy as np
+
+
+# ----------------------------------------------------------------------------- initial SDF
+def grid_axes(Lx, Ly, Lz, dx):
+    """threed.py:34-35."""
+    nx, ny, nz = int(round(Lx / dx)), int(round(Ly / dx)), int(round(Lz / dx))
+    return np.arange(nx) * dx, np.arange(ny) * dx, np.arange(nz) * dx
+
+
+def trench_mask(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole=False):
+    """Mask material nodes, threed.py:36-43 (float64, same expression order)."""
+    xs, ys, zs = grid_axes(Lx, Ly, Lz, dx)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing='ij')
+    mask_band = (Z >= sub_top) & (Z < sub_top + mask_th)
+    if hole:
+        opening = (X - Lx / 2) ** 2 + (Y - Ly / 2) ** 2 < (trench_width / 2) ** 2
+    else:
+        opening = np.abs(X - Lx / 2) < trench_width / 2
+    return mask_band & (~opening)
+
+
+def init_sdf(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole=False):
+    """Exact signed distance to the CSG solid of threed.py:51-55 (substrate half-space union the
+    masked slab), positive in solid. The reference cleans the analytic field with skfmm
+    (threed.py:56); the exact distance is the quantity skfmm approximates to O(dx)."""
+    xs, ys, zs = grid_axes(Lx, Ly, Lz, dx)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing='ij')
+    r = np.sqrt((X - Lx / 2) ** 2 + (Y - Ly / 2) ** 2) if hole else np.abs(X - Lx / 2)
+    hw, s, t = trench_width / 2.0, sub_top, sub_top + mask_th
+    dr = r - hw
+    d1 = np.where(r <= hw, np.abs(Z - s), np.hypot(dr, Z - s))            # opening floor
+    d2 = np.where(Z < s, np.hypot(dr, Z - s),
+                  np.where(Z > t, np.hypot(dr, Z - t), np.abs(dr)))        # mask sidewall
+    d3 = np.where(r >= hw, np.abs(Z - t), np.hypot(dr, Z - t))            # mask top
+    d = np.minimum(d1, np.minimum(d2, d3))
+    # the junction plane z = sub_top under the mask is interior (substrate meets mask): the
+    # analytic CSG field is 0 there, the distance is not
+    solid = (Z < s) | ((Z >= s) & (Z < t) & (r > hw))
+    return np.where(solid, d, -d)
+
+
+# ----------------------------------------------------------------------------- advection
+def advect_3d(phi, Fspeed, dx, dt):
+    """threed.py:1248-1259: phi - dt*F*sqrt(sum_ax max(D-,0)^2 + min(D+,0)^2); the missing
+    one-sided difference at a domain face is 0."""
+    phi = np.asarray(phi, np.float64)
+    g = np.zeros_like(phi)
+    for ax in range(3):
+        dm = np.zeros_like(phi)
+        dp = np.zeros_like(phi)
+        hi = [slice(None)] * 3
+        lo = [slice(None)] * 3
+        hi[ax] = slice(1, None)
+        lo[ax] = slice(0, -1)
+        diff = (phi[tuple(hi)] - phi[tuple(lo)]) / dx
+        dm[tuple(hi)] = diff
+        dp[tuple(lo)] = diff
+        g += np.maximum(dm, 0.0) ** 2 + np.minimum(dp, 0.0) ** 2
+    return phi - dt * np.asarray(Fspeed, np.float64) * np.sqrt(g)
+
+
+def advect_pinned(phi, Fspeed, mask, mask_phi, dx, dt, nsub):
+    """nsub substeps, each followed by the mask re-pin (threed.py:1761-1763; the device kernel
+    threed.py:1268-1270 pins in the same substep)."""
+    phi = np.asarray(phi, np.float64).copy()
+    mask = np.asarray(mask, bool)
+    mp = np.asarray(mask_phi, np.float64)
+    for _ in range(nsub):
+        phi = advect_3d(phi, Fspeed, dx, dt)
+        phi[mask] = mp[mask]
+    return phi
+
+
+# ----------------------------------------------------------------------------- FSM reinit
+def eik_solve(b1, b2, b3, dx):
+    """threed.py:1421-1452, vectorised: sort the three upwind minima, then the 1-, 2- and
+    3-term Godunov roots, taking the first whose value does not exceed the next minimum."""
+    a = np.sort(np.stack([np.asarray(b1, np.float64), np.asarray(b2, np.float64),
+                          np.asarray(b3, np.float64)]), axis=0)
+    a1, a2, a3 = a[0], a[1], a[2]
+    x1 = a1 + dx
+    d2 = np.maximum(2.0 * dx * dx - (a1 - a2) ** 2, 0.0)
+    x2 = 0.5 * (a1 + a2 + np.sqrt(d2))
+    s1 = a1 + a2 + a3
+    s2 = a1 * a1 + a2 * a2 + a3 * a3
+    d3 = np.maximum(s1 * s1 - 3.0 * (s2 - dx * dx), 0.0)
+    x3 = (s1 + np.sqrt(d3)) / 3.0
+    return np.where(x1 <= a2, x1, np.where(x2 <= a3, x2, x3))
+
+
+def _nbr(U, ax, side, fill):
+    """Neighbour along `ax` (side -1 / +1) with `fill` where it falls outside the grid."""
+    out = np.full_like(U, fill)
+    src = [slice(None)] * 3
+    dst = [slice(None)] * 3
+    if side < 0:
+        dst[ax] = slice(1, None)
+        src[ax] = slice(0, -1)
+    else:
+        dst[ax] = slice(0, -1)
+        src[ax] = slice(1, None)
+    out[tuple(dst)] = U[tuple(src)]
+    return out
+
+
+def _clamped(U, ax, side):
+    """Neighbour with the index clamped at the grid edge (threed.py:1463-1465)."""
+    out = U.copy()
+    src = [slice(None)] * 3
+    dst = [slice(None)] * 3
+    if side < 0:
+        dst[ax] = slice(1, None)
+        src[ax] = slice(0, -1)
+    else:
+        dst[ax] = slice(0, -1)
+        src[ax] = slice(1, None)
+    out[tuple(dst)] = U[tuple(src)]
+    return out
+
+
+def eik_init(phi0, dx, big):
+    """threed.py:1455-1479: frozen interface nodes (sign change to a face neighbour) get
+    min(|phi0|/max(|grad phi0|_central, 0.5), 1.8dx); the rest start at `big`."""
+    c = np.asarray(phi0, np.float64)
+    o = [(_clamped(c, ax, -1), _clamped(c, ax, +1)) for ax in range(3)]
+    iface = np.zeros(c.shape, bool)
+    for m, p in o:
+        iface |= (c * m < 0.0) | (c * p < 0.0)
+    g = [(p - m) * 0.5 / dx for m, p in o]
+    grad0 = np.maximum(np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2), 0.5)
+    D = np.minimum(np.abs(c) / grad0, 1.8 * dx)
+    U = np.where(iface, D, big)
+    return U, iface
+
+
+def eik_jacobi(U, frozen, dx, big):
+    """threed.py:1482-1512: U <- min(U, eik_solve(min-neighbour per axis)); frozen held;
+    absent neighbours at the grid edge count as `big`."""
+    a = [np.minimum(_nbr(U, ax, -1, big), _nbr(U, ax, +1, big)) for ax in range(3)]
+    cand = eik_solve(a[0], a[1], a[2], dx)
+    return np.where(frozen, U, np.minimum(U, cand))
+
+
+def reinit_fsm(phi, dx, band, n_iter=None):
+    """threed.py:1515-1537."""
+    phi = np.asarray(phi, np.float64)
+    big = float(band + 4.0 * dx)
+    if n_iter is None:
+        n_iter = int(np.ceil(band / dx)) + 4
+    U, frozen = eik_init(phi, dx, big)
+    for _ in range(n_iter):
+        U = eik_jacobi(U, frozen, dx, big)
+    sgn = np.sign(phi)
+    sgn = np.where(sgn == 0, 1.0, sgn)
+    return sgn * np.minimum(U, band + dx)
+
+
+# ----------------------------------------------------------------------------- depth metrics
+def _floor_z(col, zs):
+    """Lowest z of the gas run that reaches the top (threed.py:1824-1831)."""
+    k = len(col) - 1
+    if not col[k]:
+        return None
+    while k > 0 and col[k - 1]:
+        k -= 1
+    return zs[k]
+
+
+def depth3d(geo, half=1.0):
+    """threed.py:1798-1834 (center depth: median floor over central columns)."""
+    phi, xs, ys, zs = geo['phi'], geo['xs'], geo['ys'], geo['zs']
+    cx, cy = geo['Lx'] / 2, geo['Ly'] / 2
+    r_feat = geo.get('trench_width', 2.0) / 2.0 - geo['dx']
+    hx = max(min(half, r_feat), geo['dx'])
+    ic = np.where(np.abs(xs - cx) <= hx)[0]
+    if geo.get('hole', False):
+        jc = np.where(np.abs(ys - cy) <= hx)[0]
+        inside = lambda i, j: (xs[i] - cx) ** 2 + (ys[j] - cy) ** 2 <= r_feat ** 2
+    else:
+        jc = np.where(np.abs(ys - cy) <= half)[0]
+        inside = lambda i, j: True
+    floors = []
+    for i in ic:
+        for j in jc:
+            if not inside(i, j):
+                continue
+            z = _floor_z(phi[i, j, :] < 0, zs)
+            if z is not None:
+                floors.append(z)
+    if not floors:
+        return 0.0
+    return float(geo['sub_top'] - np.median(floors))
+
+
+def max_depth_3d(geo):
+    """threed.py:1841-1869 (5th-percentile floor over the feature footprint)."""
+    phi, xs, ys, zs = geo['phi'], geo['xs'], geo['ys'], geo['zs']
+    cx, cy = geo['Lx'] / 2.0, geo['Ly'] / 2.0
+    r = geo.get('trench_width', 4.0) / 2.0
+    if geo.get('hole', False):
+        ii, jj = np.where((xs[:, None] - cx) ** 2 + (ys[None, :] - cy) ** 2 <= (r + geo['dx']) ** 2)
+        cols = list(zip(ii, jj))
+    else:
+        ix = np.where(np.abs(xs - cx) <= r + geo['dx'])[0]
+        cols = [(i, j) for i in ix for j in range(len(ys))]
+    floors = []
+    for i, j in cols:
+        z = _floor_z(phi[i, j, :] < 0, zs)
+        if z is not None:
+            floors.append(z)
+    if not floors:
+        return 0.0
+    return float(geo['sub_top'] - np.percentile(floors, 5))

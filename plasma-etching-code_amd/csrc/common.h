@@ -1,0 +1,87 @@
+// Shared host/device plumbing for libpetch_hip: the context object, grow-only scratch,
+// error reporting and launch helpers. Every kernel in the library is launched on ctx->stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdarg.h>
+#include <string.h>
+
+#include "../../include/petch_hip.h"
+
+// Scratch slots owned by a context (grow-only device buffers, reused step to step).
+enum PetchScratch {
+  SCR_LS_A = 0,      // level set ping-pong / Eikonal U
+  SCR_LS_B,          // second ping-pong buffer
+  SCR_LS_FROZEN,     // Eikonal frozen flags (u8)
+  SCR_MC_CELL,       // marching cubes: per-node triangle counts (u8) and vertex-slot flags
+  SCR_MC_BLOCK,      // marching cubes: per-block partial sums
+  SCR_ADJ,           // face adjacency: vertex->face incidence
+  SCR_ADJ2,
+  SCR_RED,           // small reductions / counters
+  SCR_TRACE,         // ray-queue counters
+  SCR_SORT_A,        // radix sort buffers
+  SCR_SORT_B,
+  SCR_SORT_C,
+  SCR_SORT_D,
+  SCR_SORT_HIST,
+  SCR_RAD_A,         // radiosity CSR build
+  SCR_RAD_B,
+  SCR_RAD_C,
+  SCR_COUNT
+};
+
+struct petch_ctx {
+  int device;
+  hipStream_t stream;
+  char err[1024];
+  void* scr[SCR_COUNT];
+  size_t scr_cap[SCR_COUNT];
+  void* pinned;            // small pinned host staging area for counts / scalars
+  size_t pinned_cap;
+};
+
+struct petch_bvh {
+  int device;
+  int nf, nv;              // faces / vertices of the mesh it was built over
+  int cap_f;               // capacity (faces) of the device buffers below
+  void* nodes;             // (nf-1) internal nodes, 64 B each: child AABBs + child refs
+  void* tris;              // nf triangles in leaf order, 48 B each (v0, e1, e2 as float4)
+  int* tri_face;           // leaf slot -> original face id
+  void* work;              // build scratch: morton keys/values, parents, flags
+  size_t work_cap;
+  float root_lo[3], root_hi[3];
+};
+
+int petch_set_error(petch_ctx* ctx, int code, const char* fmt, ...);
+
+// Returns a device buffer of at least `bytes` for slot `s`, growing it if needed (NULL on failure
+// with the error recorded in ctx).
+void* petch_scratch(petch_ctx* ctx, int s, size_t bytes);
+// Pinned host staging of at least `bytes`.
+void* petch_pinned(petch_ctx* ctx, size_t bytes);
+
+#define PETCH_HIP(ctx, call)                                                              \
+  do {                                                                                    \
+    hipError_t _e = (call);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return petch_set_error((ctx), PETCH_E_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, \
+                             hipGetErrorString(_e));                                      \
+  } while (0)
+
+#define PETCH_LAUNCH_CHECK(ctx)                                                           \
+  do {                                                                                    \
+    hipError_t _e = hipGetLastError();                                                    \
+    if (_e != hipSuccess)                                                                 \
+      return petch_set_error((ctx), PETCH_E_HIP, "%s:%d kernel launch: %s", __FILE__,      \
+                             __LINE__, hipGetErrorString(_e));                            \
+  } while (0)
+
+#define PETCH_ARG(ctx, cond, ...)                                                         \
+  do {                                                                                    \
+    if (!(cond)) return petch_set_error((ctx), PETCH_E_ARG, __VA_ARGS__);                 \
+  } while (0)
+
+static inline unsigned petch_blocks(long long n, int bs) {
+  return (unsigned)((n + bs - 1) / bs);
+}

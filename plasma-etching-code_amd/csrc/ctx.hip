@@ -1,0 +1,103 @@
+// Context lifetime, stream binding, grow-only scratch and error text.
+#include "common.h"
+
+#include <stdlib.h>
+
+int petch_set_error(petch_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+void* petch_scratch(petch_ctx* ctx, int s, size_t bytes) {
+  if (s < 0 || s >= SCR_COUNT) return NULL;
+  if (bytes == 0) bytes = 256;
+  if (ctx->scr_cap[s] >= bytes) return ctx->scr[s];
+  // grow by 1.5x so slowly growing meshes do not reallocate every step
+  size_t want = bytes + bytes / 2;
+  want = (want + 255) & ~(size_t)255;
+  if (ctx->scr[s]) {
+    // the old buffer may still be read by queued kernels
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      petch_set_error(ctx, PETCH_E_HIP, "scratch grow: stream sync failed");
+      return NULL;
+    }
+    (void)hipFree(ctx->scr[s]);
+    ctx->scr[s] = NULL;
+    ctx->scr_cap[s] = 0;
+  }
+  void* p = NULL;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    petch_set_error(ctx, PETCH_E_HIP, "scratch slot %d: hipMalloc(%zu) failed", s, want);
+    return NULL;
+  }
+  ctx->scr[s] = p;
+  ctx->scr_cap[s] = want;
+  return p;
+}
+
+void* petch_pinned(petch_ctx* ctx, size_t bytes) {
+  if (ctx->pinned_cap >= bytes) return ctx->pinned;
+  if (ctx->pinned) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipHostFree(ctx->pinned);
+    ctx->pinned = NULL;
+    ctx->pinned_cap = 0;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  void* p = NULL;
+  if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+    petch_set_error(ctx, PETCH_E_HIP, "hipHostMalloc(%zu) failed", want);
+    return NULL;
+  }
+  ctx->pinned = p;
+  ctx->pinned_cap = want;
+  return p;
+}
+
+extern "C" int petch_ctx_create(int device, petch_ctx** out) {
+  if (!out) return PETCH_E_ARG;
+  *out = NULL;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PETCH_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return PETCH_E_HIP;
+  petch_ctx* c = (petch_ctx*)calloc(1, sizeof(petch_ctx));
+  if (!c) return PETCH_E_HIP;
+  c->device = device;
+  c->stream = 0;
+  *out = c;
+  return PETCH_OK;
+}
+
+extern "C" int petch_ctx_destroy(petch_ctx* ctx) {
+  if (!ctx) return PETCH_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (int s = 0; s < SCR_COUNT; ++s)
+    if (ctx->scr[s]) (void)hipFree(ctx->scr[s]);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  free(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_set_stream(petch_ctx* ctx, void* stream) {
+  if (!ctx) return PETCH_E_ARG;
+  ctx->stream = (hipStream_t)stream;
+  return PETCH_OK;
+}
+
+extern "C" int petch_synchronize(petch_ctx* ctx) {
+  if (!ctx) return PETCH_E_ARG;
+  PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PETCH_OK;
+}
+
+extern "C" const char* petch_last_error(petch_ctx* ctx) {
+  return ctx ? ctx->err : "null context";
+}
+
+extern "C" int petch_version(void) { return 1; }

@@ -1,0 +1,130 @@
+"""ctypes binding of libpetch_hip.so (declared in include/petch_hip.h).
+
+Device memory is torch tensors on the ROCm device; every call passes raw device pointers and
+enqueues on torch's current stream. There is no CPU fallback: if the library or a GPU is
+missing, the first call raises.
+"""
+import ctypes
+import os
+from ctypes import c_int, c_double, c_float, c_void_p, c_char_p, c_uint64, POINTER
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("PETCH_HIP_LIB", os.path.join(_HERE, "libpetch_hip.so"))
+
+PETCH_OK = 0
+PETCH_E_ARG = -1
+PETCH_E_HIP = -2
+PETCH_E_CAPACITY = -3
+PETCH_E_NONFINITE = -4
+PETCH_E_STATE = -5
+
+
+class PetchError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class PetchArgError(PetchError, ValueError):
+    pass
+
+
+class PetchCapacityError(PetchError):
+    pass
+
+
+class PetchNonFiniteError(PetchError):
+    pass
+
+
+_ERR = {PETCH_E_ARG: PetchArgError, PETCH_E_CAPACITY: PetchCapacityError,
+        PETCH_E_NONFINITE: PetchNonFiniteError}
+
+P = c_void_p
+# name -> argument types (all entry points return int)
+SIGNATURES = {
+    "petch_ctx_create": [c_int, POINTER(c_void_p)],
+    "petch_ctx_destroy": [P],
+    "petch_set_stream": [P, P],
+    "petch_synchronize": [P],
+    "petch_version": [],
+    "petch_init_sdf": [P, P, P, c_int, c_int, c_int, c_double, c_double, c_double, c_double,
+                       c_double, c_double, c_int],
+    "petch_advect": [P, P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, c_int],
+    "petch_reinit_fsm": [P, P, c_int, c_int, c_int, c_double, c_double, c_int],
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libpetch_hip.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libpetch_hip.so not found at {LIB_PATH}; build it with "
+                "`python plasma-etching-code_amd/build.py` (the HIP path has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        L.petch_last_error.argtypes = [P]
+        L.petch_last_error.restype = c_char_p
+        _lib = L
+    return _lib
+
+
+class Context:
+    """One petch_ctx per device; kernels go to torch's current stream on that device."""
+    _by_device = {}
+
+    def __init__(self, device):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("petch HIP path: no ROCm GPU visible (torch.cuda.is_available() "
+                               "is False); the HIP implementation has no CPU fallback")
+        self.device = int(device)
+        h = c_void_p()
+        rc = lib().petch_ctx_create(self.device, ctypes.byref(h))
+        if rc != PETCH_OK:
+            raise PetchError(rc, f"petch_ctx_create(device={self.device}) failed")
+        self.h = h
+
+    @classmethod
+    def get(cls, device=None):
+        import torch
+        if device is None:
+            device = torch.cuda.current_device()
+        if isinstance(device, torch.device):
+            device = device.index if device.index is not None else torch.cuda.current_device()
+        c = cls._by_device.get(int(device))
+        if c is None:
+            c = cls(device)
+            cls._by_device[int(device)] = c
+        return c
+
+    def call(self, name, *args):
+        import torch
+        L = lib()
+        L.petch_set_stream(self.h, c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        rc = getattr(L, name)(self.h, *args)
+        if rc != PETCH_OK:
+            msg = L.petch_last_error(self.h)
+            msg = msg.decode() if msg else ""
+            raise _ERR.get(rc, PetchError)(rc, f"{name}: {msg}")
+        return rc
+
+
+def ptr(t, dtype=None, name="tensor"):
+    """Raw device pointer of a contiguous ROCm tensor (None -> NULL)."""
+    if t is None:
+        return c_void_p(0)
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    return c_void_p(t.data_ptr())
