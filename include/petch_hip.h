@@ -67,6 +67,54 @@ int petch_advect(petch_ctx* ctx, float* phi, float* work, const float* F, const 
 int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
                      double band, int n_iter);
 
+
+/* ---------------------------------------------------------------- a2: marching cubes ----- */
+/* extract_mesh_3d / extract_mesh_3d_gpu (threed.py:62-100): iso-0 surface of phi, vertices in
+ * physical units (float32, (nv,3)), faces int32 (nf,3), normals of cross(v1-v0,v2-v0) pointing
+ * into the gas. Deterministic order: vertices by (node, axis) of their grid edge, faces by cell.
+ * Synchronises once to return nv / nf to the host. If nv > cap_v or nf > cap_f nothing is
+ * written and PETCH_E_CAPACITY is returned with *nv_host / *nf_host set to the sizes needed
+ * (pass NULL buffers and zero capacities to query the sizes). */
+int petch_marching_cubes(petch_ctx* ctx, const float* phi, int nx, int ny, int nz, double dx,
+                         float* verts, int* faces, int cap_v, int cap_f, int* nv_host,
+                         int* nf_host);
+
+/* Face centroids (nf,3) = vertex mean, areas (nf) = 0.5|cross|, unit normals (nf,3) (zero for a
+ * degenerate face) — threed.py:96-99 and _face_normals_k threed.py:228-234. Any output may be
+ * NULL. */
+int petch_mesh_attrs(petch_ctx* ctx, const float* verts, const int* faces, int nf,
+                     float* centroids, float* areas, float* normals);
+
+/* _edge_adjacency (threed.py:103-125) as a per-face table: nbr[3f+e] = the other face sharing
+ * edge e of face f (e: v0v1, v1v2, v2v0), or -1 on an open boundary. */
+int petch_face_adjacency(petch_ctx* ctx, const int* faces, int nf, int nv, int* nbr);
+
+/* _gas_normals (threed.py:1553-1568): unit normals flipped to agree with -grad(phi) (numpy
+ * gradient) at the grid node nearest each centroid. */
+int petch_gas_normals(petch_ctx* ctx, const float* phi, int nx, int ny, int nz, double dx,
+                      const float* centroids, const float* normals, int nf, float* out);
+
+/* ---------------------------------------------------------------- a3-a5: LBVH + queries -- */
+/* wp.Mesh(points, indices) (threed.py:1674-1675): a Morton-code LBVH over the triangles,
+ * rebuilt from scratch by petch_bvh_build (device buffers grow, never shrink). The handle keeps
+ * the mesh's triangles in leaf order, so verts/faces may be overwritten after the build. */
+int petch_bvh_create(petch_ctx* ctx, petch_bvh** out);
+int petch_bvh_destroy(petch_ctx* ctx, petch_bvh* bvh);
+int petch_bvh_build(petch_ctx* ctx, petch_bvh* bvh, const float* verts, const int* faces, int nv,
+                    int nf);
+/* wp.mesh_query_ray(mesh, o, d, tmax) (threed.py:306,366,524,569,629,916): closest two-sided hit
+ * with 0 < t < tmax (watertight test); face = -1 and t = -1 on a miss. t may be NULL. */
+int petch_bvh_query_ray(petch_ctx* ctx, const petch_bvh* bvh, const float* orig,
+                        const float* dir, int n, float tmax, int* face, float* t);
+/* wp.mesh_query_point_no_sign(mesh, p, max_dist) (threed.py:1328,1626): closest face within
+ * max_dist, -1 if none; dist may be NULL. */
+int petch_bvh_query_point(petch_ctx* ctx, const petch_bvh* bvh, const float* pts, int n,
+                          float max_dist, int* face, float* dist);
+/* Traversal error flag (1 = a query overflowed its stack and may have missed a face); syncs. */
+int petch_bvh_status(petch_ctx* ctx, const petch_bvh* bvh, int* err_host);
+/* Root AABB (lo xyz, hi xyz) of a BVH with at least two faces; syncs. */
+int petch_bvh_root_bounds(petch_ctx* ctx, const petch_bvh* bvh, float* lohi_host);
+
 #ifdef __cplusplus
 }
 #endif

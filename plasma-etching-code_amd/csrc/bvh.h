@@ -1,0 +1,302 @@
+// Device-side view of the triangle LBVH and its two queries:
+//   closest two-sided ray hit (wp.mesh_query_ray semantics, threed.py:306,366,524,569,629,916)
+//   closest face to a point   (wp.mesh_query_point_no_sign, threed.py:1328,1626)
+//
+// Layout in HBM:
+//   nodes: (nf-1) internal nodes of 64 B = both children's AABBs + child references
+//          (>= 0 internal node, < 0 leaf ~slot); node 0 is the root.
+//   tris:  nf leaves of 48 B in Morton order: v0 (w = face id bits), v1, v2 as float4, so a
+//          leaf is three 16-byte loads and carries the face id the flux deposits into.
+// Traversal keeps a short per-lane stack of far children (the near child is descended first);
+// a stack overflow drops the far subtree and raises an error flag instead of writing out of
+// bounds.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace petch {
+
+constexpr int kBvhStack = 64;
+
+struct __align__(16) BvhNode {
+  float4 a;  // lmin.x lmin.y lmin.z lmax.x
+  float4 b;  // lmax.y lmax.z rmin.x rmin.y
+  float4 c;  // rmin.z rmax.x rmax.y rmax.z
+  int4 d;    // left, right, -, -
+};
+
+struct BvhView {
+  const BvhNode* nodes;
+  const float4* tris;
+  int nf;
+  int* err;  // device flag: set to 1 on traversal stack overflow (may be null)
+};
+
+struct Vec3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ Vec3 v3(float x, float y, float z) { return Vec3{x, y, z}; }
+__device__ __forceinline__ Vec3 operator+(Vec3 a, Vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ Vec3 operator-(Vec3 a, Vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ Vec3 operator*(float s, Vec3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ Vec3 operator-(Vec3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ Vec3 cross(Vec3 a, Vec3 b) {
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length(Vec3 a) { return sqrtf(dot(a, a)); }
+// x/|x|, or the zero vector for a zero-length x
+__device__ __forceinline__ Vec3 normalize(Vec3 a) {
+  float l = length(a);
+  return l > 0.f ? (1.f / l) * a : v3(0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float comp(Vec3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// Per-ray constants of the watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013).
+struct RayPre {
+  Vec3 o, d, inv;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ RayPre ray_pre(Vec3 o, Vec3 d) {
+  RayPre r;
+  r.o = o;
+  r.d = d;
+  // reciprocal of a zero component is taken of +-1e-20 so slab products stay finite (no 0*inf)
+  r.inv = v3(1.f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x)),
+             1.f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y)),
+             1.f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z)));
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  int kx = kz == 2 ? 0 : kz + 1;
+  int ky = kx == 2 ? 0 : kx + 1;
+  if (comp(d, kz) < 0.f) {
+    int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  r.kx = kx;
+  r.ky = ky;
+  r.kz = kz;
+  float dz = comp(d, kz);
+  r.Sx = comp(d, kx) / dz;
+  r.Sy = comp(d, ky) / dz;
+  r.Sz = 1.f / dz;
+  return r;
+}
+
+// Two-sided watertight test; on a hit with 0 < t < tbest returns true and sets t.
+__device__ __forceinline__ bool ray_tri(const RayPre& r, Vec3 p0, Vec3 p1, Vec3 p2, float tbest,
+                                        float& t) {
+#pragma clang fp contract(off)
+  Vec3 A = p0 - r.o, B = p1 - r.o, C = p2 - r.o;
+  float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+  float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
+  float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
+  float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.f || V == 0.f || W == 0.f) {
+    // edge hit: re-evaluate the edge functions in double so shared edges stay closed
+    U = (float)((double)Cx * By - (double)Cy * Bx);
+    V = (float)((double)Ax * Cy - (double)Ay * Cx);
+    W = (float)((double)Bx * Ay - (double)By * Ax);
+  }
+  if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+  float det = U + V + W;
+  if (det == 0.f) return false;
+  float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+  if (det < 0.f) {
+    T = -T;
+    det = -det;
+  }
+  if (!(T > 0.f) || !(T < tbest * det)) return false;
+  t = T / det;
+  return t > 0.f && t < tbest;
+}
+
+// Slab test against [lo, hi]; returns entry distance (or a huge value when missed).
+__device__ __forceinline__ float ray_box(const RayPre& r, float lx, float ly, float lz, float hx,
+                                         float hy, float hz, float tbest) {
+  float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
+  float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
+  float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
+  float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
+  float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
+  // tmax widened by 2 ulp so rounding never culls a box the ray touches (conservative)
+  return (tmin <= tmax * 1.00000024f) ? tmin : 3.0e38f;
+}
+
+struct Hit {
+  int face;   // original face id, -1 on a miss
+  int slot;   // leaf slot
+  float t;
+};
+
+__device__ __forceinline__ void leaf_test(const BvhView& b, const RayPre& r, int slot, Hit& h) {
+  float4 q0 = b.tris[3 * slot], q1 = b.tris[3 * slot + 1], q2 = b.tris[3 * slot + 2];
+  float t;
+  if (ray_tri(r, v3(q0.x, q0.y, q0.z), v3(q1.x, q1.y, q1.z), v3(q2.x, q2.y, q2.z), h.t, t)) {
+    h.t = t;
+    h.face = __float_as_int(q0.w);
+    h.slot = slot;
+  }
+}
+
+// Closest hit with 0 < t < tmax.
+__device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d, float tmax) {
+  Hit h{-1, -1, tmax};
+  if (b.nf <= 0) return h;
+  RayPre r = ray_pre(o, d);
+  if (b.nf == 1) {
+    leaf_test(b, r, 0, h);
+    return h;
+  }
+  int stack[kBvhStack];
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    const BvhNode* N = b.nodes + node;
+    float4 A = N->a, B = N->b, C = N->c;
+    int4 D = N->d;
+    float tl = ray_box(r, A.x, A.y, A.z, A.w, B.x, B.y, h.t);
+    float tr = ray_box(r, B.z, B.w, C.x, C.y, C.z, C.w, h.t);
+    bool hl = tl < 1.0e38f, hr = tr < 1.0e38f;
+    if (hl && D.x < 0) {
+      leaf_test(b, r, ~D.x, h);
+      hl = false;
+    }
+    if (hr && D.y < 0) {
+      leaf_test(b, r, ~D.y, h);
+      hr = false;
+    }
+    if (hl && hr) {
+      int nearc = (tl <= tr) ? D.x : D.y;
+      int farc = (tl <= tr) ? D.y : D.x;
+      if (sp < kBvhStack) {
+        stack[sp++] = farc;
+      } else if (b.err) {
+        *b.err = 1;
+      }
+      node = nearc;
+    } else if (hl) {
+      node = D.x;
+    } else if (hr) {
+      node = D.y;
+    } else {
+      if (sp == 0) break;
+      node = stack[--sp];
+    }
+  }
+  return h;
+}
+
+// Ericson, Real-Time Collision Detection 5.1.5: closest point of triangle abc to p.
+__device__ __forceinline__ Vec3 closest_on_tri(Vec3 p, Vec3 a, Vec3 b, Vec3 c) {
+  Vec3 ab = b - a, ac = c - a, ap = p - a;
+  float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.f && d2 <= 0.f) return a;
+  Vec3 bp = p - b;
+  float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.f && d4 <= d3) return b;
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+    float v = d1 / (d1 - d3);
+    return a + v * ab;
+  }
+  Vec3 cp = p - c;
+  float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.f && d5 <= d6) return c;
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+    float w = d2 / (d2 - d6);
+    return a + w * ac;
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return b + w * (c - b);
+  }
+  float den = 1.f / (va + vb + vc);
+  float v = vb * den, w = vc * den;
+  return a + v * ab + w * ac;
+}
+
+__device__ __forceinline__ float box_dist2(Vec3 p, float lx, float ly, float lz, float hx,
+                                           float hy, float hz) {
+  float dx = fmaxf(fmaxf(lx - p.x, 0.f), p.x - hx);
+  float dy = fmaxf(fmaxf(ly - p.y, 0.f), p.y - hy);
+  float dz = fmaxf(fmaxf(lz - p.z, 0.f), p.z - hz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+struct PointHit {
+  int face;
+  float d2;
+};
+
+__device__ __forceinline__ void leaf_point(const BvhView& b, Vec3 p, int slot, PointHit& h) {
+  float4 q0 = b.tris[3 * slot], q1 = b.tris[3 * slot + 1], q2 = b.tris[3 * slot + 2];
+  Vec3 c = closest_on_tri(p, v3(q0.x, q0.y, q0.z), v3(q1.x, q1.y, q1.z), v3(q2.x, q2.y, q2.z));
+  Vec3 e = p - c;
+  float d2 = dot(e, e);
+  if (d2 < h.d2) {
+    h.d2 = d2;
+    h.face = __float_as_int(q0.w);
+  }
+}
+
+// Closest face within max_dist (distance squared < max_dist^2).
+__device__ __forceinline__ PointHit bvh_closest_point(const BvhView& b, Vec3 p, float max_dist) {
+  PointHit h{-1, max_dist * max_dist};
+  if (b.nf <= 0) return h;
+  if (b.nf == 1) {
+    leaf_point(b, p, 0, h);
+    return h;
+  }
+  int stack[kBvhStack];
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    const BvhNode* N = b.nodes + node;
+    float4 A = N->a, B = N->b, C = N->c;
+    int4 D = N->d;
+    float dl = box_dist2(p, A.x, A.y, A.z, A.w, B.x, B.y);
+    float dr = box_dist2(p, B.z, B.w, C.x, C.y, C.z, C.w);
+    bool hl = dl < h.d2, hr = dr < h.d2;
+    if (hl && D.x < 0) {
+      leaf_point(b, p, ~D.x, h);
+      hl = false;
+    }
+    if (hr && D.y < 0) {
+      leaf_point(b, p, ~D.y, h);
+      hr = false;
+    }
+    // the other child may have become irrelevant after a leaf test
+    if (hl && !(dl < h.d2)) hl = false;
+    if (hr && !(dr < h.d2)) hr = false;
+    if (hl && hr) {
+      int nearc = (dl <= dr) ? D.x : D.y;
+      int farc = (dl <= dr) ? D.y : D.x;
+      if (sp < kBvhStack) {
+        stack[sp++] = farc;
+      } else if (b.err) {
+        *b.err = 1;
+      }
+      node = nearc;
+    } else if (hl) {
+      node = D.x;
+    } else if (hr) {
+      node = D.y;
+    } else {
+      if (sp == 0) break;
+      node = stack[--sp];
+    }
+  }
+  return h;
+}
+
+}  // namespace petch

