@@ -33,6 +33,29 @@ extern "C" {
 typedef struct petch_ctx petch_ctx;
 typedef struct petch_bvh petch_bvh;
 
+/* Simulation box of the ray tracer: source plane z = z_src (Lz - dx in the reference,
+ * threed.py:477,749), lateral periodic wrap when periodic != 0 (par['periodic_y'],
+ * threed.py:289-349). */
+typedef struct petch_box {
+  float Lx, Ly, Lz, z_src;
+  int periodic;
+} petch_box;
+
+/* Source distribution (threed.py:410-458): kind 0 = neutral, cosine hemisphere about -z;
+ * 1 = ion, Gaussian tilt of std `sigma` rad per axis; 2 = ion, power-cosine cos^exponent(theta)
+ * about -z. */
+typedef struct petch_source {
+  int kind;
+  float sigma;
+  float exponent;
+} petch_source;
+
+/* Faithful-ion constants of _trace3d_ion_yield (threed.py:600-608, 800-802, 806-813). */
+typedef struct petch_ion_params {
+  float meanE, sigmaE, A_sp, Eth_sp, B_sp, A_ie, Eth_ie, A_p, Eth_p;
+  float inflect, minang, n_l, A_en, minE, thrmin, thrmax;
+} petch_ion_params;
+
 /* ---------------------------------------------------------------- context ---------------- */
 /* Replaces the implicit Warp runtime state (threed.py:22-23 wp.init / DEVICE). */
 int petch_ctx_create(int device, petch_ctx** out);
@@ -114,6 +137,38 @@ int petch_bvh_query_point(petch_ctx* ctx, const petch_bvh* bvh, const float* pts
 int petch_bvh_status(petch_ctx* ctx, const petch_bvh* bvh, int* err_host);
 /* Root AABB (lo xyz, hi xyz) of a BVH with at least two faces; syncs. */
 int petch_bvh_root_bounds(petch_ctx* ctx, const petch_bvh* bvh, float* lohi_host);
+
+/* ---------------------------------------------------------------- a6-a10, a21: ray tracing */
+/* All traces draw every random number from Philox-4x32-10 keyed by `seed`, one stream per
+ * primary ray (counter = (block, ray, 0, 0)); the source ray is the stream's first draws.
+ * Per-face outputs are (nf,) float32 and are overwritten. */
+/* _gen_source / gen_source_gpu (threed.py:441-469): the source rays the traces use. */
+int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petch_source* src, int n,
+                     uint64_t seed, float* origin, float* dir);
+/* _trace3d_cov_rr (threed.py:549-596): neutral with weight w deposited at every hit,
+ * w *= 1 - bare[face]*beta, Russian roulette below 0.1 (survivors w = 0.3), cosine re-emission,
+ * <= 1024 bounces. */
+int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                           const petch_source* src, const float* bare, float beta, int n,
+                           uint64_t seed, float* fl);
+/* _trace3d_ion_yield (threed.py:599-712): per-ion energy, yield-weighted sputter / ion-enhanced
+ * / passivation deposits, arrival weight and weighted cos at every hit, angle-dependent sticking,
+ * energy loss and coned-cosine reflection. */
+int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                          const petch_source* src, const petch_ion_params* ip, int n,
+                          uint64_t seed, float* sp, float* ie, float* ipass, float* arr,
+                          float* ang);
+/* _trace3d (threed.py:352-406): specular=0 sticking/diffuse neutral, specular=1 ion with
+ * grazing specular reflection (cos_thr, eta); <= n_reemit re-emissions; no periodic wrap. */
+int petch_trace_legacy(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                       const petch_source* src, float sticking, int n_reemit, int specular,
+                       float cos_thr, float eta, int n, uint64_t seed, float* flux,
+                       float* angacc);
+/* _trace_ff (threed.py:908-920) with the launch of threed.py:949-955: n_ff cosine rays from
+ * centroid + 1e-3*n of every face; hitface[j*n_ff + k] = first face hit or -1 (escape). */
+int petch_trace_ff(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                   const float* centroids, const float* gas_normals, int nf, int n_ff,
+                   uint64_t seed, int* hitface);
 
 #ifdef __cplusplus
 }

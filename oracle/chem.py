@@ -1,0 +1,157 @@
+"""Surface-chemistry oracle (FP64 numpy) — TEST INFRASTRUCTURE ONLY.
+
+Restates:
+  ied_yield            chemistry.py:10-31   (_ied_yield: 'mean' / 'gauss' / 'bimodal' IED)
+  yields               chemistry.py:34-39
+  angular_factors      chemistry.py:45-59
+  surface_rate_langmuir chemistry.py:62-80
+  surface_rate_belen   belen.py:26-63
+  belen_coverages      threed.py:715-737
+  faces_in_mask        threed.py:1608-1616
+  ion_params           threed.py:800-802, 811
+"""
+import numpy as np
+
+
+def ied_yield(A, Eth, par):
+    mode = par.get('ied_mode', 'mean')
+    Em = float(par['Emean'])
+    sEth = np.sqrt(max(Eth, 0.0))
+    if mode == 'mean':
+        return float(A * max(np.sqrt(max(Em, 0.0)) - sEth, 0.0))
+    if mode == 'gauss':
+        Es = float(par.get('Esig', 10.0))
+        z = np.linspace(-3.5, 3.5, 25)
+        w = np.exp(-0.5 * z * z)
+        w /= w.sum()
+        E = np.clip(Em + Es * z, 0.0, None)
+        return float((w * A * np.maximum(np.sqrt(E) - sEth, 0.0)).sum())
+    dE = float(par.get('ied_dE', 40.0))
+    phi = np.linspace(0.0, np.pi, 49)
+    E = np.clip(Em + 0.5 * dE * np.cos(phi), 0.0, None)
+    return float(np.mean(A * np.maximum(np.sqrt(E) - sEth, 0.0)))
+
+
+def yields(par):
+    return (ied_yield(par['A_ie'], par['Eth_ie'], par),
+            ied_yield(par['A_sp'], par['Eth_sp'], par),
+            ied_yield(par['A_p'], par['Eth_p'], par))
+
+
+def angular_factors(cos_i, par, mode):
+    c = np.clip(np.asarray(cos_i, np.float64), 0.0, 1.0)
+    if mode == "viennaps":
+        B = par.get('B_sp', 9.3)
+        f_sp = np.maximum((1.0 + B * (1.0 - c * c)) * c, 0.0)
+        theta = np.arccos(c)
+        f_ie = np.where(c >= 0.5, 1.0, np.maximum(3.0 - 6.0 * theta / np.pi, 0.0))
+        return f_ie, f_sp
+    return c, c
+
+
+def _mode(flags):
+    return "cosine" if flags is None else getattr(flags, "yield_angular", "cosine")
+
+
+def surface_rate_langmuir(m_i, m_F, m_O, cos_i, is_mask, par, flags=None):
+    Yie, Ysp, Yp = yields(par)
+    f_ie, f_sp = angular_factors(cos_i, par, _mode(flags))
+    Fi = par['ionFlux'] * np.asarray(m_i, np.float64)
+    Fev = par['Fflux'] * np.asarray(m_F, np.float64)
+    Fp = par['Oflux'] * np.asarray(m_O, np.float64)
+    eps = 1e-9
+    Yie_a, Ysp_a, Yp_a = Yie * f_ie, Ysp * f_sp, Yp * f_ie
+    rF = par['s_F'] * Fev / (Yie_a * Fi + eps)
+    rO = par['s_O'] * Fp / (Yp_a * Fi + eps)
+    bare = 1.0 / (1.0 + rF + rO)
+    thF = rF * bare
+    V = (1.0 / par['rho']) * (Yie_a * Fi * thF + Ysp_a * Fi * bare) * par['rate_scale']
+    V = np.array(V, np.float64)
+    V[np.asarray(is_mask, bool)] = 0.0
+    return V
+
+
+def surface_rate_belen(m_i, m_F, m_O, cos_i, is_mask, par, flags=None, ion_yield=None):
+    """belen.py:26-63; `ion_yield` = (sp, ie, ip) stands for par['_ion_yield']."""
+    Yie, Ysp, Yp = yields(par)
+    f_ie, f_sp = angular_factors(cos_i, par, _mode(flags))
+    Fi = par['ionFlux'] * np.asarray(m_i, np.float64)
+    Yie_a, Ysp_a, Yp_a = Yie * f_ie, Ysp * f_sp, Yp * f_ie
+    eps = 1e-9
+    iy = ion_yield if ion_yield is not None else par.get('_ion_yield')
+    if iy is not None:
+        _sp, _ie, _ip = (np.asarray(a, np.float64) for a in iy)
+        ionF = par['ionFlux']
+        GY_ie, GY_p, sputter, ion_enh = ionF * _ie, ionF * _ip, ionF * _sp, ionF * _ie
+    else:
+        GY_ie, GY_p, sputter, ion_enh = Yie_a * Fi, Yp_a * Fi, Ysp_a * Fi, Yie_a * Fi
+    fnE = par.get('fnorm_E', 1.0)
+    fnO = par.get('fnorm_O', 1.0)
+    Gb_E = par['Fflux'] * np.asarray(m_F, np.float64) / fnE + eps
+    Gb_P = par['Oflux'] * np.asarray(m_O, np.float64) / fnO + eps
+    a = (par['k_sigma'] + 2.0 * GY_ie) / Gb_E
+    b = (par['beta_sigma'] + GY_p) / Gb_P
+    thF = 1.0 / (1.0 + a * (1.0 + 1.0 / (b + eps)))
+    rate = par['k_sigma'] * thF / 4.0 + sputter + thF * ion_enh
+    V = np.array((1.0 / par['rho']) * rate * par['rate_scale'], np.float64)
+    V[np.asarray(is_mask, bool)] = 0.0
+    return V
+
+
+def surface_rate(m_i, m_F, m_O, cos_i, is_mask, par, flags=None, ion_yield=None):
+    """chemistry.py:83-92."""
+    model = "langmuir" if flags is None else getattr(flags, "chemistry", "langmuir")
+    if model == "langmuir":
+        return surface_rate_langmuir(m_i, m_F, m_O, cos_i, is_mask, par, flags)
+    if model == "belen":
+        return surface_rate_belen(m_i, m_F, m_O, cos_i, is_mask, par, flags, ion_yield)
+    raise ValueError(f"unknown chemistry model: {model}")
+
+
+def belen_coverages(m_i, m_F, m_O, cos_i, par, flags, ion_yield=None):
+    """threed.py:715-737 (note: ignores fnorm_E/O, unlike surface_rate_belen)."""
+    Yie, Ysp, Yp = yields(par)
+    f_ie, _ = angular_factors(cos_i, par, _mode(flags))
+    Fi = par['ionFlux'] * np.asarray(m_i, np.float64)
+    eps = 1e-9
+    iy = ion_yield if ion_yield is not None else par.get('_ion_yield')
+    if iy is not None:
+        _sp, _ie, _ip = (np.asarray(a, np.float64) for a in iy)
+        GY_ie = par['ionFlux'] * _ie
+        GY_p = par['ionFlux'] * _ip
+    else:
+        GY_ie = Yie * f_ie * Fi
+        GY_p = Yp * f_ie * Fi
+    Gb_E = par['Fflux'] * np.asarray(m_F, np.float64) + eps
+    Gb_P = par['Oflux'] * np.asarray(m_O, np.float64) + eps
+    a = (par['k_sigma'] + 2.0 * GY_ie) / Gb_E
+    b = (par['beta_sigma'] + GY_p) / Gb_P
+    thF = 1.0 / (1.0 + a * (1.0 + 1.0 / (b + eps)))
+    thO = 1.0 / (1.0 + b * (1.0 + 1.0 / (a + eps)))
+    return thF, thO
+
+
+def faces_in_mask(centroids, geo, mask_th, trench_width, hole=False):
+    c = np.asarray(centroids, np.float64)
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    in_band = (z >= geo['sub_top']) & (z < geo['sub_top'] + mask_th)
+    if hole:
+        opening = (x - geo['Lx'] / 2) ** 2 + (y - geo['Ly'] / 2) ** 2 < (trench_width / 2) ** 2
+    else:
+        opening = np.abs(x - geo['Lx'] / 2) < trench_width / 2
+    return in_band & (~opening)
+
+
+def ion_params(par):
+    """The faithful-ion constants passed at threed.py:806-813 (derived at 800-802)."""
+    inflect = float(par.get('inflectAngle', 1.55334303))
+    minang = float(par.get('minAngle', 1.3962634))
+    n_l = float(par.get('n_l', 10.0))
+    A_en = 1.0 / (1.0 + n_l * (1.5707963 / inflect - 1.0))
+    minE = float(min(par['Eth_ie'], par['Eth_sp']))
+    return dict(meanE=float(par['Emean']), sigmaE=float(par['Esig']), A_sp=float(par['A_sp']),
+                Eth_sp=float(par['Eth_sp']), B_sp=float(par['B_sp']), A_ie=float(par['A_ie']),
+                Eth_ie=float(par['Eth_ie']), A_p=float(par['A_p']), Eth_p=float(par['Eth_p']),
+                inflect=inflect, minang=minang, n_l=n_l, A_en=float(A_en), minE=minE,
+                thrmin=float(par.get('thetaRMin', 1.2217305)),
+                thrmax=float(par.get('thetaRMax', 1.5707963)))

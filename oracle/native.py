@@ -80,3 +80,87 @@ def closest_point_brute(verts, faces, pts):
     lib().oracle_closest_point_brute(_p(v), _p(f), ctypes.c_int(len(f)), _p(p), ctypes.c_int(n),
                                      _p(face), _p(dist))
     return face, dist
+
+
+# ------------------------------------------------------------------ traces
+class Box(ctypes.Structure):
+    _fields_ = [("Lx", ctypes.c_float), ("Ly", ctypes.c_float), ("Lz", ctypes.c_float),
+                ("z_src", ctypes.c_float), ("periodic", ctypes.c_int)]
+
+
+class Source(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("sigma", ctypes.c_float), ("exponent", ctypes.c_float)]
+
+
+class Ion(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_float) for k in ("meanE", "sigmaE", "A_sp", "Eth_sp", "B_sp", "A_ie",
+                                              "Eth_ie", "A_p", "Eth_p", "inflect", "minang",
+                                              "n_l", "A_en", "minE", "thrmin", "thrmax")]
+
+
+class OracleBvh:
+    """The oracle's own CPU BVH over a fixed mesh (median split; closest hit in float64)."""
+
+    def __init__(self, verts, faces):
+        L = lib()
+        L.oracle_bvh_build.restype = ctypes.c_void_p
+        self.v = np.ascontiguousarray(verts, np.float32)
+        self.f = np.ascontiguousarray(faces, np.int32)
+        self.nf = len(self.f)
+        self.h = ctypes.c_void_p(L.oracle_bvh_build(_p(self.v), _p(self.f), ctypes.c_int(self.nf)))
+
+    def __del__(self):
+        try:
+            lib().oracle_bvh_free(self.h)
+        except Exception:
+            pass
+
+    def closest_hit(self, orig, dirs, tmax=1.0e6):
+        o = np.ascontiguousarray(orig, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = len(o)
+        face = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float64)
+        lib().oracle_bvh_closest_hit(self.h, _p(o), _p(d), ctypes.c_int(n), ctypes.c_double(tmax),
+                                     _p(face), _p(t))
+        return face, t
+
+    def trace_neutral_rr(self, box, src, bare, beta, n, seed):
+        bare = np.ascontiguousarray(bare, np.float32)
+        out = np.zeros(self.nf, np.float64)
+        lib().oracle_trace_neutral_rr(self.h, ctypes.byref(box), ctypes.byref(src), _p(bare),
+                                      ctypes.c_float(beta), ctypes.c_int(n),
+                                      ctypes.c_uint64(seed), _p(out))
+        return out
+
+    def trace_ion_yield(self, box, src, ion, n, seed):
+        out = np.zeros((5, self.nf), np.float64)
+        lib().oracle_trace_ion_yield(self.h, ctypes.byref(box), ctypes.byref(src),
+                                     ctypes.byref(ion), ctypes.c_int(n), ctypes.c_uint64(seed),
+                                     _p(out))
+        return out
+
+    def trace_legacy(self, box, src, sticking, n_reemit, specular, cos_thr, eta, n, seed):
+        out = np.zeros((2, self.nf), np.float64)
+        lib().oracle_trace_legacy(self.h, ctypes.byref(box), ctypes.byref(src),
+                                  ctypes.c_float(sticking), ctypes.c_int(n_reemit),
+                                  ctypes.c_int(specular), ctypes.c_float(cos_thr),
+                                  ctypes.c_float(eta), ctypes.c_int(n), ctypes.c_uint64(seed),
+                                  _p(out))
+        return out
+
+    def trace_ff(self, box, centroids, gas_normals, n_ff, seed):
+        c = np.ascontiguousarray(centroids, np.float32)
+        g = np.ascontiguousarray(gas_normals, np.float32)
+        out = np.zeros(len(c) * n_ff, np.int32)
+        lib().oracle_trace_ff(self.h, ctypes.byref(box), _p(c), _p(g), ctypes.c_int(len(c)),
+                              ctypes.c_int(n_ff), ctypes.c_uint64(seed), _p(out))
+        return out
+
+
+def gen_source(box, src, n, seed):
+    o = np.zeros((n, 3), np.float32)
+    d = np.zeros((n, 3), np.float32)
+    lib().oracle_gen_source(ctypes.byref(box), ctypes.byref(src), ctypes.c_int(n),
+                            ctypes.c_uint64(seed), _p(o), _p(d))
+    return o, d

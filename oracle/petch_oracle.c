@@ -340,3 +340,506 @@ void oracle_closest_point_brute(const float* verts, const int* faces, int nf, co
     dist_out[i] = best >= 0 ? sqrt(bd) : -1.0;
   }
 }
+
+/* =============================================================== per-ray RNG stream ======== */
+/* Same stream layout as the device: key = seed, counter = (block, ray, 0, 0), words in order. */
+typedef struct {
+  uint32_t key[2], ray, blk, w[4];
+  int pos;
+} oc_rng;
+
+static void oc_rng_init(oc_rng* r, uint64_t seed, uint32_t ray) {
+  r->key[0] = (uint32_t)seed;
+  r->key[1] = (uint32_t)(seed >> 32);
+  r->ray = ray;
+  r->blk = 0;
+  r->pos = 4;
+}
+static uint32_t oc_next(oc_rng* r) {
+  if (r->pos == 4) {
+    uint32_t c[4] = {r->blk++, r->ray, 0u, 0u};
+    philox4x32_10(c, r->key, r->w);
+    r->pos = 0;
+  }
+  return r->w[r->pos++];
+}
+static float oc_uniform(oc_rng* r) { return (float)(oc_next(r) >> 8) * 5.9604644775390625e-8f; }
+static float oc_normal(oc_rng* r) {
+  float u1 = (float)((oc_next(r) >> 8) + 1u) * 5.9604644775390625e-8f;
+  float u2 = oc_uniform(r);
+  return sqrtf(-2.f * logf(u1)) * cosf(6.28318530717958648f * u2);
+}
+
+/* =============================================================== CPU BVH (oracle) ========== */
+/* Top-down median split over centroids, <= 4 faces per leaf; closest hit in double precision
+ * (checked against oracle_closest_hit_brute by the tests). Independent of the device LBVH. */
+typedef struct {
+  float lo[3], hi[3];
+  int left, right; /* children, or -1 */
+  int start, count; /* leaf face range in order[] */
+} oc_node;
+
+typedef struct {
+  int nf, nnodes;
+  oc_node* nodes;
+  int* order;
+  double* tri; /* 9 doubles per face: v0 v1 v2 */
+} oc_bvh;
+
+static float* oc_cen_tmp;
+static int oc_cmp_axis;
+static int oc_cmp(const void* a, const void* b) {
+  float ca = oc_cen_tmp[3 * (*(const int*)a) + oc_cmp_axis];
+  float cb = oc_cen_tmp[3 * (*(const int*)b) + oc_cmp_axis];
+  if (ca < cb) return -1;
+  if (ca > cb) return 1;
+  return (*(const int*)a) - (*(const int*)b);
+}
+
+static int oc_build_rec(oc_bvh* B, float* cen, int start, int count) {
+  int id = B->nnodes++;
+  oc_node* nd = &B->nodes[id];
+  for (int k = 0; k < 3; ++k) { nd->lo[k] = 3e38f; nd->hi[k] = -3e38f; }
+  for (int i = start; i < start + count; ++i) {
+    const double* t = B->tri + 9 * B->order[i];
+    for (int v = 0; v < 3; ++v)
+      for (int k = 0; k < 3; ++k) {
+        float x = (float)t[3 * v + k];
+        if (x < nd->lo[k]) nd->lo[k] = x;
+        if (x > nd->hi[k]) nd->hi[k] = x;
+      }
+  }
+  nd->left = nd->right = -1;
+  nd->start = start;
+  nd->count = count;
+  if (count <= 4) return id;
+  float clo[3] = {3e38f, 3e38f, 3e38f}, chi[3] = {-3e38f, -3e38f, -3e38f};
+  for (int i = start; i < start + count; ++i)
+    for (int k = 0; k < 3; ++k) {
+      float c = cen[3 * B->order[i] + k];
+      if (c < clo[k]) clo[k] = c;
+      if (c > chi[k]) chi[k] = c;
+    }
+  int ax = 0;
+  for (int k = 1; k < 3; ++k)
+    if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+  oc_cen_tmp = cen;
+  oc_cmp_axis = ax;
+  qsort(B->order + start, count, sizeof(int), oc_cmp);
+  int half = count / 2;
+  int l = oc_build_rec(B, cen, start, half);
+  int r = oc_build_rec(B, cen, start + half, count - half);
+  B->nodes[id].left = l;
+  B->nodes[id].right = r;
+  return id;
+}
+
+void* oracle_bvh_build(const float* verts, const int* faces, int nf) {
+  oc_bvh* B = (oc_bvh*)calloc(1, sizeof(oc_bvh));
+  B->nf = nf;
+  B->nodes = (oc_node*)calloc(nf > 0 ? 2 * nf : 1, sizeof(oc_node));
+  B->order = (int*)malloc(sizeof(int) * (nf > 0 ? nf : 1));
+  B->tri = (double*)malloc(sizeof(double) * 9 * (nf > 0 ? nf : 1));
+  float* cen = (float*)malloc(sizeof(float) * 3 * (nf > 0 ? nf : 1));
+  for (int f = 0; f < nf; ++f) {
+    B->order[f] = f;
+    for (int v = 0; v < 3; ++v)
+      for (int k = 0; k < 3; ++k) B->tri[9 * f + 3 * v + k] = verts[3 * faces[3 * f + v] + k];
+    for (int k = 0; k < 3; ++k)
+      cen[3 * f + k] = (float)((B->tri[9 * f + k] + B->tri[9 * f + 3 + k] + B->tri[9 * f + 6 + k]) / 3.0);
+  }
+  if (nf > 0) oc_build_rec(B, cen, 0, nf);
+  free(cen);
+  return B;
+}
+
+void oracle_bvh_free(void* h) {
+  oc_bvh* B = (oc_bvh*)h;
+  if (!B) return;
+  free(B->nodes);
+  free(B->order);
+  free(B->tri);
+  free(B);
+}
+
+static int oc_box_hit(const oc_node* nd, const double o[3], const double inv[3], double tbest) {
+  double t0 = 0.0, t1 = tbest;
+  for (int k = 0; k < 3; ++k) {
+    double a = ((double)nd->lo[k] - o[k]) * inv[k], b = ((double)nd->hi[k] - o[k]) * inv[k];
+    if (a > b) { double t = a; a = b; b = t; }
+    if (a > t0) t0 = a;
+    if (b < t1) t1 = b;
+  }
+  return t0 <= t1 * (1.0 + 1e-9) + 1e-12;
+}
+
+/* Closest hit 0 < t < tmax; returns the face (or -1) and sets *t. */
+static int oc_closest(const oc_bvh* B, const double o[3], const double d[3], double tmax,
+                      double* tout) {
+  int best = -1;
+  double bt = tmax;
+  if (B->nf == 0) return -1;
+  double inv[3];
+  for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (fabs(d[k]) > 1e-300 ? d[k] : 1e-300);
+  int stack[128], sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const oc_node* nd = &B->nodes[stack[--sp]];
+    if (!oc_box_hit(nd, o, inv, bt)) continue;
+    if (nd->left < 0) {
+      for (int i = nd->start; i < nd->start + nd->count; ++i) {
+        int f = B->order[i];
+        const double* t = B->tri + 9 * f;
+        double tt;
+        if (oc_ray_tri_d(o, d, t, t + 3, t + 6, &tt) && tt < bt) {
+          bt = tt;
+          best = f;
+        }
+      }
+    } else {
+      stack[sp++] = nd->left;
+      stack[sp++] = nd->right;
+    }
+  }
+  *tout = bt;
+  return best;
+}
+
+void oracle_bvh_closest_hit(void* h, const float* orig, const float* dir, int n, double tmax,
+                            int* face_out, double* t_out) {
+  const oc_bvh* B = (const oc_bvh*)h;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    double o[3] = {orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]};
+    double d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+    double t;
+    int f = oc_closest(B, o, d, tmax, &t);
+    face_out[i] = f;
+    t_out[i] = f >= 0 ? t : -1.0;
+  }
+}
+
+/* =============================================================== traces ==================== */
+typedef struct { float Lx, Ly, Lz, z_src; int periodic; } oc_box;
+typedef struct { int kind; float sigma, exponent; } oc_source;
+typedef struct {
+  float meanE, sigmaE, A_sp, Eth_sp, B_sp, A_ie, Eth_ie, A_p, Eth_p;
+  float inflect, minang, n_l, A_en, minE, thrmin, thrmax;
+} oc_ion;
+
+typedef struct { float x, y, z; } f3;
+static f3 mk(float x, float y, float z) { f3 r = {x, y, z}; return r; }
+static f3 add3(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static f3 sub3(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static f3 scl(float s, f3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+static float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static f3 crs(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static f3 nrmz(f3 a) {
+  float l = sqrtf(dot3(a, a));
+  return l > 0.f ? scl(1.f / l, a) : mk(0.f, 0.f, 0.f);
+}
+
+/* Source ray: threed.py:449-458 (Gaussian ion / cosine neutral) and the power-cosine option. */
+static void oc_source_ray(oc_rng* r, const oc_box* b, const oc_source* s, f3* o, f3* d) {
+  float ox = oc_uniform(r) * b->Lx;
+  float oy = oc_uniform(r) * b->Ly;
+  *o = mk(ox, oy, b->z_src);
+  if (s->kind == 1) {
+    float ax = oc_normal(r) * s->sigma;
+    float ay = oc_normal(r) * s->sigma;
+    *d = nrmz(mk(sinf(ax), sinf(ay), -cosf(ax) * cosf(ay)));
+  } else if (s->kind == 2) {
+    float ct = powf(oc_uniform(r), 1.f / (s->exponent + 1.f));
+    float st = sqrtf(fmaxf(1.f - ct * ct, 0.f));
+    float ph = 6.2831853f * oc_uniform(r);
+    *d = nrmz(mk(st * cosf(ph), st * sinf(ph), -ct));
+  } else {
+    float r1 = oc_uniform(r), r2 = oc_uniform(r);
+    float ct = sqrtf(r1), st = sqrtf(1.f - r1), ph = 6.2831853f * r2;
+    *d = nrmz(mk(st * cosf(ph), st * sinf(ph), -ct));
+  }
+}
+
+void oracle_gen_source(const oc_box* b, const oc_source* s, int n, uint64_t seed, float* origin,
+                       float* dir) {
+  for (int p = 0; p < n; ++p) {
+    oc_rng r;
+    oc_rng_init(&r, seed, (uint32_t)p);
+    f3 o, d;
+    oc_source_ray(&r, b, s, &o, &d);
+    origin[3 * p] = o.x; origin[3 * p + 1] = o.y; origin[3 * p + 2] = o.z;
+    dir[3 * p] = d.x; dir[3 * p + 1] = d.y; dir[3 * p + 2] = d.z;
+  }
+}
+
+/* Closest hit from float origin/direction; returns face, t and the unit face normal. */
+static int oc_query(const oc_bvh* B, f3 o, f3 d, float* t, f3* n) {
+  double od[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, tt;
+  int f = oc_closest(B, od, dd, 1.0e6, &tt);
+  if (f < 0) return -1;
+  *t = (float)tt;
+  const double* q = B->tri + 9 * f;
+  f3 a = mk((float)q[0], (float)q[1], (float)q[2]);
+  f3 b = mk((float)q[3], (float)q[4], (float)q[5]);
+  f3 c = mk((float)q[6], (float)q[7], (float)q[8]);
+  *n = nrmz(crs(sub3(b, a), sub3(c, a)));
+  return f;
+}
+
+/* _apply_bc (threed.py:289-349) followed by the caller's mesh_query_ray. */
+static int oc_query_bc(const oc_bvh* B, const oc_box* b, f3* o, f3 d, float* t, f3* n) {
+  if (!b->periodic) return oc_query(B, *o, d, t, n);
+  for (int w = 0; w < 256; ++w) {
+    int f = oc_query(B, *o, d, t, n);
+    if (f >= 0) return f;
+    float tx = 1.0e30f, ty = 1.0e30f, tz = 1.0e30f;
+    if (d.x > 1.0e-9f) tx = (b->Lx - o->x) / d.x;
+    if (d.x < -1.0e-9f) tx = (0.f - o->x) / d.x;
+    if (d.y > 1.0e-9f) ty = (b->Ly - o->y) / d.y;
+    if (d.y < -1.0e-9f) ty = (0.f - o->y) / d.y;
+    if (d.z > 1.0e-9f) tz = (b->Lz - o->z) / d.z;
+    if (tx < tz && tx < ty && tx > 1.0e-7f) {
+      float hy = o->y + tx * d.y, hz = o->z + tx * d.z;
+      *o = mk(d.x < 0.f ? b->Lx - 1.0e-4f : 1.0e-4f, hy, hz);
+    } else if (ty < tz && ty < 1.0e29f && ty > 1.0e-7f) {
+      float hx = o->x + ty * d.x, hz = o->z + ty * d.z;
+      *o = mk(hx, d.y < 0.f ? b->Ly - 1.0e-4f : 1.0e-4f, hz);
+    } else {
+      return -1;
+    }
+  }
+  return oc_query(B, *o, d, t, n);
+}
+
+/* cosine re-emission about n (threed.py:395-405) */
+static f3 oc_cosine(oc_rng* r, f3 n) {
+  f3 a = fabsf(n.x) > 0.9f ? mk(0.f, 1.f, 0.f) : mk(1.f, 0.f, 0.f);
+  f3 t = nrmz(crs(a, n));
+  f3 b = crs(n, t);
+  float r1 = oc_uniform(r), r2 = oc_uniform(r);
+  float ct = sqrtf(r1), st = sqrtf(1.f - r1), ph = 6.2831853f * r2;
+  return add3(add3(scl(st * cosf(ph), t), scl(st * sinf(ph), b)), scl(ct, n));
+}
+
+/* _trace3d_cov_rr (threed.py:549-596); per-face sums in double. */
+void oracle_trace_neutral_rr(void* h, const oc_box* b, const oc_source* s, const float* bare,
+                             float beta, int n, uint64_t seed, double* fl) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  int F = B->nf;
+  memset(fl, 0, sizeof(double) * F);
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc(F > 0 ? F : 1, sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+    for (int p = 0; p < n; ++p) {
+      oc_rng r;
+      oc_rng_init(&r, seed, (uint32_t)p);
+      f3 o, d, nn;
+      oc_source_ray(&r, b, s, &o, &d);
+      float w = 1.f, t;
+      for (int bounce = 0; bounce < 1024; ++bounce) {
+        int f = oc_query_bc(B, b, &o, d, &t, &nn);
+        if (f < 0) break;
+        if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
+        acc[f] += w;
+        float S = bare[f] * beta;
+        w = w - w * S;
+        if (w <= 1.0e-6f) break;
+        if (w < 0.1f) {
+          if (oc_uniform(&r) < (1.f - w / 0.3f)) break;
+          w = 0.3f;
+        }
+        f3 hit = add3(o, scl(t, d));
+        d = oc_cosine(&r, nn);
+        o = add3(hit, scl(1.0e-4f, nn));
+      }
+    }
+#pragma omp critical
+    for (int f = 0; f < F; ++f) fl[f] += acc[f];
+    free(acc);
+  }
+}
+
+/* _trace3d_ion_yield (threed.py:599-712); out = 5 x F doubles (sp, ie, ip, arr, ang). */
+void oracle_trace_ion_yield(void* h, const oc_box* b, const oc_source* s, const oc_ion* P, int n,
+                            uint64_t seed, double* out) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  int F = B->nf;
+  memset(out, 0, sizeof(double) * 5 * F);
+  const float PI = 3.14159265f, PI_2 = 1.5707963f;
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc(F > 0 ? 5 * F : 1, sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+    for (int p = 0; p < n; ++p) {
+      oc_rng r;
+      oc_rng_init(&r, seed, (uint32_t)p);
+      f3 o, d, nn;
+      oc_source_ray(&r, b, s, &o, &d);
+      float E = P->meanE + P->sigmaE * oc_normal(&r);
+      if (E < 0.f) E = 0.f;
+      float w = 1.f, t;
+      float sEsp = sqrtf(P->Eth_sp), sEie = sqrtf(P->Eth_ie), sEp = sqrtf(P->Eth_p);
+      for (int bounce = 0; bounce < 1024; ++bounce) {
+        int f = oc_query_bc(B, b, &o, d, &t, &nn);
+        if (f < 0) break;
+        if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
+        float cosT = -dot3(d, nn);
+        if (cosT < 0.f) cosT = 0.f;
+        if (cosT > 1.f) cosT = 1.f;
+        float incAng = acosf(cosT);
+        float sqrtE = sqrtf(E);
+        float f_sp = (1.f + P->B_sp * (1.f - cosT * cosT)) * cosT;
+        if (f_sp < 0.f) f_sp = 0.f;
+        float f_ie = 1.f;
+        if (cosT < 0.5f) {
+          f_ie = 3.f - 6.f * incAng / PI;
+          if (f_ie < 0.f) f_ie = 0.f;
+        }
+        acc[f] += P->A_sp * fmaxf(sqrtE - sEsp, 0.f) * f_sp * w;
+        acc[F + f] += P->A_ie * fmaxf(sqrtE - sEie, 0.f) * f_ie * w;
+        acc[2 * F + f] += P->A_p * fmaxf(sqrtE - sEp, 0.f) * f_ie * w;
+        acc[3 * F + f] += w;
+        acc[4 * F + f] += cosT * w;
+        float stick = 1.f;
+        if (incAng > P->thrmin) {
+          float rr = (incAng - P->thrmin) / (P->thrmax - P->thrmin);
+          if (rr < 0.f) rr = 0.f;
+          if (rr > 1.f) rr = 1.f;
+          stick = 1.f - rr;
+        }
+        if (stick >= 1.f) break;
+        float Eref;
+        if (incAng >= P->inflect)
+          Eref = 1.f - (1.f - P->A_en) * (PI_2 - incAng) / (PI_2 - P->inflect);
+        else
+          Eref = P->A_en * powf(incAng / P->inflect, P->n_l);
+        float newE = Eref * E;
+        for (int e = 0; e < 4; ++e) {
+          float cand = Eref * E + 0.1f * E * oc_normal(&r);
+          if (cand >= 0.f && cand <= E) {
+            newE = cand;
+            break;
+          }
+        }
+        if (newE < 0.f) newE = 0.f;
+        if (newE > E) newE = E;
+        if (newE <= P->minE) break;
+        E = newE;
+        w = w - w * stick;
+        if (w <= 1.0e-4f) break;
+        f3 hit = add3(o, scl(t, d));
+        f3 axis = nrmz(sub3(d, scl(2.f * dot3(d, nn), nn)));
+        f3 tt, bb;
+        if (axis.z < -0.9999999f) {
+          tt = mk(0.f, -1.f, 0.f);
+          bb = mk(-1.f, 0.f, 0.f);
+        } else {
+          float aa = 1.f / (1.f + axis.z);
+          float cc = -axis.x * axis.y * aa;
+          tt = mk(1.f - axis.x * axis.x * aa, cc, -axis.x);
+          bb = mk(cc, 1.f - axis.y * axis.y * aa, -axis.y);
+        }
+        float maxcone = PI_2 - fminf(incAng, P->minang);
+        float theta = 0.f;
+        for (int k = 0; k < 64; ++k) {
+          float uu = sqrtf(oc_uniform(&r));
+          float ss = sqrtf(fmaxf(1.f - uu, 0.f));
+          theta = maxcone * ss;
+          if (oc_uniform(&r) * theta * uu <= cosf(PI_2 * ss) * sinf(theta)) break;
+        }
+        float phi = 6.2831853f * oc_uniform(&r);
+        f3 nd = nrmz(add3(scl(sinf(theta), add3(scl(cosf(phi), tt), scl(sinf(phi), bb))),
+                          scl(cosf(theta), axis)));
+        float dn = dot3(nd, nn);
+        if (dn <= 0.f) nd = sub3(nd, scl(2.f * dn, nn));
+        d = nd;
+        o = add3(hit, scl(1.0e-4f, nn));
+      }
+    }
+#pragma omp critical
+    for (int f = 0; f < 5 * F; ++f) out[f] += acc[f];
+    free(acc);
+  }
+}
+
+/* _trace3d (threed.py:352-406); out = 2 x F doubles (flux, angacc). */
+void oracle_trace_legacy(void* h, const oc_box* b, const oc_source* s, float sticking,
+                         int n_reemit, int specular, float cos_thr, float eta, int n,
+                         uint64_t seed, double* out) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  int F = B->nf;
+  memset(out, 0, sizeof(double) * 2 * F);
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc(F > 0 ? 2 * F : 1, sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+    for (int p = 0; p < n; ++p) {
+      oc_rng r;
+      oc_rng_init(&r, seed, (uint32_t)p);
+      f3 o, d, nn;
+      oc_source_ray(&r, b, s, &o, &d);
+      float w = 1.f, t;
+      for (int bounce = 0; bounce <= n_reemit; ++bounce) {
+        int f = oc_query(B, o, d, &t, &nn);
+        if (f < 0) break;
+        if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
+        float cosang = -dot3(d, nn);
+        if (cosang < 0.f) cosang = 0.f;
+        if (specular) {
+          if (cosang >= cos_thr || bounce == n_reemit) {
+            acc[f] += w;
+            acc[F + f] += w * cosang;
+            break;
+          }
+          float R = (cos_thr - cosang) / cos_thr;
+          float dep = w * (1.f - R);
+          acc[f] += dep;
+          acc[F + f] += dep * cosang;
+          w = w * R * eta;
+          f3 hit = add3(o, scl(t, d));
+          d = sub3(d, scl(2.f * dot3(d, nn), nn));
+          o = add3(hit, scl(1.0e-4f, nn));
+        } else {
+          float u = oc_uniform(&r);
+          if (u < sticking || bounce == n_reemit) {
+            acc[f] += w;
+            acc[F + f] += w * cosang;
+            break;
+          }
+          f3 hit = add3(o, scl(t, d));
+          d = oc_cosine(&r, nn);
+          o = add3(hit, scl(1.0e-4f, nn));
+        }
+      }
+    }
+#pragma omp critical
+    for (int f = 0; f < 2 * F; ++f) out[f] += acc[f];
+    free(acc);
+  }
+}
+
+/* _trace_ff (threed.py:908-920) with the launch of threed.py:949-955. */
+void oracle_trace_ff(void* h, const oc_box* b, const float* cen, const float* gn, int nf, int n_ff,
+                     uint64_t seed, int* hitface) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  long long N = (long long)nf * n_ff;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long p = 0; p < N; ++p) {
+    int j = (int)(p / n_ff);
+    oc_rng r;
+    oc_rng_init(&r, seed, (uint32_t)p);
+    f3 nn = mk(gn[3 * j], gn[3 * j + 1], gn[3 * j + 2]);
+    f3 o = add3(mk(cen[3 * j], cen[3 * j + 1], cen[3 * j + 2]), scl(1.0e-3f, nn));
+    f3 a = fabsf(nn.x) > 0.9f ? mk(0.f, 1.f, 0.f) : mk(1.f, 0.f, 0.f);
+    f3 c = crs(a, nn);
+    f3 t = scl(1.f / (sqrtf(dot3(c, c)) + 1.0e-12f), c);
+    f3 bb = crs(nn, t);
+    float u1 = oc_uniform(&r), u2 = oc_uniform(&r);
+    float ct = sqrtf(u1), st = sqrtf(1.f - u1), ph = 6.2831853f * u2;
+    f3 d = add3(add3(scl(st * cosf(ph), t), scl(st * sinf(ph), bb)), scl(ct, nn));
+    float tt;
+    f3 n2;
+    hitface[p] = oc_query_bc(B, b, &o, d, &tt, &n2);
+  }
+}

@@ -8,8 +8,8 @@
 //   tris:  nf leaves of 48 B in Morton order: v0 (w = face id bits), v1, v2 as float4, so a
 //          leaf is three 16-byte loads and carries the face id the flux deposits into.
 // Traversal keeps a short per-lane stack of far children (the near child is descended first);
-// a stack overflow drops the far subtree and raises an error flag instead of writing out of
-// bounds.
+// a stack overflow drops the far subtree and raises error flag 1 instead of writing out of
+// bounds; an out-of-range reference or a step count beyond a tree's ends the query with flag 2.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -137,6 +137,7 @@ struct Hit {
 };
 
 __device__ __forceinline__ void leaf_test(const BvhView& b, const RayPre& r, int slot, Hit& h) {
+  if (slot < 0 || slot >= b.nf) return;
   float4 q0 = b.tris[3 * slot], q1 = b.tris[3 * slot + 1], q2 = b.tris[3 * slot + 2];
   float t;
   if (ray_tri(r, v3(q0.x, q0.y, q0.z), v3(q1.x, q1.y, q1.z), v3(q2.x, q2.y, q2.z), h.t, t)) {
@@ -158,7 +159,13 @@ __device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d,
   int stack[kBvhStack];
   int sp = 0;
   int node = 0;
-  while (true) {
+  // a tree visits each internal node at most once; the bound only guards a corrupt tree
+  const int max_steps = 2 * b.nf + 64;
+  for (int steps = 0;; ++steps) {
+    if (steps > max_steps || node < 0 || node >= b.nf - 1) {
+      if (b.err) *b.err = 2;
+      break;
+    }
     const BvhNode* N = b.nodes + node;
     float4 A = N->a, B = N->b, C = N->c;
     int4 D = N->d;
@@ -239,6 +246,7 @@ struct PointHit {
 };
 
 __device__ __forceinline__ void leaf_point(const BvhView& b, Vec3 p, int slot, PointHit& h) {
+  if (slot < 0 || slot >= b.nf) return;
   float4 q0 = b.tris[3 * slot], q1 = b.tris[3 * slot + 1], q2 = b.tris[3 * slot + 2];
   Vec3 c = closest_on_tri(p, v3(q0.x, q0.y, q0.z), v3(q1.x, q1.y, q1.z), v3(q2.x, q2.y, q2.z));
   Vec3 e = p - c;
@@ -260,7 +268,12 @@ __device__ __forceinline__ PointHit bvh_closest_point(const BvhView& b, Vec3 p, 
   int stack[kBvhStack];
   int sp = 0;
   int node = 0;
-  while (true) {
+  const int max_steps = 2 * b.nf + 64;
+  for (int steps = 0;; ++steps) {
+    if (steps > max_steps || node < 0 || node >= b.nf - 1) {
+      if (b.err) *b.err = 2;
+      break;
+    }
     const BvhNode* N = b.nodes + node;
     float4 A = N->a, B = N->b, C = N->c;
     int4 D = N->d;

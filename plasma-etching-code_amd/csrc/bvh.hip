@@ -211,7 +211,7 @@ __global__ void k_bvh_refit(const float4* __restrict__ tris, BvhNode* nodes,
   float hi[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z))};
   int child = ~s;
   int p = pleaf[s];
-  while (p >= 0) {
+  for (int guard = 0; p >= 0 && p < n - 1 && guard < n; ++guard) {
     BvhNode* nd = nodes + p;
     bool isleft = nd->d.x == child;
     put_child_box(nd, isleft, lo, hi);

@@ -1,0 +1,414 @@
+// Ray-traced flux kernels. One lane per primary ray; each lane owns its Philox stream (seed, ray),
+// generates its source ray in registers (no origin/direction buffers), walks its bounces through
+// the LBVH and deposits into per-face float accumulators with hardware float atomics.
+//
+//   k_trace_neutral_rr   _trace3d_cov_rr    threed.py:549-596 (RR-weighted neutral, <= 1024 bounces)
+//   k_trace_ion_yield    _trace3d_ion_yield threed.py:599-712 (faithful ion, 5 deposits per hit)
+//   k_trace_legacy       _trace3d           threed.py:352-406 (sticking neutral / specular ion)
+//   k_trace_ff           _trace_ff          threed.py:908-920 (first hit of face-emitted rays)
+//   k_gen_source         _gen_source        threed.py:441-458 (same streams, for checking)
+// Periodic lateral boundaries follow _apply_bc (threed.py:289-349); the hit found inside the wrap
+// loop is the hit of the caller's query (the reference repeats the same query), so it is reused.
+#include "common.h"
+#include "bvh.h"
+#include "rng.h"
+
+using petch::BvhView;
+using petch::Hit;
+using petch::Rng;
+using petch::Vec3;
+using petch::v3;
+
+petch::BvhView petch_bvh_view(const petch_bvh* b);
+
+namespace {
+
+constexpr int kBS = 256;
+constexpr float kPi = 3.14159265f;
+constexpr float kPi2 = 1.5707963f;
+constexpr float kTwoPi = 6.2831853f;
+
+struct Box {
+  float lx, ly, lz, zsrc;
+  int periodic;
+};
+
+enum SourceKind { SRC_NEUTRAL = 0, SRC_ION_GAUSS = 1, SRC_ION_POWCOS = 2 };
+
+struct Source {
+  int kind;
+  float sigma;     // Gaussian tilt (rad)
+  float exponent;  // power-cosine exponent
+};
+
+// Source ray on the plane z = zsrc, uniform in [0,Lx) x [0,Ly): ion = Gaussian tilt
+// (threed.py:450-453) or power-cosine cone about -z; neutral = cosine hemisphere about -z
+// (threed.py:454-457).
+__device__ __forceinline__ void gen_source(Rng& r, const Box& b, const Source& s, Vec3& o,
+                                           Vec3& d) {
+  float ox = r.uniform() * b.lx;
+  float oy = r.uniform() * b.ly;
+  o = v3(ox, oy, b.zsrc);
+  if (s.kind == SRC_ION_GAUSS) {
+    float ax = r.normal() * s.sigma;
+    float ay = r.normal() * s.sigma;
+    d = petch::normalize(v3(sinf(ax), sinf(ay), -cosf(ax) * cosf(ay)));
+  } else if (s.kind == SRC_ION_POWCOS) {
+    float ct = powf(r.uniform(), 1.f / (s.exponent + 1.f));
+    float st = sqrtf(fmaxf(1.f - ct * ct, 0.f));
+    float ph = kTwoPi * r.uniform();
+    d = petch::normalize(v3(st * cosf(ph), st * sinf(ph), -ct));
+  } else {
+    float r1 = r.uniform(), r2 = r.uniform();
+    float ct = sqrtf(r1), st = sqrtf(1.f - r1), ph = kTwoPi * r2;
+    d = petch::normalize(v3(st * cosf(ph), st * sinf(ph), -ct));
+  }
+}
+
+// Cosine-weighted direction about the unit normal n (threed.py:395-405).
+__device__ __forceinline__ Vec3 cosine_about(Rng& r, Vec3 n) {
+  Vec3 a = (fabsf(n.x) > 0.9f) ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+  Vec3 t = petch::normalize(petch::cross(a, n));
+  Vec3 bb = petch::cross(n, t);
+  float r1 = r.uniform(), r2 = r.uniform();
+  float ct = sqrtf(r1), st = sqrtf(1.f - r1), ph = kTwoPi * r2;
+  return (st * cosf(ph)) * t + (st * sinf(ph)) * bb + ct * n;
+}
+
+// _apply_bc + mesh_query_ray: with periodic lateral faces a ray that misses the surface and
+// leaves through x or y before the open top re-enters at the opposite face (<= 256 wraps).
+__device__ __forceinline__ Hit trace_bc(const BvhView& bv, const Box& b, Vec3& o, Vec3 d) {
+  if (!b.periodic) return petch::bvh_closest_hit(bv, o, d, 1.0e6f);
+  for (int w = 0; w < 256; ++w) {
+    Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f);
+    if (h.face >= 0) return h;
+    float tx = 1.0e30f, ty = 1.0e30f, tz = 1.0e30f;
+    if (d.x > 1.0e-9f) tx = (b.lx - o.x) / d.x;
+    if (d.x < -1.0e-9f) tx = (0.f - o.x) / d.x;
+    if (d.y > 1.0e-9f) ty = (b.ly - o.y) / d.y;
+    if (d.y < -1.0e-9f) ty = (0.f - o.y) / d.y;
+    if (d.z > 1.0e-9f) tz = (b.lz - o.z) / d.z;
+    if (tx < tz && tx < ty && tx > 1.0e-7f) {
+      float hy = o.y + tx * d.y, hz = o.z + tx * d.z;
+      o = v3(d.x < 0.f ? b.lx - 1.0e-4f : 1.0e-4f, hy, hz);
+    } else if (ty < tz && ty < 1.0e29f && ty > 1.0e-7f) {
+      float hx = o.x + ty * d.x, hz = o.z + ty * d.z;
+      o = v3(hx, d.y < 0.f ? b.ly - 1.0e-4f : 1.0e-4f, hz);
+    } else {
+      return h;  // leaves through the open top
+    }
+  }
+  return petch::bvh_closest_hit(bv, o, d, 1.0e6f);
+}
+
+__device__ __forceinline__ Vec3 hit_normal(const BvhView& bv, int slot) {
+  float4 q0 = bv.tris[3 * slot], q1 = bv.tris[3 * slot + 1], q2 = bv.tris[3 * slot + 2];
+  Vec3 a = v3(q0.x, q0.y, q0.z);
+  return petch::normalize(petch::cross(v3(q1.x, q1.y, q1.z) - a, v3(q2.x, q2.y, q2.z) - a));
+}
+
+__device__ __forceinline__ void deposit(float* __restrict__ acc, int face, float v) {
+  unsafeAtomicAdd(acc + face, v);
+}
+
+__global__ void __launch_bounds__(kBS) k_trace_neutral_rr(BvhView bv, Box b, Source src,
+                                                          const float* __restrict__ bare,
+                                                          float beta, int n, uint64_t seed,
+                                                          float* __restrict__ fl) {
+  int p = blockIdx.x * kBS + threadIdx.x;
+  if (p >= n) return;
+  Rng r;
+  r.init(seed, (uint32_t)p);
+  Vec3 o, d;
+  gen_source(r, b, src, o, d);
+  float w = 1.f;
+  for (int bounce = 0; bounce < 1024; ++bounce) {
+    Hit h = trace_bc(bv, b, o, d);
+    if (h.face < 0) break;
+    Vec3 nn = hit_normal(bv, h.slot);
+    if (petch::dot(nn, d) > 0.f) nn = -nn;
+    deposit(fl, h.face, w);
+    float S = bare[h.face] * beta;
+    w = w - w * S;
+    if (w <= 1.0e-6f) break;
+    if (w < 0.1f) {
+      if (r.uniform() < (1.f - w / 0.3f)) break;
+      w = 0.3f;
+    }
+    Vec3 hit = o + h.t * d;
+    d = cosine_about(r, nn);
+    o = hit + 1.0e-4f * nn;
+  }
+}
+
+struct IonParams {
+  float meanE, sigmaE, A_sp, Eth_sp, B_sp, A_ie, Eth_ie, A_p, Eth_p;
+  float inflect, minang, n_l, A_en, minE, thrmin, thrmax;
+};
+
+__global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Source src,
+                                                         IonParams P, int n, uint64_t seed,
+                                                         float* __restrict__ sp,
+                                                         float* __restrict__ ie,
+                                                         float* __restrict__ ip,
+                                                         float* __restrict__ arr,
+                                                         float* __restrict__ ang) {
+  int p = blockIdx.x * kBS + threadIdx.x;
+  if (p >= n) return;
+  Rng r;
+  r.init(seed, (uint32_t)p);
+  Vec3 o, d;
+  gen_source(r, b, src, o, d);
+  float E = P.meanE + P.sigmaE * r.normal();
+  if (E < 0.f) E = 0.f;
+  float w = 1.f;
+  const float sEsp = sqrtf(P.Eth_sp), sEie = sqrtf(P.Eth_ie), sEp = sqrtf(P.Eth_p);
+  for (int bounce = 0; bounce < 1024; ++bounce) {
+    Hit h = trace_bc(bv, b, o, d);
+    if (h.face < 0) break;
+    Vec3 nn = hit_normal(bv, h.slot);
+    if (petch::dot(nn, d) > 0.f) nn = -nn;
+    float cosT = fminf(fmaxf(-petch::dot(d, nn), 0.f), 1.f);
+    float incAng = acosf(cosT);
+    float sqrtE = sqrtf(E);
+    float f_sp = fmaxf((1.f + P.B_sp * (1.f - cosT * cosT)) * cosT, 0.f);
+    float f_ie = 1.f;
+    if (cosT < 0.5f) f_ie = fmaxf(3.f - 6.f * incAng / kPi, 0.f);
+    deposit(sp, h.face, P.A_sp * fmaxf(sqrtE - sEsp, 0.f) * f_sp * w);
+    deposit(ie, h.face, P.A_ie * fmaxf(sqrtE - sEie, 0.f) * f_ie * w);
+    deposit(ip, h.face, P.A_p * fmaxf(sqrtE - sEp, 0.f) * f_ie * w);
+    deposit(arr, h.face, w);
+    deposit(ang, h.face, cosT * w);
+    float stick = 1.f;
+    if (incAng > P.thrmin) {
+      float rr = (incAng - P.thrmin) / (P.thrmax - P.thrmin);
+      rr = fminf(fmaxf(rr, 0.f), 1.f);
+      stick = 1.f - rr;
+    }
+    if (stick >= 1.f) break;
+    float Eref;
+    if (incAng >= P.inflect)
+      Eref = 1.f - (1.f - P.A_en) * (kPi2 - incAng) / (kPi2 - P.inflect);
+    else
+      Eref = P.A_en * powf(incAng / P.inflect, P.n_l);
+    float newE = Eref * E;
+    for (int e = 0; e < 4; ++e) {
+      float cand = Eref * E + 0.1f * E * r.normal();
+      if (cand >= 0.f && cand <= E) {
+        newE = cand;
+        break;
+      }
+    }
+    newE = fminf(fmaxf(newE, 0.f), E);
+    if (newE <= P.minE) break;
+    E = newE;
+    w = w - w * stick;
+    if (w <= 1.0e-4f) break;
+    Vec3 hit = o + h.t * d;
+    Vec3 axis = petch::normalize(d - (2.f * petch::dot(d, nn)) * nn);
+    Vec3 tt, bb;
+    if (axis.z < -0.9999999f) {
+      tt = v3(0.f, -1.f, 0.f);
+      bb = v3(-1.f, 0.f, 0.f);
+    } else {
+      float aa = 1.f / (1.f + axis.z);
+      float cc = -axis.x * axis.y * aa;
+      tt = v3(1.f - axis.x * axis.x * aa, cc, -axis.x);
+      bb = v3(cc, 1.f - axis.y * axis.y * aa, -axis.y);
+    }
+    float maxcone = kPi2 - fminf(incAng, P.minang);
+    float theta = 0.f;
+    for (int t = 0; t < 64; ++t) {
+      float uu = sqrtf(r.uniform());
+      float ss = sqrtf(fmaxf(1.f - uu, 0.f));
+      theta = maxcone * ss;
+      if (r.uniform() * theta * uu <= cosf(kPi2 * ss) * sinf(theta)) break;
+    }
+    float phi = kTwoPi * r.uniform();
+    Vec3 nd = petch::normalize(sinf(theta) * (cosf(phi) * tt + sinf(phi) * bb) + cosf(theta) * axis);
+    float dn = petch::dot(nd, nn);
+    if (dn <= 0.f) nd = nd - (2.f * dn) * nn;
+    d = nd;
+    o = hit + 1.0e-4f * nn;
+  }
+}
+
+__global__ void __launch_bounds__(kBS) k_trace_legacy(BvhView bv, Source src, Box b,
+                                                      float sticking, int n_reemit, int specular,
+                                                      float cos_thr, float eta, int n,
+                                                      uint64_t seed, float* __restrict__ flux,
+                                                      float* __restrict__ angacc) {
+  int p = blockIdx.x * kBS + threadIdx.x;
+  if (p >= n) return;
+  Rng r;
+  r.init(seed, (uint32_t)p);
+  Vec3 o, d;
+  gen_source(r, b, src, o, d);
+  float w = 1.f;
+  for (int bounce = 0; bounce <= n_reemit; ++bounce) {
+    Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f);
+    if (h.face < 0) break;
+    Vec3 nn = hit_normal(bv, h.slot);
+    if (petch::dot(nn, d) > 0.f) nn = -nn;
+    float cosang = fmaxf(-petch::dot(d, nn), 0.f);
+    if (specular) {
+      if (cosang >= cos_thr || bounce == n_reemit) {
+        deposit(flux, h.face, w);
+        deposit(angacc, h.face, w * cosang);
+        break;
+      }
+      float R = (cos_thr - cosang) / cos_thr;
+      float dep = w * (1.f - R);
+      deposit(flux, h.face, dep);
+      deposit(angacc, h.face, dep * cosang);
+      w = w * R * eta;
+      Vec3 hit = o + h.t * d;
+      d = d - (2.f * petch::dot(d, nn)) * nn;
+      o = hit + 1.0e-4f * nn;
+    } else {
+      float u = r.uniform();
+      if (u < sticking || bounce == n_reemit) {
+        deposit(flux, h.face, w);
+        deposit(angacc, h.face, w * cosang);
+        break;
+      }
+      Vec3 hit = o + h.t * d;
+      d = cosine_about(r, nn);
+      o = hit + 1.0e-4f * nn;
+    }
+  }
+}
+
+// n_ff cosine rays per face from centroid + 1e-3 n (threed.py:949-955); hit face or -1.
+__global__ void __launch_bounds__(kBS) k_trace_ff(BvhView bv, Box b,
+                                                  const float* __restrict__ cen,
+                                                  const float* __restrict__ gn, int nf, int n_ff,
+                                                  uint64_t seed, int* __restrict__ hitface) {
+  long long p = (long long)blockIdx.x * kBS + threadIdx.x;
+  if (p >= (long long)nf * n_ff) return;
+  int j = (int)(p / n_ff);
+  Rng r;
+  r.init(seed, (uint32_t)p);
+  Vec3 nn = v3(gn[3 * j], gn[3 * j + 1], gn[3 * j + 2]);
+  Vec3 o = v3(cen[3 * j], cen[3 * j + 1], cen[3 * j + 2]) + 1.0e-3f * nn;
+  // _cosine_dirs (threed.py:1571-1579): t = cross(a, n) / (|cross(a, n)| + 1e-12)
+  Vec3 a = (fabsf(nn.x) > 0.9f) ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+  Vec3 c = petch::cross(a, nn);
+  Vec3 t = (1.f / (petch::length(c) + 1.0e-12f)) * c;
+  Vec3 bb = petch::cross(nn, t);
+  float u1 = r.uniform(), u2 = r.uniform();
+  float ct = sqrtf(u1), st = sqrtf(1.f - u1), ph = kTwoPi * u2;
+  Vec3 d = (st * cosf(ph)) * t + (st * sinf(ph)) * bb + ct * nn;
+  Hit h = trace_bc(bv, b, o, d);
+  hitface[p] = h.face;
+}
+
+__global__ void k_gen_source(Box b, Source src, int n, uint64_t seed, float* __restrict__ origin,
+                             float* __restrict__ dir) {
+  int p = blockIdx.x * kBS + threadIdx.x;
+  if (p >= n) return;
+  Rng r;
+  r.init(seed, (uint32_t)p);
+  Vec3 o, d;
+  gen_source(r, b, src, o, d);
+  origin[3 * p] = o.x;
+  origin[3 * p + 1] = o.y;
+  origin[3 * p + 2] = o.z;
+  dir[3 * p] = d.x;
+  dir[3 * p + 1] = d.y;
+  dir[3 * p + 2] = d.z;
+}
+
+Box make_box(const petch_box* b) { return Box{b->Lx, b->Ly, b->Lz, b->z_src, b->periodic}; }
+
+Source make_source(const petch_source* s) {
+  return Source{s->kind, s->sigma, s->exponent};
+}
+
+int zero(petch_ctx* ctx, float* p, int n) {
+  if (p && n > 0) PETCH_HIP(ctx, hipMemsetAsync(p, 0, (size_t)n * sizeof(float), ctx->stream));
+  return PETCH_OK;
+}
+
+}  // namespace
+
+extern "C" int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petch_source* src,
+                                int n, uint64_t seed, float* origin, float* dir) {
+  PETCH_ARG(ctx, ctx && box && src && origin && dir, "gen_source: null pointer");
+  PETCH_ARG(ctx, n >= 0, "gen_source: n < 0");
+  if (n == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_gen_source, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
+                     make_box(box), make_source(src), n, seed, origin, dir);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                                      const petch_source* src, const float* bare, float beta,
+                                      int n, uint64_t seed, float* fl) {
+  PETCH_ARG(ctx, ctx && bvh && box && src && bare && fl, "trace_neutral_rr: null pointer");
+  PETCH_ARG(ctx, n >= 0, "trace_neutral_rr: n < 0");
+  int rc = zero(ctx, fl, bvh->nf);
+  if (rc) return rc;
+  if (n == 0 || bvh->nf == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), make_box(box), make_source(src), bare, beta, n, seed, fl);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                                     const petch_source* src, const petch_ion_params* ip, int n,
+                                     uint64_t seed, float* sp, float* ie, float* ipass,
+                                     float* arr, float* ang) {
+  PETCH_ARG(ctx, ctx && bvh && box && src && ip && sp && ie && ipass && arr && ang,
+            "trace_ion_yield: null pointer");
+  PETCH_ARG(ctx, n >= 0, "trace_ion_yield: n < 0");
+  float* outs[5] = {sp, ie, ipass, arr, ang};
+  for (int k = 0; k < 5; ++k) {
+    int rc = zero(ctx, outs[k], bvh->nf);
+    if (rc) return rc;
+  }
+  if (n == 0 || bvh->nf == 0) return PETCH_OK;
+  IonParams P{ip->meanE,  ip->sigmaE, ip->A_sp,    ip->Eth_sp, ip->B_sp, ip->A_ie,
+              ip->Eth_ie, ip->A_p,    ip->Eth_p,   ip->inflect, ip->minang, ip->n_l,
+              ip->A_en,   ip->minE,   ip->thrmin,  ip->thrmax};
+  hipLaunchKernelGGL(k_trace_ion_yield, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), make_box(box), make_source(src), P, n, seed, sp, ie,
+                     ipass, arr, ang);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_trace_legacy(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                                  const petch_source* src, float sticking, int n_reemit,
+                                  int specular, float cos_thr, float eta, int n, uint64_t seed,
+                                  float* flux, float* angacc) {
+  PETCH_ARG(ctx, ctx && bvh && box && src && flux && angacc, "trace_legacy: null pointer");
+  PETCH_ARG(ctx, n >= 0 && n_reemit >= 0, "trace_legacy: bad size");
+  int rc = zero(ctx, flux, bvh->nf);
+  if (!rc) rc = zero(ctx, angacc, bvh->nf);
+  if (rc) return rc;
+  if (n == 0 || bvh->nf == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_trace_legacy, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), make_source(src), make_box(box), sticking, n_reemit,
+                     specular, cos_thr, eta, n, seed, flux, angacc);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_trace_ff(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                              const float* centroids, const float* gas_normals, int nf, int n_ff,
+                              uint64_t seed, int* hitface) {
+  PETCH_ARG(ctx, ctx && bvh && box && centroids && gas_normals && hitface,
+            "trace_ff: null pointer");
+  PETCH_ARG(ctx, nf >= 0 && n_ff >= 0, "trace_ff: bad size");
+  long long n = (long long)nf * n_ff;
+  PETCH_ARG(ctx, n < (1LL << 31), "trace_ff: nf*n_ff too large");
+  if (n == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_trace_ff, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), make_box(box), centroids, gas_normals, nf, n_ff, seed,
+                     hitface);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}

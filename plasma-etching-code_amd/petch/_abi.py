@@ -41,6 +41,24 @@ _ERR = {PETCH_E_ARG: PetchArgError, PETCH_E_CAPACITY: PetchCapacityError,
         PETCH_E_NONFINITE: PetchNonFiniteError}
 
 P = c_void_p
+
+
+class Box(ctypes.Structure):
+    """petch_box: ray-tracer box (source plane z_src, lateral periodic wrap flag)."""
+    _fields_ = [("Lx", c_float), ("Ly", c_float), ("Lz", c_float), ("z_src", c_float),
+                ("periodic", c_int)]
+
+
+class Source(ctypes.Structure):
+    """petch_source: 0 neutral cosine, 1 ion Gaussian tilt, 2 ion power-cosine."""
+    _fields_ = [("kind", c_int), ("sigma", c_float), ("exponent", c_float)]
+
+
+class IonParams(ctypes.Structure):
+    _fields_ = [(k, c_float) for k in ("meanE", "sigmaE", "A_sp", "Eth_sp", "B_sp", "A_ie",
+                                       "Eth_ie", "A_p", "Eth_p", "inflect", "minang", "n_l",
+                                       "A_en", "minE", "thrmin", "thrmax")]
+
 # name -> argument types (all entry points return int)
 SIGNATURES = {
     "petch_ctx_create": [c_int, POINTER(c_void_p)],
@@ -64,6 +82,14 @@ SIGNATURES = {
     "petch_bvh_query_point": [P, P, P, c_int, c_float, P, P],
     "petch_bvh_status": [P, P, POINTER(c_int)],
     "petch_bvh_root_bounds": [P, P, P],
+    "petch_gen_source": [P, POINTER(Box), POINTER(Source), c_int, c_uint64, P, P],
+    "petch_trace_neutral_rr": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int, c_uint64,
+                               P],
+    "petch_trace_ion_yield": [P, P, POINTER(Box), POINTER(Source), POINTER(IonParams), c_int,
+                              c_uint64, P, P, P, P, P],
+    "petch_trace_legacy": [P, P, POINTER(Box), POINTER(Source), c_float, c_int, c_int, c_float,
+                           c_float, c_int, c_uint64, P, P],
+    "petch_trace_ff": [P, P, POINTER(Box), P, P, c_int, c_int, c_uint64, P],
 }
 
 _lib = None

@@ -189,3 +189,68 @@ class Bvh:
                 self.h = None
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ ray tracing
+from ._abi import Box, IonParams, Source  # noqa: E402
+
+SRC_NEUTRAL, SRC_ION_GAUSS, SRC_ION_POWCOS = 0, 1, 2
+
+
+def make_box(Lx, Ly, Lz, z_src, periodic=0):
+    return Box(float(Lx), float(Ly), float(Lz), float(z_src), int(bool(periodic)))
+
+
+def make_source(kind, sigma=0.0, exponent=0.0):
+    return Source(int(kind), float(sigma), float(exponent))
+
+
+def gen_source(box, src, n, seed, device=None):
+    import ctypes
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    o = torch.empty((n, 3), dtype=f32, device=dev)
+    d = torch.empty((n, 3), dtype=f32, device=dev)
+    if n:
+        Context.get(dev).call("petch_gen_source", ctypes.byref(box), ctypes.byref(src), int(n),
+                              int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(o), ptr(d))
+    return o, d
+
+
+def trace_neutral_rr(bvh, box, src, bare, beta, n, seed, out=None):
+    """Russian-roulette neutral flux accumulator (F,) (threed.py:549-596)."""
+    import ctypes
+    fl = torch.empty((bvh.nf,), dtype=f32, device=bvh.device) if out is None else out
+    bvh.ctx.call("petch_trace_neutral_rr", bvh.h, ctypes.byref(box), ctypes.byref(src),
+                 ptr(bare, f32, "bare"), float(beta), int(n), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                 ptr(fl, f32))
+    return fl
+
+
+def trace_ion_yield(bvh, box, src, ion, n, seed, out=None):
+    """Faithful ion accumulators (5, F): sp, ie, ip, arrival, weighted cos (threed.py:599-712)."""
+    import ctypes
+    acc = torch.empty((5, bvh.nf), dtype=f32, device=bvh.device) if out is None else out
+    bvh.ctx.call("petch_trace_ion_yield", bvh.h, ctypes.byref(box), ctypes.byref(src),
+                 ctypes.byref(ion), int(n), int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(acc[0]),
+                 ptr(acc[1]), ptr(acc[2]), ptr(acc[3]), ptr(acc[4]))
+    return acc
+
+
+def trace_legacy(bvh, box, src, sticking, n_reemit, specular, cos_thr, eta, n, seed):
+    """_trace3d accumulators (2, F): flux, weighted cos (threed.py:352-406)."""
+    import ctypes
+    acc = torch.empty((2, bvh.nf), dtype=f32, device=bvh.device)
+    bvh.ctx.call("petch_trace_legacy", bvh.h, ctypes.byref(box), ctypes.byref(src),
+                 float(sticking), int(n_reemit), int(specular), float(cos_thr), float(eta), int(n),
+                 int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(acc[0]), ptr(acc[1]))
+    return acc
+
+
+def trace_ff(bvh, box, centroids, gas_normals, n_ff, seed):
+    """First-hit face of n_ff cosine rays per face, -1 = escape (threed.py:908-920, 949-955)."""
+    import ctypes
+    nf = int(centroids.shape[0])
+    hf = torch.empty((nf * int(n_ff),), dtype=i32, device=centroids.device)
+    bvh.ctx.call("petch_trace_ff", bvh.h, ctypes.byref(box), ptr(centroids, f32),
+                 ptr(gas_normals, f32), nf, int(n_ff), int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(hf))
+    return hf
