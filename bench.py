@@ -1,0 +1,199 @@
+"""Throughput of the MI355X etch-loop hot path on the single-flux-step-at-scale workload.
+
+Workload (one GPU): pre-carved trench 2 um wide and 20 um deep (aspect ratio 10) with a 2 um
+mask, dx = 0.1 um, level set 160 x 40 x 260 float32; per etch step: marching cubes -> LBVH ->
+coupled flux with 8M power-cosine ions (exponent 1000, 100 eV) and 5 flux/coverage fixed-point
+iterations of 8M F + 8M O Russian-roulette neutrals (88M primary rays per step) -> Belen rate ->
+velocity extension -> CFL substeps of upwind advection -> FSM reinit. A step is one such pass;
+the surface keeps etching from step to step.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line: steps/s (value), rays/s, ray-hits/s, the roofline of the dominant kernel
+(the neutral trace) measured with HIP events on the launch stream, and the CPU baseline (the
+oracle's C/OpenMP restatement timed on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "plasma-etching-code_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "3D etch steps/s & rays/s (1 feature, 64-batch); fwd+adjoint / fwd cost ratio"
+HBM_PEAK_GBS = 8000.0
+NEUTRAL_BYTES_PER_HIT = 48.0     # 36 B triangle + 4 B bare + 8 B accumulator read-modify-write
+ION_BYTES_PER_HIT = 76.0         # 36 B triangle + 5 x 8 B accumulators
+
+# workload: the AR-10 trench of the single-flux-step-at-scale configuration
+W = dict(Lx=16.0, Ly=4.0, Lz=26.0, dx=0.1, width=2.0, floor=2.0, sub_top=22.0, mask_th=2.0)
+N_ION = 8_000_000
+N_NEU = 8_000_000           # per species per fixed-point iteration (F and O: 16M per iteration)
+N_FP = 5
+DT = 0.075
+
+
+def workload_par():
+    from petch import PAR
+    from petch.api import SF6O2
+    m = SF6O2()
+    par = dict(m.par)
+    par.update(periodic_y=1, n_fp=N_FP, ion_source='powcos', ion_exponent=1000.0, Emean=100.0)
+    return par, m.flags
+
+
+def build_loop(n_steps, timer=None):
+    import torch
+    from petch import Flags, ops
+    from petch.threed import EtchLoop, grid_shape
+    dev = torch.device("cuda", 0)
+    nx, ny, nz = grid_shape(W['Lx'], W['Ly'], W['Lz'], W['dx'])
+    # carved trench: solid below the floor and beside the 2 um slot up to the mask top
+    phi, _ = ops.init_sdf(nx, ny, nz, W['dx'], W['Lx'], W['Ly'], W['width'],
+                          W['sub_top'] + W['mask_th'] - W['floor'], W['floor'], False, device=dev)
+    _, mask = ops.init_sdf(nx, ny, nz, W['dx'], W['Lx'], W['Ly'], W['width'], W['mask_th'],
+                           W['sub_top'], False, device=dev)
+    import numpy as np
+    geo = dict(xs=np.arange(nx) * W['dx'], ys=np.arange(ny) * W['dx'], zs=np.arange(nz) * W['dx'],
+               dx=W['dx'], Lx=W['Lx'], Ly=W['Ly'], Lz=W['Lz'], sub_top=W['sub_top'],
+               trench_width=W['width'], hole=False)
+    par, fl = workload_par()
+    flags = Flags(**fl)
+    loop = EtchLoop(phi, mask, geo, W['mask_th'], DT * n_steps, n_steps, par, flags, N_ION, N_NEU,
+                    reinit_method="fsm", timer=timer)
+    return loop, (nx, ny, nz)
+
+
+def cpu_baseline(scale=100):
+    """Oracle (C/OpenMP traces + FP64 numpy level set) on the same workload with the ray counts
+    divided by `scale`; the ray-proportional part is scaled back up."""
+    import numpy as np
+    from oracle import flux as ofl
+    from oracle import levelset as ol
+    from oracle import chem as oc
+    from oracle import native as on
+    from petch import Flags
+    par, fl = workload_par()
+    flags = Flags(**fl)
+    L = W
+    phi = ol.init_sdf(L['Lx'], L['Ly'], L['Lz'], L['dx'], L['width'],
+                      L['sub_top'] + L['mask_th'] - L['floor'], L['floor'], False).astype(np.float32)
+    mask = ol.trench_mask(L['Lx'], L['Ly'], L['Lz'], L['dx'], L['width'], L['mask_th'],
+                          L['sub_top'], False)
+    geo = dict(Lx=L['Lx'], Ly=L['Ly'], Lz=L['Lz'], dx=L['dx'], sub_top=L['sub_top'])
+    t0 = time.time()
+    v, f = on.marching_cubes(phi, L['dx'])
+    M = ofl.OracleMesh(v, f)
+    t_mesh = time.time() - t0
+    t0 = time.time()
+    out = ofl.coupled_flux(M, geo, par, N_ION // scale, N_NEU // scale, 0, flags)
+    t_flux = time.time() - t0
+    t0 = time.time()
+    is_mask = oc.faces_in_mask(M.cen, geo, L['mask_th'], L['width'], hole=False)
+    V = np.nan_to_num(oc.surface_rate(out['m_i'], out['m_F'], out['m_O'], out['cos_i'], is_mask,
+                                      par, flags, ion_yield=out['ion_yield']))
+    Fs = ofl.extend(M, V.astype(np.float32), phi, L['dx'], 4 * L['dx'])
+    vmx = max(float(V.max()), 1e-6)
+    nsub = max(1, min(int(np.ceil(vmx * DT / (0.4 * L['dx']))), 160))
+    p = ol.advect_pinned(phi.astype(np.float64), Fs, mask, phi.astype(np.float64), L['dx'],
+                         DT / nsub, nsub)
+    ol.reinit_fsm(p, L['dx'], 6 * L['dx'])
+    t_rest = time.time() - t0
+    step_s = t_mesh + t_rest + t_flux * scale
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=1.0 / step_s, unit="steps/s", cores=cores, kind="port",
+                sample=(f"one etch step of the same trench state on the oracle: marching cubes, "
+                        f"rate, extension, {nsub} advection substeps and FSM reinit at full size "
+                        f"({t_mesh + t_rest:.1f} s) plus the coupled flux at 1/{scale} of the rays "
+                        f"({t_flux:.1f} s, scaled x{scale}); traces in C with OpenMP, level set "
+                        f"in single-threaded numpy"))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the etch step of one feature runs on one device "
+              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from petch import ops
+    from petch.profiling import StageTimer
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    timer = StageTimer(enabled=True)
+    total = a.warmup + a.steps
+    loop, shape = build_loop(total, timer)
+    print(f"bench: trench {shape} n_ion={N_ION} n_neu={N_NEU}x2x{N_FP}", file=sys.stderr, flush=True)
+    for s in range(a.warmup):
+        t = time.time()
+        loop.step(s)
+        torch.cuda.synchronize()
+        print(f"bench: warmup step {s} {time.time() - t:.2f}s faces={loop.last['nf']}",
+              file=sys.stderr, flush=True)
+    timer.resolve()
+    timer.reset()
+    ops.trace_hit_counts(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    faces = []
+    for s in range(a.warmup, total):
+        info = loop.step(s)
+        faces.append(info['nf'])
+        print(f"bench: step {s} faces={info['nf']} nsub={info['nsub']}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.time() - t0
+    hits_n, hits_i = ops.trace_hit_counts(reset=True)
+    timer.resolve()
+    ms_step = 1000.0 * wall / a.steps
+    rays_step = N_ION + N_FP * 2 * N_NEU
+    nt_ms = timer.ms["neutral_trace"]
+    nt_cnt = max(timer.count["neutral_trace"], 1)
+    avg_launch_ms = nt_ms / nt_cnt
+    bytes_launch = NEUTRAL_BYTES_PER_HIT * hits_n / nt_cnt
+    achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
+    res = {
+        "metric": METRIC,
+        "value": a.steps / wall,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "single flux step at scale: AR-10 trench (W 2 um, depth 20 um, "
+                               "2 um mask, dx 0.1 um, 160x40x260), 8M power-cosine ions "
+                               "(n=1000, 100 eV) + 5 fixed-point iterations x (8M F + 8M O) "
+                               "Russian-roulette neutrals per step, periodic lateral BC",
+                   "grid": list(shape), "faces_mean": sum(faces) / len(faces),
+                   "rays_per_step": rays_step},
+        "rays_per_s": rays_step * a.steps / wall,
+        "ray_hits_per_s": (hits_n + hits_i) / wall,
+        "stages_ms_per_step": {k: v / a.steps for k, v in sorted(timer.ms.items())},
+        "roofline": {"kernel": "k_trace_neutral_rr", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_launch_ms,
+                     "hits_per_launch": hits_n / nt_cnt,
+                     "bytes_per_hit": NEUTRAL_BYTES_PER_HIT},
+    }
+    if not a.no_cpu_baseline:
+        try:
+            res["cpu_baseline"] = cpu_baseline()
+        except Exception as e:                              # reported, not fatal
+            res["cpu_baseline"] = {"value": None, "error": repr(e)}
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

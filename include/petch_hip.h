@@ -56,6 +56,21 @@ typedef struct petch_ion_params {
   float inflect, minang, n_l, A_en, minE, thrmin, thrmax;
 } petch_ion_params;
 
+/* Chemistry constants (PAR, params.py:13-76) with the energy-integrated yields Yie/Ysp/Yp of
+ * chemistry._yields (chemistry.py:34-39, evaluated on the host); angular_viennaps selects
+ * Flags.yield_angular == 'viennaps', chemistry_belen selects Flags.chemistry == 'belen'. */
+typedef struct petch_chem_params {
+  double ionFlux, Fflux, Oflux, k_sigma, beta_sigma, rho, rate_scale, B_sp, fnorm_E, fnorm_O;
+  double Yie, Ysp, Yp, s_F, s_O;
+  int angular_viennaps, chemistry_belen;
+} petch_chem_params;
+
+/* Mask geometry of faces_in_mask (threed.py:1608-1616). */
+typedef struct petch_mask_geom {
+  double sub_top, mask_th, Lx, Ly, trench_width;
+  int hole;
+} petch_mask_geom;
+
 /* ---------------------------------------------------------------- context ---------------- */
 /* Replaces the implicit Warp runtime state (threed.py:22-23 wp.init / DEVICE). */
 int petch_ctx_create(int device, petch_ctx** out);
@@ -164,11 +179,71 @@ int petch_trace_legacy(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* bo
                        const petch_source* src, float sticking, int n_reemit, int specular,
                        float cos_thr, float eta, int n, uint64_t seed, float* flux,
                        float* angacc);
+/* Ray-hit counters accumulated by petch_trace_neutral_rr (counts_host[0]) and
+ * petch_trace_ion_yield (counts_host[1]) on this context since the last reset (work accounting
+ * for throughput reports; no reference counterpart). counts_host may be NULL; reset != 0 zeroes
+ * them after the read. Synchronises when reading. */
+int petch_trace_hit_counts(petch_ctx* ctx, int reset, unsigned long long* counts_host);
 /* _trace_ff (threed.py:908-920) with the launch of threed.py:949-955: n_ff cosine rays from
  * centroid + 1e-3*n of every face; hitface[j*n_ff + k] = first face hit or -1 (escape). */
 int petch_trace_ff(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
                    const float* centroids, const float* gas_normals, int nf, int n_ff,
                    uint64_t seed, int* hitface);
+
+/* ---------------------------------------------------------------- a11-a13, a16: per face - */
+/* _norm_clip (threed.py:165-170): out = clamp((fl / max(area,1e-9)) * scale, lo, hi); pass
+ * hi = +inf for the unclipped faithful-ion yields (threed.py:814-821). */
+int petch_normalize(petch_ctx* ctx, const float* fl, const float* areas, int F, float scale,
+                    float lo, float hi, float* out);
+/* _cos_k (threed.py:179-185): out = ang/fl where fl > 0, else 0. */
+int petch_cos_ratio(petch_ctx* ctx, const float* fl, const float* ang, int F, float* out);
+/* smooth_flux (threed.py:129-148, blend_each = 0: n_iter passes then one blend with the input)
+ * or smooth_flux_dev applied n_iter times (threed.py:195-214, blend_each = 1). nbr is the table
+ * of petch_face_adjacency, normals unit face normals. src != out. */
+int petch_smooth(petch_ctx* ctx, const float* src, const int* nbr, const float* normals, int F,
+                 int n_iter, float alpha, int blend_each, float* out);
+/* _belen_coverages + the bare update (threed.py:715-737, 903-904): bare = clip(1-thF-thO, 0, 1).
+ * With iy_ie/iy_ip (the normalised faithful-ion yields, par['_ion_yield']) GY comes from them,
+ * else from m_i, cos_i and the analytic yields. thF/thO may be NULL. */
+int petch_belen_coverage(petch_ctx* ctx, const float* m_i, const float* m_F, const float* m_O,
+                         const float* cos_i, const float* iy_ie, const float* iy_ip,
+                         const petch_chem_params* cp, int F, float* bare, float* thF,
+                         float* thO);
+/* surface_rate (chemistry.py:83-92 -> belen.py:26-63 or chemistry.py:62-80), V = 0 on mask
+ * faces, non-finite V -> 0 (threed.py:1723-1728). Mask faces are is_mask[f] != 0 when is_mask is
+ * given, else faces_in_mask of the centroids (threed.py:1608-1616). red[0] = max(V) as float
+ * bits (V >= 0), red[1] = number of non-finite values replaced. iy_* may be NULL (analytic
+ * yields). */
+int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float* m_F, const float* m_O,
+                       const float* cos_i, const float* iy_sp, const float* iy_ie,
+                       const float* iy_ip, const float* centroids, const uint8_t* is_mask,
+                       const petch_mask_geom* mg, const petch_chem_params* cp, int F, float* V,
+                       unsigned* red);
+
+/* ---------------------------------------------------------------- a15, a17 --------------- */
+/* extend_velocity_gpu + _extend_kernel (threed.py:1324-1346): Fs = V[nearest face] at nodes
+ * with |phi| < band, 0 elsewhere (Fs is the full (nx,ny,nz) grid). */
+int petch_extend(petch_ctx* ctx, const petch_bvh* bvh, const float* V, const float* phi, int nx,
+                 int ny, int nz, double dx, double band, float* Fs);
+/* _nearest_face_cov / gpu_warmstart_bare (threed.py:1619-1639): out[i] = src[nearest face of
+ * pts[i]] on the given BVH, `miss` when the mesh is empty. */
+int petch_nearest_face_value(petch_ctx* ctx, const petch_bvh* bvh, const float* src,
+                             const float* pts, int n, float miss, float* out);
+
+/* ---------------------------------------------------------------- a21: radiosity -------- */
+/* Form-factor assembly of mc_flux_3d_radiosity (threed.py:956-960) from the petch_trace_ff
+ * output: D[j] = escapes of face j / n_ff (float64), and the rays j -> i as a CSR of rows i
+ * (row_ptr (nf+1), col up to nf*n_ff entries holding the source face of each ray, rows in ray
+ * order). */
+int petch_radiosity_assemble(petch_ctx* ctx, const int* hitface, int nf, int n_ff, int* row_ptr,
+                             int* col, double* D);
+/* Solve G = D + Aff diag(1 - clip(bare*beta,0,1)) G (threed.py:965-985) in float64 by Jacobi
+ * from G0 = D: n_iter > 0 sweeps, or n_iter <= 0 until max|dG| <= tol*max|G| (checked every 50
+ * sweeps, at most 20000). G (nf, float64) receives the solution, out = clip(G, 0, 8) as float32.
+ * *iters_host (may be NULL) = sweeps done. */
+int petch_radiosity_solve(petch_ctx* ctx, const int* row_ptr, const int* col, const double* D,
+                          int nf, int n_ff, const float* bare, double beta, int n_iter, double tol,
+                          double* G, float* out, int* iters_host);
 
 #ifdef __cplusplus
 }

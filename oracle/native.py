@@ -125,6 +125,14 @@ class OracleBvh:
                                      _p(face), _p(t))
         return face, t
 
+    def closest_point(self, pts):
+        p = np.ascontiguousarray(pts, np.float32)
+        n = len(p)
+        face = np.zeros(n, np.int32)
+        dist = np.zeros(n, np.float64)
+        lib().oracle_bvh_closest_point(self.h, _p(p), ctypes.c_int(n), _p(face), _p(dist))
+        return face, dist
+
     def trace_neutral_rr(self, box, src, bare, beta, n, seed):
         bare = np.ascontiguousarray(bare, np.float32)
         out = np.zeros(self.nf, np.float64)

@@ -549,8 +549,9 @@ static void oc_source_ray(oc_rng* r, const oc_box* b, const oc_source* s, f3* o,
     float ay = oc_normal(r) * s->sigma;
     *d = nrmz(mk(sinf(ax), sinf(ay), -cosf(ax) * cosf(ay)));
   } else if (s->kind == 2) {
-    float ct = powf(oc_uniform(r), 1.f / (s->exponent + 1.f));
-    float st = sqrtf(fmaxf(1.f - ct * ct, 0.f));
+    float L = logf(fmaxf(oc_uniform(r), 1.0e-30f)) / (s->exponent + 1.f);
+    float ct = expf(L);
+    float st = sqrtf(fmaxf(-expm1f(L) * (1.f + ct), 0.f));
     float ph = 6.2831853f * oc_uniform(r);
     *d = nrmz(mk(st * cosf(ph), st * sinf(ph), -ct));
   } else {
@@ -841,5 +842,55 @@ void oracle_trace_ff(void* h, const oc_box* b, const float* cen, const float* gn
     float tt;
     f3 n2;
     hitface[p] = oc_query_bc(B, b, &o, d, &tt, &n2);
+  }
+}
+
+/* Closest face to each point with the oracle BVH (box-distance pruning; exact in double).
+ * Ties keep the first face found; the tests accept any face at the minimum distance. */
+static double oc_box_d2(const oc_node* nd, const double p[3]) {
+  double s = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double d = 0.0;
+    if (p[k] < nd->lo[k]) d = nd->lo[k] - p[k];
+    else if (p[k] > nd->hi[k]) d = p[k] - nd->hi[k];
+    s += d * d;
+  }
+  return s;
+}
+
+void oracle_bvh_closest_point(void* h, const float* pts, int n, int* face_out, double* dist_out) {
+  const oc_bvh* B = (const oc_bvh*)h;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    int best = -1;
+    double bd = 1e300;
+    if (B->nf > 0) {
+      int stack[128], sp = 0;
+      stack[sp++] = 0;
+      while (sp) {
+        const oc_node* nd = &B->nodes[stack[--sp]];
+        if (oc_box_d2(nd, p) > bd) continue;
+        if (nd->left < 0) {
+          for (int k = nd->start; k < nd->start + nd->count; ++k) {
+            int f = B->order[k];
+            const double* t = B->tri + 9 * f;
+            double q[3];
+            oc_closest_on_tri(p, t, t + 3, t + 6, q);
+            double dd = (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
+                        (p[2] - q[2]) * (p[2] - q[2]);
+            if (dd < bd) {
+              bd = dd;
+              best = f;
+            }
+          }
+        } else {
+          stack[sp++] = nd->left;
+          stack[sp++] = nd->right;
+        }
+      }
+    }
+    face_out[i] = best;
+    dist_out[i] = best >= 0 ? sqrt(bd) : -1.0;
   }
 }

@@ -28,6 +28,7 @@ enum PetchScratch {
   SCR_RAD_A,         // radiosity CSR build
   SCR_RAD_B,
   SCR_RAD_C,
+  SCR_SMOOTH,        // flux smoothing ping-pong
   SCR_COUNT
 };
 

@@ -1,0 +1,326 @@
+// Per-face and per-node kernels between the traces and the level set:
+//   petch_normalize       m = clamp((fl/max(A,1e-9))*scale, lo, hi)      threed.py:165-170,814-821
+//   petch_cos_ratio       cos_i = ang/fl where fl > 0 else 0            threed.py:179-185,821
+//   petch_smooth          edge-neighbour normal-weighted smoothing       threed.py:129-148,195-214
+//   petch_belen_coverage  Belen coverages -> bare = clip(1-thF-thO,0,1)  threed.py:715-737,903-904
+//   petch_surface_rate    V per face (belen / langmuir), mask faces 0,  chemistry.py:62-92,
+//                         non-finite -> 0 (counted), running max of V   belen.py:26-63,
+//                                                                       threed.py:1608-1616,1721-1728,1750
+//   petch_extend          Fs[node] = V[nearest face] in |phi| < band     threed.py:1324-1346
+//   petch_nearest_face_value  out[i] = src[nearest face of pts[i]]       threed.py:1619-1639
+// The chemistry is evaluated in float64 (the reference evaluates it on the host in float64) and
+// rounded to float32 once.
+#include "common.h"
+#include "bvh.h"
+
+petch::BvhView petch_bvh_view(const petch_bvh* b);
+
+namespace {
+
+constexpr int kBS = 256;
+
+__global__ void k_normalize(const float* __restrict__ fl, const float* __restrict__ area, int F,
+                            float scale, float lo, float hi, float* __restrict__ out) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  float A = fmaxf(area[f], 1.0e-9f);
+  out[f] = fminf(fmaxf((fl[f] / A) * scale, lo), hi);
+}
+
+__global__ void k_cos_ratio(const float* __restrict__ fl, const float* __restrict__ ang, int F,
+                            float* __restrict__ out) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  float x = fl[f];
+  out[f] = x > 0.f ? ang[f] / x : 0.f;
+}
+
+// One smoothing pass: out_i = (x_i + sum_j w_ij x_j) / (1 + sum_j w_ij), w_ij = max(0, n_i.n_j),
+// j over the (up to 3) edge neighbours; blended out = (1-alpha)*raw + alpha*out when alpha < 1.
+__global__ void k_smooth(const float* __restrict__ x, const int* __restrict__ nbr,
+                         const float* __restrict__ nrm, int F, float alpha,
+                         const float* __restrict__ raw, float* __restrict__ out) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  float nx = nrm[3 * f], ny = nrm[3 * f + 1], nz = nrm[3 * f + 2];
+  float num = x[f], den = 1.f;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    int g = nbr[3 * f + e];
+    if (g < 0 || g >= F) continue;
+    float w = fmaxf(0.f, nx * nrm[3 * g] + ny * nrm[3 * g + 1] + nz * nrm[3 * g + 2]);
+    num += x[g] * w;
+    den += w;
+  }
+  float s = num / fmaxf(den, 1.0e-12f);
+  out[f] = (alpha < 1.f) ? (1.f - alpha) * raw[f] + alpha * s : s;
+}
+
+struct Chem {
+  double ionFlux, Fflux, Oflux, k_sigma, beta_sigma, rho, rate_scale, B_sp, fnE, fnO;
+  double Yie, Ysp, Yp, s_F, s_O;
+  int viennaps, belen;
+};
+
+__device__ __forceinline__ void angular(double cos_i, const Chem& c, double& f_ie, double& f_sp) {
+  double x = fmin(fmax(cos_i, 0.0), 1.0);
+  if (c.viennaps) {
+    f_sp = fmax((1.0 + c.B_sp * (1.0 - x * x)) * x, 0.0);
+    double theta = acos(x);
+    f_ie = (x >= 0.5) ? 1.0 : fmax(3.0 - 6.0 * theta / 3.141592653589793, 0.0);
+  } else {
+    f_ie = x;
+    f_sp = x;
+  }
+}
+
+__global__ void k_belen_coverage(const float* __restrict__ m_i, const float* __restrict__ m_F,
+                                 const float* __restrict__ m_O, const float* __restrict__ cos_i,
+                                 const float* __restrict__ iy_ie, const float* __restrict__ iy_ip,
+                                 Chem c, int F, float* __restrict__ bare,
+                                 float* __restrict__ thF_out, float* __restrict__ thO_out) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  const double eps = 1e-9;
+  double GY_ie, GY_p;
+  if (iy_ie) {
+    GY_ie = c.ionFlux * (double)iy_ie[f];
+    GY_p = c.ionFlux * (double)iy_ip[f];
+  } else {
+    double f_ie, f_sp;
+    angular((double)cos_i[f], c, f_ie, f_sp);
+    double Fi = c.ionFlux * (double)m_i[f];
+    GY_ie = c.Yie * f_ie * Fi;
+    GY_p = c.Yp * f_ie * Fi;
+  }
+  double GbE = c.Fflux * (double)m_F[f] + eps;
+  double GbP = c.Oflux * (double)m_O[f] + eps;
+  double a = (c.k_sigma + 2.0 * GY_ie) / GbE;
+  double b = (c.beta_sigma + GY_p) / GbP;
+  double thF = 1.0 / (1.0 + a * (1.0 + 1.0 / (b + eps)));
+  double thO = 1.0 / (1.0 + b * (1.0 + 1.0 / (a + eps)));
+  double br = 1.0 - thF - thO;
+  bare[f] = (float)fmin(fmax(br, 0.0), 1.0);
+  if (thF_out) thF_out[f] = (float)thF;
+  if (thO_out) thO_out[f] = (float)thO;
+}
+
+struct MaskGeom {
+  double sub_top, top, cx, cy, hw, hw2;
+  int hole;
+};
+
+__global__ void k_surface_rate(const float* __restrict__ m_i, const float* __restrict__ m_F,
+                               const float* __restrict__ m_O, const float* __restrict__ cos_i,
+                               const float* __restrict__ iy_sp, const float* __restrict__ iy_ie,
+                               const float* __restrict__ iy_ip,
+                               const float* __restrict__ centroids,
+                               const uint8_t* __restrict__ is_mask, MaskGeom mg, Chem c, int F,
+                               float* __restrict__ V, unsigned* __restrict__ red) {
+#pragma clang fp contract(off)
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  const double eps = 1e-9;
+  double f_ie, f_sp;
+  angular((double)cos_i[f], c, f_ie, f_sp);
+  double Fi = c.ionFlux * (double)m_i[f];
+  double Yie_a = c.Yie * f_ie, Ysp_a = c.Ysp * f_sp, Yp_a = c.Yp * f_ie;
+  double v;
+  if (c.belen) {
+    double GY_ie, GY_p, sputter, ion_enh;
+    if (iy_ie) {
+      GY_ie = c.ionFlux * (double)iy_ie[f];
+      GY_p = c.ionFlux * (double)iy_ip[f];
+      sputter = c.ionFlux * (double)iy_sp[f];
+      ion_enh = GY_ie;
+    } else {
+      GY_ie = Yie_a * Fi;
+      GY_p = Yp_a * Fi;
+      sputter = Ysp_a * Fi;
+      ion_enh = Yie_a * Fi;
+    }
+    double GbE = c.Fflux * (double)m_F[f] / c.fnE + eps;
+    double GbP = c.Oflux * (double)m_O[f] / c.fnO + eps;
+    double a = (c.k_sigma + 2.0 * GY_ie) / GbE;
+    double b = (c.beta_sigma + GY_p) / GbP;
+    double thF = 1.0 / (1.0 + a * (1.0 + 1.0 / (b + eps)));
+    double rate = c.k_sigma * thF / 4.0 + sputter + thF * ion_enh;
+    v = (1.0 / c.rho) * rate * c.rate_scale;
+  } else {
+    double Fev = c.Fflux * (double)m_F[f], Fp = c.Oflux * (double)m_O[f];
+    double rF = c.s_F * Fev / (Yie_a * Fi + eps);
+    double rO = c.s_O * Fp / (Yp_a * Fi + eps);
+    double br = 1.0 / (1.0 + rF + rO);
+    double thF = rF * br;
+    v = (1.0 / c.rho) * (Yie_a * Fi * thF + Ysp_a * Fi * br) * c.rate_scale;
+  }
+  bool masked;
+  if (is_mask) {
+    masked = is_mask[f] != 0;
+  } else {
+    // faces_in_mask on the centroid (threed.py:1608-1616)
+    double x = centroids[3 * f], y = centroids[3 * f + 1], z = centroids[3 * f + 2];
+    bool in_band = (z >= mg.sub_top) && (z < mg.top);
+    bool opening;
+    if (mg.hole) {
+      double ax = x - mg.cx, ay = y - mg.cy;
+      opening = ax * ax + ay * ay < mg.hw2;
+    } else {
+      opening = fabs(x - mg.cx) < mg.hw;
+    }
+    masked = in_band && !opening;
+  }
+  if (masked) v = 0.0;
+  float vf = (float)v;
+  if (!(fabsf(vf) <= 3.402823466e38f)) {  // NaN or inf
+    vf = 0.f;
+    atomicAdd(red + 1, 1u);
+  }
+  V[f] = vf;
+  // V >= 0 here, so the float bit pattern orders like an unsigned integer
+  atomicMax(red, __float_as_uint(vf > 0.f ? vf : 0.f));
+}
+
+__global__ void k_extend(petch::BvhView bv, const float* __restrict__ V,
+                         const float* __restrict__ phi, int nx, int ny, int nz, double dx,
+                         double band, float* __restrict__ Fs) {
+  long long n = (long long)blockIdx.x * kBS + threadIdx.x;
+  long long N = (long long)nx * ny * nz;
+  if (n >= N) return;
+  float out = 0.f;
+  if (fabs((double)phi[n]) < band) {
+    int k = (int)(n % nz);
+    int j = (int)((n / nz) % ny);
+    int i = (int)(n / ((long long)nz * ny));
+    petch::Vec3 p = petch::v3((float)((double)i * dx), (float)((double)j * dx), (float)((double)k * dx));
+    petch::PointHit h = petch::bvh_closest_point(bv, p, 1.0e6f);
+    if (h.face >= 0) out = V[h.face];
+  }
+  Fs[n] = out;
+}
+
+__global__ void k_nearest_value(petch::BvhView bv, const float* __restrict__ src,
+                                const float* __restrict__ pts, int n, float miss,
+                                float* __restrict__ out) {
+  int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  petch::PointHit h =
+      petch::bvh_closest_point(bv, petch::v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), 1.0e6f);
+  out[i] = h.face >= 0 ? src[h.face] : miss;
+}
+
+Chem make_chem(const petch_chem_params* p) {
+  Chem c;
+  c.ionFlux = p->ionFlux; c.Fflux = p->Fflux; c.Oflux = p->Oflux; c.k_sigma = p->k_sigma;
+  c.beta_sigma = p->beta_sigma; c.rho = p->rho; c.rate_scale = p->rate_scale; c.B_sp = p->B_sp;
+  c.fnE = p->fnorm_E; c.fnO = p->fnorm_O; c.Yie = p->Yie; c.Ysp = p->Ysp; c.Yp = p->Yp;
+  c.s_F = p->s_F; c.s_O = p->s_O; c.viennaps = p->angular_viennaps; c.belen = p->chemistry_belen;
+  return c;
+}
+
+}  // namespace
+
+extern "C" int petch_normalize(petch_ctx* ctx, const float* fl, const float* areas, int F,
+                               float scale, float lo, float hi, float* out) {
+  PETCH_ARG(ctx, ctx && fl && areas && out, "normalize: null pointer");
+  PETCH_ARG(ctx, F >= 0, "normalize: F < 0");
+  if (F == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_normalize, dim3(petch_blocks(F, kBS)), dim3(kBS), 0, ctx->stream, fl, areas,
+                     F, scale, lo, hi, out);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_cos_ratio(petch_ctx* ctx, const float* fl, const float* ang, int F,
+                               float* out) {
+  PETCH_ARG(ctx, ctx && fl && ang && out, "cos_ratio: null pointer");
+  if (F <= 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_cos_ratio, dim3(petch_blocks(F, kBS)), dim3(kBS), 0, ctx->stream, fl, ang,
+                     F, out);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_smooth(petch_ctx* ctx, const float* src, const int* nbr, const float* normals,
+                            int F, int n_iter, float alpha, int blend_each, float* out) {
+  PETCH_ARG(ctx, ctx && src && nbr && normals && out, "smooth: null pointer");
+  PETCH_ARG(ctx, F >= 0, "smooth: F < 0");
+  PETCH_ARG(ctx, src != out, "smooth: src and out must differ");
+  if (F == 0) return PETCH_OK;
+  if (n_iter <= 0 || alpha <= 0.f) {
+    PETCH_HIP(ctx, hipMemcpyAsync(out, src, (size_t)F * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    return PETCH_OK;
+  }
+  float* tmp = (float*)petch_scratch(ctx, SCR_SMOOTH, (size_t)F * 4);
+  if (!tmp) return PETCH_E_HIP;
+  dim3 g(petch_blocks(F, kBS)), b(kBS);
+  // ping-pong so the last pass lands in `out`
+  const float* cur = src;
+  for (int it = 0; it < n_iter; ++it) {
+    float* dst = ((n_iter - 1 - it) % 2 == 0) ? out : tmp;
+    bool last = it == n_iter - 1;
+    // blend_each: every pass blends with its own input (device path, threed.py:195-214);
+    // otherwise only the last pass blends, with the original input (threed.py:129-148)
+    float a = blend_each ? alpha : (last ? alpha : 1.f);
+    const float* raw = blend_each ? cur : src;
+    hipLaunchKernelGGL(k_smooth, g, b, 0, ctx->stream, cur, nbr, normals, F, a, raw, dst);
+    PETCH_LAUNCH_CHECK(ctx);
+    cur = dst;
+  }
+  return PETCH_OK;
+}
+
+extern "C" int petch_belen_coverage(petch_ctx* ctx, const float* m_i, const float* m_F,
+                                    const float* m_O, const float* cos_i, const float* iy_ie,
+                                    const float* iy_ip, const petch_chem_params* cp, int F,
+                                    float* bare, float* thF, float* thO) {
+  PETCH_ARG(ctx, ctx && m_F && m_O && cp && bare, "belen_coverage: null pointer");
+  PETCH_ARG(ctx, (iy_ie && iy_ip) || (m_i && cos_i), "belen_coverage: need ion yields or m_i/cos_i");
+  if (F <= 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_belen_coverage, dim3(petch_blocks(F, kBS)), dim3(kBS), 0, ctx->stream, m_i,
+                     m_F, m_O, cos_i, iy_ie, iy_ip, make_chem(cp), F, bare, thF, thO);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float* m_F,
+                                  const float* m_O, const float* cos_i, const float* iy_sp,
+                                  const float* iy_ie, const float* iy_ip, const float* centroids,
+                                  const uint8_t* is_mask, const petch_mask_geom* mg,
+                                  const petch_chem_params* cp, int F, float* V, unsigned* red) {
+  PETCH_ARG(ctx, ctx && m_i && m_F && m_O && cos_i && mg && cp && V && red,
+            "surface_rate: null pointer");
+  PETCH_ARG(ctx, centroids || is_mask, "surface_rate: need centroids or an explicit mask");
+  PETCH_ARG(ctx, !iy_ie || (iy_sp && iy_ip), "surface_rate: ion yields need all three channels");
+  PETCH_HIP(ctx, hipMemsetAsync(red, 0, 2 * sizeof(unsigned), ctx->stream));
+  if (F <= 0) return PETCH_OK;
+  MaskGeom g{mg->sub_top, mg->sub_top + mg->mask_th, mg->Lx / 2.0, mg->Ly / 2.0,
+             mg->trench_width / 2.0, (mg->trench_width / 2.0) * (mg->trench_width / 2.0),
+             mg->hole};
+  hipLaunchKernelGGL(k_surface_rate, dim3(petch_blocks(F, kBS)), dim3(kBS), 0, ctx->stream, m_i,
+                     m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask, g, make_chem(cp), F,
+                     V, red);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_extend(petch_ctx* ctx, const petch_bvh* bvh, const float* V, const float* phi,
+                            int nx, int ny, int nz, double dx, double band, float* Fs) {
+  PETCH_ARG(ctx, ctx && bvh && V && phi && Fs, "extend: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "extend: bad shape");
+  long long N = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(k_extend, dim3(petch_blocks(N, kBS)), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), V, phi, nx, ny, nz, dx, band, Fs);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_nearest_face_value(petch_ctx* ctx, const petch_bvh* bvh, const float* src,
+                                        const float* pts, int n, float miss, float* out) {
+  PETCH_ARG(ctx, ctx && bvh && src && pts && out, "nearest_face_value: null pointer");
+  if (n <= 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_nearest_value, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), src, pts, n, miss, out);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}

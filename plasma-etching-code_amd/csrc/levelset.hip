@@ -98,29 +98,30 @@ __global__ void k_advect(const float* __restrict__ phi, const float* __restrict_
 }
 
 // ------------------------------------------------------------------------------ FSM reinit
-// Exact Godunov update of sum_d (x - a_d)_+^2 = dx^2 with the staged 1 -> 2 -> 3 term form.
-__device__ __forceinline__ float eik_solve(float a1, float a2, float a3, float dx) {
-  float t;
+// Exact Godunov update of sum_d (x - a_d)_+^2 = dx^2 with the staged 1 -> 2 -> 3 term form,
+// evaluated in float64 relative to the smallest neighbour (y = x - a1) so the three-term root
+// has no cancellation against the O(band) magnitudes; the result is rounded to float32 once.
+__device__ __forceinline__ float eik_solve(float f1, float f2, float f3, double dx) {
+  double a1 = f1, a2 = f2, a3 = f3, t;
   if (a1 > a2) { t = a1; a1 = a2; a2 = t; }
   if (a2 > a3) { t = a2; a2 = a3; a3 = t; }
   if (a1 > a2) { t = a1; a1 = a2; a2 = t; }
-  float x = a1 + dx;
-  if (x <= a2) return x;
-  float d2 = 2.f * dx * dx - (a1 - a2) * (a1 - a2);
-  d2 = fmaxf(d2, 0.f);
-  x = 0.5f * (a1 + a2 + sqrtf(d2));
-  if (x <= a3) return x;
-  float s1 = a1 + a2 + a3;
-  float s2 = a1 * a1 + a2 * a2 + a3 * a3;
-  float d3 = s1 * s1 - 3.f * (s2 - dx * dx);
-  d3 = fmaxf(d3, 0.f);
-  return (s1 + sqrtf(d3)) / 3.f;
+  double x = a1 + dx;
+  if (x <= a2) return (float)x;
+  double b2 = a2 - a1, b3 = a3 - a1;
+  double d2 = fmax(2.0 * dx * dx - b2 * b2, 0.0);
+  double y = 0.5 * (b2 + sqrt(d2));
+  if (a1 + y <= a3) return (float)(a1 + y);
+  double s1 = b2 + b3;
+  double d3 = fmax(s1 * s1 - 3.0 * (b2 * b2 + b3 * b3 - dx * dx), 0.0);
+  return (float)(a1 + (s1 + sqrt(d3)) / 3.0);
 }
 
 // Frozen interface seed: nodes with a sign change to a face neighbour get the sub-cell distance
 // |phi0| / max(|grad phi0|_central, 0.5), capped at 1.8 dx; every other node starts at `big`.
+// Evaluated in float64 from the float32 inputs, rounded once.
 __global__ void k_eik_init(const float* __restrict__ phi0, float* __restrict__ U,
-                           uint8_t* __restrict__ frozen, int nx, int ny, int nz, float inv_dx,
+                           uint8_t* __restrict__ frozen, int nx, int ny, int nz, double dx,
                            float big) {
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long N = (long long)nx * ny * nz;
@@ -133,15 +134,15 @@ __global__ void k_eik_init(const float* __restrict__ phi0, float* __restrict__ U
   float oxm = phi0[i > 0 ? n - sx : n], oxp = phi0[i < nx - 1 ? n + sx : n];
   float oym = phi0[j > 0 ? n - sy : n], oyp = phi0[j < ny - 1 ? n + sy : n];
   float ozm = phi0[k > 0 ? n - 1 : n], ozp = phi0[k < nz - 1 ? n + 1 : n];
-  bool iface = (c * oxm < 0.f) || (c * oxp < 0.f) || (c * oym < 0.f) || (c * oyp < 0.f) ||
-               (c * ozm < 0.f) || (c * ozp < 0.f);
+  // sign change to a face neighbour (products of floats are exact in double)
+  double cd = c;
+  bool iface = (cd * oxm < 0.0) || (cd * oxp < 0.0) || (cd * oym < 0.0) || (cd * oyp < 0.0) ||
+               (cd * ozm < 0.0) || (cd * ozp < 0.0);
   if (iface) {
-    float gx = (oxp - oxm) * 0.5f * inv_dx;
-    float gy = (oyp - oym) * 0.5f * inv_dx;
-    float gz = (ozp - ozm) * 0.5f * inv_dx;
-    float grad0 = fmaxf(sqrtf(gx * gx + gy * gy + gz * gz), 0.5f);
-    float dxl = 1.f / inv_dx;
-    U[n] = fminf(fabsf(c) / grad0, 1.8f * dxl);
+    double h = 0.5 / dx;
+    double gx = ((double)oxp - oxm) * h, gy = ((double)oyp - oym) * h, gz = ((double)ozp - ozm) * h;
+    double grad0 = fmax(sqrt(gx * gx + gy * gy + gz * gz), 0.5);
+    U[n] = (float)fmin(fabs(cd) / grad0, 1.8 * dx);
     frozen[n] = 1;
   } else {
     U[n] = big;
@@ -150,7 +151,7 @@ __global__ void k_eik_init(const float* __restrict__ phi0, float* __restrict__ U
 }
 
 __global__ void k_eik_jacobi(const float* __restrict__ U, const uint8_t* __restrict__ frozen,
-                             float* __restrict__ out, int nx, int ny, int nz, float dx,
+                             float* __restrict__ out, int nx, int ny, int nz, double dx,
                              float big) {
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long N = (long long)nx * ny * nz;
@@ -234,12 +235,10 @@ extern "C" int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int 
   uint8_t* fr = (uint8_t*)petch_scratch(ctx, SCR_LS_FROZEN, N);
   if (!U || !U2 || !fr) return PETCH_E_HIP;
   dim3 g(petch_blocks(N, kBS)), b(kBS);
-  hipLaunchKernelGGL(k_eik_init, g, b, 0, ctx->stream, phi, U, fr, nx, ny, nz, (float)(1.0 / dx),
-                     (float)big);
+  hipLaunchKernelGGL(k_eik_init, g, b, 0, ctx->stream, phi, U, fr, nx, ny, nz, dx, (float)big);
   PETCH_LAUNCH_CHECK(ctx);
   for (int it = 0; it < n_iter; ++it) {
-    hipLaunchKernelGGL(k_eik_jacobi, g, b, 0, ctx->stream, U, fr, U2, nx, ny, nz, (float)dx,
-                       (float)big);
+    hipLaunchKernelGGL(k_eik_jacobi, g, b, 0, ctx->stream, U, fr, U2, nx, ny, nz, dx, (float)big);
     PETCH_LAUNCH_CHECK(ctx);
     float* t = U; U = U2; U2 = t;
   }

@@ -26,16 +26,17 @@ __global__ void k_mesh_attrs(const float* __restrict__ verts, const int* __restr
     cen[3 * f + 1] = (ay + by + cy) / 3.f;
     cen[3 * f + 2] = (az + bz + cz) / 3.f;
   }
-  float ux = bx - ax, uy = by - ay, uz = bz - az;
-  float vx = cx - ax, vy = cy - ay, vz = cz - az;
-  float nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
-  float len = sqrtf(nx * nx + ny * ny + nz * nz);
-  if (area) area[f] = 0.5f * len;
+  // cross product in float64: slivers (vertices near a grid node) keep an accurate normal
+  double ux = (double)bx - ax, uy = (double)by - ay, uz = (double)bz - az;
+  double vx = (double)cx - ax, vy = (double)cy - ay, vz = (double)cz - az;
+  double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+  double len = sqrt(nx * nx + ny * ny + nz * nz);
+  if (area) area[f] = (float)(0.5 * len);
   if (nrm) {
-    float inv = len > 0.f ? 1.f / len : 0.f;
-    nrm[3 * f + 0] = nx * inv;
-    nrm[3 * f + 1] = ny * inv;
-    nrm[3 * f + 2] = nz * inv;
+    double inv = len > 0.0 ? 1.0 / len : 0.0;
+    nrm[3 * f + 0] = (float)(nx * inv);
+    nrm[3 * f + 1] = (float)(ny * inv);
+    nrm[3 * f + 2] = (float)(nz * inv);
   }
 }
 

@@ -23,6 +23,16 @@ petch::BvhView petch_bvh_view(const petch_bvh* b);
 
 namespace {
 
+// Per-context 64-bit hit counters (slot 0 neutral, 1 ion), read by petch_trace_hit_counts.
+unsigned long long* petch_hit_counter(petch_ctx* ctx, int which) {
+  bool fresh = ctx->scr_cap[SCR_TRACE] == 0;
+  unsigned long long* c =
+      (unsigned long long*)petch_scratch(ctx, SCR_TRACE, 4 * sizeof(unsigned long long));
+  if (c && fresh && hipMemsetAsync(c, 0, 4 * sizeof(unsigned long long), ctx->stream) != hipSuccess)
+    return nullptr;
+  return c ? c + which : nullptr;
+}
+
 constexpr int kBS = 256;
 constexpr float kPi = 3.14159265f;
 constexpr float kPi2 = 1.5707963f;
@@ -54,8 +64,10 @@ __device__ __forceinline__ void gen_source(Rng& r, const Box& b, const Source& s
     float ay = r.normal() * s.sigma;
     d = petch::normalize(v3(sinf(ax), sinf(ay), -cosf(ax) * cosf(ay)));
   } else if (s.kind == SRC_ION_POWCOS) {
-    float ct = powf(r.uniform(), 1.f / (s.exponent + 1.f));
-    float st = sqrtf(fmaxf(1.f - ct * ct, 0.f));
+    // cos(theta) = u^(1/(n+1)); 1 - cos from expm1 keeps sin(theta) accurate near the axis
+    float L = logf(fmaxf(r.uniform(), 1.0e-30f)) / (s.exponent + 1.f);
+    float ct = expf(L);
+    float st = sqrtf(fmaxf(-expm1f(L) * (1.f + ct), 0.f));
     float ph = kTwoPi * r.uniform();
     d = petch::normalize(v3(st * cosf(ph), st * sinf(ph), -ct));
   } else {
@@ -111,23 +123,34 @@ __device__ __forceinline__ void deposit(float* __restrict__ acc, int face, float
   unsafeAtomicAdd(acc + face, v);
 }
 
+// Adds the lane's hit count to a 64-bit device counter with one atomic per wavefront.
+__device__ __forceinline__ void count_hits(unsigned long long* __restrict__ ctr, unsigned hits) {
+  if (!ctr) return;
+  unsigned long long v = hits;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
+}
+
 __global__ void __launch_bounds__(kBS) k_trace_neutral_rr(BvhView bv, Box b, Source src,
                                                           const float* __restrict__ bare,
                                                           float beta, int n, uint64_t seed,
-                                                          float* __restrict__ fl) {
+                                                          float* __restrict__ fl,
+                                                          unsigned long long* __restrict__ hits) {
   int p = blockIdx.x * kBS + threadIdx.x;
-  if (p >= n) return;
+  unsigned nh = 0;
   Rng r;
   r.init(seed, (uint32_t)p);
   Vec3 o, d;
   gen_source(r, b, src, o, d);
   float w = 1.f;
-  for (int bounce = 0; bounce < 1024; ++bounce) {
+  for (int bounce = 0; bounce < (p < n ? 1024 : 0); ++bounce) {
     Hit h = trace_bc(bv, b, o, d);
     if (h.face < 0) break;
     Vec3 nn = hit_normal(bv, h.slot);
     if (petch::dot(nn, d) > 0.f) nn = -nn;
     deposit(fl, h.face, w);
+    ++nh;
     float S = bare[h.face] * beta;
     w = w - w * S;
     if (w <= 1.0e-6f) break;
@@ -139,6 +162,7 @@ __global__ void __launch_bounds__(kBS) k_trace_neutral_rr(BvhView bv, Box b, Sou
     d = cosine_about(r, nn);
     o = hit + 1.0e-4f * nn;
   }
+  count_hits(hits, nh);
 }
 
 struct IonParams {
@@ -152,9 +176,10 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
                                                          float* __restrict__ ie,
                                                          float* __restrict__ ip,
                                                          float* __restrict__ arr,
-                                                         float* __restrict__ ang) {
+                                                         float* __restrict__ ang,
+                                                         unsigned long long* __restrict__ hits) {
   int p = blockIdx.x * kBS + threadIdx.x;
-  if (p >= n) return;
+  unsigned nh = 0;
   Rng r;
   r.init(seed, (uint32_t)p);
   Vec3 o, d;
@@ -163,7 +188,7 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
   if (E < 0.f) E = 0.f;
   float w = 1.f;
   const float sEsp = sqrtf(P.Eth_sp), sEie = sqrtf(P.Eth_ie), sEp = sqrtf(P.Eth_p);
-  for (int bounce = 0; bounce < 1024; ++bounce) {
+  for (int bounce = 0; bounce < (p < n ? 1024 : 0); ++bounce) {
     Hit h = trace_bc(bv, b, o, d);
     if (h.face < 0) break;
     Vec3 nn = hit_normal(bv, h.slot);
@@ -179,6 +204,7 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
     deposit(ip, h.face, P.A_p * fmaxf(sqrtE - sEp, 0.f) * f_ie * w);
     deposit(arr, h.face, w);
     deposit(ang, h.face, cosT * w);
+    ++nh;
     float stick = 1.f;
     if (incAng > P.thrmin) {
       float rr = (incAng - P.thrmin) / (P.thrmax - P.thrmin);
@@ -231,6 +257,7 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
     d = nd;
     o = hit + 1.0e-4f * nn;
   }
+  count_hits(hits, nh);
 }
 
 __global__ void __launch_bounds__(kBS) k_trace_legacy(BvhView bv, Source src, Box b,
@@ -352,7 +379,8 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   if (rc) return rc;
   if (n == 0 || bvh->nf == 0) return PETCH_OK;
   hipLaunchKernelGGL(k_trace_neutral_rr, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), make_box(box), make_source(src), bare, beta, n, seed, fl);
+                     petch_bvh_view(bvh), make_box(box), make_source(src), bare, beta, n, seed, fl,
+                     petch_hit_counter(ctx, 0));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
@@ -375,7 +403,7 @@ extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const
               ip->A_en,   ip->minE,   ip->thrmin,  ip->thrmax};
   hipLaunchKernelGGL(k_trace_ion_yield, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
                      petch_bvh_view(bvh), make_box(box), make_source(src), P, n, seed, sp, ie,
-                     ipass, arr, ang);
+                     ipass, arr, ang, petch_hit_counter(ctx, 1));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
@@ -410,5 +438,18 @@ extern "C" int petch_trace_ff(petch_ctx* ctx, const petch_bvh* bvh, const petch_
                      petch_bvh_view(bvh), make_box(box), centroids, gas_normals, nf, n_ff, seed,
                      hitface);
   PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_trace_hit_counts(petch_ctx* ctx, int reset, unsigned long long* counts_host) {
+  PETCH_ARG(ctx, ctx, "trace_hit_counts: null context");
+  unsigned long long* c = petch_hit_counter(ctx, 0);
+  if (!c) return PETCH_E_HIP;
+  if (counts_host) {
+    PETCH_HIP(ctx, hipMemcpyAsync(counts_host, c, 2 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  if (reset) PETCH_HIP(ctx, hipMemsetAsync(c, 0, 4 * sizeof(unsigned long long), ctx->stream));
   return PETCH_OK;
 }

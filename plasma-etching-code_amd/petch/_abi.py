@@ -54,6 +54,20 @@ class Source(ctypes.Structure):
     _fields_ = [("kind", c_int), ("sigma", c_float), ("exponent", c_float)]
 
 
+class ChemParams(ctypes.Structure):
+    """petch_chem_params."""
+    _fields_ = [(k, c_double) for k in ("ionFlux", "Fflux", "Oflux", "k_sigma", "beta_sigma",
+                                        "rho", "rate_scale", "B_sp", "fnorm_E", "fnorm_O", "Yie",
+                                        "Ysp", "Yp", "s_F", "s_O")] + \
+              [("angular_viennaps", c_int), ("chemistry_belen", c_int)]
+
+
+class MaskGeom(ctypes.Structure):
+    """petch_mask_geom."""
+    _fields_ = [(k, c_double) for k in ("sub_top", "mask_th", "Lx", "Ly", "trench_width")] + \
+              [("hole", c_int)]
+
+
 class IonParams(ctypes.Structure):
     _fields_ = [(k, c_float) for k in ("meanE", "sigmaE", "A_sp", "Eth_sp", "B_sp", "A_ie",
                                        "Eth_ie", "A_p", "Eth_p", "inflect", "minang", "n_l",
@@ -90,6 +104,18 @@ SIGNATURES = {
     "petch_trace_legacy": [P, P, POINTER(Box), POINTER(Source), c_float, c_int, c_int, c_float,
                            c_float, c_int, c_uint64, P, P],
     "petch_trace_ff": [P, P, POINTER(Box), P, P, c_int, c_int, c_uint64, P],
+    "petch_trace_hit_counts": [P, c_int, P],
+    "petch_normalize": [P, P, P, c_int, c_float, c_float, c_float, P],
+    "petch_cos_ratio": [P, P, P, c_int, P],
+    "petch_smooth": [P, P, P, P, c_int, c_int, c_float, c_int, P],
+    "petch_belen_coverage": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P],
+    "petch_surface_rate": [P, P, P, P, P, P, P, P, P, P, POINTER(MaskGeom),
+                           POINTER(ChemParams), c_int, P, P],
+    "petch_extend": [P, P, P, P, c_int, c_int, c_int, c_double, c_double, P],
+    "petch_nearest_face_value": [P, P, P, P, c_int, c_float, P],
+    "petch_radiosity_assemble": [P, P, c_int, c_int, P, P, P],
+    "petch_radiosity_solve": [P, P, P, P, c_int, c_int, P, c_double, c_int, c_double, P, P,
+                              POINTER(c_int)],
 }
 
 _lib = None

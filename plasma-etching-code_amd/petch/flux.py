@@ -1,0 +1,221 @@
+"""Device-resident flux stages: the surface mesh of a step and the three neutral-transport
+orchestrators the etch loop dispatches to (threed.py:1687-1718). All tensors stay on the GPU;
+every kernel is in libpetch_hip.so.
+
+  SurfaceMesh          MC + attributes + adjacency + LBVH of one step (threed.py:1673-1675)
+  coupled_flux         mc_flux_3d_coupled    threed.py:740-905
+  radiosity_flux       mc_flux_3d_radiosity  threed.py:923-991 (+ _ions_deterministic 994-1029)
+  uncoupled_flux       mc_flux_3d            threed.py:472-507
+"""
+import math
+
+import torch
+
+from . import ops
+from .chemistry import chem_params
+from .profiling import NULL as _NULL_TIMER
+
+SEED_MASK = 0xFFFFFFFFFFFFFFFF
+
+
+class SurfaceMesh:
+    """The triangle surface of phi with everything the flux and extension stages need."""
+
+    def __init__(self, device):
+        self.device = device
+        self.bufs = ops.MeshBuffers(device)
+        self.bvh = ops.Bvh(device)
+        self.nf = 0
+        self.nv = 0
+
+    def build(self, phi, dx):
+        v, f = ops.marching_cubes(phi, dx, self.bufs)
+        self.verts, self.faces = v, f
+        self.nv, self.nf = int(v.shape[0]), int(f.shape[0])
+        self.centroids, self.areas, self.normals = ops.mesh_attrs(v, f)
+        self.nbr = ops.face_adjacency(f, self.nv)
+        self.bvh.build(v, f)
+        return self
+
+
+def ion_params(par):
+    """Faithful-ion constants as passed at threed.py:806-813 (derived at 800-802)."""
+    inflect = float(par.get('inflectAngle', 1.55334303))
+    minang = float(par.get('minAngle', 1.3962634))
+    n_l = float(par.get('n_l', 10.0))
+    A_en = 1.0 / (1.0 + n_l * (1.5707963 / inflect - 1.0))
+    minE = float(min(par['Eth_ie'], par['Eth_sp']))
+    return ops.IonParams(
+        meanE=float(par['Emean']), sigmaE=float(par['Esig']), A_sp=float(par['A_sp']),
+        Eth_sp=float(par['Eth_sp']), B_sp=float(par['B_sp']), A_ie=float(par['A_ie']),
+        Eth_ie=float(par['Eth_ie']), A_p=float(par['A_p']), Eth_p=float(par['Eth_p']),
+        inflect=inflect, minang=minang, n_l=n_l, A_en=float(A_en), minE=minE,
+        thrmin=float(par.get('thetaRMin', 1.2217305)), thrmax=float(par.get('thetaRMax', 1.5707963)))
+
+
+def ion_source(par):
+    """Ion source: the reference's Gaussian tilt (sigma = ion_ang_sigma, threed.py:450-453), or
+    a power-cosine cone cos^n about -z with par['ion_source'] = 'powcos' and par['ion_exponent']."""
+    if par.get('ion_source', 'gauss') == 'powcos':
+        return ops.make_source(ops.SRC_ION_POWCOS, 0.0, float(par.get('ion_exponent', 1000.0)))
+    return ops.make_source(ops.SRC_ION_GAUSS, float(par['ion_ang_sigma']))
+
+
+def box_of(geo, par):
+    """Tracer box: source plane z = Lz - dx (threed.py:477), periodic wrap from par."""
+    return ops.make_box(geo['Lx'], geo['Ly'], geo['Lz'], geo['Lz'] - geo['dx'],
+                        int(par.get('periodic_y', 0)))
+
+
+def _faithful(flags):
+    return (flags is not None and getattr(flags, "ion_reflection", False)
+            and getattr(flags, "chemistry", "langmuir") == "belen")
+
+
+def _check_unsupported(par, flags):
+    if flags is not None and getattr(flags, "surface_charging", False) and \
+            float(par.get('charge_alpha', 0.0)) > 0.0:
+        raise NotImplementedError("surface charging (threed.py:847-868) is outside the HIP "
+                                  "etch-loop path")
+    if flags is not None and getattr(flags, "floor_charge_throttle", False):
+        raise NotImplementedError("floor_charge_throttle is outside the HIP etch-loop path")
+
+
+def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, timer=_NULL_TIMER):
+    """mc_flux_3d_coupled (threed.py:740-905) on the device.
+
+    Returns dict(m_i, m_F, m_O, cos_i, bare, ion_yield) of (F,) float32 device tensors;
+    ion_yield = (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None."""
+    _check_unsupported(par, flags)
+    F = mesh.nf
+    Lx, Ly = geo['Lx'], geo['Ly']
+    A_src = Lx * Ly
+    betaE = par.get('betaE', 0.7)
+    betaO = par.get('betaO', 1.0)
+    n_smooth = int(par.get('flux_smooth', 1))
+    alpha = float(par.get('flux_smooth_alpha', 1.0))
+    # device-resident normalize+smooth of the reference's GPU path (threed.py:757-759, 783-790)
+    dev = bool(par.get('device_flux', True)) and bool(par.get('flux_smooth_gpu', True))
+    box = box_of(geo, par)
+    A = mesh.areas
+
+    def host_smooth(x):                       # smooth_flux / smooth_flux_gpu semantics
+        return ops.smooth(x, mesh.nbr, mesh.normals, n_smooth, alpha, False) if n_smooth > 0 else x
+
+    def finish(fl, scale, hi):                # _finish (threed.py:783-790) or the host clip+smooth
+        m = ops.normalize(fl, A, scale, 0.0, hi)
+        if n_smooth <= 0:
+            return m
+        return ops.smooth(m, mesh.nbr, mesh.normals, n_smooth, alpha, dev)
+
+    ion_seed = (seed * 9 + 1) & SEED_MASK
+    if _faithful(flags):
+        with timer.region("ion_trace"):
+            acc = ops.trace_ion_yield(mesh.bvh, box, ion_source(par), ion_params(par), n_ion,
+                                      ion_seed)
+        norm = A_src / n_ion
+        ion_yield = tuple(host_smooth(ops.normalize(acc[k], A, norm, -ops.INF, ops.INF))
+                          for k in range(3))
+        m_i = host_smooth(ops.normalize(acc[3], A, norm, -ops.INF, ops.INF))
+        cos_i = ops.cos_ratio(acc[3], acc[4])
+    else:
+        acc = ops.trace_legacy(mesh.bvh, box, ion_source(par), 1.0, 0, 0, 0.34, 0.8, n_ion,
+                               ion_seed)
+        ion_yield = None
+        cos_i = ops.cos_ratio(acc[0], acc[1])
+        m_i = finish(acc[0], A_src / n_ion, 1.5)
+
+    if par.get('cov_transport', 'rr') != 'rr':
+        raise NotImplementedError("cov_transport='cap' (_trace3d_cov) is not part of the HIP path")
+    nsrc = ops.make_source(ops.SRC_NEUTRAL)
+    chem = chem_params(par, flags)
+
+    def neutral(beta, bare, sd):
+        with timer.region("neutral_trace"):
+            fl = ops.trace_neutral_rr(mesh.bvh, box, nsrc, bare, beta, n_neu, sd & SEED_MASK)
+        return finish(fl, A_src / n_neu, 8.0)
+
+    n_fp = int(par.get('n_fp', 1 if bare_init is not None else 4))
+    if bare_init is None:
+        bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
+    else:
+        bare = torch.clamp(bare_init.to(torch.float32), 0.0, 1.0)
+    m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
+    for it in range(n_fp):
+        m_F = neutral(betaE, bare, seed * 9 + 2 + 2 * it)
+        m_O = neutral(betaO, bare, seed * 9 + 3 + 2 * it)
+        bare = ops.belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem)
+    return dict(m_i=m_i, m_F=m_F, m_O=m_O, cos_i=cos_i, bare=bare, ion_yield=ion_yield)
+
+
+def deterministic_ions(mesh, geo, par, n_ion, seed, flags):
+    """_ions_deterministic (threed.py:994-1029): faithful ion without smoothing, or the
+    first-hit ion clipped at 1.5. Returns (m_i, cos_i, ion_yield)."""
+    box = box_of(geo, par)
+    A_src = geo['Lx'] * geo['Ly']
+    A = mesh.areas
+    if _faithful(flags):
+        acc = ops.trace_ion_yield(mesh.bvh, box, ion_source(par), ion_params(par), n_ion,
+                                  seed & SEED_MASK)
+        norm = A_src / n_ion
+        iy = tuple(ops.normalize(acc[k], A, norm, -ops.INF, ops.INF) for k in range(3))
+        m_i = ops.normalize(acc[3], A, norm, -ops.INF, ops.INF)
+        return m_i, ops.cos_ratio(acc[3], acc[4]), iy
+    acc = ops.trace_legacy(mesh.bvh, box, ion_source(par), 1.0, 0, 0, 0.34, 0.8, n_ion,
+                           seed & SEED_MASK)
+    return ops.normalize(acc[0], A, A_src / n_ion, 0.0, 1.5), ops.cos_ratio(acc[0], acc[1]), None
+
+
+def radiosity_flux(mesh, phi, geo, par, n_ion, seed, flags, n_ff=64, n_fp=4):
+    """mc_flux_3d_radiosity (threed.py:923-991): form factors by n_ff first-hit cosine rays per
+    face, then n_fp x {G = D + Aff diag(1-s) G solved for F and O, Belen coverages}."""
+    _check_unsupported(par, flags)
+    F = mesh.nf
+    gn = ops.gas_normals(phi, geo['dx'], mesh.centroids, mesh.normals)
+    m_i, cos_i, ion_yield = deterministic_ions(mesh, geo, par, n_ion, seed * 9 + 1, flags)
+    hit = ops.trace_ff(mesh.bvh, box_of(geo, par), mesh.centroids, gn, n_ff,
+                       (seed * 9 + 5) & SEED_MASK)
+    row_ptr, col, D = ops.radiosity_assemble(hit, F, n_ff)
+    solver = par.get('radiosity_solver', 'jacobi')
+    n_iter = 40 if solver == 'jacobi' else 0
+    betaE = par.get('betaE', 0.7)
+    betaO = par.get('betaO', 1.0)
+    chem = chem_params(par, flags)
+    bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
+    m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
+    for _ in range(n_fp):
+        m_F = ops.radiosity_solve(row_ptr, col, D, F, n_ff, bare, betaE, n_iter)[0]
+        m_O = ops.radiosity_solve(row_ptr, col, D, F, n_ff, bare, betaO, n_iter)[0]
+        bare = ops.belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem)
+    return dict(m_i=m_i, m_F=m_F, m_O=m_O, cos_i=cos_i, bare=bare, ion_yield=ion_yield)
+
+
+def uncoupled_flux(mesh, geo, par, n_ion, n_neu, seed, ion_reflection=False):
+    """mc_flux_3d (threed.py:472-507): legacy ion (specular reflection with 3 re-emissions when
+    ion_reflection), sticking neutrals with 12 re-emissions; no smoothing."""
+    box = box_of(geo, par)
+    A_src = geo['Lx'] * geo['Ly']
+    A = mesh.areas
+    if ion_reflection:
+        acc = ops.trace_legacy(mesh.bvh, box, ion_source(par), 1.0, 3, 1, 0.34, 0.8, n_ion,
+                               (seed * 9 + 1) & SEED_MASK)
+    else:
+        acc = ops.trace_legacy(mesh.bvh, box, ion_source(par), 1.0, 0, 0, 0.34, 0.8, n_ion,
+                               (seed * 9 + 1) & SEED_MASK)
+    nsrc = ops.make_source(ops.SRC_NEUTRAL)
+    aF = ops.trace_legacy(mesh.bvh, box, nsrc, par['s_F'], 12, 0, 0.34, 0.8, n_neu,
+                          (seed * 9 + 2) & SEED_MASK)
+    aO = ops.trace_legacy(mesh.bvh, box, nsrc, par['s_O'], 12, 0, 0.34, 0.8, n_neu,
+                          (seed * 9 + 3) & SEED_MASK)
+    m_i = ops.normalize(acc[0], A, A_src / n_ion, 0.0, 1.5)
+    m_F = ops.normalize(aF[0], A, A_src / n_neu, 0.0, 4.0)
+    m_O = ops.normalize(aO[0], A, A_src / n_neu, 0.0, 4.0)
+    cos_i = ops.cos_ratio(acc[0], acc[1])
+    return dict(m_i=m_i, m_F=m_F, m_O=m_O, cos_i=cos_i, bare=None, ion_yield=None)
+
+
+def cfl_substeps(vmax, dt, dx):
+    """nsub = max(1, min(ceil(Vmax*dt/(0.4dx)), 160)) with Vmax floored at 1e-6
+    (threed.py:1750-1756)."""
+    Vmax = max(vmax if math.isfinite(vmax) else 0.0, 1e-6)
+    return max(1, min(int(math.ceil(Vmax * dt / (0.4 * dx))), 160)), Vmax

@@ -254,3 +254,130 @@ def trace_ff(bvh, box, centroids, gas_normals, n_ff, seed):
     bvh.ctx.call("petch_trace_ff", bvh.h, ctypes.byref(box), ptr(centroids, f32),
                  ptr(gas_normals, f32), nf, int(n_ff), int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(hf))
     return hf
+
+
+# ------------------------------------------------------------------ per-face kernels
+from ._abi import ChemParams, MaskGeom  # noqa: E402
+
+INF = float("inf")
+
+
+def normalize(fl, areas, scale, lo=0.0, hi=INF, out=None):
+    """clamp((fl / max(area,1e-9)) * scale, lo, hi) (threed.py:165-170)."""
+    F = int(fl.shape[0])
+    out = torch.empty_like(fl) if out is None else out
+    if F:
+        _ctx(fl).call("petch_normalize", ptr(fl, f32), ptr(areas, f32), F, float(scale),
+                      float(lo), float(hi), ptr(out, f32))
+    return out
+
+
+def cos_ratio(fl, ang):
+    F = int(fl.shape[0])
+    out = torch.empty_like(fl)
+    if F:
+        _ctx(fl).call("petch_cos_ratio", ptr(fl, f32), ptr(ang, f32), F, ptr(out, f32))
+    return out
+
+
+def smooth(src, nbr, normals, n_iter=1, alpha=1.0, blend_each=False):
+    """Edge-neighbour smoothing (threed.py:129-148; blend_each=True: threed.py:195-214)."""
+    F = int(src.shape[0])
+    out = torch.empty_like(src)
+    if F:
+        _ctx(src).call("petch_smooth", ptr(src, f32), ptr(nbr, i32), ptr(normals, f32), F,
+                       int(n_iter), float(alpha), int(bool(blend_each)), ptr(out, f32))
+    return out
+
+
+def belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem):
+    """bare = clip(1 - thF - thO, 0, 1) with the Belen coverages (threed.py:715-737, 903-904)."""
+    F = int(m_F.shape[0])
+    bare = torch.empty_like(m_F)
+    if F:
+        ie = ion_yield[1] if ion_yield is not None else None
+        ip = ion_yield[2] if ion_yield is not None else None
+        _ctx(m_F).call("petch_belen_coverage", ptr(m_i), ptr(m_F, f32), ptr(m_O, f32), ptr(cos_i),
+                       ptr(ie), ptr(ip), ctypes_ref(chem), F, ptr(bare), ptr(None), ptr(None))
+    return bare
+
+
+def surface_rate(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, is_mask=None,
+                 red=None):
+    """(V (F,), red (2,) uint32 = [max V bits, non-finite count]) on the device."""
+    F = int(m_F.shape[0])
+    V = torch.empty_like(m_F)
+    if red is None:
+        red = torch.zeros((2,), dtype=torch.int32, device=m_F.device)
+    sp, ie, ip = ion_yield if ion_yield is not None else (None, None, None)
+    _ctx(m_F).call("petch_surface_rate", ptr(m_i, f32), ptr(m_F, f32), ptr(m_O, f32),
+                   ptr(cos_i, f32), ptr(sp), ptr(ie), ptr(ip), ptr(centroids),
+                   ptr(is_mask), ctypes_ref(mask_geom), ctypes_ref(chem), F, ptr(V), ptr(red))
+    return V, red
+
+
+def read_rate_reduction(red):
+    """(Vmax, n_nonfinite) from the surface-rate reduction buffer (one device sync)."""
+    import numpy as np
+    h = red.cpu().numpy().astype(np.int32)
+    vmax = float(np.frombuffer(h[:1].tobytes(), dtype=np.float32)[0])
+    return vmax, int(h[1])
+
+
+def extend(bvh, V, phi, dx, band, out=None):
+    """Fs = V[nearest face] where |phi| < band, else 0 (threed.py:1324-1346)."""
+    nx, ny, nz = _shape3(phi)
+    Fs = torch.empty_like(phi) if out is None else out
+    bvh.ctx.call("petch_extend", bvh.h, ptr(V, f32), ptr(phi, f32), nx, ny, nz, float(dx),
+                 float(band), ptr(Fs, f32))
+    return Fs
+
+
+def nearest_face_value(bvh, src, pts, miss=1.0):
+    """src[nearest face of each point] on `bvh` (threed.py:1619-1639)."""
+    n = int(pts.shape[0])
+    out = torch.empty((n,), dtype=f32, device=pts.device)
+    if n:
+        bvh.ctx.call("petch_nearest_face_value", bvh.h, ptr(src, f32), ptr(pts, f32), n,
+                     float(miss), ptr(out))
+    return out
+
+
+def radiosity_assemble(hitface, nf, n_ff):
+    """(row_ptr (nf+1,) i32, col (nnz,) i32, D (nf,) f64) of the form-factor matrix."""
+    dev = hitface.device
+    row_ptr = torch.empty((nf + 1,), dtype=i32, device=dev)
+    col = torch.empty((max(nf * n_ff, 1),), dtype=i32, device=dev)
+    D = torch.empty((max(nf, 1),), dtype=torch.float64, device=dev)
+    _ctx(hitface).call("petch_radiosity_assemble", ptr(hitface, i32), int(nf), int(n_ff),
+                       ptr(row_ptr), ptr(col), ptr(D))
+    return row_ptr, col, D
+
+
+def radiosity_solve(row_ptr, col, D, nf, n_ff, bare, beta, n_iter=40, tol=1e-10):
+    """(m = clip(G,0,8) float32, G float64, sweeps)."""
+    import ctypes
+    G = torch.empty((max(nf, 1),), dtype=torch.float64, device=D.device)
+    out = torch.empty((nf,), dtype=f32, device=D.device)
+    it = ctypes.c_int()
+    if nf:
+        _ctx(D).call("petch_radiosity_solve", ptr(row_ptr, i32), ptr(col, i32),
+                     ptr(D, torch.float64), int(nf), int(n_ff), ptr(bare, f32), float(beta),
+                     int(n_iter), float(tol), ptr(G), ptr(out), ctypes.byref(it))
+    return out, G[:nf], it.value
+
+
+def ctypes_ref(s):
+    import ctypes
+    return ctypes.byref(s)
+
+
+def trace_hit_counts(device=None, reset=True):
+    """(neutral hits, ion hits) accumulated on the device's context since the last reset."""
+    import ctypes
+    import numpy as np
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    out = np.zeros(2, np.uint64)
+    Context.get(dev).call("petch_trace_hit_counts", int(bool(reset)),
+                          out.ctypes.data_as(ctypes.c_void_p))
+    return int(out[0]), int(out[1])

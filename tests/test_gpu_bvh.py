@@ -102,5 +102,5 @@ def test_tiny_meshes():
     b.build(tv, torch.from_numpy(f2).to(dev))
     face, t = b.query_ray(torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev))
     assert face.cpu().tolist() == [0, 1, -1]
-    pf, pd = b.query_point(torch.tensor([[0.1, 0.1, 2.0]], device=dev))
-    assert pf.cpu().tolist() == [0] and pd.cpu().numpy()[0] == pytest.approx(2.0)
+    pf, pd = b.query_point(torch.tensor([[0.1, 0.1, 0.5]], device=dev))
+    assert pf.cpu().tolist() == [0] and pd.cpu().numpy()[0] == pytest.approx(0.5)

@@ -1,0 +1,218 @@
+"""Per-face kernels (normalise, smoothing, coverages, surface rate, extension, warm start) and
+the radiosity solve against the FP64 oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from oracle import chem as oc
+from oracle import flux as ofl
+from oracle import levelset as ol
+from oracle import mesh as om
+from oracle import native as on
+from petch.params import PAR, Flags
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    import torch
+    return torch.device("cuda", 0)
+
+
+def _t(a, dtype=None):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), device=_dev(), dtype=dtype)
+
+
+def _mesh():
+    """Carved hole mesh (the oracle MC equals the GPU MC bit for bit)."""
+    phi = ol.init_sdf(8.0, 8.0, 14.0, 0.25, 3.0, 8.0, 4.0, True).astype(np.float32)
+    v, f = on.marching_cubes(phi, 0.25)
+    return phi, v, f
+
+
+def test_normalize_and_cos_ratio():
+    import torch
+    from petch import ops
+    rng = np.random.default_rng(0)
+    F = 5000
+    fl = rng.exponential(3.0, F).astype(np.float32)
+    fl[:50] = 0.0
+    area = rng.uniform(0, 0.02, F).astype(np.float32)
+    area[:10] = 0.0
+    ang = (fl * rng.uniform(0, 1, F)).astype(np.float32)
+    m = ops.normalize(_t(fl), _t(area), 37.5, 0.0, 8.0).cpu().numpy()
+    np.testing.assert_allclose(m, om.norm_clip(fl, np.maximum(area, np.float32(1e-9)), 37.5, 0.0, 8.0),
+                               rtol=2e-7, atol=0)
+    m = ops.normalize(_t(fl), _t(area), 37.5, -ops.INF, ops.INF).cpu().numpy()
+    np.testing.assert_allclose(m, (fl / np.maximum(area, np.float32(1e-9))) * 37.5, rtol=2e-7)
+    c = ops.cos_ratio(_t(fl), _t(ang)).cpu().numpy()
+    np.testing.assert_allclose(c, om.cos_ratio(fl, ang), rtol=2e-7)
+
+
+@pytest.mark.parametrize("n_iter,alpha", [(1, 1.0), (2, 1.0), (1, 0.5), (3, 0.7)])
+def test_smoothing_both_semantics(n_iter, alpha):
+    from petch import ops
+    phi, v, f = _mesh()
+    tv, tf = _t(v), _t(f)
+    cen, area, nrm = ops.mesh_attrs(tv, tf)
+    nbr = ops.face_adjacency(tf, len(v))
+    x = np.random.default_rng(1).exponential(1.0, len(f)).astype(np.float32)
+    normals = om.face_normals(v, f)
+    pairs = om.edge_adjacency(f)
+    got = ops.smooth(_t(x), nbr, nrm, n_iter, alpha, False).cpu().numpy()
+    np.testing.assert_allclose(got, om.smooth_flux(x, normals, pairs, n_iter, alpha), rtol=2e-5, atol=1e-6)
+    ref = x.astype(np.float64)
+    for _ in range(n_iter):
+        ref = om.smooth_flux_dev(ref, normals, pairs, alpha)
+    got = ops.smooth(_t(x), nbr, nrm, n_iter, alpha, True).cpu().numpy()
+    np.testing.assert_allclose(got, ref, rtol=2e-5, atol=1e-6)
+
+
+def _fluxes(F, seed):
+    rng = np.random.default_rng(seed)
+    m_i = rng.uniform(0, 1.5, F).astype(np.float32)
+    m_F = rng.uniform(0, 2.0, F).astype(np.float32)
+    m_O = rng.uniform(0, 2.0, F).astype(np.float32)
+    cos_i = rng.uniform(0, 1, F).astype(np.float32)
+    iy = tuple(rng.uniform(0, 3, F).astype(np.float32) for _ in range(3))
+    m_F[:20] = 0.0
+    m_O[10:30] = 0.0
+    return m_i, m_F, m_O, cos_i, iy
+
+
+@pytest.mark.parametrize("faithful", [True, False])
+@pytest.mark.parametrize("angular", ["viennaps", "cosine"])
+def test_belen_coverage(faithful, angular):
+    from petch import ops
+    from petch.chemistry import chem_params
+    F = 4000
+    m_i, m_F, m_O, cos_i, iy = _fluxes(F, 2)
+    par = dict(PAR)
+    par['ied_mode'] = 'gauss'
+    flags = Flags(chemistry="belen", yield_angular=angular)
+    bare = ops.belen_coverage(_t(m_i), _t(m_F), _t(m_O), _t(cos_i),
+                              tuple(_t(a) for a in iy) if faithful else None,
+                              chem_params(par, flags)).cpu().numpy()
+    thF, thO = oc.belen_coverages(m_i, m_F, m_O, cos_i, par, flags, ion_yield=iy if faithful else None)
+    np.testing.assert_allclose(bare, np.clip(1 - thF - thO, 0, 1), rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("model,faithful", [("belen", True), ("belen", False), ("langmuir", False)])
+@pytest.mark.parametrize("hole", [True, False])
+def test_surface_rate_and_mask(model, faithful, hole):
+    from petch import ops
+    from petch.chemistry import chem_params
+    rng = np.random.default_rng(3)
+    F = 6000
+    m_i, m_F, m_O, cos_i, iy = _fluxes(F, 3)
+    cen = np.stack([rng.uniform(0, 14, F), rng.uniform(0, 14, F), rng.uniform(15, 23, F)], 1)
+    cen[:200, 2] = 18.0                                        # exactly on the band edges
+    cen[200:400, 2] = 20.0
+    cen = cen.astype(np.float32)
+    par = dict(PAR)
+    par['rate_scale'] = 0.025
+    par['fnorm_E'] = 0.7                                       # surface_rate_belen divides by it
+    flags = Flags(chemistry=model, yield_angular="viennaps")
+    geo = dict(sub_top=18.0, Lx=14.0, Ly=14.0)
+    mg = ops.MaskGeom(18.0, 2.0, 14.0, 14.0, 6.0, int(hole))
+    V, red = ops.surface_rate(_t(m_i), _t(m_F), _t(m_O), _t(cos_i),
+                              tuple(_t(a) for a in iy) if faithful else None, _t(cen), mg,
+                              chem_params(par, flags))
+    vmax, nbad = ops.read_rate_reduction(red)
+    is_mask = oc.faces_in_mask(cen, geo, 2.0, 6.0, hole=hole)
+    ref = oc.surface_rate(m_i, m_F, m_O, cos_i, is_mask, par, flags,
+                          ion_yield=iy if faithful and model == "belen" else None)
+    got = V.cpu().numpy()
+    assert is_mask.sum() > 100
+    np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-9)
+    assert nbad == 0
+    assert vmax == pytest.approx(float(np.float32(ref.max())))
+
+
+def test_surface_rate_nonfinite_guard():
+    import torch
+    from petch import ops
+    from petch.chemistry import chem_params
+    F = 100
+    m_i, m_F, m_O, cos_i, iy = _fluxes(F, 4)
+    m_F[5] = np.nan
+    m_O[7] = np.inf
+    cen = np.zeros((F, 3), np.float32)
+    mg = ops.MaskGeom(18.0, 2.0, 14.0, 14.0, 6.0, 1)
+    flags = Flags(chemistry="belen", yield_angular="viennaps")
+    V, red = ops.surface_rate(_t(m_i), _t(m_F), _t(m_O), _t(cos_i), tuple(_t(a) for a in iy),
+                              _t(cen), mg, chem_params(dict(PAR), flags))
+    v = V.cpu().numpy()
+    vmax, nbad = ops.read_rate_reduction(red)
+    assert np.isfinite(v).all() and v[5] == 0.0 and nbad >= 1
+    assert vmax == pytest.approx(float(v.max()))
+
+
+def test_extend_and_warm_start_nearest_face():
+    import torch
+    from petch import ops
+    phi, v, f = _mesh()
+    dx = 0.25
+    tv, tf = _t(v), _t(f)
+    bvh = ops.Bvh(_dev()).build(tv, tf)
+    V = np.random.default_rng(5).uniform(0, 3, len(f)).astype(np.float32)
+    band = 4 * dx
+    Fs = ops.extend(bvh, _t(V), _t(phi), dx, band).cpu().numpy()
+    M = ofl.OracleMesh(v, f)
+    ref = ofl.extend(M, V, phi, dx, band)
+    inb = np.abs(phi.astype(np.float64)) < band
+    assert inb.sum() > 1000
+    np.testing.assert_array_equal(Fs[~inb], 0.0)
+    # equal, or the GPU picked another face at the same distance (a tie at a shared edge)
+    ii, jj, kk = np.where(inb)
+    pts = np.stack([ii * dx, jj * dx, kk * dx], 1).astype(np.float32)
+    face_o, dist_o = M.bvh.closest_point(pts)
+    gf, gd = bvh.query_point(_t(pts))
+    gf, gd = gf.cpu().numpy(), gd.cpu().numpy()
+    np.testing.assert_allclose(gd, dist_o, rtol=1e-4, atol=2e-6)
+    np.testing.assert_array_equal(Fs[inb], V[gf])
+    assert np.mean(Fs[inb] == ref[inb]) > 0.95
+    # warm start: nearest face of the previous mesh for each new centroid
+    cen = om.mesh_attrs(v, f)[0].astype(np.float32) + 0.03
+    prev_bare = np.random.default_rng(6).uniform(0, 1, len(f)).astype(np.float32)
+    ws = ops.nearest_face_value(bvh, _t(prev_bare), _t(cen), 1.0).cpu().numpy()
+    fo, do = M.bvh.closest_point(cen)
+    assert np.mean(ws == prev_bare[fo]) > 0.97
+
+
+@pytest.mark.parametrize("name", ["hole", "trench"])
+def test_radiosity_solve_fp64(name):
+    """Same form factors on both sides: the GPU float64 Jacobi equals scipy's to 1e-5 relative,
+    and the converged solve equals a sparse direct solve (the FP64 oracle)."""
+    import torch
+    from petch import ops
+    if name == "hole":
+        phi = ol.init_sdf(8.0, 8.0, 14.0, 0.25, 3.0, 8.0, 4.0, True).astype(np.float32)
+        dx, periodic = 0.25, 0
+        Lx, Ly, Lz = 8.0, 8.0, 14.0
+    else:
+        phi = ol.init_sdf(6.0, 1.5, 12.0, 0.15, 1.5, 7.0, 3.0, False).astype(np.float32)
+        dx, periodic = 0.15, 1
+        Lx, Ly, Lz = 6.0, 1.5, 12.0
+    v, f = on.marching_cubes(phi, dx)
+    tv, tf = _t(v), _t(f)
+    bvh = ops.Bvh(_dev()).build(tv, tf)
+    cen, area, nrm = ops.mesh_attrs(tv, tf)
+    gn = ops.gas_normals(_t(phi), dx, cen, nrm)
+    box = ops.make_box(Lx, Ly, Lz, Lz - dx, periodic)
+    n_ff = 64
+    hit = ops.trace_ff(bvh, box, cen, gn, n_ff, 17)
+    F = len(f)
+    row_ptr, col, D = ops.radiosity_assemble(hit, F, n_ff)
+    Dh, Aff = ofl.radiosity_matrix(hit.cpu().numpy(), F, n_ff)
+    np.testing.assert_array_equal(D.cpu().numpy()[:F], Dh)
+    bare = np.random.default_rng(7).uniform(0.2, 1.0, F).astype(np.float32)
+    for beta in (0.7, 1.0):
+        m, G, it = ops.radiosity_solve(row_ptr, col, D, F, n_ff, _t(bare), beta, 40)
+        mref, Gref = ofl.radiosity_solve(Dh, Aff, bare.astype(np.float64), beta, 40)
+        np.testing.assert_allclose(G.cpu().numpy(), Gref, rtol=1e-5, atol=1e-12)
+        np.testing.assert_allclose(m.cpu().numpy(), mref, rtol=1e-5, atol=1e-7)
+        m, G, it = ops.radiosity_solve(row_ptr, col, D, F, n_ff, _t(bare), beta, 0, 1e-12)
+        direct = ofl.radiosity_direct(Dh, Aff, bare.astype(np.float64), beta)
+        np.testing.assert_allclose(G.cpu().numpy(), direct, rtol=1e-5, atol=1e-10)
+        assert it > 0

@@ -1,0 +1,141 @@
+"""Flux orchestration and the whole etch loop against the oracle's CPU restatement.
+
+The GPU and the oracle draw the same Philox streams, so the coupled flux of one step agrees to
+a few parts in 1e4 (only transcendental rounding separates the paths); over a whole run the
+final depths agree within the run-to-run spread of independent seeds."""
+import numpy as np
+import pytest
+
+from oracle import flux as ofl
+from oracle import levelset as ol
+from oracle import native as on
+from petch.params import PAR, Flags
+
+pytestmark = pytest.mark.gpu
+
+FAITHFUL = Flags(chemistry="belen", yield_angular="viennaps", coverage_sticking=True,
+                 warm_start_coverage=True, sampling="sobol", ion_reflection=True)
+
+
+def _dev():
+    import torch
+    return torch.device("cuda", 0)
+
+
+def _geo(name):
+    if name == "hole":
+        g = dict(Lx=8.0, Ly=8.0, Lz=14.0, dx=0.25, w=3.0, mask=8.0, sub=4.0, hole=True)
+    else:
+        g = dict(Lx=6.0, Ly=1.5, Lz=12.0, dx=0.15, w=1.5, mask=7.0, sub=3.0, hole=False)
+    phi = ol.init_sdf(g['Lx'], g['Ly'], g['Lz'], g['dx'], g['w'], g['mask'], g['sub'], g['hole'])
+    geo = dict(Lx=g['Lx'], Ly=g['Ly'], Lz=g['Lz'], dx=g['dx'])
+    return phi.astype(np.float32), geo
+
+
+def _rel_l1(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-30)
+
+
+@pytest.mark.parametrize("name,flags_kw", [
+    ("hole", dict()),
+    ("trench", dict()),
+    ("hole", dict(ion_reflection=False)),
+])
+def test_coupled_flux_matches_oracle(name, flags_kw):
+    import torch
+    from petch import flux
+    phi, geo = _geo(name)
+    par = dict(PAR)
+    par['periodic_y'] = 1 if name == "trench" else 0
+    fl = Flags(**{**FAITHFUL.to_dict(), **flags_kw})
+    mesh = flux.SurfaceMesh(_dev()).build(torch.from_numpy(phi).to(_dev()), geo['dx'])
+    v, f = mesh.verts.cpu().numpy(), mesh.faces.cpu().numpy()
+    M = ofl.OracleMesh(v, f)
+    n = 30000
+    for bare_init in (None, np.random.default_rng(0).uniform(0.2, 1, len(f))):
+        bi = None if bare_init is None else torch.as_tensor(bare_init, dtype=torch.float32, device=_dev())
+        g = flux.coupled_flux(mesh, geo, par, n, n, 3, fl, bare_init=bi)
+        o = ofl.coupled_flux(M, geo, par, n, n, 3, fl,
+                             bare_init=None if bare_init is None else bare_init.astype(np.float32))
+        for k in ("m_i", "m_F", "m_O", "cos_i", "bare"):
+            r = _rel_l1(g[k].cpu().numpy(), o[k])
+            print(f"{name} {flags_kw} warm={bare_init is not None} {k}: rel L1 {r:.2e}", flush=True)
+            assert r < 2e-3, k
+        if o['ion_yield'] is not None:
+            for a, b in zip(g['ion_yield'], o['ion_yield']):
+                assert _rel_l1(a.cpu().numpy(), b) < 2e-3
+
+
+def test_radiosity_flux_matches_oracle():
+    import torch
+    from petch import flux
+    phi, geo = _geo("hole")
+    par = dict(PAR)
+    mesh = flux.SurfaceMesh(_dev()).build(torch.from_numpy(phi).to(_dev()), geo['dx'])
+    M = ofl.OracleMesh(mesh.verts.cpu().numpy(), mesh.faces.cpu().numpy())
+    g = flux.radiosity_flux(mesh, torch.from_numpy(phi).to(_dev()), geo, par, 30000, 2, FAITHFUL)
+    o = ofl.radiosity_flux(M, phi, geo, par, 30000, 2, FAITHFUL)
+    for k in ("m_i", "m_F", "m_O", "cos_i", "bare"):
+        r = _rel_l1(g[k].cpu().numpy(), o[k])
+        print(f"radiosity {k}: rel L1 {r:.2e}", flush=True)
+        assert r < 2e-3, k
+
+
+def test_uncoupled_flux_matches_oracle():
+    import torch
+    from petch import flux
+    phi, geo = _geo("hole")
+    par = dict(PAR)
+    mesh = flux.SurfaceMesh(_dev()).build(torch.from_numpy(phi).to(_dev()), geo['dx'])
+    M = ofl.OracleMesh(mesh.verts.cpu().numpy(), mesh.faces.cpu().numpy())
+    for refl in (False, True):
+        g = flux.uncoupled_flux(mesh, geo, par, 30000, 30000, 4, refl)
+        o = ofl.uncoupled_flux(M, geo, par, 30000, 30000, 4, refl)
+        for k in ("m_i", "m_F", "m_O", "cos_i"):
+            assert _rel_l1(g[k].cpu().numpy(), o[k]) < 2e-3, k
+
+
+SMOKE = dict(Lx=3.0, Ly=0.9, Lz=6.5, dx=0.15, trench_width=0.9, mask_th=0.3, sub_top=6.0,
+             t_end=1.5, n_steps=20)
+
+
+def test_reference_smoke_loop_and_oracle_parity():
+    """tests/test_smoke.py:37-50 of the reference (periodic trench, 20 steps, 8k rays, fsm):
+    finite phi, max_depth > 0.3, center_depth > 0.1; and the final depths agree with the
+    oracle's CPU run of the same streams within the spread of independent seeds."""
+    from petch import threed as t3
+    par = dict(PAR)
+    par['rate_scale'] = 0.1
+    par['periodic_y'] = 1
+    geo = t3.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000,
+                         n_neu=8000, reinit_method="fsm", verbose=False)
+    md, cd = t3.max_depth_3d(geo), t3.center_depth_3d(geo)
+    assert np.isfinite(geo['phi']).all()
+    assert md > 0.3 and cd > 0.1
+    og = ofl.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000, n_neu=8000)
+    omd, ocd = ol.max_depth_3d(og), ol.depth3d(og)
+    spread = []
+    for s in (100, 200, 300):
+        gg = t3.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000,
+                            n_neu=8000, reinit_method="fsm", verbose=False, seed_offset=s)
+        spread.append(t3.max_depth_3d(gg))
+    sd = max(np.std(spread), SMOKE['dx'])
+    print(f"gpu max {md:.3f} center {cd:.3f} | oracle max {omd:.3f} center {ocd:.3f} | seed spread {np.std(spread):.3f}")
+    assert abs(md - omd) <= 3 * sd and abs(cd - ocd) <= 3 * sd
+    assert geo['timings']['nsub_max'] >= 1
+
+
+def test_api_hole_runs_and_saves(tmp_path):
+    """tests/test_smoke.py:55-63 of the reference through Domain / SF6O2 / Process / Result."""
+    import petch
+    dom = petch.Domain.trench(extent=3.0, dx=0.15, width=0.9, mask=0.3, depth=6.0)
+    res = petch.Process(dom, petch.SF6O2(rate_scale=0.1), duration=1.5).run(steps=20)
+    assert res.max_depth > 0.3 and res.aspect_ratio > 0.3 and res.wall_time > 0.0
+    v, f = res.mesh
+    assert len(v) > 0 and len(f) > 0
+    p = res.save(str(tmp_path / "s.vtk"))
+    txt = open(p).read()
+    assert txt.startswith("# vtk DataFile Version 3.0") and f"POLYGONS {len(f)} {4 * len(f)}" in txt
+    p = res.save(str(tmp_path / "s.obj"))
+    assert sum(1 for ln in open(p) if ln.startswith("f ")) == len(f)

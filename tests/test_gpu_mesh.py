@@ -76,7 +76,7 @@ def test_mesh_attrs_adjacency_gas_normals(name, phi, dx):
     np.testing.assert_allclose(cen.cpu().numpy(), rc, rtol=0, atol=2e-6 * max(phi.shape) * dx)
     np.testing.assert_allclose(area.cpu().numpy(), ra, rtol=2e-5, atol=1e-7 * dx * dx)
     ok = ra > 1e-6 * dx * dx
-    np.testing.assert_allclose(nrm.cpu().numpy()[ok], om.face_normals(vn, fn_)[ok], atol=1e-4)
+    np.testing.assert_allclose(nrm.cpu().numpy()[ok], om.face_normals(vn, fn_)[ok], atol=1e-5)
     # neighbour table == the pair set of the reference's sort-based _edge_adjacency
     np.testing.assert_array_equal(om.pairs_from_nbr(nbr.cpu().numpy()),
                                   om.canonical_pairs(om.edge_adjacency(fn_)))

@@ -163,6 +163,8 @@ def test_flat_open_field_unity():
     n = 200000
     out = ops.trace_neutral_rr(bvh, box, ops.make_source(0), torch.ones(len(f), device=dev), 0.7,
                                n, 5).cpu().numpy().astype(np.float64)
-    assert n - 5 <= out.sum() <= n
+    # every ray deposits once except the few grazing ones that use up the 256 lateral wraps of
+    # threed.py:305 before reaching the floor (P(cos < ~0.005) ~ 3e-5 per emission)
+    assert n * (1 - 1e-4) <= out.sum() <= n
     m = out / (0.5 * 0.5 * 0.5) * (Lx * Ly / n)
     assert abs(m.mean() - 1.0) < 1e-4 and m.std() < 0.05
