@@ -119,7 +119,20 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run plain device copies of 256 MiB (reference for PMC byte counts)")
     a = ap.parse_args(argv)
+    if a.copy_probe:
+        import torch
+        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+        x.fill_(1.0)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": x.numel() * 4, "copy_probe_bytes_written": x.numel() * 4,
+                          "dispatches": 5}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the etch step of one feature runs on one device "
               "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)

@@ -1,6 +1,8 @@
-// Ray-traced flux kernels. One lane per primary ray; each lane owns its Philox stream (seed, ray),
-// generates its source ray in registers (no origin/direction buffers), walks its bounces through
-// the LBVH and deposits into per-face float accumulators with hardware float atomics.
+// Ray-traced flux kernels. Each ray owns its Philox stream (seed, ray) and generates its source
+// ray in registers (no origin/direction buffers). The RR-neutral and faithful-ion kernels run a
+// persistent grid over a ray queue: a lane whose ray ends takes the next ray id, so divergent
+// bounce counts do not idle the wavefront. Deposits go to per-face float accumulators with
+// hardware float atomics.
 //
 //   k_trace_neutral_rr   _trace3d_cov_rr    threed.py:549-596 (RR-weighted neutral, <= 1024 bounces)
 //   k_trace_ion_yield    _trace3d_ion_yield threed.py:599-712 (faithful ion, 5 deposits per hit)
@@ -23,7 +25,8 @@ petch::BvhView petch_bvh_view(const petch_bvh* b);
 
 namespace {
 
-// Per-context 64-bit hit counters (slot 0 neutral, 1 ion), read by petch_trace_hit_counts.
+// Per-context 64-bit hit counters (slot 0 neutral, 1 ion), read by petch_trace_hit_counts;
+// slots 2-3 hold the ray-queue cursor of the persistent trace launches.
 unsigned long long* petch_hit_counter(petch_ctx* ctx, int which) {
   bool fresh = ctx->scr_cap[SCR_TRACE] == 0;
   unsigned long long* c =
@@ -132,31 +135,80 @@ __device__ __forceinline__ void count_hits(unsigned long long* __restrict__ ctr,
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
 }
 
+// Ray queue shared by a persistent grid: each wavefront refills its finished lanes with new ray
+// ids (one atomic per wavefront per refill), so lanes whose rays bounce many times deep in a
+// feature do not hold the whole wavefront hostage to its longest ray. `exhausted` is uniform
+// across the wavefront, so the refill stops after the first empty claim.
+struct RayQueue {
+  unsigned* next;
+  int n;
+};
+
+__device__ __forceinline__ bool claim_ray(const RayQueue& q, bool active, bool& exhausted,
+                                          unsigned& ray) {
+  if (exhausted) return false;
+  const int lane = threadIdx.x & 63;
+  unsigned long long idle = __ballot(!active);
+  if (!idle) return false;
+  int leader = __ffsll((unsigned long long)idle) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(q.next, (unsigned)__popcll(idle));
+  base = __shfl(base, leader, 64);
+  if (base >= (unsigned)q.n) exhausted = true;
+  if (active) return false;
+  unsigned long long below = (lane == 0) ? 0ull : (idle & ((1ull << lane) - 1ull));
+  ray = base + (unsigned)__popcll(below);
+  return ray < (unsigned)q.n;
+}
+
 __global__ void __launch_bounds__(kBS) k_trace_neutral_rr(BvhView bv, Box b, Source src,
                                                           const float* __restrict__ bare,
-                                                          float beta, int n, uint64_t seed,
+                                                          float beta, RayQueue q, uint64_t seed,
                                                           float* __restrict__ fl,
                                                           unsigned long long* __restrict__ hits) {
-  int p = blockIdx.x * kBS + threadIdx.x;
   unsigned nh = 0;
+  bool active = false, exhausted = false;
   Rng r;
-  r.init(seed, (uint32_t)p);
   Vec3 o, d;
-  gen_source(r, b, src, o, d);
-  float w = 1.f;
-  for (int bounce = 0; bounce < (p < n ? 1024 : 0); ++bounce) {
+  float w = 0.f;
+  int bounce = 0;
+  while (true) {
+    unsigned ray;
+    if (claim_ray(q, active, exhausted, ray)) {
+      r.init(seed, ray);
+      gen_source(r, b, src, o, d);
+      w = 1.f;
+      bounce = 0;
+      active = true;
+    }
+    if (!__ballot(active)) break;
+    if (!active) continue;
+    // one bounce of _trace3d_cov_rr (threed.py:565-596)
     Hit h = trace_bc(bv, b, o, d);
-    if (h.face < 0) break;
+    if (h.face < 0) {
+      active = false;
+      continue;
+    }
     Vec3 nn = hit_normal(bv, h.slot);
     if (petch::dot(nn, d) > 0.f) nn = -nn;
     deposit(fl, h.face, w);
     ++nh;
     float S = bare[h.face] * beta;
     w = w - w * S;
-    if (w <= 1.0e-6f) break;
+    if (w <= 1.0e-6f) {
+      active = false;
+      continue;
+    }
     if (w < 0.1f) {
-      if (r.uniform() < (1.f - w / 0.3f)) break;
+      if (r.uniform() < (1.f - w / 0.3f)) {
+        active = false;
+        continue;
+      }
       w = 0.3f;
+    }
+    if (++bounce >= 1024) {
+      active = false;
+      continue;
     }
     Vec3 hit = o + h.t * d;
     d = cosine_about(r, nn);
@@ -171,26 +223,39 @@ struct IonParams {
 };
 
 __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Source src,
-                                                         IonParams P, int n, uint64_t seed,
+                                                         IonParams P, RayQueue q, uint64_t seed,
                                                          float* __restrict__ sp,
                                                          float* __restrict__ ie,
                                                          float* __restrict__ ip,
                                                          float* __restrict__ arr,
                                                          float* __restrict__ ang,
                                                          unsigned long long* __restrict__ hits) {
-  int p = blockIdx.x * kBS + threadIdx.x;
   unsigned nh = 0;
+  bool active = false, exhausted = false;
   Rng r;
-  r.init(seed, (uint32_t)p);
   Vec3 o, d;
-  gen_source(r, b, src, o, d);
-  float E = P.meanE + P.sigmaE * r.normal();
-  if (E < 0.f) E = 0.f;
-  float w = 1.f;
+  float E = 0.f, w = 0.f;
+  int bounce = 0;
   const float sEsp = sqrtf(P.Eth_sp), sEie = sqrtf(P.Eth_ie), sEp = sqrtf(P.Eth_p);
-  for (int bounce = 0; bounce < (p < n ? 1024 : 0); ++bounce) {
+  while (true) {
+    unsigned ray;
+    if (claim_ray(q, active, exhausted, ray)) {
+      r.init(seed, ray);
+      gen_source(r, b, src, o, d);
+      E = P.meanE + P.sigmaE * r.normal();
+      if (E < 0.f) E = 0.f;
+      w = 1.f;
+      bounce = 0;
+      active = true;
+    }
+    if (!__ballot(active)) break;
+    if (!active) continue;
+    // one bounce of _trace3d_ion_yield (threed.py:625-712)
     Hit h = trace_bc(bv, b, o, d);
-    if (h.face < 0) break;
+    if (h.face < 0) {
+      active = false;
+      continue;
+    }
     Vec3 nn = hit_normal(bv, h.slot);
     if (petch::dot(nn, d) > 0.f) nn = -nn;
     float cosT = fminf(fmaxf(-petch::dot(d, nn), 0.f), 1.f);
@@ -211,7 +276,10 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
       rr = fminf(fmaxf(rr, 0.f), 1.f);
       stick = 1.f - rr;
     }
-    if (stick >= 1.f) break;
+    if (stick >= 1.f) {
+      active = false;
+      continue;
+    }
     float Eref;
     if (incAng >= P.inflect)
       Eref = 1.f - (1.f - P.A_en) * (kPi2 - incAng) / (kPi2 - P.inflect);
@@ -226,10 +294,16 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
       }
     }
     newE = fminf(fmaxf(newE, 0.f), E);
-    if (newE <= P.minE) break;
+    if (newE <= P.minE) {
+      active = false;
+      continue;
+    }
     E = newE;
     w = w - w * stick;
-    if (w <= 1.0e-4f) break;
+    if (w <= 1.0e-4f || ++bounce >= 1024) {
+      active = false;
+      continue;
+    }
     Vec3 hit = o + h.t * d;
     Vec3 axis = petch::normalize(d - (2.f * petch::dot(d, nn)) * nn);
     Vec3 tt, bb;
@@ -352,6 +426,27 @@ Source make_source(const petch_source* s) {
   return Source{s->kind, s->sigma, s->exponent};
 }
 
+// Grid of the persistent traces: enough workgroups to fill every CU several times over, never
+// more than the rays need.
+unsigned persistent_blocks(petch_ctx* ctx, int n) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  long long want = (long long)cus * 8;
+  long long need = ((long long)n + kBS - 1) / kBS;
+  return (unsigned)(need < want ? need : want);
+}
+
+int make_queue(petch_ctx* ctx, int n, RayQueue& q) {
+  unsigned long long* c = petch_hit_counter(ctx, 2);
+  if (!c) return PETCH_E_HIP;
+  q.next = (unsigned*)c;
+  q.n = n;
+  PETCH_HIP(ctx, hipMemsetAsync(q.next, 0, sizeof(unsigned), ctx->stream));
+  return PETCH_OK;
+}
+
 int zero(petch_ctx* ctx, float* p, int n) {
   if (p && n > 0) PETCH_HIP(ctx, hipMemsetAsync(p, 0, (size_t)n * sizeof(float), ctx->stream));
   return PETCH_OK;
@@ -378,9 +473,12 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   int rc = zero(ctx, fl, bvh->nf);
   if (rc) return rc;
   if (n == 0 || bvh->nf == 0) return PETCH_OK;
-  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), make_box(box), make_source(src), bare, beta, n, seed, fl,
-                     petch_hit_counter(ctx, 0));
+  RayQueue q;
+  int rc2 = make_queue(ctx, n, q);
+  if (rc2) return rc2;
+  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(persistent_blocks(ctx, n)), dim3(kBS), 0,
+                     ctx->stream, petch_bvh_view(bvh), make_box(box), make_source(src), bare, beta,
+                     q, seed, fl, petch_hit_counter(ctx, 0));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
@@ -401,9 +499,12 @@ extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const
   IonParams P{ip->meanE,  ip->sigmaE, ip->A_sp,    ip->Eth_sp, ip->B_sp, ip->A_ie,
               ip->Eth_ie, ip->A_p,    ip->Eth_p,   ip->inflect, ip->minang, ip->n_l,
               ip->A_en,   ip->minE,   ip->thrmin,  ip->thrmax};
-  hipLaunchKernelGGL(k_trace_ion_yield, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), make_box(box), make_source(src), P, n, seed, sp, ie,
-                     ipass, arr, ang, petch_hit_counter(ctx, 1));
+  RayQueue q;
+  int rc2 = make_queue(ctx, n, q);
+  if (rc2) return rc2;
+  hipLaunchKernelGGL(k_trace_ion_yield, dim3(persistent_blocks(ctx, n)), dim3(kBS), 0,
+                     ctx->stream, petch_bvh_view(bvh), make_box(box), make_source(src), P, q, seed,
+                     sp, ie, ipass, arr, ang, petch_hit_counter(ctx, 1));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

@@ -171,13 +171,27 @@ def test_extend_and_warm_start_nearest_face():
     gf, gd = gf.cpu().numpy(), gd.cpu().numpy()
     np.testing.assert_allclose(gd, dist_o, rtol=1e-4, atol=2e-6)
     np.testing.assert_array_equal(Fs[inb], V[gf])
-    assert np.mean(Fs[inb] == ref[inb]) > 0.95
+    # a grid point whose closest surface point is a shared edge or vertex has several nearest
+    # faces; which one a BVH returns is traversal order. Where the two sides differ, the GPU's
+    # face must be at the oracle's minimum distance.
+    _assert_tie_only(v, f, pts, Fs[inb] != ref[inb], gf, dist_o)
     # warm start: nearest face of the previous mesh for each new centroid
     cen = om.mesh_attrs(v, f)[0].astype(np.float32) + 0.03
     prev_bare = np.random.default_rng(6).uniform(0, 1, len(f)).astype(np.float32)
     ws = ops.nearest_face_value(bvh, _t(prev_bare), _t(cen), 1.0).cpu().numpy()
     fo, do = M.bvh.closest_point(cen)
-    assert np.mean(ws == prev_bare[fo]) > 0.97
+    gf2, _ = bvh.query_point(_t(cen))
+    gf2 = gf2.cpu().numpy()
+    np.testing.assert_array_equal(ws, prev_bare[gf2])
+    _assert_tie_only(v, f, cen, ws != prev_bare[fo], gf2, do)
+
+
+def _assert_tie_only(v, f, pts, differ, gpu_face, dist_min):
+    idx = np.flatnonzero(differ)
+    print(f"nearest-face ties resolved differently: {len(idx)} of {len(differ)}", flush=True)
+    for i in idx[:2000]:
+        _, d = on.closest_point_brute(v, f[gpu_face[i]:gpu_face[i] + 1], pts[i:i + 1])
+        assert abs(d[0] - dist_min[i]) <= 1e-4 * max(dist_min[i], 1.0), (i, d[0], dist_min[i])
 
 
 @pytest.mark.parametrize("name", ["hole", "trench"])

@@ -100,10 +100,9 @@ SMOKE = dict(Lx=3.0, Ly=0.9, Lz=6.5, dx=0.15, trench_width=0.9, mask_th=0.3, sub
              t_end=1.5, n_steps=20)
 
 
-def test_reference_smoke_loop_and_oracle_parity():
+def test_reference_smoke_loop():
     """tests/test_smoke.py:37-50 of the reference (periodic trench, 20 steps, 8k rays, fsm):
-    finite phi, max_depth > 0.3, center_depth > 0.1; and the final depths agree with the
-    oracle's CPU run of the same streams within the spread of independent seeds."""
+    finite phi, max_depth > 0.3, center_depth > 0.1."""
     from petch import threed as t3
     par = dict(PAR)
     par['rate_scale'] = 0.1
@@ -113,6 +112,22 @@ def test_reference_smoke_loop_and_oracle_parity():
     md, cd = t3.max_depth_3d(geo), t3.center_depth_3d(geo)
     assert np.isfinite(geo['phi']).all()
     assert md > 0.3 and cd > 0.1
+    assert geo['timings']['nsub_max'] >= 1
+
+
+@pytest.mark.parametrize("rate_scale", [0.1, 0.01])
+def test_smoke_loop_oracle_parity(rate_scale):
+    """The smoke loop against the oracle's CPU run of the same Philox streams: final max and
+    center depths agree within the spread of independent seeds (floored at dx / 4). At
+    rate_scale 0.1 the trench reaches the bottom of the domain on both sides; 0.01 stops
+    mid-way, where the depth is sensitive to every flux value."""
+    from petch import threed as t3
+    par = dict(PAR)
+    par['rate_scale'] = rate_scale
+    par['periodic_y'] = 1
+    geo = t3.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000,
+                         n_neu=8000, reinit_method="fsm", verbose=False)
+    md, cd = t3.max_depth_3d(geo), t3.center_depth_3d(geo)
     og = ofl.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000, n_neu=8000)
     omd, ocd = ol.max_depth_3d(og), ol.depth3d(og)
     spread = []
@@ -120,10 +135,13 @@ def test_reference_smoke_loop_and_oracle_parity():
         gg = t3.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000,
                             n_neu=8000, reinit_method="fsm", verbose=False, seed_offset=s)
         spread.append(t3.max_depth_3d(gg))
-    sd = max(np.std(spread), SMOKE['dx'])
-    print(f"gpu max {md:.3f} center {cd:.3f} | oracle max {omd:.3f} center {ocd:.3f} | seed spread {np.std(spread):.3f}")
+    sd = max(np.std(spread), SMOKE['dx'] / 4)
+    band = np.abs(og['phi']) < 2 * SMOKE['dx']
+    dphi = float(np.abs(geo['phi'][band] - og['phi'][band]).max())
+    print(f"rate_scale {rate_scale}: gpu max {md:.3f} center {cd:.3f} | oracle max {omd:.3f} "
+          f"center {ocd:.3f} | seed spread {np.std(spread):.3f} | max |dphi| near front {dphi:.3e}",
+          flush=True)
     assert abs(md - omd) <= 3 * sd and abs(cd - ocd) <= 3 * sd
-    assert geo['timings']['nsub_max'] >= 1
 
 
 def test_api_hole_runs_and_saves(tmp_path):
@@ -139,3 +157,18 @@ def test_api_hole_runs_and_saves(tmp_path):
     assert txt.startswith("# vtk DataFile Version 3.0") and f"POLYGONS {len(f)} {4 * len(f)}" in txt
     p = res.save(str(tmp_path / "s.obj"))
     assert sum(1 for ln in open(p) if ln.startswith("f ")) == len(f)
+
+
+@pytest.mark.parametrize("kw,flag_kw", [
+    (dict(surf_smooth=0.5), {}),
+    ({}, dict(redeposition=True)),
+    ({}, dict(neutral_transport="knudsen")),
+    (dict(reinit_method="gpu"), {}),
+])
+def test_out_of_path_options_raise(kw, flag_kw):
+    """Options outside the etch-loop hot path fail loudly instead of silently changing physics."""
+    from petch import threed as t3
+    fl = Flags(**{**FAITHFUL.to_dict(), **flag_kw})
+    with pytest.raises(NotImplementedError):
+        t3.run_etch_3d(**{**SMOKE, "n_steps": 1}, par=dict(PAR), flags=fl, n_ion=1000, n_neu=1000,
+                       verbose=False, **{"reinit_method": "fsm", **kw})
