@@ -7,9 +7,15 @@
 //          (>= 0 internal node, < 0 leaf ~slot); node 0 is the root.
 //   tris:  nf leaves of 48 B in Morton order: v0 (w = face id bits), v1, v2 as float4, so a
 //          leaf is three 16-byte loads and carries the face id the flux deposits into.
-// Traversal keeps a short per-lane stack of far children (the near child is descended first);
-// a stack overflow drops the far subtree and raises error flag 1 instead of writing out of
-// bounds; an out-of-range reference or a step count beyond a tree's ends the query with flag 2.
+//   wide:  4-wide nodes of 128 B collapsed from the binary tree: the binary node at every even
+//          depth stores its up-to-four grandchildren (a child that is a leaf stays a child), at
+//          the binary node's own index. Ray queries walk the wide nodes, which halves the chain
+//          of dependent node fetches; point queries walk the binary nodes.
+// Traversal keeps a short per-lane stack of far children with their entry distances (the
+// nearest child is descended first, and a popped entry beyond the best hit so far is dropped
+// unfetched); a stack overflow drops the far subtree and raises error flag 1 instead of writing
+// out of bounds; an out-of-range reference or a step count beyond a tree's ends the query with
+// flag 2.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -17,6 +23,7 @@
 namespace petch {
 
 constexpr int kBvhStack = 64;
+constexpr float kBoxSlack = 1.00000024f;  // 1 + 2 ulp
 
 struct __align__(16) BvhNode {
   float4 a;  // lmin.x lmin.y lmin.z lmax.x
@@ -25,8 +32,18 @@ struct __align__(16) BvhNode {
   int4 d;    // left, right, -, -
 };
 
+// Four children's boxes, one child per float4 lane, and their references: >= 0 a wide node
+// (indexed like its binary node), < 0 a leaf ~slot, kWideEmpty an unused lane.
+struct __align__(16) BvhWide {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  int4 ref;
+  int4 pad;
+};
+constexpr int kWideEmpty = (int)0x80000000;
+
 struct BvhView {
   const BvhNode* nodes;
+  const BvhWide* wide;
   const float4* tris;
   int nf;
   int* err;  // device flag: set to 1 on traversal stack overflow (may be null)
@@ -127,7 +144,7 @@ __device__ __forceinline__ float ray_box(const RayPre& r, float lx, float ly, fl
   float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
   float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
   // tmax widened by 2 ulp so rounding never culls a box the ray touches (conservative)
-  return (tmin <= tmax * 1.00000024f) ? tmin : 3.0e38f;
+  return (tmin <= tmax * kBoxSlack) ? tmin : 3.0e38f;  // 3e38: missed
 }
 
 struct Hit {
@@ -147,6 +164,30 @@ __device__ __forceinline__ void leaf_test(const BvhView& b, const RayPre& r, int
   }
 }
 
+constexpr float kMiss = 3.0e38f;   // entry distance of a box the ray misses
+constexpr float kHitLim = 1.0e38f;
+
+__device__ __forceinline__ void sort2(float& ta, int& ca, float& tb, int& cb) {
+  const bool sw = tb < ta;
+  const float t = sw ? tb : ta;
+  tb = sw ? ta : tb;
+  ta = t;
+  const int c = sw ? cb : ca;
+  cb = sw ? ca : cb;
+  ca = c;
+}
+
+__device__ __forceinline__ void push_far(const BvhView& b, int* stack, float* stack_t, int& sp,
+                                         int node, float t) {
+  if (sp < kBvhStack) {
+    stack[sp] = node;
+    stack_t[sp] = t;
+    ++sp;
+  } else if (b.err) {
+    *b.err = 1;
+  }
+}
+
 // Closest hit with 0 < t < tmax.
 __device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d, float tmax) {
   Hit h{-1, -1, tmax};
@@ -157,43 +198,50 @@ __device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d,
     return h;
   }
   int stack[kBvhStack];
+  float stack_t[kBvhStack];
   int sp = 0;
   int node = 0;
-  // a tree visits each internal node at most once; the bound only guards a corrupt tree
+  // a tree visits each wide node at most once; the bound only guards a corrupt tree
   const int max_steps = 2 * b.nf + 64;
   for (int steps = 0;; ++steps) {
     if (steps > max_steps || node < 0 || node >= b.nf - 1) {
       if (b.err) *b.err = 2;
       break;
     }
-    const BvhNode* N = b.nodes + node;
-    float4 A = N->a, B = N->b, C = N->c;
-    int4 D = N->d;
-    float tl = ray_box(r, A.x, A.y, A.z, A.w, B.x, B.y, h.t);
-    float tr = ray_box(r, B.z, B.w, C.x, C.y, C.z, C.w, h.t);
-    bool hl = tl < 1.0e38f, hr = tr < 1.0e38f;
-    if (hl && D.x < 0) {
-      leaf_test(b, r, ~D.x, h);
-      hl = false;
-    }
-    if (hr && D.y < 0) {
-      leaf_test(b, r, ~D.y, h);
-      hr = false;
-    }
-    if (hl && hr) {
-      int nearc = (tl <= tr) ? D.x : D.y;
-      int farc = (tl <= tr) ? D.y : D.x;
-      if (sp < kBvhStack) {
-        stack[sp++] = farc;
-      } else if (b.err) {
-        *b.err = 1;
-      }
-      node = nearc;
-    } else if (hl) {
-      node = D.x;
-    } else if (hr) {
-      node = D.y;
+    const BvhWide* N = b.wide + node;
+    const float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
+    const int4 ref = N->ref;
+    int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
+    float t0 = c0 != kWideEmpty ? ray_box(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, h.t) : kMiss;
+    float t1 = c1 != kWideEmpty ? ray_box(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, h.t) : kMiss;
+    float t2 = c2 != kWideEmpty ? ray_box(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, h.t) : kMiss;
+    float t3 = c3 != kWideEmpty ? ray_box(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, h.t) : kMiss;
+    // leaves are tested on the spot
+    if (t0 < kHitLim && c0 < 0) { leaf_test(b, r, ~c0, h); t0 = kMiss; }
+    if (t1 < kHitLim && c1 < 0) { leaf_test(b, r, ~c1, h); t1 = kMiss; }
+    if (t2 < kHitLim && c2 < 0) { leaf_test(b, r, ~c2, h); t2 = kMiss; }
+    if (t3 < kHitLim && c3 < 0) { leaf_test(b, r, ~c3, h); t3 = kMiss; }
+    // inner children that now start beyond the best hit are dropped
+    const float lim = h.t * kBoxSlack;
+    if (!(t0 <= lim)) t0 = kMiss;
+    if (!(t1 <= lim)) t1 = kMiss;
+    if (!(t2 <= lim)) t2 = kMiss;
+    if (!(t3 <= lim)) t3 = kMiss;
+    // ascending by entry distance (5-exchange network)
+    sort2(t0, c0, t1, c1);
+    sort2(t2, c2, t3, c3);
+    sort2(t0, c0, t2, c2);
+    sort2(t1, c1, t3, c3);
+    sort2(t1, c1, t2, c2);
+    if (t0 < kHitLim) {
+      // push the farther ones far-to-near, descend the nearest
+      if (t3 < kHitLim) push_far(b, stack, stack_t, sp, c3, t3);
+      if (t2 < kHitLim) push_far(b, stack, stack_t, sp, c2, t2);
+      if (t1 < kHitLim) push_far(b, stack, stack_t, sp, c1, t1);
+      node = c0;
     } else {
+      const float lim2 = h.t * kBoxSlack;
+      while (sp > 0 && !(stack_t[sp - 1] <= lim2)) --sp;
       if (sp == 0) break;
       node = stack[--sp];
     }

@@ -6,6 +6,8 @@
 //   karras   internal nodes from the sorted codes (Karras, HPG 2012), ties broken by index
 //   refit    leaves climb to the root; at each node the second arriving thread (atomic flag)
 //            unions the two child boxes and continues, so each node is built exactly once
+//   wide     every binary node at even depth (parity from its parent chain) gathers its
+//            grandchildren into a 4-wide node at its own index
 // plus the two query entry points used by the tests (the trace kernels inline the traversal).
 #include "common.h"
 #include "bvh.h"
@@ -15,6 +17,7 @@ int petch_radix_sort_pairs(petch_ctx* ctx, uint32_t* keys, uint32_t* vals, uint3
 
 using petch::BvhNode;
 using petch::BvhView;
+using petch::BvhWide;
 
 namespace {
 
@@ -231,6 +234,62 @@ __global__ void k_bvh_refit(const float4* __restrict__ tris, BvhNode* nodes,
   }
 }
 
+__global__ void k_bvh_wide(const BvhNode* __restrict__ nodes, const int* __restrict__ pint, int n,
+                           BvhWide* __restrict__ wide) {
+  int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n - 1) return;
+  int depth = 0;
+  int p = pint[i];
+  for (int guard = 0; p >= 0 && p < n - 1 && guard < n; ++guard) {
+    ++depth;
+    p = pint[p];
+  }
+  if (depth & 1) return;
+  float lo[4][3], hi[4][3];
+  int ref[4];
+  int m = 0;
+  const BvhNode& N = nodes[i];
+  const int kids[2] = {N.d.x, N.d.y};
+  for (int side = 0; side < 2; ++side) {
+    const int c = kids[side];
+    if (c < 0 || c >= n - 1) {  // a leaf (an out-of-range reference is kept for the guard)
+      const float* q = (const float*)&N + 6 * side;
+      for (int k = 0; k < 3; ++k) {
+        lo[m][k] = q[k];
+        hi[m][k] = q[3 + k];
+      }
+      ref[m++] = c;
+    } else {
+      const BvhNode& C = nodes[c];
+      const float* q = (const float*)&C;
+      for (int g = 0; g < 2; ++g) {
+        for (int k = 0; k < 3; ++k) {
+          lo[m][k] = q[6 * g + k];
+          hi[m][k] = q[6 * g + 3 + k];
+        }
+        ref[m++] = g == 0 ? C.d.x : C.d.y;
+      }
+    }
+  }
+  for (; m < 4; ++m) {
+    for (int k = 0; k < 3; ++k) {
+      lo[m][k] = 0.f;
+      hi[m][k] = 0.f;
+    }
+    ref[m] = petch::kWideEmpty;
+  }
+  BvhWide w;
+  w.lox = make_float4(lo[0][0], lo[1][0], lo[2][0], lo[3][0]);
+  w.hix = make_float4(hi[0][0], hi[1][0], hi[2][0], hi[3][0]);
+  w.loy = make_float4(lo[0][1], lo[1][1], lo[2][1], lo[3][1]);
+  w.hiy = make_float4(hi[0][1], hi[1][1], hi[2][1], hi[3][1]);
+  w.loz = make_float4(lo[0][2], lo[1][2], lo[2][2], lo[3][2]);
+  w.hiz = make_float4(hi[0][2], hi[1][2], hi[2][2], hi[3][2]);
+  w.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+  w.pad = make_int4(0, 0, 0, 0);
+  wide[i] = w;
+}
+
 __global__ void k_query_ray(BvhView b, const float* __restrict__ o, const float* __restrict__ d,
                             int n, float tmax, int* __restrict__ face, float* __restrict__ tout) {
   int i = blockIdx.x * kBS + threadIdx.x;
@@ -256,6 +315,7 @@ __global__ void k_query_point(BvhView b, const float* __restrict__ p, int n, flo
 BvhView petch_bvh_view(const petch_bvh* b) {
   BvhView v;
   v.nodes = (const BvhNode*)b->nodes;
+  v.wide = (const BvhWide*)b->wide;
   v.tris = (const float4*)b->tris;
   v.nf = b->nf;
   v.err = b->work ? carve(b->work, b->cap_f).err : nullptr;
@@ -275,6 +335,7 @@ extern "C" int petch_bvh_destroy(petch_ctx* ctx, petch_bvh* b) {
   if (!b) return PETCH_OK;
   if (ctx) (void)hipStreamSynchronize(ctx->stream);
   if (b->nodes) (void)hipFree(b->nodes);
+  if (b->wide) (void)hipFree(b->wide);
   if (b->tris) (void)hipFree(b->tris);
   if (b->work) (void)hipFree(b->work);
   free(b);
@@ -291,11 +352,13 @@ extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts,
   if (nf > b->cap_f) {
     PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (b->nodes) (void)hipFree(b->nodes);
+    if (b->wide) (void)hipFree(b->wide);
     if (b->tris) (void)hipFree(b->tris);
     if (b->work) (void)hipFree(b->work);
-    b->nodes = b->tris = b->work = nullptr;
+    b->nodes = b->wide = b->tris = b->work = nullptr;
     int cap = nf + nf / 2 + 256;
     PETCH_HIP(ctx, hipMalloc(&b->nodes, (size_t)cap * sizeof(BvhNode)));
+    PETCH_HIP(ctx, hipMalloc(&b->wide, (size_t)cap * sizeof(BvhWide)));
     PETCH_HIP(ctx, hipMalloc(&b->tris, (size_t)cap * 48));
     PETCH_HIP(ctx, hipMalloc(&b->work, work_bytes(cap)));
     b->cap_f = cap;
@@ -303,6 +366,7 @@ extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts,
   if (b->cap_f == 0) {  // never built: keep a valid error flag
     int cap = 256;
     PETCH_HIP(ctx, hipMalloc(&b->nodes, (size_t)cap * sizeof(BvhNode)));
+    PETCH_HIP(ctx, hipMalloc(&b->wide, (size_t)cap * sizeof(BvhWide)));
     PETCH_HIP(ctx, hipMalloc(&b->tris, (size_t)cap * 48));
     PETCH_HIP(ctx, hipMalloc(&b->work, work_bytes(cap)));
     b->cap_f = cap;
@@ -331,6 +395,9 @@ extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts,
     PETCH_HIP(ctx, hipMemsetAsync(w.flags, 0, (size_t)nf * sizeof(int), ctx->stream));
     hipLaunchKernelGGL(k_bvh_refit, dim3(g), dim3(kBS), 0, ctx->stream, (const float4*)b->tris,
                        (BvhNode*)b->nodes, w.pint, w.pleaf, w.flags, nf, w.root);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_bvh_wide, dim3(petch_blocks(nf - 1, kBS)), dim3(kBS), 0, ctx->stream,
+                       (const BvhNode*)b->nodes, w.pint, nf, (BvhWide*)b->wide);
     PETCH_LAUNCH_CHECK(ctx);
   }
   b->nf = nf;

@@ -47,8 +47,8 @@ struct petch_bvh {
   int nf, nv;              // faces / vertices of the mesh it was built over
   int cap_f;               // capacity (faces) of the device buffers below
   void* nodes;             // (nf-1) internal nodes, 64 B each: child AABBs + child refs
+  void* wide;              // 4-wide nodes, 128 B each, at the even-depth binary nodes' indices
   void* tris;              // nf triangles in leaf order, 48 B each (v0, e1, e2 as float4)
-  int* tri_face;           // leaf slot -> original face id
   void* work;              // build scratch: morton keys/values, parents, flags
   size_t work_cap;
   float root_lo[3], root_hi[3];

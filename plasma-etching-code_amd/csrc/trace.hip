@@ -90,28 +90,39 @@ __device__ __forceinline__ Vec3 cosine_about(Rng& r, Vec3 n) {
   return (st * cosf(ph)) * t + (st * sinf(ph)) * bb + ct * n;
 }
 
+// One wrap of _apply_bc (threed.py:289-349) for a ray that missed the surface: if it leaves
+// through a lateral face of the box before the open top it re-enters at the opposite face with
+// the same direction (origin updated, returns true); otherwise it escapes (returns false).
+__device__ __forceinline__ bool wrap_ray(const Box& b, Vec3& o, Vec3 d) {
+  float tx = 1.0e30f, ty = 1.0e30f, tz = 1.0e30f;
+  if (d.x > 1.0e-9f) tx = (b.lx - o.x) / d.x;
+  if (d.x < -1.0e-9f) tx = (0.f - o.x) / d.x;
+  if (d.y > 1.0e-9f) ty = (b.ly - o.y) / d.y;
+  if (d.y < -1.0e-9f) ty = (0.f - o.y) / d.y;
+  if (d.z > 1.0e-9f) tz = (b.lz - o.z) / d.z;
+  if (tx < tz && tx < ty && tx > 1.0e-7f) {
+    float hy = o.y + tx * d.y, hz = o.z + tx * d.z;
+    o = v3(d.x < 0.f ? b.lx - 1.0e-4f : 1.0e-4f, hy, hz);
+    return true;
+  }
+  if (ty < tz && ty < 1.0e29f && ty > 1.0e-7f) {
+    float hx = o.x + ty * d.x, hz = o.z + ty * d.z;
+    o = v3(hx, d.y < 0.f ? b.ly - 1.0e-4f : 1.0e-4f, hz);
+    return true;
+  }
+  return false;
+}
+
+constexpr int kMaxWraps = 256;
+
 // _apply_bc + mesh_query_ray: with periodic lateral faces a ray that misses the surface and
-// leaves through x or y before the open top re-enters at the opposite face (<= 256 wraps).
+// leaves through x or y before the open top re-enters at the opposite face (<= 256 wraps, then
+// one last query whose result stands).
 __device__ __forceinline__ Hit trace_bc(const BvhView& bv, const Box& b, Vec3& o, Vec3 d) {
   if (!b.periodic) return petch::bvh_closest_hit(bv, o, d, 1.0e6f);
-  for (int w = 0; w < 256; ++w) {
+  for (int w = 0; w < kMaxWraps; ++w) {
     Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f);
-    if (h.face >= 0) return h;
-    float tx = 1.0e30f, ty = 1.0e30f, tz = 1.0e30f;
-    if (d.x > 1.0e-9f) tx = (b.lx - o.x) / d.x;
-    if (d.x < -1.0e-9f) tx = (0.f - o.x) / d.x;
-    if (d.y > 1.0e-9f) ty = (b.ly - o.y) / d.y;
-    if (d.y < -1.0e-9f) ty = (0.f - o.y) / d.y;
-    if (d.z > 1.0e-9f) tz = (b.lz - o.z) / d.z;
-    if (tx < tz && tx < ty && tx > 1.0e-7f) {
-      float hy = o.y + tx * d.y, hz = o.z + tx * d.z;
-      o = v3(d.x < 0.f ? b.lx - 1.0e-4f : 1.0e-4f, hy, hz);
-    } else if (ty < tz && ty < 1.0e29f && ty > 1.0e-7f) {
-      float hx = o.x + ty * d.x, hz = o.z + ty * d.z;
-      o = v3(hx, d.y < 0.f ? b.ly - 1.0e-4f : 1.0e-4f, hz);
-    } else {
-      return h;  // leaves through the open top
-    }
+    if (h.face >= 0 || !wrap_ray(b, o, d)) return h;
   }
   return petch::bvh_closest_hit(bv, o, d, 1.0e6f);
 }
@@ -171,7 +182,7 @@ __global__ void __launch_bounds__(kBS) k_trace_neutral_rr(BvhView bv, Box b, Sou
   Rng r;
   Vec3 o, d;
   float w = 0.f;
-  int bounce = 0;
+  int bounce = 0, wraps = 0;
   while (true) {
     unsigned ray;
     if (claim_ray(q, active, exhausted, ray)) {
@@ -179,16 +190,23 @@ __global__ void __launch_bounds__(kBS) k_trace_neutral_rr(BvhView bv, Box b, Sou
       gen_source(r, b, src, o, d);
       w = 1.f;
       bounce = 0;
+      wraps = 0;
       active = true;
     }
     if (!__ballot(active)) break;
     if (!active) continue;
-    // one bounce of _trace3d_cov_rr (threed.py:565-596)
-    Hit h = trace_bc(bv, b, o, d);
+    // one surface query of _trace3d_cov_rr (threed.py:565-596); a periodic wrap of _apply_bc
+    // is its own iteration, so a lane that wraps does not stall the others of its wavefront
+    Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f);
     if (h.face < 0) {
+      if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
+        ++wraps;
+        continue;
+      }
       active = false;
       continue;
     }
+    wraps = 0;
     Vec3 nn = hit_normal(bv, h.slot);
     if (petch::dot(nn, d) > 0.f) nn = -nn;
     deposit(fl, h.face, w);
@@ -235,7 +253,7 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
   Rng r;
   Vec3 o, d;
   float E = 0.f, w = 0.f;
-  int bounce = 0;
+  int bounce = 0, wraps = 0;
   const float sEsp = sqrtf(P.Eth_sp), sEie = sqrtf(P.Eth_ie), sEp = sqrtf(P.Eth_p);
   while (true) {
     unsigned ray;
@@ -246,16 +264,22 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
       if (E < 0.f) E = 0.f;
       w = 1.f;
       bounce = 0;
+      wraps = 0;
       active = true;
     }
     if (!__ballot(active)) break;
     if (!active) continue;
-    // one bounce of _trace3d_ion_yield (threed.py:625-712)
-    Hit h = trace_bc(bv, b, o, d);
+    // one surface query of _trace3d_ion_yield (threed.py:625-712); periodic wraps as above
+    Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f);
     if (h.face < 0) {
+      if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
+        ++wraps;
+        continue;
+      }
       active = false;
       continue;
     }
+    wraps = 0;
     Vec3 nn = hit_normal(bv, h.slot);
     if (petch::dot(nn, d) > 0.f) nn = -nn;
     float cosT = fminf(fmaxf(-petch::dot(d, nn), 0.f), 1.f);

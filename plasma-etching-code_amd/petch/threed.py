@@ -374,6 +374,10 @@ class EtchLoop:
                                         mesh.centroids, self.mg, self.chem, red=self.red)
         if out is not None:
             vmx, nbad = ops.read_rate_reduction(self.red)
+            bvh_err = mesh.bvh.status()
+            if bvh_err:
+                raise RuntimeError(f"step {step}: BVH traversal guard tripped (code {bvh_err}: "
+                                   "1 stack overflow, 2 corrupt tree); the flux is incomplete")
             if nbad:
                 warnings.warn(f"step {step}: {nbad} non-finite surface velocities (NaN/inf) zeroed; "
                               "check flux normalization / coverage if this persists.", RuntimeWarning)
