@@ -7,11 +7,13 @@ iterations of 8M F + 8M O Russian-roulette neutrals (88M primary rays per step) 
 velocity extension -> CFL substeps of upwind advection -> FSM reinit. A step is one such pass;
 the surface keeps etching from step to step.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--no-small-hole]
 
 Prints one JSON line: steps/s (value), rays/s, ray-hits/s, the roofline of the dominant kernel
-(the neutral trace) measured with HIP events on the launch stream, and the CPU baseline (the
-oracle's C/OpenMP restatement timed on a bounded sample of the same workload).
+(the neutral trace) measured with HIP events on the launch stream, the CPU baseline (the
+oracle's C/OpenMP restatement timed on a bounded sample of the same workload), and beside them
+the wall time of the reference's own small-hole loop (scripts/vps_sweep.py) against its
+published RTX 3090 figure.
 """
 import argparse
 import json
@@ -68,6 +70,29 @@ def build_loop(n_steps, timer=None):
     return loop, (nx, ny, nz)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_trace_neutral_rr_trench_ar10_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per k_trace_neutral_rr launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    summarised in profiles/ (KiB per dispatch, each scaled by what the same counter reads for
+    a plain device copy of known size). None when no summary is present."""
+    try:
+        vals = {}
+        for ln in open(TRAFFIC_FILE):
+            if ":" in ln and not ln.startswith("#"):
+                k, v = ln.split(":", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+        fetch = vals["fetch_kib_per_dispatch"] * vals["fetch_scale"]
+        write = vals["write_kib_per_dispatch"] * vals["write_scale"]
+        return 1024.0 * (fetch + write), os.path.relpath(TRAFFIC_FILE, ROOT)
+    except (OSError, KeyError):
+        return None, None
+
+
 def cpu_baseline(scale=100):
     """Oracle (C/OpenMP traces + FP64 numpy level set) on the same workload with the ray counts
     divided by `scale`; the ray-proportional part is scaled back up."""
@@ -113,12 +138,51 @@ def cpu_baseline(scale=100):
                         f"in single-threaded numpy"))
 
 
+# The reference's own timed loop (scripts/vps_sweep.py:46-55): hole of diameter 6 um, extent 14,
+# dx 0.25 (56 x 56 x 96), 40 steps over 3.0 s, 30k ion + 30k neutral rays per species, n_fp 1,
+# betaE 0.7, coverage sticking + warm start, fsm reinit; published wall time 1.61 s on an
+# RTX 3090 (box_results/vps_sweep_result.json:16-20), i.e. 24.8 steps/s.
+SMALL_HOLE_REF_WALL_S = 1.6116929054260254
+
+
+def small_hole_run(n_steps=40, rate=0.15):
+    import petch
+    from petch import threed as t3
+    p = dict(petch.PAR)
+    p['rate_scale'] = rate
+    p['betaE'] = 0.7
+    p.update(flux_smooth_gpu=True, gpu_source=True, gpu_mesh=True, gpu_warmstart=True,
+             device_flux=True, n_fp=1)
+    geo = dict(Lx=14.0, Ly=14.0, Lz=24, mask_th=2, sub_top=18, hole=True, t_end=3.0)
+    flags = petch.Flags(coverage_sticking=True, sampling="sobol", warm_start_coverage=True)
+    t0 = time.time()
+    g = t3.run_etch_3d(trench_width=6.0, dx=0.25, n_steps=n_steps, par=p, flags=flags,
+                       n_ion=30000, n_neu=30000, reinit_method="fsm", verbose=False, **geo)
+    return time.time() - t0, t3.center_depth_3d(g)
+
+
+def small_hole():
+    """Whole-call wall time of the reference's own small-hole loop, for comparison with its
+    published figure (not the headline value: that loop is launch-bound, not ray-bound)."""
+    small_hole_run(n_steps=4)                      # warm-up, as the reference's script does
+    walls, depth = [], None
+    for _ in range(3):
+        w, depth = small_hole_run()
+        walls.append(w)
+    w = sorted(walls)[1]
+    return {"workload": "hole d 6 um, 56x56x96, 40 steps, 30k rays/species (scripts/vps_sweep.py)",
+            "wall_s": w, "steps_per_s": 40.0 / w, "center_depth_um": depth,
+            "reference_wall_s_rtx3090": SMALL_HOLE_REF_WALL_S,
+            "speedup_vs_reference": SMALL_HOLE_REF_WALL_S / w}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-small-hole", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run plain device copies of 256 MiB (reference for PMC byte counts)")
     a = ap.parse_args(argv)
@@ -172,6 +236,7 @@ def main(argv=None):
     avg_launch_ms = nt_ms / nt_cnt
     bytes_launch = NEUTRAL_BYTES_PER_HIT * hits_n / nt_cnt
     achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic()
     res = {
         "metric": METRIC,
         "value": a.steps / wall,
@@ -196,10 +261,13 @@ def main(argv=None):
         "stages_ms_per_step": {k: v / a.steps for k, v in sorted(timer.ms.items())},
         "roofline": {"kernel": "k_trace_neutral_rr", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_launch_ms,
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "avg_launch_ms": avg_launch_ms,
                      "hits_per_launch": hits_n / nt_cnt,
                      "bytes_per_hit": NEUTRAL_BYTES_PER_HIT},
     }
+    if not a.no_small_hole:
+        res["small_hole"] = small_hole()
     if not a.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline()

@@ -177,76 +177,142 @@ __device__ __forceinline__ void sort2(float& ta, int& ca, float& tb, int& cb) {
   ca = c;
 }
 
-__device__ __forceinline__ void push_far(const BvhView& b, int* stack, float* stack_t, int& sp,
-                                         int node, float t) {
+
+// Per-lane traversal stack of (node, entry distance). The first L entries live in LDS, in this
+// lane's column of a workgroup array with row stride S (lane-major, so the lanes of a wavefront
+// hit distinct banks whatever their depths); deeper entries spill to a small scratch array.
+// With L = 0 the whole stack is in scratch.
+template <int S, int L>
+struct TravStack {
+  int* lds_n;
+  float* lds_t;
+  int spill_n[kBvhStack - L];
+  float spill_t[kBvhStack - L];
+  __device__ __forceinline__ void put(int i, int n, float t) {
+    if (L > 0 && i < L) {
+      lds_n[i * S] = n;
+      lds_t[i * S] = t;
+    } else {
+      spill_n[i - L] = n;
+      spill_t[i - L] = t;
+    }
+  }
+  __device__ __forceinline__ int node(int i) const {
+    if (L > 0 && i < L) return lds_n[i * S];
+    return spill_n[i - L];
+  }
+  __device__ __forceinline__ float dist(int i) const {
+    if (L > 0 && i < L) return lds_t[i * S];
+    return spill_t[i - L];
+  }
+};
+
+// Closest-hit query as a loop of steps: trav_begin sets it up, each trav_step visits one wide
+// node (its leaf children are tested on the spot) and returns true once the query is finished,
+// with the closest hit in T.h. Trav holds only scalars so that it lives in registers; the stack
+// is separate.
+constexpr int kTravDone = -1;
+
+struct Trav {
+  RayPre r;
+  Hit h;
+  int node, sp, steps;
+};
+
+__device__ __forceinline__ void trav_begin(const BvhView& b, Trav& T, Vec3 o, Vec3 d, float tmax) {
+  T.r = ray_pre(o, d);
+  T.h = Hit{-1, -1, tmax};
+  T.sp = 0;
+  T.steps = 0;
+  T.node = 0;
+  if (b.nf == 1) leaf_test(b, T.r, 0, T.h);
+  if (b.nf <= 1) T.node = kTravDone;
+}
+
+template <class Stack>
+__device__ __forceinline__ void push_far(const BvhView& b, Stack& stk, int& sp, int node, float t) {
   if (sp < kBvhStack) {
-    stack[sp] = node;
-    stack_t[sp] = t;
+    stk.put(sp, node, t);
     ++sp;
   } else if (b.err) {
     *b.err = 1;
   }
 }
 
-// Closest hit with 0 < t < tmax.
-__device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d, float tmax) {
-  Hit h{-1, -1, tmax};
-  if (b.nf <= 0) return h;
-  RayPre r = ray_pre(o, d);
-  if (b.nf == 1) {
-    leaf_test(b, r, 0, h);
-    return h;
-  }
-  int stack[kBvhStack];
-  float stack_t[kBvhStack];
-  int sp = 0;
-  int node = 0;
+template <class Stack>
+__device__ __forceinline__ bool trav_step(const BvhView& b, Trav& T, Stack& stk) {
+  const int node = T.node;
+  if (node == kTravDone) return true;
   // a tree visits each wide node at most once; the bound only guards a corrupt tree
-  const int max_steps = 2 * b.nf + 64;
-  for (int steps = 0;; ++steps) {
-    if (steps > max_steps || node < 0 || node >= b.nf - 1) {
-      if (b.err) *b.err = 2;
-      break;
-    }
-    const BvhWide* N = b.wide + node;
-    const float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
-    const int4 ref = N->ref;
-    int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
-    float t0 = c0 != kWideEmpty ? ray_box(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, h.t) : kMiss;
-    float t1 = c1 != kWideEmpty ? ray_box(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, h.t) : kMiss;
-    float t2 = c2 != kWideEmpty ? ray_box(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, h.t) : kMiss;
-    float t3 = c3 != kWideEmpty ? ray_box(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, h.t) : kMiss;
-    // leaves are tested on the spot
-    if (t0 < kHitLim && c0 < 0) { leaf_test(b, r, ~c0, h); t0 = kMiss; }
-    if (t1 < kHitLim && c1 < 0) { leaf_test(b, r, ~c1, h); t1 = kMiss; }
-    if (t2 < kHitLim && c2 < 0) { leaf_test(b, r, ~c2, h); t2 = kMiss; }
-    if (t3 < kHitLim && c3 < 0) { leaf_test(b, r, ~c3, h); t3 = kMiss; }
-    // inner children that now start beyond the best hit are dropped
-    const float lim = h.t * kBoxSlack;
-    if (!(t0 <= lim)) t0 = kMiss;
-    if (!(t1 <= lim)) t1 = kMiss;
-    if (!(t2 <= lim)) t2 = kMiss;
-    if (!(t3 <= lim)) t3 = kMiss;
-    // ascending by entry distance (5-exchange network)
-    sort2(t0, c0, t1, c1);
-    sort2(t2, c2, t3, c3);
-    sort2(t0, c0, t2, c2);
-    sort2(t1, c1, t3, c3);
-    sort2(t1, c1, t2, c2);
-    if (t0 < kHitLim) {
-      // push the farther ones far-to-near, descend the nearest
-      if (t3 < kHitLim) push_far(b, stack, stack_t, sp, c3, t3);
-      if (t2 < kHitLim) push_far(b, stack, stack_t, sp, c2, t2);
-      if (t1 < kHitLim) push_far(b, stack, stack_t, sp, c1, t1);
-      node = c0;
-    } else {
-      const float lim2 = h.t * kBoxSlack;
-      while (sp > 0 && !(stack_t[sp - 1] <= lim2)) --sp;
-      if (sp == 0) break;
-      node = stack[--sp];
-    }
+  if (++T.steps > 2 * b.nf + 64 || node < 0 || node >= b.nf - 1) {
+    if (b.err) *b.err = 2;
+    T.node = kTravDone;
+    return true;
   }
-  return h;
+  const RayPre& r = T.r;
+  Hit& h = T.h;
+  const BvhWide* N = b.wide + node;
+  const float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
+  const int4 ref = N->ref;
+  int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
+  float t0 = c0 != kWideEmpty ? ray_box(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, h.t) : kMiss;
+  float t1 = c1 != kWideEmpty ? ray_box(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, h.t) : kMiss;
+  float t2 = c2 != kWideEmpty ? ray_box(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, h.t) : kMiss;
+  float t3 = c3 != kWideEmpty ? ray_box(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, h.t) : kMiss;
+  // leaves are tested on the spot
+  if (t0 < kHitLim && c0 < 0) { leaf_test(b, r, ~c0, h); t0 = kMiss; }
+  if (t1 < kHitLim && c1 < 0) { leaf_test(b, r, ~c1, h); t1 = kMiss; }
+  if (t2 < kHitLim && c2 < 0) { leaf_test(b, r, ~c2, h); t2 = kMiss; }
+  if (t3 < kHitLim && c3 < 0) { leaf_test(b, r, ~c3, h); t3 = kMiss; }
+  // inner children that now start beyond the best hit are dropped
+  const float lim = h.t * kBoxSlack;
+  if (!(t0 <= lim)) t0 = kMiss;
+  if (!(t1 <= lim)) t1 = kMiss;
+  if (!(t2 <= lim)) t2 = kMiss;
+  if (!(t3 <= lim)) t3 = kMiss;
+  // ascending by entry distance (5-exchange network)
+  sort2(t0, c0, t1, c1);
+  sort2(t2, c2, t3, c3);
+  sort2(t0, c0, t2, c2);
+  sort2(t1, c1, t3, c3);
+  sort2(t1, c1, t2, c2);
+  int sp = T.sp;
+  if (t0 < kHitLim) {
+    // push the farther ones far-to-near, descend the nearest
+    if (t3 < kHitLim) push_far(b, stk, sp, c3, t3);
+    if (t2 < kHitLim) push_far(b, stk, sp, c2, t2);
+    if (t1 < kHitLim) push_far(b, stk, sp, c1, t1);
+    T.sp = sp;
+    T.node = c0;
+    return false;
+  }
+  while (sp > 0 && !(stk.dist(sp - 1) <= lim)) --sp;
+  if (sp == 0) {
+    T.sp = 0;
+    T.node = kTravDone;
+    return true;
+  }
+  --sp;
+  T.node = stk.node(sp);
+  T.sp = sp;
+  return false;
+}
+
+// Closest hit with 0 < t < tmax, on the caller's stack.
+template <class Stack>
+__device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d, float tmax,
+                                               Stack& stk) {
+  Trav T;
+  trav_begin(b, T, o, d, tmax);
+  while (!trav_step(b, T, stk)) {
+  }
+  return T.h;
+}
+
+// Closest hit with 0 < t < tmax (stack in scratch).
+__device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d, float tmax) {
+  TravStack<1, 0> stk;
+  return bvh_closest_hit(b, o, d, tmax, stk);
 }
 
 // Ericson, Real-Time Collision Detection 5.1.5: closest point of triangle abc to p.

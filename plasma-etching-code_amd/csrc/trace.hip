@@ -1,8 +1,9 @@
 // Ray-traced flux kernels. Each ray owns its Philox stream (seed, ray) and generates its source
 // ray in registers (no origin/direction buffers). The RR-neutral and faithful-ion kernels run a
-// persistent grid over a ray queue: a lane whose ray ends takes the next ray id, so divergent
-// bounce counts do not idle the wavefront. Deposits go to per-face float accumulators with
-// hardware float atomics.
+// persistent grid over a ray queue: every loop iteration is one surface query per busy lane,
+// and a lane whose ray ends takes the next ray id, so divergent bounce and wrap counts do not
+// idle the wavefront.
+// Deposits go to per-face float accumulators with hardware float atomics.
 //
 //   k_trace_neutral_rr   _trace3d_cov_rr    threed.py:549-596 (RR-weighted neutral, <= 1024 bounces)
 //   k_trace_ion_yield    _trace3d_ion_yield threed.py:599-712 (faithful ion, 5 deposits per hit)
@@ -172,32 +173,42 @@ __device__ __forceinline__ bool claim_ray(const RayQueue& q, bool active, bool& 
   return ray < (unsigned)q.n;
 }
 
-__global__ void __launch_bounds__(kBS) k_trace_neutral_rr(BvhView bv, Box b, Source src,
-                                                          const float* __restrict__ bare,
-                                                          float beta, RayQueue q, uint64_t seed,
-                                                          float* __restrict__ fl,
-                                                          unsigned long long* __restrict__ hits) {
+// Persistent trace loop shared by the RR-neutral and faithful-ion kernels. Every iteration is
+// one surface query per busy lane: a hit is handed to the policy, a miss wraps through a
+// periodic lateral face (_apply_bc) as an iteration of its own, or ends the ray. Lanes whose
+// rays ended take new ray ids from the queue at the top of the next iteration. The policy P
+// supplies the per-ray payload: start() draws the source ray, hit() deposits and returns the
+// next ray (true) or ends it (false). Per ray, the Philox draws happen in the same order
+// whatever the schedule, so the result does not depend on it.
+constexpr int kLdsStack = 16;  // stack entries per lane in LDS: 32 KB per workgroup
+
+template <class P>
+__device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b,
+                                                 const Source& src, const P& pol,
+                                                 const RayQueue& q, uint64_t seed,
+                                                 unsigned long long* __restrict__ hits) {
+  __shared__ int lds_n[kLdsStack * kBS];
+  __shared__ float lds_t[kLdsStack * kBS];
+  petch::TravStack<kBS, kLdsStack> stk;
+  stk.lds_n = lds_n + threadIdx.x;
+  stk.lds_t = lds_t + threadIdx.x;
+  typename P::Ray R;
+  Rng rng;
+  Vec3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, -1.f);
+  int wraps = 0;
   unsigned nh = 0;
   bool active = false, exhausted = false;
-  Rng r;
-  Vec3 o, d;
-  float w = 0.f;
-  int bounce = 0, wraps = 0;
   while (true) {
     unsigned ray;
     if (claim_ray(q, active, exhausted, ray)) {
-      r.init(seed, ray);
-      gen_source(r, b, src, o, d);
-      w = 1.f;
-      bounce = 0;
+      rng.init(seed, ray);
+      pol.start(rng, R, b, src, o, d);
       wraps = 0;
       active = true;
     }
     if (!__ballot(active)) break;
     if (!active) continue;
-    // one surface query of _trace3d_cov_rr (threed.py:565-596); a periodic wrap of _apply_bc
-    // is its own iteration, so a lane that wraps does not stall the others of its wavefront
-    Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f);
+    const Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f, stk);
     if (h.face < 0) {
       if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
         ++wraps;
@@ -207,85 +218,88 @@ __global__ void __launch_bounds__(kBS) k_trace_neutral_rr(BvhView bv, Box b, Sou
       continue;
     }
     wraps = 0;
-    Vec3 nn = hit_normal(bv, h.slot);
-    if (petch::dot(nn, d) > 0.f) nn = -nn;
-    deposit(fl, h.face, w);
     ++nh;
-    float S = bare[h.face] * beta;
-    w = w - w * S;
-    if (w <= 1.0e-6f) {
-      active = false;
-      continue;
-    }
-    if (w < 0.1f) {
-      if (r.uniform() < (1.f - w / 0.3f)) {
-        active = false;
-        continue;
-      }
-      w = 0.3f;
-    }
-    if (++bounce >= 1024) {
-      active = false;
-      continue;
-    }
-    Vec3 hit = o + h.t * d;
-    d = cosine_about(r, nn);
-    o = hit + 1.0e-4f * nn;
+    active = pol.hit(rng, R, bv, h, o, d);
   }
   count_hits(hits, nh);
 }
+
+// _trace3d_cov_rr (threed.py:549-596): deposit w at every hit, w *= 1 - bare*beta, Russian
+// roulette below 0.1 (survivors carry 0.3), cosine re-emission, <= 1024 bounces.
+struct NeutralRR {
+  const float* __restrict__ bare;
+  float beta;
+  float* __restrict__ fl;
+  struct Ray {
+    float w;
+    int bounce;
+  };
+  __device__ __forceinline__ void start(Rng& r, Ray& R, const Box& b, const Source& src, Vec3& o,
+                                        Vec3& d) const {
+    gen_source(r, b, src, o, d);
+    R.w = 1.f;
+    R.bounce = 0;
+  }
+  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const BvhView& bv, const Hit& h, Vec3& o,
+                                      Vec3& d) const {
+    Vec3 nn = hit_normal(bv, h.slot);
+    if (petch::dot(nn, d) > 0.f) nn = -nn;
+    deposit(fl, h.face, R.w);
+    float w = R.w;
+    const float S = bare[h.face] * beta;
+    w = w - w * S;
+    if (w <= 1.0e-6f) return false;
+    if (w < 0.1f) {
+      if (r.uniform() < (1.f - w / 0.3f)) return false;
+      w = 0.3f;
+    }
+    if (++R.bounce >= 1024) return false;
+    R.w = w;
+    const Vec3 p = o + h.t * d;
+    d = cosine_about(r, nn);
+    o = p + 1.0e-4f * nn;
+    return true;
+  }
+};
 
 struct IonParams {
   float meanE, sigmaE, A_sp, Eth_sp, B_sp, A_ie, Eth_ie, A_p, Eth_p;
   float inflect, minang, n_l, A_en, minE, thrmin, thrmax;
 };
 
-__global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Source src,
-                                                         IonParams P, RayQueue q, uint64_t seed,
-                                                         float* __restrict__ sp,
-                                                         float* __restrict__ ie,
-                                                         float* __restrict__ ip,
-                                                         float* __restrict__ arr,
-                                                         float* __restrict__ ang,
-                                                         unsigned long long* __restrict__ hits) {
-  unsigned nh = 0;
-  bool active = false, exhausted = false;
-  Rng r;
-  Vec3 o, d;
-  float E = 0.f, w = 0.f;
-  int bounce = 0, wraps = 0;
-  const float sEsp = sqrtf(P.Eth_sp), sEie = sqrtf(P.Eth_ie), sEp = sqrtf(P.Eth_p);
-  while (true) {
-    unsigned ray;
-    if (claim_ray(q, active, exhausted, ray)) {
-      r.init(seed, ray);
-      gen_source(r, b, src, o, d);
-      E = P.meanE + P.sigmaE * r.normal();
-      if (E < 0.f) E = 0.f;
-      w = 1.f;
-      bounce = 0;
-      wraps = 0;
-      active = true;
-    }
-    if (!__ballot(active)) break;
-    if (!active) continue;
-    // one surface query of _trace3d_ion_yield (threed.py:625-712); periodic wraps as above
-    Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f);
-    if (h.face < 0) {
-      if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
-        ++wraps;
-        continue;
-      }
-      active = false;
-      continue;
-    }
-    wraps = 0;
+// _trace3d_ion_yield (threed.py:599-712): energy ~ N(meanE, sigmaE) clamped at 0; every hit
+// deposits the sputter / ion-enhanced / passivation yields, the arrival weight and the weighted
+// cosine; sticking ramps from 1 below thetaRMin to 0 at 90 deg; energy loss by the inflection
+// law; coned-cosine re-emission about the specular direction.
+struct IonYield {
+  IonParams P;
+  float sEsp, sEie, sEp;
+  float* __restrict__ sp;
+  float* __restrict__ ie;
+  float* __restrict__ ip;
+  float* __restrict__ arr;
+  float* __restrict__ ang;
+  struct Ray {
+    float E, w;
+    int bounce;
+  };
+  __device__ __forceinline__ void start(Rng& r, Ray& R, const Box& b, const Source& src, Vec3& o,
+                                        Vec3& d) const {
+    gen_source(r, b, src, o, d);
+    float E = P.meanE + P.sigmaE * r.normal();
+    R.E = E < 0.f ? 0.f : E;
+    R.w = 1.f;
+    R.bounce = 0;
+  }
+  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const BvhView& bv, const Hit& h, Vec3& o,
+                                      Vec3& d) const {
     Vec3 nn = hit_normal(bv, h.slot);
     if (petch::dot(nn, d) > 0.f) nn = -nn;
-    float cosT = fminf(fmaxf(-petch::dot(d, nn), 0.f), 1.f);
-    float incAng = acosf(cosT);
-    float sqrtE = sqrtf(E);
-    float f_sp = fmaxf((1.f + P.B_sp * (1.f - cosT * cosT)) * cosT, 0.f);
+    const float E = R.E, w = R.w;
+    const float cosT = fminf(fmaxf(-petch::dot(d, nn), 0.f), 1.f);
+    const float incAng = acosf(cosT);
+    const float sqrtE = sqrtf(E);
+    const float f_sp = fmaxf((1.f + P.B_sp * (1.f - cosT * cosT)) * cosT, 0.f);
     float f_ie = 1.f;
     if (cosT < 0.5f) f_ie = fmaxf(3.f - 6.f * incAng / kPi, 0.f);
     deposit(sp, h.face, P.A_sp * fmaxf(sqrtE - sEsp, 0.f) * f_sp * w);
@@ -293,17 +307,13 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
     deposit(ip, h.face, P.A_p * fmaxf(sqrtE - sEp, 0.f) * f_ie * w);
     deposit(arr, h.face, w);
     deposit(ang, h.face, cosT * w);
-    ++nh;
     float stick = 1.f;
     if (incAng > P.thrmin) {
       float rr = (incAng - P.thrmin) / (P.thrmax - P.thrmin);
       rr = fminf(fmaxf(rr, 0.f), 1.f);
       stick = 1.f - rr;
     }
-    if (stick >= 1.f) {
-      active = false;
-      continue;
-    }
+    if (stick >= 1.f) return false;
     float Eref;
     if (incAng >= P.inflect)
       Eref = 1.f - (1.f - P.A_en) * (kPi2 - incAng) / (kPi2 - P.inflect);
@@ -318,18 +328,13 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
       }
     }
     newE = fminf(fmaxf(newE, 0.f), E);
-    if (newE <= P.minE) {
-      active = false;
-      continue;
-    }
-    E = newE;
-    w = w - w * stick;
-    if (w <= 1.0e-4f || ++bounce >= 1024) {
-      active = false;
-      continue;
-    }
-    Vec3 hit = o + h.t * d;
-    Vec3 axis = petch::normalize(d - (2.f * petch::dot(d, nn)) * nn);
+    if (newE <= P.minE) return false;
+    R.E = newE;
+    const float w2 = w - w * stick;
+    if (w2 <= 1.0e-4f || ++R.bounce >= 1024) return false;
+    R.w = w2;
+    const Vec3 p = o + h.t * d;
+    const Vec3 axis = petch::normalize(d - (2.f * petch::dot(d, nn)) * nn);
     Vec3 tt, bb;
     if (axis.z < -0.9999999f) {
       tt = v3(0.f, -1.f, 0.f);
@@ -340,7 +345,7 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
       tt = v3(1.f - axis.x * axis.x * aa, cc, -axis.x);
       bb = v3(cc, 1.f - axis.y * axis.y * aa, -axis.y);
     }
-    float maxcone = kPi2 - fminf(incAng, P.minang);
+    const float maxcone = kPi2 - fminf(incAng, P.minang);
     float theta = 0.f;
     for (int t = 0; t < 64; ++t) {
       float uu = sqrtf(r.uniform());
@@ -348,14 +353,29 @@ __global__ void __launch_bounds__(kBS) k_trace_ion_yield(BvhView bv, Box b, Sour
       theta = maxcone * ss;
       if (r.uniform() * theta * uu <= cosf(kPi2 * ss) * sinf(theta)) break;
     }
-    float phi = kTwoPi * r.uniform();
+    const float phi = kTwoPi * r.uniform();
     Vec3 nd = petch::normalize(sinf(theta) * (cosf(phi) * tt + sinf(phi) * bb) + cosf(theta) * axis);
-    float dn = petch::dot(nd, nn);
+    const float dn = petch::dot(nd, nn);
     if (dn <= 0.f) nd = nd - (2.f * dn) * nn;
     d = nd;
-    o = hit + 1.0e-4f * nn;
+    o = p + 1.0e-4f * nn;
+    return true;
   }
-  count_hits(hits, nh);
+};
+
+// 5 waves per SIMD: the register budget that still leaves the loop without spills
+#define PETCH_TRACE_BOUNDS __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5)))
+
+__global__ void PETCH_TRACE_BOUNDS k_trace_neutral_rr(BvhView bv, Box b, Source src, NeutralRR pol,
+                                                      RayQueue q, uint64_t seed,
+                                                      unsigned long long* __restrict__ hits) {
+  persistent_trace(bv, b, src, pol, q, seed, hits);
+}
+
+__global__ void PETCH_TRACE_BOUNDS k_trace_ion_yield(BvhView bv, Box b, Source src, IonYield pol,
+                                                     RayQueue q, uint64_t seed,
+                                                     unsigned long long* __restrict__ hits) {
+  persistent_trace(bv, b, src, pol, q, seed, hits);
 }
 
 __global__ void __launch_bounds__(kBS) k_trace_legacy(BvhView bv, Source src, Box b,
@@ -450,14 +470,18 @@ Source make_source(const petch_source* s) {
   return Source{s->kind, s->sigma, s->exponent};
 }
 
-// Grid of the persistent traces: enough workgroups to fill every CU several times over, never
-// more than the rays need.
-unsigned persistent_blocks(petch_ctx* ctx, int n) {
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+// Grid of a persistent trace: as many workgroups as can be resident at once (CUs x the
+// kernel's occupancy, from its register and LDS use), never more than the rays need.
+unsigned persistent_blocks(petch_ctx* ctx, const void* kernel, int n) {
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+          hipSuccess ||
       cus <= 0)
     cus = 256;
-  long long want = (long long)cus * 8;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBS, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 4;
+  long long want = (long long)cus * per_cu;
   long long need = ((long long)n + kBS - 1) / kBS;
   return (unsigned)(need < want ? need : want);
 }
@@ -500,9 +524,11 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   RayQueue q;
   int rc2 = make_queue(ctx, n, q);
   if (rc2) return rc2;
-  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(persistent_blocks(ctx, n)), dim3(kBS), 0,
-                     ctx->stream, petch_bvh_view(bvh), make_box(box), make_source(src), bare, beta,
-                     q, seed, fl, petch_hit_counter(ctx, 0));
+  NeutralRR pol{bare, beta, fl};
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr, n);
+  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(grid), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), make_box(box), make_source(src), pol, q, seed,
+                     petch_hit_counter(ctx, 0));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
@@ -526,9 +552,11 @@ extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const
   RayQueue q;
   int rc2 = make_queue(ctx, n, q);
   if (rc2) return rc2;
-  hipLaunchKernelGGL(k_trace_ion_yield, dim3(persistent_blocks(ctx, n)), dim3(kBS), 0,
-                     ctx->stream, petch_bvh_view(bvh), make_box(box), make_source(src), P, q, seed,
-                     sp, ie, ipass, arr, ang, petch_hit_counter(ctx, 1));
+  IonYield pol{P, sqrtf(P.Eth_sp), sqrtf(P.Eth_ie), sqrtf(P.Eth_p), sp, ie, ipass, arr, ang};
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_ion_yield, n);
+  hipLaunchKernelGGL(k_trace_ion_yield, dim3(grid), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), make_box(box), make_source(src), pol, q, seed,
+                     petch_hit_counter(ctx, 1));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
