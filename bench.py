@@ -93,9 +93,9 @@ def measured_traffic():
         return None, None
 
 
-def cpu_baseline(scale=100):
-    """Oracle (C/OpenMP traces + FP64 numpy level set) on the same workload with the ray counts
-    divided by `scale`; the ray-proportional part is scaled back up."""
+def cpu_baseline(scale=1):
+    """Oracle (C/OpenMP traces + FP64 numpy level set) on one step of the same workload; with
+    scale > 1 the ray counts are divided by `scale` and the ray-proportional time scaled back."""
     import numpy as np
     from oracle import flux as ofl
     from oracle import levelset as ol
@@ -133,9 +133,10 @@ def cpu_baseline(scale=100):
     return dict(value=1.0 / step_s, unit="steps/s", cores=cores, kind="port",
                 sample=(f"one etch step of the same trench state on the oracle: marching cubes, "
                         f"rate, extension, {nsub} advection substeps and FSM reinit at full size "
-                        f"({t_mesh + t_rest:.1f} s) plus the coupled flux at 1/{scale} of the rays "
-                        f"({t_flux:.1f} s, scaled x{scale}); traces in C with OpenMP, level set "
-                        f"in single-threaded numpy"))
+                        f"({t_mesh + t_rest:.1f} s) plus the coupled flux with "
+                        f"{'all' if scale == 1 else f'1/{scale} of'} the rays ({t_flux:.1f} s"
+                        f"{'' if scale == 1 else f', scaled x{scale}'}); traces in C with OpenMP, "
+                        f"level set in single-threaded numpy"))
 
 
 # The reference's own timed loop (scripts/vps_sweep.py:46-55): hole of diameter 6 um, extent 14,

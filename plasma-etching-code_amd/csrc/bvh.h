@@ -180,8 +180,8 @@ __device__ __forceinline__ void sort2(float& ta, int& ca, float& tb, int& cb) {
 
 // Per-lane traversal stack of (node, entry distance). The first L entries live in LDS, in this
 // lane's column of a workgroup array with row stride S (lane-major, so the lanes of a wavefront
-// hit distinct banks whatever their depths); deeper entries spill to a small scratch array.
-// With L = 0 the whole stack is in scratch.
+// hit distinct banks whatever their depths); deeper entries spill to a scratch array. With
+// L = 0 the whole stack is in scratch.
 template <int S, int L>
 struct TravStack {
   int* lds_n;
@@ -206,6 +206,7 @@ struct TravStack {
     return spill_t[i - L];
   }
 };
+using ScratchStack = TravStack<1, 0>;
 
 // Closest-hit query as a loop of steps: trav_begin sets it up, each trav_step visits one wide
 // node (its leaf children are tested on the spot) and returns true once the query is finished,
@@ -219,8 +220,9 @@ struct Trav {
   int node, sp, steps;
 };
 
-__device__ __forceinline__ void trav_begin(const BvhView& b, Trav& T, Vec3 o, Vec3 d, float tmax) {
-  T.r = ray_pre(o, d);
+__device__ __forceinline__ void trav_begin(const BvhView& b, Trav& T, const RayPre& r,
+                                           float tmax) {
+  T.r = r;
   T.h = Hit{-1, -1, tmax};
   T.sp = 0;
   T.steps = 0;
@@ -299,20 +301,22 @@ __device__ __forceinline__ bool trav_step(const BvhView& b, Trav& T, Stack& stk)
 }
 
 // Closest hit with 0 < t < tmax, on the caller's stack.
+// (r from ray_pre; a caller that only moves the origin of a ray, as a periodic wrap does, can
+// keep r and overwrite r.o.)
 template <class Stack>
-__device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d, float tmax,
+__device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, const RayPre& r, float tmax,
                                                Stack& stk) {
   Trav T;
-  trav_begin(b, T, o, d, tmax);
+  trav_begin(b, T, r, tmax);
   while (!trav_step(b, T, stk)) {
   }
   return T.h;
 }
 
-// Closest hit with 0 < t < tmax (stack in scratch).
+// Closest hit with 0 < t < tmax.
 __device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, Vec3 o, Vec3 d, float tmax) {
-  TravStack<1, 0> stk;
-  return bvh_closest_hit(b, o, d, tmax, stk);
+  ScratchStack stk;
+  return bvh_closest_hit(b, ray_pre(o, d), tmax, stk);
 }
 
 // Ericson, Real-Time Collision Detection 5.1.5: closest point of triangle abc to p.

@@ -1,8 +1,8 @@
 // Ray-traced flux kernels. Each ray owns its Philox stream (seed, ray) and generates its source
 // ray in registers (no origin/direction buffers). The RR-neutral and faithful-ion kernels run a
-// persistent grid over a ray queue: every loop iteration is one surface query per busy lane,
-// and a lane whose ray ends takes the next ray id, so divergent bounce and wrap counts do not
-// idle the wavefront.
+// persistent grid over a ray queue and advance every lane's BVH query one wide node per loop
+// iteration; hits are handled, and lanes whose rays ended refilled, in batches, so neither
+// divergent query lengths nor divergent bounce and wrap counts idle the wavefront for long.
 // Deposits go to per-face float accumulators with hardware float atomics.
 //
 //   k_trace_neutral_rr   _trace3d_cov_rr    threed.py:549-596 (RR-weighted neutral, <= 1024 bounces)
@@ -173,14 +173,19 @@ __device__ __forceinline__ bool claim_ray(const RayQueue& q, bool active, bool& 
   return ray < (unsigned)q.n;
 }
 
-// Persistent trace loop shared by the RR-neutral and faithful-ion kernels. Every iteration is
-// one surface query per busy lane: a hit is handed to the policy, a miss wraps through a
-// periodic lateral face (_apply_bc) as an iteration of its own, or ends the ray. Lanes whose
-// rays ended take new ray ids from the queue at the top of the next iteration. The policy P
-// supplies the per-ray payload: start() draws the source ray, hit() deposits and returns the
-// next ray (true) or ends it (false). Per ray, the Philox draws happen in the same order
-// whatever the schedule, so the result does not depend on it.
-constexpr int kLdsStack = 16;  // stack entries per lane in LDS: 32 KB per workgroup
+// Persistent trace loop shared by the RR-neutral and faithful-ion kernels. Each lane is idle,
+// traversing (one wide BVH node per loop iteration) or holding a surface hit. A query that
+// ends in a miss is handled on the spot: a periodic wrap (_apply_bc) moves the origin and
+// restarts the query, otherwise the ray escapes and the lane goes idle. Hits are handled in
+// batches: only when at most kRefillAt lanes of the wavefront are still traversing does it run
+// the policy's hit code and refill idle lanes from the ray queue, so the (short, very uneven)
+// queries keep most lanes busy. The policy P supplies the per-ray payload: start() draws the
+// source ray, hit() deposits and returns the next ray (true) or ends it (false). Per ray, the
+// Philox draws happen in the same order whatever the schedule, so the result does not depend on
+// it.
+constexpr int kRefillAt = 24;
+constexpr int kLdsStack = 8;  // traversal stack entries per lane held in LDS (16 KB/workgroup)
+enum LaneState { kIdle = 0, kTrav = 1, kHit = 2 };
 
 template <class P>
 __device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b,
@@ -192,34 +197,50 @@ __device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b
   petch::TravStack<kBS, kLdsStack> stk;
   stk.lds_n = lds_n + threadIdx.x;
   stk.lds_t = lds_t + threadIdx.x;
+  petch::Trav T;
   typename P::Ray R;
   Rng rng;
   Vec3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, -1.f);
+  petch::RayPre rp;  // per-direction constants of the current ray; a wrap only moves rp.o
   int wraps = 0;
   unsigned nh = 0;
-  bool active = false, exhausted = false;
+  int st = kIdle;
+  bool exhausted = false;
   while (true) {
-    unsigned ray;
-    if (claim_ray(q, active, exhausted, ray)) {
-      rng.init(seed, ray);
-      pol.start(rng, R, b, src, o, d);
-      wraps = 0;
-      active = true;
-    }
-    if (!__ballot(active)) break;
-    if (!active) continue;
-    const Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f, stk);
-    if (h.face < 0) {
-      if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
-        ++wraps;
-        continue;
+    if (__popcll(__ballot(st == kTrav)) <= kRefillAt) {
+      if (st == kHit) {
+        ++nh;
+        if (pol.hit(rng, R, bv, T.h, o, d)) {
+          rp = petch::ray_pre(o, d);
+          wraps = 0;
+          petch::trav_begin(bv, T, rp, 1.0e6f);
+          st = kTrav;
+        } else {
+          st = kIdle;
+        }
       }
-      active = false;
-      continue;
+      unsigned ray;
+      if (claim_ray(q, st != kIdle, exhausted, ray)) {
+        rng.init(seed, ray);
+        pol.start(rng, R, b, src, o, d);
+        rp = petch::ray_pre(o, d);
+        wraps = 0;
+        petch::trav_begin(bv, T, rp, 1.0e6f);
+        st = kTrav;
+      }
+      if (!__ballot(st == kTrav)) break;
     }
-    wraps = 0;
-    ++nh;
-    active = pol.hit(rng, R, bv, h, o, d);
+    if (st == kTrav && petch::trav_step(bv, T, stk)) {
+      if (T.h.face >= 0) {
+        st = kHit;
+      } else if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
+        rp.o = o;
+        ++wraps;
+        petch::trav_begin(bv, T, rp, 1.0e6f);
+      } else {
+        st = kIdle;
+      }
+    }
   }
   count_hits(hits, nh);
 }
