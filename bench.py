@@ -28,6 +28,9 @@ for _p in (os.path.join(ROOT, "plasma-etching-code_amd"), ROOT):
 
 METRIC = "3D etch steps/s & rays/s (1 feature, 64-batch); fwd+adjoint / fwd cost ratio"
 HBM_PEAK_GBS = 8000.0
+# Each k_trace_neutral_rr launch carries the F and the O rays of one fixed-point iteration
+# (2 x 8M primary rays); a ray-hit of either species reads its 36 B triangle and 4 B bare value
+# and read-modify-writes its 4 B accumulator.
 NEUTRAL_BYTES_PER_HIT = 48.0     # 36 B triangle + 4 B bare + 8 B accumulator read-modify-write
 ION_BYTES_PER_HIT = 76.0         # 36 B triangle + 5 x 8 B accumulators
 

@@ -166,6 +166,15 @@ int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petch_source* s
 int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
                            const petch_source* src, const float* bare, float beta, int n,
                            uint64_t seed, float* fl);
+/* The two _trace3d_cov_rr launches of one flux/coverage iteration (F then O, threed.py:876-880)
+ * as one launch over the same surface and coverage: species a (beta_a, n_a rays, seed_a, fl_a)
+ * and species b. Each ray keeps the stream it has in two petch_trace_neutral_rr calls, so the
+ * outputs equal theirs up to the order of the float atomic adds; one launch halves the tail of
+ * rays still bouncing when the queue runs dry. No reference counterpart of its own. */
+int petch_trace_neutral_rr_pair(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                                const petch_source* src, const float* bare, float beta_a,
+                                int n_a, uint64_t seed_a, float* fl_a, float beta_b, int n_b,
+                                uint64_t seed_b, float* fl_b);
 /* _trace3d_ion_yield (threed.py:599-712): per-ion energy, yield-weighted sputter / ion-enhanced
  * / passivation deposits, arrival weight and weighted cos at every hit, angle-dependent sticking,
  * energy loss and coned-cosine reflection. */
@@ -179,7 +188,7 @@ int petch_trace_legacy(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* bo
                        const petch_source* src, float sticking, int n_reemit, int specular,
                        float cos_thr, float eta, int n, uint64_t seed, float* flux,
                        float* angacc);
-/* Ray-hit counters accumulated by petch_trace_neutral_rr (counts_host[0]) and
+/* Ray-hit counters accumulated by petch_trace_neutral_rr[_pair] (counts_host[0]) and
  * petch_trace_ion_yield (counts_host[1]) on this context since the last reset (work accounting
  * for throughput reports; no reference counterpart). counts_host may be NULL; reset != 0 zeroes
  * them after the read. Synchronises when reading. */

@@ -179,10 +179,10 @@ __device__ __forceinline__ bool claim_ray(const RayQueue& q, bool active, bool& 
 // restarts the query, otherwise the ray escapes and the lane goes idle. Hits are handled in
 // batches: only when at most kRefillAt lanes of the wavefront are still traversing does it run
 // the policy's hit code and refill idle lanes from the ray queue, so the (short, very uneven)
-// queries keep most lanes busy. The policy P supplies the per-ray payload: start() draws the
-// source ray, hit() deposits and returns the next ray (true) or ends it (false). Per ray, the
-// Philox draws happen in the same order whatever the schedule, so the result does not depend on
-// it.
+// queries keep most lanes busy. The policy P supplies the per-ray payload: start() keys the
+// ray's Philox stream and draws its source ray, hit() deposits and returns the next ray (true)
+// or ends it (false). Per ray, the Philox draws happen in the same order whatever the schedule,
+// so the result does not depend on it.
 constexpr int kRefillAt = 24;
 constexpr int kLdsStack = 8;  // traversal stack entries per lane held in LDS (16 KB/workgroup)
 enum LaneState { kIdle = 0, kTrav = 1, kHit = 2 };
@@ -190,7 +190,7 @@ enum LaneState { kIdle = 0, kTrav = 1, kHit = 2 };
 template <class P>
 __device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b,
                                                  const Source& src, const P& pol,
-                                                 const RayQueue& q, uint64_t seed,
+                                                 const RayQueue& q,
                                                  unsigned long long* __restrict__ hits) {
   __shared__ int lds_n[kLdsStack * kBS];
   __shared__ float lds_t[kLdsStack * kBS];
@@ -221,8 +221,7 @@ __device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b
       }
       unsigned ray;
       if (claim_ray(q, st != kIdle, exhausted, ray)) {
-        rng.init(seed, ray);
-        pol.start(rng, R, b, src, o, d);
+        pol.start(rng, R, ray, b, src, o, d);
         rp = petch::ray_pre(o, d);
         wraps = 0;
         petch::trav_begin(bv, T, rp, 1.0e6f);
@@ -246,17 +245,25 @@ __device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b
 }
 
 // _trace3d_cov_rr (threed.py:549-596): deposit w at every hit, w *= 1 - bare*beta, Russian
-// roulette below 0.1 (survivors carry 0.3), cosine re-emission, <= 1024 bounces.
+// roulette below 0.1 (survivors carry 0.3), cosine re-emission, <= 1024 bounces. One launch
+// can carry two species over the same surface (the F and O traces of one fixed-point
+// iteration, threed.py:876-880): queue ids [0, n0) are species 0 with stream (seed[0], id),
+// the rest species 1 with stream (seed[1], id - n0), exactly the streams of two launches.
 struct NeutralRR {
   const float* __restrict__ bare;
-  float beta;
-  float* __restrict__ fl;
+  float beta[2];
+  float* fl[2];
+  uint64_t seed[2];
+  unsigned n0;
   struct Ray {
     float w;
     int bounce;
+    int sp;
   };
-  __device__ __forceinline__ void start(Rng& r, Ray& R, const Box& b, const Source& src, Vec3& o,
-                                        Vec3& d) const {
+  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, const Box& b,
+                                        const Source& src, Vec3& o, Vec3& d) const {
+    R.sp = ray >= n0;
+    r.init(R.sp ? seed[1] : seed[0], R.sp ? ray - n0 : ray);
     gen_source(r, b, src, o, d);
     R.w = 1.f;
     R.bounce = 0;
@@ -265,9 +272,9 @@ struct NeutralRR {
                                       Vec3& d) const {
     Vec3 nn = hit_normal(bv, h.slot);
     if (petch::dot(nn, d) > 0.f) nn = -nn;
-    deposit(fl, h.face, R.w);
+    deposit(R.sp ? fl[1] : fl[0], h.face, R.w);
     float w = R.w;
-    const float S = bare[h.face] * beta;
+    const float S = bare[h.face] * (R.sp ? beta[1] : beta[0]);
     w = w - w * S;
     if (w <= 1.0e-6f) return false;
     if (w < 0.1f) {
@@ -300,12 +307,14 @@ struct IonYield {
   float* __restrict__ ip;
   float* __restrict__ arr;
   float* __restrict__ ang;
+  uint64_t seed;
   struct Ray {
     float E, w;
     int bounce;
   };
-  __device__ __forceinline__ void start(Rng& r, Ray& R, const Box& b, const Source& src, Vec3& o,
-                                        Vec3& d) const {
+  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, const Box& b,
+                                        const Source& src, Vec3& o, Vec3& d) const {
+    r.init(seed, ray);
     gen_source(r, b, src, o, d);
     float E = P.meanE + P.sigmaE * r.normal();
     R.E = E < 0.f ? 0.f : E;
@@ -388,15 +397,15 @@ struct IonYield {
 #define PETCH_TRACE_BOUNDS __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5)))
 
 __global__ void PETCH_TRACE_BOUNDS k_trace_neutral_rr(BvhView bv, Box b, Source src, NeutralRR pol,
-                                                      RayQueue q, uint64_t seed,
+                                                      RayQueue q,
                                                       unsigned long long* __restrict__ hits) {
-  persistent_trace(bv, b, src, pol, q, seed, hits);
+  persistent_trace(bv, b, src, pol, q, hits);
 }
 
 __global__ void PETCH_TRACE_BOUNDS k_trace_ion_yield(BvhView bv, Box b, Source src, IonYield pol,
-                                                     RayQueue q, uint64_t seed,
+                                                     RayQueue q,
                                                      unsigned long long* __restrict__ hits) {
-  persistent_trace(bv, b, src, pol, q, seed, hits);
+  persistent_trace(bv, b, src, pol, q, hits);
 }
 
 __global__ void __launch_bounds__(kBS) k_trace_legacy(BvhView bv, Source src, Box b,
@@ -534,6 +543,23 @@ extern "C" int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petc
   return PETCH_OK;
 }
 
+namespace {
+
+int launch_neutral(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                   const petch_source* src, const NeutralRR& pol, int n) {
+  RayQueue q;
+  int rc = make_queue(ctx, n, q);
+  if (rc) return rc;
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr, n);
+  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(grid), dim3(kBS), 0, ctx->stream,
+                     petch_bvh_view(bvh), make_box(box), make_source(src), pol, q,
+                     petch_hit_counter(ctx, 0));
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+}  // namespace
+
 extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
                                       const petch_source* src, const float* bare, float beta,
                                       int n, uint64_t seed, float* fl) {
@@ -542,16 +568,29 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   int rc = zero(ctx, fl, bvh->nf);
   if (rc) return rc;
   if (n == 0 || bvh->nf == 0) return PETCH_OK;
-  RayQueue q;
-  int rc2 = make_queue(ctx, n, q);
-  if (rc2) return rc2;
-  NeutralRR pol{bare, beta, fl};
-  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr, n);
-  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(grid), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), make_box(box), make_source(src), pol, q, seed,
-                     petch_hit_counter(ctx, 0));
-  PETCH_LAUNCH_CHECK(ctx);
-  return PETCH_OK;
+  NeutralRR pol{bare, {beta, beta}, {fl, fl}, {seed, seed}, (unsigned)n};
+  return launch_neutral(ctx, bvh, box, src, pol, n);
+}
+
+extern "C" int petch_trace_neutral_rr_pair(petch_ctx* ctx, const petch_bvh* bvh,
+                                           const petch_box* box, const petch_source* src,
+                                           const float* bare, float beta_a, int n_a,
+                                           uint64_t seed_a, float* fl_a, float beta_b, int n_b,
+                                           uint64_t seed_b, float* fl_b) {
+  PETCH_ARG(ctx, ctx && bvh && box && src && bare && fl_a && fl_b,
+            "trace_neutral_rr_pair: null pointer");
+  PETCH_ARG(ctx, n_a >= 0 && n_b >= 0, "trace_neutral_rr_pair: negative ray count");
+  PETCH_ARG(ctx, (long long)n_a + n_b <= 0x7fffffffLL,
+            "trace_neutral_rr_pair: n_a + n_b > 2^31-1");
+  PETCH_ARG(ctx, fl_a != fl_b, "trace_neutral_rr_pair: the two species need distinct outputs");
+  int rc = zero(ctx, fl_a, bvh->nf);
+  if (rc) return rc;
+  rc = zero(ctx, fl_b, bvh->nf);
+  if (rc) return rc;
+  const int n = n_a + n_b;
+  if (n == 0 || bvh->nf == 0) return PETCH_OK;
+  NeutralRR pol{bare, {beta_a, beta_b}, {fl_a, fl_b}, {seed_a, seed_b}, (unsigned)n_a};
+  return launch_neutral(ctx, bvh, box, src, pol, n);
 }
 
 extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
@@ -573,10 +612,11 @@ extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const
   RayQueue q;
   int rc2 = make_queue(ctx, n, q);
   if (rc2) return rc2;
-  IonYield pol{P, sqrtf(P.Eth_sp), sqrtf(P.Eth_ie), sqrtf(P.Eth_p), sp, ie, ipass, arr, ang};
+  IonYield pol{P, sqrtf(P.Eth_sp), sqrtf(P.Eth_ie), sqrtf(P.Eth_p), sp, ie, ipass, arr, ang,
+               seed};
   const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_ion_yield, n);
   hipLaunchKernelGGL(k_trace_ion_yield, dim3(grid), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), make_box(box), make_source(src), pol, q, seed,
+                     petch_bvh_view(bvh), make_box(box), make_source(src), pol, q,
                      petch_hit_counter(ctx, 1));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;

@@ -99,6 +99,8 @@ SIGNATURES = {
     "petch_gen_source": [P, POINTER(Box), POINTER(Source), c_int, c_uint64, P, P],
     "petch_trace_neutral_rr": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int, c_uint64,
                                P],
+    "petch_trace_neutral_rr_pair": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int,
+                                    c_uint64, P, c_float, c_int, c_uint64, P],
     "petch_trace_ion_yield": [P, P, POINTER(Box), POINTER(Source), POINTER(IonParams), c_int,
                               c_uint64, P, P, P, P, P],
     "petch_trace_legacy": [P, P, POINTER(Box), POINTER(Source), c_float, c_int, c_int, c_float,

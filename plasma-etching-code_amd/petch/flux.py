@@ -130,10 +130,14 @@ def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, time
     nsrc = ops.make_source(ops.SRC_NEUTRAL)
     chem = chem_params(par, flags)
 
-    def neutral(beta, bare, sd):
+    def neutral_pair(bare, it):
+        # the F and O traces of one iteration (threed.py:876-880), seeds seed*9+2+2it and
+        # seed*9+3+2it, as one launch
         with timer.region("neutral_trace"):
-            fl = ops.trace_neutral_rr(mesh.bvh, box, nsrc, bare, beta, n_neu, sd & SEED_MASK)
-        return finish(fl, A_src / n_neu, 8.0)
+            fF, fO = ops.trace_neutral_rr_pair(mesh.bvh, box, nsrc, bare,
+                                               betaE, n_neu, (seed * 9 + 2 + 2 * it) & SEED_MASK,
+                                               betaO, n_neu, (seed * 9 + 3 + 2 * it) & SEED_MASK)
+        return finish(fF, A_src / n_neu, 8.0), finish(fO, A_src / n_neu, 8.0)
 
     n_fp = int(par.get('n_fp', 1 if bare_init is not None else 4))
     if bare_init is None:
@@ -142,8 +146,7 @@ def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, time
         bare = torch.clamp(bare_init.to(torch.float32), 0.0, 1.0)
     m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
     for it in range(n_fp):
-        m_F = neutral(betaE, bare, seed * 9 + 2 + 2 * it)
-        m_O = neutral(betaO, bare, seed * 9 + 3 + 2 * it)
+        m_F, m_O = neutral_pair(bare, it)
         bare = ops.belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem)
     return dict(m_i=m_i, m_F=m_F, m_O=m_O, cos_i=cos_i, bare=bare, ion_yield=ion_yield)
 

@@ -226,6 +226,19 @@ def trace_neutral_rr(bvh, box, src, bare, beta, n, seed, out=None):
     return fl
 
 
+def trace_neutral_rr_pair(bvh, box, src, bare, beta_a, n_a, seed_a, beta_b, n_b, seed_b):
+    """Two Russian-roulette neutral species over the same surface and coverage in one launch;
+    each output equals trace_neutral_rr with the same (beta, n, seed)."""
+    import ctypes
+    fa = torch.empty((bvh.nf,), dtype=f32, device=bvh.device)
+    fb = torch.empty((bvh.nf,), dtype=f32, device=bvh.device)
+    bvh.ctx.call("petch_trace_neutral_rr_pair", bvh.h, ctypes.byref(box), ctypes.byref(src),
+                 ptr(bare, f32, "bare"), float(beta_a), int(n_a), int(seed_a) & 0xFFFFFFFFFFFFFFFF,
+                 ptr(fa, f32), float(beta_b), int(n_b), int(seed_b) & 0xFFFFFFFFFFFFFFFF,
+                 ptr(fb, f32))
+    return fa, fb
+
+
 def trace_ion_yield(bvh, box, src, ion, n, seed, out=None):
     """Faithful ion accumulators (5, F): sp, ie, ip, arrival, weighted cos (threed.py:599-712)."""
     import ctypes

@@ -79,6 +79,32 @@ def test_neutral_rr_parity(name):
 
 
 @pytest.mark.parametrize("name", ["hole", "trench"])
+def test_neutral_rr_pair_equals_two_launches(name):
+    """The paired F+O launch carries each ray on the stream it has in a single launch, so each
+    output equals the single launch's up to the order of the float atomic adds; the oracle's
+    single-species trace with the same seed pins the pair's second species too."""
+    import torch
+    from petch import ops
+    g, v, f, bvh, box, obox, obvh, dev = _setup(name)
+    bare = np.random.default_rng(5).uniform(0.2, 1.0, len(f)).astype(np.float32)
+    tb = torch.from_numpy(bare).to(dev)
+    src = ops.make_source(0)
+    n_a, n_b = 40000, 25000
+    fa, fb = ops.trace_neutral_rr_pair(bvh, box, src, tb, 0.7, n_a, 11, 1.0, n_b, 12)
+    sa = ops.trace_neutral_rr(bvh, box, src, tb, 0.7, n_a, 11).cpu().numpy()
+    sb = ops.trace_neutral_rr(bvh, box, src, tb, 1.0, n_b, 12).cpu().numpy()
+    fa, fb = fa.cpu().numpy(), fb.cpu().numpy()
+    np.testing.assert_allclose(fa, sa, rtol=2e-5, atol=1e-3)
+    np.testing.assert_allclose(fb, sb, rtol=2e-5, atol=1e-3)
+    ob = obvh.trace_neutral_rr(obox, on.Source(0, 0.0, 0.0), bare, 1.0, n_b, 12)
+    rel = np.abs(fb - ob).sum() / np.abs(ob).sum()
+    print(f"{name}: pair species b vs oracle same-seed L1 rel {rel:.2e}", flush=True)
+    assert rel < 0.03
+    with pytest.raises(Exception):
+        ops.trace_neutral_rr_pair(bvh, box, src, tb, 0.7, -1, 11, 1.0, n_b, 12)
+
+
+@pytest.mark.parametrize("name", ["hole", "trench"])
 def test_ion_yield_parity(name):
     from petch import ops
     g, v, f, bvh, box, obox, obvh, dev = _setup(name)
