@@ -33,7 +33,9 @@ struct __align__(16) BvhNode {
 };
 
 // Four children's boxes, one child per float4 lane, and their references: >= 0 a wide node
-// (indexed like its binary node), < 0 a leaf ~slot, kWideEmpty an unused lane.
+// (indexed like its binary node), < 0 a leaf ~slot, kWideEmpty an unused lane. An unused lane's
+// box has every bound at +inf: each slab then gives +inf (or -inf) at both planes, so the slab
+// test misses it for every ray without looking at the reference.
 struct __align__(16) BvhWide {
   float4 lox, hix, loy, hiy, loz, hiz;
   int4 ref;
@@ -257,15 +259,25 @@ __device__ __forceinline__ bool trav_step(const BvhView& b, Trav& T, Stack& stk)
   const float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
   const int4 ref = N->ref;
   int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
-  float t0 = c0 != kWideEmpty ? ray_box(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, h.t) : kMiss;
-  float t1 = c1 != kWideEmpty ? ray_box(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, h.t) : kMiss;
-  float t2 = c2 != kWideEmpty ? ray_box(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, h.t) : kMiss;
-  float t3 = c3 != kWideEmpty ? ray_box(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, h.t) : kMiss;
-  // leaves are tested on the spot
-  if (t0 < kHitLim && c0 < 0) { leaf_test(b, r, ~c0, h); t0 = kMiss; }
-  if (t1 < kHitLim && c1 < 0) { leaf_test(b, r, ~c1, h); t1 = kMiss; }
-  if (t2 < kHitLim && c2 < 0) { leaf_test(b, r, ~c2, h); t2 = kMiss; }
-  if (t3 < kHitLim && c3 < 0) { leaf_test(b, r, ~c3, h); t3 = kMiss; }
+  // unused lanes carry a box at +inf that every ray misses, so all four tests run branch-free
+  // and the node's seven loads issue together
+  float t0 = ray_box(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, h.t);
+  float t1 = ray_box(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, h.t);
+  float t2 = ray_box(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, h.t);
+  float t3 = ray_box(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, h.t);
+  // leaves are tested on the spot, in child order, as one loop over the lane's hit leaves (the
+  // wavefront runs as many triangle tests as its busiest lane needs, not one per child slot)
+  unsigned lm = (t0 < kHitLim && c0 < 0 ? 1u : 0u) | (t1 < kHitLim && c1 < 0 ? 2u : 0u) |
+                (t2 < kHitLim && c2 < 0 ? 4u : 0u) | (t3 < kHitLim && c3 < 0 ? 8u : 0u);
+  if (lm & 1u) t0 = kMiss;
+  if (lm & 2u) t1 = kMiss;
+  if (lm & 4u) t2 = kMiss;
+  if (lm & 8u) t3 = kMiss;
+  while (lm) {
+    const int c = (lm & 1u) ? c0 : (lm & 2u) ? c1 : (lm & 4u) ? c2 : c3;
+    lm &= lm - 1u;
+    leaf_test(b, r, ~c, h);
+  }
   // inner children that now start beyond the best hit are dropped
   const float lim = h.t * kBoxSlack;
   if (!(t0 <= lim)) t0 = kMiss;

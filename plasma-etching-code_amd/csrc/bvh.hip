@@ -273,8 +273,8 @@ __global__ void k_bvh_wide(const BvhNode* __restrict__ nodes, const int* __restr
   }
   for (; m < 4; ++m) {
     for (int k = 0; k < 3; ++k) {
-      lo[m][k] = 0.f;
-      hi[m][k] = 0.f;
+      lo[m][k] = INFINITY;
+      hi[m][k] = INFINITY;
     }
     ref[m] = petch::kWideEmpty;
   }

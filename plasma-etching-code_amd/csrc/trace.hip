@@ -393,18 +393,17 @@ struct IonYield {
   }
 };
 
-// 5 waves per SIMD: the register budget that still leaves the loop without spills
-#define PETCH_TRACE_BOUNDS __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5)))
-
-__global__ void PETCH_TRACE_BOUNDS k_trace_neutral_rr(BvhView bv, Box b, Source src, NeutralRR pol,
-                                                      RayQueue q,
-                                                      unsigned long long* __restrict__ hits) {
+// The neutral loop fits 96 registers without spills, which gives 5 waves per SIMD; the ion
+// loop needs more and runs best at the compiler's own choice (4 waves per SIMD), measured.
+__global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5)))
+k_trace_neutral_rr(BvhView bv, Box b, Source src, NeutralRR pol, RayQueue q,
+                   unsigned long long* __restrict__ hits) {
   persistent_trace(bv, b, src, pol, q, hits);
 }
 
-__global__ void PETCH_TRACE_BOUNDS k_trace_ion_yield(BvhView bv, Box b, Source src, IonYield pol,
-                                                     RayQueue q,
-                                                     unsigned long long* __restrict__ hits) {
+__global__ void __launch_bounds__(kBS)
+k_trace_ion_yield(BvhView bv, Box b, Source src, IonYield pol, RayQueue q,
+                  unsigned long long* __restrict__ hits) {
   persistent_trace(bv, b, src, pol, q, hits);
 }
 
