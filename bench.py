@@ -7,13 +7,14 @@ iterations of 8M F + 8M O Russian-roulette neutrals (88M primary rays per step) 
 velocity extension -> CFL substeps of upwind advection -> FSM reinit. A step is one such pass;
 the surface keeps etching from step to step.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--no-small-hole]
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--no-small-hole] [--no-batch]
 
 Prints one JSON line: steps/s (value), rays/s, ray-hits/s, the roofline of the dominant kernel
 (the neutral trace) measured with HIP events on the launch stream, the CPU baseline (the
 oracle's C/OpenMP restatement timed on a bounded sample of the same workload), and beside them
 the wall time of the reference's own small-hole loop (scripts/vps_sweep.py) against its
-published RTX 3090 figure.
+published RTX 3090 figure, and the batched configuration (64 features in lockstep, one launch
+per trace) against the same features run one after the other.
 """
 import argparse
 import json
@@ -180,6 +181,61 @@ def small_hole():
             "speedup_vs_reference": SMALL_HOLE_REF_WALL_S / w}
 
 
+def batch_features(n_feat, seed=1234):
+    """Seeded features of the batched configuration: holes and trenches alternating, diameter
+    U(2,10) um, mask U(1,3) um, dx 0.25, square lateral extent U(12,40) um (48^2..160^2
+    cells), height U(20,50) um (80..200 cells), ion energy U(50,200) eV, O/F flux ratio
+    U(0.1,1.0); 400k rays per feature and step split 1:1:1 (ion : F : O)."""
+    import numpy as np
+    from petch.api import SF6O2
+    from petch import Flags
+    out = []
+    for f in range(n_feat):
+        r = np.random.default_rng(seed + f)
+        d, mask = r.uniform(2, 10), r.uniform(1, 3)
+        ext = max(round(r.uniform(12, 40) * 4) / 4, round((d + 4) * 4) / 4)
+        Lz = round(r.uniform(20, 50) * 4) / 4
+        m = SF6O2(meanE=float(r.uniform(50, 200)))
+        par = dict(m.par)
+        par['Oflux'] = float(r.uniform(0.1, 1.0)) * par['Fflux']
+        hole = f % 2 == 0
+        par['periodic_y'] = 0 if hole else 1
+        out.append(dict(Lx=ext, Ly=ext, Lz=Lz, dx=0.25, trench_width=d, mask_th=mask,
+                        sub_top=Lz - mask - 2.0, hole=hole, par=par, flags=Flags(**m.flags),
+                        n_ion=133334, n_neu=133333, seed_offset=f))
+    return out
+
+
+def batched(n_feat=64, n_steps=5):
+    """The batched configuration on its first n_steps steps (dt of a 40-step, 3 s run): all
+    features in lockstep with one launch per trace (petch.batch) against the same features run
+    one after the other through run_etch_3d. Same streams, so the depths must agree."""
+    import torch
+    from petch import batch as pb
+    from petch import threed as t3
+    feats = batch_features(n_feat)
+    t_end = 3.0 * n_steps / 40.0
+    pb.run_etch_3d_batch(batch_features(4, seed=99), t_end=t_end, n_steps=2)      # warm-up
+    torch.cuda.synchronize()
+    t0 = time.time()
+    gb = pb.run_etch_3d_batch(feats, t_end=t_end, n_steps=n_steps)
+    torch.cuda.synchronize()
+    wb = time.time() - t0
+    t0 = time.time()
+    gs = [t3.run_etch_3d(**ft, t_end=t_end, n_steps=n_steps, reinit_method="fsm", verbose=False)
+          for ft in feats]
+    torch.cuda.synchronize()
+    ws = time.time() - t0
+    dd = max(abs(t3.max_depth_3d(a) - t3.max_depth_3d(b)) for a, b in zip(gb, gs))
+    per40 = 40.0 / n_steps
+    return {"workload": f"{n_feat} seeded holes/trenches (d U(2,10) um, dx 0.25, 48^2..160^2 x "
+                        f"80..200 cells, 400k rays/feature/step), first {n_steps} of 40 steps",
+            "batched_wall_s": wb, "sequential_wall_s": ws,
+            "features_per_s_batched": n_feat / (wb * per40),
+            "features_per_s_sequential": n_feat / (ws * per40),
+            "speedup": ws / wb, "max_depth_diff_um": dd}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +243,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small-hole", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run plain device copies of 256 MiB (reference for PMC byte counts)")
     a = ap.parse_args(argv)
@@ -272,6 +329,8 @@ def main(argv=None):
     }
     if not a.no_small_hole:
         res["small_hole"] = small_hole()
+    if not a.no_batch:
+        res["batch"] = batched()
     if not a.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline()

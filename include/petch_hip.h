@@ -188,8 +188,48 @@ int petch_trace_legacy(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* bo
                        const petch_source* src, float sticking, int n_reemit, int specular,
                        float cos_thr, float eta, int n, uint64_t seed, float* flux,
                        float* angacc);
-/* Ray-hit counters accumulated by petch_trace_neutral_rr[_pair] (counts_host[0]) and
- * petch_trace_ion_yield (counts_host[1]) on this context since the last reset (work accounting
+/* ---------------------------------------------------------------- batches of features */
+/* Many features (each its own surface, box, coverage and recipe) traced from one ray queue in
+ * one launch; feature f's rays carry exactly the streams a single-feature call with the same
+ * seeds would give them, so each feature's outputs equal that call's up to the order of the
+ * float atomic adds. No reference counterpart: the reference traces one feature per call. */
+typedef struct petch_neutral_feature {
+  const petch_bvh* bvh;
+  petch_box box;
+  const float* bare;            /* (nf,) coverage of this feature's faces */
+  float beta_a;                 /* species a (F): sticking factor, rays, stream, output */
+  int n_a;
+  uint64_t seed_a;
+  float* fl_a;
+  float beta_b;                 /* species b (O) */
+  int n_b;
+  uint64_t seed_b;
+  float* fl_b;
+} petch_neutral_feature;
+
+typedef struct petch_ion_feature {
+  const petch_bvh* bvh;
+  petch_box box;
+  petch_source src;
+  petch_ion_params ip;
+  int n;
+  uint64_t seed;
+  float* sp;
+  float* ie;
+  float* ipass;
+  float* arr;
+  float* ang;
+} petch_ion_feature;
+
+/* petch_trace_neutral_rr_pair for every feature of the batch, in one launch (src: the cosine
+ * neutral source shared by all features). Outputs are overwritten. */
+int petch_trace_neutral_rr_batch(petch_ctx* ctx, const petch_source* src, int nfeat,
+                                 const petch_neutral_feature* feats);
+/* petch_trace_ion_yield for every feature of the batch, in one launch. */
+int petch_trace_ion_yield_batch(petch_ctx* ctx, int nfeat, const petch_ion_feature* feats);
+
+/* Ray-hit counters accumulated by petch_trace_neutral_rr[_pair|_batch] (counts_host[0]) and
+ * petch_trace_ion_yield[_batch] (counts_host[1]) on this context since the last reset (work accounting
  * for throughput reports; no reference counterpart). counts_host may be NULL; reset != 0 zeroes
  * them after the read. Synchronises when reading. */
 int petch_trace_hit_counts(petch_ctx* ctx, int reset, unsigned long long* counts_host);

@@ -29,6 +29,7 @@ enum PetchScratch {
   SCR_RAD_B,
   SCR_RAD_C,
   SCR_SMOOTH,        // flux smoothing ping-pong
+  SCR_TABLE,         // per-launch descriptor tables (batched traces)
   SCR_COUNT
 };
 
@@ -40,6 +41,12 @@ struct petch_ctx {
   size_t scr_cap[SCR_COUNT];
   void* pinned;            // small pinned host staging area for counts / scalars
   size_t pinned_cap;
+  // ring of pinned staging buffers for descriptor tables: slot i is refilled only after the
+  // copy that read it last (event i) has run, so the host never overwrites data in flight
+  void* ring[8];
+  size_t ring_cap[8];
+  hipEvent_t ring_ev[8];
+  int ring_next;
 };
 
 struct petch_bvh {
@@ -55,6 +62,11 @@ struct petch_bvh {
 };
 
 int petch_set_error(petch_ctx* ctx, int code, const char* fmt, ...);
+
+// Copies `bytes` of host data into the SCR_TABLE device buffer on ctx->stream (stream-ordered:
+// kernels launched afterwards on the stream see it; a later upload waits for them in stream
+// order) and returns the device address, or NULL on failure.
+void* petch_upload_table(petch_ctx* ctx, const void* host, size_t bytes);
 
 // Returns a device buffer of at least `bytes` for slot `s`, growing it if needed (NULL on failure
 // with the error recorded in ctx).

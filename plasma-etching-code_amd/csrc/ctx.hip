@@ -2,6 +2,7 @@
 #include "common.h"
 
 #include <stdlib.h>
+#include <string.h>
 
 int petch_set_error(petch_ctx* ctx, int code, const char* fmt, ...) {
   if (ctx) {
@@ -59,6 +60,42 @@ void* petch_pinned(petch_ctx* ctx, size_t bytes) {
   return p;
 }
 
+void* petch_upload_table(petch_ctx* ctx, const void* host, size_t bytes) {
+  void* dev = petch_scratch(ctx, SCR_TABLE, bytes);
+  if (!dev) return NULL;
+  const int i = ctx->ring_next;
+  ctx->ring_next = (i + 1) % 8;
+  if (ctx->ring_ev[i] && hipEventSynchronize(ctx->ring_ev[i]) != hipSuccess) {
+    petch_set_error(ctx, PETCH_E_HIP, "upload_table: event wait failed");
+    return NULL;
+  }
+  if (ctx->ring_cap[i] < bytes) {
+    if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
+    ctx->ring[i] = NULL;
+    ctx->ring_cap[i] = 0;
+    size_t want = bytes < 16384 ? 16384 : bytes;
+    if (hipHostMalloc(&ctx->ring[i], want, hipHostMallocDefault) != hipSuccess) {
+      ctx->ring[i] = NULL;
+      petch_set_error(ctx, PETCH_E_HIP, "hipHostMalloc(%zu) failed", want);
+      return NULL;
+    }
+    ctx->ring_cap[i] = want;
+  }
+  if (!ctx->ring_ev[i] &&
+      hipEventCreateWithFlags(&ctx->ring_ev[i], hipEventDisableTiming) != hipSuccess) {
+    ctx->ring_ev[i] = NULL;
+    petch_set_error(ctx, PETCH_E_HIP, "upload_table: hipEventCreate failed");
+    return NULL;
+  }
+  memcpy(ctx->ring[i], host, bytes);
+  if (hipMemcpyAsync(dev, ctx->ring[i], bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipEventRecord(ctx->ring_ev[i], ctx->stream) != hipSuccess) {
+    petch_set_error(ctx, PETCH_E_HIP, "upload_table: copy failed");
+    return NULL;
+  }
+  return dev;
+}
+
 extern "C" int petch_ctx_create(int device, petch_ctx** out) {
   if (!out) return PETCH_E_ARG;
   *out = NULL;
@@ -80,6 +117,10 @@ extern "C" int petch_ctx_destroy(petch_ctx* ctx) {
   for (int s = 0; s < SCR_COUNT; ++s)
     if (ctx->scr[s]) (void)hipFree(ctx->scr[s]);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (int i = 0; i < 8; ++i) {
+    if (ctx->ring[i]) (void)hipHostFree(ctx->ring[i]);
+    if (ctx->ring_ev[i]) (void)hipEventDestroy(ctx->ring_ev[i]);
+  }
   free(ctx);
   return PETCH_OK;
 }

@@ -16,6 +16,8 @@
 #include "bvh.h"
 #include "rng.h"
 
+#include <vector>
+
 using petch::BvhView;
 using petch::Hit;
 using petch::Rng;
@@ -179,18 +181,17 @@ __device__ __forceinline__ bool claim_ray(const RayQueue& q, bool active, bool& 
 // restarts the query, otherwise the ray escapes and the lane goes idle. Hits are handled in
 // batches: only when at most kRefillAt lanes of the wavefront are still traversing does it run
 // the policy's hit code and refill idle lanes from the ray queue, so the (short, very uneven)
-// queries keep most lanes busy. The policy P supplies the per-ray payload: start() keys the
-// ray's Philox stream and draws its source ray, hit() deposits and returns the next ray (true)
-// or ends it (false). Per ray, the Philox draws happen in the same order whatever the schedule,
+// queries keep most lanes busy. The policy P supplies the surface and the per-ray payload:
+// view() / box() give the BVH and tracer box the ray lives in, start() keys the ray's Philox
+// stream and draws its source ray, hit() deposits and returns the next ray (true) or ends it
+// (false). Per ray, the Philox draws happen in the same order whatever the schedule,
 // so the result does not depend on it.
 constexpr int kRefillAt = 24;
 constexpr int kLdsStack = 8;  // traversal stack entries per lane held in LDS (16 KB/workgroup)
 enum LaneState { kIdle = 0, kTrav = 1, kHit = 2 };
 
 template <class P>
-__device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b,
-                                                 const Source& src, const P& pol,
-                                                 const RayQueue& q,
+__device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q,
                                                  unsigned long long* __restrict__ hits) {
   __shared__ int lds_n[kLdsStack * kBS];
   __shared__ float lds_t[kLdsStack * kBS];
@@ -210,10 +211,10 @@ __device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b
     if (__popcll(__ballot(st == kTrav)) <= kRefillAt) {
       if (st == kHit) {
         ++nh;
-        if (pol.hit(rng, R, bv, T.h, o, d)) {
+        if (pol.hit(rng, R, T.h, o, d)) {
           rp = petch::ray_pre(o, d);
           wraps = 0;
-          petch::trav_begin(bv, T, rp, 1.0e6f);
+          petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
           st = kTrav;
         } else {
           st = kIdle;
@@ -221,21 +222,22 @@ __device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b
       }
       unsigned ray;
       if (claim_ray(q, st != kIdle, exhausted, ray)) {
-        pol.start(rng, R, ray, b, src, o, d);
+        pol.start(rng, R, ray, o, d);
         rp = petch::ray_pre(o, d);
         wraps = 0;
-        petch::trav_begin(bv, T, rp, 1.0e6f);
+        petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
         st = kTrav;
       }
       if (!__ballot(st == kTrav)) break;
     }
-    if (st == kTrav && petch::trav_step(bv, T, stk)) {
+    if (st == kTrav && petch::trav_step(pol.view(R), T, stk)) {
+      const Box& b = pol.box(R);
       if (T.h.face >= 0) {
         st = kHit;
       } else if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
         rp.o = o;
         ++wraps;
-        petch::trav_begin(bv, T, rp, 1.0e6f);
+        petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
       } else {
         st = kIdle;
       }
@@ -249,7 +251,33 @@ __device__ __forceinline__ void persistent_trace(const BvhView& bv, const Box& b
 // can carry two species over the same surface (the F and O traces of one fixed-point
 // iteration, threed.py:876-880): queue ids [0, n0) are species 0 with stream (seed[0], id),
 // the rest species 1 with stream (seed[1], id - n0), exactly the streams of two launches.
+// One Russian-roulette bounce (threed.py:565-596) of a ray with weight w that hit face h:
+// deposits w, applies the sticking S, plays the roulette; returns false when the ray ends, else
+// re-emits it about the oriented normal.
+__device__ __forceinline__ bool rr_bounce(Rng& r, float& w, int& bounce, const BvhView& bv,
+                                          const Hit& h, float* __restrict__ fl, float S, Vec3& o,
+                                          Vec3& d) {
+  Vec3 nn = hit_normal(bv, h.slot);
+  if (petch::dot(nn, d) > 0.f) nn = -nn;
+  deposit(fl, h.face, w);
+  float w2 = w - w * S;
+  if (w2 <= 1.0e-6f) return false;
+  if (w2 < 0.1f) {
+    if (r.uniform() < (1.f - w2 / 0.3f)) return false;
+    w2 = 0.3f;
+  }
+  if (++bounce >= 1024) return false;
+  w = w2;
+  const Vec3 p = o + h.t * d;
+  d = cosine_about(r, nn);
+  o = p + 1.0e-4f * nn;
+  return true;
+}
+
 struct NeutralRR {
+  BvhView bv;
+  Box b;
+  Source src;
   const float* __restrict__ bare;
   float beta[2];
   float* fl[2];
@@ -260,33 +288,18 @@ struct NeutralRR {
     int bounce;
     int sp;
   };
-  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, const Box& b,
-                                        const Source& src, Vec3& o, Vec3& d) const {
+  __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
+  __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
+  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, Vec3& o, Vec3& d) const {
     R.sp = ray >= n0;
     r.init(R.sp ? seed[1] : seed[0], R.sp ? ray - n0 : ray);
     gen_source(r, b, src, o, d);
     R.w = 1.f;
     R.bounce = 0;
   }
-  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const BvhView& bv, const Hit& h, Vec3& o,
-                                      Vec3& d) const {
-    Vec3 nn = hit_normal(bv, h.slot);
-    if (petch::dot(nn, d) > 0.f) nn = -nn;
-    deposit(R.sp ? fl[1] : fl[0], h.face, R.w);
-    float w = R.w;
+  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const Hit& h, Vec3& o, Vec3& d) const {
     const float S = bare[h.face] * (R.sp ? beta[1] : beta[0]);
-    w = w - w * S;
-    if (w <= 1.0e-6f) return false;
-    if (w < 0.1f) {
-      if (r.uniform() < (1.f - w / 0.3f)) return false;
-      w = 0.3f;
-    }
-    if (++R.bounce >= 1024) return false;
-    R.w = w;
-    const Vec3 p = o + h.t * d;
-    d = cosine_about(r, nn);
-    o = p + 1.0e-4f * nn;
-    return true;
+    return rr_bounce(r, R.w, R.bounce, bv, h, R.sp ? fl[1] : fl[0], S, o, d);
   }
 };
 
@@ -299,21 +312,100 @@ struct IonParams {
 // deposits the sputter / ion-enhanced / passivation yields, the arrival weight and the weighted
 // cosine; sticking ramps from 1 below thetaRMin to 0 at 90 deg; energy loss by the inflection
 // law; coned-cosine re-emission about the specular direction.
+// One faithful-ion bounce (threed.py:625-712) of an ion with energy Ein and weight win that hit
+// face h: deposits the five yields, then sticks, or loses energy and re-emits about the
+// specular direction (returns true with the next ray in o, d). The constants come from F:
+// F.P (IonParams), F.sE (square roots of the three threshold energies) and F.out (sp, ie, ip,
+// arr, ang accumulators).
+template <class Fc>
+__device__ __forceinline__ bool ion_bounce(Rng& r, float& Ein, float& win, int& bounce,
+                                           const BvhView& bv, const Hit& h, const Fc& F, Vec3& o,
+                                           Vec3& d) {
+  const IonParams& P = F.P;
+  Vec3 nn = hit_normal(bv, h.slot);
+  if (petch::dot(nn, d) > 0.f) nn = -nn;
+  const float E = Ein, w = win;
+  const float cosT = fminf(fmaxf(-petch::dot(d, nn), 0.f), 1.f);
+  const float incAng = acosf(cosT);
+  const float sqrtE = sqrtf(E);
+  const float f_sp = fmaxf((1.f + P.B_sp * (1.f - cosT * cosT)) * cosT, 0.f);
+  float f_ie = 1.f;
+  if (cosT < 0.5f) f_ie = fmaxf(3.f - 6.f * incAng / kPi, 0.f);
+  deposit(F.out[0], h.face, P.A_sp * fmaxf(sqrtE - F.sE[0], 0.f) * f_sp * w);
+  deposit(F.out[1], h.face, P.A_ie * fmaxf(sqrtE - F.sE[1], 0.f) * f_ie * w);
+  deposit(F.out[2], h.face, P.A_p * fmaxf(sqrtE - F.sE[2], 0.f) * f_ie * w);
+  deposit(F.out[3], h.face, w);
+  deposit(F.out[4], h.face, cosT * w);
+  float stick = 1.f;
+  if (incAng > P.thrmin) {
+    float rr = (incAng - P.thrmin) / (P.thrmax - P.thrmin);
+    rr = fminf(fmaxf(rr, 0.f), 1.f);
+    stick = 1.f - rr;
+  }
+  if (stick >= 1.f) return false;
+  float Eref;
+  if (incAng >= P.inflect)
+    Eref = 1.f - (1.f - P.A_en) * (kPi2 - incAng) / (kPi2 - P.inflect);
+  else
+    Eref = P.A_en * powf(incAng / P.inflect, P.n_l);
+  float newE = Eref * E;
+  for (int e = 0; e < 4; ++e) {
+    float cand = Eref * E + 0.1f * E * r.normal();
+    if (cand >= 0.f && cand <= E) {
+      newE = cand;
+      break;
+    }
+  }
+  newE = fminf(fmaxf(newE, 0.f), E);
+  if (newE <= P.minE) return false;
+  Ein = newE;
+  const float w2 = w - w * stick;
+  if (w2 <= 1.0e-4f || ++bounce >= 1024) return false;
+  win = w2;
+  const Vec3 p = o + h.t * d;
+  const Vec3 axis = petch::normalize(d - (2.f * petch::dot(d, nn)) * nn);
+  Vec3 tt, bb;
+  if (axis.z < -0.9999999f) {
+    tt = v3(0.f, -1.f, 0.f);
+    bb = v3(-1.f, 0.f, 0.f);
+  } else {
+    float aa = 1.f / (1.f + axis.z);
+    float cc = -axis.x * axis.y * aa;
+    tt = v3(1.f - axis.x * axis.x * aa, cc, -axis.x);
+    bb = v3(cc, 1.f - axis.y * axis.y * aa, -axis.y);
+  }
+  const float maxcone = kPi2 - fminf(incAng, P.minang);
+  float theta = 0.f;
+  for (int t = 0; t < 64; ++t) {
+    float uu = sqrtf(r.uniform());
+    float ss = sqrtf(fmaxf(1.f - uu, 0.f));
+    theta = maxcone * ss;
+    if (r.uniform() * theta * uu <= cosf(kPi2 * ss) * sinf(theta)) break;
+  }
+  const float phi = kTwoPi * r.uniform();
+  Vec3 nd = petch::normalize(sinf(theta) * (cosf(phi) * tt + sinf(phi) * bb) + cosf(theta) * axis);
+  const float dn = petch::dot(nd, nn);
+  if (dn <= 0.f) nd = nd - (2.f * dn) * nn;
+  d = nd;
+  o = p + 1.0e-4f * nn;
+  return true;
+}
+
 struct IonYield {
+  BvhView bv;
+  Box b;
+  Source src;
   IonParams P;
-  float sEsp, sEie, sEp;
-  float* __restrict__ sp;
-  float* __restrict__ ie;
-  float* __restrict__ ip;
-  float* __restrict__ arr;
-  float* __restrict__ ang;
+  float sE[3];
+  float* out[5];
   uint64_t seed;
   struct Ray {
     float E, w;
     int bounce;
   };
-  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, const Box& b,
-                                        const Source& src, Vec3& o, Vec3& d) const {
+  __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
+  __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
+  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, Vec3& o, Vec3& d) const {
     r.init(seed, ray);
     gen_source(r, b, src, o, d);
     float E = P.meanE + P.sigmaE * r.normal();
@@ -321,90 +413,149 @@ struct IonYield {
     R.w = 1.f;
     R.bounce = 0;
   }
-  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const BvhView& bv, const Hit& h, Vec3& o,
-                                      Vec3& d) const {
-    Vec3 nn = hit_normal(bv, h.slot);
-    if (petch::dot(nn, d) > 0.f) nn = -nn;
-    const float E = R.E, w = R.w;
-    const float cosT = fminf(fmaxf(-petch::dot(d, nn), 0.f), 1.f);
-    const float incAng = acosf(cosT);
-    const float sqrtE = sqrtf(E);
-    const float f_sp = fmaxf((1.f + P.B_sp * (1.f - cosT * cosT)) * cosT, 0.f);
-    float f_ie = 1.f;
-    if (cosT < 0.5f) f_ie = fmaxf(3.f - 6.f * incAng / kPi, 0.f);
-    deposit(sp, h.face, P.A_sp * fmaxf(sqrtE - sEsp, 0.f) * f_sp * w);
-    deposit(ie, h.face, P.A_ie * fmaxf(sqrtE - sEie, 0.f) * f_ie * w);
-    deposit(ip, h.face, P.A_p * fmaxf(sqrtE - sEp, 0.f) * f_ie * w);
-    deposit(arr, h.face, w);
-    deposit(ang, h.face, cosT * w);
-    float stick = 1.f;
-    if (incAng > P.thrmin) {
-      float rr = (incAng - P.thrmin) / (P.thrmax - P.thrmin);
-      rr = fminf(fmaxf(rr, 0.f), 1.f);
-      stick = 1.f - rr;
-    }
-    if (stick >= 1.f) return false;
-    float Eref;
-    if (incAng >= P.inflect)
-      Eref = 1.f - (1.f - P.A_en) * (kPi2 - incAng) / (kPi2 - P.inflect);
-    else
-      Eref = P.A_en * powf(incAng / P.inflect, P.n_l);
-    float newE = Eref * E;
-    for (int e = 0; e < 4; ++e) {
-      float cand = Eref * E + 0.1f * E * r.normal();
-      if (cand >= 0.f && cand <= E) {
-        newE = cand;
-        break;
-      }
-    }
-    newE = fminf(fmaxf(newE, 0.f), E);
-    if (newE <= P.minE) return false;
-    R.E = newE;
-    const float w2 = w - w * stick;
-    if (w2 <= 1.0e-4f || ++R.bounce >= 1024) return false;
-    R.w = w2;
-    const Vec3 p = o + h.t * d;
-    const Vec3 axis = petch::normalize(d - (2.f * petch::dot(d, nn)) * nn);
-    Vec3 tt, bb;
-    if (axis.z < -0.9999999f) {
-      tt = v3(0.f, -1.f, 0.f);
-      bb = v3(-1.f, 0.f, 0.f);
-    } else {
-      float aa = 1.f / (1.f + axis.z);
-      float cc = -axis.x * axis.y * aa;
-      tt = v3(1.f - axis.x * axis.x * aa, cc, -axis.x);
-      bb = v3(cc, 1.f - axis.y * axis.y * aa, -axis.y);
-    }
-    const float maxcone = kPi2 - fminf(incAng, P.minang);
-    float theta = 0.f;
-    for (int t = 0; t < 64; ++t) {
-      float uu = sqrtf(r.uniform());
-      float ss = sqrtf(fmaxf(1.f - uu, 0.f));
-      theta = maxcone * ss;
-      if (r.uniform() * theta * uu <= cosf(kPi2 * ss) * sinf(theta)) break;
-    }
-    const float phi = kTwoPi * r.uniform();
-    Vec3 nd = petch::normalize(sinf(theta) * (cosf(phi) * tt + sinf(phi) * bb) + cosf(theta) * axis);
-    const float dn = petch::dot(nd, nn);
-    if (dn <= 0.f) nd = nd - (2.f * dn) * nn;
-    d = nd;
-    o = p + 1.0e-4f * nn;
-    return true;
+  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const Hit& h, Vec3& o, Vec3& d) const {
+    return ion_bounce(r, R.E, R.w, R.bounce, bv, h, *this, o, d);
   }
 };
+
+// ---------------------------------------------------------------- batches of features
+// A batched launch traces many features (each its own surface, box, coverage and recipe) from
+// one ray queue: feature f owns queue ids [end[f-1], end[f]). A lane keeps its ray's feature
+// index and the feature's BVH view and box in registers; the per-hit constants are read from
+// the feature's descriptor (a few hundred bytes, cache resident).
+struct NeutralFeature {
+  BvhView bv;
+  Box b;
+  const float* bare;
+  float beta[2];
+  float* fl[2];
+  uint64_t seed[2];
+  unsigned n0;   // rays of species 0; the feature's remaining rays are species 1
+  unsigned end;  // exclusive end of the feature's ids in the batch queue
+};
+
+struct IonFeature {
+  BvhView bv;
+  Box b;
+  Source src;
+  IonParams P;
+  float sE[3];
+  float* out[5];
+  uint64_t seed;
+  unsigned end;
+};
+
+// first feature whose id range ends after `ray`
+template <class Fd>
+__device__ __forceinline__ int locate_feature(const Fd* __restrict__ feat, int nfeat,
+                                              unsigned ray) {
+  int lo = 0, hi = nfeat - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (feat[mid].end > ray) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+struct NeutralRRBatch {
+  const NeutralFeature* __restrict__ feat;
+  int nfeat;
+  Source src;
+  struct Ray {
+    float w;
+    int bounce;
+    int sp;
+    int f;
+    BvhView bv;
+    Box b;
+  };
+  __device__ __forceinline__ const BvhView& view(const Ray& R) const { return R.bv; }
+  __device__ __forceinline__ const Box& box(const Ray& R) const { return R.b; }
+  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, Vec3& o, Vec3& d) const {
+    R.f = locate_feature(feat, nfeat, ray);
+    const NeutralFeature& F = feat[R.f];
+    const unsigned local = ray - (R.f ? feat[R.f - 1].end : 0u);
+    R.sp = local >= F.n0;
+    r.init(R.sp ? F.seed[1] : F.seed[0], R.sp ? local - F.n0 : local);
+    R.bv = F.bv;
+    R.b = F.b;
+    gen_source(r, R.b, src, o, d);
+    R.w = 1.f;
+    R.bounce = 0;
+  }
+  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const Hit& h, Vec3& o, Vec3& d) const {
+    const NeutralFeature& F = feat[R.f];
+    const float S = F.bare[h.face] * (R.sp ? F.beta[1] : F.beta[0]);
+    return rr_bounce(r, R.w, R.bounce, R.bv, h, R.sp ? F.fl[1] : F.fl[0], S, o, d);
+  }
+};
+
+struct IonYieldBatch {
+  const IonFeature* __restrict__ feat;
+  int nfeat;
+  struct Ray {
+    float E, w;
+    int bounce;
+    int f;
+    BvhView bv;
+    Box b;
+  };
+  __device__ __forceinline__ const BvhView& view(const Ray& R) const { return R.bv; }
+  __device__ __forceinline__ const Box& box(const Ray& R) const { return R.b; }
+  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, Vec3& o, Vec3& d) const {
+    R.f = locate_feature(feat, nfeat, ray);
+    const IonFeature& F = feat[R.f];
+    r.init(F.seed, ray - (R.f ? feat[R.f - 1].end : 0u));
+    R.bv = F.bv;
+    R.b = F.b;
+    gen_source(r, R.b, F.src, o, d);
+    float E = F.P.meanE + F.P.sigmaE * r.normal();
+    R.E = E < 0.f ? 0.f : E;
+    R.w = 1.f;
+    R.bounce = 0;
+  }
+  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const Hit& h, Vec3& o, Vec3& d) const {
+    return ion_bounce(r, R.E, R.w, R.bounce, R.bv, h, feat[R.f], o, d);
+  }
+};
+
+// zero the per-face outputs of every feature of a batch (blockIdx.y = feature)
+__global__ void k_zero_neutral_batch(const NeutralFeature* __restrict__ feat) {
+  const NeutralFeature& F = feat[blockIdx.y];
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < F.bv.nf; i += gridDim.x * kBS) {
+    F.fl[0][i] = 0.f;
+    F.fl[1][i] = 0.f;
+  }
+}
+
+__global__ void k_zero_ion_batch(const IonFeature* __restrict__ feat) {
+  const IonFeature& F = feat[blockIdx.y];
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < F.bv.nf; i += gridDim.x * kBS) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) F.out[k][i] = 0.f;
+  }
+}
 
 // The neutral loop fits 96 registers without spills, which gives 5 waves per SIMD; the ion
 // loop needs more and runs best at the compiler's own choice (4 waves per SIMD), measured.
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5)))
-k_trace_neutral_rr(BvhView bv, Box b, Source src, NeutralRR pol, RayQueue q,
-                   unsigned long long* __restrict__ hits) {
-  persistent_trace(bv, b, src, pol, q, hits);
+k_trace_neutral_rr(NeutralRR pol, RayQueue q, unsigned long long* __restrict__ hits) {
+  persistent_trace(pol, q, hits);
 }
 
 __global__ void __launch_bounds__(kBS)
-k_trace_ion_yield(BvhView bv, Box b, Source src, IonYield pol, RayQueue q,
-                  unsigned long long* __restrict__ hits) {
-  persistent_trace(bv, b, src, pol, q, hits);
+k_trace_ion_yield(IonYield pol, RayQueue q, unsigned long long* __restrict__ hits) {
+  persistent_trace(pol, q, hits);
+}
+
+__global__ void __launch_bounds__(kBS)
+k_trace_neutral_rr_batch(NeutralRRBatch pol, RayQueue q, unsigned long long* __restrict__ hits) {
+  persistent_trace(pol, q, hits);
+}
+
+__global__ void __launch_bounds__(kBS)
+k_trace_ion_yield_batch(IonYieldBatch pol, RayQueue q, unsigned long long* __restrict__ hits) {
+  persistent_trace(pol, q, hits);
 }
 
 __global__ void __launch_bounds__(kBS) k_trace_legacy(BvhView bv, Source src, Box b,
@@ -544,14 +695,12 @@ extern "C" int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petc
 
 namespace {
 
-int launch_neutral(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
-                   const petch_source* src, const NeutralRR& pol, int n) {
+int launch_neutral(petch_ctx* ctx, const NeutralRR& pol, int n) {
   RayQueue q;
   int rc = make_queue(ctx, n, q);
   if (rc) return rc;
   const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr, n);
-  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(grid), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), make_box(box), make_source(src), pol, q,
+  hipLaunchKernelGGL(k_trace_neutral_rr, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
                      petch_hit_counter(ctx, 0));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
@@ -567,8 +716,9 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   int rc = zero(ctx, fl, bvh->nf);
   if (rc) return rc;
   if (n == 0 || bvh->nf == 0) return PETCH_OK;
-  NeutralRR pol{bare, {beta, beta}, {fl, fl}, {seed, seed}, (unsigned)n};
-  return launch_neutral(ctx, bvh, box, src, pol, n);
+  NeutralRR pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, {beta, beta},
+                {fl, fl}, {seed, seed}, (unsigned)n};
+  return launch_neutral(ctx, pol, n);
 }
 
 extern "C" int petch_trace_neutral_rr_pair(petch_ctx* ctx, const petch_bvh* bvh,
@@ -588,8 +738,9 @@ extern "C" int petch_trace_neutral_rr_pair(petch_ctx* ctx, const petch_bvh* bvh,
   if (rc) return rc;
   const int n = n_a + n_b;
   if (n == 0 || bvh->nf == 0) return PETCH_OK;
-  NeutralRR pol{bare, {beta_a, beta_b}, {fl_a, fl_b}, {seed_a, seed_b}, (unsigned)n_a};
-  return launch_neutral(ctx, bvh, box, src, pol, n);
+  NeutralRR pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, {beta_a, beta_b},
+                {fl_a, fl_b}, {seed_a, seed_b}, (unsigned)n_a};
+  return launch_neutral(ctx, pol, n);
 }
 
 extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
@@ -611,11 +762,121 @@ extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const
   RayQueue q;
   int rc2 = make_queue(ctx, n, q);
   if (rc2) return rc2;
-  IonYield pol{P, sqrtf(P.Eth_sp), sqrtf(P.Eth_ie), sqrtf(P.Eth_p), sp, ie, ipass, arr, ang,
+  IonYield pol{petch_bvh_view(bvh), make_box(box), make_source(src), P,
+               {sqrtf(P.Eth_sp), sqrtf(P.Eth_ie), sqrtf(P.Eth_p)}, {sp, ie, ipass, arr, ang},
                seed};
   const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_ion_yield, n);
-  hipLaunchKernelGGL(k_trace_ion_yield, dim3(grid), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), make_box(box), make_source(src), pol, q,
+  hipLaunchKernelGGL(k_trace_ion_yield, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
+                     petch_hit_counter(ctx, 1));
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_trace_neutral_rr_batch(petch_ctx* ctx, const petch_source* src, int nfeat,
+                                            const petch_neutral_feature* feats) {
+  PETCH_ARG(ctx, ctx && src && (nfeat == 0 || feats), "trace_neutral_rr_batch: null pointer");
+  PETCH_ARG(ctx, nfeat >= 0, "trace_neutral_rr_batch: nfeat < 0");
+  if (nfeat == 0) return PETCH_OK;
+  std::vector<NeutralFeature> t(nfeat);
+  long long end = 0;
+  int max_nf = 0;
+  for (int f = 0; f < nfeat; ++f) {
+    const petch_neutral_feature& a = feats[f];
+    PETCH_ARG(ctx, a.bvh, "trace_neutral_rr_batch: feature %d has no BVH", f);
+    // an empty surface has empty outputs (possibly NULL) and takes no rays
+    PETCH_ARG(ctx, a.bvh->nf == 0 || (a.bare && a.fl_a && a.fl_b),
+              "trace_neutral_rr_batch: feature %d has a null pointer", f);
+    PETCH_ARG(ctx, a.bvh->nf == 0 || a.fl_a != a.fl_b,
+              "trace_neutral_rr_batch: feature %d: outputs must differ", f);
+    PETCH_ARG(ctx, a.n_a >= 0 && a.n_b >= 0, "trace_neutral_rr_batch: feature %d: n < 0", f);
+    NeutralFeature& d = t[f];
+    d.bv = petch_bvh_view(a.bvh);
+    d.b = make_box(&a.box);
+    d.bare = a.bare;
+    d.beta[0] = a.beta_a;
+    d.beta[1] = a.beta_b;
+    d.fl[0] = a.fl_a;
+    d.fl[1] = a.fl_b;
+    d.seed[0] = a.seed_a;
+    d.seed[1] = a.seed_b;
+    const long long n = a.bvh->nf > 0 ? (long long)a.n_a + a.n_b : 0;
+    d.n0 = a.bvh->nf > 0 ? (unsigned)a.n_a : 0u;
+    end += n;
+    PETCH_ARG(ctx, end <= 0x7fffffffLL, "trace_neutral_rr_batch: more than 2^31-1 rays");
+    d.end = (unsigned)end;
+    if (a.bvh->nf > max_nf) max_nf = a.bvh->nf;
+  }
+  const NeutralFeature* dev =
+      (const NeutralFeature*)petch_upload_table(ctx, t.data(), t.size() * sizeof(NeutralFeature));
+  if (!dev) return PETCH_E_HIP;
+  if (max_nf > 0) {
+    const unsigned gx = petch_blocks(max_nf, kBS) < 64 ? petch_blocks(max_nf, kBS) : 64;
+    hipLaunchKernelGGL(k_zero_neutral_batch, dim3(gx, nfeat), dim3(kBS), 0, ctx->stream, dev);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  if (end == 0) return PETCH_OK;
+  RayQueue q;
+  int rc = make_queue(ctx, (int)end, q);
+  if (rc) return rc;
+  NeutralRRBatch pol{dev, nfeat, make_source(src)};
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr_batch, (int)end);
+  hipLaunchKernelGGL(k_trace_neutral_rr_batch, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
+                     petch_hit_counter(ctx, 0));
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_trace_ion_yield_batch(petch_ctx* ctx, int nfeat,
+                                           const petch_ion_feature* feats) {
+  PETCH_ARG(ctx, ctx && (nfeat == 0 || feats), "trace_ion_yield_batch: null pointer");
+  PETCH_ARG(ctx, nfeat >= 0, "trace_ion_yield_batch: nfeat < 0");
+  if (nfeat == 0) return PETCH_OK;
+  std::vector<IonFeature> t(nfeat);
+  long long end = 0;
+  int max_nf = 0;
+  for (int f = 0; f < nfeat; ++f) {
+    const petch_ion_feature& a = feats[f];
+    PETCH_ARG(ctx, a.bvh, "trace_ion_yield_batch: feature %d has no BVH", f);
+    PETCH_ARG(ctx, a.bvh->nf == 0 || (a.sp && a.ie && a.ipass && a.arr && a.ang),
+              "trace_ion_yield_batch: feature %d has a null pointer", f);
+    PETCH_ARG(ctx, a.n >= 0, "trace_ion_yield_batch: feature %d: n < 0", f);
+    IonFeature& d = t[f];
+    d.bv = petch_bvh_view(a.bvh);
+    d.b = make_box(&a.box);
+    d.src = make_source(&a.src);
+    const petch_ion_params& ip = a.ip;
+    d.P = IonParams{ip.meanE,  ip.sigmaE, ip.A_sp,   ip.Eth_sp,  ip.B_sp,   ip.A_ie,
+                    ip.Eth_ie, ip.A_p,    ip.Eth_p,  ip.inflect, ip.minang, ip.n_l,
+                    ip.A_en,   ip.minE,   ip.thrmin, ip.thrmax};
+    d.sE[0] = sqrtf(ip.Eth_sp);
+    d.sE[1] = sqrtf(ip.Eth_ie);
+    d.sE[2] = sqrtf(ip.Eth_p);
+    d.out[0] = a.sp;
+    d.out[1] = a.ie;
+    d.out[2] = a.ipass;
+    d.out[3] = a.arr;
+    d.out[4] = a.ang;
+    d.seed = a.seed;
+    end += a.bvh->nf > 0 ? a.n : 0;
+    PETCH_ARG(ctx, end <= 0x7fffffffLL, "trace_ion_yield_batch: more than 2^31-1 rays");
+    d.end = (unsigned)end;
+    if (a.bvh->nf > max_nf) max_nf = a.bvh->nf;
+  }
+  const IonFeature* dev =
+      (const IonFeature*)petch_upload_table(ctx, t.data(), t.size() * sizeof(IonFeature));
+  if (!dev) return PETCH_E_HIP;
+  if (max_nf > 0) {
+    const unsigned gx = petch_blocks(max_nf, kBS) < 64 ? petch_blocks(max_nf, kBS) : 64;
+    hipLaunchKernelGGL(k_zero_ion_batch, dim3(gx, nfeat), dim3(kBS), 0, ctx->stream, dev);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  if (end == 0) return PETCH_OK;
+  RayQueue q;
+  int rc = make_queue(ctx, (int)end, q);
+  if (rc) return rc;
+  IonYieldBatch pol{dev, nfeat};
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_ion_yield_batch, (int)end);
+  hipLaunchKernelGGL(k_trace_ion_yield_batch, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
                      petch_hit_counter(ctx, 1));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;

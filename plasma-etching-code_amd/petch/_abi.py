@@ -73,6 +73,20 @@ class IonParams(ctypes.Structure):
                                        "Eth_ie", "A_p", "Eth_p", "inflect", "minang", "n_l",
                                        "A_en", "minE", "thrmin", "thrmax")]
 
+class NeutralFeature(ctypes.Structure):
+    """petch_neutral_feature: one feature of a batched Russian-roulette neutral launch."""
+    _fields_ = [("bvh", c_void_p), ("box", Box), ("bare", c_void_p),
+                ("beta_a", c_float), ("n_a", c_int), ("seed_a", c_uint64), ("fl_a", c_void_p),
+                ("beta_b", c_float), ("n_b", c_int), ("seed_b", c_uint64), ("fl_b", c_void_p)]
+
+
+class IonFeature(ctypes.Structure):
+    """petch_ion_feature: one feature of a batched faithful-ion launch."""
+    _fields_ = [("bvh", c_void_p), ("box", Box), ("src", Source), ("ip", IonParams),
+                ("n", c_int), ("seed", c_uint64), ("sp", c_void_p), ("ie", c_void_p),
+                ("ipass", c_void_p), ("arr", c_void_p), ("ang", c_void_p)]
+
+
 # name -> argument types (all entry points return int)
 SIGNATURES = {
     "petch_ctx_create": [c_int, POINTER(c_void_p)],
@@ -101,6 +115,8 @@ SIGNATURES = {
                                P],
     "petch_trace_neutral_rr_pair": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int,
                                     c_uint64, P, c_float, c_int, c_uint64, P],
+    "petch_trace_neutral_rr_batch": [P, POINTER(Source), c_int, P],
+    "petch_trace_ion_yield_batch": [P, c_int, P],
     "petch_trace_ion_yield": [P, P, POINTER(Box), POINTER(Source), POINTER(IonParams), c_int,
                               c_uint64, P, P, P, P, P],
     "petch_trace_legacy": [P, P, POINTER(Box), POINTER(Source), c_float, c_int, c_int, c_float,

@@ -1,0 +1,108 @@
+"""Many features etched in lockstep on one GPU.
+
+The reference runs one feature per run_etch_3d call. Here a BatchLoop advances any number of
+features (each its own grid, mask, recipe, ray counts and seeds) step by step: every feature
+builds its surface, then the ray traces of all features' coupled fluxes run as ONE launch per
+trace (petch_trace_ion_yield_batch, petch_trace_neutral_rr_batch: a persistent ray queue
+spanning every feature), then every feature finishes its step (rate, extension, advection,
+reinit). Each feature's rays draw exactly the Philox streams of its own run_etch_3d, so a batched
+feature follows its single run up to the order of the float atomic adds.
+
+    geos = run_etch_3d_batch([dict(Lx=14, Ly=14, Lz=24, dx=0.25, trench_width=6, mask_th=2,
+                                   sub_top=18, hole=True, par=p, flags=fl, n_ion=30000,
+                                   n_neu=30000), ...], t_end=3.0, n_steps=40)
+"""
+import time
+
+import torch
+
+from . import flux as _flux
+from . import ops
+from .threed import EtchLoop, _FluxSteps, _dev, _np, make_trench_3d
+
+
+class BatchLoop:
+    """EtchLoops advanced together; their coupled-flux traces are batched per launch."""
+
+    def __init__(self, loops, timer=None):
+        from .profiling import NULL
+        self.loops = list(loops)
+        self.timer = NULL if timer is None else timer
+
+    def step(self, step):
+        outs = [None] * len(self.loops)
+        pending = {}
+        for i, lp in enumerate(self.loops):
+            job = lp.begin_step(step)
+            if isinstance(job, _FluxSteps):
+                try:
+                    pending[i] = (job.steps, next(job.steps))
+                except StopIteration as stop:
+                    outs[i] = stop.value
+            else:
+                outs[i] = job
+        with self.timer.region("flux"):
+            while pending:
+                for kind in ("ion", "neutral"):
+                    idx = [i for i, (_, req) in pending.items() if req[0] == kind]
+                    if not idx:
+                        continue
+                    reqs = [pending[i][1][1] for i in idx]
+                    if kind == "ion":
+                        with self.timer.region("ion_trace"):
+                            res = ops.trace_ion_yield_batch(reqs)
+                    else:
+                        with self.timer.region("neutral_trace"):
+                            res = ops.trace_neutral_rr_batch(_flux.NEUTRAL_SOURCE, reqs)
+                    for i, r in zip(idx, res):
+                        gen = pending[i][0]
+                        try:
+                            pending[i] = (gen, gen.send(r))
+                        except StopIteration as stop:
+                            outs[i] = stop.value
+                            del pending[i]
+        return [lp.end_step(step, out) for lp, out in zip(self.loops, outs)]
+
+
+_GEO_KEYS = ("Lx", "Ly", "Lz", "dx", "trench_width", "mask_th", "sub_top", "hole")
+
+
+def run_etch_3d_batch(features, t_end=2.0, n_steps=20, reinit_method="fsm", timer=None):
+    """run_etch_3d for every feature of `features` at once, in lockstep.
+
+    features: dicts of run_etch_3d keyword arguments (geometry Lx, Ly, Lz, dx, trench_width,
+    mask_th, sub_top, hole; par, flags, n_ion, n_neu, seed_offset, reinit_every, extend,
+    rays_per_point, etch_stop_z); t_end, n_steps and reinit_method are shared. Returns one
+    geo dict per feature, as run_etch_3d does."""
+    _dev()
+    loops, geos = [], []
+    for ft in features:
+        unknown = set(ft) - set(_GEO_KEYS) - {"par", "flags", "n_ion", "n_neu", "seed_offset",
+                                             "reinit_every", "extend", "rays_per_point",
+                                             "etch_stop_z"}
+        if unknown:
+            raise TypeError(f"run_etch_3d_batch: unsupported feature keys {sorted(unknown)}")
+        g = {k: ft[k] for k in _GEO_KEYS if k in ft}
+        geo = make_trench_3d(g.get("Lx", 10.0), g.get("Ly", 4.0), g.get("Lz", 14.0),
+                             g.get("dx", 0.4), g.get("trench_width", 4.0), g.get("mask_th", 2.0),
+                             g.get("sub_top", 10.0), hole=g.get("hole", False))
+        phi_d, mask_d = geo.pop('_phi_d'), geo.pop('_mask_d')
+        loops.append(EtchLoop(phi_d, mask_d, dict(geo), g.get("mask_th", 2.0), t_end, n_steps,
+                              ft.get("par"), ft.get("flags"), ft.get("n_ion", 20000),
+                              ft.get("n_neu", 20000), reinit_every=ft.get("reinit_every", 1),
+                              extend=ft.get("extend", "gpu"), reinit_method=reinit_method,
+                              seed_offset=ft.get("seed_offset", 0),
+                              rays_per_point=ft.get("rays_per_point"),
+                              etch_stop_z=ft.get("etch_stop_z"), timer=timer))
+        geos.append(geo)
+    batch = BatchLoop(loops, timer)
+    t0 = time.time()
+    for step in range(n_steps):
+        batch.step(step)
+    torch.cuda.synchronize()
+    total = time.time() - t0
+    for geo, lp in zip(geos, loops):
+        lp.timings['total'] = total
+        geo['phi'] = _np(lp.phi)
+        geo['timings'] = lp.timings
+    return geos

@@ -81,11 +81,17 @@ def _check_unsupported(par, flags):
         raise NotImplementedError("floor_charge_throttle is outside the HIP etch-loop path")
 
 
-def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, timer=_NULL_TIMER):
-    """mc_flux_3d_coupled (threed.py:740-905) on the device.
+NEUTRAL_SOURCE = ops.make_source(ops.SRC_NEUTRAL)
 
-    Returns dict(m_i, m_F, m_O, cos_i, bare, ion_yield) of (F,) float32 device tensors;
-    ion_yield = (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None."""
+
+def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None):
+    """mc_flux_3d_coupled (threed.py:740-905) as a generator over its ray traces.
+
+    It yields ("ion", request) and ("neutral", request) dicts and is sent the trace outputs
+    back, so one driver can run it alone (run_traces) and another can advance many features in
+    lockstep with one batched launch per trace (petch.batch). Its return value (StopIteration)
+    is dict(m_i, m_F, m_O, cos_i, bare, ion_yield) of (F,) float32 device tensors; ion_yield =
+    (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None."""
     _check_unsupported(par, flags)
     F = mesh.nf
     Lx, Ly = geo['Lx'], geo['Ly']
@@ -110,9 +116,8 @@ def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, time
 
     ion_seed = (seed * 9 + 1) & SEED_MASK
     if _faithful(flags):
-        with timer.region("ion_trace"):
-            acc = ops.trace_ion_yield(mesh.bvh, box, ion_source(par), ion_params(par), n_ion,
-                                      ion_seed)
+        acc = yield ("ion", dict(bvh=mesh.bvh, box=box, src=ion_source(par), ion=ion_params(par),
+                                 n=n_ion, seed=ion_seed))
         norm = A_src / n_ion
         ion_yield = tuple(host_smooth(ops.normalize(acc[k], A, norm, -ops.INF, ops.INF))
                           for k in range(3))
@@ -127,17 +132,7 @@ def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, time
 
     if par.get('cov_transport', 'rr') != 'rr':
         raise NotImplementedError("cov_transport='cap' (_trace3d_cov) is not part of the HIP path")
-    nsrc = ops.make_source(ops.SRC_NEUTRAL)
     chem = chem_params(par, flags)
-
-    def neutral_pair(bare, it):
-        # the F and O traces of one iteration (threed.py:876-880), seeds seed*9+2+2it and
-        # seed*9+3+2it, as one launch
-        with timer.region("neutral_trace"):
-            fF, fO = ops.trace_neutral_rr_pair(mesh.bvh, box, nsrc, bare,
-                                               betaE, n_neu, (seed * 9 + 2 + 2 * it) & SEED_MASK,
-                                               betaO, n_neu, (seed * 9 + 3 + 2 * it) & SEED_MASK)
-        return finish(fF, A_src / n_neu, 8.0), finish(fO, A_src / n_neu, 8.0)
 
     n_fp = int(par.get('n_fp', 1 if bare_init is not None else 4))
     if bare_init is None:
@@ -146,9 +141,43 @@ def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, time
         bare = torch.clamp(bare_init.to(torch.float32), 0.0, 1.0)
     m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
     for it in range(n_fp):
-        m_F, m_O = neutral_pair(bare, it)
+        # the F and O traces of one iteration (threed.py:876-880), seeds seed*9+2+2it and
+        # seed*9+3+2it, as one launch
+        sF, sO = (seed * 9 + 2 + 2 * it) & SEED_MASK, (seed * 9 + 3 + 2 * it) & SEED_MASK
+        fF, fO = yield ("neutral", dict(bvh=mesh.bvh, box=box, bare=bare, beta_a=betaE,
+                                        n_a=n_neu, seed_a=sF, beta_b=betaO, n_b=n_neu, seed_b=sO))
+        m_F, m_O = finish(fF, A_src / n_neu, 8.0), finish(fO, A_src / n_neu, 8.0)
         bare = ops.belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem)
     return dict(m_i=m_i, m_F=m_F, m_O=m_O, cos_i=cos_i, bare=bare, ion_yield=ion_yield)
+
+
+def run_traces(steps, timer=_NULL_TIMER):
+    """Drives a coupled_flux_steps generator with single-feature traces; returns its result."""
+    try:
+        req = next(steps)
+        while True:
+            kind, r = req
+            if kind == "ion":
+                with timer.region("ion_trace"):
+                    out = ops.trace_ion_yield(r['bvh'], r['box'], r['src'], r['ion'], r['n'],
+                                              r['seed'])
+            else:
+                with timer.region("neutral_trace"):
+                    out = ops.trace_neutral_rr_pair(r['bvh'], r['box'], NEUTRAL_SOURCE, r['bare'],
+                                                    r['beta_a'], r['n_a'], r['seed_a'],
+                                                    r['beta_b'], r['n_b'], r['seed_b'])
+            req = steps.send(out)
+    except StopIteration as stop:
+        return stop.value
+
+
+def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, timer=_NULL_TIMER):
+    """mc_flux_3d_coupled (threed.py:740-905) on the device, one feature.
+
+    Returns dict(m_i, m_F, m_O, cos_i, bare, ion_yield) of (F,) float32 device tensors;
+    ion_yield = (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None."""
+    return run_traces(coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init),
+                      timer)
 
 
 def deterministic_ions(mesh, geo, par, n_ion, seed, flags):

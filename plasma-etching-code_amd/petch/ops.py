@@ -239,6 +239,53 @@ def trace_neutral_rr_pair(bvh, box, src, bare, beta_a, n_a, seed_a, beta_b, n_b,
     return fa, fb
 
 
+def trace_neutral_rr_batch(src, items):
+    """One launch for many features. items: dicts with bvh, box, bare, beta_a, n_a, seed_a,
+    beta_b, n_b, seed_b; returns [(fl_a, fl_b)] per item, each equal to trace_neutral_rr_pair
+    on that feature alone."""
+    import ctypes
+    from ._abi import NeutralFeature
+    if not items:
+        return []
+    outs = []
+    arr = (NeutralFeature * len(items))()
+    for k, it in enumerate(items):
+        bvh = it['bvh']
+        fa = torch.empty((bvh.nf,), dtype=f32, device=bvh.device)
+        fb = torch.empty((bvh.nf,), dtype=f32, device=bvh.device)
+        outs.append((fa, fb))
+        bare = ptr(it['bare'], f32, "bare") if bvh.nf else None
+        arr[k] = NeutralFeature(bvh.h, it['box'], bare,
+                                float(it['beta_a']), int(it['n_a']),
+                                int(it['seed_a']) & 0xFFFFFFFFFFFFFFFF, ptr(fa, f32),
+                                float(it['beta_b']), int(it['n_b']),
+                                int(it['seed_b']) & 0xFFFFFFFFFFFFFFFF, ptr(fb, f32))
+    items[0]['bvh'].ctx.call("petch_trace_neutral_rr_batch", ctypes.byref(src), len(items),
+                             ctypes.cast(arr, ctypes.c_void_p))
+    return outs
+
+
+def trace_ion_yield_batch(items):
+    """One launch for many features. items: dicts with bvh, box, src, ion, n, seed; returns a
+    (5, F) accumulator per item, each equal to trace_ion_yield on that feature alone."""
+    import ctypes
+    from ._abi import IonFeature
+    if not items:
+        return []
+    outs = []
+    arr = (IonFeature * len(items))()
+    for k, it in enumerate(items):
+        bvh = it['bvh']
+        acc = torch.empty((5, bvh.nf), dtype=f32, device=bvh.device)
+        outs.append(acc)
+        arr[k] = IonFeature(bvh.h, it['box'], it['src'], it['ion'], int(it['n']),
+                            int(it['seed']) & 0xFFFFFFFFFFFFFFFF, ptr(acc[0]), ptr(acc[1]),
+                            ptr(acc[2]), ptr(acc[3]), ptr(acc[4]))
+    items[0]['bvh'].ctx.call("petch_trace_ion_yield_batch", len(items),
+                             ctypes.cast(arr, ctypes.c_void_p))
+    return outs
+
+
 def trace_ion_yield(bvh, box, src, ion, n, seed, out=None):
     """Faithful ion accumulators (5, F): sp, ie, ip, arrival, weighted cos (threed.py:599-712)."""
     import ctypes

@@ -276,6 +276,13 @@ def _reinit(phi_d, dx, band, method):
                               "(supported: 'fsm', 'skfmm', 'skfmm_full')")
 
 
+class _FluxSteps:
+    """A coupled-flux generator (flux.coupled_flux_steps) waiting for its traces."""
+
+    def __init__(self, steps):
+        self.steps = steps
+
+
 class EtchLoop:
     """Device state and one etch step of run_etch_3d (threed.py:1671-1792).
 
@@ -332,40 +339,69 @@ class EtchLoop:
         self.last = {}
 
     def step(self, step):
+        """One etch step; the coupled flux's traces run one launch at a time."""
+        job = self.begin_step(step)
+        if isinstance(job, _FluxSteps):
+            with self.timer.region("flux"):
+                tf = time.time()
+                out = _flux.run_traces(job.steps, self.timer)
+                self.timings['flux'] += time.time() - tf
+        else:
+            out = job
+        return self.end_step(step, out)
+
+    def begin_step(self, step):
+        """Mesh, BVH and warm start of one step, then the flux: returns its result (dict or
+        None), or, for the coupled flux, a _FluxSteps whose trace requests the caller runs
+        (alone, or batched with other features) before passing the result to end_step."""
         T, par, flags, dx, dev = self.timer, self.par, self.flags, self.dx, self.dev
         tim = self.timings
         tm = time.time()
         with T.region("mesh"):
             mesh = self.meshes[step % 2].build(self.phi, dx)
         tim['mesh'] += time.time() - tm
+        self._mesh = mesh
         if self.rays_per_point is not None:
             n_ion_s = n_neu_s = max(int(self.rays_per_point) * mesh.nf, 1000)
         else:
             n_ion_s, n_neu_s = self.n_ion, self.n_neu
-        tf = time.time()
+        self._rays = (n_ion_s, n_neu_s)
         seed = step + self.seed_offset
-        with T.region("flux"):
-            if mesh.nf == 0:
-                out = None
-            elif self.nt == "radiosity":
-                out = _flux.radiosity_flux(mesh, self.phi, self.geo, par, n_ion_s, seed, flags)
-            elif getattr(flags, "coverage_sticking", False):
-                bi = None
-                if self.warm and self.prev_bare is not None and bool(torch.isfinite(mesh.centroids).all()):
-                    if self.gpu_ws:                 # nearest previous face (threed.py:1700-1701)
-                        bi = ops.nearest_face_value(self.prev_mesh.bvh, self.prev_bare,
-                                                    mesh.centroids, 1.0)
-                    else:                           # nearest previous centroid (threed.py:1702-1705)
-                        idx = torch.arange(self.prev_cen.shape[0], dtype=torch.int32, device=dev)
-                        self.pt_bvh.build(self.prev_cen, idx[:, None].repeat(1, 3).contiguous())
-                        bi = ops.nearest_face_value(self.pt_bvh, self.prev_bare, mesh.centroids, 1.0)
-                out = _flux.coupled_flux(mesh, self.geo, par, n_ion_s, n_neu_s, seed, flags,
-                                         bare_init=bi, timer=T)
-                self.prev_mesh, self.prev_bare, self.prev_cen = mesh, out['bare'], mesh.centroids
-            else:
-                out = _flux.uncoupled_flux(mesh, self.geo, self.mc_par, n_ion_s, n_neu_s, seed,
-                                           getattr(flags, "ion_reflection", False))
-        tim['flux'] += time.time() - tf
+        if mesh.nf == 0:
+            return None
+        tf = time.time()
+        try:
+            with T.region("flux"):
+                if self.nt == "radiosity":
+                    return _flux.radiosity_flux(mesh, self.phi, self.geo, par, n_ion_s, seed, flags)
+                if getattr(flags, "coverage_sticking", False):
+                    bi = None
+                    if self.warm and self.prev_bare is not None and \
+                            bool(torch.isfinite(mesh.centroids).all()):
+                        if self.gpu_ws:             # nearest previous face (threed.py:1700-1701)
+                            bi = ops.nearest_face_value(self.prev_mesh.bvh, self.prev_bare,
+                                                        mesh.centroids, 1.0)
+                        else:                       # nearest previous centroid (threed.py:1702-1705)
+                            idx = torch.arange(self.prev_cen.shape[0], dtype=torch.int32, device=dev)
+                            self.pt_bvh.build(self.prev_cen, idx[:, None].repeat(1, 3).contiguous())
+                            bi = ops.nearest_face_value(self.pt_bvh, self.prev_bare,
+                                                        mesh.centroids, 1.0)
+                    return _FluxSteps(_flux.coupled_flux_steps(mesh, self.geo, par, n_ion_s,
+                                                               n_neu_s, seed, flags,
+                                                               bare_init=bi))
+                return _flux.uncoupled_flux(mesh, self.geo, self.mc_par, n_ion_s, n_neu_s, seed,
+                                            getattr(flags, "ion_reflection", False))
+        finally:
+            tim['flux'] += time.time() - tf
+
+    def end_step(self, step, out):
+        """Rate, extension, CFL, advection and reinit of the step begun by begin_step."""
+        T, par, flags, dx, dev = self.timer, self.par, self.flags, self.dx, self.dev
+        tim = self.timings
+        mesh = self._mesh
+        if out is not None and getattr(flags, "coverage_sticking", False) and \
+                self.nt != "radiosity":
+            self.prev_mesh, self.prev_bare, self.prev_cen = mesh, out['bare'], mesh.centroids
         trt = time.time()
         with T.region("rate"):
             if out is not None:
@@ -410,7 +446,8 @@ class EtchLoop:
             with T.region("reinit"):
                 _reinit(self.phi, dx, self.band + 2.0 * dx, self.reinit_method)
             tim['reinit'] += time.time() - tr
-        self.last = dict(nf=mesh.nf, nv=mesh.nv, Vmax=Vmax, nsub=nsub, n_ion=n_ion_s, n_neu=n_neu_s)
+        self.last = dict(nf=mesh.nf, nv=mesh.nv, Vmax=Vmax, nsub=nsub, n_ion=self._rays[0],
+                         n_neu=self._rays[1])
         return self.last
 
 
