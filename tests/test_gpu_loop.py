@@ -128,6 +128,8 @@ def _floor_depth(phi, dx, sub_top):
         if p[k] < 0.0 or k == len(p) - 1:
             continue
         zs.append((k + p[k] / (p[k] - p[k + 1])) * dx)
+    if not zs:                                         # etched through to the domain bottom
+        return float("nan")
     return sub_top - float(np.mean(zs))
 
 
