@@ -7,10 +7,10 @@
 //          (>= 0 internal node, < 0 leaf ~slot); node 0 is the root.
 //   tris:  nf leaves of 48 B in Morton order: v0 (w = face id bits), v1, v2 as float4, so a
 //          leaf is three 16-byte loads and carries the face id the flux deposits into.
-//   wide:  4-wide nodes of 128 B collapsed from the binary tree: the binary node at every even
-//          depth stores its up-to-four grandchildren (a child that is a leaf stays a child), at
-//          the binary node's own index. Ray queries walk the wide nodes, which halves the chain
-//          of dependent node fetches; point queries walk the binary nodes.
+//   wide:  4-wide nodes of 128 B collapsed from the binary tree, each stored at the index of
+//          the binary node it replaces (children: up to four descendants; a leaf stays a leaf).
+//          Ray queries walk the wide nodes, which halves the chain of dependent node fetches;
+//          point queries walk the binary nodes.
 // Traversal keeps a short per-lane stack of far children with their entry distances (the
 // nearest child is descended first, and a popped entry beyond the best hit so far is dropped
 // unfetched); a stack overflow drops the far subtree and raises error flag 1 instead of writing

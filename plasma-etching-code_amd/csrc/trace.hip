@@ -10,6 +10,7 @@
 //   k_trace_legacy       _trace3d           threed.py:352-406 (sticking neutral / specular ion)
 //   k_trace_ff           _trace_ff          threed.py:908-920 (first hit of face-emitted rays)
 //   k_gen_source         _gen_source        threed.py:441-458 (same streams, for checking)
+//   k_trace_*_batch      the same two traces for many features from one ray queue
 // Periodic lateral boundaries follow _apply_bc (threed.py:289-349); the hit found inside the wrap
 // loop is the hit of the caller's query (the reference repeats the same query), so it is reused.
 #include "common.h"
