@@ -150,6 +150,10 @@ int petch_bvh_query_point(petch_ctx* ctx, const petch_bvh* bvh, const float* pts
                           float max_dist, int* face, float* dist);
 /* Traversal error flag (1 = a query overflowed its stack and may have missed a face); syncs. */
 int petch_bvh_status(petch_ctx* ctx, const petch_bvh* bvh, int* err_host);
+/* The same flag copied into device memory err_dev (int) on the stream, without a host wait, so
+ * a caller can gather the flags of many BVHs and read them all at once (no reference
+ * counterpart). */
+int petch_bvh_status_async(petch_ctx* ctx, const petch_bvh* bvh, int* err_dev);
 /* Root AABB (lo xyz, hi xyz) of a BVH with at least two faces; syncs. */
 int petch_bvh_root_bounds(petch_ctx* ctx, const petch_bvh* bvh, float* lohi_host);
 

@@ -557,6 +557,17 @@ extern "C" int petch_bvh_status(petch_ctx* ctx, const petch_bvh* b, int* err_hos
   return PETCH_OK;
 }
 
+extern "C" int petch_bvh_status_async(petch_ctx* ctx, const petch_bvh* b, int* err_dev) {
+  PETCH_ARG(ctx, ctx && b && err_dev, "bvh_status_async: null pointer");
+  if (!b->work) {
+    PETCH_HIP(ctx, hipMemsetAsync(err_dev, 0, sizeof(int), ctx->stream));
+    return PETCH_OK;
+  }
+  PETCH_HIP(ctx, hipMemcpyAsync(err_dev, carve(b->work, b->cap_f).err, sizeof(int),
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  return PETCH_OK;
+}
+
 extern "C" int petch_bvh_root_bounds(petch_ctx* ctx, const petch_bvh* b, float* lohi_host) {
   PETCH_ARG(ctx, ctx && b && lohi_host, "bvh_root_bounds: null pointer");
   PETCH_ARG(ctx, b->work && b->nf > 1, "bvh_root_bounds: BVH has no internal nodes");

@@ -5,8 +5,10 @@ features (each its own grid, mask, recipe, ray counts and seeds) step by step: e
 builds its surface, then the ray traces of all features' coupled fluxes run as ONE launch per
 trace (petch_trace_ion_yield_batch, petch_trace_neutral_rr_batch: a persistent ray queue
 spanning every feature), then every feature finishes its step (rate, extension, advection,
-reinit). Each feature's rays draw exactly the Philox streams of its own run_etch_3d, so a batched
-feature follows its single run up to the order of the float atomic adds.
+reinit). The host reads a step needs (warm-start guard, max V, BVH guard) are gathered over all
+features into one read each, instead of one per feature. Each feature's rays draw exactly the
+Philox streams of its own run_etch_3d, so a batched feature follows its single run up to the
+order of the float atomic adds.
 
     geos = run_etch_3d_batch([dict(Lx=14, Ly=14, Lz=24, dx=0.25, trench_width=6, mask_th=2,
                                    sub_top=18, hole=True, par=p, flags=fl, n_ion=30000,
@@ -18,7 +20,7 @@ import torch
 
 from . import flux as _flux
 from . import ops
-from .threed import EtchLoop, _FluxSteps, _dev, _np, make_trench_3d
+from .threed import EtchLoop, _FluxSteps, _dev, _np, _trench_geo
 
 
 class BatchLoop:
@@ -30,10 +32,18 @@ class BatchLoop:
         self.timer = NULL if timer is None else timer
 
     def step(self, step):
-        outs = [None] * len(self.loops)
+        loops = self.loops
+        # 1. every feature's mesh and BVH; the warm-start guards of all features in one read
+        finite = [lp.begin_mesh(step) for lp in loops]
+        need = [i for i, f in enumerate(finite) if f is not None]
+        if need:
+            vals = torch.stack([finite[i] for i in need]).cpu().tolist()
+            for i, v in zip(need, vals):
+                finite[i] = bool(v)
+        outs = [None] * len(loops)
         pending = {}
-        for i, lp in enumerate(self.loops):
-            job = lp.begin_step(step)
+        for i, lp in enumerate(loops):
+            job = lp.begin_flux(step, finite[i])
             if isinstance(job, _FluxSteps):
                 try:
                     pending[i] = (job.steps, next(job.steps))
@@ -41,6 +51,7 @@ class BatchLoop:
                     outs[i] = stop.value
             else:
                 outs[i] = job
+        # 2. the coupled fluxes: one launch per trace for all features
         with self.timer.region("flux"):
             while pending:
                 for kind in ("ion", "neutral"):
@@ -61,7 +72,21 @@ class BatchLoop:
                         except StopIteration as stop:
                             outs[i] = stop.value
                             del pending[i]
-        return [lp.end_step(step, out) for lp, out in zip(self.loops, outs)]
+        # 3. every feature's rate; the rate reductions and BVH guards of all features in one read
+        has = [lp.end_rate(step, out) for lp, out in zip(loops, outs)]
+        reads = [(0.0, 0, 0)] * len(loops)
+        live = [i for i, h in enumerate(has) if h]
+        if live:
+            buf = torch.zeros((len(live), 3), dtype=torch.int32, device=loops[live[0]].dev)
+            buf[:, :2] = torch.stack([loops[i].red for i in live])
+            for r, i in enumerate(live):
+                loops[i]._mesh.bvh.status_into(buf[r, 2:3])
+            host = buf.cpu().numpy()
+            for r, i in enumerate(live):
+                vmx, nbad = ops.decode_rate_reduction(host[r, :2])
+                reads[i] = (vmx, nbad, int(host[r, 2]))
+        # 4. extension, CFL, advection and reinit of every feature
+        return [lp.end_finish(step, *rd) for lp, rd in zip(loops, reads)]
 
 
 _GEO_KEYS = ("Lx", "Ly", "Lz", "dx", "trench_width", "mask_th", "sub_top", "hole")
@@ -83,9 +108,9 @@ def run_etch_3d_batch(features, t_end=2.0, n_steps=20, reinit_method="fsm", time
         if unknown:
             raise TypeError(f"run_etch_3d_batch: unsupported feature keys {sorted(unknown)}")
         g = {k: ft[k] for k in _GEO_KEYS if k in ft}
-        geo = make_trench_3d(g.get("Lx", 10.0), g.get("Ly", 4.0), g.get("Lz", 14.0),
-                             g.get("dx", 0.4), g.get("trench_width", 4.0), g.get("mask_th", 2.0),
-                             g.get("sub_top", 10.0), hole=g.get("hole", False))
+        geo = _trench_geo(g.get("Lx", 10.0), g.get("Ly", 4.0), g.get("Lz", 14.0),
+                          g.get("dx", 0.4), g.get("trench_width", 4.0), g.get("mask_th", 2.0),
+                          g.get("sub_top", 10.0), g.get("hole", False))
         phi_d, mask_d = geo.pop('_phi_d'), geo.pop('_mask_d')
         loops.append(EtchLoop(phi_d, mask_d, dict(geo), g.get("mask_th", 2.0), t_end, n_steps,
                               ft.get("par"), ft.get("flags"), ft.get("n_ion", 20000),

@@ -174,6 +174,10 @@ class Bvh:
         self.ctx.call("petch_bvh_status", self.h, ctypes.byref(e))
         return e.value
 
+    def status_into(self, dst):
+        """Copies the guard flag into the int32 device tensor element dst (no host wait)."""
+        self.ctx.call("petch_bvh_status_async", self.h, ptr(dst, i32))
+
     def root_bounds(self):
         import numpy as np
         out = np.zeros(6, np.float32)
@@ -378,8 +382,13 @@ def surface_rate(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, is
 
 def read_rate_reduction(red):
     """(Vmax, n_nonfinite) from the surface-rate reduction buffer (one device sync)."""
+    return decode_rate_reduction(red.cpu().numpy())
+
+
+def decode_rate_reduction(h):
+    """(Vmax, n_nonfinite) from a host copy of the reduction pair [max V bits, count]."""
     import numpy as np
-    h = red.cpu().numpy().astype(np.int32)
+    h = np.asarray(h, dtype=np.int32)
     vmax = float(np.frombuffer(h[:1].tobytes(), dtype=np.float32)[0])
     return vmax, int(h[1])
 

@@ -47,11 +47,19 @@ def make_trench_3d(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole=False):
     """Initial level set and mask (threed.py:32-59): exact signed distance to the CSG solid
     (substrate half-space union the masked slab), phi > 0 solid. The geo dict carries numpy
     copies like the reference plus the device tensors under '_phi_d' / '_mask_d'."""
+    geo = _trench_geo(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole)
+    geo['phi'] = _np(geo['_phi_d'])
+    return geo
+
+
+def _trench_geo(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole):
+    """make_trench_3d without the host copy of the initial phi (the etch loops replace it with
+    the final phi, so copying the initial one would be wasted)."""
     nx, ny, nz = grid_shape(Lx, Ly, Lz, dx)
     phi_d, mask_d = ops.init_sdf(nx, ny, nz, dx, Lx, Ly, trench_width, mask_th, sub_top, hole,
                                  device=_dev())
     xs, ys, zs = np.arange(nx) * dx, np.arange(ny) * dx, np.arange(nz) * dx
-    return dict(xs=xs, ys=ys, zs=zs, dx=dx, phi=_np(phi_d), mask=_np(mask_d, bool),
+    return dict(xs=xs, ys=ys, zs=zs, dx=dx, mask=_np(mask_d, bool),
                 Lx=Lx, Ly=Ly, Lz=Lz, sub_top=sub_top, trench_width=trench_width, hole=hole,
                 _phi_d=phi_d, _mask_d=mask_d)
 
@@ -354,18 +362,33 @@ class EtchLoop:
         """Mesh, BVH and warm start of one step, then the flux: returns its result (dict or
         None), or, for the coupled flux, a _FluxSteps whose trace requests the caller runs
         (alone, or batched with other features) before passing the result to end_step."""
-        T, par, flags, dx, dev = self.timer, self.par, self.flags, self.dx, self.dev
-        tim = self.timings
+        finite = self.begin_mesh(step)
+        return self.begin_flux(step, None if finite is None else bool(finite))
+
+    def begin_mesh(self, step):
+        """Mesh + BVH of the step. Returns the device flag 'all centroids finite' that gates the
+        warm start (threed.py:1699), or None when the step needs no warm start."""
+        T, dx = self.timer, self.dx
         tm = time.time()
         with T.region("mesh"):
             mesh = self.meshes[step % 2].build(self.phi, dx)
-        tim['mesh'] += time.time() - tm
+        self.timings['mesh'] += time.time() - tm
         self._mesh = mesh
         if self.rays_per_point is not None:
-            n_ion_s = n_neu_s = max(int(self.rays_per_point) * mesh.nf, 1000)
+            n = max(int(self.rays_per_point) * mesh.nf, 1000)
+            self._rays = (n, n)
         else:
-            n_ion_s, n_neu_s = self.n_ion, self.n_neu
-        self._rays = (n_ion_s, n_neu_s)
+            self._rays = (self.n_ion, self.n_neu)
+        if mesh.nf and self.nt != "radiosity" and getattr(self.flags, "coverage_sticking", False) \
+                and self.warm and self.prev_bare is not None:
+            return torch.isfinite(mesh.centroids).all()
+        return None
+
+    def begin_flux(self, step, finite):
+        """The flux of the step begun by begin_mesh; `finite` is that flag read on the host."""
+        T, par, flags, dev = self.timer, self.par, self.flags, self.dev
+        mesh = self._mesh
+        n_ion_s, n_neu_s = self._rays
         seed = step + self.seed_offset
         if mesh.nf == 0:
             return None
@@ -376,8 +399,7 @@ class EtchLoop:
                     return _flux.radiosity_flux(mesh, self.phi, self.geo, par, n_ion_s, seed, flags)
                 if getattr(flags, "coverage_sticking", False):
                     bi = None
-                    if self.warm and self.prev_bare is not None and \
-                            bool(torch.isfinite(mesh.centroids).all()):
+                    if finite:
                         if self.gpu_ws:             # nearest previous face (threed.py:1700-1701)
                             bi = ops.nearest_face_value(self.prev_mesh.bvh, self.prev_bare,
                                                         mesh.centroids, 1.0)
@@ -392,34 +414,49 @@ class EtchLoop:
                 return _flux.uncoupled_flux(mesh, self.geo, self.mc_par, n_ion_s, n_neu_s, seed,
                                             getattr(flags, "ion_reflection", False))
         finally:
-            tim['flux'] += time.time() - tf
+            self.timings['flux'] += time.time() - tf
 
     def end_step(self, step, out):
         """Rate, extension, CFL, advection and reinit of the step begun by begin_step."""
-        T, par, flags, dx, dev = self.timer, self.par, self.flags, self.dx, self.dev
-        tim = self.timings
+        if self.end_rate(step, out):
+            vmx, nbad = ops.read_rate_reduction(self.red)
+            return self.end_finish(step, vmx, nbad, self._mesh.bvh.status())
+        return self.end_finish(step, 0.0, 0, 0)
+
+    def end_rate(self, step, out):
+        """Surface rate of the step (no host read). Returns False when there is no surface."""
+        T, flags = self.timer, self.flags
         mesh = self._mesh
         if out is not None and getattr(flags, "coverage_sticking", False) and \
                 self.nt != "radiosity":
             self.prev_mesh, self.prev_bare, self.prev_cen = mesh, out['bare'], mesh.centroids
+        self._V = None
+        if out is None:
+            return False
         trt = time.time()
         with T.region("rate"):
-            if out is not None:
-                iy = out['ion_yield'] if getattr(flags, "chemistry", "") == "belen" else None
-                V, _ = ops.surface_rate(out['m_i'], out['m_F'], out['m_O'], out['cos_i'], iy,
-                                        mesh.centroids, self.mg, self.chem, red=self.red)
-        if out is not None:
-            vmx, nbad = ops.read_rate_reduction(self.red)
-            bvh_err = mesh.bvh.status()
-            if bvh_err:
-                raise RuntimeError(f"step {step}: BVH traversal guard tripped (code {bvh_err}: "
-                                   "1 stack overflow, 2 corrupt tree); the flux is incomplete")
-            if nbad:
-                warnings.warn(f"step {step}: {nbad} non-finite surface velocities (NaN/inf) zeroed; "
-                              "check flux normalization / coverage if this persists.", RuntimeWarning)
-        else:
-            V, vmx = None, 0.0
-        tim['rate'] = tim.get('rate', 0.0) + time.time() - trt
+            iy = out['ion_yield'] if getattr(flags, "chemistry", "") == "belen" else None
+            self._V, _ = ops.surface_rate(out['m_i'], out['m_F'], out['m_O'], out['cos_i'], iy,
+                                          mesh.centroids, self.mg, self.chem, red=self.red)
+        self.timings['rate'] = self.timings.get('rate', 0.0) + time.time() - trt
+        return True
+
+    def end_finish(self, step, vmx, nbad, bvh_err):
+        """Extension, CFL, advection and reinit, given the host reads of the rate reduction
+        (max V, non-finite count) and of the BVH guard flag."""
+        T, dx, dev = self.timer, self.dx, self.dev
+        tim = self.timings
+        mesh = self._mesh
+        V = self._V
+        if bvh_err:
+            raise RuntimeError(f"step {step}: BVH traversal guard tripped (code {bvh_err}: "
+                               "1 stack overflow, 2 corrupt tree, 3 tree too deep); the flux is "
+                               "incomplete")
+        if nbad:
+            warnings.warn(f"step {step}: {nbad} non-finite surface velocities (NaN/inf) zeroed; "
+                          "check flux normalization / coverage if this persists.", RuntimeWarning)
+        if V is None:
+            vmx = 0.0
         te = time.time()
         with T.region("extend"):
             if V is None:
@@ -461,7 +498,7 @@ def run_etch_3d(Lx=10.0, Ly=4.0, Lz=14.0, dx=0.4, trench_width=4.0, mask_th=2.0,
     if surf_smooth and surf_smooth > 0.0:
         raise NotImplementedError("surf_smooth (a scipy gaussian_filter) is outside the HIP etch loop")
     _dev()
-    geo = make_trench_3d(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole=hole)
+    geo = _trench_geo(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole)
     phi_d, mask_d = geo.pop('_phi_d'), geo.pop('_mask_d')
     loop = EtchLoop(phi_d, mask_d, dict(geo), mask_th, t_end, n_steps, par, flags, n_ion, n_neu,
                     reinit_every=reinit_every, extend=extend, reinit_method=reinit_method,

@@ -10,6 +10,19 @@ class FakeLoop:
     def __init__(self, k, n_fp):
         self.k, self.n_fp, self.log = k, n_fp, []
 
+    def begin_mesh(self, step):
+        return None
+
+    def begin_flux(self, step, finite):
+        return self.begin_step(step)
+
+    def end_rate(self, step, out):
+        self._out = out
+        return False
+
+    def end_finish(self, step, vmx, nbad, err):
+        return self.end_step(step, self._out)
+
     def begin_step(self, step):
         def steps():
             acc = yield ("ion", dict(k=self.k, step=step))
