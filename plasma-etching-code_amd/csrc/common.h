@@ -57,6 +57,8 @@ struct petch_bvh {
   void* wide;              // 4-wide nodes, 128 B each, at the even-depth binary nodes' indices
   void* tris;              // nf triangles in leaf order, 48 B each (v0, e1, e2 as float4)
   void* work;              // build scratch: morton keys/values, parents, flags
+  void* aux;               // clustering scratch and tree of the large-mesh build
+  size_t aux_cap;
   size_t work_cap;
   float root_lo[3], root_hi[3];
 };
