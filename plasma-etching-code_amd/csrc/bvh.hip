@@ -13,6 +13,7 @@
 // plus the two query entry points used by the tests (the trace kernels inline the traversal).
 #include "common.h"
 #include "bvh.h"
+#include "scan.h"
 
 int petch_radix_sort_pairs(petch_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* ktmp,
                            uint32_t* vtmp, int n, int bits);
@@ -408,10 +409,12 @@ __global__ void __launch_bounds__(kWideBS) k_bvh_wide_sah(const BvhNode* __restr
 // union box has the smallest surface area; mutual nearest pairs merge into a new internal node;
 // the survivors are compacted in order, and the rounds repeat until one cluster is left. Node
 // indices count down from n-2, so the last merge is the root, node 0, as in the Morton tree.
-// Builds a tree closer to a surface-area-optimal one than the Morton split, at ~6 launches per
-// round. If kPlocRounds rounds do not get down to one cluster the Morton tree is kept.
+// Builds a tree closer to a surface-area-optimal one than the Morton split. The first rounds
+// run as grid-wide launches; the remaining few thousand clusters are finished by one workgroup
+// in a single launch. If the rounds do not get down to one cluster the Morton tree is kept.
 constexpr int kPlocRadius = 16;
-constexpr int kPlocRounds = 48;
+constexpr int kPlocRounds = 6;          // rounds as grid-wide launches (the big early rounds)
+constexpr int kPlocFinishRounds = 512;  // then one workgroup finishes the (small) rest
 
 struct PlocWork {
   int *ca, *cb;     // cluster refs (>= 0 node, < 0 leaf ~slot), ping-pong
@@ -542,6 +545,97 @@ __global__ void k_ploc_scatter(const int* __restrict__ ca, const float4* __restr
   bb[2 * k + 1] = ba[2 * i + 1];
 }
 
+// The remaining rounds in one workgroup: the same nearest-neighbour, merge and in-order
+// compaction steps, separated by workgroup barriers instead of launches.
+constexpr int kFinBS = 1024;
+
+__global__ void __launch_bounds__(kFinBS) k_ploc_finish(int* __restrict__ ca,
+                                                        float4* __restrict__ ba,
+                                                        int* __restrict__ cb,
+                                                        float4* __restrict__ bb,
+                                                        int* __restrict__ nn,
+                                                        int* __restrict__ valid,
+                                                        int* __restrict__ cnt,
+                                                        BvhNode* __restrict__ nodes,
+                                                        int* __restrict__ alloc, int n) {
+  __shared__ int s_count;
+  if (threadIdx.x == 0) s_count = *cnt;
+  __syncthreads();
+  int* A = ca;
+  float4* AB = ba;
+  int* B = cb;
+  float4* BB = bb;
+  for (int round = 0; round < kPlocFinishRounds; ++round) {
+    const int count = s_count;
+    if (count <= 1) break;
+    for (int i = threadIdx.x; i < count; i += kFinBS) {
+      const float4 lo = AB[2 * i], hi = AB[2 * i + 1];
+      int best = -1;
+      float best_a = INFINITY;
+      const int j0 = i - kPlocRadius < 0 ? 0 : i - kPlocRadius;
+      const int j1 = i + kPlocRadius > count - 1 ? count - 1 : i + kPlocRadius;
+      for (int j = j0; j <= j1; ++j) {
+        if (j == i) continue;
+        const float a = union_area(lo, hi, AB[2 * j], AB[2 * j + 1]);
+        if (a < best_a) {
+          best_a = a;
+          best = j;
+        }
+      }
+      nn[i] = best;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += kFinBS) {
+      const int j = nn[i];
+      if (j < 0 || j >= count || nn[j] != i) {
+        valid[i] = 1;
+        continue;
+      }
+      if (i > j) {
+        valid[i] = 0;
+        continue;
+      }
+      const int node = (n - 2) - atomicAdd(alloc, 1);
+      if (node < 0) {
+        valid[i] = 1;
+        continue;
+      }
+      const float4 l0 = AB[2 * i], l1 = AB[2 * i + 1], r0 = AB[2 * j], r1 = AB[2 * j + 1];
+      BvhNode nd;
+      nd.a = make_float4(l0.x, l0.y, l0.z, l1.x);
+      nd.b = make_float4(l1.y, l1.z, r0.x, r0.y);
+      nd.c = make_float4(r0.z, r1.x, r1.y, r1.z);
+      nd.d = make_int4(A[i], A[j], 0, 0);
+      nodes[node] = nd;
+      A[i] = node;
+      AB[2 * i] = make_float4(fminf(l0.x, r0.x), fminf(l0.y, r0.y), fminf(l0.z, r0.z), 0.f);
+      AB[2 * i + 1] = make_float4(fmaxf(l1.x, r1.x), fmaxf(l1.y, r1.y), fmaxf(l1.z, r1.z), 0.f);
+      valid[i] = 1;
+    }
+    __syncthreads();
+    int carry = 0;
+    for (int base = 0; base < count; base += kFinBS) {
+      const int i = base + threadIdx.x;
+      const int v = i < count ? valid[i] : 0;
+      int tot;
+      const int ex = petch::block_excl_scan<kFinBS>(v, tot);
+      if (v) {
+        const int k = carry + ex;
+        B[k] = A[i];
+        BB[2 * k] = AB[2 * i];
+        BB[2 * k + 1] = AB[2 * i + 1];
+      }
+      carry += tot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_count = carry;
+    __syncthreads();
+    int* ti = A; A = B; B = ti;
+    float4* tf = AB; AB = BB; BB = tf;
+  }
+  if (threadIdx.x == 0) *cnt = s_count;
+}
+
 // Adopts the clustered tree when it is complete (one cluster left after n-1 merges).
 __global__ void k_ploc_commit(const BvhNode* __restrict__ src, const int* __restrict__ cnt,
                               const int* __restrict__ alloc, int n,
@@ -641,8 +735,12 @@ int ploc_build(petch_ctx* ctx, petch_bvh* b, int nf) {
     int* ti = ca; ca = cb; cb = ti;
     float4* tf = ba; ba = bb; bb = tf;
   }
+  int* cnt = p.cnt + (kPlocRounds & 1);
+  hipLaunchKernelGGL(k_ploc_finish, dim3(1), dim3(kFinBS), 0, ctx->stream, ca, ba, cb, bb, p.nn,
+                     p.valid, cnt, p.nodes, p.alloc, nf);
+  PETCH_LAUNCH_CHECK(ctx);
   hipLaunchKernelGGL(k_ploc_commit, dim3(g < 256 ? g : 256), dim3(kBS), 0, ctx->stream, p.nodes,
-                     p.cnt + (kPlocRounds & 1), p.alloc, nf, (BvhNode*)b->nodes);
+                     cnt, p.alloc, nf, (BvhNode*)b->nodes);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
