@@ -140,6 +140,12 @@ int petch_bvh_create(petch_ctx* ctx, petch_bvh** out);
 int petch_bvh_destroy(petch_ctx* ctx, petch_bvh* bvh);
 int petch_bvh_build(petch_ctx* ctx, petch_bvh* bvh, const float* verts, const int* faces, int nv,
                     int nf);
+/* Build quality of the next petch_bvh_build calls: 0 Morton tree collapsed to 4-wide nodes by a
+ * fixed rule (fastest build), 1 Morton tree with an area-driven collapse, 2 clustered tree
+ * (PLOC) with the area-driven collapse (slowest build, fastest traces), -1 (default) 2 for
+ * meshes of 32k faces or more and 0 below. A caller that knows its ray budget picks the
+ * trade-off (no reference counterpart: wp.Mesh has one builder). */
+int petch_bvh_set_quality(petch_ctx* ctx, petch_bvh* bvh, int quality);
 /* wp.mesh_query_ray(mesh, o, d, tmax) (threed.py:306,366,524,569,629,916): closest two-sided hit
  * with 0 < t < tmax (watertight test); face = -1 and t = -1 on a miss. t may be NULL. */
 int petch_bvh_query_ray(petch_ctx* ctx, const petch_bvh* bvh, const float* orig,

@@ -302,8 +302,10 @@ __device__ __forceinline__ float box_area(const float* lo, const float* hi) {
 
 constexpr int kWideBS = 1024;
 constexpr int kWideMaxLevels = 96;
-// The area-driven collapse runs in one workgroup (~0.28 ms at 47k faces) and saves ~3 % of
-// trace time; below this size the per-level grandchildren rule (one parallel pass) is used.
+// The area-driven collapse runs in one workgroup (~0.28 ms at 47k faces) and the clustering in
+// ~50 launches (~1 ms); together they save ~7 % of trace time. Unless the caller chose a
+// quality (petch_bvh_set_quality), meshes below this size get the Morton tree and the
+// per-level grandchildren rule (one parallel pass).
 constexpr int kWideSahMinFaces = 32768;
 
 // Top-down collapse into 4-wide nodes, one workgroup: starting from the root, each wide node
@@ -681,7 +683,16 @@ extern "C" int petch_bvh_create(petch_ctx* ctx, petch_bvh** out) {
   petch_bvh* b = (petch_bvh*)calloc(1, sizeof(petch_bvh));
   if (!b) return petch_set_error(ctx, PETCH_E_HIP, "bvh_create: out of host memory");
   b->device = ctx->device;
+  b->quality = -1;
   *out = b;
+  return PETCH_OK;
+}
+
+extern "C" int petch_bvh_set_quality(petch_ctx* ctx, petch_bvh* b, int quality) {
+  PETCH_ARG(ctx, ctx && b, "bvh_set_quality: null pointer");
+  PETCH_ARG(ctx, quality >= -1 && quality <= 2, "bvh_set_quality: quality %d not in -1..2",
+            quality);
+  b->quality = quality;
   return PETCH_OK;
 }
 
@@ -801,9 +812,12 @@ extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts,
     hipLaunchKernelGGL(k_bvh_refit, dim3(g), dim3(kBS), 0, ctx->stream, (const float4*)b->tris,
                        (BvhNode*)b->nodes, w.pint, w.pleaf, w.flags, nf, w.root);
     PETCH_LAUNCH_CHECK(ctx);
-    if (nf >= kWideSahMinFaces) {
+    const int q = b->quality >= 0 ? b->quality : (nf >= kWideSahMinFaces ? 2 : 0);
+    if (q >= 2) {
       int rc2 = ploc_build(ctx, b, nf);
       if (rc2 != PETCH_OK) return rc2;
+    }
+    if (q >= 1) {
       hipLaunchKernelGGL(k_bvh_wide_sah, dim3(1), dim3(kWideBS), 0, ctx->stream,
                          (const BvhNode*)b->nodes, nf, (BvhWide*)b->wide, (int*)w.ktmp,
                          (int*)w.vtmp, w.err);

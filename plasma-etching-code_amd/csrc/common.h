@@ -54,12 +54,14 @@ struct petch_bvh {
   int nf, nv;              // faces / vertices of the mesh it was built over
   int cap_f;               // capacity (faces) of the device buffers below
   void* nodes;             // (nf-1) internal nodes, 64 B each: child AABBs + child refs
-  void* wide;              // 4-wide nodes, 128 B each, at the even-depth binary nodes' indices
+  void* wide;              // 4-wide nodes, 128 B each, at the indices of the binary nodes they replace
   void* tris;              // nf triangles in leaf order, 48 B each (v0, e1, e2 as float4)
   void* work;              // build scratch: morton keys/values, parents, flags
   void* aux;               // clustering scratch and tree of the large-mesh build
   size_t aux_cap;
   size_t work_cap;
+  int quality;             // build quality: -1 by mesh size, 0 Morton + fixed collapse,
+                           // 1 Morton + area collapse, 2 clustered (PLOC) + area collapse
   float root_lo[3], root_hi[3];
 };
 

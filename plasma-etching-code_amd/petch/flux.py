@@ -19,12 +19,18 @@ SEED_MASK = 0xFFFFFFFFFFFFFFFF
 
 
 class SurfaceMesh:
-    """The triangle surface of phi with everything the flux and extension stages need."""
+    """The triangle surface of phi with everything the flux and extension stages need.
 
-    def __init__(self, device):
+    rays_per_step (optional): the rays a step will trace on this surface. A clustered BVH
+    (slower build, faster traces) pays off from ~1000 rays per face; below that the fast
+    Morton build is used. Without it the BVH picks by mesh size."""
+
+    def __init__(self, device, rays_per_step=None):
         self.device = device
         self.bufs = ops.MeshBuffers(device)
         self.bvh = ops.Bvh(device)
+        self.rays_per_step = rays_per_step
+        self._quality = None
         self.nf = 0
         self.nv = 0
 
@@ -34,6 +40,11 @@ class SurfaceMesh:
         self.nv, self.nf = int(v.shape[0]), int(f.shape[0])
         self.centroids, self.areas, self.normals = ops.mesh_attrs(v, f)
         self.nbr = ops.face_adjacency(f, self.nv)
+        if self.rays_per_step is not None:
+            q = 2 if self.rays_per_step >= 1000 * max(self.nf, 1) else 0
+            if q != self._quality:
+                self.bvh.set_quality(q)
+                self._quality = q
         self.bvh.build(v, f)
         return self
 

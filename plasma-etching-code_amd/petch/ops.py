@@ -144,6 +144,11 @@ class Bvh:
         self.h = h
         self.nf = 0
 
+    def set_quality(self, quality):
+        """-1 by mesh size (default), 0 fastest build ... 2 fastest traces."""
+        self.ctx.call("petch_bvh_set_quality", self.h, int(quality))
+        return self
+
     def build(self, verts, faces):
         self.nf = int(faces.shape[0])
         self.ctx.call("petch_bvh_build", self.h, ptr(verts, f32, "verts"), ptr(faces, i32, "faces"),

@@ -338,7 +338,12 @@ class EtchLoop:
         self.mg = ops.MaskGeom(float(geo['sub_top']), float(mask_th), float(geo['Lx']),
                                float(geo['Ly']), float(geo['trench_width']), int(bool(geo['hole'])))
         self.chem = chem_params(par, flags)
-        self.meshes = [_flux.SurfaceMesh(self.dev), _flux.SurfaceMesh(self.dev)]
+        # the ray budget of a step picks the BVH build (clustered trees pay off at ~1000 rays
+        # per face): ions + F and O neutrals per fixed-point iteration
+        rays = None
+        if rays_per_point is None:
+            rays = int(n_ion) + 2 * int(n_neu) * max(1, int(par.get('n_fp', 1)))
+        self.meshes = [_flux.SurfaceMesh(self.dev, rays), _flux.SurfaceMesh(self.dev, rays)]
         self.prev_mesh = self.prev_bare = self.prev_cen = None
         self.pt_bvh = None if self.gpu_ws else ops.Bvh(self.dev)
         self.red = torch.zeros((2,), dtype=torch.int32, device=self.dev)

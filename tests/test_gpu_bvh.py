@@ -26,8 +26,12 @@ def _mesh(name):
     return on.marching_cubes(phi.astype(np.float32), 0.1) + ((0, 16.0),)
 
 
-@pytest.mark.parametrize("name", ["sphere", "hole56", "trench160"])
-def test_ray_queries_match_brute_force(name):
+@pytest.mark.parametrize("name,quality", [("sphere", -1), ("hole56", -1), ("trench160", -1),
+                                          ("hole56", 0), ("hole56", 1), ("hole56", 2),
+                                          ("sphere", 2), ("trench160", 0)])
+def test_ray_queries_match_brute_force(name, quality):
+    """Every build (Morton or clustered tree, fixed or area-driven 4-wide collapse) answers
+    closest-hit queries like brute force over all faces."""
     import torch
     from petch import ops
     v, f, (lo, hi) = _mesh(name)
@@ -39,7 +43,8 @@ def test_ray_queries_match_brute_force(name):
     d[: n // 4] = [0, 0, -1]                                  # axis-aligned rays too
     d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
     dev = _dev()
-    b = ops.Bvh(dev).build(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev))
+    b = ops.Bvh(dev).set_quality(quality).build(torch.from_numpy(v).to(dev),
+                                                 torch.from_numpy(f).to(dev))
     face, t = b.query_ray(torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev))
     torch.cuda.synchronize()
     assert b.status() == 0
