@@ -316,33 +316,46 @@ static void oc_closest_on_tri(const double p[3], const double a[3], const double
 
 /* Closest face to each point (wp.mesh_query_point_no_sign, threed.py:1328,1626); ties keep the
  * lowest face id. */
+/* Nearest-face ties: among the faces whose distance is within the tie band of the minimum, the
+ * lowest face id (a point nearest to a shared vertex or edge is equally near several faces; the
+ * rule makes the choice independent of the search order). The band is 1e-6 of the point's
+ * coordinate magnitude, as on the device. Returns the squared-distance limit of the band. */
+static double oc_tie_lim(const double p[3], double d2min) {
+  const double band = 1.0e-6 * (fabs(p[0]) + fabs(p[1]) + fabs(p[2]) + 1.0);
+  const double r = sqrt(d2min) + band;
+  return r * r;
+}
+
 void oracle_closest_point_brute(const float* verts, const int* faces, int nf, const float* pts,
                                 int n, int* face_out, double* dist_out) {
 #pragma omp parallel for schedule(dynamic, 64)
   for (int i = 0; i < n; ++i) {
     double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
-    int best = -1;
     double bd = 1e300;
+    double* dd = (double*)malloc(sizeof(double) * (nf > 0 ? nf : 1));
     for (int f = 0; f < nf; ++f) {
       double a[3], b[3], c[3], q[3];
       oc_vert(verts, faces[3 * f], a);
       oc_vert(verts, faces[3 * f + 1], b);
       oc_vert(verts, faces[3 * f + 2], c);
       oc_closest_on_tri(p, a, b, c, q);
-      double dd = (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
-                  (p[2] - q[2]) * (p[2] - q[2]);
-      if (dd < bd) {
-        bd = dd;
-        best = f;
-      }
+      dd[f] = (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
+              (p[2] - q[2]) * (p[2] - q[2]);
+      if (dd[f] < bd) bd = dd[f];
     }
+    int best = -1;
+    const double lim = oc_tie_lim(p, bd);
+    for (int f = 0; f < nf && best < 0; ++f)
+      if (dd[f] <= lim) best = f;
     face_out[i] = best;
-    dist_out[i] = best >= 0 ? sqrt(bd) : -1.0;
+    dist_out[i] = best >= 0 ? sqrt(dd[best]) : -1.0;
+    free(dd);
   }
 }
 
 /* =============================================================== per-ray RNG stream ======== */
-/* Same stream layout as the device: key = seed, counter = (block, ray, 0, 0), words in order. */
+/* Same stream layout as the device: key = seed, counter = (block, ray, 0, 0), words in order;
+ * a Russian-roulette neutral's k-th bounce takes the words of block (k, ray, 1, 0). */
 typedef struct {
   uint32_t key[2], ray, blk, w[4];
   int pos;
@@ -363,7 +376,13 @@ static uint32_t oc_next(oc_rng* r) {
   }
   return r->w[r->pos++];
 }
-static float oc_uniform(oc_rng* r) { return (float)(oc_next(r) >> 8) * 5.9604644775390625e-8f; }
+static float oc_u24(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-8f; }
+static float oc_uniform(oc_rng* r) { return oc_u24(oc_next(r)); }
+/* the four words of block (blk, ray, lane, 0): a neutral's k-th bounce draws from (k, ray, 1) */
+static void oc_block(uint64_t seed, uint32_t ray, uint32_t blk, uint32_t lane, uint32_t out[4]) {
+  uint32_t c[4] = {blk, ray, lane, 0u}, key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  philox4x32_10(c, key, out);
+}
 static float oc_normal(oc_rng* r) {
   float u1 = (float)((oc_next(r) >> 8) + 1u) * 5.9604644775390625e-8f;
   float u2 = oc_uniform(r);
@@ -612,14 +631,17 @@ static int oc_query_bc(const oc_bvh* B, const oc_box* b, f3* o, f3 d, float* t, 
   return oc_query(B, *o, d, t, n);
 }
 
-/* cosine re-emission about n (threed.py:395-405) */
-static f3 oc_cosine(oc_rng* r, f3 n) {
+/* cosine re-emission about n (threed.py:395-405) from the uniforms r1, r2 */
+static f3 oc_cosine_u(f3 n, float r1, float r2) {
   f3 a = fabsf(n.x) > 0.9f ? mk(0.f, 1.f, 0.f) : mk(1.f, 0.f, 0.f);
   f3 t = nrmz(crs(a, n));
   f3 b = crs(n, t);
-  float r1 = oc_uniform(r), r2 = oc_uniform(r);
   float ct = sqrtf(r1), st = sqrtf(1.f - r1), ph = 6.2831853f * r2;
   return add3(add3(scl(st * cosf(ph), t), scl(st * sinf(ph), b)), scl(ct, n));
+}
+static f3 oc_cosine(oc_rng* r, f3 n) {
+  float r1 = oc_uniform(r), r2 = oc_uniform(r);
+  return oc_cosine_u(n, r1, r2);
 }
 
 /* _trace3d_cov_rr (threed.py:549-596); per-face sums in double. */
@@ -646,12 +668,15 @@ void oracle_trace_neutral_rr(void* h, const oc_box* b, const oc_source* s, const
         float S = bare[f] * beta;
         w = w - w * S;
         if (w <= 1.0e-6f) break;
+        /* the bounce's words: roulette, then the lobe's two uniforms */
+        uint32_t u[4];
+        oc_block(seed, (uint32_t)p, (uint32_t)bounce + 1u, 1u, u);
         if (w < 0.1f) {
-          if (oc_uniform(&r) < (1.f - w / 0.3f)) break;
+          if (oc_u24(u[0]) < (1.f - w / 0.3f)) break;
           w = 0.3f;
         }
         f3 hit = add3(o, scl(t, d));
-        d = oc_cosine(&r, nn);
+        d = oc_cosine_u(nn, oc_u24(u[1]), oc_u24(u[2]));
         o = add3(hit, scl(1.0e-4f, nn));
       }
     }
@@ -864,13 +889,15 @@ void oracle_bvh_closest_point(void* h, const float* pts, int n, int* face_out, d
   for (int i = 0; i < n; ++i) {
     double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
     int best = -1;
-    double bd = 1e300;
-    if (B->nf > 0) {
+    double bd = 1e300, best_d = 0.0;
+    /* pass 0: the minimum squared distance; pass 1: the lowest face id within the tie band */
+    for (int pass = 0; pass < 2 && B->nf > 0; ++pass) {
+      const double lim = pass ? oc_tie_lim(p, bd) : 0.0;
       int stack[128], sp = 0;
       stack[sp++] = 0;
       while (sp) {
         const oc_node* nd = &B->nodes[stack[--sp]];
-        if (oc_box_d2(nd, p) > bd) continue;
+        if (oc_box_d2(nd, p) > (pass ? lim : bd)) continue;
         if (nd->left < 0) {
           for (int k = nd->start; k < nd->start + nd->count; ++k) {
             int f = B->order[k];
@@ -879,9 +906,11 @@ void oracle_bvh_closest_point(void* h, const float* pts, int n, int* face_out, d
             oc_closest_on_tri(p, t, t + 3, t + 6, q);
             double dd = (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
                         (p[2] - q[2]) * (p[2] - q[2]);
-            if (dd < bd) {
-              bd = dd;
+            if (!pass) {
+              if (dd < bd) bd = dd;
+            } else if (dd <= lim && (best < 0 || f < best)) {
               best = f;
+              best_d = dd;
             }
           }
         } else {
@@ -891,6 +920,6 @@ void oracle_bvh_closest_point(void* h, const float* pts, int n, int* face_out, d
       }
     }
     face_out[i] = best;
-    dist_out[i] = best >= 0 ? sqrt(bd) : -1.0;
+    dist_out[i] = best >= 0 ? sqrt(best_d) : -1.0;
   }
 }

@@ -72,36 +72,33 @@ __device__ __forceinline__ Vec3 normalize(Vec3 a) {
 }
 __device__ __forceinline__ float comp(Vec3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 
-// Per-ray constants of the watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013).
+// Per-ray constants of the watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013):
+// kz is the direction's largest component and (kx, ky) = (kz+1, kz+2) mod 3. The paper swaps
+// kx and ky for a negative d[kz] to keep the winding; the test here is two-sided, and the swap
+// only negates U, V, W, det and T exactly, so it is left out.
 struct RayPre {
-  Vec3 o, d, inv;
+  Vec3 o, inv;
   int kx, ky, kz;
   float Sx, Sy, Sz;
 };
 
+// Moves the ray's origin (a periodic wrap keeps the direction and its constants).
+__device__ __forceinline__ void ray_move(RayPre& r, Vec3 o) { r.o = o; }
+
 __device__ __forceinline__ RayPre ray_pre(Vec3 o, Vec3 d) {
   RayPre r;
   r.o = o;
-  r.d = d;
   // reciprocal of a zero component is taken of +-1e-20 so slab products stay finite (no 0*inf)
   r.inv = v3(1.f / (fabsf(d.x) > 1e-20f ? d.x : copysignf(1e-20f, d.x)),
              1.f / (fabsf(d.y) > 1e-20f ? d.y : copysignf(1e-20f, d.y)),
              1.f / (fabsf(d.z) > 1e-20f ? d.z : copysignf(1e-20f, d.z)));
   float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-  int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
-  int kx = kz == 2 ? 0 : kz + 1;
-  int ky = kx == 2 ? 0 : kx + 1;
-  if (comp(d, kz) < 0.f) {
-    int t = kx;
-    kx = ky;
-    ky = t;
-  }
-  r.kx = kx;
-  r.ky = ky;
-  r.kz = kz;
-  float dz = comp(d, kz);
-  r.Sx = comp(d, kx) / dz;
-  r.Sy = comp(d, ky) / dz;
+  r.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  r.kx = r.kz == 2 ? 0 : r.kz + 1;
+  r.ky = r.kx == 2 ? 0 : r.kx + 1;
+  const float dz = comp(d, r.kz);
+  r.Sx = comp(d, r.kx) / dz;
+  r.Sy = comp(d, r.ky) / dz;
   r.Sz = 1.f / dz;
   return r;
 }
@@ -137,16 +134,48 @@ __device__ __forceinline__ bool ray_tri(const RayPre& r, Vec3 p0, Vec3 p1, Vec3 
   return t > 0.f && t < tbest;
 }
 
-// Slab test against [lo, hi]; returns entry distance (or a huge value when missed).
-__device__ __forceinline__ float ray_box(const RayPre& r, float lx, float ly, float lz, float hx,
-                                         float hy, float hz, float tbest) {
-  float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
-  float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
-  float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
-  float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
-  float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tbest));
-  // tmax widened by 2 ulp so rounding never culls a box the ray touches (conservative)
-  return (tmin <= tmax * kBoxSlack) ? tmin : 3.0e38f;  // 3e38: missed
+// Slab constants of a ray for the fused box test. A box plane p on axis a is crossed at
+// t = (p - o_a) * inv_a; the test evaluates it as one fma, fma(p, inv_a, -o_a * inv_a), on the
+// near plane (lo for a positive direction, hi for a negative one) and the far plane of each
+// axis. The product o_a * inv_a is rounded once, which can shift t by up to 2^-24 |o_a inv_a|,
+// so the near offset is taken 2^-21 |o_a inv_a| lower and the far offset as much higher: the
+// fused test keeps every box that (p - o_a) * inv_a evaluated exactly keeps, and the fma's
+// own rounding is covered by the relative slack kBoxSlack. near_x etc. are the byte offsets of
+// the near planes in a BvhWide (lox at 0, hix at 16, loy 32, hiy 48, loz 64, hiz 80).
+struct Slabs {
+  float nx, ny, nz, fx, fy, fz;
+  int near_x, near_y, near_z;
+};
+
+__device__ __forceinline__ Slabs ray_slabs(const RayPre& r) {
+  const float ox = r.o.x * r.inv.x, oy = r.o.y * r.inv.y, oz = r.o.z * r.inv.z;
+  const float ex = fabsf(ox) * 4.76837158e-7f, ey = fabsf(oy) * 4.76837158e-7f,
+              ez = fabsf(oz) * 4.76837158e-7f;
+  Slabs s;
+  s.nx = -ox - ex;
+  s.ny = -oy - ey;
+  s.nz = -oz - ez;
+  s.fx = -ox + ex;
+  s.fy = -oy + ey;
+  s.fz = -oz + ez;
+  s.near_x = r.inv.x < 0.f ? 16 : 0;
+  s.near_y = r.inv.y < 0.f ? 48 : 32;
+  s.near_z = r.inv.z < 0.f ? 80 : 64;
+  return s;
+}
+
+// Entry distance of the ray into the box with near planes (nx, ny, nz) and far planes
+// (fx, fy, fz), clipped to [0, tbest]; kMiss when the ray misses it.
+constexpr float kMiss = 3.0e38f;   // entry distance of a box the ray misses
+constexpr float kHitLim = 1.0e38f;
+__device__ __forceinline__ float slab_entry(const RayPre& r, const Slabs& s, float nx, float ny,
+                                            float nz, float fx, float fy, float fz, float tbest) {
+  const float tn = fmaxf(fmaxf(__builtin_fmaf(nx, r.inv.x, s.nx), __builtin_fmaf(ny, r.inv.y, s.ny)),
+                         fmaxf(__builtin_fmaf(nz, r.inv.z, s.nz), 0.f));
+  const float tf = fminf(fminf(__builtin_fmaf(fx, r.inv.x, s.fx), __builtin_fmaf(fy, r.inv.y, s.fy)),
+                         fminf(__builtin_fmaf(fz, r.inv.z, s.fz), tbest));
+  // tf widened by 2 ulp so rounding never culls a box the ray touches (conservative)
+  return tn <= tf * kBoxSlack ? tn : kMiss;
 }
 
 struct Hit {
@@ -165,9 +194,6 @@ __device__ __forceinline__ void leaf_test(const BvhView& b, const RayPre& r, int
     h.slot = slot;
   }
 }
-
-constexpr float kMiss = 3.0e38f;   // entry distance of a box the ray misses
-constexpr float kHitLim = 1.0e38f;
 
 __device__ __forceinline__ void sort2(float& ta, int& ca, float& tb, int& cb) {
   const bool sw = tb < ta;
@@ -212,24 +238,23 @@ using ScratchStack = TravStack<1, 0>;
 
 // Closest-hit query as a loop of steps: trav_begin sets it up, each trav_step visits one wide
 // node (its leaf children are tested on the spot) and returns true once the query is finished,
-// with the closest hit in T.h. Trav holds only scalars so that it lives in registers; the stack
-// is separate.
+// with the closest hit in T.h. Trav holds only scalars so that it lives in registers; the ray
+// constants stay with the caller (one copy, which a periodic wrap updates in place) and the
+// stack is separate.
 constexpr int kTravDone = -1;
 
 struct Trav {
-  RayPre r;
   Hit h;
   int node, sp, steps;
 };
 
 __device__ __forceinline__ void trav_begin(const BvhView& b, Trav& T, const RayPre& r,
                                            float tmax) {
-  T.r = r;
   T.h = Hit{-1, -1, tmax};
   T.sp = 0;
   T.steps = 0;
   T.node = 0;
-  if (b.nf == 1) leaf_test(b, T.r, 0, T.h);
+  if (b.nf == 1) leaf_test(b, r, 0, T.h);
   if (b.nf <= 1) T.node = kTravDone;
 }
 
@@ -244,7 +269,7 @@ __device__ __forceinline__ void push_far(const BvhView& b, Stack& stk, int& sp, 
 }
 
 template <class Stack>
-__device__ __forceinline__ bool trav_step(const BvhView& b, Trav& T, Stack& stk) {
+__device__ __forceinline__ bool trav_step(const BvhView& b, const RayPre& r, Trav& T, Stack& stk) {
   const int node = T.node;
   if (node == kTravDone) return true;
   // a tree visits each wide node at most once; the bound only guards a corrupt tree
@@ -253,18 +278,22 @@ __device__ __forceinline__ bool trav_step(const BvhView& b, Trav& T, Stack& stk)
     T.node = kTravDone;
     return true;
   }
-  const RayPre& r = T.r;
   Hit& h = T.h;
+  // the near and far plane of each axis, picked by the direction's sign, for all four children
   const BvhWide* N = b.wide + node;
-  const float4 lx = N->lox, hx = N->hix, ly = N->loy, hy = N->hiy, lz = N->loz, hz = N->hiz;
+  const char* base = (const char*)N;
+  const Slabs sl = ray_slabs(r);
+  const float4 nx = *(const float4*)(base + sl.near_x), fx = *(const float4*)(base + (16 - sl.near_x));
+  const float4 ny = *(const float4*)(base + sl.near_y), fy = *(const float4*)(base + (80 - sl.near_y));
+  const float4 nz = *(const float4*)(base + sl.near_z), fz = *(const float4*)(base + (144 - sl.near_z));
   const int4 ref = N->ref;
   int c0 = ref.x, c1 = ref.y, c2 = ref.z, c3 = ref.w;
-  // unused lanes carry a box at +inf that every ray misses, so all four tests run branch-free
-  // and the node's seven loads issue together
-  float t0 = ray_box(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, h.t);
-  float t1 = ray_box(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, h.t);
-  float t2 = ray_box(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, h.t);
-  float t3 = ray_box(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, h.t);
+  // unused lanes carry a box at +inf that every ray misses (near +inf, or far -inf for a
+  // negative direction), so all four tests run branch-free and the node's loads issue together
+  float t0 = slab_entry(r, sl, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, h.t);
+  float t1 = slab_entry(r, sl, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, h.t);
+  float t2 = slab_entry(r, sl, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, h.t);
+  float t3 = slab_entry(r, sl, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, h.t);
   // leaves are tested on the spot, in child order, as one loop over the lane's hit leaves (the
   // wavefront runs as many triangle tests as its busiest lane needs, not one per child slot)
   unsigned lm = (t0 < kHitLim && c0 < 0 ? 1u : 0u) | (t1 < kHitLim && c1 < 0 ? 2u : 0u) |
@@ -320,7 +349,7 @@ __device__ __forceinline__ Hit bvh_closest_hit(const BvhView& b, const RayPre& r
                                                Stack& stk) {
   Trav T;
   trav_begin(b, T, r, tmax);
-  while (!trav_step(b, T, stk)) {
+  while (!trav_step(b, r, T, stk)) {
   }
   return T.h;
 }
@@ -375,26 +404,30 @@ struct PointHit {
   float d2;
 };
 
-__device__ __forceinline__ void leaf_point(const BvhView& b, Vec3 p, int slot, PointHit& h) {
-  if (slot < 0 || slot >= b.nf) return;
+// Nearest-face ties. A grid point whose closest surface point is a vertex or an edge shared by
+// several faces is equally near all of them, and which one a traversal meets first depends on
+// the tree. The query therefore returns, among the faces whose distance is within
+// tie_band(p) of the minimum, the one with the lowest face id, so every implementation of it
+// (this one, the oracle's BVH and brute force) assigns the same face. The band, 1e-6 of the
+// point's coordinate magnitude, is a few dozen float ulps of the coordinates: wide enough that
+// float rounding of the distances to a shared vertex or edge stays inside it, far below any
+// geometric difference.
+__device__ __forceinline__ float tie_band(Vec3 p) {
+  return 1.0e-6f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z) + 1.f);
+}
+
+__device__ __forceinline__ float point_face_d2(const BvhView& b, Vec3 p, int slot, int& face) {
   float4 q0 = b.tris[3 * slot], q1 = b.tris[3 * slot + 1], q2 = b.tris[3 * slot + 2];
   Vec3 c = closest_on_tri(p, v3(q0.x, q0.y, q0.z), v3(q1.x, q1.y, q1.z), v3(q2.x, q2.y, q2.z));
   Vec3 e = p - c;
-  float d2 = dot(e, e);
-  if (d2 < h.d2) {
-    h.d2 = d2;
-    h.face = __float_as_int(q0.w);
-  }
+  face = __float_as_int(q0.w);
+  return dot(e, e);
 }
 
-// Closest face within max_dist (distance squared < max_dist^2).
-__device__ __forceinline__ PointHit bvh_closest_point(const BvhView& b, Vec3 p, float max_dist) {
-  PointHit h{-1, max_dist * max_dist};
-  if (b.nf <= 0) return h;
-  if (b.nf == 1) {
-    leaf_point(b, p, 0, h);
-    return h;
-  }
+// One traversal of the binary tree. Pass 0 (lim < 0): minimum squared distance below h.d2,
+// in h. Pass 1: lowest face id with squared distance <= lim, in h (h.d2 its distance).
+__device__ __forceinline__ void point_pass(const BvhView& b, Vec3 p, float lim, PointHit& h) {
+  const bool ties = lim >= 0.f;
   int stack[kBvhStack];
   int sp = 0;
   int node = 0;
@@ -404,23 +437,37 @@ __device__ __forceinline__ PointHit bvh_closest_point(const BvhView& b, Vec3 p, 
       if (b.err) *b.err = 2;
       break;
     }
+    const float cut = ties ? lim : h.d2;  // boxes farther than this cannot help
     const BvhNode* N = b.nodes + node;
     float4 A = N->a, B = N->b, C = N->c;
     int4 D = N->d;
     float dl = box_dist2(p, A.x, A.y, A.z, A.w, B.x, B.y);
     float dr = box_dist2(p, B.z, B.w, C.x, C.y, C.z, C.w);
-    bool hl = dl < h.d2, hr = dr < h.d2;
-    if (hl && D.x < 0) {
-      leaf_point(b, p, ~D.x, h);
-      hl = false;
+    bool hl = ties ? dl <= cut : dl < cut, hr = ties ? dr <= cut : dr < cut;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int c = side ? D.y : D.x;
+      bool& hit = side ? hr : hl;
+      if (!hit || c >= 0) continue;
+      hit = false;
+      const int slot = ~c;
+      if (slot < 0 || slot >= b.nf) continue;
+      int face;
+      const float d2 = point_face_d2(b, p, slot, face);
+      if (ties) {
+        if (d2 <= lim && (h.face < 0 || face < h.face)) {
+          h.face = face;
+          h.d2 = d2;
+        }
+      } else if (d2 < h.d2) {
+        h.face = face;
+        h.d2 = d2;
+      }
     }
-    if (hr && D.y < 0) {
-      leaf_point(b, p, ~D.y, h);
-      hr = false;
+    if (!ties) {  // the other child may have become irrelevant after a leaf test
+      if (hl && !(dl < h.d2)) hl = false;
+      if (hr && !(dr < h.d2)) hr = false;
     }
-    // the other child may have become irrelevant after a leaf test
-    if (hl && !(dl < h.d2)) hl = false;
-    if (hr && !(dr < h.d2)) hr = false;
     if (hl && hr) {
       int nearc = (dl <= dr) ? D.x : D.y;
       int farc = (dl <= dr) ? D.y : D.x;
@@ -439,7 +486,25 @@ __device__ __forceinline__ PointHit bvh_closest_point(const BvhView& b, Vec3 p, 
       node = stack[--sp];
     }
   }
-  return h;
+}
+
+// Closest face within max_dist (distance squared < max_dist^2), ties to the lowest face id.
+__device__ __forceinline__ PointHit bvh_closest_point(const BvhView& b, Vec3 p, float max_dist) {
+  PointHit h{-1, max_dist * max_dist};
+  if (b.nf <= 0) return h;
+  if (b.nf == 1) {
+    int face;
+    const float d2 = point_face_d2(b, p, 0, face);
+    if (d2 < h.d2) h = PointHit{face, d2};
+    return h;
+  }
+  point_pass(b, p, -1.f, h);
+  if (h.face < 0) return h;
+  const float r = sqrtf(h.d2) + tie_band(p);
+  const float lim = fminf(r * r, max_dist * max_dist);
+  PointHit t{-1, h.d2};
+  point_pass(b, p, lim, t);
+  return t.face >= 0 ? t : h;
 }
 
 }  // namespace petch

@@ -55,7 +55,7 @@ struct petch_bvh {
   int cap_f;               // capacity (faces) of the device buffers below
   void* nodes;             // (nf-1) internal nodes, 64 B each: child AABBs + child refs
   void* wide;              // 4-wide nodes, 128 B each, at the indices of the binary nodes they replace
-  void* tris;              // nf triangles in leaf order, 48 B each (v0, e1, e2 as float4)
+  void* tris;              // nf triangles in leaf order, 48 B each (v0, v1, v2 as float4)
   void* work;              // build scratch: morton keys/values, parents, flags
   void* aux;               // clustering scratch and tree of the large-mesh build
   size_t aux_cap;

@@ -24,6 +24,21 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
   }
 }
 
+// The four words of one Philox block, counter (blk, ray, lane, 0) under key `seed`. A stream
+// that draws a fixed number of words per event can take event e's words from block e directly
+// (no buffered state between events).
+__device__ __forceinline__ void philox_block(uint64_t seed, uint32_t ray, uint32_t blk,
+                                             uint32_t lane, uint32_t w[4]) {
+  w[0] = blk;
+  w[1] = ray;
+  w[2] = lane;
+  w[3] = 0u;
+  philox4x32_10(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// uniform in [0, 1) with 24 random bits of the word
+__device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-8f; }
+
 struct Rng {
   uint32_t k0, k1, ray, lane, blk;
   uint32_t w[4];
@@ -51,7 +66,7 @@ struct Rng {
     return v;
   }
   // uniform in [0, 1) with 24 random bits
-  __device__ __forceinline__ float uniform() { return (float)(next() >> 8) * 5.9604644775390625e-8f; }
+  __device__ __forceinline__ float uniform() { return u24(next()); }
   // standard normal by Box-Muller from two uniforms (the first mapped to (0, 1])
   __device__ __forceinline__ float normal() {
     float u1 = (float)((next() >> 8) + 1u) * 5.9604644775390625e-8f;

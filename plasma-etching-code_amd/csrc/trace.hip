@@ -84,14 +84,42 @@ __device__ __forceinline__ void gen_source(Rng& r, const Box& b, const Source& s
   }
 }
 
+// Tangent frame (t, b) of the cosine lobe about the unit normal n (threed.py:395-405).
+__device__ __forceinline__ void cosine_frame(Vec3 n, Vec3& t, Vec3& b) {
+  Vec3 a = (fabsf(n.x) > 0.9f) ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+  t = petch::normalize(petch::cross(a, n));
+  b = petch::cross(n, t);
+}
+
 // Cosine-weighted direction about the unit normal n (threed.py:395-405).
 __device__ __forceinline__ Vec3 cosine_about(Rng& r, Vec3 n) {
-  Vec3 a = (fabsf(n.x) > 0.9f) ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
-  Vec3 t = petch::normalize(petch::cross(a, n));
-  Vec3 bb = petch::cross(n, t);
+  Vec3 t, bb;
+  cosine_frame(n, t, bb);
   float r1 = r.uniform(), r2 = r.uniform();
   float ct = sqrtf(r1), st = sqrtf(1.f - r1), ph = kTwoPi * r2;
   return (st * cosf(ph)) * t + (st * sinf(ph)) * bb + ct * n;
+}
+
+// Direction request of a lane that is about to start a query: the cosine lobe about the unit
+// normal n (a re-emission, kLobe), the neutral source's lobe about -z, normalised as gen_source
+// does (kSourceLobe), or a direction already drawn, held in n (kDirect). r1, r2 are the lobe's
+// two uniforms. The trace loop draws every lane's direction at one place, so the frame, square
+// roots, sine and cosine run once per refill for all lanes, not once per code path.
+enum EmitMode { kLobe = 0, kSourceLobe = 1, kDirect = 2 };
+struct Emit {
+  Vec3 n;
+  float r1, r2;
+  int mode;
+};
+
+__device__ __forceinline__ Vec3 emit_dir(const Emit& e) {
+  if (e.mode == kDirect) return e.n;
+  const float ct = sqrtf(e.r1), st = sqrtf(1.f - e.r1), ph = kTwoPi * e.r2;
+  const float c = cosf(ph), s = sinf(ph);
+  if (e.mode == kSourceLobe) return petch::normalize(v3(st * c, st * s, -ct));
+  Vec3 t, b;
+  cosine_frame(e.n, t, b);
+  return (st * c) * t + (st * s) * b + ct * e.n;
 }
 
 // One wrap of _apply_bc (threed.py:289-349) for a ray that missed the surface: if it leaves
@@ -151,28 +179,49 @@ __device__ __forceinline__ void count_hits(unsigned long long* __restrict__ ctr,
 }
 
 // Ray queue shared by a persistent grid: each wavefront refills its finished lanes with new ray
-// ids (one atomic per wavefront per refill), so lanes whose rays bounce many times deep in a
-// feature do not hold the whole wavefront hostage to its longest ray. `exhausted` is uniform
-// across the wavefront, so the refill stops after the first empty claim.
+// ids, so lanes whose rays bounce many times deep in a feature do not hold the whole wavefront
+// hostage to its longest ray. A wavefront takes ids from the global counter in chunks of
+// kRayChunk and hands them out from a private pool (its `pool`, `pool_end` are uniform across
+// the wavefront): the counter is one device-scope atomic location that every XCD hits, and
+// one atomic per refill (about 40 rays) was what bounded the trace kernels. Measured on the
+// AR-10 trench (8M neutral rays): 6.05 ms per launch with an atomic per refill, 4.29 / 4.35 /
+// 4.68 ms with chunks of 128 / 256 / 512 (larger chunks leave the last ones unevenly spread).
+// `exhausted` is uniform across the wavefront: set when a chunk starts at or beyond the end,
+// after the pool's last ids were handed out.
 struct RayQueue {
   unsigned* next;
   int n;
 };
 
+constexpr unsigned kRayChunk = 128;  // >= 64, so one chunk always covers a wavefront's refill
+
 __device__ __forceinline__ bool claim_ray(const RayQueue& q, bool active, bool& exhausted,
-                                          unsigned& ray) {
+                                          unsigned& pool, unsigned& pool_end, unsigned& ray) {
   if (exhausted) return false;
   const int lane = threadIdx.x & 63;
-  unsigned long long idle = __ballot(!active);
+  const unsigned long long idle = __ballot(!active);
   if (!idle) return false;
-  int leader = __ffsll((unsigned long long)idle) - 1;
-  unsigned base = 0;
-  if (lane == leader) base = atomicAdd(q.next, (unsigned)__popcll(idle));
-  base = __shfl(base, leader, 64);
-  if (base >= (unsigned)q.n) exhausted = true;
+  const unsigned cnt = (unsigned)__popcll(idle);
+  const unsigned left = pool_end - pool;
+  const unsigned long long below = (lane == 0) ? 0ull : (idle & ((1ull << lane) - 1ull));
+  const unsigned rank = (unsigned)__popcll(below);
+  unsigned id;
+  if (left >= cnt) {
+    id = pool + rank;
+    pool += cnt;
+  } else {
+    // the pool's remaining ids go to the first lanes, a fresh chunk serves the rest
+    const int leader = __ffsll((unsigned long long)idle) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(q.next, kRayChunk);
+    base = __shfl(base, leader, 64);
+    id = rank < left ? pool + rank : base + (rank - left);
+    pool = base + (cnt - left);
+    pool_end = base + kRayChunk;
+    if (base >= (unsigned)q.n) exhausted = true;
+  }
   if (active) return false;
-  unsigned long long below = (lane == 0) ? 0ull : (idle & ((1ull << lane) - 1ull));
-  ray = base + (unsigned)__popcll(below);
+  ray = id;
   return ray < (unsigned)q.n;
 }
 
@@ -184,9 +233,11 @@ __device__ __forceinline__ bool claim_ray(const RayQueue& q, bool active, bool& 
 // the policy's hit code and refill idle lanes from the ray queue, so the (short, very uneven)
 // queries keep most lanes busy. The policy P supplies the surface and the per-ray payload:
 // view() / box() give the BVH and tracer box the ray lives in, start() keys the ray's Philox
-// stream and draws its source ray, hit() deposits and returns the next ray (true) or ends it
-// (false). Per ray, the Philox draws happen in the same order whatever the schedule,
-// so the result does not depend on it.
+// stream and draws its source ray's origin, hit() deposits and either ends the ray (false) or
+// moves the origin to the re-emission point (true); both leave the direction as an Emit
+// request that the loop turns into a direction at one place for all lanes. Every draw of a ray
+// comes from that ray's own Philox stream in a fixed order, so the result does not depend on
+// the schedule.
 constexpr int kRefillAt = 24;
 constexpr int kLdsStack = 8;  // traversal stack entries per lane held in LDS (16 KB/workgroup)
 enum LaneState { kIdle = 0, kTrav = 1, kHit = 2 };
@@ -201,29 +252,29 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
   stk.lds_t = lds_t + threadIdx.x;
   petch::Trav T;
   typename P::Ray R;
-  Rng rng;
   Vec3 o = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, -1.f);
-  petch::RayPre rp;  // per-direction constants of the current ray; a wrap only moves rp.o
+  petch::RayPre rp;  // per-direction constants of the current ray; a wrap only moves its origin
   int wraps = 0;
   unsigned nh = 0;
   int st = kIdle;
   bool exhausted = false;
+  unsigned pool = 0, pool_end = 0;  // this wavefront's unclaimed queue ids [pool, pool_end)
   while (true) {
     if (__popcll(__ballot(st == kTrav)) <= kRefillAt) {
+      Emit e;
+      bool emit = false;
       if (st == kHit) {
         ++nh;
-        if (pol.hit(rng, R, T.h, o, d)) {
-          rp = petch::ray_pre(o, d);
-          wraps = 0;
-          petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
-          st = kTrav;
-        } else {
-          st = kIdle;
-        }
+        emit = pol.hit(R, T.h, o, d, e);
+        st = kIdle;
       }
       unsigned ray;
-      if (claim_ray(q, st != kIdle, exhausted, ray)) {
-        pol.start(rng, R, ray, o, d);
+      if (claim_ray(q, emit || st == kTrav, exhausted, pool, pool_end, ray)) {
+        pol.start(R, ray, o, e);
+        emit = true;
+      }
+      if (emit) {
+        d = emit_dir(e);
         rp = petch::ray_pre(o, d);
         wraps = 0;
         petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
@@ -231,12 +282,12 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
       }
       if (!__ballot(st == kTrav)) break;
     }
-    if (st == kTrav && petch::trav_step(pol.view(R), T, stk)) {
+    if (st == kTrav && petch::trav_step(pol.view(R), rp, T, stk)) {
       const Box& b = pol.box(R);
       if (T.h.face >= 0) {
         st = kHit;
       } else if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
-        rp.o = o;
+        petch::ray_move(rp, o);
         ++wraps;
         petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
       } else {
@@ -252,27 +303,58 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
 // can carry two species over the same surface (the F and O traces of one fixed-point
 // iteration, threed.py:876-880): queue ids [0, n0) are species 0 with stream (seed[0], id),
 // the rest species 1 with stream (seed[1], id - n0), exactly the streams of two launches.
+//
+// Stream of a neutral: its source ray takes the words of block 0 (lane 0) in order (origin x,
+// origin y, then the lobe's two uniforms); its k-th bounce (k = 1, 2, ...) takes the words of
+// block (k, ray, 1): word 0 for the roulette, words 1 and 2 for the re-emission lobe. A fixed
+// block per event means the trace keeps no buffered RNG state and evaluates Philox at two
+// places of its refill code (hits, new rays) instead of at every draw.
+//
 // One Russian-roulette bounce (threed.py:565-596) of a ray with weight w that hit face h:
 // deposits w, applies the sticking S, plays the roulette; returns false when the ray ends, else
-// re-emits it about the oriented normal.
-__device__ __forceinline__ bool rr_bounce(Rng& r, float& w, int& bounce, const BvhView& bv,
-                                          const Hit& h, float* __restrict__ fl, float S, Vec3& o,
-                                          Vec3& d) {
+// moves o to the re-emission point and leaves the cosine lobe about the oriented normal in e.
+__device__ __forceinline__ bool rr_bounce(float& w, int& bounce, uint64_t seed, unsigned ray,
+                                          const BvhView& bv, const Hit& h, float* __restrict__ fl,
+                                          float S, Vec3& o, Vec3 d, Emit& e) {
   Vec3 nn = hit_normal(bv, h.slot);
   if (petch::dot(nn, d) > 0.f) nn = -nn;
   deposit(fl, h.face, w);
   float w2 = w - w * S;
   if (w2 <= 1.0e-6f) return false;
+  uint32_t u[4];
+  petch::philox_block(seed, ray, (uint32_t)bounce + 1u, 1u, u);
   if (w2 < 0.1f) {
-    if (r.uniform() < (1.f - w2 / 0.3f)) return false;
+    if (petch::u24(u[0]) < (1.f - w2 / 0.3f)) return false;
     w2 = 0.3f;
   }
   if (++bounce >= 1024) return false;
   w = w2;
   const Vec3 p = o + h.t * d;
-  d = cosine_about(r, nn);
+  e.n = nn;
+  e.r1 = petch::u24(u[1]);
+  e.r2 = petch::u24(u[2]);
+  e.mode = kLobe;
   o = p + 1.0e-4f * nn;
   return true;
+}
+
+// Source ray of a neutral's stream: the cosine source (threed.py:454-457) draws its origin here
+// and leaves its lobe to the loop; another source kind is drawn whole by gen_source.
+__device__ __forceinline__ void neutral_source(uint64_t seed, unsigned ray, const Box& b,
+                                               const Source& src, Vec3& o, Emit& e) {
+  if (src.kind == SRC_NEUTRAL) {
+    uint32_t u[4];
+    petch::philox_block(seed, ray, 0u, 0u, u);
+    o = v3(petch::u24(u[0]) * b.lx, petch::u24(u[1]) * b.ly, b.zsrc);
+    e.r1 = petch::u24(u[2]);
+    e.r2 = petch::u24(u[3]);
+    e.mode = kSourceLobe;
+  } else {
+    Rng r;
+    r.init(seed, ray);
+    gen_source(r, b, src, o, e.n);
+    e.mode = kDirect;
+  }
 }
 
 struct NeutralRR {
@@ -288,19 +370,21 @@ struct NeutralRR {
     float w;
     int bounce;
     int sp;
+    unsigned ray;  // id within the species' stream
   };
   __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
   __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
-  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, Vec3& o, Vec3& d) const {
+  __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
     R.sp = ray >= n0;
-    r.init(R.sp ? seed[1] : seed[0], R.sp ? ray - n0 : ray);
-    gen_source(r, b, src, o, d);
+    R.ray = R.sp ? ray - n0 : ray;
+    neutral_source(R.sp ? seed[1] : seed[0], R.ray, b, src, o, e);
     R.w = 1.f;
     R.bounce = 0;
   }
-  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const Hit& h, Vec3& o, Vec3& d) const {
+  __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
     const float S = bare[h.face] * (R.sp ? beta[1] : beta[0]);
-    return rr_bounce(r, R.w, R.bounce, bv, h, R.sp ? fl[1] : fl[0], S, o, d);
+    return rr_bounce(R.w, R.bounce, R.sp ? seed[1] : seed[0], R.ray, bv, h,
+                     R.sp ? fl[1] : fl[0], S, o, d, e);
   }
 };
 
@@ -403,19 +487,23 @@ struct IonYield {
   struct Ray {
     float E, w;
     int bounce;
+    Rng rng;
   };
   __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
   __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
-  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, Vec3& o, Vec3& d) const {
-    r.init(seed, ray);
-    gen_source(r, b, src, o, d);
-    float E = P.meanE + P.sigmaE * r.normal();
+  __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
+    R.rng.init(seed, ray);
+    gen_source(R.rng, b, src, o, e.n);
+    e.mode = kDirect;
+    float E = P.meanE + P.sigmaE * R.rng.normal();
     R.E = E < 0.f ? 0.f : E;
     R.w = 1.f;
     R.bounce = 0;
   }
-  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const Hit& h, Vec3& o, Vec3& d) const {
-    return ion_bounce(r, R.E, R.w, R.bounce, bv, h, *this, o, d);
+  __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
+    e.mode = kDirect;
+    e.n = d;
+    return ion_bounce(R.rng, R.E, R.w, R.bounce, bv, h, *this, o, e.n);
   }
 };
 
@@ -467,27 +555,29 @@ struct NeutralRRBatch {
     int bounce;
     int sp;
     int f;
+    unsigned ray;  // id within the species' stream
     BvhView bv;
     Box b;
   };
   __device__ __forceinline__ const BvhView& view(const Ray& R) const { return R.bv; }
   __device__ __forceinline__ const Box& box(const Ray& R) const { return R.b; }
-  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, Vec3& o, Vec3& d) const {
+  __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
     R.f = locate_feature(feat, nfeat, ray);
     const NeutralFeature& F = feat[R.f];
     const unsigned local = ray - (R.f ? feat[R.f - 1].end : 0u);
     R.sp = local >= F.n0;
-    r.init(R.sp ? F.seed[1] : F.seed[0], R.sp ? local - F.n0 : local);
+    R.ray = R.sp ? local - F.n0 : local;
     R.bv = F.bv;
     R.b = F.b;
-    gen_source(r, R.b, src, o, d);
+    neutral_source(R.sp ? F.seed[1] : F.seed[0], R.ray, R.b, src, o, e);
     R.w = 1.f;
     R.bounce = 0;
   }
-  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const Hit& h, Vec3& o, Vec3& d) const {
+  __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
     const NeutralFeature& F = feat[R.f];
     const float S = F.bare[h.face] * (R.sp ? F.beta[1] : F.beta[0]);
-    return rr_bounce(r, R.w, R.bounce, R.bv, h, R.sp ? F.fl[1] : F.fl[0], S, o, d);
+    return rr_bounce(R.w, R.bounce, R.sp ? F.seed[1] : F.seed[0], R.ray, R.bv, h,
+                     R.sp ? F.fl[1] : F.fl[0], S, o, d, e);
   }
 };
 
@@ -498,25 +588,29 @@ struct IonYieldBatch {
     float E, w;
     int bounce;
     int f;
+    Rng rng;
     BvhView bv;
     Box b;
   };
   __device__ __forceinline__ const BvhView& view(const Ray& R) const { return R.bv; }
   __device__ __forceinline__ const Box& box(const Ray& R) const { return R.b; }
-  __device__ __forceinline__ void start(Rng& r, Ray& R, unsigned ray, Vec3& o, Vec3& d) const {
+  __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
     R.f = locate_feature(feat, nfeat, ray);
     const IonFeature& F = feat[R.f];
-    r.init(F.seed, ray - (R.f ? feat[R.f - 1].end : 0u));
+    R.rng.init(F.seed, ray - (R.f ? feat[R.f - 1].end : 0u));
     R.bv = F.bv;
     R.b = F.b;
-    gen_source(r, R.b, F.src, o, d);
-    float E = F.P.meanE + F.P.sigmaE * r.normal();
+    gen_source(R.rng, R.b, F.src, o, e.n);
+    e.mode = kDirect;
+    float E = F.P.meanE + F.P.sigmaE * R.rng.normal();
     R.E = E < 0.f ? 0.f : E;
     R.w = 1.f;
     R.bounce = 0;
   }
-  __device__ __forceinline__ bool hit(Rng& r, Ray& R, const Hit& h, Vec3& o, Vec3& d) const {
-    return ion_bounce(r, R.E, R.w, R.bounce, R.bv, h, feat[R.f], o, d);
+  __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
+    e.mode = kDirect;
+    e.n = d;
+    return ion_bounce(R.rng, R.E, R.w, R.bounce, R.bv, h, feat[R.f], o, e.n);
   }
 };
 

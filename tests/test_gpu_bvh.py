@@ -81,9 +81,10 @@ def test_point_queries_match_brute_force(name):
     rf, rd = on.closest_point_brute(v, f, p)
     assert (face >= 0).all()
     np.testing.assert_allclose(dist, rd, rtol=1e-4, atol=2e-6)
-    # a different face than brute force's lowest-id choice only at (near) ties in distance,
-    # which the distance check above already bounds
-    assert (face == rf).mean() > 0.9
+    # both sides send nearest-face ties (shared edges and vertices) to the lowest face id; they
+    # differ only where float and double rounding put a face on opposite sides of the tie band,
+    # which for random points happens beside almost coplanar neighbours (a few per thousand)
+    assert (face == rf).mean() > 0.99
 
 
 def test_tiny_meshes():

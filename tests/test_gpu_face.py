@@ -172,9 +172,11 @@ def test_extend_and_warm_start_nearest_face():
     np.testing.assert_allclose(gd, dist_o, rtol=1e-4, atol=2e-6)
     np.testing.assert_array_equal(Fs[inb], V[gf])
     # a grid point whose closest surface point is a shared edge or vertex has several nearest
-    # faces; which one a BVH returns is traversal order. Where the two sides differ, the GPU's
-    # face must be at the oracle's minimum distance.
-    _assert_tie_only(v, f, pts, Fs[inb] != ref[inb], gf, dist_o)
+    # faces; both sides take the lowest face id among them, so the extensions agree except
+    # where float and double rounding straddle the tie band, and there only between tied faces
+    differ = Fs[inb] != ref[inb]
+    assert differ.mean() < 1e-3, differ.mean()
+    _assert_tie_only(v, f, pts, differ, gf, dist_o)
     # warm start: nearest face of the previous mesh for each new centroid
     cen = om.mesh_attrs(v, f)[0].astype(np.float32) + 0.03
     prev_bare = np.random.default_rng(6).uniform(0, 1, len(f)).astype(np.float32)
@@ -183,6 +185,7 @@ def test_extend_and_warm_start_nearest_face():
     gf2, _ = bvh.query_point(_t(cen))
     gf2 = gf2.cpu().numpy()
     np.testing.assert_array_equal(ws, prev_bare[gf2])
+    assert (gf2 != fo).mean() < 1e-3
     _assert_tie_only(v, f, cen, ws != prev_bare[fo], gf2, do)
 
 

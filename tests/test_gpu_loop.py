@@ -135,37 +135,47 @@ def _floor_depth(phi, dx, sub_top):
 
 @pytest.mark.parametrize("rate_scale", [0.1, 0.01])
 def test_smoke_loop_oracle_parity(rate_scale):
-    """The smoke loop against the oracle's CPU run of the same Philox streams: final max and
-    center depths agree within 3x the spread of independent seeds. Both depths are whole grid
-    cells, so the spread is floored at dx / 3 (one cell at 3 sigma). At rate_scale 0.1 the
-    trench reaches the bottom of the domain on both sides; at 0.01 it stops mid-way, and there
-    the interpolated floor depth (not quantised) must agree within 3x its seed spread too."""
+    """The smoke loop's depth distribution against the oracle's: six seeds on each side.
+
+    Both sides draw the same Philox streams and per step agree closely (same fluxes to float
+    rounding, identical nearest-face extension), but 20 coupled steps amplify the float-atomic
+    and float32-vs-float64 differences, so single runs of one seed end a cell or so apart; the
+    test compares the seed distributions instead. The interpolated floor depth (not quantised)
+    must agree in mean within 3 standard errors of the difference (each side's spread floored
+    at 0.02 dx); the max and center depths are whole grid cells, so their means must agree
+    within one cell. At rate_scale 0.1 the trench reaches the bottom of the domain; at 0.01 it
+    stops mid-way."""
     from petch import threed as t3
     par = dict(PAR)
     par['rate_scale'] = rate_scale
     par['periodic_y'] = 1
-    geo = t3.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000,
-                         n_neu=8000, reinit_method="fsm", verbose=False)
-    md, cd = t3.max_depth_3d(geo), t3.center_depth_3d(geo)
-    fd = _floor_depth(geo['phi'], SMOKE['dx'], SMOKE['sub_top'])
-    og = ofl.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000, n_neu=8000)
-    omd, ocd = ol.max_depth_3d(og), ol.depth3d(og)
-    ofd = _floor_depth(og['phi'], SMOKE['dx'], SMOKE['sub_top'])
-    spread, fspread = [], []
-    for s in (100, 200, 300):
+    dx = SMOKE['dx']
+    seeds = (0, 100, 200, 300, 400, 500)
+    g_md, g_cd, g_fd, o_md, o_cd, o_fd = [], [], [], [], [], []
+    for s in seeds:
         gg = t3.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000,
                             n_neu=8000, reinit_method="fsm", verbose=False, seed_offset=s)
-        spread.append(t3.max_depth_3d(gg))
-        fspread.append(_floor_depth(gg['phi'], SMOKE['dx'], SMOKE['sub_top']))
-    sd = max(np.std(spread), SMOKE['dx'] / 3)
-    band = np.abs(og['phi']) < 2 * SMOKE['dx']
-    dphi = float(np.abs(geo['phi'][band] - og['phi'][band]).max())
-    print(f"rate_scale {rate_scale}: gpu max {md:.3f} center {cd:.3f} floor {fd:.4f} | oracle "
-          f"max {omd:.3f} center {ocd:.3f} floor {ofd:.4f} | seed spread max {np.std(spread):.3f} "
-          f"floor {np.std(fspread):.4f} | max |dphi| near front {dphi:.3e}", flush=True)
-    assert abs(md - omd) <= 3 * sd and abs(cd - ocd) <= 3 * sd
-    if rate_scale < 0.05:
-        assert abs(fd - ofd) <= 3 * max(np.std(fspread), 0.02 * SMOKE['dx'])
+        g_md.append(t3.max_depth_3d(gg))
+        g_cd.append(t3.center_depth_3d(gg))
+        g_fd.append(_floor_depth(gg['phi'], dx, SMOKE['sub_top']))
+        og = ofl.run_etch_3d(**SMOKE, hole=False, par=dict(par), flags=FAITHFUL, n_ion=8000,
+                             n_neu=8000, seed_offset=s)
+        o_md.append(ol.max_depth_3d(og))
+        o_cd.append(ol.depth3d(og))
+        o_fd.append(_floor_depth(og['phi'], dx, SMOKE['sub_top']))
+    g_fd, o_fd = np.array(g_fd), np.array(o_fd)
+    ok = np.isfinite(g_fd) & np.isfinite(o_fd)
+    print(f"rate_scale {rate_scale}: gpu max {np.mean(g_md):.3f} center {np.mean(g_cd):.3f} "
+          f"floor {np.mean(g_fd[ok]):.4f}+-{np.std(g_fd[ok]):.4f} | oracle max {np.mean(o_md):.3f} "
+          f"center {np.mean(o_cd):.3f} floor {np.mean(o_fd[ok]):.4f}+-{np.std(o_fd[ok]):.4f}",
+          flush=True)
+    assert ok.sum() >= 4
+    sg = max(np.std(g_fd[ok], ddof=1), 0.02 * dx)
+    so = max(np.std(o_fd[ok], ddof=1), 0.02 * dx)
+    se = np.sqrt(sg ** 2 / ok.sum() + so ** 2 / ok.sum())
+    assert abs(np.mean(g_fd[ok]) - np.mean(o_fd[ok])) <= 3 * se
+    assert abs(np.mean(g_md) - np.mean(o_md)) <= dx
+    assert abs(np.mean(g_cd) - np.mean(o_cd)) <= dx
 
 
 def test_api_hole_runs_and_saves(tmp_path):
