@@ -229,6 +229,10 @@ struct TravStack {
     if (L > 0 && i < L) return lds_n[i * S];
     return spill_n[i - L];
   }
+  static constexpr int kLds = L;
+  // entry i of the LDS part (i < L)
+  __device__ __forceinline__ float lds_dist(int i) const { return lds_t[i * S]; }
+  __device__ __forceinline__ int lds_node(int i) const { return lds_n[i * S]; }
   __device__ __forceinline__ float dist(int i) const {
     if (L > 0 && i < L) return lds_t[i * S];
     return spill_t[i - L];
@@ -329,14 +333,23 @@ __device__ __forceinline__ bool trav_step(const BvhView& b, const RayPre& r, Tra
     T.node = c0;
     return false;
   }
-  while (sp > 0 && !(stk.dist(sp - 1) <= lim)) --sp;
+  // pop, dropping entries that start beyond the best hit; while every lane's stack top is in
+  // LDS (nearly always) the reads are LDS-only, otherwise they may reach the scratch part
+  if (Stack::kLds > 0 && !__any(sp > Stack::kLds)) {
+    while (sp > 0 && !(stk.lds_dist(sp - 1) <= lim)) --sp;
+  } else {
+    while (sp > 0 && !(stk.dist(sp - 1) <= lim)) --sp;
+  }
   if (sp == 0) {
     T.sp = 0;
     T.node = kTravDone;
     return true;
   }
   --sp;
-  T.node = stk.node(sp);
+  if (Stack::kLds > 0 && sp < Stack::kLds)
+    T.node = stk.lds_node(sp);
+  else
+    T.node = stk.node(sp);
   T.sp = sp;
   return false;
 }

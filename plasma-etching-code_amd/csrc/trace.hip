@@ -125,24 +125,34 @@ __device__ __forceinline__ Vec3 emit_dir(const Emit& e) {
 // One wrap of _apply_bc (threed.py:289-349) for a ray that missed the surface: if it leaves
 // through a lateral face of the box before the open top it re-enters at the opposite face with
 // the same direction (origin updated, returns true); otherwise it escapes (returns false).
-__device__ __forceinline__ bool wrap_ray(const Box& b, Vec3& o, Vec3 d) {
+// The exit face is chosen by comparing the distances to the three faces computed with the
+// direction's reciprocals inv (a ray's constants already hold them); only the chosen face's
+// distance is then divided exactly, as _apply_bc computes it, for the re-entry point. The
+// choice can differ from comparing three exact quotients only where two faces are reached
+// within an ulp of each other (a corner exit).
+__device__ __forceinline__ bool wrap_ray(const Box& b, Vec3& o, Vec3 d, Vec3 inv) {
   float tx = 1.0e30f, ty = 1.0e30f, tz = 1.0e30f;
-  if (d.x > 1.0e-9f) tx = (b.lx - o.x) / d.x;
-  if (d.x < -1.0e-9f) tx = (0.f - o.x) / d.x;
-  if (d.y > 1.0e-9f) ty = (b.ly - o.y) / d.y;
-  if (d.y < -1.0e-9f) ty = (0.f - o.y) / d.y;
-  if (d.z > 1.0e-9f) tz = (b.lz - o.z) / d.z;
+  const float ex = d.x > 0.f ? b.lx : 0.f, ey = d.y > 0.f ? b.ly : 0.f;
+  if (fabsf(d.x) > 1.0e-9f) tx = (ex - o.x) * inv.x;
+  if (fabsf(d.y) > 1.0e-9f) ty = (ey - o.y) * inv.y;
+  if (d.z > 1.0e-9f) tz = (b.lz - o.z) * inv.z;
   if (tx < tz && tx < ty && tx > 1.0e-7f) {
+    tx = (ex - o.x) / d.x;
     float hy = o.y + tx * d.y, hz = o.z + tx * d.z;
     o = v3(d.x < 0.f ? b.lx - 1.0e-4f : 1.0e-4f, hy, hz);
     return true;
   }
   if (ty < tz && ty < 1.0e29f && ty > 1.0e-7f) {
+    ty = (ey - o.y) / d.y;
     float hx = o.x + ty * d.x, hz = o.z + ty * d.z;
     o = v3(hx, d.y < 0.f ? b.ly - 1.0e-4f : 1.0e-4f, hz);
     return true;
   }
   return false;
+}
+
+__device__ __forceinline__ bool wrap_ray(const Box& b, Vec3& o, Vec3 d) {
+  return wrap_ray(b, o, d, v3(1.f / d.x, 1.f / d.y, 1.f / d.z));
 }
 
 constexpr int kMaxWraps = 256;
@@ -286,7 +296,7 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
       const Box& b = pol.box(R);
       if (T.h.face >= 0) {
         st = kHit;
-      } else if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d)) {
+      } else if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d, rp.inv)) {
         petch::ray_move(rp, o);
         ++wraps;
         petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
