@@ -289,6 +289,31 @@ int petch_extend(petch_ctx* ctx, const petch_bvh* bvh, const float* V, const flo
 int petch_nearest_face_value(petch_ctx* ctx, const petch_bvh* bvh, const float* src,
                              const float* pts, int n, float miss, float* out);
 
+/* petch_extend for many grids, each with its own surface, in one pass (the batched etch loop).
+ * No reference counterpart: the reference extends one feature per call. */
+typedef struct petch_extend_item {
+  const petch_bvh* bvh;
+  const float* V;
+  const float* phi;
+  float* Fs;
+  int nx, ny, nz;
+  double dx;
+  double band;
+} petch_extend_item;
+int petch_extend_batch(petch_ctx* ctx, int n, const petch_extend_item* items);
+
+/* petch_nearest_face_value for many point sets, each on its own surface, in one launch (the
+ * batched etch loop's warm starts). No reference counterpart. */
+typedef struct petch_nearest_item {
+  const petch_bvh* bvh;
+  const float* src;
+  const float* pts;
+  int n;
+  float miss;
+  float* out;
+} petch_nearest_item;
+int petch_nearest_face_value_batch(petch_ctx* ctx, int n, const petch_nearest_item* items);
+
 /* ---------------------------------------------------------------- a21: radiosity -------- */
 /* Form-factor assembly of mc_flux_3d_radiosity (threed.py:956-960) from the petch_trace_ff
  * output: D[j] = escapes of face j / n_ff (float64), and the rays j -> i as a CSR of rows i

@@ -8,10 +8,14 @@
 //                                                                       threed.py:1608-1616,1721-1728,1750
 //   petch_extend          Fs[node] = V[nearest face] in |phi| < band     threed.py:1324-1346
 //   petch_nearest_face_value  out[i] = src[nearest face of pts[i]]       threed.py:1619-1639
+//   petch_extend_batch, petch_nearest_face_value_batch: the same for many grids / point sets in
+//                         one pass (the batched etch loop)
 // The chemistry is evaluated in float64 (the reference evaluates it on the host in float64) and
 // rounded to float32 once.
 #include "common.h"
 #include "bvh.h"
+
+#include <vector>
 
 petch::BvhView petch_bvh_view(const petch_bvh* b);
 
@@ -181,32 +185,87 @@ __global__ void k_surface_rate(const float* __restrict__ m_i, const float* __res
   atomicMax(red, __float_as_uint(vf > 0.f ? vf : 0.f));
 }
 
-__global__ void k_extend(petch::BvhView bv, const float* __restrict__ V,
-                         const float* __restrict__ phi, int nx, int ny, int nz, double dx,
-                         double band, float* __restrict__ Fs) {
-  long long n = (long long)blockIdx.x * kBS + threadIdx.x;
-  long long N = (long long)nx * ny * nz;
-  if (n >= N) return;
-  float out = 0.f;
-  if (fabs((double)phi[n]) < band) {
-    int k = (int)(n % nz);
-    int j = (int)((n / nz) % ny);
-    int i = (int)(n / ((long long)nz * ny));
-    petch::Vec3 p = petch::v3((float)((double)i * dx), (float)((double)j * dx), (float)((double)k * dx));
-    petch::PointHit h = petch::bvh_closest_point(bv, p, 1.0e6f);
-    if (h.face >= 0) out = V[h.face];
+// Velocity extension of one or many grids in one pass: flag every grid's band (and zero its
+// Fs), scan the flags of all grids together, scatter the band nodes' global indices, then one
+// nearest-face query per listed node, each on its own grid's surface. The queries are
+// latency-bound chains of node fetches (a few tens of microseconds each), so a grid's
+// extension alone leaves the GPU mostly idle; the queries of many grids in one launch overlap.
+struct ExtendItem {
+  petch::BvhView bv;
+  const float* V;
+  const float* phi;
+  float* Fs;
+  int ny, nz;
+  double dx, band;
+  long long end;  // exclusive end of this grid's nodes in the global node numbering
+};
+
+__device__ __forceinline__ int locate_item(const ExtendItem* __restrict__ it, int n, long long g,
+                                           long long& local) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (it[mid].end > g) hi = mid; else lo = mid + 1;
   }
-  Fs[n] = out;
+  local = g - (lo ? it[lo - 1].end : 0);
+  return lo;
 }
 
-__global__ void k_nearest_value(petch::BvhView bv, const float* __restrict__ src,
-                                const float* __restrict__ pts, int n, float miss,
-                                float* __restrict__ out) {
-  int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n) return;
-  petch::PointHit h =
-      petch::bvh_closest_point(bv, petch::v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), 1.0e6f);
-  out[i] = h.face >= 0 ? src[h.face] : miss;
+__global__ void k_band_flags(const ExtendItem* __restrict__ it, int n, long long total,
+                             int* __restrict__ flag) {
+  const long long g = (long long)blockIdx.x * kBS + threadIdx.x;
+  if (g >= total) return;
+  long long m;
+  const ExtendItem& e = it[locate_item(it, n, g, m)];
+  flag[g] = fabs((double)e.phi[m]) < e.band ? 1 : 0;
+  e.Fs[m] = 0.f;
+}
+
+__global__ void k_band_compact(const int* __restrict__ flag, const int* __restrict__ pos,
+                               long long total, int* __restrict__ idx) {
+  const long long g = (long long)blockIdx.x * kBS + threadIdx.x;
+  if (g >= total) return;
+  if (flag[g]) idx[pos[g]] = (int)g;
+}
+
+__global__ void k_extend(const ExtendItem* __restrict__ it, int n, const int* __restrict__ idx,
+                         const int* __restrict__ count) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= *count) return;
+  long long m;
+  const ExtendItem& e = it[locate_item(it, n, idx[i], m)];
+  const int k = (int)(m % e.nz);
+  const int j = (int)((m / e.nz) % e.ny);
+  const int ii = (int)(m / ((long long)e.nz * e.ny));
+  const petch::Vec3 p = petch::v3((float)((double)ii * e.dx), (float)((double)j * e.dx),
+                                  (float)((double)k * e.dx));
+  const petch::PointHit h = petch::bvh_closest_point(e.bv, p, 1.0e6f);
+  e.Fs[m] = h.face >= 0 ? e.V[h.face] : 0.f;
+}
+
+// Nearest-face values of many point sets, each on its own surface, in one launch.
+struct NearestItem {
+  petch::BvhView bv;
+  const float* src;
+  const float* pts;
+  float* out;
+  float miss;
+  long long end;
+};
+
+__global__ void k_nearest_batch(const NearestItem* __restrict__ it, int n, long long total) {
+  const long long g = (long long)blockIdx.x * kBS + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (it[mid].end > g) hi = mid; else lo = mid + 1;
+  }
+  const NearestItem& e = it[lo];
+  const long long i = g - (lo ? it[lo - 1].end : 0);
+  const petch::PointHit h = petch::bvh_closest_point(
+      e.bv, petch::v3(e.pts[3 * i], e.pts[3 * i + 1], e.pts[3 * i + 2]), 1.0e6f);
+  e.out[i] = h.face >= 0 ? e.src[h.face] : e.miss;
 }
 
 Chem make_chem(const petch_chem_params* p) {
@@ -304,13 +363,68 @@ extern "C" int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float*
   return PETCH_OK;
 }
 
+extern "C" int petch_extend_batch(petch_ctx* ctx, int n, const petch_extend_item* items) {
+  PETCH_ARG(ctx, ctx && n >= 0 && (n == 0 || items), "extend_batch: null pointer or n < 0");
+  if (n == 0) return PETCH_OK;
+  std::vector<ExtendItem> t((size_t)n);
+  long long total = 0;
+  for (int f = 0; f < n; ++f) {
+    const petch_extend_item& a = items[f];
+    PETCH_ARG(ctx, a.bvh && a.V && a.phi && a.Fs, "extend_batch: item %d has a null pointer", f);
+    PETCH_ARG(ctx, a.nx > 0 && a.ny > 0 && a.nz > 0, "extend_batch: item %d has a bad shape", f);
+    total += (long long)a.nx * a.ny * a.nz;
+    t[f] = ExtendItem{petch_bvh_view(a.bvh), a.V, a.phi, a.Fs, a.ny, a.nz, a.dx, a.band, total};
+  }
+  PETCH_ARG(ctx, total < 0x7fffffffLL, "extend_batch: %lld nodes exceed the int32 node index",
+            total);
+  const ExtendItem* dev = (const ExtendItem*)petch_upload_table(ctx, t.data(), t.size() * sizeof(ExtendItem));
+  // flags, their scan, and the node list with the band's size after it (scratch slots of the
+  // level-set stages, which run after the extension on the same stream)
+  int* flag = (int*)petch_scratch(ctx, SCR_LS_A, (size_t)total * sizeof(int));
+  int* pos = (int*)petch_scratch(ctx, SCR_LS_B, (size_t)total * sizeof(int));
+  int* idx = (int*)petch_scratch(ctx, SCR_LS_FROZEN, (size_t)(total + 1) * sizeof(int));
+  if (!dev || !flag || !pos || !idx) return PETCH_E_HIP;
+  const unsigned g = petch_blocks(total, kBS);
+  hipLaunchKernelGGL(k_band_flags, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, total, flag);
+  PETCH_LAUNCH_CHECK(ctx);
+  int rc = petch_exclusive_scan(ctx, flag, pos, total, idx + total);
+  if (rc != PETCH_OK) return rc;
+  hipLaunchKernelGGL(k_band_compact, dim3(g), dim3(kBS), 0, ctx->stream, flag, pos, total, idx);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, idx, idx + total);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
 extern "C" int petch_extend(petch_ctx* ctx, const petch_bvh* bvh, const float* V, const float* phi,
                             int nx, int ny, int nz, double dx, double band, float* Fs) {
   PETCH_ARG(ctx, ctx && bvh && V && phi && Fs, "extend: null pointer");
   PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "extend: bad shape");
-  long long N = (long long)nx * ny * nz;
-  hipLaunchKernelGGL(k_extend, dim3(petch_blocks(N, kBS)), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), V, phi, nx, ny, nz, dx, band, Fs);
+  const petch_extend_item it{bvh, V, phi, Fs, nx, ny, nz, dx, band};
+  return petch_extend_batch(ctx, 1, &it);
+}
+
+extern "C" int petch_nearest_face_value_batch(petch_ctx* ctx, int n,
+                                              const petch_nearest_item* items) {
+  PETCH_ARG(ctx, ctx && n >= 0 && (n == 0 || items), "nearest_face_value_batch: null pointer or n < 0");
+  std::vector<NearestItem> t;
+  t.reserve((size_t)n);
+  long long total = 0;
+  for (int f = 0; f < n; ++f) {
+    const petch_nearest_item& a = items[f];
+    PETCH_ARG(ctx, a.n >= 0, "nearest_face_value_batch: item %d has n < 0", f);
+    if (a.n == 0) continue;
+    PETCH_ARG(ctx, a.bvh && a.src && a.pts && a.out,
+              "nearest_face_value_batch: item %d has a null pointer", f);
+    total += a.n;
+    t.push_back(NearestItem{petch_bvh_view(a.bvh), a.src, a.pts, a.out, a.miss, total});
+  }
+  if (total == 0) return PETCH_OK;
+  const NearestItem* dev =
+      (const NearestItem*)petch_upload_table(ctx, t.data(), t.size() * sizeof(NearestItem));
+  if (!dev) return PETCH_E_HIP;
+  hipLaunchKernelGGL(k_nearest_batch, dim3(petch_blocks(total, kBS)), dim3(kBS), 0, ctx->stream,
+                     dev, (int)t.size(), total);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
@@ -319,8 +433,6 @@ extern "C" int petch_nearest_face_value(petch_ctx* ctx, const petch_bvh* bvh, co
                                         const float* pts, int n, float miss, float* out) {
   PETCH_ARG(ctx, ctx && bvh && src && pts && out, "nearest_face_value: null pointer");
   if (n <= 0) return PETCH_OK;
-  hipLaunchKernelGGL(k_nearest_value, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream,
-                     petch_bvh_view(bvh), src, pts, n, miss, out);
-  PETCH_LAUNCH_CHECK(ctx);
-  return PETCH_OK;
+  const petch_nearest_item it{bvh, src, pts, n, miss, out};
+  return petch_nearest_face_value_batch(ctx, 1, &it);
 }

@@ -87,6 +87,18 @@ class IonFeature(ctypes.Structure):
                 ("ipass", c_void_p), ("arr", c_void_p), ("ang", c_void_p)]
 
 
+class ExtendItem(ctypes.Structure):
+    """petch_extend_item: one grid of a batched velocity extension."""
+    _fields_ = [("bvh", c_void_p), ("V", c_void_p), ("phi", c_void_p), ("Fs", c_void_p),
+                ("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_double), ("band", c_double)]
+
+
+class NearestItem(ctypes.Structure):
+    """petch_nearest_item: one point set of a batched nearest-face lookup."""
+    _fields_ = [("bvh", c_void_p), ("src", c_void_p), ("pts", c_void_p), ("n", c_int),
+                ("miss", c_float), ("out", c_void_p)]
+
+
 # name -> argument types (all entry points return int)
 SIGNATURES = {
     "petch_ctx_create": [c_int, POINTER(c_void_p)],
@@ -133,6 +145,8 @@ SIGNATURES = {
                            POINTER(ChemParams), c_int, P, P],
     "petch_extend": [P, P, P, P, c_int, c_int, c_int, c_double, c_double, P],
     "petch_nearest_face_value": [P, P, P, P, c_int, c_float, P],
+    "petch_extend_batch": [P, c_int, P],
+    "petch_nearest_face_value_batch": [P, c_int, P],
     "petch_radiosity_assemble": [P, P, c_int, c_int, P, P, P],
     "petch_radiosity_solve": [P, P, P, P, c_int, c_int, P, c_double, c_int, c_double, P, P,
                               POINTER(c_int)],

@@ -6,9 +6,10 @@ builds its surface, then the ray traces of all features' coupled fluxes run as O
 trace (petch_trace_ion_yield_batch, petch_trace_neutral_rr_batch: a persistent ray queue
 spanning every feature), then every feature finishes its step (rate, extension, advection,
 reinit). The host reads a step needs (warm-start guard, max V, BVH guard) are gathered over all
-features into one read each, instead of one per feature. Each feature's rays draw exactly the
-Philox streams of its own run_etch_3d, so a batched feature follows its single run up to the
-order of the float atomic adds.
+features into one read each, instead of one per feature, and the warm-start lookups and the
+velocity extensions of all features run as one launch each. Each feature's rays draw exactly
+the Philox streams of its own run_etch_3d, so a batched feature follows its single run up to
+the order of the float atomic adds.
 
     geos = run_etch_3d_batch([dict(Lx=14, Ly=14, Lz=24, dx=0.25, trench_width=6, mask_th=2,
                                    sub_top=18, hole=True, par=p, flags=fl, n_ion=30000,
@@ -40,10 +41,18 @@ class BatchLoop:
             vals = torch.stack([finite[i] for i in need]).cpu().tolist()
             for i, v in zip(need, vals):
                 finite[i] = bool(v)
+        # the warm-start lookups of all features in one launch
+        ws_idx, ws_items = [], []
+        for i, lp in enumerate(loops):
+            it = lp.warm_start_item(finite[i])
+            if it is not None:
+                ws_idx.append(i)
+                ws_items.append(it)
+        warm = dict(zip(ws_idx, ops.nearest_face_value_batch(ws_items, 1.0)))
         outs = [None] * len(loops)
         pending = {}
         for i, lp in enumerate(loops):
-            job = lp.begin_flux(step, finite[i])
+            job = lp.begin_flux(step, finite[i], bare_init=warm.get(i))
             if isinstance(job, _FluxSteps):
                 try:
                     pending[i] = (job.steps, next(job.steps))
@@ -85,8 +94,16 @@ class BatchLoop:
             for r, i in enumerate(live):
                 vmx, nbad = ops.decode_rate_reduction(host[r, :2])
                 reads[i] = (vmx, nbad, int(host[r, 2]))
-        # 4. extension, CFL, advection and reinit of every feature
-        return [lp.end_finish(step, *rd) for lp, rd in zip(loops, reads)]
+        # 4. the velocity extension of all features in one pass, then CFL, advection and reinit
+        #    of every feature
+        ext = [lp.extend_item() if h and not rd[2] else None
+               for lp, h, rd in zip(loops, has, reads)]
+        items = [e for e in ext if e is not None]
+        if items:
+            with self.timer.region("extend"):
+                ops.extend_batch(items)
+        return [lp.end_finish(step, *rd, extended=e is not None)
+                for lp, rd, e in zip(loops, reads, ext)]
 
 
 _GEO_KEYS = ("Lx", "Ly", "Lz", "dx", "trench_width", "mask_th", "sub_top", "hole")

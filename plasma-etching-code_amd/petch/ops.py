@@ -407,6 +407,40 @@ def extend(bvh, V, phi, dx, band, out=None):
     return Fs
 
 
+def extend_batch(items):
+    """extend for many grids in one pass. items: (bvh, V, phi, dx, band, out) tuples; each
+    out (the grid's Fs) is written as extend would write it."""
+    import ctypes
+    from ._abi import ExtendItem
+    if not items:
+        return
+    arr = (ExtendItem * len(items))()
+    for k, (bvh, V, phi, dx, band, out) in enumerate(items):
+        nx, ny, nz = _shape3(phi)
+        arr[k] = ExtendItem(bvh.h, ptr(V, f32), ptr(phi, f32), ptr(out, f32), nx, ny, nz,
+                            float(dx), float(band))
+    items[0][0].ctx.call("petch_extend_batch", len(items), ctypes.cast(arr, ctypes.c_void_p))
+
+
+def nearest_face_value_batch(items, miss=1.0):
+    """nearest_face_value for many point sets in one launch. items: (bvh, src, pts) tuples;
+    returns one output per item."""
+    import ctypes
+    from ._abi import NearestItem
+    outs = []
+    arr = (NearestItem * len(items))()
+    for k, (bvh, src, pts) in enumerate(items):
+        n = int(pts.shape[0])
+        out = torch.empty((n,), dtype=f32, device=pts.device)
+        outs.append(out)
+        arr[k] = NearestItem(bvh.h, ptr(src, f32) if n else None, ptr(pts, f32) if n else None,
+                             n, float(miss), ptr(out, f32) if n else None)
+    if items:
+        items[0][0].ctx.call("petch_nearest_face_value_batch", len(items),
+                             ctypes.cast(arr, ctypes.c_void_p))
+    return outs
+
+
 def nearest_face_value(bvh, src, pts, miss=1.0):
     """src[nearest face of each point] on `bvh` (threed.py:1619-1639)."""
     n = int(pts.shape[0])

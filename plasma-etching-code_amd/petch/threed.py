@@ -389,8 +389,19 @@ class EtchLoop:
             return torch.isfinite(mesh.centroids).all()
         return None
 
-    def begin_flux(self, step, finite):
-        """The flux of the step begun by begin_mesh; `finite` is that flag read on the host."""
+    def warm_start_item(self, finite):
+        """(bvh, src, points) of this step's warm-start lookup (nearest previous face of every
+        new centroid, threed.py:1700-1701) for ops.nearest_face_value_batch, or None when the
+        step has no such lookup."""
+        if not finite or not self.gpu_ws or self._mesh.nf == 0 or self.nt == "radiosity" or \
+                not getattr(self.flags, "coverage_sticking", False):
+            return None
+        return (self.prev_mesh.bvh, self.prev_bare, self._mesh.centroids)
+
+    def begin_flux(self, step, finite, bare_init=None):
+        """The flux of the step begun by begin_mesh; `finite` is that flag read on the host.
+        bare_init: the warm-start values when the caller already looked them up (the batched
+        loop does all features' lookups in one launch)."""
         T, par, flags, dev = self.timer, self.par, self.flags, self.dev
         mesh = self._mesh
         n_ion_s, n_neu_s = self._rays
@@ -404,7 +415,9 @@ class EtchLoop:
                     return _flux.radiosity_flux(mesh, self.phi, self.geo, par, n_ion_s, seed, flags)
                 if getattr(flags, "coverage_sticking", False):
                     bi = None
-                    if finite:
+                    if finite and bare_init is not None:
+                        bi = bare_init
+                    elif finite:
                         if self.gpu_ws:             # nearest previous face (threed.py:1700-1701)
                             bi = ops.nearest_face_value(self.prev_mesh.bvh, self.prev_bare,
                                                         mesh.centroids, 1.0)
@@ -446,9 +459,17 @@ class EtchLoop:
         self.timings['rate'] = self.timings.get('rate', 0.0) + time.time() - trt
         return True
 
-    def end_finish(self, step, vmx, nbad, bvh_err):
+    def extend_item(self):
+        """(bvh, V, phi, dx, band, Fs) of this step's velocity extension for ops.extend_batch,
+        or None when the step extends otherwise (no surface, or the nearest-centroid mode)."""
+        if self._V is None or self.extend_mode != "gpu":
+            return None
+        return (self._mesh.bvh, self._V, self.phi, self.dx, self.band, self.Fs)
+
+    def end_finish(self, step, vmx, nbad, bvh_err, extended=False):
         """Extension, CFL, advection and reinit, given the host reads of the rate reduction
-        (max V, non-finite count) and of the BVH guard flag."""
+        (max V, non-finite count) and of the BVH guard flag. extended: Fs already holds this
+        step's extension (the batched loop extends all features in one pass)."""
         T, dx, dev = self.timer, self.dx, self.dev
         tim = self.timings
         mesh = self._mesh
@@ -464,7 +485,9 @@ class EtchLoop:
             vmx = 0.0
         te = time.time()
         with T.region("extend"):
-            if V is None:
+            if extended:
+                pass
+            elif V is None:
                 self.Fs.zero_()
             elif self.extend_mode == "gpu":
                 ops.extend(mesh.bvh, V, self.phi, dx, self.band, out=self.Fs)

@@ -1,6 +1,9 @@
 """The lockstep driver on the host (no GPU): every feature's trace requests are served by one
 batched call per kind, in order, and each feature gets back its own result."""
+import types
+
 import pytest
+import torch
 
 from petch import batch as B
 from petch.threed import _FluxSteps
@@ -13,14 +16,25 @@ class FakeLoop:
     def begin_mesh(self, step):
         return None
 
-    def begin_flux(self, step, finite):
+    def warm_start_item(self, finite):
+        return ("ws", self.k) if self.k == 1 else None
+
+    def begin_flux(self, step, finite, bare_init=None):
+        self.bare_init = bare_init
         return self.begin_step(step)
 
     def end_rate(self, step, out):
         self._out = out
-        return False
+        self.red = torch.zeros(2, dtype=torch.int32)
+        self.dev = torch.device("cpu")
+        self._mesh = types.SimpleNamespace(bvh=types.SimpleNamespace(status_into=lambda t: None))
+        return out is not None
 
-    def end_finish(self, step, vmx, nbad, err):
+    def extend_item(self):
+        return ("ext", self.k)
+
+    def end_finish(self, step, vmx, nbad, err, extended=False):
+        self.extended = extended
         return self.end_step(step, self._out)
 
     def begin_step(self, step):
@@ -51,12 +65,27 @@ def test_lockstep_batches_each_kind(monkeypatch):
         calls.append(("neutral", [r['k'] for r in reqs]))
         return [("neutral", r['k'], r['it']) for r in reqs]
 
+    def ws_batch(items, miss):
+        calls.append(("ws", list(items)))
+        return [("bare", it[1]) for it in items]
+
+    def ext_batch(items):
+        calls.append(("ext", list(items)))
+
     monkeypatch.setattr(B.ops, "trace_ion_yield_batch", ion_batch)
     monkeypatch.setattr(B.ops, "trace_neutral_rr_batch", neutral_batch)
+    monkeypatch.setattr(B.ops, "nearest_face_value_batch", ws_batch)
+    monkeypatch.setattr(B.ops, "extend_batch", ext_batch)
+    monkeypatch.setattr(B.ops, "decode_rate_reduction", lambda r: (1.0, 0))
     loops = [FakeLoop(0, 1), FakeLoop(1, 4), FakeLoop(99, 0), FakeLoop(2, 1)]
     outs = B.BatchLoop(loops).step(0)
-    assert calls == [("ion", [0, 1, 2]), ("neutral", [0, 1, 2]), ("neutral", [1]),
-                     ("neutral", [1]), ("neutral", [1])]
+    # one warm-start launch for the features that have a lookup, one launch per trace kind and
+    # fixed-point round, one extension pass for every feature with a surface rate
+    assert calls == [("ws", [("ws", 1)]), ("ion", [0, 1, 2]), ("neutral", [0, 1, 2]),
+                     ("neutral", [1]), ("neutral", [1]), ("neutral", [1]),
+                     ("ext", [("ext", 0), ("ext", 1), ("ext", 2)])]
+    assert [lp.bare_init for lp in loops] == [None, ("bare", 1), None, None]
+    assert [lp.extended for lp in loops] == [True, True, False, True]
     assert outs == [dict(k=0, n=1), dict(k=1, n=4), None, dict(k=2, n=1)]
 
 

@@ -233,3 +233,35 @@ def test_radiosity_solve_fp64(name):
         direct = ofl.radiosity_direct(Dh, Aff, bare.astype(np.float64), beta)
         np.testing.assert_allclose(G.cpu().numpy(), direct, rtol=1e-5, atol=1e-10)
         assert it > 0
+
+
+def test_batched_extension_and_warm_start_equal_single_calls():
+    """extend_batch / nearest_face_value_batch over grids of different shapes and surfaces give
+    each grid exactly what a single call gives it (the same queries, one launch)."""
+    import torch
+    from petch import ops
+    grids = [(ol.init_sdf(8.0, 8.0, 14.0, 0.25, 3.0, 8.0, 4.0, True), 0.25),
+             (ol.init_sdf(6.0, 3.0, 10.0, 0.2, 2.0, 1.0, 6.0, False), 0.2),
+             (ol.init_sdf(5.0, 5.0, 8.0, 0.25, 2.0, 1.0, 5.0, True), 0.25)]
+    items, singles, ws_items, ws_singles = [], [], [], []
+    rng = np.random.default_rng(8)
+    for phi, dx in grids:
+        phi = phi.astype(np.float32)
+        v, f = on.marching_cubes(phi, dx)
+        bvh = ops.Bvh(_dev()).build(_t(v), _t(f))
+        V = _t(rng.uniform(0, 3, len(f)).astype(np.float32))
+        tphi = _t(phi)
+        singles.append(ops.extend(bvh, V, tphi, dx, 4 * dx).cpu().numpy())
+        out = torch.full_like(tphi, -7.0)
+        items.append((bvh, V, tphi, dx, 4 * dx, out))
+        pts = _t((om.mesh_attrs(v, f)[0] + 0.03).astype(np.float32))
+        src = _t(rng.uniform(0, 1, len(f)).astype(np.float32))
+        ws_singles.append(ops.nearest_face_value(bvh, src, pts, 1.0).cpu().numpy())
+        ws_items.append((bvh, src, pts))
+    ops.extend_batch(items)
+    for (_, _, _, _, _, out), ref in zip(items, singles):
+        np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    outs = ops.nearest_face_value_batch(ws_items, 1.0)
+    for out, ref in zip(outs, ws_singles):
+        np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    assert ops.nearest_face_value_batch([], 1.0) == []
