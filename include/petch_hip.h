@@ -117,6 +117,22 @@ int petch_marching_cubes(petch_ctx* ctx, const float* phi, int nx, int ny, int n
                          float* verts, int* faces, int cap_v, int cap_f, int* nv_host,
                          int* nf_host);
 
+/* petch_marching_cubes for many grids with one host read (the batched etch loop): every grid's
+ * counts come back in one copy, then every grid's surface is emitted. nv_out / nf_out are host
+ * pointers; on PETCH_E_CAPACITY every item's counts are set and nothing was written. No
+ * reference counterpart: the reference meshes one feature per call. */
+typedef struct petch_mc_item {
+  const float* phi;
+  int nx, ny, nz;
+  double dx;
+  float* verts;
+  int* faces;
+  int cap_v, cap_f;
+  int* nv_out;
+  int* nf_out;
+} petch_mc_item;
+int petch_marching_cubes_batch(petch_ctx* ctx, int n, const petch_mc_item* items);
+
 /* Face centroids (nf,3) = vertex mean, areas (nf) = 0.5|cross|, unit normals (nf,3) (zero for a
  * degenerate face) — threed.py:96-99 and _face_normals_k threed.py:228-234. Any output may be
  * NULL. */

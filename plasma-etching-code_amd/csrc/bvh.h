@@ -232,6 +232,10 @@ struct TravStack {
   static constexpr int kLds = L;
   // entry i of the LDS part (i < L)
   __device__ __forceinline__ float lds_dist(int i) const { return lds_t[i * S]; }
+  __device__ __forceinline__ void lds_put(int i, int n, float t) {
+    lds_n[i * S] = n;
+    lds_t[i * S] = t;
+  }
   __device__ __forceinline__ int lds_node(int i) const { return lds_n[i * S]; }
   __device__ __forceinline__ float dist(int i) const {
     if (L > 0 && i < L) return lds_t[i * S];
@@ -325,10 +329,17 @@ __device__ __forceinline__ bool trav_step(const BvhView& b, const RayPre& r, Tra
   sort2(t1, c1, t2, c2);
   int sp = T.sp;
   if (t0 < kHitLim) {
-    // push the farther ones far-to-near, descend the nearest
-    if (t3 < kHitLim) push_far(b, stk, sp, c3, t3);
-    if (t2 < kHitLim) push_far(b, stk, sp, c2, t2);
-    if (t1 < kHitLim) push_far(b, stk, sp, c1, t1);
+    // push the farther ones far-to-near, descend the nearest; while every lane has room for
+    // three more entries in the LDS part, the pushes go there without the overflow checks
+    if (Stack::kLds >= 3 && !__any(sp > Stack::kLds - 3)) {
+      if (t3 < kHitLim) stk.lds_put(sp++, c3, t3);
+      if (t2 < kHitLim) stk.lds_put(sp++, c2, t2);
+      if (t1 < kHitLim) stk.lds_put(sp++, c1, t1);
+    } else {
+      if (t3 < kHitLim) push_far(b, stk, sp, c3, t3);
+      if (t2 < kHitLim) push_far(b, stk, sp, c2, t2);
+      if (t1 < kHitLim) push_far(b, stk, sp, c1, t1);
+    }
     T.sp = sp;
     T.node = c0;
     return false;

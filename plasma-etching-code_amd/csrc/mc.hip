@@ -20,6 +20,8 @@
 // sums -> host reads the two totals (capacity check) -> emit vertices (+ per-node vertex base)
 // -> emit faces.
 #include "common.h"
+
+#include <vector>
 #include "scan.h"
 
 namespace {
@@ -357,5 +359,83 @@ extern "C" int petch_marching_cubes(petch_ctx* ctx, const float* phi, int nx, in
   hipLaunchKernelGGL(k_mc_faces, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, phi, g, N, bt,
                      vbase, vflg, faces, cap_f);
   PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_marching_cubes_batch(petch_ctx* ctx, int n, const petch_mc_item* items) {
+  PETCH_ARG(ctx, ctx && n >= 0 && (n == 0 || items), "marching_cubes_batch: null pointer or n < 0");
+  if (n == 0) return PETCH_OK;
+  // block partial sums of every grid side by side, then the totals of every grid
+  std::vector<long long> off((size_t)n + 1, 0), cells((size_t)n, 0);
+  long long maxN = 0;
+  for (int f = 0; f < n; ++f) {
+    const petch_mc_item& a = items[f];
+    PETCH_ARG(ctx, a.phi && a.nv_out && a.nf_out, "marching_cubes_batch: item %d has a null pointer", f);
+    PETCH_ARG(ctx, a.nx > 0 && a.ny > 0 && a.nz > 0, "marching_cubes_batch: item %d has a bad shape", f);
+    PETCH_ARG(ctx, a.cap_v >= 0 && a.cap_f >= 0, "marching_cubes_batch: item %d has a negative capacity", f);
+    *a.nv_out = 0;
+    *a.nf_out = 0;
+    const bool empty = a.nx < 2 || a.ny < 2 || a.nz < 2;
+    const long long N = empty ? 0 : (long long)a.nx * a.ny * a.nz;
+    cells[f] = N;
+    if (N > maxN) maxN = N;
+    off[f + 1] = off[f] + 2 * ((N + kMcBS - 1) / kMcBS);
+  }
+  int* blk = (int*)petch_scratch(ctx, SCR_MC_BLOCK, (size_t)(off[n] + 2 * n) * sizeof(int));
+  int* h = (int*)petch_pinned(ctx, (size_t)(2 * n) * sizeof(int));
+  if (!blk || !h) return PETCH_E_HIP;
+  int* totals = blk + off[n];
+  PETCH_HIP(ctx, hipMemsetAsync(totals, 0, (size_t)(2 * n) * sizeof(int), ctx->stream));
+  for (int f = 0; f < n; ++f) {
+    if (cells[f] == 0) continue;
+    const petch_mc_item& a = items[f];
+    const long long N = cells[f], nb = (N + kMcBS - 1) / kMcBS;
+    Grid g{a.nx, a.ny, a.nz, (long long)a.ny * a.nz, (long long)a.nz};
+    int* bv = blk + off[f];
+    int* bt = bv + nb;
+    hipLaunchKernelGGL(k_mc_count, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, a.phi, g, N,
+                       bv, bt);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(petch::k_scan_partials2, dim3(1), dim3(1024), 0, ctx->stream, bv, bt,
+                       (int)nb, totals + 2 * f);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  // the one host read of the batch: every grid's vertex and face count
+  PETCH_HIP(ctx, hipMemcpyAsync(h, totals, (size_t)(2 * n) * sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int short_of = -1;
+  for (int f = 0; f < n; ++f) {
+    const petch_mc_item& a = items[f];
+    *a.nv_out = h[2 * f];
+    *a.nf_out = h[2 * f + 1];
+    if (h[2 * f] > a.cap_v || h[2 * f + 1] > a.cap_f || (h[2 * f] > 0 && !a.verts) ||
+        (h[2 * f + 1] > 0 && !a.faces))
+      short_of = f;
+  }
+  if (short_of >= 0)
+    return petch_set_error(ctx, PETCH_E_CAPACITY,
+                           "marching_cubes_batch: item %d needs %d vertices / %d faces, capacity "
+                           "%d / %d (every item's counts are in its nv_out / nf_out)", short_of,
+                           h[2 * short_of], h[2 * short_of + 1], items[short_of].cap_v,
+                           items[short_of].cap_f);
+  unsigned char* cell = (unsigned char*)petch_scratch(ctx, SCR_MC_CELL, (size_t)maxN * 5);
+  if (maxN > 0 && !cell) return PETCH_E_HIP;
+  for (int f = 0; f < n; ++f) {
+    const petch_mc_item& a = items[f];
+    if (cells[f] == 0 || (h[2 * f] == 0 && h[2 * f + 1] == 0)) continue;
+    const long long N = cells[f], nb = (N + kMcBS - 1) / kMcBS;
+    Grid g{a.nx, a.ny, a.nz, (long long)a.ny * a.nz, (long long)a.nz};
+    int* bv = blk + off[f];
+    int* bt = bv + nb;
+    int* vbase = (int*)cell;
+    uint8_t* vflg = cell + 4 * N;
+    hipLaunchKernelGGL(k_mc_verts, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, a.phi, g, N,
+                       a.dx, bv, vbase, vflg, a.verts, a.cap_v);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_mc_faces, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, a.phi, g, N,
+                       bt, vbase, vflg, a.faces, a.cap_f);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
   return PETCH_OK;
 }

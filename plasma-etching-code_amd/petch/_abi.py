@@ -87,6 +87,13 @@ class IonFeature(ctypes.Structure):
                 ("ipass", c_void_p), ("arr", c_void_p), ("ang", c_void_p)]
 
 
+class McItem(ctypes.Structure):
+    """petch_mc_item: one grid of a batched marching cubes."""
+    _fields_ = [("phi", c_void_p), ("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_double),
+                ("verts", c_void_p), ("faces", c_void_p), ("cap_v", c_int), ("cap_f", c_int),
+                ("nv_out", c_void_p), ("nf_out", c_void_p)]
+
+
 class ExtendItem(ctypes.Structure):
     """petch_extend_item: one grid of a batched velocity extension."""
     _fields_ = [("bvh", c_void_p), ("V", c_void_p), ("phi", c_void_p), ("Fs", c_void_p),
@@ -146,6 +153,7 @@ SIGNATURES = {
     "petch_extend": [P, P, P, P, c_int, c_int, c_int, c_double, c_double, P],
     "petch_nearest_face_value": [P, P, P, P, c_int, c_float, P],
     "petch_extend_batch": [P, c_int, P],
+    "petch_marching_cubes_batch": [P, c_int, P],
     "petch_nearest_face_value_batch": [P, c_int, P],
     "petch_radiosity_assemble": [P, P, c_int, c_int, P, P, P],
     "petch_radiosity_solve": [P, P, P, P, c_int, c_int, P, c_double, c_int, c_double, P, P,

@@ -6,10 +6,10 @@ builds its surface, then the ray traces of all features' coupled fluxes run as O
 trace (petch_trace_ion_yield_batch, petch_trace_neutral_rr_batch: a persistent ray queue
 spanning every feature), then every feature finishes its step (rate, extension, advection,
 reinit). The host reads a step needs (warm-start guard, max V, BVH guard) are gathered over all
-features into one read each, instead of one per feature, and the warm-start lookups and the
-velocity extensions of all features run as one launch each. Each feature's rays draw exactly
-the Philox streams of its own run_etch_3d, so a batched feature follows its single run up to
-the order of the float atomic adds.
+features into one read each, instead of one per feature (the marching-cubes counts of all
+features too), and the warm-start lookups and the velocity extensions of all features run as
+one launch each. Each feature's rays draw exactly the Philox streams of its own run_etch_3d,
+so a batched feature follows its single run up to the order of the float atomic adds.
 
     geos = run_etch_3d_batch([dict(Lx=14, Ly=14, Lz=24, dx=0.25, trench_width=6, mask_th=2,
                                    sub_top=18, hole=True, par=p, flags=fl, n_ion=30000,
@@ -34,8 +34,11 @@ class BatchLoop:
 
     def step(self, step):
         loops = self.loops
-        # 1. every feature's mesh and BVH; the warm-start guards of all features in one read
-        finite = [lp.begin_mesh(step) for lp in loops]
+        # 1. every feature's mesh (one host read for all marching-cubes counts) and BVH; the
+        #    warm-start guards of all features in one read
+        with self.timer.region("mesh"):
+            mcs = ops.marching_cubes_batch([lp.mesh_item(step) for lp in loops])
+        finite = [lp.begin_mesh(step, mc) for lp, mc in zip(loops, mcs)]
         need = [i for i, f in enumerate(finite) if f is not None]
         if need:
             vals = torch.stack([finite[i] for i in need]).cpu().tolist()

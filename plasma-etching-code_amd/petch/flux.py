@@ -35,7 +35,11 @@ class SurfaceMesh:
         self.nv = 0
 
     def build(self, phi, dx):
-        v, f = ops.marching_cubes(phi, dx, self.bufs)
+        return self.build_from(*ops.marching_cubes(phi, dx, self.bufs))
+
+    def build_from(self, v, f):
+        """Everything after marching cubes, from its (verts, faces) (the batched loop meshes
+        all features' grids with one host read, then builds each feature's surface)."""
         self.verts, self.faces = v, f
         self.nv, self.nf = int(v.shape[0]), int(f.shape[0])
         self.centroids, self.areas, self.normals = ops.mesh_attrs(v, f)

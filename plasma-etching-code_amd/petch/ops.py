@@ -98,6 +98,36 @@ def marching_cubes(phi, dx, bufs=None):
     return bufs.verts[:nv.value], bufs.faces[:nf.value]
 
 
+def marching_cubes_batch(items):
+    """marching_cubes for many grids with one host read of their counts. items: (phi, dx, bufs)
+    tuples (bufs: MeshBuffers, grown as needed); returns (verts, faces) per item."""
+    import ctypes
+    import numpy as np
+    from ._abi import McItem, PetchCapacityError
+    if not items:
+        return []
+    counts = np.zeros((len(items), 2), np.int32)
+    base = counts.ctypes.data
+    for _ in range(2):
+        arr = (McItem * len(items))()
+        for k, (phi, dx, bufs) in enumerate(items):
+            nx, ny, nz = _shape3(phi)
+            bufs.ensure(1, 1)
+            arr[k] = McItem(ptr(phi, f32, "phi"), nx, ny, nz, float(dx), ptr(bufs.verts),
+                            ptr(bufs.faces), bufs.cap_v, bufs.cap_f, base + 8 * k, base + 8 * k + 4)
+        try:
+            _ctx(items[0][0]).call("petch_marching_cubes_batch", len(items),
+                                   ctypes.cast(arr, ctypes.c_void_p))
+            break
+        except PetchCapacityError:
+            for (phi, dx, bufs), (nv, nf) in zip(items, counts.tolist()):
+                bufs.ensure(nv, nf)
+    else:
+        raise RuntimeError("marching_cubes_batch: capacity retry failed")
+    return [(bufs.verts[:nv], bufs.faces[:nf])
+            for (_, _, bufs), (nv, nf) in zip(items, counts.tolist())]
+
+
 def mesh_attrs(verts, faces):
     """(centroids (F,3), areas (F,), unit normals (F,3)) in float32 (threed.py:96-99, 228-234)."""
     nf = int(faces.shape[0])

@@ -370,13 +370,22 @@ class EtchLoop:
         finite = self.begin_mesh(step)
         return self.begin_flux(step, None if finite is None else bool(finite))
 
-    def begin_mesh(self, step):
+    def mesh_item(self, step):
+        """(phi, dx, buffers) of this step's marching cubes for ops.marching_cubes_batch."""
+        return (self.phi, self.dx, self.meshes[step % 2].bufs)
+
+    def begin_mesh(self, step, mc=None):
         """Mesh + BVH of the step. Returns the device flag 'all centroids finite' that gates the
-        warm start (threed.py:1699), or None when the step needs no warm start."""
+        warm start (threed.py:1699), or None when the step needs no warm start. mc: this step's
+        marching-cubes (verts, faces) when the caller already extracted them (the batched loop
+        meshes all features with one host read)."""
         T, dx = self.timer, self.dx
         tm = time.time()
         with T.region("mesh"):
-            mesh = self.meshes[step % 2].build(self.phi, dx)
+            if mc is None:
+                mesh = self.meshes[step % 2].build(self.phi, dx)
+            else:
+                mesh = self.meshes[step % 2].build_from(*mc)
         self.timings['mesh'] += time.time() - tm
         self._mesh = mesh
         if self.rays_per_point is not None:

@@ -13,7 +13,11 @@ class FakeLoop:
     def __init__(self, k, n_fp):
         self.k, self.n_fp, self.log = k, n_fp, []
 
-    def begin_mesh(self, step):
+    def mesh_item(self, step):
+        return ("mc", self.k)
+
+    def begin_mesh(self, step, mc=None):
+        self.mc = mc
         return None
 
     def warm_start_item(self, finite):
@@ -65,6 +69,10 @@ def test_lockstep_batches_each_kind(monkeypatch):
         calls.append(("neutral", [r['k'] for r in reqs]))
         return [("neutral", r['k'], r['it']) for r in reqs]
 
+    def mc_batch(items):
+        calls.append(("mc", list(items)))
+        return [("mesh", it[1]) for it in items]
+
     def ws_batch(items, miss):
         calls.append(("ws", list(items)))
         return [("bare", it[1]) for it in items]
@@ -74,16 +82,20 @@ def test_lockstep_batches_each_kind(monkeypatch):
 
     monkeypatch.setattr(B.ops, "trace_ion_yield_batch", ion_batch)
     monkeypatch.setattr(B.ops, "trace_neutral_rr_batch", neutral_batch)
+    monkeypatch.setattr(B.ops, "marching_cubes_batch", mc_batch)
     monkeypatch.setattr(B.ops, "nearest_face_value_batch", ws_batch)
     monkeypatch.setattr(B.ops, "extend_batch", ext_batch)
     monkeypatch.setattr(B.ops, "decode_rate_reduction", lambda r: (1.0, 0))
     loops = [FakeLoop(0, 1), FakeLoop(1, 4), FakeLoop(99, 0), FakeLoop(2, 1)]
     outs = B.BatchLoop(loops).step(0)
-    # one warm-start launch for the features that have a lookup, one launch per trace kind and
-    # fixed-point round, one extension pass for every feature with a surface rate
-    assert calls == [("ws", [("ws", 1)]), ("ion", [0, 1, 2]), ("neutral", [0, 1, 2]),
+    # one marching-cubes pass for all features, one warm-start launch for the features that have
+    # a lookup, one launch per trace kind and fixed-point round, one extension pass for every
+    # feature with a surface rate
+    assert calls == [("mc", [("mc", 0), ("mc", 1), ("mc", 99), ("mc", 2)]),
+                     ("ws", [("ws", 1)]), ("ion", [0, 1, 2]), ("neutral", [0, 1, 2]),
                      ("neutral", [1]), ("neutral", [1]), ("neutral", [1]),
                      ("ext", [("ext", 0), ("ext", 1), ("ext", 2)])]
+    assert [lp.mc for lp in loops] == [("mesh", 0), ("mesh", 1), ("mesh", 99), ("mesh", 2)]
     assert [lp.bare_init for lp in loops] == [None, ("bare", 1), None, None]
     assert [lp.extended for lp in loops] == [True, True, False, True]
     assert outs == [dict(k=0, n=1), dict(k=1, n=4), None, dict(k=2, n=1)]

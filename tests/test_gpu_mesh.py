@@ -82,3 +82,25 @@ def test_mesh_attrs_adjacency_gas_normals(name, phi, dx):
                                   om.canonical_pairs(om.edge_adjacency(fn_)))
     rg = om.gas_normals(vn, fn_, rc, phi, dx)
     np.testing.assert_allclose(gn.cpu().numpy()[ok], rg[ok], atol=1e-4)
+
+
+def test_batched_marching_cubes_equals_single_calls():
+    """marching_cubes_batch over grids of different shapes (one host read for all counts) gives
+    each grid exactly the single call's surface, and grows undersized buffers."""
+    import torch
+    from petch import ops
+    d = torch.device("cuda", 0)
+    grids = [(ol.init_sdf(8.0, 8.0, 14.0, 0.25, 3.0, 8.0, 4.0, True), 0.25),
+             (ol.init_sdf(6.0, 3.0, 10.0, 0.2, 2.0, 1.0, 6.0, False), 0.2),
+             (np.ones((4, 4, 4), np.float64), 0.5)]                   # no surface
+    items, singles = [], []
+    for phi, dx in grids:
+        tp = torch.from_numpy(phi.astype(np.float32)).to(d)
+        v, f = ops.marching_cubes(tp, dx)
+        singles.append((v.cpu().numpy(), f.cpu().numpy()))
+        items.append((tp, dx, ops.MeshBuffers(d)))                    # empty: forces a retry
+    outs = ops.marching_cubes_batch(items)
+    for (v, f), (rv, rf) in zip(outs, singles):
+        np.testing.assert_array_equal(v.cpu().numpy(), rv)
+        np.testing.assert_array_equal(f.cpu().numpy(), rf)
+    assert len(singles[2][1]) == 0 and len(outs[2][1]) == 0
