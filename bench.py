@@ -241,6 +241,47 @@ def batched(n_feat=64, n_steps=5):
             "wall_speedup": ws / wb, "max_depth_diff_um": dd}
 
 
+def adjoint_cost(nsub=3, reps=10):
+    """Cost of the differentiable advection (petch.adjoint.advect) on the trench grid: nsub
+    substeps forward against forward + backward (the HIP adjoint through every substep). Only
+    the advection stage is differentiable so far; the rest of the step is not."""
+    import torch
+    from petch import adjoint, ops
+    from petch.threed import grid_shape
+    dev = torch.device("cuda", 0)
+    nx, ny, nz = grid_shape(W['Lx'], W['Ly'], W['Lz'], W['dx'])
+    phi, mask = ops.init_sdf(nx, ny, nz, W['dx'], W['Lx'], W['Ly'], W['width'],
+                             W['sub_top'] + W['mask_th'] - W['floor'], W['floor'], False,
+                             device=dev)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    F = torch.where(phi.abs() < 4 * W['dx'], torch.rand(phi.shape, generator=g).to(dev), 0.0)
+    mask_phi = phi.clone()
+    w = torch.rand(phi.shape, generator=g).to(dev)
+    dt = 0.4 * W['dx'] / nsub
+
+    def run(grad):
+        x = phi.clone().requires_grad_(grad)
+        f = F.clone().requires_grad_(grad)
+        out = adjoint.advect(x, f, mask, mask_phi, W['dx'], dt, nsub)
+        if grad:
+            (out * w).sum().backward()
+
+    res = {}
+    for grad in (False, True):
+        run(grad)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            run(grad)
+        e.record()
+        torch.cuda.synchronize()
+        res["fwd_adj_ms" if grad else "fwd_ms"] = s.elapsed_time(e) / reps
+    return {"scope": f"level-set advection only ({nsub} substeps, {nx}x{ny}x{nz}); the flux, "
+                     "extension and reinit are not differentiated yet",
+            **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -336,6 +377,7 @@ def main(argv=None):
         res["small_hole"] = small_hole()
     if not a.no_batch:
         res["batch"] = batched()
+        res["adjoint"] = adjoint_cost()
     if not a.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline()

@@ -97,6 +97,14 @@ int petch_init_sdf(petch_ctx* ctx, float* phi, uint8_t* mask, int nx, int ny, in
  * buffer used for ping-pong. dt is the substep size (dt_step/nsub). */
 int petch_advect(petch_ctx* ctx, float* phi, float* work, const float* F, const uint8_t* mask,
                  const float* mask_phi, int nx, int ny, int nz, double dx, double dt, int nsub);
+/* Adjoint of one advection substep (petch_advect with nsub = 1) at the state phi it started
+ * from: gphi = dL/dphi for gout = dL/d(output), and gF += dL/dF. The upwind choices are those
+ * of phi; where |grad phi|_upwind = 0 the sqrt term contributes no derivative; mask nodes pass
+ * no gradient. No reference counterpart: the reference's etch loop is not differentiated
+ * (its only gradients are demo tapes, scripts/inverse_design.py:95-101). */
+int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const float* F, const uint8_t* mask,
+                         const float* gout, int nx, int ny, int nz, double dx, double dt,
+                         float* gphi, float* gF);
 
 /* ---------------------------------------------------------------- a20: FSM reinit -------- */
 /* reinit_fsm (threed.py:1515-1537) with _eik_init / _eik_jacobi / _eik_solve

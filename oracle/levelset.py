@@ -73,6 +73,50 @@ def advect_3d(phi, Fspeed, dx, dt):
     return phi - dt * np.asarray(Fspeed, np.float64) * np.sqrt(g)
 
 
+def advect_3d_adjoint(phi, Fspeed, mask, dx, dt, gout):
+    """Adjoint of one pinned advection substep (advect_3d, then out = mask_phi on mask nodes)
+    at the state phi: returns (dL/dphi, dL/dF) for dL/dout = gout. With G = sum_ax
+    max(D-,0)^2 + min(D+,0)^2, out = phi - dt F sqrt(G) away from the mask, so
+    dout/dF = -dt sqrt(G) and dout/dphi goes through phi itself and through the one-sided
+    differences of the six neighbours; where G = 0 the sqrt term has no derivative (0 is taken),
+    and mask nodes pass no gradient (they are re-pinned to a constant). Not in the reference,
+    which differentiates nothing in the etch loop; pinned by central differences of advect_3d."""
+    phi = np.asarray(phi, np.float64)
+    F = np.asarray(Fspeed, np.float64)
+    free = ~np.asarray(mask, bool)
+    go = np.where(free, np.asarray(gout, np.float64), 0.0)
+    dmp = []
+    G = np.zeros_like(phi)
+    for ax in range(3):
+        dm = np.zeros_like(phi)
+        dp = np.zeros_like(phi)
+        hi = [slice(None)] * 3
+        lo = [slice(None)] * 3
+        hi[ax] = slice(1, None)
+        lo[ax] = slice(0, -1)
+        diff = (phi[tuple(hi)] - phi[tuple(lo)]) / dx
+        dm[tuple(hi)] = diff
+        dp[tuple(lo)] = diff
+        dmp.append((np.maximum(dm, 0.0), np.minimum(dp, 0.0), tuple(hi), tuple(lo)))
+        G += np.maximum(dm, 0.0) ** 2 + np.minimum(dp, 0.0) ** 2
+    sq = np.sqrt(G)
+    # dL/dG at every node: gout * (-dt F / (2 sqrt G)), 0 where G = 0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        w = np.where(sq > 0.0, -dt * F * go / (2.0 * sq), 0.0)
+    gphi = go.copy()
+    for mp, mn, hi, lo in dmp:
+        # G_n holds max(D-_n,0)^2 with D-_n = (phi_n - phi_{n-1})/dx and min(D+_n,0)^2 with
+        # D+_n = (phi_{n+1} - phi_n)/dx; scatter dG_n/dphi onto phi_n and its neighbours
+        a = w * 2.0 * mp / dx                      # d/dphi_n of the D- term, -a onto phi_{n-1}
+        b = w * 2.0 * mn / dx                      # d/dphi_{n+1} of the D+ term, -b onto phi_n
+        gphi += a
+        gphi[lo] -= a[hi]
+        gphi -= b
+        gphi[hi] += b[lo]
+    gF = -dt * np.sqrt(G) * go
+    return gphi, gF
+
+
 def advect_pinned(phi, Fspeed, mask, mask_phi, dx, dt, nsub):
     """nsub substeps, each followed by the mask re-pin (threed.py:1761-1763; the device kernel
     threed.py:1268-1270 pins in the same substep)."""

@@ -97,6 +97,82 @@ __global__ void k_advect(const float* __restrict__ phi, const float* __restrict_
   out[n] = c - dt * F[n] * sqrtf(g);
 }
 
+// ------------------------------------------------------------------------------ advection adjoint
+// Adjoint of one substep at a state phi: out = phi - dt F sqrt(G) off the mask, mask_phi on it,
+// with G = sum over axes of max(D-,0)^2 + min(D+,0)^2. Node m gathers what it contributes to
+// its own output and to its six neighbours' G; where G = 0 the sqrt term has no derivative (0
+// is taken); mask nodes pass no gradient. gF accumulates (the speed is shared by the substeps).
+struct AdjStencil {
+  float G, mp[3], mn[3];  // G and, per axis, max(D-,0) and min(D+,0)
+};
+
+__device__ __forceinline__ AdjStencil adj_stencil(const float* __restrict__ phi, long long n,
+                                                  int i, int j, int k, int nx, int ny, int nz,
+                                                  float inv_dx) {
+  const long long sx = (long long)ny * nz, sy = nz;
+  const float c = phi[n];
+  AdjStencil a;
+  float dm, dp;
+  dm = (i >= 1) ? (c - phi[n - sx]) * inv_dx : 0.f;
+  dp = (i <= nx - 2) ? (phi[n + sx] - c) * inv_dx : 0.f;
+  a.mp[0] = fmaxf(dm, 0.f);
+  a.mn[0] = fminf(dp, 0.f);
+  dm = (j >= 1) ? (c - phi[n - sy]) * inv_dx : 0.f;
+  dp = (j <= ny - 2) ? (phi[n + sy] - c) * inv_dx : 0.f;
+  a.mp[1] = fmaxf(dm, 0.f);
+  a.mn[1] = fminf(dp, 0.f);
+  dm = (k >= 1) ? (c - phi[n - 1]) * inv_dx : 0.f;
+  dp = (k <= nz - 2) ? (phi[n + 1] - c) * inv_dx : 0.f;
+  a.mp[2] = fmaxf(dm, 0.f);
+  a.mn[2] = fminf(dp, 0.f);
+  a.G = a.mp[0] * a.mp[0] + a.mn[0] * a.mn[0] + a.mp[1] * a.mp[1] + a.mn[1] * a.mn[1] +
+        a.mp[2] * a.mp[2] + a.mn[2] * a.mn[2];
+  return a;
+}
+
+// dL/dG at node n: -dt F gout / (2 sqrt G), 0 on the mask or where G = 0
+__device__ __forceinline__ float adj_weight(const AdjStencil& a, const float* __restrict__ F,
+                                            const uint8_t* __restrict__ mask,
+                                            const float* __restrict__ gout, long long n,
+                                            float dt) {
+  if (mask[n] || !(a.G > 0.f)) return 0.f;
+  return -dt * F[n] * gout[n] / (2.f * sqrtf(a.G));
+}
+
+__global__ void k_advect_adjoint(const float* __restrict__ phi, const float* __restrict__ F,
+                                 const uint8_t* __restrict__ mask, const float* __restrict__ gout,
+                                 float* __restrict__ gphi, float* __restrict__ gF, int nx, int ny,
+                                 int nz, float inv_dx, float dt) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long N = (long long)nx * ny * nz;
+  if (m >= N) return;
+  const int k = (int)(m % nz);
+  const int j = (int)((m / nz) % ny);
+  const int i = (int)(m / ((long long)nz * ny));
+  const long long st[3] = {(long long)ny * nz, (long long)nz, 1};
+  const int idx[3] = {i, j, k}, lim[3] = {nx, ny, nz};
+  const AdjStencil self = adj_stencil(phi, m, i, j, k, nx, ny, nz, inv_dx);
+  const float wm = adj_weight(self, F, mask, gout, m, dt);
+  float g = mask[m] ? 0.f : gout[m];
+  for (int ax = 0; ax < 3; ++ax) g += wm * 2.f * (self.mp[ax] - self.mn[ax]) * inv_dx;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (idx[ax] + 1 < lim[ax]) {  // n = m + e_ax: its D- is (phi_n - phi_m)/dx
+      const long long n = m + st[ax];
+      const AdjStencil a = adj_stencil(phi, n, i + (ax == 0), j + (ax == 1), k + (ax == 2), nx,
+                                       ny, nz, inv_dx);
+      g -= adj_weight(a, F, mask, gout, n, dt) * 2.f * a.mp[ax] * inv_dx;
+    }
+    if (idx[ax] >= 1) {           // n = m - e_ax: its D+ is (phi_m - phi_n)/dx
+      const long long n = m - st[ax];
+      const AdjStencil a = adj_stencil(phi, n, i - (ax == 0), j - (ax == 1), k - (ax == 2), nx,
+                                       ny, nz, inv_dx);
+      g += adj_weight(a, F, mask, gout, n, dt) * 2.f * a.mn[ax] * inv_dx;
+    }
+  }
+  gphi[m] = g;
+  if (!mask[m]) gF[m] += -dt * sqrtf(self.G) * gout[m];
+}
+
 // ------------------------------------------------------------------------------ FSM reinit
 // Exact Godunov update of sum_d (x - a_d)_+^2 = dx^2 with the staged 1 -> 2 -> 3 term form,
 // evaluated in float64 relative to the smallest neighbour (y = x - a1) so the three-term root
@@ -243,6 +319,19 @@ extern "C" int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int 
     float* t = U; U = U2; U2 = t;
   }
   hipLaunchKernelGGL(k_eik_final, g, b, 0, ctx->stream, phi, U, N, (float)(band + dx));
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const float* F,
+                                    const uint8_t* mask, const float* gout, int nx, int ny, int nz,
+                                    double dx, double dt, float* gphi, float* gF) {
+  PETCH_ARG(ctx, ctx && phi && F && mask && gout && gphi && gF, "advect_adjoint: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "advect_adjoint: bad shape %d,%d,%d", nx, ny, nz);
+  PETCH_ARG(ctx, gphi != gout && gphi != phi, "advect_adjoint: gphi must not alias gout or phi");
+  const long long N = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(k_advect_adjoint, dim3(petch_blocks(N, kBS)), dim3(kBS), 0, ctx->stream, phi,
+                     F, mask, gout, gphi, gF, nx, ny, nz, (float)(1.0 / dx), (float)dt);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

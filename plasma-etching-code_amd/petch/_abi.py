@@ -153,6 +153,7 @@ SIGNATURES = {
     "petch_extend": [P, P, P, P, c_int, c_int, c_int, c_double, c_double, P],
     "petch_nearest_face_value": [P, P, P, P, c_int, c_float, P],
     "petch_extend_batch": [P, c_int, P],
+    "petch_advect_adjoint": [P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, P, P],
     "petch_marching_cubes_batch": [P, c_int, P],
     "petch_nearest_face_value_batch": [P, c_int, P],
     "petch_radiosity_assemble": [P, P, c_int, c_int, P, P, P],

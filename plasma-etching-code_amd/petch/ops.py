@@ -45,6 +45,17 @@ def advect(phi, F, mask, mask_phi, dx, dt, nsub, work=None):
     return phi
 
 
+def advect_adjoint(phi, F, mask, gout, dx, dt, gF):
+    """Adjoint of one advection substep at the state phi: returns dL/dphi for
+    gout = dL/d(output) and adds dL/dF to gF (petch_advect_adjoint)."""
+    nx, ny, nz = _shape3(phi)
+    gphi = torch.empty_like(phi)
+    _ctx(phi).call("petch_advect_adjoint", ptr(phi, f32, "phi"), ptr(F, f32, "F"),
+                   ptr(mask, u8, "mask"), ptr(gout, f32, "gout"), nx, ny, nz, float(dx),
+                   float(dt), ptr(gphi, f32), ptr(gF, f32, "gF"))
+    return gphi
+
+
 def reinit_fsm(phi, dx, band, n_iter=None):
     """Jacobi Godunov-Eikonal reinit in place (threed.py:1515-1537)."""
     nx, ny, nz = _shape3(phi)

@@ -177,3 +177,37 @@ def test_depth_metrics_on_known_floor():
     phi2[16, 16, 4] = -1.0
     geo['phi'] = phi2
     assert ol.depth3d(geo) == pytest.approx(sub_top - floor)
+
+
+def _adjoint_case(seed):
+    rng = np.random.default_rng(seed)
+    shape, dx, dt = (7, 6, 9), 0.1, 0.02
+    phi = rng.normal(size=shape)
+    F = rng.uniform(0.2, 2.0, shape)
+    mask = rng.uniform(size=shape) < 0.2
+    mp = rng.normal(size=shape)
+    go = rng.normal(size=shape)
+    return phi, F, mask, mp, go, dx, dt
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_advect_adjoint_matches_central_differences(seed):
+    """The oracle's adjoint of a pinned advection substep against float64 central differences
+    of advect_3d (directional derivatives along random directions in phi and in F)."""
+    phi, F, mask, mp, go, dx, dt = _adjoint_case(seed)
+
+    def loss(p, f):
+        return float((go * np.where(mask, mp, ol.advect_3d(p, f, dx, dt))).sum())
+
+    gphi, gF = ol.advect_3d_adjoint(phi, F, mask, dx, dt, go)
+    rng = np.random.default_rng(seed + 10)
+    h = 1e-6
+    for _ in range(3):
+        v = rng.normal(size=phi.shape)
+        fd = (loss(phi + h * v, F) - loss(phi - h * v, F)) / (2 * h)
+        assert abs(fd - (gphi * v).sum()) <= 1e-6 * max(1.0, abs(fd))
+        u = rng.normal(size=phi.shape)
+        fd = (loss(phi, F + h * u) - loss(phi, F - h * u)) / (2 * h)
+        assert abs(fd - (gF * u).sum()) <= 1e-6 * max(1.0, abs(fd))
+    # mask nodes pass nothing through: their F gradient is exactly 0
+    assert np.all(gF[mask] == 0.0)
