@@ -282,6 +282,49 @@ def adjoint_cost(nsub=3, reps=10):
             **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
 
 
+def radiosity_adjoint_cost(reps=5, n_ff=64, tol=1e-10):
+    """Cost of the differentiable radiosity solve (petch.adjoint.radiosity) on the trench
+    surface: the converged solve (tol) forward against forward + transposed solve, with the
+    form factors of n_ff rays per face assembled once outside the timing."""
+    import torch
+    from petch import adjoint, ops
+    from petch.flux import SurfaceMesh
+    from petch.threed import grid_shape
+    dev = torch.device("cuda", 0)
+    nx, ny, nz = grid_shape(W['Lx'], W['Ly'], W['Lz'], W['dx'])
+    phi, _ = ops.init_sdf(nx, ny, nz, W['dx'], W['Lx'], W['Ly'], W['width'],
+                          W['sub_top'] + W['mask_th'] - W['floor'], W['floor'], False, device=dev)
+    mesh = SurfaceMesh(dev).build(phi, W['dx'])
+    gn = ops.gas_normals(phi, W['dx'], mesh.centroids, mesh.normals)
+    box = ops.make_box(W['Lx'], W['Ly'], W['Lz'], W['Lz'] - W['dx'], 1)
+    hit = ops.trace_ff(mesh.bvh, box, mesh.centroids, gn, n_ff, 5)
+    nf = mesh.nf
+    row_ptr, col, D = ops.radiosity_assemble(hit, nf, n_ff)
+    g = torch.Generator(device="cpu").manual_seed(4)
+    bare = (0.2 + 0.8 * torch.rand(nf, generator=g)).to(dev)
+    w = torch.rand(nf, generator=g).to(dev)
+
+    def run(grad):
+        x = bare.clone().requires_grad_(grad)
+        m = adjoint.radiosity(x, D[:nf], hit, row_ptr, col, n_ff, 0.7, 0, tol)
+        if grad:
+            (m * w).sum().backward()
+
+    res = {}
+    for grad in (False, True):
+        run(grad)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            run(grad)
+        e.record()
+        torch.cuda.synchronize()
+        res["fwd_adj_ms" if grad else "fwd_ms"] = s.elapsed_time(e) / reps
+    return {"scope": f"radiosity solve to tol {tol:g} ({nf} faces, {n_ff} form-factor rays each)",
+            **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,6 +421,7 @@ def main(argv=None):
     if not a.no_batch:
         res["batch"] = batched()
         res["adjoint"] = adjoint_cost()
+        res["adjoint_radiosity"] = radiosity_adjoint_cost()
     if not a.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline()

@@ -352,6 +352,17 @@ int petch_radiosity_assemble(petch_ctx* ctx, const int* hitface, int nf, int n_f
 int petch_radiosity_solve(petch_ctx* ctx, const int* row_ptr, const int* col, const double* D,
                           int nf, int n_ff, const float* bare, double beta, int n_iter, double tol,
                           double* G, float* out, int* iters_host);
+/* Adjoint of petch_radiosity_solve at its solution G (float64) for the output gradient gm
+ * (float32, d loss / d out): the transposed system lambda = gG + diag(q) Aff^T lambda, with
+ * q = 1 - clip(bare*beta,0,1) and gG = gm where 0 <= G <= 8, solved by Jacobi from lambda0 = gG
+ * under the same n_iter / tol rule as the solve; Aff^T is read straight from hitface (the
+ * petch_trace_ff output the CSR was assembled from). gD (nf, float64) = lambda = d loss / dD;
+ * gbare (nf, float32) = -beta [0 <= bare*beta <= 1] G (Aff^T lambda) = d loss / d bare.
+ * No reference counterpart (the reference does not differentiate its radiosity solve,
+ * threed.py:965-985); it is the implicit-function adjoint, exact for a converged solve. */
+int petch_radiosity_adjoint(petch_ctx* ctx, const int* hitface, int nf, int n_ff, const double* G,
+                            const float* bare, double beta, const float* gm, int n_iter,
+                            double tol, double* gD, float* gbare, int* iters_host);
 
 #ifdef __cplusplus
 }

@@ -129,6 +129,29 @@ def radiosity_direct(D, Aff, bare, beta):
     return spl.spsolve((sp.identity(len(D), format='csc') - M).tocsc(), D)
 
 
+def radiosity_adjoint(D, Aff, G, bare, beta, gm, n_iter=0):
+    """Adjoint of radiosity_solve / radiosity_direct at the solution G (not in the reference):
+    lambda solves (I - Aff diag(q))^T lambda = gG with q = 1 - clip(bare beta, 0, 1) and
+    gG = gm where 0 <= G <= 8 (n_iter > 0: that many transposed Jacobi sweeps from gG; else a
+    sparse direct solve). Returns (dloss/dD = lambda, dloss/dbare =
+    -beta [0 <= bare beta <= 1] G (Aff^T lambda))."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    sb = np.asarray(bare, np.float64) * beta
+    q = 1.0 - np.clip(sb, 0.0, 1.0)
+    G = np.asarray(G, np.float64)
+    gG = np.where((G >= 0.0) & (G <= 8.0), np.asarray(gm, np.float64), 0.0)
+    MT = Aff.multiply(q[None, :]).T.tocsr()
+    if n_iter > 0:
+        lam = gG.copy()
+        for _ in range(n_iter):
+            lam = gG + MT.dot(lam)
+    else:
+        lam = spl.spsolve((sp.identity(len(gG), format='csc') - MT).tocsc(), gG)
+    gq = G * Aff.T.dot(lam)
+    return lam, np.where((sb >= 0.0) & (sb <= 1.0), -beta * gq, 0.0)
+
+
 def deterministic_ions(M, geo, par, n_ion, seed, flags):
     """_ions_deterministic (threed.py:994-1029)."""
     box = _box(geo, par)

@@ -159,6 +159,8 @@ SIGNATURES = {
     "petch_radiosity_assemble": [P, P, c_int, c_int, P, P, P],
     "petch_radiosity_solve": [P, P, P, P, c_int, c_int, P, c_double, c_int, c_double, P, P,
                               POINTER(c_int)],
+    "petch_radiosity_adjoint": [P, P, c_int, c_int, P, P, c_double, P, c_int, c_double, P, P,
+                                POINTER(c_int)],
 }
 
 _lib = None

@@ -516,6 +516,25 @@ def radiosity_solve(row_ptr, col, D, nf, n_ff, bare, beta, n_iter=40, tol=1e-10)
     return out, G[:nf], it.value
 
 
+def radiosity_adjoint(hitface, nf, n_ff, G, bare, beta, gm, n_iter=40, tol=1e-10):
+    """(gD float64 = d loss / dD, gbare float32 = d loss / d bare, sweeps) for the output gradient
+    gm of radiosity_solve at its solution G (petch_radiosity_adjoint)."""
+    import ctypes
+    for t, n, name in ((hitface, nf * n_ff, "hitface"), (G, nf, "G"), (bare, nf, "bare"),
+                       (gm, nf, "gm")):
+        if t.numel() < n:
+            raise ValueError(f"radiosity_adjoint: {name} holds {t.numel()} values, needs {n}")
+    gD = torch.empty((max(nf, 1),), dtype=torch.float64, device=G.device)
+    gbare = torch.empty((nf,), dtype=f32, device=G.device)
+    it = ctypes.c_int()
+    if nf:
+        _ctx(G).call("petch_radiosity_adjoint", ptr(hitface, i32, "hitface"), int(nf), int(n_ff),
+                     ptr(G, torch.float64, "G"), ptr(bare, f32, "bare"), float(beta),
+                     ptr(gm, f32, "gm"), int(n_iter), float(tol), ptr(gD), ptr(gbare),
+                     ctypes.byref(it))
+    return gD[:nf], gbare, it.value
+
+
 def ctypes_ref(s):
     import ctypes
     return ctypes.byref(s)

@@ -197,11 +197,8 @@ def _assert_tie_only(v, f, pts, differ, gpu_face, dist_min):
         assert abs(d[0] - dist_min[i]) <= 1e-4 * max(dist_min[i], 1.0), (i, d[0], dist_min[i])
 
 
-@pytest.mark.parametrize("name", ["hole", "trench"])
-def test_radiosity_solve_fp64(name):
-    """Same form factors on both sides: the GPU float64 Jacobi equals scipy's to 1e-5 relative,
-    and the converged solve equals a sparse direct solve (the FP64 oracle)."""
-    import torch
+def _ff_case(name):
+    """Form-factor rays of a hole or a trench surface: (hitface, F, n_ff, row_ptr, col, D)."""
     from petch import ops
     if name == "hole":
         phi = ol.init_sdf(8.0, 8.0, 14.0, 0.25, 3.0, 8.0, 4.0, True).astype(np.float32)
@@ -221,6 +218,15 @@ def test_radiosity_solve_fp64(name):
     hit = ops.trace_ff(bvh, box, cen, gn, n_ff, 17)
     F = len(f)
     row_ptr, col, D = ops.radiosity_assemble(hit, F, n_ff)
+    return hit, F, n_ff, row_ptr, col, D
+
+
+@pytest.mark.parametrize("name", ["hole", "trench"])
+def test_radiosity_solve_fp64(name):
+    """Same form factors on both sides: the GPU float64 Jacobi equals scipy's to 1e-5 relative,
+    and the converged solve equals a sparse direct solve (the FP64 oracle)."""
+    from petch import ops
+    hit, F, n_ff, row_ptr, col, D = _ff_case(name)
     Dh, Aff = ofl.radiosity_matrix(hit.cpu().numpy(), F, n_ff)
     np.testing.assert_array_equal(D.cpu().numpy()[:F], Dh)
     bare = np.random.default_rng(7).uniform(0.2, 1.0, F).astype(np.float32)
@@ -233,6 +239,44 @@ def test_radiosity_solve_fp64(name):
         direct = ofl.radiosity_direct(Dh, Aff, bare.astype(np.float64), beta)
         np.testing.assert_allclose(G.cpu().numpy(), direct, rtol=1e-5, atol=1e-10)
         assert it > 0
+
+
+@pytest.mark.parametrize("name", ["hole", "trench"])
+def test_radiosity_adjoint_matches_oracle(name):
+    """The transposed solve on the GPU (gathering over hitface) against the oracle's adjoint on
+    the scipy matrix of the same rays: 40 sweeps against 40 transposed sweeps, and the converged
+    adjoint (through the autograd Function) against the direct sparse solve, to 1e-5 of the
+    largest gradient. Some faces get bare*beta > 1 (no gradient through the clip)."""
+    import torch
+    from petch import adjoint, ops
+    hit, F, n_ff, row_ptr, col, D = _ff_case(name)
+    Dh, Aff = ofl.radiosity_matrix(hit.cpu().numpy(), F, n_ff)
+    rng = np.random.default_rng(11)
+    bare = rng.uniform(0.2, 1.0, F).astype(np.float32)
+    bare[::17] = 1.5
+    w = rng.normal(size=F).astype(np.float32)
+    b64 = bare.astype(np.float64)
+
+    def close(a, b):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-5 * max(np.abs(b).max(), 1e-30))
+
+    for beta in (0.7, 1.0):
+        m, G, _ = ops.radiosity_solve(row_ptr, col, D, F, n_ff, _t(bare), beta, 40)
+        gD, gb, it = ops.radiosity_adjoint(hit, F, n_ff, G, _t(bare), beta, _t(w), 40)
+        assert it == 40
+        rD, rb = ofl.radiosity_adjoint(Dh, Aff, G.cpu().numpy(), b64, beta, w.astype(np.float64),
+                                       n_iter=40)
+        close(gD.cpu().numpy(), rD)
+        close(gb.cpu().numpy(), rb)
+        assert np.all(gb.cpu().numpy()[bare * beta > 1.0] == 0.0)
+        x = _t(bare).requires_grad_(True)
+        d = D[:F].clone().requires_grad_(True)
+        out = adjoint.radiosity(x, d, hit, row_ptr, col, n_ff, beta, 0, 1e-12)
+        (out * _t(w)).sum().backward()
+        Gd = ofl.radiosity_direct(Dh, Aff, b64, beta)
+        rD, rb = ofl.radiosity_adjoint(Dh, Aff, Gd, b64, beta, w.astype(np.float64))
+        close(d.grad.cpu().numpy(), rD)
+        close(x.grad.cpu().numpy(), rb)
 
 
 def test_batched_extension_and_warm_start_equal_single_calls():
