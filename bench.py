@@ -325,6 +325,47 @@ def radiosity_adjoint_cost(reps=5, n_ff=64, tol=1e-10):
             **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
 
 
+def neutral_adjoint_cost(reps=3, beta=0.7):
+    """Cost of the differentiable RR neutral trace (petch.adjoint.neutral_rr) on the trench
+    surface with the bench's 8M rays of one species: the trace forward against forward +
+    path-replay adjoint (every roulette segment of two or more hits traced twice more)."""
+    import torch
+    from petch import adjoint, ops
+    from petch.flux import SurfaceMesh
+    from petch.threed import grid_shape
+    dev = torch.device("cuda", 0)
+    nx, ny, nz = grid_shape(W['Lx'], W['Ly'], W['Lz'], W['dx'])
+    phi, _ = ops.init_sdf(nx, ny, nz, W['dx'], W['Lx'], W['Ly'], W['width'],
+                          W['sub_top'] + W['mask_th'] - W['floor'], W['floor'], False, device=dev)
+    mesh = SurfaceMesh(dev, rays_per_step=N_NEU * 2 * N_FP).build(phi, W['dx'])
+    box = ops.make_box(W['Lx'], W['Ly'], W['Lz'], W['Lz'] - W['dx'], 1)
+    src = ops.make_source(ops.SRC_NEUTRAL)
+    g = torch.Generator(device="cpu").manual_seed(6)
+    bare = (0.2 + 0.8 * torch.rand(mesh.nf, generator=g)).to(dev)
+    w = torch.randn(mesh.nf, generator=g).to(dev)
+
+    def run(grad):
+        x = bare.clone().requires_grad_(grad)
+        fl = adjoint.neutral_rr(mesh.bvh, box, src, x, beta, N_NEU, 11)
+        if grad:
+            (fl * w).sum().backward()
+
+    res = {}
+    for grad in (False, True):
+        run(grad)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            run(grad)
+        e.record()
+        torch.cuda.synchronize()
+        res["fwd_adj_ms" if grad else "fwd_ms"] = s.elapsed_time(e) / reps
+    return {"scope": f"RR neutral trace, {N_NEU} rays of one species on the trench surface "
+                     f"({mesh.nf} faces), path-replay adjoint to bare",
+            **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,6 +463,7 @@ def main(argv=None):
         res["batch"] = batched()
         res["adjoint"] = adjoint_cost()
         res["adjoint_radiosity"] = radiosity_adjoint_cost()
+        res["adjoint_neutral"] = neutral_adjoint_cost()
     if not a.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline()

@@ -97,11 +97,18 @@ int petch_init_sdf(petch_ctx* ctx, float* phi, uint8_t* mask, int nx, int ny, in
  * buffer used for ping-pong. dt is the substep size (dt_step/nsub). */
 int petch_advect(petch_ctx* ctx, float* phi, float* work, const float* F, const uint8_t* mask,
                  const float* mask_phi, int nx, int ny, int nz, double dx, double dt, int nsub);
+/* One advection substep of petch_advect out of place: out = the substep from phi (out != phi).
+ * No reference counterpart: the differentiable advection keeps every substep's state with it. */
+int petch_advect_step(petch_ctx* ctx, const float* phi, const float* F, const uint8_t* mask,
+                      const float* mask_phi, int nx, int ny, int nz, double dx, double dt,
+                      float* out);
 /* Adjoint of one advection substep (petch_advect with nsub = 1) at the state phi it started
  * from: gphi = dL/dphi for gout = dL/d(output), and gF += dL/dF. The upwind choices are those
  * of phi; where |grad phi|_upwind = 0 the sqrt term contributes no derivative; mask nodes pass
- * no gradient. No reference counterpart: the reference's etch loop is not differentiated
- * (its only gradients are demo tapes, scripts/inverse_design.py:95-101). */
+ * no gradient. gphi must not alias phi, gout or gF. Two launches (each node's own stencil,
+ * then what its neighbours' stencils owe it) through a grid-sized scratch buffer. No reference
+ * counterpart: the reference's etch loop is not differentiated (its only gradients are demo
+ * tapes, scripts/inverse_design.py:95-101). */
 int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const float* F, const uint8_t* mask,
                          const float* gout, int nx, int ny, int nz, double dx, double dt,
                          float* gphi, float* gF);
@@ -200,6 +207,16 @@ int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petch_source* s
 int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
                            const petch_source* src, const float* bare, float beta, int n,
                            uint64_t seed, float* fl);
+/* Path-replay adjoint of petch_trace_neutral_rr with the same (bare, beta, n, seed): for
+ * gfl = d loss / d fl, gbare (nf, zeroed first) = d loss / d bare with every ray's path and its
+ * roulette / termination decisions held fixed (the rays redraw their streams). Each ray traces
+ * every roulette segment of two or more hits twice: once to total gfl[f_k] w_k over the
+ * segment, once to add -beta (segment total - running total) / (1 - S_i) to gbare[f_i].
+ * gbare must not alias bare or gfl. No reference counterpart (the reference does not
+ * differentiate _trace3d_cov_rr, threed.py:549-596). */
+int petch_trace_neutral_rr_adjoint(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                                   const petch_source* src, const float* bare, float beta, int n,
+                                   uint64_t seed, const float* gfl, float* gbare);
 /* The two _trace3d_cov_rr launches of one flux/coverage iteration (F then O, threed.py:876-880)
  * as one launch over the same surface and coverage: species a (beta_a, n_a rays, seed_a, fl_a)
  * and species b. Each ray keeps the stream it has in two petch_trace_neutral_rr calls, so the
@@ -292,6 +309,16 @@ int petch_belen_coverage(petch_ctx* ctx, const float* m_i, const float* m_F, con
                          const float* cos_i, const float* iy_ie, const float* iy_ip,
                          const petch_chem_params* cp, int F, float* bare, float* thF,
                          float* thO);
+/* Closed-form adjoint of petch_belen_coverage for gbare = d loss / d bare (zero where the clip
+ * of 1 - thF - thO is active): gm_F, gm_O = d loss / d m_F, d m_O; with iy_ie/iy_ip also
+ * g_ie, g_ip = d loss / d iy_ie, d iy_ip, else gm_i = d loss / d m_i (cos_i held constant).
+ * gm_i, g_ie and g_ip may be NULL. No reference counterpart (the reference does not
+ * differentiate _belen_coverages, threed.py:715-737). */
+int petch_belen_coverage_adjoint(petch_ctx* ctx, const float* m_i, const float* m_F,
+                                 const float* m_O, const float* cos_i, const float* iy_ie,
+                                 const float* iy_ip, const petch_chem_params* cp, int F,
+                                 const float* gbare, float* gm_F, float* gm_O, float* gm_i,
+                                 float* g_ie, float* g_ip);
 /* surface_rate (chemistry.py:83-92 -> belen.py:26-63 or chemistry.py:62-80), V = 0 on mask
  * faces, non-finite V -> 0 (threed.py:1723-1728). Mask faces are is_mask[f] != 0 when is_mask is
  * given, else faces_in_mask of the centroids (threed.py:1608-1616). red[0] = max(V) as float

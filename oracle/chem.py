@@ -7,6 +7,8 @@ Restates:
   surface_rate_langmuir chemistry.py:62-80
   surface_rate_belen   belen.py:26-63
   belen_coverages      threed.py:715-737
+  belen_bare_adjoint   derivative of clip(1 - thF - thO, 0, 1) (not in the reference), by
+                       complex-step differentiation of belen_coverages
   faces_in_mask        threed.py:1608-1616
   ion_params           threed.py:800-802, 811
 """
@@ -112,23 +114,56 @@ def belen_coverages(m_i, m_F, m_O, cos_i, par, flags, ion_yield=None):
     """threed.py:715-737 (note: ignores fnorm_E/O, unlike surface_rate_belen)."""
     Yie, Ysp, Yp = yields(par)
     f_ie, _ = angular_factors(cos_i, par, _mode(flags))
-    Fi = par['ionFlux'] * np.asarray(m_i, np.float64)
+    Fi = par['ionFlux'] * _num(m_i)
     eps = 1e-9
     iy = ion_yield if ion_yield is not None else par.get('_ion_yield')
     if iy is not None:
-        _sp, _ie, _ip = (np.asarray(a, np.float64) for a in iy)
+        _sp, _ie, _ip = (_num(a) for a in iy)
         GY_ie = par['ionFlux'] * _ie
         GY_p = par['ionFlux'] * _ip
     else:
         GY_ie = Yie * f_ie * Fi
         GY_p = Yp * f_ie * Fi
-    Gb_E = par['Fflux'] * np.asarray(m_F, np.float64) + eps
-    Gb_P = par['Oflux'] * np.asarray(m_O, np.float64) + eps
+    Gb_E = par['Fflux'] * _num(m_F) + eps
+    Gb_P = par['Oflux'] * _num(m_O) + eps
     a = (par['k_sigma'] + 2.0 * GY_ie) / Gb_E
     b = (par['beta_sigma'] + GY_p) / Gb_P
     thF = 1.0 / (1.0 + a * (1.0 + 1.0 / (b + eps)))
     thO = 1.0 / (1.0 + b * (1.0 + 1.0 / (a + eps)))
     return thF, thO
+
+
+def _num(x):
+    """float64 array, or complex128 when x is complex (complex-step derivatives)."""
+    return np.asarray(x) if np.iscomplexobj(x) else np.asarray(x, np.float64)
+
+
+def belen_bare_adjoint(m_i, m_F, m_O, cos_i, par, flags, gbare, ion_yield=None):
+    """d loss / d inputs of bare = clip(1 - thF - thO, 0, 1) for gbare = d loss / d bare, by
+    complex-step differentiation of belen_coverages: every face's bare depends on that face's
+    inputs only, so one imaginary step h on all faces at once gives the whole diagonal
+    Jacobian as Im(bare) / h, exact to rounding. Returns dict(m_F, m_O, and m_i or ie, ip);
+    cos_i is held constant and the clip passes the gradient where 0 <= 1 - thF - thO <= 1."""
+    h = 1e-30
+    ins = dict(m_i=m_i, m_F=m_F, m_O=m_O)
+    if ion_yield is not None:
+        ins.update(sp=ion_yield[0], ie=ion_yield[1], ip=ion_yield[2])
+
+    def bare_of(v):
+        iy = (v['sp'], v['ie'], v['ip']) if ion_yield is not None else None
+        thF, thO = belen_coverages(v['m_i'], v['m_F'], v['m_O'], cos_i, par, flags, ion_yield=iy)
+        return 1.0 - thF - thO
+
+    base = {k: np.asarray(x, np.float64) for k, x in ins.items()}
+    br = bare_of(base)
+    g = np.where((br >= 0.0) & (br <= 1.0), np.asarray(gbare, np.float64), 0.0)
+    keys = ("m_F", "m_O") + (("ie", "ip") if ion_yield is not None else ("m_i",))
+    out = {}
+    for k in keys:
+        v = dict(base)
+        v[k] = base[k] + 1j * h
+        out[k] = g * bare_of(v).imag / h
+    return out
 
 
 def faces_in_mask(centroids, geo, mask_th, trench_width, hole=False):

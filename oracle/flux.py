@@ -152,6 +152,39 @@ def radiosity_adjoint(D, Aff, G, bare, beta, gm, n_iter=0):
     return lam, np.where((sb >= 0.0) & (sb <= 1.0), -beta * gq, 0.0)
 
 
+def radiosity_coverage(D, Aff, m_i, cos_i, par, flags, n_fp=4, ion_yield=None, w=None):
+    """The coverage fixed point of radiosity_flux (threed.py:963-990) with converged solves:
+    n_fp x {m_F, m_O = clip(G) of the F and O solves at bare; bare = Belen(m_i, m_F, m_O)} from
+    bare = 1. With w, also the adjoint of loss = w . bare_final (not in the reference), unrolled
+    over the n_fp iterations, each solve by radiosity_adjoint and each Belen step by
+    belen_bare_adjoint. Returns (bare, m_F, m_O[, grads dict(m_i or ie/ip, D)])."""
+    betaE, betaO = par.get('betaE', 0.7), par.get('betaO', 1.0)
+    bare = np.ones(len(D))
+    tape = []
+    for _ in range(n_fp):
+        GF = radiosity_direct(D, Aff, bare, betaE)
+        GO = radiosity_direct(D, Aff, bare, betaO)
+        mF, mO = np.clip(GF, 0.0, 8.0), np.clip(GO, 0.0, 8.0)
+        tape.append((bare, GF, GO, mF, mO))
+        thF, thO = oc.belen_coverages(m_i, mF, mO, cos_i, par, flags, ion_yield=ion_yield)
+        bare = np.clip(1.0 - thF - thO, 0.0, 1.0)
+    if w is None:
+        return bare, mF, mO
+    keys = ("ie", "ip") if ion_yield is not None else ("m_i",)
+    grads = {k: np.zeros(len(D)) for k in keys}
+    grads["D"] = np.zeros(len(D))
+    gbare = np.asarray(w, np.float64)
+    for b_in, GF, GO, mF, mO in reversed(tape):
+        g = oc.belen_bare_adjoint(m_i, mF, mO, cos_i, par, flags, gbare, ion_yield=ion_yield)
+        for k in keys:
+            grads[k] += g[k]
+        dF, bF = radiosity_adjoint(D, Aff, GF, b_in, betaE, g["m_F"])
+        dO, bO = radiosity_adjoint(D, Aff, GO, b_in, betaO, g["m_O"])
+        grads["D"] += dF + dO
+        gbare = bF + bO
+    return bare, mF, mO, grads
+
+
 def deterministic_ions(M, geo, par, n_ion, seed, flags):
     """_ions_deterministic (threed.py:994-1029)."""
     box = _box(geo, par)

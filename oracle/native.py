@@ -141,6 +141,27 @@ class OracleBvh:
                                       ctypes.c_uint64(seed), _p(out))
         return out
 
+    def trace_neutral_rr_adjoint(self, box, src, bare, beta, n, seed, gfl):
+        """d loss / d bare of trace_neutral_rr for gfl = d loss / d fl, paths and decisions fixed."""
+        bare = np.ascontiguousarray(bare, np.float32)
+        g = np.ascontiguousarray(gfl, np.float64)
+        out = np.zeros(self.nf, np.float64)
+        lib().oracle_trace_neutral_rr_adjoint(self.h, ctypes.byref(box), ctypes.byref(src),
+                                              _p(bare), ctypes.c_float(beta), ctypes.c_int(n),
+                                              ctypes.c_uint64(seed), _p(g), _p(out))
+        return out
+
+    def trace_neutral_rr_frozen(self, box, src, bare_dec, bare_w, beta, n, seed):
+        """trace_neutral_rr with paths and decisions from bare_dec, weights (float64) from
+        bare_w."""
+        a = np.ascontiguousarray(bare_dec, np.float32)
+        b = np.ascontiguousarray(bare_w, np.float32)
+        out = np.zeros(self.nf, np.float64)
+        lib().oracle_trace_neutral_rr_frozen(self.h, ctypes.byref(box), ctypes.byref(src), _p(a),
+                                             _p(b), ctypes.c_float(beta), ctypes.c_int(n),
+                                             ctypes.c_uint64(seed), _p(out))
+        return out
+
     def trace_ion_yield(self, box, src, ion, n, seed):
         out = np.zeros((5, self.nf), np.float64)
         lib().oracle_trace_ion_yield(self.h, ctypes.byref(box), ctypes.byref(src),

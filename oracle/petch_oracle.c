@@ -686,6 +686,124 @@ void oracle_trace_neutral_rr(void* h, const oc_box* b, const oc_source* s, const
   }
 }
 
+/* oracle_trace_neutral_rr with the decisions frozen (test instrument, not in the reference): the
+ * paths, roulettes and terminations are those of the sticking bare_dec, while the deposited
+ * weights follow bare_w (w_{k+1} = w_k (1 - bare_w beta), 0.3 where the bare_dec ray survived a
+ * roulette). At bare_w = bare_dec it is oracle_trace_neutral_rr; its central differences in
+ * bare_w are the derivative the path adjoint computes, with no decision flipping. */
+void oracle_trace_neutral_rr_frozen(void* h, const oc_box* b, const oc_source* s,
+                                    const float* bare_dec, const float* bare_w, float beta, int n,
+                                    uint64_t seed, double* fl) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  int F = B->nf;
+  memset(fl, 0, sizeof(double) * F);
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc(F > 0 ? F : 1, sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+    for (int p = 0; p < n; ++p) {
+      oc_rng r;
+      oc_rng_init(&r, seed, (uint32_t)p);
+      f3 o, d, nn;
+      oc_source_ray(&r, b, s, &o, &d);
+      float w = 1.f, t;
+      double ww = 1.0;
+      for (int bounce = 0; bounce < 1024; ++bounce) {
+        int f = oc_query_bc(B, b, &o, d, &t, &nn);
+        if (f < 0) break;
+        if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
+        acc[f] += ww;
+        w = w - w * (bare_dec[f] * beta);
+        ww = ww * (1.0 - (double)bare_w[f] * beta);
+        if (w <= 1.0e-6f) break;
+        uint32_t u[4];
+        oc_block(seed, (uint32_t)p, (uint32_t)bounce + 1u, 1u, u);
+        if (w < 0.1f) {
+          if (oc_u24(u[0]) < (1.f - w / 0.3f)) break;
+          w = 0.3f;
+          ww = 0.3;
+        }
+        f3 hit = add3(o, scl(t, d));
+        d = oc_cosine_u(nn, oc_u24(u[1]), oc_u24(u[2]));
+        o = add3(hit, scl(1.0e-4f, nn));
+      }
+    }
+#pragma omp critical
+    for (int f = 0; f < F; ++f) fl[f] += acc[f];
+    free(acc);
+  }
+}
+
+/* Path adjoint of oracle_trace_neutral_rr (not in the reference): gbare = d loss / d bare for
+ * gfl = d loss / d fl, with every path's geometry and its roulette / termination decisions held
+ * fixed. Each path is traced once and stored (face, weight, sticking, whether its next weight
+ * was reset to 0.3 by a survived roulette); then, from the last hit back,
+ *   Q_k = 0 where the next weight does not depend on w_k (reset at k, or the last hit), else
+ *   Q_k = gfl[f_{k+1}] + (1 - S_{k+1}) Q_{k+1} with Q_{k+1} as defined for hit k+1,
+ * is the derivative of the path's later deposits by w_{k+1}, and d loss / d S_k = -w_k Q_k
+ * (w_{k+1} = w_k (1 - S_k)), so gbare[f_k] += -beta w_k Q_k. Per-face sums in double. */
+void oracle_trace_neutral_rr_adjoint(void* h, const oc_box* b, const oc_source* s,
+                                     const float* bare, float beta, int n, uint64_t seed,
+                                     const double* gfl, double* gbare) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  int F = B->nf;
+  memset(gbare, 0, sizeof(double) * F);
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc(F > 0 ? F : 1, sizeof(double));
+    int* pf = (int*)malloc(sizeof(int) * 1024);
+    float* pw = (float*)malloc(sizeof(float) * 1024);
+    float* pS = (float*)malloc(sizeof(float) * 1024);
+    char* prs = (char*)malloc(1024);
+#pragma omp for schedule(dynamic, 256)
+    for (int p = 0; p < n; ++p) {
+      oc_rng r;
+      oc_rng_init(&r, seed, (uint32_t)p);
+      f3 o, d, nn;
+      oc_source_ray(&r, b, s, &o, &d);
+      float w = 1.f, t;
+      int K = 0;
+      for (int bounce = 0; bounce < 1024; ++bounce) {
+        int f = oc_query_bc(B, b, &o, d, &t, &nn);
+        if (f < 0) break;
+        if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
+        float S = bare[f] * beta;
+        pf[K] = f;
+        pw[K] = w;
+        pS[K] = S;
+        prs[K] = 0;
+        ++K;
+        w = w - w * S;
+        if (w <= 1.0e-6f) break;
+        uint32_t u[4];
+        oc_block(seed, (uint32_t)p, (uint32_t)bounce + 1u, 1u, u);
+        if (w < 0.1f) {
+          if (oc_u24(u[0]) < (1.f - w / 0.3f)) break;
+          w = 0.3f;
+          prs[K - 1] = 1;
+        }
+        f3 hit = add3(o, scl(t, d));
+        d = oc_cosine_u(nn, oc_u24(u[1]), oc_u24(u[2]));
+        o = add3(hit, scl(1.0e-4f, nn));
+      }
+      double Qn = 0.0; /* Q of the hit after k */
+      for (int k = K - 1; k >= 0; --k) {
+        double Q = 0.0;
+        if (k + 1 < K && !prs[k]) Q = gfl[pf[k + 1]] + (1.0 - (double)pS[k + 1]) * Qn;
+        if (!prs[k]) acc[pf[k]] += -(double)beta * (double)pw[k] * Q;
+        Qn = Q;
+      }
+    }
+#pragma omp critical
+    for (int f = 0; f < F; ++f) gbare[f] += acc[f];
+    free(acc);
+    free(pf);
+    free(pw);
+    free(pS);
+    free(prs);
+  }
+}
+
 /* _trace3d_ion_yield (threed.py:599-712); out = 5 x F doubles (sp, ie, ip, arr, ang). */
 void oracle_trace_ion_yield(void* h, const oc_box* b, const oc_source* s, const oc_ion* P, int n,
                             uint64_t seed, double* out) {

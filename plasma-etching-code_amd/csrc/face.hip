@@ -3,6 +3,7 @@
 //   petch_cos_ratio       cos_i = ang/fl where fl > 0 else 0            threed.py:179-185,821
 //   petch_smooth          edge-neighbour normal-weighted smoothing       threed.py:129-148,195-214
 //   petch_belen_coverage  Belen coverages -> bare = clip(1-thF-thO,0,1)  threed.py:715-737,903-904
+//   petch_belen_coverage_adjoint  its closed-form derivative (no reference counterpart)
 //   petch_surface_rate    V per face (belen / langmuir), mask faces 0,  chemistry.py:62-92,
 //                         non-finite -> 0 (counted), running max of V   belen.py:26-63,
 //                                                                       threed.py:1608-1616,1721-1728,1750
@@ -107,6 +108,58 @@ __global__ void k_belen_coverage(const float* __restrict__ m_i, const float* __r
   bare[f] = (float)fmin(fmax(br, 0.0), 1.0);
   if (thF_out) thF_out[f] = (float)thF;
   if (thO_out) thO_out[f] = (float)thO;
+}
+
+// Closed-form adjoint of k_belen_coverage for gbare = d loss / d bare (the clip passes the
+// gradient where 0 <= 1 - thF - thO <= 1). With a = (k_sigma + 2 GY_ie) / GbE and
+// b = (beta_sigma + GY_p) / GbP:
+//   dthF/da = -(1 + 1/(b+eps)) thF^2   dthF/db = a thF^2 / (b+eps)^2
+//   dthO/db = -(1 + 1/(a+eps)) thO^2   dthO/da = b thO^2 / (a+eps)^2
+// and da/dm_F = -a Fflux / GbE, db/dm_O = -b Oflux / GbP, da/dGY_ie = 2 / GbE, db/dGY_p = 1 / GbP.
+// Outputs d loss / d m_F and d m_O; with ion yields also d / d iy_ie and d iy_ip (g_ie, g_ip), else
+// d / d m_i through GY = Y f_ie ionFlux m_i (gm_i); cos_i is held constant. Optional outputs may
+// be NULL.
+__global__ void k_belen_coverage_adjoint(
+    const float* __restrict__ m_i, const float* __restrict__ m_F, const float* __restrict__ m_O,
+    const float* __restrict__ cos_i, const float* __restrict__ iy_ie,
+    const float* __restrict__ iy_ip, Chem c, int F, const float* __restrict__ gbare,
+    float* __restrict__ gm_F, float* __restrict__ gm_O, float* __restrict__ gm_i,
+    float* __restrict__ g_ie, float* __restrict__ g_ip) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  const double eps = 1e-9;
+  double GY_ie, GY_p, f_ie = 0.0;
+  if (iy_ie) {
+    GY_ie = c.ionFlux * (double)iy_ie[f];
+    GY_p = c.ionFlux * (double)iy_ip[f];
+  } else {
+    double f_sp;
+    angular((double)cos_i[f], c, f_ie, f_sp);
+    double Fi = c.ionFlux * (double)m_i[f];
+    GY_ie = c.Yie * f_ie * Fi;
+    GY_p = c.Yp * f_ie * Fi;
+  }
+  double GbE = c.Fflux * (double)m_F[f] + eps;
+  double GbP = c.Oflux * (double)m_O[f] + eps;
+  double a = (c.k_sigma + 2.0 * GY_ie) / GbE;
+  double b = (c.beta_sigma + GY_p) / GbP;
+  double ib = 1.0 / (b + eps), ia = 1.0 / (a + eps);
+  double thF = 1.0 / (1.0 + a * (1.0 + ib));
+  double thO = 1.0 / (1.0 + b * (1.0 + ia));
+  double br = 1.0 - thF - thO;
+  double g = (br >= 0.0 && br <= 1.0) ? (double)gbare[f] : 0.0;
+  // bare = 1 - thF - thO
+  double ga = -g * (-(1.0 + ib) * thF * thF + b * thO * thO * ia * ia);
+  double gb = -g * (a * thF * thF * ib * ib - (1.0 + ia) * thO * thO);
+  gm_F[f] = (float)(ga * (-a * c.Fflux / GbE));
+  gm_O[f] = (float)(gb * (-b * c.Oflux / GbP));
+  double gGie = ga * 2.0 / GbE, gGp = gb / GbP;
+  if (iy_ie) {
+    if (g_ie) g_ie[f] = (float)(gGie * c.ionFlux);
+    if (g_ip) g_ip[f] = (float)(gGp * c.ionFlux);
+  } else if (gm_i) {
+    gm_i[f] = (float)((gGie * c.Yie + gGp * c.Yp) * f_ie * c.ionFlux);
+  }
 }
 
 struct MaskGeom {
@@ -338,6 +391,24 @@ extern "C" int petch_belen_coverage(petch_ctx* ctx, const float* m_i, const floa
   if (F <= 0) return PETCH_OK;
   hipLaunchKernelGGL(k_belen_coverage, dim3(petch_blocks(F, kBS)), dim3(kBS), 0, ctx->stream, m_i,
                      m_F, m_O, cos_i, iy_ie, iy_ip, make_chem(cp), F, bare, thF, thO);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_belen_coverage_adjoint(petch_ctx* ctx, const float* m_i, const float* m_F,
+                                            const float* m_O, const float* cos_i,
+                                            const float* iy_ie, const float* iy_ip,
+                                            const petch_chem_params* cp, int F,
+                                            const float* gbare, float* gm_F, float* gm_O,
+                                            float* gm_i, float* g_ie, float* g_ip) {
+  PETCH_ARG(ctx, ctx && m_F && m_O && cp && gbare && gm_F && gm_O,
+            "belen_coverage_adjoint: null pointer");
+  PETCH_ARG(ctx, (iy_ie && iy_ip) || (m_i && cos_i),
+            "belen_coverage_adjoint: need ion yields or m_i/cos_i");
+  if (F <= 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_belen_coverage_adjoint, dim3(petch_blocks(F, kBS)), dim3(kBS), 0,
+                     ctx->stream, m_i, m_F, m_O, cos_i, iy_ie, iy_ip, make_chem(cp), F, gbare,
+                     gm_F, gm_O, gm_i, g_ie, g_ip);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

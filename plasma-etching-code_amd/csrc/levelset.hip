@@ -139,10 +139,37 @@ __device__ __forceinline__ float adj_weight(const AdjStencil& a, const float* __
   return -dt * F[n] * gout[n] / (2.f * sqrtf(a.G));
 }
 
-__global__ void k_advect_adjoint(const float* __restrict__ phi, const float* __restrict__ F,
-                                 const uint8_t* __restrict__ mask, const float* __restrict__ gout,
-                                 float* __restrict__ gphi, float* __restrict__ gF, int nx, int ny,
-                                 int nz, float inv_dx, float dt) {
+// Pass 1 of the adjoint, per node m: w[m] = dL/dG at m (adj_weight of m's own stencil), gF[m]
+// += -dt sqrt(G_m) gout[m] off the mask, and gphi[m] = the local terms (the pass-through of
+// gout off the mask plus m's own stencil differentiated in phi_m).
+__global__ void k_advect_adjoint_local(const float* __restrict__ phi,
+                                       const float* __restrict__ F,
+                                       const uint8_t* __restrict__ mask,
+                                       const float* __restrict__ gout, float* __restrict__ w,
+                                       float* __restrict__ gphi, float* __restrict__ gF, int nx,
+                                       int ny, int nz, float inv_dx, float dt) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long N = (long long)nx * ny * nz;
+  if (m >= N) return;
+  const int k = (int)(m % nz);
+  const int j = (int)((m / nz) % ny);
+  const int i = (int)(m / ((long long)nz * ny));
+  const AdjStencil self = adj_stencil(phi, m, i, j, k, nx, ny, nz, inv_dx);
+  const float wm = adj_weight(self, F, mask, gout, m, dt);
+  const bool msk = mask[m] != 0;
+  float g = msk ? 0.f : gout[m];
+  for (int ax = 0; ax < 3; ++ax) g += wm * 2.f * (self.mp[ax] - self.mn[ax]) * inv_dx;
+  w[m] = wm;
+  gphi[m] = g;
+  if (!msk) gF[m] += -dt * sqrtf(self.G) * gout[m];
+}
+
+// Pass 2, per node m: add what m's six neighbours' stencils owe phi_m. Neighbour n = m + e_ax
+// uses phi_m in its D- = (phi_n - phi_m)/dx, neighbour n = m - e_ax in its D+ =
+// (phi_m - phi_n)/dx, so each term needs only w[n] and the two values, not n's whole stencil.
+__global__ void k_advect_adjoint_gather(const float* __restrict__ phi,
+                                        const float* __restrict__ w, float* __restrict__ gphi,
+                                        int nx, int ny, int nz, float inv_dx) {
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long N = (long long)nx * ny * nz;
   if (m >= N) return;
@@ -151,26 +178,19 @@ __global__ void k_advect_adjoint(const float* __restrict__ phi, const float* __r
   const int i = (int)(m / ((long long)nz * ny));
   const long long st[3] = {(long long)ny * nz, (long long)nz, 1};
   const int idx[3] = {i, j, k}, lim[3] = {nx, ny, nz};
-  const AdjStencil self = adj_stencil(phi, m, i, j, k, nx, ny, nz, inv_dx);
-  const float wm = adj_weight(self, F, mask, gout, m, dt);
-  float g = mask[m] ? 0.f : gout[m];
-  for (int ax = 0; ax < 3; ++ax) g += wm * 2.f * (self.mp[ax] - self.mn[ax]) * inv_dx;
+  const float c = phi[m];
+  float g = gphi[m];
   for (int ax = 0; ax < 3; ++ax) {
-    if (idx[ax] + 1 < lim[ax]) {  // n = m + e_ax: its D- is (phi_n - phi_m)/dx
+    if (idx[ax] + 1 < lim[ax]) {
       const long long n = m + st[ax];
-      const AdjStencil a = adj_stencil(phi, n, i + (ax == 0), j + (ax == 1), k + (ax == 2), nx,
-                                       ny, nz, inv_dx);
-      g -= adj_weight(a, F, mask, gout, n, dt) * 2.f * a.mp[ax] * inv_dx;
+      g -= w[n] * 2.f * fmaxf((phi[n] - c) * inv_dx, 0.f) * inv_dx;
     }
-    if (idx[ax] >= 1) {           // n = m - e_ax: its D+ is (phi_m - phi_n)/dx
+    if (idx[ax] >= 1) {
       const long long n = m - st[ax];
-      const AdjStencil a = adj_stencil(phi, n, i - (ax == 0), j - (ax == 1), k - (ax == 2), nx,
-                                       ny, nz, inv_dx);
-      g += adj_weight(a, F, mask, gout, n, dt) * 2.f * a.mn[ax] * inv_dx;
+      g += w[n] * 2.f * fminf((c - phi[n]) * inv_dx, 0.f) * inv_dx;
     }
   }
   gphi[m] = g;
-  if (!mask[m]) gF[m] += -dt * sqrtf(self.G) * gout[m];
 }
 
 // ------------------------------------------------------------------------------ FSM reinit
@@ -298,6 +318,19 @@ extern "C" int petch_advect(petch_ctx* ctx, float* phi, float* work, const float
   return PETCH_OK;
 }
 
+extern "C" int petch_advect_step(petch_ctx* ctx, const float* phi, const float* F,
+                                 const uint8_t* mask, const float* mask_phi, int nx, int ny,
+                                 int nz, double dx, double dt, float* out) {
+  PETCH_ARG(ctx, ctx && phi && F && mask && mask_phi && out, "advect_step: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "advect_step: bad shape %d,%d,%d", nx, ny, nz);
+  PETCH_ARG(ctx, out != phi, "advect_step: out must not alias phi");
+  const long long N = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(k_advect, dim3(petch_blocks(N, kBS)), dim3(kBS), 0, ctx->stream, phi, F,
+                     mask, mask_phi, out, nx, ny, nz, (float)(1.0 / dx), (float)dt);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
 extern "C" int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
                                 double band, int n_iter) {
   PETCH_ARG(ctx, ctx && phi, "reinit_fsm: null pointer");
@@ -328,10 +361,17 @@ extern "C" int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const floa
                                     double dx, double dt, float* gphi, float* gF) {
   PETCH_ARG(ctx, ctx && phi && F && mask && gout && gphi && gF, "advect_adjoint: null pointer");
   PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "advect_adjoint: bad shape %d,%d,%d", nx, ny, nz);
-  PETCH_ARG(ctx, gphi != gout && gphi != phi, "advect_adjoint: gphi must not alias gout or phi");
+  PETCH_ARG(ctx, gphi != gout && gphi != phi && gphi != gF,
+            "advect_adjoint: gphi must not alias gout, phi or gF");
   const long long N = (long long)nx * ny * nz;
-  hipLaunchKernelGGL(k_advect_adjoint, dim3(petch_blocks(N, kBS)), dim3(kBS), 0, ctx->stream, phi,
-                     F, mask, gout, gphi, gF, nx, ny, nz, (float)(1.0 / dx), (float)dt);
+  float* w = (float*)petch_scratch(ctx, SCR_LS_A, N * sizeof(float));
+  if (!w) return PETCH_E_HIP;
+  const float inv = (float)(1.0 / dx);
+  dim3 g(petch_blocks(N, kBS)), b(kBS);
+  hipLaunchKernelGGL(k_advect_adjoint_local, g, b, 0, ctx->stream, phi, F, mask, gout, w, gphi, gF,
+                     nx, ny, nz, inv, (float)dt);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_advect_adjoint_gather, g, b, 0, ctx->stream, phi, w, gphi, nx, ny, nz, inv);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

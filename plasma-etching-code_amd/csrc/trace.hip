@@ -250,7 +250,9 @@ __device__ __forceinline__ bool claim_ray(const RayQueue& q, bool active, bool& 
 // the schedule.
 constexpr int kRefillAt = 24;
 constexpr int kLdsStack = 8;  // traversal stack entries per lane held in LDS (16 KB/workgroup)
-enum LaneState { kIdle = 0, kTrav = 1, kHit = 2 };
+// kMiss: the ray left the domain and the policy replays paths (P::kReplay), so it gets a say
+// (P::miss) before the lane goes idle.
+enum LaneState { kIdle = 0, kTrav = 1, kHit = 2, kMiss = 3 };
 
 template <class P>
 __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q,
@@ -278,6 +280,12 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
         emit = pol.hit(R, T.h, o, d, e);
         st = kIdle;
       }
+      if constexpr (P::kReplay) {
+        if (st == kMiss) {
+          emit = pol.miss(R, o, e);
+          st = kIdle;
+        }
+      }
       unsigned ray;
       if (claim_ray(q, emit || st == kTrav, exhausted, pool, pool_end, ray)) {
         pol.start(R, ray, o, e);
@@ -301,7 +309,7 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
         ++wraps;
         petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
       } else {
-        st = kIdle;
+        st = P::kReplay ? kMiss : kIdle;
       }
     }
   }
@@ -323,28 +331,49 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
 // One Russian-roulette bounce (threed.py:565-596) of a ray with weight w that hit face h:
 // deposits w, applies the sticking S, plays the roulette; returns false when the ray ends, else
 // moves o to the re-emission point and leaves the cosine lobe about the oriented normal in e.
-__device__ __forceinline__ bool rr_bounce(float& w, int& bounce, uint64_t seed, unsigned ray,
-                                          const BvhView& bv, const Hit& h, float* __restrict__ fl,
-                                          float S, Vec3& o, Vec3 d, Emit& e) {
-  Vec3 nn = hit_normal(bv, h.slot);
-  if (petch::dot(nn, d) > 0.f) nn = -nn;
-  deposit(fl, h.face, w);
-  float w2 = w - w * S;
+// The weight step of one bounce, shared by the trace and its adjoint so that both take the same
+// decisions: from weight w and sticking S at the ray's bounce `bounce`, w2 = w (1 - S), or 0.3
+// after a survived roulette (*reset); false when the ray ends (w2 <= 1e-6, roulette, or the
+// 1024-bounce cap), else ++bounce. u receives the bounce's Philox words.
+__device__ __forceinline__ bool rr_weight(float w, float S, uint64_t seed, unsigned ray,
+                                          int& bounce, float& w2, bool& reset, uint32_t u[4]) {
+  w2 = w - w * S;
+  reset = false;
   if (w2 <= 1.0e-6f) return false;
-  uint32_t u[4];
   petch::philox_block(seed, ray, (uint32_t)bounce + 1u, 1u, u);
   if (w2 < 0.1f) {
     if (petch::u24(u[0]) < (1.f - w2 / 0.3f)) return false;
     w2 = 0.3f;
+    reset = true;
   }
   if (++bounce >= 1024) return false;
-  w = w2;
+  return true;
+}
+
+// Re-emission of a continuing ray: o moves to 1e-4 off the hit point along the oriented normal
+// nn, and e holds the cosine lobe about nn from the bounce's words 1 and 2.
+__device__ __forceinline__ void rr_emit(const Hit& h, Vec3 nn, const uint32_t u[4], Vec3& o,
+                                        Vec3 d, Emit& e) {
   const Vec3 p = o + h.t * d;
   e.n = nn;
   e.r1 = petch::u24(u[1]);
   e.r2 = petch::u24(u[2]);
   e.mode = kLobe;
   o = p + 1.0e-4f * nn;
+}
+
+__device__ __forceinline__ bool rr_bounce(float& w, int& bounce, uint64_t seed, unsigned ray,
+                                          const BvhView& bv, const Hit& h, float* __restrict__ fl,
+                                          float S, Vec3& o, Vec3 d, Emit& e) {
+  Vec3 nn = hit_normal(bv, h.slot);
+  if (petch::dot(nn, d) > 0.f) nn = -nn;
+  deposit(fl, h.face, w);
+  float w2;
+  bool reset;
+  uint32_t u[4];
+  if (!rr_weight(w, S, seed, ray, bounce, w2, reset, u)) return false;
+  w = w2;
+  rr_emit(h, nn, u, o, d, e);
   return true;
 }
 
@@ -368,6 +397,7 @@ __device__ __forceinline__ void neutral_source(uint64_t seed, unsigned ray, cons
 }
 
 struct NeutralRR {
+  static constexpr bool kReplay = false;
   BvhView bv;
   Box b;
   Source src;
@@ -395,6 +425,98 @@ struct NeutralRR {
     const float S = bare[h.face] * (R.sp ? beta[1] : beta[0]);
     return rr_bounce(R.w, R.bounce, R.sp ? seed[1] : seed[0], R.ray, bv, h,
                      R.sp ? fl[1] : fl[0], S, o, d, e);
+  }
+};
+
+// Path-replay adjoint of the single-species trace (no reference counterpart): for gfl = d loss /
+// d fl it accumulates d loss / d bare into gbare, with every ray following the path the trace
+// drew (same Philox stream, same decisions; the roulette and termination decisions and the path
+// geometry are held fixed). Along a ray the deposits are w_k at hit k, w_{k+1} = w_k (1 - S_k)
+// with S_k = bare[f_k] beta, except that a survived roulette sets the next weight to 0.3. A
+// segment is a run of hits between such resets (or the ray's start or end); within one,
+//   d loss / d S_i = -(sum over later hits k of the segment of gfl[f_k] w_k) / (1 - S_i)
+//                  = -(T - P_i) / (1 - S_i),
+// with T the segment's total of gfl[f_k] w_k and P_i its running sum up to hit i. A ray
+// therefore traces each segment twice: phase 0 accumulates T, then it restarts from the
+// segment's first emission (o0, e0, w0, b0 kept in registers) and phase 1 replays the segment
+// hit for hit, adding -beta (T - P_i) / (1 - S_i) to gbare[f_i]. One-hit segments owe nothing
+// and are not replayed. T and P are float64, so T - P_i keeps its digits when S_i is close to 1.
+struct NeutralRRAdjoint {
+  static constexpr bool kReplay = true;
+  BvhView bv;
+  Box b;
+  Source src;
+  const float* __restrict__ bare;
+  const float* __restrict__ gfl;
+  float* gbare;
+  float beta;
+  uint64_t seed;
+  struct Ray {
+    float w, w0;
+    int bounce, b0, bend;  // hit index now, at the segment's start, of the segment's last hit
+    int phase;
+    unsigned ray;
+    double T, P;
+    Vec3 o0;
+    Emit e0;
+  };
+  __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
+  __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
+  __device__ __forceinline__ static void begin_segment(Ray& R, const Vec3& o, const Emit& e) {
+    R.o0 = o;
+    R.e0 = e;
+    R.w0 = R.w;
+    R.b0 = R.bounce;
+    R.phase = 0;
+    R.T = 0.0;
+  }
+  __device__ __forceinline__ static bool replay(Ray& R, int bend, Vec3& o, Emit& e) {
+    R.bend = bend;
+    R.phase = 1;
+    R.P = 0.0;
+    R.w = R.w0;
+    R.bounce = R.b0;
+    o = R.o0;
+    e = R.e0;
+    return true;
+  }
+  __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
+    R.ray = ray;
+    neutral_source(seed, ray, b, src, o, e);
+    R.w = 1.f;
+    R.bounce = 0;
+    begin_segment(R, o, e);
+  }
+  __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
+    Vec3 nn = hit_normal(bv, h.slot);
+    if (petch::dot(nn, d) > 0.f) nn = -nn;
+    const int i = R.bounce;
+    const float S = bare[h.face] * beta;
+    const double gw = (double)gfl[h.face] * (double)R.w;
+    float w2;
+    bool reset;
+    uint32_t u[4];
+    const bool cont = rr_weight(R.w, S, seed, R.ray, R.bounce, w2, reset, u);
+    if (R.phase == 0) {
+      R.T += gw;
+      if ((!cont || reset) && i > R.b0) return replay(R, i, o, e);
+    } else {
+      R.P += gw;
+      const double q = 1.0 - (double)S;
+      if (i != R.bend && q > 0.0)
+        deposit(gbare, h.face, (float)(-(double)beta * (R.T - R.P) / q));
+    }
+    if (!cont) return false;
+    R.w = w2;
+    rr_emit(h, nn, u, o, d, e);
+    if (reset) begin_segment(R, o, e);
+    return true;
+  }
+  // The ray left the domain after hit R.bounce - 1: a phase-0 segment of two or more hits is
+  // replayed; a replayed segment (phase 1) ends the ray.
+  __device__ __forceinline__ bool miss(Ray& R, Vec3& o, Emit& e) const {
+    if (R.phase == 0 && R.bounce - 1 > R.b0) return replay(R, R.bounce - 1, o, e);
+    return false;
   }
 };
 
@@ -487,6 +609,7 @@ __device__ __forceinline__ bool ion_bounce(Rng& r, float& Ein, float& win, int& 
 }
 
 struct IonYield {
+  static constexpr bool kReplay = false;
   BvhView bv;
   Box b;
   Source src;
@@ -557,6 +680,7 @@ __device__ __forceinline__ int locate_feature(const Fd* __restrict__ feat, int n
 }
 
 struct NeutralRRBatch {
+  static constexpr bool kReplay = false;
   const NeutralFeature* __restrict__ feat;
   int nfeat;
   Source src;
@@ -592,6 +716,7 @@ struct NeutralRRBatch {
 };
 
 struct IonYieldBatch {
+  static constexpr bool kReplay = false;
   const IonFeature* __restrict__ feat;
   int nfeat;
   struct Ray {
@@ -645,6 +770,11 @@ __global__ void k_zero_ion_batch(const IonFeature* __restrict__ feat) {
 // loop needs more and runs best at the compiler's own choice (4 waves per SIMD), measured.
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5)))
 k_trace_neutral_rr(NeutralRR pol, RayQueue q, unsigned long long* __restrict__ hits) {
+  persistent_trace(pol, q, hits);
+}
+
+__global__ void __launch_bounds__(kBS)
+k_trace_neutral_rr_adjoint(NeutralRRAdjoint pol, RayQueue q, unsigned long long* __restrict__ hits) {
   persistent_trace(pol, q, hits);
 }
 
@@ -824,6 +954,30 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   NeutralRR pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, {beta, beta},
                 {fl, fl}, {seed, seed}, (unsigned)n};
   return launch_neutral(ctx, pol, n);
+}
+
+extern "C" int petch_trace_neutral_rr_adjoint(petch_ctx* ctx, const petch_bvh* bvh,
+                                              const petch_box* box, const petch_source* src,
+                                              const float* bare, float beta, int n, uint64_t seed,
+                                              const float* gfl, float* gbare) {
+  PETCH_ARG(ctx, ctx && bvh && box && src && bare && gfl && gbare,
+            "trace_neutral_rr_adjoint: null pointer");
+  PETCH_ARG(ctx, n >= 0, "trace_neutral_rr_adjoint: n < 0");
+  PETCH_ARG(ctx, gbare != bare && gbare != gfl,
+            "trace_neutral_rr_adjoint: gbare must not alias bare or gfl");
+  int rc = zero(ctx, gbare, bvh->nf);
+  if (rc) return rc;
+  if (n == 0 || bvh->nf == 0) return PETCH_OK;
+  NeutralRRAdjoint pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, gfl, gbare,
+                       beta, seed};
+  RayQueue q;
+  rc = make_queue(ctx, n, q);
+  if (rc) return rc;
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr_adjoint, n);
+  hipLaunchKernelGGL(k_trace_neutral_rr_adjoint, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
+                     (unsigned long long*)nullptr);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
 }
 
 extern "C" int petch_trace_neutral_rr_pair(petch_ctx* ctx, const petch_bvh* bvh,

@@ -134,6 +134,8 @@ SIGNATURES = {
     "petch_gen_source": [P, POINTER(Box), POINTER(Source), c_int, c_uint64, P, P],
     "petch_trace_neutral_rr": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int, c_uint64,
                                P],
+    "petch_trace_neutral_rr_adjoint": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int,
+                                       c_uint64, P, P],
     "petch_trace_neutral_rr_pair": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int,
                                     c_uint64, P, c_float, c_int, c_uint64, P],
     "petch_trace_neutral_rr_batch": [P, POINTER(Source), c_int, P],
@@ -148,11 +150,14 @@ SIGNATURES = {
     "petch_cos_ratio": [P, P, P, c_int, P],
     "petch_smooth": [P, P, P, P, c_int, c_int, c_float, c_int, P],
     "petch_belen_coverage": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P],
+    "petch_belen_coverage_adjoint": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P,
+                                     P, P, P],
     "petch_surface_rate": [P, P, P, P, P, P, P, P, P, P, POINTER(MaskGeom),
                            POINTER(ChemParams), c_int, P, P],
     "petch_extend": [P, P, P, P, c_int, c_int, c_int, c_double, c_double, P],
     "petch_nearest_face_value": [P, P, P, P, c_int, c_float, P],
     "petch_extend_batch": [P, c_int, P],
+    "petch_advect_step": [P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, P],
     "petch_advect_adjoint": [P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, P, P],
     "petch_marching_cubes_batch": [P, c_int, P],
     "petch_nearest_face_value_batch": [P, c_int, P],

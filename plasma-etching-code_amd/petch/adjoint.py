@@ -15,6 +15,19 @@ The radiosity solve of mc_flux_3d_radiosity (threed.py:965-985) carries its impl
 m = clip(G, 0, 8) of G = D + Aff diag(1 - clip(bare beta, 0, 1)) G and back-propagates through
 the transposed solve (petch_radiosity_adjoint) to bare and to the escape fractions D. The form
 factors (hitface, the CSR) are constants: they come from discrete first hits.
+
+The Belen coverage update bare = clip(1 - thF - thO, 0, 1) (threed.py:715-737) carries its
+closed-form derivative (petch_belen_coverage_adjoint): `belen(m_i, m_F, m_O, cos_i, ion_yield,
+chem)`. `radiosity_coverage` chains both into the coverage fixed point of mc_flux_3d_radiosity
+(threed.py:963-990): n_fp rounds of the F and O solves and the Belen update from bare = 1,
+differentiable in the ion terms (m_i, or the ion yields) and the escape fractions D, the
+adjoint unrolled over the n_fp rounds.
+
+The Russian-roulette neutral trace (_trace3d_cov_rr, threed.py:549-596) carries a path-replay
+adjoint (petch_trace_neutral_rr_adjoint): `neutral_rr(bvh, box, src, bare, beta, n, seed)`
+returns the raw per-face accumulator and back-propagates to bare with every ray's path and its
+roulette decisions held fixed (common random numbers: the backward pass redraws the same Philox
+streams, so nothing is stored per ray).
 """
 import torch
 
@@ -25,15 +38,12 @@ class _Advect(torch.autograd.Function):
     @staticmethod
     def forward(ctx, phi, F, mask, mask_phi, dx, dt, nsub):
         states = [phi.detach().contiguous()]
-        cur = states[0]
+        Fd = F.detach().contiguous()
         for _ in range(int(nsub)):
-            nxt = cur.clone()
-            ops.advect(nxt, F.detach(), mask, mask_phi, dx, dt, 1)
-            states.append(nxt)
-            cur = nxt
-        ctx.save_for_backward(*states[:-1], F.detach(), mask)
+            states.append(ops.advect_step(states[-1], Fd, mask, mask_phi, dx, dt))
+        ctx.save_for_backward(*states[:-1], Fd, mask)
         ctx.dx, ctx.dt = float(dx), float(dt)
-        return cur.clone()
+        return states[-1] if nsub > 0 else states[0].clone()
 
     @staticmethod
     def backward(ctx, gout):
@@ -75,3 +85,69 @@ def radiosity(bare, D, hitface, row_ptr, col, n_ff, beta, n_iter=0, tol=1e-10):
     ops.radiosity_assemble built from hitface), differentiable in bare and D. n_iter <= 0 solves
     to max|dG| <= tol max|G|, where the implicit adjoint is exact."""
     return _Radiosity.apply(bare, D, hitface, row_ptr, col, n_ff, beta, n_iter, tol)
+
+
+class _Belen(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, m_i, m_F, m_O, cos_i, iy_ie, iy_ip, chem):
+        iy = (None, iy_ie, iy_ip) if iy_ie is not None else None
+        bare = ops.belen_coverage(m_i, m_F.contiguous(), m_O.contiguous(), cos_i, iy, chem)
+        ctx.save_for_backward(m_i, m_F, m_O, cos_i, iy_ie, iy_ip)
+        ctx.chem = chem
+        return bare
+
+    @staticmethod
+    def backward(ctx, gbare):
+        m_i, m_F, m_O, cos_i, iy_ie, iy_ip = ctx.saved_tensors
+        iy = (None, iy_ie, iy_ip) if iy_ie is not None else None
+        gm_F, gm_O, gm_i, g_ie, g_ip = ops.belen_coverage_adjoint(
+            m_i, m_F.contiguous(), m_O.contiguous(), cos_i, iy, ctx.chem,
+            gbare.contiguous().float())
+        return gm_i, gm_F, gm_O, None, g_ie, g_ip, None
+
+
+def belen(m_i, m_F, m_O, cos_i, ion_yield, chem):
+    """bare = clip(1 - thF - thO, 0, 1) (ops.belen_coverage), differentiable in m_F, m_O and
+    the ion terms: m_i without ion yields, else the yields ion_yield[1], ion_yield[2]."""
+    ie, ip = (ion_yield[1], ion_yield[2]) if ion_yield is not None else (None, None)
+    return _Belen.apply(m_i, m_F, m_O, cos_i, ie, ip, chem)
+
+
+def radiosity_coverage(m_i, cos_i, ion_yield, D, hitface, row_ptr, col, n_ff, chem, betaE,
+                       betaO, n_fp=4, n_iter=0, tol=1e-10):
+    """(bare, m_F, m_O) of the coverage fixed point (the loop of flux.radiosity_flux) on the
+    form factors of hitface / (row_ptr, col, D), differentiable in m_i or the ion yields and in
+    D. n_iter <= 0 solves each radiosity system to tol (where its implicit adjoint is exact)."""
+    F = int(cos_i.shape[0])
+    bare = torch.ones((F,), dtype=torch.float32, device=cos_i.device)
+    m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=cos_i.device)
+    Dv = D[:F]
+    for _ in range(n_fp):
+        m_F = radiosity(bare, Dv, hitface, row_ptr, col, n_ff, betaE, n_iter, tol)
+        m_O = radiosity(bare, Dv, hitface, row_ptr, col, n_ff, betaO, n_iter, tol)
+        bare = belen(m_i, m_F, m_O, cos_i, ion_yield, chem)
+    return bare, m_F, m_O
+
+
+class _NeutralRR(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, bare, bvh, box, src, beta, n, seed):
+        b = bare.detach().contiguous()
+        fl = ops.trace_neutral_rr(bvh, box, src, b, beta, n, seed)
+        ctx.save_for_backward(b)
+        ctx.args = (bvh, box, src, float(beta), int(n), int(seed))
+        return fl
+
+    @staticmethod
+    def backward(ctx, gfl):
+        (b,) = ctx.saved_tensors
+        bvh, box, src, beta, n, seed = ctx.args
+        gbare = ops.trace_neutral_rr_adjoint(bvh, box, src, b, beta, n, seed,
+                                             gfl.contiguous().float())
+        return gbare, None, None, None, None, None, None
+
+
+def neutral_rr(bvh, box, src, bare, beta, n, seed):
+    """Raw RR neutral accumulator (ops.trace_neutral_rr), differentiable in bare. The BVH must
+    be unchanged between the forward and the backward pass."""
+    return _NeutralRR.apply(bare, bvh, box, src, beta, n, seed)

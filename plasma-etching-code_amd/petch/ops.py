@@ -45,6 +45,20 @@ def advect(phi, F, mask, mask_phi, dx, dt, nsub, work=None):
     return phi
 
 
+def advect_step(phi, F, mask, mask_phi, dx, dt, out=None):
+    """One substep of advect out of place (petch_advect_step): returns out."""
+    nx, ny, nz = _shape3(phi)
+    for t, n in ((F, "F"), (mask_phi, "mask_phi"), (mask, "mask")):
+        if tuple(t.shape) != (nx, ny, nz):
+            raise ValueError(f"advect_step: {n} shape {tuple(t.shape)} != phi shape {(nx, ny, nz)}")
+    if out is None:
+        out = torch.empty_like(phi)
+    _ctx(phi).call("petch_advect_step", ptr(phi, f32, "phi"), ptr(F, f32, "F"),
+                   ptr(mask, u8, "mask"), ptr(mask_phi, f32, "mask_phi"), nx, ny, nz, float(dx),
+                   float(dt), ptr(out, f32, "out"))
+    return out
+
+
 def advect_adjoint(phi, F, mask, gout, dx, dt, gF):
     """Adjoint of one advection substep at the state phi: returns dL/dphi for
     gout = dL/d(output) and adds dL/dF to gF (petch_advect_adjoint)."""
@@ -276,6 +290,21 @@ def trace_neutral_rr(bvh, box, src, bare, beta, n, seed, out=None):
     return fl
 
 
+def trace_neutral_rr_adjoint(bvh, box, src, bare, beta, n, seed, gfl):
+    """d loss / d bare (F,) of trace_neutral_rr(bvh, box, src, bare, beta, n, seed) for
+    gfl = d loss / d fl, paths and roulette decisions held fixed (petch_trace_neutral_rr_adjoint)."""
+    import ctypes
+    for t, name in ((bare, "bare"), (gfl, "gfl")):
+        if t.numel() < bvh.nf:
+            raise ValueError(f"trace_neutral_rr_adjoint: {name} holds {t.numel()} values, "
+                             f"needs {bvh.nf}")
+    gbare = torch.empty((bvh.nf,), dtype=f32, device=bvh.device)
+    bvh.ctx.call("petch_trace_neutral_rr_adjoint", bvh.h, ctypes.byref(box), ctypes.byref(src),
+                 ptr(bare, f32, "bare"), float(beta), int(n), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                 ptr(gfl, f32, "gfl"), ptr(gbare, f32))
+    return gbare
+
+
 def trace_neutral_rr_pair(bvh, box, src, bare, beta_a, n_a, seed_a, beta_b, n_b, seed_b):
     """Two Russian-roulette neutral species over the same surface and coverage in one launch;
     each output equals trace_neutral_rr with the same (beta, n, seed)."""
@@ -410,6 +439,31 @@ def belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem):
         _ctx(m_F).call("petch_belen_coverage", ptr(m_i), ptr(m_F, f32), ptr(m_O, f32), ptr(cos_i),
                        ptr(ie), ptr(ip), ctypes_ref(chem), F, ptr(bare), ptr(None), ptr(None))
     return bare
+
+
+def belen_coverage_adjoint(m_i, m_F, m_O, cos_i, ion_yield, chem, gbare):
+    """Adjoint of belen_coverage (petch_belen_coverage_adjoint): (gm_F, gm_O, gm_i, g_ie, g_ip)
+    for gbare = d loss / d bare; gm_i is None with ion yields, g_ie/g_ip are None without."""
+    F = int(m_F.shape[0])
+    for t, name in ((m_O, "m_O"), (gbare, "gbare")) + \
+            (((ion_yield[1], "iy_ie"), (ion_yield[2], "iy_ip")) if ion_yield is not None
+             else ((m_i, "m_i"), (cos_i, "cos_i"))):
+        if t.numel() < F:
+            raise ValueError(f"belen_coverage_adjoint: {name} holds {t.numel()} values, needs {F}")
+    gm_F, gm_O = torch.empty_like(m_F), torch.empty_like(m_F)
+    gm_i = g_ie = g_ip = None
+    if ion_yield is not None:
+        ie, ip = ion_yield[1], ion_yield[2]
+        g_ie, g_ip = torch.empty_like(m_F), torch.empty_like(m_F)
+    else:
+        ie = ip = None
+        gm_i = torch.empty_like(m_F)
+    if F:
+        _ctx(m_F).call("petch_belen_coverage_adjoint", ptr(m_i), ptr(m_F, f32, "m_F"),
+                       ptr(m_O, f32, "m_O"), ptr(cos_i), ptr(ie), ptr(ip), ctypes_ref(chem), F,
+                       ptr(gbare, f32, "gbare"), ptr(gm_F), ptr(gm_O), ptr(gm_i), ptr(g_ie),
+                       ptr(g_ip))
+    return gm_F, gm_O, gm_i, g_ie, g_ip
 
 
 def surface_rate(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, is_mask=None,

@@ -8,10 +8,12 @@ import pytest
 from oracle import flux as ofl
 
 
-def _case(seed, nf=48, n_ff=16, p_esc=0.3):
+def _case(seed, nf=48, n_ff=16, p_esc=0.3, escape_each=False):
     rng = np.random.default_rng(seed)
     hit = rng.integers(0, nf, nf * n_ff).astype(np.int64)
     hit[rng.random(nf * n_ff) < p_esc] = -1
+    if escape_each:
+        hit[::n_ff] = -1               # D >= 1/n_ff, so G > 0 stays off the clip at 0
     D, Aff = ofl.radiosity_matrix(hit, nf, n_ff)
     return rng, hit, D, Aff, nf, n_ff
 
@@ -73,3 +75,71 @@ def test_radiosity_adjoint_clip_and_jacobi():
     d_jac = ofl.radiosity_adjoint(D, Aff, G, bare, 0.7, w, n_iter=400)
     for a, b in zip(d_direct, d_jac):
         np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
+
+
+def _chem_case(seed, F=40):
+    from petch.params import PAR, Flags
+    rng = np.random.default_rng(seed)
+    par = dict(PAR)
+    fl = Flags(chemistry="belen", ion_reflection=True)
+    m_i = rng.uniform(0.1, 1.2, F)
+    cos_i = rng.uniform(0.0, 1.0, F)
+    iy = tuple(rng.uniform(0.0, 0.5, F) for _ in range(3))
+    return rng, par, fl, m_i, cos_i, iy
+
+
+@pytest.mark.parametrize("yields", [False, True])
+def test_belen_adjoint_matches_central_differences(yields):
+    """The complex-step derivative of the Belen bare update against float64 central differences,
+    for the m_F, m_O and ion inputs (m_i, or the ion yields)."""
+    from oracle import chem as oc
+    rng, par, fl, m_i, cos_i, iy = _chem_case(3)
+    F = len(m_i)
+    m_F, m_O = rng.uniform(0.01, 1.0, F), rng.uniform(0.01, 1.0, F)
+    iy = iy if yields else None
+    g = rng.normal(size=F)
+    out = oc.belen_bare_adjoint(m_i, m_F, m_O, cos_i, par, fl, g, ion_yield=iy)
+
+    def loss(**kw):
+        a = dict(m_i=m_i, m_F=m_F, m_O=m_O)
+        y = list(iy) if iy is not None else None
+        for k, v in kw.items():
+            if k in a:
+                a[k] = v
+            else:
+                y[{"ie": 1, "ip": 2}[k]] = v
+        thF, thO = oc.belen_coverages(a["m_i"], a["m_F"], a["m_O"], cos_i, par, fl,
+                                      ion_yield=tuple(y) if y is not None else None)
+        return float(g @ np.clip(1.0 - thF - thO, 0.0, 1.0))
+
+    base = dict(m_i=m_i, m_F=m_F, m_O=m_O)
+    if iy is not None:
+        base.update(ie=iy[1], ip=iy[2])
+    for k, grad in out.items():
+        v = rng.normal(size=F)
+        h = 1e-7 * max(1.0, float(np.abs(base[k]).max()))
+        fd = (loss(**{k: base[k] + h * v}) - loss(**{k: base[k] - h * v})) / (2 * h)
+        assert abs(fd - grad @ v) <= 1e-5 * max(1.0, abs(fd)), (k, fd, grad @ v)
+    assert set(out) == ({"m_F", "m_O", "ie", "ip"} if yields else {"m_F", "m_O", "m_i"})
+
+
+def test_coverage_fixed_point_adjoint_matches_central_differences():
+    """The unrolled adjoint of the radiosity coverage loop (n_fp rounds of two solves and a Belen
+    update) against float64 central differences in m_i and D. Every face has an escaping ray:
+    a face with D = 0 and q = 0 upstream has G = 0 on the clip's kink, where central differences
+    average the two one-sided slopes."""
+    rng, hit, D, Aff, nf, n_ff = _case(2, nf=40, escape_each=True)
+    _, par, fl, m_i, cos_i, _ = _chem_case(4, F=nf)
+    w = rng.normal(size=nf)
+    _, _, _, grads = ofl.radiosity_coverage(D, Aff, m_i, cos_i, par, fl, n_fp=3, w=w)
+
+    def loss(mi, DD):
+        return float(w @ ofl.radiosity_coverage(DD, Aff, mi, cos_i, par, fl, n_fp=3)[0])
+
+    for _ in range(2):
+        v = rng.normal(size=nf)
+        h = 1e-6
+        fd = (loss(m_i + h * v, D) - loss(m_i - h * v, D)) / (2 * h)
+        assert abs(fd - grads["m_i"] @ v) <= 1e-5 * max(1.0, abs(fd)), (fd, grads["m_i"] @ v)
+        fd = (loss(m_i, D + h * v) - loss(m_i, D - h * v)) / (2 * h)
+        assert abs(fd - grads["D"] @ v) <= 1e-5 * max(1.0, abs(fd)), (fd, grads["D"] @ v)

@@ -113,3 +113,34 @@ def test_oracle_bvh_matches_brute_force():
     fo, to = on.OracleBvh(v, f).closest_hit(o, d)
     np.testing.assert_array_equal(fo, fb)
     np.testing.assert_allclose(to, tb)
+
+
+@pytest.mark.parametrize("beta", [0.7, 1.0])
+def test_rr_path_adjoint_matches_frozen_decision_differences(beta):
+    """The oracle's path adjoint of the RR neutral trace (stored paths, backward recursion over
+    each roulette segment) against central differences of the same trace with its paths and
+    roulette / termination decisions frozen (oracle_trace_neutral_rr_frozen, which equals the
+    trace where the two bare arrays agree). Roulette is frequent at these stickings, so the
+    reset of the weight to 0.3 is exercised."""
+    from oracle import levelset as ol
+    phi = ol.init_sdf(6.0, 1.5, 12.0, 0.15, 1.5, 7.0, 3.0, False).astype(np.float32)
+    v, f = on.marching_cubes(phi, 0.15)
+    bvh = on.OracleBvh(v, f)
+    box = on.Box(6.0, 1.5, 12.0, 12.0 - 0.15, 1)
+    src = on.Source(0, 0, 0)
+    F = len(f)
+    rng = np.random.default_rng(int(beta * 10))
+    g = rng.normal(size=F)
+    bare = rng.uniform(0.2, 1.0, F).astype(np.float32)
+    n = 20000
+    plain = bvh.trace_neutral_rr(box, src, bare, beta, n, 7)
+    frozen = bvh.trace_neutral_rr_frozen(box, src, bare, bare, beta, n, 7)
+    np.testing.assert_allclose(frozen, plain, rtol=0, atol=1e-6 * np.abs(plain).max())
+    gb = bvh.trace_neutral_rr_adjoint(box, src, bare, beta, n, 7, g)
+    h = 1e-2
+    for _ in range(3):
+        u = rng.normal(size=F)
+        lp = g @ bvh.trace_neutral_rr_frozen(box, src, bare, bare + h * u, beta, n, 7)
+        lm = g @ bvh.trace_neutral_rr_frozen(box, src, bare, bare - h * u, beta, n, 7)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - gb @ u) <= 1e-3 * abs(fd) + 1e-3, (fd, gb @ u)
