@@ -302,6 +302,16 @@ int petch_cos_ratio(petch_ctx* ctx, const float* fl, const float* ang, int F, fl
  * of petch_face_adjacency, normals unit face normals. src != out. */
 int petch_smooth(petch_ctx* ctx, const float* src, const int* nbr, const float* normals, int F,
                  int n_iter, float alpha, int blend_each, float* out);
+/* Adjoint of petch_normalize at fl: gfl = gout * scale / max(area,1e-9) where
+ * lo <= (fl/area) scale <= hi, else 0. No reference counterpart. */
+int petch_normalize_adjoint(petch_ctx* ctx, const float* fl, const float* areas, int F,
+                            float scale, float lo, float hi, const float* gout, float* gfl);
+/* Adjoint of petch_smooth with the same (nbr, normals, n_iter, alpha, blend_each): the passes'
+ * transposes in reverse order, gsrc = d loss / d src for gout = d loss / d out (the smoothing is
+ * linear). Uses that edge adjacency is symmetric, as on a manifold mesh. gout != gsrc. No
+ * reference counterpart. */
+int petch_smooth_adjoint(petch_ctx* ctx, const int* nbr, const float* normals, int F, int n_iter,
+                         float alpha, int blend_each, const float* gout, float* gsrc);
 /* _belen_coverages + the bare update (threed.py:715-737, 903-904): bare = clip(1-thF-thO, 0, 1).
  * With iy_ie/iy_ip (the normalised faithful-ion yields, par['_ion_yield']) GY comes from them,
  * else from m_i, cos_i and the analytic yields. thF/thO may be NULL. */

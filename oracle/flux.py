@@ -98,6 +98,91 @@ def coupled_flux(M, geo, par, n_ion, n_neu, seed, flags, bare_init=None):
     return dict(m_i=m_i, m_F=m_F, m_O=m_O, cos_i=cos_i, bare=bare, ion_yield=iy)
 
 
+def _smooth_matrix(M, n_smooth, alpha, dev):
+    """The linear map of finish()'s smoothing (om.smooth_flux / smooth_flux_dev) as a scipy
+    matrix: one pass is P = D^-1 (I + W), W_ij = max(0, n_i.n_j) over edge pairs."""
+    import scipy.sparse as sp
+    F = M.nf
+    I = sp.identity(F, format='csr')
+    if n_smooth <= 0 or alpha <= 0.0 or len(M.pairs) == 0:
+        return I
+    i, j = M.pairs[:, 0], M.pairs[:, 1]
+    w = np.maximum(0.0, np.sum(M.normals[i] * M.normals[j], axis=1))
+    W = sp.coo_matrix((np.concatenate([w, w]), (np.concatenate([i, j]), np.concatenate([j, i]))),
+                      shape=(F, F)).tocsr()
+    P = sp.diags(1.0 / (1.0 + np.asarray(W.sum(axis=1)).ravel())) @ (I + W)
+    if dev:
+        B = (1.0 - alpha) * I + alpha * P if alpha < 1.0 else P
+        T = I
+        for _ in range(n_smooth):
+            T = B @ T
+        return T.tocsr()
+    T = I
+    for _ in range(n_smooth):
+        T = P @ T
+    return ((1.0 - alpha) * I + alpha * T).tocsr() if alpha < 1.0 else T.tocsr()
+
+
+def coupled_coverage(M, geo, par, flags, m_i, cos_i, n_neu, seed, n_fp, ion_yield=None, w=None,
+                     frozen=None):
+    """The neutral rounds of coupled_flux (threed.py:855-904) from given ion terms: n_fp x {RR
+    traces of F and O at bare, finish (clip-normalise, smooth), Belen update} from bare = 1.
+    Returns (bare, m_F, m_O, bares) with bares the bare each round traced at. With w, also the
+    adjoint of w . bare_final (not in the reference): grads dict(m_i or ie/ip), unrolled over the
+    rounds, each trace by its path adjoint (paths and decisions fixed), finish by its transpose
+    (a scipy matrix of the smoothing, transposed), Belen by belen_bare_adjoint. With frozen (a
+    list of per-round bare), each round's traces keep the paths and decisions of frozen[it] and
+    take their weights from the current bare (oracle_trace_neutral_rr_frozen): central
+    differences of that are what the adjoint computes."""
+    A_src = geo['Lx'] * geo['Ly']
+    A = np.maximum(M.area, 1.0e-9)
+    betaE, betaO = par.get('betaE', 0.7), par.get('betaO', 1.0)
+    n_smooth = int(par.get('flux_smooth', 1))
+    alpha = float(par.get('flux_smooth_alpha', 1.0))
+    dev = bool(par.get('device_flux', True)) and bool(par.get('flux_smooth_gpu', True))
+    box = _box(geo, par)
+    nsrc = on.Source(0, 0.0, 0.0)
+    T = _smooth_matrix(M, n_smooth, alpha, dev)
+    scale = A_src / n_neu
+
+    def trace(beta, bare, sd, it):
+        if frozen is None:
+            return M.bvh.trace_neutral_rr(box, nsrc, bare.astype(np.float32), beta, n_neu, sd)
+        return M.bvh.trace_neutral_rr_frozen(box, nsrc, frozen[it].astype(np.float32),
+                                             bare.astype(np.float32), beta, n_neu, sd)
+
+    bare = np.ones(M.nf)
+    bares, tape = [], []
+    m_F = m_O = np.zeros(M.nf)
+    for it in range(n_fp):
+        bares.append(bare)
+        sF, sO = seed * 9 + 2 + 2 * it, seed * 9 + 3 + 2 * it
+        fF, fO = trace(betaE, bare, sF, it), trace(betaO, bare, sO, it)
+        m_F = T.dot(np.clip((fF / A) * scale, 0.0, 8.0))
+        m_O = T.dot(np.clip((fO / A) * scale, 0.0, 8.0))
+        tape.append((bare, fF, fO, m_F, m_O, sF, sO))
+        thF, thO = oc.belen_coverages(m_i, m_F, m_O, cos_i, par, flags, ion_yield=ion_yield)
+        bare = np.clip(1.0 - thF - thO, 0.0, 1.0)
+    if w is None:
+        return bare, m_F, m_O, bares
+    keys = ("ie", "ip") if ion_yield is not None else ("m_i",)
+    grads = {k: np.zeros(M.nf) for k in keys}
+    gbare = np.asarray(w, np.float64)
+    TT = T.T.tocsr()
+    for b_in, fF, fO, mF, mO, sF, sO in reversed(tape):
+        g = oc.belen_bare_adjoint(m_i, mF, mO, cos_i, par, flags, gbare, ion_yield=ion_yield)
+        for k in keys:
+            grads[k] += g[k]
+        gb = np.zeros(M.nf)
+        for fl, gm, beta, sd in ((fF, g["m_F"], betaE, sF), (fO, g["m_O"], betaO, sO)):
+            v = (fl / A) * scale
+            gfl = np.where((v >= 0.0) & (v <= 8.0), TT.dot(gm) * scale / A, 0.0)
+            gb += M.bvh.trace_neutral_rr_adjoint(box, nsrc, b_in.astype(np.float32), beta, n_neu,
+                                                 sd, gfl)
+        gbare = gb
+    return bare, m_F, m_O, bares, grads
+
+
 def radiosity_matrix(hit, nf, n_ff):
     """threed.py:956-960: D (escape fraction) and Aff[i,j] = rays j->i / n_ff (scipy CSR)."""
     import scipy.sparse as sp

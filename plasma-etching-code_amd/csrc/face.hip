@@ -2,6 +2,7 @@
 //   petch_normalize       m = clamp((fl/max(A,1e-9))*scale, lo, hi)      threed.py:165-170,814-821
 //   petch_cos_ratio       cos_i = ang/fl where fl > 0 else 0            threed.py:179-185,821
 //   petch_smooth          edge-neighbour normal-weighted smoothing       threed.py:129-148,195-214
+//   petch_normalize_adjoint, petch_smooth_adjoint: their transposes (no reference counterpart)
 //   petch_belen_coverage  Belen coverages -> bare = clip(1-thF-thO,0,1)  threed.py:715-737,903-904
 //   petch_belen_coverage_adjoint  its closed-form derivative (no reference counterpart)
 //   petch_surface_rate    V per face (belen / langmuir), mask faces 0,  chemistry.py:62-92,
@@ -59,6 +60,58 @@ __global__ void k_smooth(const float* __restrict__ x, const int* __restrict__ nb
   }
   float s = num / fmaxf(den, 1.0e-12f);
   out[f] = (alpha < 1.f) ? (1.f - alpha) * raw[f] + alpha * s : s;
+}
+
+// Adjoint of k_normalize: gfl = gout * scale / A where the clamp is inactive (lo <= v <= hi).
+__global__ void k_normalize_adjoint(const float* __restrict__ fl, const float* __restrict__ area,
+                                    int F, float scale, float lo, float hi,
+                                    const float* __restrict__ gout, float* __restrict__ gfl) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  float A = fmaxf(area[f], 1.0e-9f);
+  float v = (fl[f] / A) * scale;
+  gfl[f] = (v >= lo && v <= hi) ? gout[f] * (scale / A) : 0.f;
+}
+
+// The smoothing pass is s = D^-1 (I + W) x with W_fg = max(0, n_f.n_g) over edge neighbours and
+// D_f = 1 + sum_g W_fg. Edge adjacency is symmetric and so is W, so the transposed pass is
+// (I + W) D^-1 y: z = y / D (k_smooth_den), then out_g = a (z_g + sum_f W_gf z_f) + c y_g
+// (k_smooth_transposed; c carries a per-pass blend with its own input).
+__global__ void k_smooth_den(const float* __restrict__ y, const int* __restrict__ nbr,
+                             const float* __restrict__ nrm, int F, float* __restrict__ z) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  float nx = nrm[3 * f], ny = nrm[3 * f + 1], nz = nrm[3 * f + 2];
+  float den = 1.f;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    int g = nbr[3 * f + e];
+    if (g < 0 || g >= F) continue;
+    den += fmaxf(0.f, nx * nrm[3 * g] + ny * nrm[3 * g + 1] + nz * nrm[3 * g + 2]);
+  }
+  z[f] = y[f] / fmaxf(den, 1.0e-12f);
+}
+
+__global__ void k_smooth_transposed(const float* __restrict__ z, const float* y,
+                                    const int* __restrict__ nbr, const float* __restrict__ nrm,
+                                    int F, float a, float c, float* out) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  float nx = nrm[3 * f], ny = nrm[3 * f + 1], nz = nrm[3 * f + 2];
+  float t = z[f];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    int g = nbr[3 * f + e];
+    if (g < 0 || g >= F) continue;
+    t += z[g] * fmaxf(0.f, nx * nrm[3 * g] + ny * nrm[3 * g + 1] + nz * nrm[3 * g + 2]);
+  }
+  float yf = y[f];  // y may alias out: each lane reads its own entry before writing it
+  out[f] = a * t + c * yf;
+}
+
+__global__ void k_axpy(float* __restrict__ y, const float* __restrict__ x, float a, int F) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f < F) y[f] += a * x[f];
 }
 
 struct Chem {
@@ -343,6 +396,18 @@ extern "C" int petch_normalize(petch_ctx* ctx, const float* fl, const float* are
   return PETCH_OK;
 }
 
+extern "C" int petch_normalize_adjoint(petch_ctx* ctx, const float* fl, const float* areas,
+                                       int F, float scale, float lo, float hi, const float* gout,
+                                       float* gfl) {
+  PETCH_ARG(ctx, ctx && fl && areas && gout && gfl, "normalize_adjoint: null pointer");
+  PETCH_ARG(ctx, F >= 0, "normalize_adjoint: F < 0");
+  if (F == 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_normalize_adjoint, dim3(petch_blocks(F, kBS)), dim3(kBS), 0, ctx->stream,
+                     fl, areas, F, scale, lo, hi, gout, gfl);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
 extern "C" int petch_cos_ratio(petch_ctx* ctx, const float* fl, const float* ang, int F,
                                float* out) {
   PETCH_ARG(ctx, ctx && fl && ang && out, "cos_ratio: null pointer");
@@ -378,6 +443,43 @@ extern "C" int petch_smooth(petch_ctx* ctx, const float* src, const int* nbr, co
     hipLaunchKernelGGL(k_smooth, g, b, 0, ctx->stream, cur, nbr, normals, F, a, raw, dst);
     PETCH_LAUNCH_CHECK(ctx);
     cur = dst;
+  }
+  return PETCH_OK;
+}
+
+extern "C" int petch_smooth_adjoint(petch_ctx* ctx, const int* nbr, const float* normals, int F,
+                                    int n_iter, float alpha, int blend_each, const float* gout,
+                                    float* gsrc) {
+  PETCH_ARG(ctx, ctx && nbr && normals && gout && gsrc, "smooth_adjoint: null pointer");
+  PETCH_ARG(ctx, F >= 0, "smooth_adjoint: F < 0");
+  PETCH_ARG(ctx, gout != gsrc, "smooth_adjoint: gout and gsrc must differ");
+  if (F == 0) return PETCH_OK;
+  if (n_iter <= 0 || alpha <= 0.f) {
+    PETCH_HIP(ctx, hipMemcpyAsync(gsrc, gout, (size_t)F * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    return PETCH_OK;
+  }
+  float* z = (float*)petch_scratch(ctx, SCR_SMOOTH, (size_t)F * 4);
+  if (!z) return PETCH_E_HIP;
+  dim3 g(petch_blocks(F, kBS)), b(kBS);
+  // the passes of petch_smooth in reverse; the first transposed pass reads gout, the others run
+  // in place on gsrc
+  const float* y = gout;
+  for (int it = n_iter - 1; it >= 0; --it) {
+    const bool last = it == n_iter - 1;
+    const float a = blend_each ? alpha : (last ? alpha : 1.f);
+    const bool blend = a < 1.f;
+    const float aa = blend ? a : 1.f;
+    const float c = (blend && blend_each) ? 1.f - a : 0.f;
+    hipLaunchKernelGGL(k_smooth_den, g, b, 0, ctx->stream, y, nbr, normals, F, z);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_smooth_transposed, g, b, 0, ctx->stream, z, y, nbr, normals, F, aa, c,
+                       gsrc);
+    PETCH_LAUNCH_CHECK(ctx);
+    y = gsrc;
+  }
+  if (!blend_each && alpha < 1.f) {  // the final blend's (1 - alpha) src term
+    hipLaunchKernelGGL(k_axpy, g, b, 0, ctx->stream, gsrc, gout, 1.f - alpha, F);
+    PETCH_LAUNCH_CHECK(ctx);
   }
   return PETCH_OK;
 }

@@ -149,6 +149,8 @@ SIGNATURES = {
     "petch_normalize": [P, P, P, c_int, c_float, c_float, c_float, P],
     "petch_cos_ratio": [P, P, P, c_int, P],
     "petch_smooth": [P, P, P, P, c_int, c_int, c_float, c_int, P],
+    "petch_normalize_adjoint": [P, P, P, c_int, c_float, c_float, c_float, P, P],
+    "petch_smooth_adjoint": [P, P, P, c_int, c_int, c_float, c_int, P, P],
     "petch_belen_coverage": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P],
     "petch_belen_coverage_adjoint": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P,
                                      P, P, P],

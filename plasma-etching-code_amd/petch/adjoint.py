@@ -27,7 +27,10 @@ The Russian-roulette neutral trace (_trace3d_cov_rr, threed.py:549-596) carries 
 adjoint (petch_trace_neutral_rr_adjoint): `neutral_rr(bvh, box, src, bare, beta, n, seed)`
 returns the raw per-face accumulator and back-propagates to bare with every ray's path and its
 roulette decisions held fixed (common random numbers: the backward pass redraws the same Philox
-streams, so nothing is stored per ray).
+streams, so nothing is stored per ray). `coupled_coverage` chains it with the clip-normalisation
+and smoothing of the fluxes (petch_normalize_adjoint, petch_smooth_adjoint) and the Belen update
+into the neutral rounds of the MC coupled flux (threed.py:855-904), differentiable in the ion
+terms.
 """
 import torch
 
@@ -151,3 +154,60 @@ def neutral_rr(bvh, box, src, bare, beta, n, seed):
     """Raw RR neutral accumulator (ops.trace_neutral_rr), differentiable in bare. The BVH must
     be unchanged between the forward and the backward pass."""
     return _NeutralRR.apply(bare, bvh, box, src, beta, n, seed)
+
+
+class _Finish(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, fl, areas, nbr, normals, scale, hi, n_smooth, alpha, blend_each):
+        m = ops.normalize(fl.detach().contiguous(), areas, scale, 0.0, hi)
+        if n_smooth > 0:
+            m = ops.smooth(m, nbr, normals, n_smooth, alpha, blend_each)
+        ctx.save_for_backward(fl.detach().contiguous(), areas, nbr, normals)
+        ctx.args = (float(scale), float(hi), int(n_smooth), float(alpha), bool(blend_each))
+        return m
+
+    @staticmethod
+    def backward(ctx, gm):
+        fl, areas, nbr, normals = ctx.saved_tensors
+        scale, hi, n_smooth, alpha, blend_each = ctx.args
+        g = gm.contiguous().float()
+        if n_smooth > 0:
+            g = ops.smooth_adjoint(nbr, normals, n_smooth, alpha, blend_each, g)
+        return (ops.normalize_adjoint(fl, areas, scale, 0.0, hi, g), None, None, None, None, None,
+                None, None, None)
+
+
+def finish(fl, mesh, scale, hi, n_smooth, alpha, blend_each):
+    """clamp((fl / A) * scale, 0, hi) then the flux smoothing, as the coupled flux's finish step,
+    differentiable in fl."""
+    return _Finish.apply(fl, mesh.areas, mesh.nbr, mesh.normals, scale, hi, n_smooth, alpha,
+                         blend_each)
+
+
+def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, n_fp=None):
+    """(bare, m_F, m_O) of the neutral rounds of flux.coupled_flux_steps for given ion terms
+    (m_i, cos_i, ion_yield): n_fp x {the F and O RR traces at bare (same seeds and streams as the
+    coupled flux; two single-species launches instead of the pair), finish, Belen update} from
+    bare = 1. Differentiable in m_i (without ion yields) or the ion yields, through every round:
+    the traces by their path-replay adjoint, finish by its transpose, Belen in closed form."""
+    from .chemistry import chem_params
+    from .flux import NEUTRAL_SOURCE, SEED_MASK, box_of
+    F = mesh.nf
+    A_src = geo['Lx'] * geo['Ly']
+    betaE, betaO = par.get('betaE', 0.7), par.get('betaO', 1.0)
+    n_smooth = int(par.get('flux_smooth', 1))
+    alpha = float(par.get('flux_smooth_alpha', 1.0))
+    blend_each = bool(par.get('device_flux', True)) and bool(par.get('flux_smooth_gpu', True))
+    box = box_of(geo, par)
+    chem = chem_params(par, flags)
+    n_fp = int(par.get('n_fp', 4)) if n_fp is None else int(n_fp)
+    bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
+    m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
+    for it in range(n_fp):
+        sF, sO = (seed * 9 + 2 + 2 * it) & SEED_MASK, (seed * 9 + 3 + 2 * it) & SEED_MASK
+        fF = neutral_rr(mesh.bvh, box, NEUTRAL_SOURCE, bare, betaE, n_neu, sF)
+        fO = neutral_rr(mesh.bvh, box, NEUTRAL_SOURCE, bare, betaO, n_neu, sO)
+        m_F = finish(fF, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
+        m_O = finish(fO, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
+        bare = belen(m_i, m_F, m_O, cos_i, ion_yield, chem)
+    return bare, m_F, m_O

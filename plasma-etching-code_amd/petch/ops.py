@@ -429,6 +429,33 @@ def smooth(src, nbr, normals, n_iter=1, alpha=1.0, blend_each=False):
     return out
 
 
+def normalize_adjoint(fl, areas, scale, lo, hi, gout):
+    """d loss / d fl of normalize(fl, areas, scale, lo, hi) for gout (petch_normalize_adjoint)."""
+    F = int(fl.shape[0])
+    for t, name in ((areas, "areas"), (gout, "gout")):
+        if t.numel() < F:
+            raise ValueError(f"normalize_adjoint: {name} holds {t.numel()} values, needs {F}")
+    gfl = torch.empty_like(fl)
+    if F:
+        _ctx(fl).call("petch_normalize_adjoint", ptr(fl, f32, "fl"), ptr(areas, f32, "areas"), F,
+                      float(scale), float(lo), float(hi), ptr(gout, f32, "gout"), ptr(gfl, f32))
+    return gfl
+
+
+def smooth_adjoint(nbr, normals, n_iter, alpha, blend_each, gout):
+    """d loss / d src of smooth(src, nbr, normals, n_iter, alpha, blend_each) for gout
+    (petch_smooth_adjoint; the smoothing is linear, so src is not needed)."""
+    F = int(gout.shape[0])
+    if nbr.numel() < 3 * F or normals.numel() < 3 * F:
+        raise ValueError("smooth_adjoint: nbr and normals need 3 values per face")
+    gsrc = torch.empty_like(gout)
+    if F:
+        _ctx(gout).call("petch_smooth_adjoint", ptr(nbr, i32, "nbr"), ptr(normals, f32, "normals"),
+                        F, int(n_iter), float(alpha), int(bool(blend_each)), ptr(gout, f32, "gout"),
+                        ptr(gsrc, f32))
+    return gsrc
+
+
 def belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem):
     """bare = clip(1 - thF - thO, 0, 1) with the Belen coverages (threed.py:715-737, 903-904)."""
     F = int(m_F.shape[0])

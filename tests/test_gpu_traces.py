@@ -240,3 +240,86 @@ def test_neutral_rr_path_adjoint_matches_oracle(name):
     (fl * tg).sum().backward()
     np.testing.assert_allclose(x.grad.cpu().numpy(), gs, rtol=1e-4,
                                atol=1e-5 * np.abs(gs).max())
+
+
+def _surface(name):
+    """flux.SurfaceMesh of a test geometry and the oracle mesh of the same triangles."""
+    import torch
+    from oracle import flux as ofl
+    from petch import flux
+    g = GEOMS[name]
+    phi = ol.init_sdf(g['Lx'], g['Ly'], g['Lz'], g['dx'], g['w'], g['mask'], g['sub'], g['hole'])
+    dev = torch.device("cuda", 0)
+    mesh = flux.SurfaceMesh(dev).build(torch.from_numpy(phi.astype(np.float32)).to(dev), g['dx'])
+    M = ofl.OracleMesh(mesh.verts.cpu().numpy(), mesh.faces.cpu().numpy())
+    geo = dict(Lx=g['Lx'], Ly=g['Ly'], Lz=g['Lz'], dx=g['dx'])
+    return g, mesh, M, geo, dev
+
+
+def test_normalize_and_smooth_adjoints_match_oracle():
+    """petch_normalize_adjoint and petch_smooth_adjoint against the oracle's explicit transposes
+    (the clip mask, and the scipy matrix of the smoothing passes, transposed), for both
+    smoothing semantics; the edge adjacency they rely on is symmetric."""
+    import torch
+    from oracle import flux as ofl
+    from petch import ops
+    g, mesh, M, geo, dev = _surface("trench")
+    nbr = mesh.nbr.cpu().numpy().reshape(-1, 3)
+    for f_, row in enumerate(nbr):
+        for x in row:
+            if x >= 0:
+                assert f_ in nbr[x], (f_, x)
+    rng = np.random.default_rng(4)
+    gm = rng.normal(size=mesh.nf).astype(np.float32)
+    tg = torch.from_numpy(gm).to(dev)
+    for n_s, a, each in ((1, 1.0, True), (2, 0.6, True), (2, 0.6, False), (3, 1.0, False)):
+        got = ops.smooth_adjoint(mesh.nbr, mesh.normals, n_s, a, each, tg).cpu().numpy()
+        T = ofl._smooth_matrix(M, n_s, a, each)
+        np.testing.assert_allclose(got, T.T.dot(gm.astype(np.float64)), rtol=0,
+                                   atol=1e-5 * np.abs(gm).max())
+    fl = rng.uniform(0.0, 3.0, mesh.nf).astype(np.float32)
+    A = mesh.areas.cpu().numpy()
+    scale = 0.002
+    got = ops.normalize_adjoint(torch.from_numpy(fl).to(dev), mesh.areas, scale, 0.0, 8.0,
+                                tg).cpu().numpy()
+    Ad = np.maximum(A.astype(np.float64), 1e-9)
+    v = fl / Ad * scale
+    ref = np.where((v >= 0.0) & (v <= 8.0), gm * scale / Ad, 0.0)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-7 * np.abs(ref).max())
+
+
+def test_coupled_coverage_adjoint_matches_oracle():
+    """The MC coverage loop (two rounds of F and O RR traces, finish, Belen) differentiated on
+    the GPU (path-replay trace adjoints, transposed finish, closed-form Belen) against the
+    oracle's unrolled adjoint of the same loop on the same Philox streams: the gradient of
+    w . bare in m_i within 5 % L1, and its directional derivatives within 5 %."""
+    import torch
+    from oracle import flux as ofl
+    from petch import adjoint
+    from petch.params import Flags
+    g, mesh, M, geo, dev = _surface("trench")
+    par = dict(PAR)
+    par['periodic_y'] = g['periodic']
+    fl = Flags(chemistry="belen", ion_reflection=True)
+    rng = np.random.default_rng(9)
+    m_i = rng.uniform(0.05, 1.2, mesh.nf).astype(np.float32)
+    cos_i = rng.uniform(0.0, 1.0, mesh.nf).astype(np.float32)
+    w = rng.normal(size=mesh.nf).astype(np.float32)
+    n = 20000
+    x = torch.from_numpy(m_i).to(dev).requires_grad_(True)
+    bare, _, _ = adjoint.coupled_coverage(mesh, geo, par, fl, x, torch.from_numpy(cos_i).to(dev),
+                                          None, n, 3, n_fp=2)
+    (bare * torch.from_numpy(w).to(dev)).sum().backward()
+    got = x.grad.cpu().numpy().astype(np.float64)
+    rb, _, _, _, grads = ofl.coupled_coverage(M, geo, par, fl, m_i.astype(np.float64),
+                                              cos_i.astype(np.float64), n, 3, 2,
+                                              w=w.astype(np.float64))
+    ref = grads["m_i"]
+    rel_bare = np.abs(bare.detach().cpu().numpy() - rb).sum() / np.abs(rb).sum()
+    rel = np.abs(got - ref).sum() / np.abs(ref).sum()
+    print(f"coupled coverage: bare L1 rel {rel_bare:.2e}, d/dm_i L1 rel {rel:.2e}", flush=True)
+    assert rel_bare < 0.02
+    assert rel < 0.05
+    for _ in range(2):
+        u = rng.normal(size=mesh.nf)
+        assert abs(got @ u - ref @ u) <= 0.05 * np.abs(ref * u).sum()

@@ -144,3 +144,47 @@ def test_rr_path_adjoint_matches_frozen_decision_differences(beta):
         lm = g @ bvh.trace_neutral_rr_frozen(box, src, bare, bare - h * u, beta, n, 7)
         fd = (lp - lm) / (2 * h)
         assert abs(fd - gb @ u) <= 1e-3 * abs(fd) + 1e-3, (fd, gb @ u)
+
+
+def test_coupled_coverage_adjoint_matches_frozen_decision_differences():
+    """The oracle's unrolled adjoint of the MC coverage loop (two rounds of F and O traces,
+    clip-normalisation, smoothing, Belen update) in m_i against central differences of the same
+    loop with every round's paths and decisions frozen. The smoothing matrix equals the
+    smoothing functions it transposes."""
+    from oracle import flux as ofl
+    from oracle import levelset as ol
+    from oracle import mesh as om
+    from petch.params import Flags
+    phi = ol.init_sdf(6.0, 1.5, 12.0, 0.15, 1.5, 7.0, 3.0, False).astype(np.float32)
+    v, f = on.marching_cubes(phi, 0.15)
+    M = ofl.OracleMesh(v, f)
+    geo = dict(Lx=6.0, Ly=1.5, Lz=12.0, dx=0.15)
+    par = dict(PAR)
+    par['periodic_y'] = 1
+    fl = Flags(chemistry="belen", ion_reflection=True)
+    rng = np.random.default_rng(8)
+    x = rng.uniform(0.0, 2.0, M.nf)
+    for dev, n_s, a in ((True, 1, 1.0), (True, 2, 0.6), (False, 2, 0.6)):
+        T = ofl._smooth_matrix(M, n_s, a, dev)
+        ref = x.copy()
+        if dev:
+            for _ in range(n_s):
+                ref = om.smooth_flux_dev(ref, M.normals, M.pairs, a)
+        else:
+            ref = om.smooth_flux(x, M.normals, M.pairs, n_s, a)
+        np.testing.assert_allclose(T.dot(x), ref, rtol=1e-12, atol=1e-12)
+    m_i = rng.uniform(0.05, 1.2, M.nf)
+    cos_i = rng.uniform(0.0, 1.0, M.nf)
+    w = rng.normal(size=M.nf)
+    n = 6000
+    bare, _, _, bares, grads = ofl.coupled_coverage(M, geo, par, fl, m_i, cos_i, n, 3, 2, w=w)
+    h = 1e-4
+    for _ in range(2):
+        u = rng.normal(size=M.nf)
+        lp = w @ ofl.coupled_coverage(M, geo, par, fl, m_i + h * u, cos_i, n, 3, 2,
+                                      frozen=bares)[0]
+        lm = w @ ofl.coupled_coverage(M, geo, par, fl, m_i - h * u, cos_i, n, 3, 2,
+                                      frozen=bares)[0]
+        fd = (lp - lm) / (2 * h)
+        an = grads["m_i"] @ u
+        assert abs(fd - an) <= 2e-3 * abs(fd) + 1e-4, (fd, an)
