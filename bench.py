@@ -242,9 +242,9 @@ def batched(n_feat=64, n_steps=5):
 
 
 def adjoint_cost(nsub=3, reps=10):
-    """Cost of the differentiable advection (petch.adjoint.advect) on the trench grid: nsub
-    substeps forward against forward + backward (the HIP adjoint through every substep). Only
-    the advection stage is differentiable so far; the rest of the step is not."""
+    """Cost of the differentiable advection (petch.adjoint.advect) on the trench grid: the loss
+    (w . phi after nsub substeps) against the loss and its gradient (the HIP adjoint through
+    every substep)."""
     import torch
     from petch import adjoint, ops
     from petch.threed import grid_shape
@@ -262,9 +262,9 @@ def adjoint_cost(nsub=3, reps=10):
     def run(grad):
         x = phi.clone().requires_grad_(grad)
         f = F.clone().requires_grad_(grad)
-        out = adjoint.advect(x, f, mask, mask_phi, W['dx'], dt, nsub)
+        loss = (adjoint.advect(x, f, mask, mask_phi, W['dx'], dt, nsub) * w).sum()
         if grad:
-            (out * w).sum().backward()
+            loss.backward()
 
     res = {}
     for grad in (False, True):
@@ -277,8 +277,8 @@ def adjoint_cost(nsub=3, reps=10):
         e.record()
         torch.cuda.synchronize()
         res["fwd_adj_ms" if grad else "fwd_ms"] = s.elapsed_time(e) / reps
-    return {"scope": f"level-set advection only ({nsub} substeps, {nx}x{ny}x{nz}); the flux, "
-                     "extension and reinit are not differentiated yet",
+    return {"scope": f"level-set advection, {nsub} substeps on {nx}x{ny}x{nz}; fwd = the loss "
+                     "w . phi, fwd_adj = the loss and its gradient in phi and F",
             **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
 
 
@@ -306,9 +306,9 @@ def radiosity_adjoint_cost(reps=5, n_ff=64, tol=1e-10):
 
     def run(grad):
         x = bare.clone().requires_grad_(grad)
-        m = adjoint.radiosity(x, D[:nf], hit, row_ptr, col, n_ff, 0.7, 0, tol)
+        loss = (adjoint.radiosity(x, D[:nf], hit, row_ptr, col, n_ff, 0.7, 0, tol) * w).sum()
         if grad:
-            (m * w).sum().backward()
+            loss.backward()
 
     res = {}
     for grad in (False, True):
@@ -346,9 +346,9 @@ def neutral_adjoint_cost(reps=3, beta=0.7):
 
     def run(grad):
         x = bare.clone().requires_grad_(grad)
-        fl = adjoint.neutral_rr(mesh.bvh, box, src, x, beta, N_NEU, 11)
+        loss = (adjoint.neutral_rr(mesh.bvh, box, src, x, beta, N_NEU, 11) * w).sum()
         if grad:
-            (fl * w).sum().backward()
+            loss.backward()
 
     res = {}
     for grad in (False, True):
