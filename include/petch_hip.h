@@ -339,12 +339,26 @@ int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float* m_F, const
                        const float* iy_ip, const float* centroids, const uint8_t* is_mask,
                        const petch_mask_geom* mg, const petch_chem_params* cp, int F, float* V,
                        unsigned* red);
+/* Closed-form adjoint of petch_surface_rate for gV = d loss / dV: gm_F, gm_O = d loss / d m_F,
+ * d m_O; with ion yields (belen) g_sp, g_ie, g_ip = d loss / d iy_sp, iy_ie, iy_ip, else
+ * gm_i = d loss / d m_i. Mask faces and non-finite V pass no gradient; cos_i is held constant.
+ * gm_i, g_sp, g_ie, g_ip may be NULL. No reference counterpart. */
+int petch_surface_rate_adjoint(petch_ctx* ctx, const float* m_i, const float* m_F,
+                               const float* m_O, const float* cos_i, const float* iy_sp,
+                               const float* iy_ie, const float* iy_ip, const float* centroids,
+                               const uint8_t* is_mask, const petch_mask_geom* mg,
+                               const petch_chem_params* cp, int F, const float* gV, float* gm_F,
+                               float* gm_O, float* gm_i, float* g_sp, float* g_ie, float* g_ip);
 
 /* ---------------------------------------------------------------- a15, a17 --------------- */
 /* extend_velocity_gpu + _extend_kernel (threed.py:1324-1346): Fs = V[nearest face] at nodes
  * with |phi| < band, 0 elsewhere (Fs is the full (nx,ny,nz) grid). */
 int petch_extend(petch_ctx* ctx, const petch_bvh* bvh, const float* V, const float* phi, int nx,
                  int ny, int nz, double dx, double band, float* Fs);
+/* Adjoint of petch_extend (linear in V): gV (nf, zeroed first) = sum of gFs over the band nodes
+ * whose nearest face each face is. gFs != gV. No reference counterpart. */
+int petch_extend_adjoint(petch_ctx* ctx, const petch_bvh* bvh, const float* phi, int nx, int ny,
+                         int nz, double dx, double band, const float* gFs, float* gV);
 /* _nearest_face_cov / gpu_warmstart_bare (threed.py:1619-1639): out[i] = src[nearest face of
  * pts[i]] on the given BVH, `miss` when the mesh is empty. */
 int petch_nearest_face_value(petch_ctx* ctx, const petch_bvh* bvh, const float* src,

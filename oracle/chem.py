@@ -9,6 +9,7 @@ Restates:
   belen_coverages      threed.py:715-737
   belen_bare_adjoint   derivative of clip(1 - thF - thO, 0, 1) (not in the reference), by
                        complex-step differentiation of belen_coverages
+  surface_rate_adjoint derivative of surface_rate (not in the reference), by complex step
   faces_in_mask        threed.py:1608-1616
   ion_params           threed.py:800-802, 811
 """
@@ -58,9 +59,9 @@ def _mode(flags):
 def surface_rate_langmuir(m_i, m_F, m_O, cos_i, is_mask, par, flags=None):
     Yie, Ysp, Yp = yields(par)
     f_ie, f_sp = angular_factors(cos_i, par, _mode(flags))
-    Fi = par['ionFlux'] * np.asarray(m_i, np.float64)
-    Fev = par['Fflux'] * np.asarray(m_F, np.float64)
-    Fp = par['Oflux'] * np.asarray(m_O, np.float64)
+    Fi = par['ionFlux'] * _num(m_i)
+    Fev = par['Fflux'] * _num(m_F)
+    Fp = par['Oflux'] * _num(m_O)
     eps = 1e-9
     Yie_a, Ysp_a, Yp_a = Yie * f_ie, Ysp * f_sp, Yp * f_ie
     rF = par['s_F'] * Fev / (Yie_a * Fi + eps)
@@ -68,7 +69,7 @@ def surface_rate_langmuir(m_i, m_F, m_O, cos_i, is_mask, par, flags=None):
     bare = 1.0 / (1.0 + rF + rO)
     thF = rF * bare
     V = (1.0 / par['rho']) * (Yie_a * Fi * thF + Ysp_a * Fi * bare) * par['rate_scale']
-    V = np.array(V, np.float64)
+    V = np.array(V, np.result_type(V, np.float64))
     V[np.asarray(is_mask, bool)] = 0.0
     return V
 
@@ -77,25 +78,26 @@ def surface_rate_belen(m_i, m_F, m_O, cos_i, is_mask, par, flags=None, ion_yield
     """belen.py:26-63; `ion_yield` = (sp, ie, ip) stands for par['_ion_yield']."""
     Yie, Ysp, Yp = yields(par)
     f_ie, f_sp = angular_factors(cos_i, par, _mode(flags))
-    Fi = par['ionFlux'] * np.asarray(m_i, np.float64)
+    Fi = par['ionFlux'] * _num(m_i)
     Yie_a, Ysp_a, Yp_a = Yie * f_ie, Ysp * f_sp, Yp * f_ie
     eps = 1e-9
     iy = ion_yield if ion_yield is not None else par.get('_ion_yield')
     if iy is not None:
-        _sp, _ie, _ip = (np.asarray(a, np.float64) for a in iy)
+        _sp, _ie, _ip = (_num(a) for a in iy)
         ionF = par['ionFlux']
         GY_ie, GY_p, sputter, ion_enh = ionF * _ie, ionF * _ip, ionF * _sp, ionF * _ie
     else:
         GY_ie, GY_p, sputter, ion_enh = Yie_a * Fi, Yp_a * Fi, Ysp_a * Fi, Yie_a * Fi
     fnE = par.get('fnorm_E', 1.0)
     fnO = par.get('fnorm_O', 1.0)
-    Gb_E = par['Fflux'] * np.asarray(m_F, np.float64) / fnE + eps
-    Gb_P = par['Oflux'] * np.asarray(m_O, np.float64) / fnO + eps
+    Gb_E = par['Fflux'] * _num(m_F) / fnE + eps
+    Gb_P = par['Oflux'] * _num(m_O) / fnO + eps
     a = (par['k_sigma'] + 2.0 * GY_ie) / Gb_E
     b = (par['beta_sigma'] + GY_p) / Gb_P
     thF = 1.0 / (1.0 + a * (1.0 + 1.0 / (b + eps)))
     rate = par['k_sigma'] * thF / 4.0 + sputter + thF * ion_enh
-    V = np.array((1.0 / par['rho']) * rate * par['rate_scale'], np.float64)
+    V = (1.0 / par['rho']) * rate * par['rate_scale']
+    V = np.array(V, np.result_type(V, np.float64))
     V[np.asarray(is_mask, bool)] = 0.0
     return V
 
@@ -163,6 +165,28 @@ def belen_bare_adjoint(m_i, m_F, m_O, cos_i, par, flags, gbare, ion_yield=None):
         v = dict(base)
         v[k] = base[k] + 1j * h
         out[k] = g * bare_of(v).imag / h
+    return out
+
+
+def surface_rate_adjoint(m_i, m_F, m_O, cos_i, is_mask, par, flags, gV, ion_yield=None):
+    """d loss / d inputs of surface_rate for gV = d loss / dV, by complex-step differentiation
+    (each face's V depends on that face's inputs only): dict(m_F, m_O, and m_i or sp, ie, ip);
+    cos_i constant; mask faces pass nothing."""
+    h = 1e-30
+    base = dict(m_i=np.asarray(m_i, np.float64), m_F=np.asarray(m_F, np.float64),
+                m_O=np.asarray(m_O, np.float64))
+    if ion_yield is not None:
+        base.update(sp=np.asarray(ion_yield[0], np.float64), ie=np.asarray(ion_yield[1], np.float64),
+                    ip=np.asarray(ion_yield[2], np.float64))
+    keys = ("m_F", "m_O") + (("sp", "ie", "ip") if ion_yield is not None else ("m_i",))
+    g = np.asarray(gV, np.float64)
+    out = {}
+    for k in keys:
+        v = dict(base)
+        v[k] = base[k] + 1j * h
+        iy = (v['sp'], v['ie'], v['ip']) if ion_yield is not None else None
+        V = surface_rate(v['m_i'], v['m_F'], v['m_O'], cos_i, is_mask, par, flags, ion_yield=iy)
+        out[k] = g * np.imag(V) / h
     return out
 
 

@@ -9,6 +9,7 @@
 //                         non-finite -> 0 (counted), running max of V   belen.py:26-63,
 //                                                                       threed.py:1608-1616,1721-1728,1750
 //   petch_extend          Fs[node] = V[nearest face] in |phi| < band     threed.py:1324-1346
+//   petch_surface_rate_adjoint, petch_extend_adjoint: their adjoints (no reference counterpart)
 //   petch_nearest_face_value  out[i] = src[nearest face of pts[i]]       threed.py:1619-1639
 //   petch_extend_batch, petch_nearest_face_value_batch: the same for many grids / point sets in
 //                         one pass (the batched etch loop)
@@ -220,6 +221,23 @@ struct MaskGeom {
   int hole;
 };
 
+// faces_in_mask on the centroid (threed.py:1608-1616), or the given mask
+__device__ __forceinline__ bool face_masked(const uint8_t* __restrict__ is_mask,
+                                            const float* __restrict__ centroids,
+                                            const MaskGeom& mg, int f) {
+  if (is_mask) return is_mask[f] != 0;
+  double x = centroids[3 * f], y = centroids[3 * f + 1], z = centroids[3 * f + 2];
+  bool in_band = (z >= mg.sub_top) && (z < mg.top);
+  bool opening;
+  if (mg.hole) {
+    double ax = x - mg.cx, ay = y - mg.cy;
+    opening = ax * ax + ay * ay < mg.hw2;
+  } else {
+    opening = fabs(x - mg.cx) < mg.hw;
+  }
+  return in_band && !opening;
+}
+
 __global__ void k_surface_rate(const float* __restrict__ m_i, const float* __restrict__ m_F,
                                const float* __restrict__ m_O, const float* __restrict__ cos_i,
                                const float* __restrict__ iy_sp, const float* __restrict__ iy_ie,
@@ -264,23 +282,7 @@ __global__ void k_surface_rate(const float* __restrict__ m_i, const float* __res
     double thF = rF * br;
     v = (1.0 / c.rho) * (Yie_a * Fi * thF + Ysp_a * Fi * br) * c.rate_scale;
   }
-  bool masked;
-  if (is_mask) {
-    masked = is_mask[f] != 0;
-  } else {
-    // faces_in_mask on the centroid (threed.py:1608-1616)
-    double x = centroids[3 * f], y = centroids[3 * f + 1], z = centroids[3 * f + 2];
-    bool in_band = (z >= mg.sub_top) && (z < mg.top);
-    bool opening;
-    if (mg.hole) {
-      double ax = x - mg.cx, ay = y - mg.cy;
-      opening = ax * ax + ay * ay < mg.hw2;
-    } else {
-      opening = fabs(x - mg.cx) < mg.hw;
-    }
-    masked = in_band && !opening;
-  }
-  if (masked) v = 0.0;
+  if (face_masked(is_mask, centroids, mg, f)) v = 0.0;
   float vf = (float)v;
   if (!(fabsf(vf) <= 3.402823466e38f)) {  // NaN or inf
     vf = 0.f;
@@ -289,6 +291,93 @@ __global__ void k_surface_rate(const float* __restrict__ m_i, const float* __res
   V[f] = vf;
   // V >= 0 here, so the float bit pattern orders like an unsigned integer
   atomicMax(red, __float_as_uint(vf > 0.f ? vf : 0.f));
+}
+
+// Closed-form adjoint of k_surface_rate for gV = d loss / dV (zero on mask faces and where V
+// was not finite): gm_F, gm_O, and d / d m_i (gm_i) or, with ion yields, d / d iy_sp, iy_ie,
+// iy_ip (g_sp, g_ie, g_ip); cos_i is held constant. Optional outputs may be NULL.
+//   belen:    V = K (k_sigma thF / 4 + sputter + thF ion_enh), thF = 1 / (1 + a (1 + 1/(b+eps)))
+//   langmuir: V = K (Yie_a Fi thF + Ysp_a Fi br), br = 1 / (1 + rF + rO), thF = rF br
+// with K = rate_scale / rho.
+__global__ void k_surface_rate_adjoint(
+    const float* __restrict__ m_i, const float* __restrict__ m_F, const float* __restrict__ m_O,
+    const float* __restrict__ cos_i, const float* __restrict__ iy_sp,
+    const float* __restrict__ iy_ie, const float* __restrict__ iy_ip,
+    const float* __restrict__ centroids, const uint8_t* __restrict__ is_mask, MaskGeom mg, Chem c,
+    int F, const float* __restrict__ gV, float* __restrict__ gm_F, float* __restrict__ gm_O,
+    float* __restrict__ gm_i, float* __restrict__ g_sp, float* __restrict__ g_ie,
+    float* __restrict__ g_ip) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  const double eps = 1e-9;
+  double f_ie, f_sp;
+  angular((double)cos_i[f], c, f_ie, f_sp);
+  const double Fi = c.ionFlux * (double)m_i[f];
+  const double Yie_a = c.Yie * f_ie, Ysp_a = c.Ysp * f_sp, Yp_a = c.Yp * f_ie;
+  const double K = c.rate_scale / c.rho;
+  double v, dF = 0.0, dO = 0.0, dI = 0.0, dsp = 0.0, die = 0.0, dip = 0.0;
+  if (c.belen) {
+    double GY_ie, GY_p, sputter, ion_enh;
+    if (iy_ie) {
+      GY_ie = c.ionFlux * (double)iy_ie[f];
+      GY_p = c.ionFlux * (double)iy_ip[f];
+      sputter = c.ionFlux * (double)iy_sp[f];
+      ion_enh = GY_ie;
+    } else {
+      GY_ie = Yie_a * Fi;
+      GY_p = Yp_a * Fi;
+      sputter = Ysp_a * Fi;
+      ion_enh = Yie_a * Fi;
+    }
+    const double GbE = c.Fflux * (double)m_F[f] / c.fnE + eps;
+    const double GbP = c.Oflux * (double)m_O[f] / c.fnO + eps;
+    const double a = (c.k_sigma + 2.0 * GY_ie) / GbE;
+    const double b = (c.beta_sigma + GY_p) / GbP;
+    const double ib = 1.0 / (b + eps);
+    const double thF = 1.0 / (1.0 + a * (1.0 + ib));
+    v = K * (c.k_sigma * thF / 4.0 + sputter + thF * ion_enh);
+    const double dth = K * (c.k_sigma / 4.0 + ion_enh);   // dV / dthF
+    const double ta = -(1.0 + ib) * thF * thF;             // dthF / da
+    const double tb = a * thF * thF * ib * ib;             // dthF / db
+    dF = dth * ta * (-a * (c.Fflux / c.fnE) / GbE);
+    dO = dth * tb * (-b * (c.Oflux / c.fnO) / GbP);
+    const double dGie = dth * ta * 2.0 / GbE, dGp = dth * tb / GbP;  // dV / dGY_ie, dGY_p
+    if (iy_ie) {
+      dsp = K * c.ionFlux;
+      die = (dGie + K * thF) * c.ionFlux;
+      dip = dGp * c.ionFlux;
+    } else {
+      dI = (dGie * Yie_a + dGp * Yp_a + K * Ysp_a + K * thF * Yie_a) * c.ionFlux;
+    }
+  } else {
+    const double Fev = c.Fflux * (double)m_F[f], Fp = c.Oflux * (double)m_O[f];
+    const double dE = Yie_a * Fi + eps, dP = Yp_a * Fi + eps;
+    const double rF = c.s_F * Fev / dE, rO = c.s_O * Fp / dP;
+    const double br = 1.0 / (1.0 + rF + rO);
+    const double thF = rF * br;
+    v = K * (Yie_a * Fi * thF + Ysp_a * Fi * br);
+    // dV/drF and dV/drO at fixed Fi (dbr/dr = -br^2, dthF/drF = br - rF br^2, dthF/drO = -rF br^2)
+    const double vF = K * (Yie_a * Fi * (br - rF * br * br) - Ysp_a * Fi * br * br);
+    const double vO = K * (-Yie_a * Fi * rF * br * br - Ysp_a * Fi * br * br);
+    dF = vF * c.s_F * c.Fflux / dE;
+    dO = vO * c.s_O * c.Oflux / dP;
+    // m_i enters through Fi: explicitly, and through rF, rO (drF/dFi = -rF Yie_a / dE)
+    const double dFi = K * (Yie_a * thF + Ysp_a * br) + vF * (-rF * Yie_a / dE) +
+                       vO * (-rO * Yp_a / dP);
+    dI = dFi * c.ionFlux;
+  }
+  double g = (double)gV[f];
+  if (face_masked(is_mask, centroids, mg, f) || !(fabs((double)(float)v) <= 3.402823466e38))
+    g = 0.0;
+  gm_F[f] = (float)(g * dF);
+  gm_O[f] = (float)(g * dO);
+  if (iy_ie && c.belen) {
+    if (g_sp) g_sp[f] = (float)(g * dsp);
+    if (g_ie) g_ie[f] = (float)(g * die);
+    if (g_ip) g_ip[f] = (float)(g * dip);
+  } else if (gm_i) {
+    gm_i[f] = (float)(g * dI);
+  }
 }
 
 // Velocity extension of one or many grids in one pass: flag every grid's band (and zero its
@@ -318,13 +407,13 @@ __device__ __forceinline__ int locate_item(const ExtendItem* __restrict__ it, in
 }
 
 __global__ void k_band_flags(const ExtendItem* __restrict__ it, int n, long long total,
-                             int* __restrict__ flag) {
+                             int* __restrict__ flag, int zero_fs) {
   const long long g = (long long)blockIdx.x * kBS + threadIdx.x;
   if (g >= total) return;
   long long m;
   const ExtendItem& e = it[locate_item(it, n, g, m)];
   flag[g] = fabs((double)e.phi[m]) < e.band ? 1 : 0;
-  e.Fs[m] = 0.f;
+  if (zero_fs) e.Fs[m] = 0.f;
 }
 
 __global__ void k_band_compact(const int* __restrict__ flag, const int* __restrict__ pos,
@@ -347,6 +436,24 @@ __global__ void k_extend(const ExtendItem* __restrict__ it, int n, const int* __
                                   (float)((double)k * e.dx));
   const petch::PointHit h = petch::bvh_closest_point(e.bv, p, 1.0e6f);
   e.Fs[m] = h.face >= 0 ? e.V[h.face] : 0.f;
+}
+
+// Adjoint of k_extend (one grid): the extension copies V[nearest face] to each band node, so
+// its transpose adds each band node's gFs (read through the item's Fs) to gV[nearest face].
+__global__ void k_extend_adjoint(const ExtendItem* __restrict__ it, int n,
+                                 const int* __restrict__ idx, const int* __restrict__ count,
+                                 float* __restrict__ gV) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= *count) return;
+  long long m;
+  const ExtendItem& e = it[locate_item(it, n, idx[i], m)];
+  const int k = (int)(m % e.nz);
+  const int j = (int)((m / e.nz) % e.ny);
+  const int ii = (int)(m / ((long long)e.nz * e.ny));
+  const petch::Vec3 p = petch::v3((float)((double)ii * e.dx), (float)((double)j * e.dx),
+                                  (float)((double)k * e.dx));
+  const petch::PointHit h = petch::bvh_closest_point(e.bv, p, 1.0e6f);
+  if (h.face >= 0) atomicAdd(gV + h.face, e.Fs[m]);
 }
 
 // Nearest-face values of many point sets, each on its own surface, in one launch.
@@ -515,6 +622,28 @@ extern "C" int petch_belen_coverage_adjoint(petch_ctx* ctx, const float* m_i, co
   return PETCH_OK;
 }
 
+extern "C" int petch_surface_rate_adjoint(
+    petch_ctx* ctx, const float* m_i, const float* m_F, const float* m_O, const float* cos_i,
+    const float* iy_sp, const float* iy_ie, const float* iy_ip, const float* centroids,
+    const uint8_t* is_mask, const petch_mask_geom* mg, const petch_chem_params* cp, int F,
+    const float* gV, float* gm_F, float* gm_O, float* gm_i, float* g_sp, float* g_ie,
+    float* g_ip) {
+  PETCH_ARG(ctx, ctx && m_i && m_F && m_O && cos_i && mg && cp && gV && gm_F && gm_O,
+            "surface_rate_adjoint: null pointer");
+  PETCH_ARG(ctx, centroids || is_mask, "surface_rate_adjoint: need centroids or an explicit mask");
+  PETCH_ARG(ctx, !iy_ie || (iy_sp && iy_ip),
+            "surface_rate_adjoint: ion yields need all three channels");
+  if (F <= 0) return PETCH_OK;
+  MaskGeom g{mg->sub_top, mg->sub_top + mg->mask_th, mg->Lx / 2.0, mg->Ly / 2.0,
+             mg->trench_width / 2.0, (mg->trench_width / 2.0) * (mg->trench_width / 2.0),
+             mg->hole};
+  hipLaunchKernelGGL(k_surface_rate_adjoint, dim3(petch_blocks(F, kBS)), dim3(kBS), 0,
+                     ctx->stream, m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask, g,
+                     make_chem(cp), F, gV, gm_F, gm_O, gm_i, g_sp, g_ie, g_ip);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
 extern "C" int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float* m_F,
                                   const float* m_O, const float* cos_i, const float* iy_sp,
                                   const float* iy_ie, const float* iy_ip, const float* centroids,
@@ -536,7 +665,11 @@ extern "C" int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float*
   return PETCH_OK;
 }
 
-extern "C" int petch_extend_batch(petch_ctx* ctx, int n, const petch_extend_item* items) {
+// The band flags (zeroing Fs, which the extension then fills on the band), their scan and the
+// band list of the items, then the extension queries; with gV_adjoint (one item) the
+// transposed queries instead, which read the item's Fs as gFs (left untouched) and add it into
+// gV_adjoint.
+static int extend_run(petch_ctx* ctx, int n, const petch_extend_item* items, float* gV_adjoint) {
   PETCH_ARG(ctx, ctx && n >= 0 && (n == 0 || items), "extend_batch: null pointer or n < 0");
   if (n == 0) return PETCH_OK;
   std::vector<ExtendItem> t((size_t)n);
@@ -558,15 +691,25 @@ extern "C" int petch_extend_batch(petch_ctx* ctx, int n, const petch_extend_item
   int* idx = (int*)petch_scratch(ctx, SCR_LS_FROZEN, (size_t)(total + 1) * sizeof(int));
   if (!dev || !flag || !pos || !idx) return PETCH_E_HIP;
   const unsigned g = petch_blocks(total, kBS);
-  hipLaunchKernelGGL(k_band_flags, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, total, flag);
+  hipLaunchKernelGGL(k_band_flags, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, total, flag,
+                     gV_adjoint ? 0 : 1);
   PETCH_LAUNCH_CHECK(ctx);
   int rc = petch_exclusive_scan(ctx, flag, pos, total, idx + total);
   if (rc != PETCH_OK) return rc;
   hipLaunchKernelGGL(k_band_compact, dim3(g), dim3(kBS), 0, ctx->stream, flag, pos, total, idx);
   PETCH_LAUNCH_CHECK(ctx);
-  hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, idx, idx + total);
+  if (gV_adjoint) {
+    hipLaunchKernelGGL(k_extend_adjoint, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, idx,
+                       idx + total, gV_adjoint);
+  } else {
+    hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, idx, idx + total);
+  }
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
+}
+
+extern "C" int petch_extend_batch(petch_ctx* ctx, int n, const petch_extend_item* items) {
+  return extend_run(ctx, n, items, nullptr);
 }
 
 extern "C" int petch_extend(petch_ctx* ctx, const petch_bvh* bvh, const float* V, const float* phi,
@@ -575,6 +718,19 @@ extern "C" int petch_extend(petch_ctx* ctx, const petch_bvh* bvh, const float* V
   PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "extend: bad shape");
   const petch_extend_item it{bvh, V, phi, Fs, nx, ny, nz, dx, band};
   return petch_extend_batch(ctx, 1, &it);
+}
+
+extern "C" int petch_extend_adjoint(petch_ctx* ctx, const petch_bvh* bvh, const float* phi, int nx,
+                                    int ny, int nz, double dx, double band, const float* gFs,
+                                    float* gV) {
+  PETCH_ARG(ctx, ctx && bvh && phi && gFs && gV, "extend_adjoint: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "extend_adjoint: bad shape");
+  PETCH_ARG(ctx, (const void*)gFs != (const void*)gV, "extend_adjoint: gFs and gV must differ");
+  const long long N = (long long)nx * ny * nz;
+  PETCH_ARG(ctx, N < 0x7fffffffLL, "extend_adjoint: %lld nodes exceed the int32 node index", N);
+  PETCH_HIP(ctx, hipMemsetAsync(gV, 0, (size_t)bvh->nf * sizeof(float), ctx->stream));
+  const petch_extend_item it{bvh, gV, phi, const_cast<float*>(gFs), nx, ny, nz, dx, band};
+  return extend_run(ctx, 1, &it, gV);
 }
 
 extern "C" int petch_nearest_face_value_batch(petch_ctx* ctx, int n,

@@ -31,6 +31,12 @@ streams, so nothing is stored per ray). `coupled_coverage` chains it with the cl
 and smoothing of the fluxes (petch_normalize_adjoint, petch_smooth_adjoint) and the Belen update
 into the neutral rounds of the MC coupled flux (threed.py:855-904), differentiable in the ion
 terms.
+
+The etch rate and the velocity extension carry their adjoints too: `surface_rate(...)`
+(petch_surface_rate_adjoint, closed form for the Belen and Langmuir chemistries) and
+`extend(bvh, V, phi, dx, band)` (petch_extend_adjoint: each band node's gradient goes back to its
+nearest face; the nearest-face assignment is the geometry and stays fixed). With advect they
+take a loss on the advanced level set back to the fluxes of a step.
 """
 import torch
 
@@ -211,3 +217,55 @@ def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, 
         m_O = finish(fO, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
         bare = belen(m_i, m_F, m_O, cos_i, ion_yield, chem)
     return bare, m_F, m_O
+
+
+class _SurfaceRate(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, mask_geom, chem,
+                is_mask):
+        iy = (iy_sp, iy_ie, iy_ip) if iy_ie is not None else None
+        V, red = ops.surface_rate(m_i.contiguous(), m_F.contiguous(), m_O.contiguous(), cos_i,
+                                  iy, centroids, mask_geom, chem, is_mask=is_mask)
+        ctx.save_for_backward(m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask)
+        ctx.args = (mask_geom, chem)
+        return V
+
+    @staticmethod
+    def backward(ctx, gV):
+        m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask = ctx.saved_tensors
+        mask_geom, chem = ctx.args
+        iy = (iy_sp, iy_ie, iy_ip) if iy_ie is not None else None
+        gm_F, gm_O, gm_i, g_sp, g_ie, g_ip = ops.surface_rate_adjoint(
+            m_i.contiguous(), m_F.contiguous(), m_O.contiguous(), cos_i, iy, centroids,
+            mask_geom, chem, gV.contiguous().float(), is_mask=is_mask)
+        return gm_i, gm_F, gm_O, None, g_sp, g_ie, g_ip, None, None, None, None
+
+
+def surface_rate(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, is_mask=None):
+    """V (ops.surface_rate), differentiable in m_F, m_O and the ion terms (m_i, or the ion
+    yields when given)."""
+    sp, ie, ip = ion_yield if ion_yield is not None else (None, None, None)
+    return _SurfaceRate.apply(m_i, m_F, m_O, cos_i, sp, ie, ip, centroids, mask_geom, chem,
+                              is_mask)
+
+
+class _Extend(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, V, bvh, phi, dx, band):
+        Fs = ops.extend(bvh, V.detach().contiguous(), phi, dx, band)
+        ctx.save_for_backward(phi)
+        ctx.args = (bvh, float(dx), float(band))
+        return Fs
+
+    @staticmethod
+    def backward(ctx, gFs):
+        (phi,) = ctx.saved_tensors
+        bvh, dx, band = ctx.args
+        return ops.extend_adjoint(bvh, phi, dx, band, gFs.contiguous().float()), None, None, \
+            None, None
+
+
+def extend(bvh, V, phi, dx, band):
+    """Fs = V[nearest face] on the band |phi| < band (ops.extend), differentiable in V. The BVH
+    must be unchanged between the forward and the backward pass."""
+    return _Extend.apply(V, bvh, phi, dx, band)

@@ -493,6 +493,29 @@ def belen_coverage_adjoint(m_i, m_F, m_O, cos_i, ion_yield, chem, gbare):
     return gm_F, gm_O, gm_i, g_ie, g_ip
 
 
+def surface_rate_adjoint(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, gV,
+                         is_mask=None):
+    """Adjoint of surface_rate (petch_surface_rate_adjoint): (gm_F, gm_O, gm_i, g_sp, g_ie,
+    g_ip) for gV = d loss / dV; gm_i is None with ion yields, the g_* are None without."""
+    F = int(m_F.shape[0])
+    for t, name in ((m_i, "m_i"), (m_O, "m_O"), (cos_i, "cos_i"), (gV, "gV")):
+        if t.numel() < F:
+            raise ValueError(f"surface_rate_adjoint: {name} holds {t.numel()} values, needs {F}")
+    sp, ie, ip = ion_yield if ion_yield is not None else (None, None, None)
+    gm_F, gm_O = torch.empty_like(m_F), torch.empty_like(m_F)
+    gm_i = g_sp = g_ie = g_ip = None
+    if ion_yield is not None:
+        g_sp, g_ie, g_ip = (torch.empty_like(m_F) for _ in range(3))
+    else:
+        gm_i = torch.empty_like(m_F)
+    if F:
+        _ctx(m_F).call("petch_surface_rate_adjoint", ptr(m_i, f32), ptr(m_F, f32), ptr(m_O, f32),
+                       ptr(cos_i, f32), ptr(sp), ptr(ie), ptr(ip), ptr(centroids), ptr(is_mask),
+                       ctypes_ref(mask_geom), ctypes_ref(chem), F, ptr(gV, f32, "gV"), ptr(gm_F),
+                       ptr(gm_O), ptr(gm_i), ptr(g_sp), ptr(g_ie), ptr(g_ip))
+    return gm_F, gm_O, gm_i, g_sp, g_ie, g_ip
+
+
 def surface_rate(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, is_mask=None,
                  red=None):
     """(V (F,), red (2,) uint32 = [max V bits, non-finite count]) on the device."""
@@ -527,6 +550,18 @@ def extend(bvh, V, phi, dx, band, out=None):
     bvh.ctx.call("petch_extend", bvh.h, ptr(V, f32), ptr(phi, f32), nx, ny, nz, float(dx),
                  float(band), ptr(Fs, f32))
     return Fs
+
+
+def extend_adjoint(bvh, phi, dx, band, gFs):
+    """d loss / dV (nf,) of extend(bvh, V, phi, dx, band) for gFs = d loss / dFs
+    (petch_extend_adjoint)."""
+    nx, ny, nz = _shape3(phi)
+    if tuple(gFs.shape) != (nx, ny, nz):
+        raise ValueError(f"extend_adjoint: gFs shape {tuple(gFs.shape)} != phi shape {(nx, ny, nz)}")
+    gV = torch.empty((max(bvh.nf, 1),), dtype=f32, device=phi.device)
+    bvh.ctx.call("petch_extend_adjoint", bvh.h, ptr(phi, f32, "phi"), nx, ny, nz, float(dx),
+                 float(band), ptr(gFs, f32, "gFs"), ptr(gV, f32))
+    return gV[:bvh.nf]
 
 
 def extend_batch(items):

@@ -371,3 +371,73 @@ def test_coverage_fixed_point_adjoint_matches_oracle():
     np.testing.assert_allclose(bare.detach().cpu().numpy(), rb, rtol=0, atol=1e-5)
     for got, r in ((mi.grad, grads["m_i"]), (d.grad, grads["D"])):
         np.testing.assert_allclose(got.cpu().numpy(), r, rtol=0, atol=1e-4 * np.abs(r).max())
+
+
+@pytest.mark.parametrize("chem,yields", [("belen", False), ("belen", True), ("langmuir", False)])
+def test_surface_rate_adjoint_matches_oracle(chem, yields):
+    """petch_surface_rate_adjoint (closed form) against the oracle's complex-step derivative of
+    the same rate, with mask faces (no gradient); through the autograd Function as well."""
+    import torch
+    from petch import adjoint, ops
+    from petch.chemistry import chem_params
+    rng = np.random.default_rng(31)
+    F = 4000
+    par = dict(PAR)
+    fl = Flags(chemistry=chem, ion_reflection=True)
+    m_i = rng.uniform(0.05, 1.3, F).astype(np.float32)
+    cos_i = rng.uniform(0.0, 1.0, F).astype(np.float32)
+    m_F = rng.uniform(0.01, 1.5, F).astype(np.float32)
+    m_O = rng.uniform(0.01, 1.5, F).astype(np.float32)
+    msk = rng.random(F) < 0.1
+    iy = tuple(rng.uniform(0.0, 0.5, F).astype(np.float32) for _ in range(3)) if yields else None
+    g = rng.normal(size=F).astype(np.float32)
+    cp = chem_params(par, fl)
+    tiy = tuple(_t(a) for a in iy) if iy else None
+    tm = _t(msk.astype(np.uint8))
+    out = ops.surface_rate_adjoint(_t(m_i), _t(m_F), _t(m_O), _t(cos_i), tiy, None,
+                                   ops.MaskGeom(), cp, _t(g), is_mask=tm)
+    ref = oc.surface_rate_adjoint(m_i.astype(np.float64), m_F.astype(np.float64),
+                                  m_O.astype(np.float64), cos_i.astype(np.float64), msk, par, fl,
+                                  g.astype(np.float64),
+                                  ion_yield=tuple(a.astype(np.float64) for a in iy) if iy else None)
+    for name, got in zip(("m_F", "m_O", "m_i", "sp", "ie", "ip"), out):
+        if name not in ref:
+            assert got is None
+            continue
+        r = ref[name]
+        np.testing.assert_allclose(got.cpu().numpy(), r, rtol=1e-5, atol=1e-6 * np.abs(r).max(),
+                                   err_msg=name)
+        assert np.all(got.cpu().numpy()[msk] == 0.0)
+    xF = _t(m_F).requires_grad_(True)
+    V = adjoint.surface_rate(_t(m_i), xF, _t(m_O), _t(cos_i), tiy, None, ops.MaskGeom(), cp,
+                             is_mask=tm)
+    (V * _t(g)).sum().backward()
+    np.testing.assert_allclose(xF.grad.cpu().numpy(), out[0].cpu().numpy(), rtol=0, atol=0)
+
+
+def test_extend_adjoint_is_the_transpose():
+    """petch_extend_adjoint sends each band node's gradient to its nearest face: it equals the
+    scatter over the GPU's own nearest faces, and <gFs, extend(V)> = <extend_adjoint(gFs), V>
+    (the extension is linear in V). Through the autograd Function as well."""
+    import torch
+    from petch import adjoint, ops
+    phi, v, f = _mesh()
+    dx = 0.25
+    bvh = ops.Bvh(_dev()).build(_t(v), _t(f))
+    band = 4 * dx
+    rng = np.random.default_rng(12)
+    gFs = rng.normal(size=phi.shape).astype(np.float32)
+    gV = ops.extend_adjoint(bvh, _t(phi), dx, band, _t(gFs)).cpu().numpy().astype(np.float64)
+    inb = np.abs(phi.astype(np.float64)) < band
+    ii, jj, kk = np.where(inb)
+    pts = np.stack([ii * dx, jj * dx, kk * dx], 1).astype(np.float32)
+    gf, _ = bvh.query_point(_t(pts))
+    ref = np.bincount(gf.cpu().numpy(), weights=gFs[inb].astype(np.float64), minlength=len(f))
+    np.testing.assert_allclose(gV, ref, rtol=1e-5, atol=1e-4)
+    V = rng.uniform(0.0, 3.0, len(f)).astype(np.float32)
+    Fs = ops.extend(bvh, _t(V), _t(phi), dx, band).cpu().numpy().astype(np.float64)
+    lhs, rhs = float((gFs * Fs).sum()), float(gV @ V)
+    assert abs(lhs - rhs) <= 1e-5 * (np.abs(gFs * Fs).sum())
+    x = _t(V).requires_grad_(True)
+    (adjoint.extend(bvh, x, _t(phi), dx, band) * _t(gFs)).sum().backward()
+    np.testing.assert_allclose(x.grad.cpu().numpy(), gV, rtol=1e-5, atol=1e-4)
