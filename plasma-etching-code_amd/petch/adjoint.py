@@ -269,3 +269,21 @@ def extend(bvh, V, phi, dx, band):
     """Fs = V[nearest face] on the band |phi| < band (ops.extend), differentiable in V. The BVH
     must be unchanged between the forward and the backward pass."""
     return _Extend.apply(V, bvh, phi, dx, band)
+
+
+def advance(mesh, phi, mask, mask_phi, dx, dt, nsub, band, m_i, m_F, m_O, cos_i, ion_yield,
+            mask_geom, chem):
+    """The rest of an etch step after the flux (threed.py:1719-1760 without the reinit): the
+    surface rate of the fluxes, its extension to the band |phi| < band and nsub upwind substeps
+    of dt / nsub, differentiable in the fluxes (m_i, m_F, m_O, the ion yields) and in phi
+    through the advection (the surface, its BVH and the nearest-face assignment stay fixed).
+    With radiosity_coverage or coupled_coverage in front of it, a loss on the advanced level set
+    reaches the ion terms:
+
+        bare, m_F, m_O = adjoint.coupled_coverage(mesh, geo, par, flags, m_i, cos_i, None, n, s)
+        phi1 = adjoint.advance(mesh, phi, mask, mask_phi, dx, dt, nsub, band, m_i, m_F, m_O,
+                               cos_i, None, mask_geom, chem)
+    """
+    V = surface_rate(m_i, m_F, m_O, cos_i, ion_yield, mesh.centroids, mask_geom, chem)
+    Fs = extend(mesh.bvh, V, phi.detach().contiguous(), dx, band)
+    return advect(phi, Fs, mask, mask_phi, dx, dt / nsub, nsub)

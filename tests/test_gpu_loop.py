@@ -210,3 +210,60 @@ def test_out_of_path_options_raise(kw, flag_kw):
     with pytest.raises(NotImplementedError):
         t3.run_etch_3d(**{**SMOKE, "n_steps": 1}, par=dict(PAR), flags=fl, n_ion=1000, n_neu=1000,
                        verbose=False, **{"reinit_method": "fsm", **kw})
+
+
+def test_differentiable_step_matches_finite_differences():
+    """A loss on the level set after a differentiable step on a fixed trench surface: the
+    radiosity coverage loop (two rounds), the surface rate, the extension and two advection
+    substeps (adjoint.radiosity_coverage + adjoint.advance). The whole forward is deterministic
+    here, so its gradient in m_i must match float64 central differences of the float32 loss
+    along random directions (5 %, for float32 rounding of phi)."""
+    import torch
+    from petch import adjoint, flux, ops
+    from petch.chemistry import chem_params
+    phi, geo = _geo("trench")
+    dx = geo['dx']
+    dev = _dev()
+    tphi = torch.from_numpy(phi).to(dev)
+    mesh = flux.SurfaceMesh(dev).build(tphi, dx)
+    gn = ops.gas_normals(tphi, dx, mesh.centroids, mesh.normals)
+    box = ops.make_box(geo['Lx'], geo['Ly'], geo['Lz'], geo['Lz'] - dx, 1)
+    n_ff = 64
+    hit = ops.trace_ff(mesh.bvh, box, mesh.centroids, gn, n_ff, 21)
+    row_ptr, col, D = ops.radiosity_assemble(hit, mesh.nf, n_ff)
+    par = dict(PAR)
+    fl = Flags(chemistry="belen", ion_reflection=True)
+    chem = chem_params(par, fl)
+    rng = np.random.default_rng(17)
+    m_i = torch.from_numpy(rng.uniform(0.2, 1.2, mesh.nf).astype(np.float32)).to(dev)
+    cos_i = torch.from_numpy(rng.uniform(0.3, 1.0, mesh.nf).astype(np.float32)).to(dev)
+    mask = torch.zeros_like(tphi, dtype=torch.uint8)
+    mg = ops.MaskGeom()
+    band = 3 * dx
+    w = torch.from_numpy(rng.normal(size=phi.shape)).to(dev)
+
+    def loss(x):
+        _, m_F, m_O = adjoint.radiosity_coverage(x, cos_i, None, D, hit, row_ptr, col, n_ff,
+                                                 chem, par.get('betaE', 0.7),
+                                                 par.get('betaO', 1.0), n_fp=2, tol=1e-10)
+        phi1 = adjoint.advance(mesh, tphi, mask, tphi, dx, dt, 2, band, x, m_F, m_O, cos_i,
+                               None, mg, chem)
+        return (phi1.double() * w).sum()
+
+    with torch.no_grad():
+        _, m_F0, m_O0 = adjoint.radiosity_coverage(m_i, cos_i, None, D, hit, row_ptr, col, n_ff,
+                                                   chem, 0.7, 1.0, n_fp=2)
+        V0, _ = ops.surface_rate(m_i, m_F0, m_O0, cos_i, None, mesh.centroids, mg, chem)
+    dt = 0.4 * dx / float(V0.max())
+    x = m_i.clone().requires_grad_(True)
+    loss(x).backward()
+    g = x.grad.double()
+    assert float(g.abs().max()) > 0
+    h = 3e-3
+    for k in range(3):
+        u = torch.from_numpy(rng.normal(size=mesh.nf).astype(np.float32)).to(dev)
+        with torch.no_grad():
+            fd = float(loss(m_i + h * u) - loss(m_i - h * u)) / (2 * h)
+        an = float((g * u.double()).sum())
+        print(f"direction {k}: fd {fd:.6e} adjoint {an:.6e}", flush=True)
+        assert abs(fd - an) <= 0.05 * abs(fd) + 1e-6 * float((g.abs() * u.abs()).sum())
