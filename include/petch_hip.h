@@ -342,13 +342,16 @@ int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float* m_F, const
 /* Closed-form adjoint of petch_surface_rate for gV = d loss / dV: gm_F, gm_O = d loss / d m_F,
  * d m_O; with ion yields (belen) g_sp, g_ie, g_ip = d loss / d iy_sp, iy_ie, iy_ip, else
  * gm_i = d loss / d m_i. Mask faces and non-finite V pass no gradient; cos_i is held constant.
- * gm_i, g_sp, g_ie, g_ip may be NULL. No reference counterpart. */
+ * gY (3 x F: Yie, Ysp, Yp rows) receives each face's share of d loss / d the chemistry's yield
+ * coefficients (analytic-yield path; zero with ion yields). gm_i, g_sp, g_ie, g_ip and gY may
+ * be NULL. No reference counterpart. */
 int petch_surface_rate_adjoint(petch_ctx* ctx, const float* m_i, const float* m_F,
                                const float* m_O, const float* cos_i, const float* iy_sp,
                                const float* iy_ie, const float* iy_ip, const float* centroids,
                                const uint8_t* is_mask, const petch_mask_geom* mg,
                                const petch_chem_params* cp, int F, const float* gV, float* gm_F,
-                               float* gm_O, float* gm_i, float* g_sp, float* g_ie, float* g_ip);
+                               float* gm_O, float* gm_i, float* g_sp, float* g_ie, float* g_ip,
+                               float* gY);
 
 /* ---------------------------------------------------------------- a15, a17 --------------- */
 /* extend_velocity_gpu + _extend_kernel (threed.py:1324-1346): Fs = V[nearest face] at nodes

@@ -295,7 +295,10 @@ __global__ void k_surface_rate(const float* __restrict__ m_i, const float* __res
 
 // Closed-form adjoint of k_surface_rate for gV = d loss / dV (zero on mask faces and where V
 // was not finite): gm_F, gm_O, and d / d m_i (gm_i) or, with ion yields, d / d iy_sp, iy_ie,
-// iy_ip (g_sp, g_ie, g_ip); cos_i is held constant. Optional outputs may be NULL.
+// iy_ip (g_sp, g_ie, g_ip); cos_i is held constant. gY (3 x F) receives each face's share of
+// d loss / d (Yie, Ysp, Yp), the yield coefficients of the analytic-yield path (zero with ion
+// yields), whose face sums give the gradient in the recipe (ion energy, yield prefactors).
+// Optional outputs may be NULL.
 //   belen:    V = K (k_sigma thF / 4 + sputter + thF ion_enh), thF = 1 / (1 + a (1 + 1/(b+eps)))
 //   langmuir: V = K (Yie_a Fi thF + Ysp_a Fi br), br = 1 / (1 + rF + rO), thF = rF br
 // with K = rate_scale / rho.
@@ -306,10 +309,11 @@ __global__ void k_surface_rate_adjoint(
     const float* __restrict__ centroids, const uint8_t* __restrict__ is_mask, MaskGeom mg, Chem c,
     int F, const float* __restrict__ gV, float* __restrict__ gm_F, float* __restrict__ gm_O,
     float* __restrict__ gm_i, float* __restrict__ g_sp, float* __restrict__ g_ie,
-    float* __restrict__ g_ip) {
+    float* __restrict__ g_ip, float* __restrict__ gY) {
   int f = blockIdx.x * kBS + threadIdx.x;
   if (f >= F) return;
   const double eps = 1e-9;
+  double yie = 0.0, ysp = 0.0, yp = 0.0;  // dV / dYie, dYsp, dYp (analytic-yield path)
   double f_ie, f_sp;
   angular((double)cos_i[f], c, f_ie, f_sp);
   const double Fi = c.ionFlux * (double)m_i[f];
@@ -348,6 +352,9 @@ __global__ void k_surface_rate_adjoint(
       dip = dGp * c.ionFlux;
     } else {
       dI = (dGie * Yie_a + dGp * Yp_a + K * Ysp_a + K * thF * Yie_a) * c.ionFlux;
+      yie = (dGie + K * thF) * f_ie * Fi;
+      ysp = K * f_sp * Fi;
+      yp = dGp * f_ie * Fi;
     }
   } else {
     const double Fev = c.Fflux * (double)m_F[f], Fp = c.Oflux * (double)m_O[f];
@@ -365,6 +372,9 @@ __global__ void k_surface_rate_adjoint(
     const double dFi = K * (Yie_a * thF + Ysp_a * br) + vF * (-rF * Yie_a / dE) +
                        vO * (-rO * Yp_a / dP);
     dI = dFi * c.ionFlux;
+    yie = K * f_ie * Fi * thF + vF * (-rF * f_ie * Fi / dE);
+    ysp = K * f_sp * Fi * br;
+    yp = vO * (-rO * f_ie * Fi / dP);
   }
   double g = (double)gV[f];
   if (face_masked(is_mask, centroids, mg, f) || !(fabs((double)(float)v) <= 3.402823466e38))
@@ -377,6 +387,11 @@ __global__ void k_surface_rate_adjoint(
     if (g_ip) g_ip[f] = (float)(g * dip);
   } else if (gm_i) {
     gm_i[f] = (float)(g * dI);
+  }
+  if (gY) {
+    gY[f] = (float)(g * yie);
+    gY[F + f] = (float)(g * ysp);
+    gY[2 * F + f] = (float)(g * yp);
   }
 }
 
@@ -627,7 +642,7 @@ extern "C" int petch_surface_rate_adjoint(
     const float* iy_sp, const float* iy_ie, const float* iy_ip, const float* centroids,
     const uint8_t* is_mask, const petch_mask_geom* mg, const petch_chem_params* cp, int F,
     const float* gV, float* gm_F, float* gm_O, float* gm_i, float* g_sp, float* g_ie,
-    float* g_ip) {
+    float* g_ip, float* gY) {
   PETCH_ARG(ctx, ctx && m_i && m_F && m_O && cos_i && mg && cp && gV && gm_F && gm_O,
             "surface_rate_adjoint: null pointer");
   PETCH_ARG(ctx, centroids || is_mask, "surface_rate_adjoint: need centroids or an explicit mask");
@@ -639,7 +654,7 @@ extern "C" int petch_surface_rate_adjoint(
              mg->hole};
   hipLaunchKernelGGL(k_surface_rate_adjoint, dim3(petch_blocks(F, kBS)), dim3(kBS), 0,
                      ctx->stream, m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask, g,
-                     make_chem(cp), F, gV, gm_F, gm_O, gm_i, g_sp, g_ie, g_ip);
+                     make_chem(cp), F, gV, gm_F, gm_O, gm_i, g_sp, g_ie, g_ip, gY);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

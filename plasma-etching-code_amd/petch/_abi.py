@@ -159,7 +159,7 @@ SIGNATURES = {
     "petch_extend": [P, P, P, P, c_int, c_int, c_int, c_double, c_double, P],
     "petch_extend_adjoint": [P, P, P, c_int, c_int, c_int, c_double, c_double, P, P],
     "petch_surface_rate_adjoint": [P, P, P, P, P, P, P, P, P, P, POINTER(MaskGeom),
-                                   POINTER(ChemParams), c_int, P, P, P, P, P, P, P],
+                                   POINTER(ChemParams), c_int, P, P, P, P, P, P, P, P],
     "petch_nearest_face_value": [P, P, P, P, c_int, c_float, P],
     "petch_extend_batch": [P, c_int, P],
     "petch_advect_step": [P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, P],

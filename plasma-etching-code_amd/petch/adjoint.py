@@ -222,31 +222,40 @@ def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, 
 class _SurfaceRate(torch.autograd.Function):
     @staticmethod
     def forward(ctx, m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, mask_geom, chem,
-                is_mask):
+                is_mask, Y):
+        if Y is not None:
+            chem = type(chem).from_buffer_copy(chem)
+            chem.Yie, chem.Ysp, chem.Yp = (float(y) for y in Y.detach().cpu().tolist())
         iy = (iy_sp, iy_ie, iy_ip) if iy_ie is not None else None
         V, red = ops.surface_rate(m_i.contiguous(), m_F.contiguous(), m_O.contiguous(), cos_i,
                                   iy, centroids, mask_geom, chem, is_mask=is_mask)
         ctx.save_for_backward(m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask)
-        ctx.args = (mask_geom, chem)
+        ctx.args = (mask_geom, chem, Y is not None)
         return V
 
     @staticmethod
     def backward(ctx, gV):
         m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask = ctx.saved_tensors
-        mask_geom, chem = ctx.args
+        mask_geom, chem, want_y = ctx.args
         iy = (iy_sp, iy_ie, iy_ip) if iy_ie is not None else None
-        gm_F, gm_O, gm_i, g_sp, g_ie, g_ip = ops.surface_rate_adjoint(
+        out = ops.surface_rate_adjoint(
             m_i.contiguous(), m_F.contiguous(), m_O.contiguous(), cos_i, iy, centroids,
-            mask_geom, chem, gV.contiguous().float(), is_mask=is_mask)
-        return gm_i, gm_F, gm_O, None, g_sp, g_ie, g_ip, None, None, None, None
+            mask_geom, chem, gV.contiguous().float(), is_mask=is_mask, per_face_yields=want_y)
+        gm_F, gm_O, gm_i, g_sp, g_ie, g_ip = out[:6]
+        gY = out[6].double().sum(dim=1) if want_y else None
+        return gm_i, gm_F, gm_O, None, g_sp, g_ie, g_ip, None, None, None, None, gY
 
 
-def surface_rate(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, is_mask=None):
+def surface_rate(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, is_mask=None,
+                 Y=None):
     """V (ops.surface_rate), differentiable in m_F, m_O and the ion terms (m_i, or the ion
-    yields when given)."""
+    yields when given). Y, a (3,) tensor, replaces chem's yield coefficients (Yie, Ysp, Yp) and
+    is differentiable too: with Y computed from the ion energy in torch (e.g.
+    A * relu(sqrt(E) - sqrt(Eth)) for the 'mean' IED, chemistry.py:10-39), a loss on the rate
+    reaches the recipe's energy, as in scripts/inverse_design.py:20-45."""
     sp, ie, ip = ion_yield if ion_yield is not None else (None, None, None)
     return _SurfaceRate.apply(m_i, m_F, m_O, cos_i, sp, ie, ip, centroids, mask_geom, chem,
-                              is_mask)
+                              is_mask, Y)
 
 
 class _Extend(torch.autograd.Function):

@@ -494,9 +494,10 @@ def belen_coverage_adjoint(m_i, m_F, m_O, cos_i, ion_yield, chem, gbare):
 
 
 def surface_rate_adjoint(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, gV,
-                         is_mask=None):
+                         is_mask=None, per_face_yields=False):
     """Adjoint of surface_rate (petch_surface_rate_adjoint): (gm_F, gm_O, gm_i, g_sp, g_ie,
-    g_ip) for gV = d loss / dV; gm_i is None with ion yields, the g_* are None without."""
+    g_ip) for gV = d loss / dV; gm_i is None with ion yields, the g_* are None without. With
+    per_face_yields, a seventh value: (3, F) per-face shares of d loss / d (Yie, Ysp, Yp)."""
     F = int(m_F.shape[0])
     for t, name in ((m_i, "m_i"), (m_O, "m_O"), (cos_i, "cos_i"), (gV, "gV")):
         if t.numel() < F:
@@ -508,11 +509,14 @@ def surface_rate_adjoint(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, 
         g_sp, g_ie, g_ip = (torch.empty_like(m_F) for _ in range(3))
     else:
         gm_i = torch.empty_like(m_F)
+    gY = torch.zeros((3, F), dtype=f32, device=m_F.device) if per_face_yields else None
     if F:
         _ctx(m_F).call("petch_surface_rate_adjoint", ptr(m_i, f32), ptr(m_F, f32), ptr(m_O, f32),
                        ptr(cos_i, f32), ptr(sp), ptr(ie), ptr(ip), ptr(centroids), ptr(is_mask),
                        ctypes_ref(mask_geom), ctypes_ref(chem), F, ptr(gV, f32, "gV"), ptr(gm_F),
-                       ptr(gm_O), ptr(gm_i), ptr(g_sp), ptr(g_ie), ptr(g_ip))
+                       ptr(gm_O), ptr(gm_i), ptr(g_sp), ptr(g_ie), ptr(g_ip), ptr(gY))
+    if per_face_yields:
+        return gm_F, gm_O, gm_i, g_sp, g_ie, g_ip, gY
     return gm_F, gm_O, gm_i, g_sp, g_ie, g_ip
 
 

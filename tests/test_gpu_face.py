@@ -441,3 +441,46 @@ def test_extend_adjoint_is_the_transpose():
     x = _t(V).requires_grad_(True)
     (adjoint.extend(bvh, x, _t(phi), dx, band) * _t(gFs)).sum().backward()
     np.testing.assert_allclose(x.grad.cpu().numpy(), gV, rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.parametrize("chem", ["belen", "langmuir"])
+def test_rate_gradient_in_ion_energy(chem):
+    """The recipe gradient of scripts/inverse_design.py:20-45 on this path: the yields from the
+    ion energy E ('mean' IED, Y = A max(sqrt(E) - sqrt(Eth), 0)) in torch, the rate on the GPU,
+    d loss / dE through the rate adjoint's yield-coefficient output, against float64 central
+    differences of the oracle's rate in par['Emean']."""
+    import torch
+    from petch import adjoint, ops
+    from petch.chemistry import chem_params
+    rng = np.random.default_rng(41)
+    F = 3000
+    par = dict(PAR)
+    par['ied_mode'] = 'mean'
+    par['Emean'] = 120.0
+    fl = Flags(chemistry=chem)
+    m_i = rng.uniform(0.05, 1.3, F).astype(np.float32)
+    cos_i = rng.uniform(0.0, 1.0, F).astype(np.float32)
+    m_F = rng.uniform(0.01, 1.5, F).astype(np.float32)
+    m_O = rng.uniform(0.01, 1.5, F).astype(np.float32)
+    msk = rng.random(F) < 0.1
+    w = rng.normal(size=F)
+    dev = _dev()
+    E = torch.tensor(par['Emean'], dtype=torch.float64, requires_grad=True)
+    A = torch.tensor([par['A_ie'], par['A_sp'], par['A_p']], dtype=torch.float64)
+    Eth = torch.tensor([par['Eth_ie'], par['Eth_sp'], par['Eth_p']], dtype=torch.float64)
+    Y = A * torch.relu(torch.sqrt(E) - torch.sqrt(Eth))
+    V = adjoint.surface_rate(_t(m_i), _t(m_F), _t(m_O), _t(cos_i), None, None, ops.MaskGeom(),
+                             chem_params(par, fl), is_mask=_t(msk.astype(np.uint8)), Y=Y)
+    (V.double() * torch.from_numpy(w).to(dev)).sum().backward()
+
+    def loss(Em):
+        p = dict(par)
+        p['Emean'] = Em
+        return float(w @ oc.surface_rate(m_i.astype(np.float64), m_F.astype(np.float64),
+                                         m_O.astype(np.float64), cos_i.astype(np.float64), msk,
+                                         p, fl))
+
+    h = 1e-3
+    fd = (loss(par['Emean'] + h) - loss(par['Emean'] - h)) / (2 * h)
+    print(f"{chem}: d loss / dE fd {fd:.6e} adjoint {float(E.grad):.6e}", flush=True)
+    assert abs(float(E.grad) - fd) <= 1e-4 * abs(fd)
