@@ -230,7 +230,7 @@ class _SurfaceRate(torch.autograd.Function):
         V, red = ops.surface_rate(m_i.contiguous(), m_F.contiguous(), m_O.contiguous(), cos_i,
                                   iy, centroids, mask_geom, chem, is_mask=is_mask)
         ctx.save_for_backward(m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask)
-        ctx.args = (mask_geom, chem, Y is not None)
+        ctx.args = (mask_geom, chem, (Y.device, Y.dtype) if Y is not None else None)
         return V
 
     @staticmethod
@@ -240,9 +240,10 @@ class _SurfaceRate(torch.autograd.Function):
         iy = (iy_sp, iy_ie, iy_ip) if iy_ie is not None else None
         out = ops.surface_rate_adjoint(
             m_i.contiguous(), m_F.contiguous(), m_O.contiguous(), cos_i, iy, centroids,
-            mask_geom, chem, gV.contiguous().float(), is_mask=is_mask, per_face_yields=want_y)
+            mask_geom, chem, gV.contiguous().float(), is_mask=is_mask,
+            per_face_yields=want_y is not None)
         gm_F, gm_O, gm_i, g_sp, g_ie, g_ip = out[:6]
-        gY = out[6].double().sum(dim=1) if want_y else None
+        gY = out[6].double().sum(dim=1).to(device=want_y[0], dtype=want_y[1]) if want_y else None
         return gm_i, gm_F, gm_O, None, g_sp, g_ie, g_ip, None, None, None, None, gY
 
 
