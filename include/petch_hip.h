@@ -233,6 +233,13 @@ int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const petch_box*
                           const petch_source* src, const petch_ion_params* ip, int n,
                           uint64_t seed, float* sp, float* ie, float* ipass, float* arr,
                           float* ang);
+/* Forward-mode derivative of petch_trace_ion_yield (same ip, n, seed, so the same rays and
+ * decisions) in the mean ion energy ip->meanE: dsp, die, dip (nf each, zeroed first) =
+ * d (sp, ie, ipass) / d meanE per face. Every energy step scales an ion's energy by a factor
+ * independent of it, so dE/dmeanE = E / E0 along a path. No reference counterpart. */
+int petch_trace_ion_yield_tangent(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                                  const petch_source* src, const petch_ion_params* ip, int n,
+                                  uint64_t seed, float* dsp, float* die, float* dip);
 /* _trace3d (threed.py:352-406): specular=0 sticking/diffuse neutral, specular=1 ion with
  * grazing specular reflection (cos_thr, eta); <= n_reemit re-emissions; no periodic wrap. */
 int petch_trace_legacy(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,

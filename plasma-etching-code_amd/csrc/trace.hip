@@ -537,7 +537,7 @@ struct IonParams {
 template <class Fc>
 __device__ __forceinline__ bool ion_bounce(Rng& r, float& Ein, float& win, int& bounce,
                                            const BvhView& bv, const Hit& h, const Fc& F, Vec3& o,
-                                           Vec3& d) {
+                                           Vec3& d, float E0 = 0.f) {
   const IonParams& P = F.P;
   Vec3 nn = hit_normal(bv, h.slot);
   if (petch::dot(nn, d) > 0.f) nn = -nn;
@@ -548,11 +548,22 @@ __device__ __forceinline__ bool ion_bounce(Rng& r, float& Ein, float& win, int& 
   const float f_sp = fmaxf((1.f + P.B_sp * (1.f - cosT * cosT)) * cosT, 0.f);
   float f_ie = 1.f;
   if (cosT < 0.5f) f_ie = fmaxf(3.f - 6.f * incAng / kPi, 0.f);
-  deposit(F.out[0], h.face, P.A_sp * fmaxf(sqrtE - F.sE[0], 0.f) * f_sp * w);
-  deposit(F.out[1], h.face, P.A_ie * fmaxf(sqrtE - F.sE[1], 0.f) * f_ie * w);
-  deposit(F.out[2], h.face, P.A_p * fmaxf(sqrtE - F.sE[2], 0.f) * f_ie * w);
-  deposit(F.out[3], h.face, w);
-  deposit(F.out[4], h.face, cosT * w);
+  if constexpr (Fc::kTangent) {
+    // d / d meanE of the three yield deposits: every energy step below scales E by a factor
+    // that does not depend on E (its acceptance test compares the factor with [0, 1]), so
+    // dE / dE0 = E / E0 along the path, and dE0 / dmeanE = 1 (0 where E0 was clamped to 0,
+    // where sqrtE = 0 deposits nothing).
+    const float s = w * sqrtE / (2.f * E0);
+    deposit(F.out[0], h.face, sqrtE > F.sE[0] ? P.A_sp * f_sp * s : 0.f);
+    deposit(F.out[1], h.face, sqrtE > F.sE[1] ? P.A_ie * f_ie * s : 0.f);
+    deposit(F.out[2], h.face, sqrtE > F.sE[2] ? P.A_p * f_ie * s : 0.f);
+  } else {
+    deposit(F.out[0], h.face, P.A_sp * fmaxf(sqrtE - F.sE[0], 0.f) * f_sp * w);
+    deposit(F.out[1], h.face, P.A_ie * fmaxf(sqrtE - F.sE[1], 0.f) * f_ie * w);
+    deposit(F.out[2], h.face, P.A_p * fmaxf(sqrtE - F.sE[2], 0.f) * f_ie * w);
+    deposit(F.out[3], h.face, w);
+    deposit(F.out[4], h.face, cosT * w);
+  }
   float stick = 1.f;
   if (incAng > P.thrmin) {
     float rr = (incAng - P.thrmin) / (P.thrmax - P.thrmin);
@@ -610,6 +621,7 @@ __device__ __forceinline__ bool ion_bounce(Rng& r, float& Ein, float& win, int& 
 
 struct IonYield {
   static constexpr bool kReplay = false;
+  static constexpr bool kTangent = false;
   BvhView bv;
   Box b;
   Source src;
@@ -640,6 +652,44 @@ struct IonYield {
   }
 };
 
+// Forward-mode derivative of the faithful-ion trace in the mean ion energy (no reference
+// counterpart): the same rays, energies, decisions and paths as IonYield with the same seed;
+// out[0..2] receive d (sp, ie, ip) / d meanE per face. The arrival weight and the cosine
+// channel do not depend on the energy.
+struct IonYieldTangent {
+  static constexpr bool kReplay = false;
+  static constexpr bool kTangent = true;
+  BvhView bv;
+  Box b;
+  Source src;
+  IonParams P;
+  float sE[3];
+  float* out[3];
+  uint64_t seed;
+  struct Ray {
+    float E, w, E0;
+    int bounce;
+    Rng rng;
+  };
+  __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
+  __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
+  __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
+    R.rng.init(seed, ray);
+    gen_source(R.rng, b, src, o, e.n);
+    e.mode = kDirect;
+    float E = P.meanE + P.sigmaE * R.rng.normal();
+    R.E = E < 0.f ? 0.f : E;
+    R.E0 = R.E;
+    R.w = 1.f;
+    R.bounce = 0;
+  }
+  __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
+    e.mode = kDirect;
+    e.n = d;
+    return ion_bounce(R.rng, R.E, R.w, R.bounce, bv, h, *this, o, e.n, R.E0);
+  }
+};
+
 // ---------------------------------------------------------------- batches of features
 // A batched launch traces many features (each its own surface, box, coverage and recipe) from
 // one ray queue: feature f owns queue ids [end[f-1], end[f]). A lane keeps its ray's feature
@@ -657,6 +707,7 @@ struct NeutralFeature {
 };
 
 struct IonFeature {
+  static constexpr bool kTangent = false;
   BvhView bv;
   Box b;
   Source src;
@@ -780,6 +831,11 @@ k_trace_neutral_rr_adjoint(NeutralRRAdjoint pol, RayQueue q, unsigned long long*
 
 __global__ void __launch_bounds__(kBS)
 k_trace_ion_yield(IonYield pol, RayQueue q, unsigned long long* __restrict__ hits) {
+  persistent_trace(pol, q, hits);
+}
+
+__global__ void __launch_bounds__(kBS)
+k_trace_ion_yield_tangent(IonYieldTangent pol, RayQueue q, unsigned long long* __restrict__ hits) {
   persistent_trace(pol, q, hits);
 }
 
@@ -1000,6 +1056,34 @@ extern "C" int petch_trace_neutral_rr_pair(petch_ctx* ctx, const petch_bvh* bvh,
   NeutralRR pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, {beta_a, beta_b},
                 {fl_a, fl_b}, {seed_a, seed_b}, (unsigned)n_a};
   return launch_neutral(ctx, pol, n);
+}
+
+extern "C" int petch_trace_ion_yield_tangent(petch_ctx* ctx, const petch_bvh* bvh,
+                                             const petch_box* box, const petch_source* src,
+                                             const petch_ion_params* ip, int n, uint64_t seed,
+                                             float* dsp, float* die, float* dip) {
+  PETCH_ARG(ctx, ctx && bvh && box && src && ip && dsp && die && dip,
+            "trace_ion_yield_tangent: null pointer");
+  PETCH_ARG(ctx, n >= 0, "trace_ion_yield_tangent: n < 0");
+  float* outs[3] = {dsp, die, dip};
+  for (int k = 0; k < 3; ++k) {
+    int rc = zero(ctx, outs[k], bvh->nf);
+    if (rc) return rc;
+  }
+  if (n == 0 || bvh->nf == 0) return PETCH_OK;
+  IonParams P{ip->meanE,  ip->sigmaE, ip->A_sp,    ip->Eth_sp, ip->B_sp, ip->A_ie,
+              ip->Eth_ie, ip->A_p,    ip->Eth_p,   ip->inflect, ip->minang, ip->n_l,
+              ip->A_en,   ip->minE,   ip->thrmin,  ip->thrmax};
+  RayQueue q;
+  int rc2 = make_queue(ctx, n, q);
+  if (rc2) return rc2;
+  IonYieldTangent pol{petch_bvh_view(bvh), make_box(box), make_source(src), P,
+                      {sqrtf(P.Eth_sp), sqrtf(P.Eth_ie), sqrtf(P.Eth_p)}, {dsp, die, dip}, seed};
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_ion_yield_tangent, n);
+  hipLaunchKernelGGL(k_trace_ion_yield_tangent, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
+                     (unsigned long long*)nullptr);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
 }
 
 extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,

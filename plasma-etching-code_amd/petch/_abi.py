@@ -142,6 +142,8 @@ SIGNATURES = {
     "petch_trace_ion_yield_batch": [P, c_int, P],
     "petch_trace_ion_yield": [P, P, POINTER(Box), POINTER(Source), POINTER(IonParams), c_int,
                               c_uint64, P, P, P, P, P],
+    "petch_trace_ion_yield_tangent": [P, P, POINTER(Box), POINTER(Source), POINTER(IonParams),
+                                      c_int, c_uint64, P, P, P],
     "petch_trace_legacy": [P, P, POINTER(Box), POINTER(Source), c_float, c_int, c_int, c_float,
                            c_float, c_int, c_uint64, P, P],
     "petch_trace_ff": [P, P, POINTER(Box), P, P, c_int, c_int, c_uint64, P],

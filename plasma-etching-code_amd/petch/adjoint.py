@@ -297,3 +297,28 @@ def advance(mesh, phi, mask, mask_phi, dx, dt, nsub, band, m_i, m_F, m_O, cos_i,
     V = surface_rate(m_i, m_F, m_O, cos_i, ion_yield, mesh.centroids, mask_geom, chem)
     Fs = extend(mesh.bvh, V, phi.detach().contiguous(), dx, band)
     return advect(phi, Fs, mask, mask_phi, dx, dt / nsub, nsub)
+
+
+class _IonYield(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, E, bvh, box, src, ion, n, seed):
+        ion = type(ion).from_buffer_copy(ion)
+        ion.meanE = float(E.detach().cpu())
+        acc = ops.trace_ion_yield(bvh, box, src, ion, n, seed)
+        ctx.args = (bvh, box, src, ion, int(n), int(seed), E.device, E.dtype)
+        return acc
+
+    @staticmethod
+    def backward(ctx, gacc):
+        bvh, box, src, ion, n, seed, dev, dt = ctx.args
+        t = ops.trace_ion_yield_tangent(bvh, box, src, ion, n, seed)
+        gE = (gacc[:3].double() * t.double()).sum()
+        return gE.to(device=dev, dtype=dt), None, None, None, None, None, None
+
+
+def ion_yield(bvh, box, src, ion, n, seed, E):
+    """The faithful-ion accumulators (5, F) of ops.trace_ion_yield with the mean ion energy
+    taken from the 0-d tensor E, differentiable in E: the backward pass traces the same ions
+    once more in forward mode (petch_trace_ion_yield_tangent), since E is one scalar. The BVH
+    must be unchanged between the forward and the backward pass."""
+    return _IonYield.apply(E, bvh, box, src, ion, n, seed)

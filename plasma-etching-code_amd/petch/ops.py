@@ -375,6 +375,17 @@ def trace_ion_yield(bvh, box, src, ion, n, seed, out=None):
     return acc
 
 
+def trace_ion_yield_tangent(bvh, box, src, ion, n, seed):
+    """d (sp, ie, ip) / d ion.meanE (3, F) of trace_ion_yield with the same arguments
+    (petch_trace_ion_yield_tangent: forward mode along the same paths)."""
+    import ctypes
+    t = torch.empty((3, bvh.nf), dtype=f32, device=bvh.device)
+    bvh.ctx.call("petch_trace_ion_yield_tangent", bvh.h, ctypes.byref(box), ctypes.byref(src),
+                 ctypes.byref(ion), int(n), int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(t[0]), ptr(t[1]),
+                 ptr(t[2]))
+    return t
+
+
 def trace_legacy(bvh, box, src, sticking, n_reemit, specular, cos_thr, eta, n, seed):
     """_trace3d accumulators (2, F): flux, weighted cos (threed.py:352-406)."""
     import ctypes
