@@ -164,29 +164,30 @@ def neutral_rr(bvh, box, src, bare, beta, n, seed):
 
 class _Finish(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, fl, areas, nbr, normals, scale, hi, n_smooth, alpha, blend_each):
-        m = ops.normalize(fl.detach().contiguous(), areas, scale, 0.0, hi)
+    def forward(ctx, fl, areas, nbr, normals, scale, lo, hi, n_smooth, alpha, blend_each):
+        m = ops.normalize(fl.detach().contiguous(), areas, scale, lo, hi)
         if n_smooth > 0:
             m = ops.smooth(m, nbr, normals, n_smooth, alpha, blend_each)
         ctx.save_for_backward(fl.detach().contiguous(), areas, nbr, normals)
-        ctx.args = (float(scale), float(hi), int(n_smooth), float(alpha), bool(blend_each))
+        ctx.args = (float(scale), float(lo), float(hi), int(n_smooth), float(alpha),
+                    bool(blend_each))
         return m
 
     @staticmethod
     def backward(ctx, gm):
         fl, areas, nbr, normals = ctx.saved_tensors
-        scale, hi, n_smooth, alpha, blend_each = ctx.args
+        scale, lo, hi, n_smooth, alpha, blend_each = ctx.args
         g = gm.contiguous().float()
         if n_smooth > 0:
             g = ops.smooth_adjoint(nbr, normals, n_smooth, alpha, blend_each, g)
-        return (ops.normalize_adjoint(fl, areas, scale, 0.0, hi, g), None, None, None, None, None,
-                None, None, None)
+        return (ops.normalize_adjoint(fl, areas, scale, lo, hi, g), None, None, None, None, None,
+                None, None, None, None)
 
 
-def finish(fl, mesh, scale, hi, n_smooth, alpha, blend_each):
-    """clamp((fl / A) * scale, 0, hi) then the flux smoothing, as the coupled flux's finish step,
-    differentiable in fl."""
-    return _Finish.apply(fl, mesh.areas, mesh.nbr, mesh.normals, scale, hi, n_smooth, alpha,
+def finish(fl, mesh, scale, hi, n_smooth, alpha, blend_each, lo=0.0):
+    """clamp((fl / A) * scale, lo, hi) then the flux smoothing, as the coupled flux's finish
+    step (lo = -inf, hi = inf and one final blend for the ion yields), differentiable in fl."""
+    return _Finish.apply(fl, mesh.areas, mesh.nbr, mesh.normals, scale, lo, hi, n_smooth, alpha,
                          blend_each)
 
 
@@ -322,3 +323,25 @@ def ion_yield(bvh, box, src, ion, n, seed, E):
     once more in forward mode (petch_trace_ion_yield_tangent), since E is one scalar. The BVH
     must be unchanged between the forward and the backward pass."""
     return _IonYield.apply(E, bvh, box, src, ion, n, seed)
+
+
+def ion_terms(mesh, geo, par, n_ion, seed, E):
+    """(m_i, cos_i, ion_yield) of the faithful-ion part of flux.coupled_flux_steps
+    (threed.py:806-823) with the mean ion energy from the 0-d tensor E, differentiable in E: the
+    ion trace by its forward-mode derivative, the normalisation and smoothing by their
+    transposes. cos_i does not depend on the energy."""
+    from .flux import SEED_MASK, box_of, ion_params, ion_source
+    A_src = geo['Lx'] * geo['Ly']
+    n_smooth = int(par.get('flux_smooth', 1))
+    alpha = float(par.get('flux_smooth_alpha', 1.0))
+    acc = ion_yield(mesh.bvh, box_of(geo, par), ion_source(par), ion_params(par), n_ion,
+                    (seed * 9 + 1) & SEED_MASK, E)
+    norm = A_src / n_ion
+
+    def fin(x):
+        return finish(x, mesh, norm, ops.INF, n_smooth, alpha, False, lo=-ops.INF)
+
+    iy = tuple(fin(acc[k]) for k in range(3))
+    m_i = fin(acc[3])
+    cos_i = ops.cos_ratio(acc[3].detach().contiguous(), acc[4].detach().contiguous())
+    return m_i, cos_i, iy

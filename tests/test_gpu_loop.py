@@ -267,3 +267,56 @@ def test_differentiable_step_matches_finite_differences():
         an = float((g * u.double()).sum())
         print(f"direction {k}: fd {fd:.6e} adjoint {an:.6e}", flush=True)
         assert abs(fd - an) <= 0.05 * abs(fd) + 1e-6 * float((g.abs() * u.abs()).sum())
+
+
+def test_ion_energy_gradient_through_a_step():
+    """d loss / d (mean ion energy) through a whole step on a fixed trench surface with the
+    deterministic radiosity transport: the faithful ion trace (forward mode in the energy),
+    its normalisation and smoothing, the radiosity coverage loop with the ion yields, the rate,
+    the extension and two advection substeps. The ions follow the same paths at E +- h, so the
+    forward is smooth in E and float64 central differences must match within 5 %."""
+    import torch
+    from petch import adjoint, flux, ops
+    from petch.chemistry import chem_params
+    phi, geo = _geo("trench")
+    dx = geo['dx']
+    dev = _dev()
+    tphi = torch.from_numpy(phi).to(dev)
+    mesh = flux.SurfaceMesh(dev).build(tphi, dx)
+    par = dict(PAR)
+    par['periodic_y'] = 1
+    fl = Flags(chemistry="belen", ion_reflection=True)
+    gn = ops.gas_normals(tphi, dx, mesh.centroids, mesh.normals)
+    n_ff = 64
+    hit = ops.trace_ff(mesh.bvh, flux.box_of(geo, par), mesh.centroids, gn, n_ff, 21)
+    row_ptr, col, D = ops.radiosity_assemble(hit, mesh.nf, n_ff)
+    chem = chem_params(par, fl)
+    mask = torch.zeros_like(tphi, dtype=torch.uint8)
+    mg = ops.MaskGeom()
+    band = 3 * dx
+    w = torch.from_numpy(np.random.default_rng(23).normal(size=phi.shape)).to(dev)
+    n_ion = 30000
+
+    def loss(E):
+        m_i, cos_i, iy = adjoint.ion_terms(mesh, geo, par, n_ion, 5, E)
+        _, m_F, m_O = adjoint.radiosity_coverage(m_i, cos_i, iy, D, hit, row_ptr, col, n_ff, chem,
+                                                 0.7, 1.0, n_fp=2, tol=1e-10)
+        phi1 = adjoint.advance(mesh, tphi, mask, tphi, dx, dt, 2, band, m_i, m_F, m_O, cos_i, iy,
+                               mg, chem)
+        return (phi1.double() * w).sum()
+
+    E0 = float(par['Emean'])
+    with torch.no_grad():
+        m_i, cos_i, iy = adjoint.ion_terms(mesh, geo, par, n_ion, 5, torch.tensor(E0))
+        _, m_F, m_O = adjoint.radiosity_coverage(m_i, cos_i, iy, D, hit, row_ptr, col, n_ff, chem,
+                                                 0.7, 1.0, n_fp=2)
+        V0, _ = ops.surface_rate(m_i, m_F, m_O, cos_i, iy, mesh.centroids, mg, chem)
+    dt = 0.4 * dx / float(V0.max())
+    E = torch.tensor(E0, dtype=torch.float64, requires_grad=True)
+    loss(E).backward()
+    h = 0.5
+    with torch.no_grad():
+        fd = float(loss(torch.tensor(E0 + h)) - loss(torch.tensor(E0 - h))) / (2 * h)
+    print(f"d loss / dE: fd {fd:.6e} adjoint {float(E.grad):.6e}", flush=True)
+    assert abs(fd) > 0
+    assert abs(float(E.grad) - fd) <= 0.05 * abs(fd)
