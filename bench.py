@@ -366,6 +366,68 @@ def neutral_adjoint_cost(reps=3, beta=0.7):
             **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
 
 
+def step_adjoint_cost(reps=2, nsub=3):
+    """Cost of one differentiable etch step on the trench surface of the workload, from the mean
+    ion energy to a loss on the advanced level set: the faithful ion trace (8M ions), N_FP rounds
+    of F and O RR traces (8M rays each) with finish and Belen updates, the rate, the extension
+    and nsub advection substeps. fwd = the loss; fwd_adj = the loss and d loss / dE (forward-mode
+    ion trace, path-replay RR adjoints, the closed-form and transposed stages). The geometry (MC,
+    BVH) is built once outside the timing and the reinit is not part of it."""
+    import torch
+    from petch import adjoint, ops
+    from petch.chemistry import chem_params
+    from petch.flux import SurfaceMesh
+    from petch.params import Flags
+    from petch.threed import grid_shape
+    dev = torch.device("cuda", 0)
+    nx, ny, nz = grid_shape(W['Lx'], W['Ly'], W['Lz'], W['dx'])
+    phi, _ = ops.init_sdf(nx, ny, nz, W['dx'], W['Lx'], W['Ly'], W['width'],
+                          W['sub_top'] + W['mask_th'] - W['floor'], W['floor'], False, device=dev)
+    _, mask = ops.init_sdf(nx, ny, nz, W['dx'], W['Lx'], W['Ly'], W['width'], W['mask_th'],
+                           W['sub_top'], False, device=dev)
+    mesh = SurfaceMesh(dev, rays_per_step=N_NEU * 2 * N_FP).build(phi, W['dx'])
+    geo = dict(Lx=W['Lx'], Ly=W['Ly'], Lz=W['Lz'], dx=W['dx'])
+    par, fl = workload_par()
+    flags = Flags(**fl)
+    chem = chem_params(par, flags)
+    mg = ops.MaskGeom(float(W['sub_top']), float(W['mask_th']), float(W['Lx']), float(W['Ly']),
+                      float(W['width']), 0)
+    iy_rate = getattr(flags, "chemistry", "") == "belen"
+    w = torch.randn(phi.shape, generator=torch.Generator(device="cpu").manual_seed(8)).to(dev)
+    band = 4 * W['dx']                       # the etch loop's extension band
+
+    def run(grad):
+        E = torch.tensor(float(par['Emean']), dtype=torch.float64, requires_grad=grad)
+        m_i, cos_i, iy = adjoint.ion_terms(mesh, geo, par, N_ION, 1, E)
+        _, m_F, m_O = adjoint.coupled_coverage(mesh, geo, par, flags, m_i, cos_i, iy, N_NEU, 1,
+                                               n_fp=N_FP)
+        phi1 = adjoint.advance(mesh, phi, mask, phi, W['dx'], DT, nsub, band, m_i, m_F, m_O,
+                               cos_i, iy if iy_rate else None, mg, chem)
+        loss = (phi1 * w).sum()
+        if grad:
+            loss.backward()
+            return float(E.grad)
+        return None
+
+    res = {}
+    for grad in (False, True):
+        run(grad)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            g = run(grad)
+        e.record()
+        torch.cuda.synchronize()
+        res["fwd_adj_ms" if grad else "fwd_ms"] = s.elapsed_time(e) / reps
+        if grad:
+            res["dloss_dE"] = g
+    return {"scope": f"one etch step from the ion energy on the trench surface ({mesh.nf} faces): "
+                     f"{N_ION} ions, {N_FP} x ({N_NEU} F + {N_NEU} O) RR neutrals, rate, "
+                     f"extension, {nsub} advection substeps; geometry fixed, no reinit",
+            **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -464,6 +526,7 @@ def main(argv=None):
         res["adjoint"] = adjoint_cost()
         res["adjoint_radiosity"] = radiosity_adjoint_cost()
         res["adjoint_neutral"] = neutral_adjoint_cost()
+        res["adjoint_step"] = step_adjoint_cost()
     if not a.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline()
