@@ -452,7 +452,8 @@ def main(argv=None):
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the etch step of one feature runs on one device "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "(batches of features would be split across GPUs, one process each, with no "
+              "communication inside a step); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     from petch import ops
