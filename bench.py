@@ -438,7 +438,14 @@ def main(argv=None):
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run plain device copies of 256 MiB (reference for PMC byte counts)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time the differentiable pieces (one JSON line; for profiling)")
     a = ap.parse_args(argv)
+    if a.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_cost(), "adjoint_radiosity": radiosity_adjoint_cost(),
+                          "adjoint_neutral": neutral_adjoint_cost(),
+                          "adjoint_step": step_adjoint_cost()}), flush=True)
+        return
     if a.copy_probe:
         import torch
         x = torch.empty(64 << 20, dtype=torch.float32, device="cuda")

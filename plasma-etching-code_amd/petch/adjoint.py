@@ -45,29 +45,36 @@ from . import ops
 
 class _Advect(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, phi, F, mask, mask_phi, dx, dt, nsub):
+    def forward(ctx, phi, F, mask, mask_phi, dx, dt, nsub, recompute):
         states = [phi.detach().contiguous()]
         Fd = F.detach().contiguous()
         for _ in range(int(nsub)):
             states.append(ops.advect_step(states[-1], Fd, mask, mask_phi, dx, dt))
-        ctx.save_for_backward(*states[:-1], Fd, mask)
-        ctx.dx, ctx.dt = float(dx), float(dt)
+        kept = states[:1] if recompute else states[:-1]
+        ctx.save_for_backward(*kept, Fd, mask, mask_phi)
+        ctx.args = (float(dx), float(dt), int(nsub), bool(recompute))
         return states[-1] if nsub > 0 else states[0].clone()
 
     @staticmethod
     def backward(ctx, gout):
-        *states, F, mask = ctx.saved_tensors
+        *states, F, mask, mask_phi = ctx.saved_tensors
+        dx, dt, nsub, recompute = ctx.args
+        if recompute:  # the substeps are deterministic: rebuild the states from the input
+            for _ in range(nsub - 1):
+                states.append(ops.advect_step(states[-1], F, mask, mask_phi, dx, dt))
         gF = torch.zeros_like(F)
         g = gout.contiguous()
-        for s in reversed(states):
-            g = ops.advect_adjoint(s, F, mask, g, ctx.dx, ctx.dt, gF)
-        return g, gF, None, None, None, None, None
+        for s in reversed(states[:nsub]):
+            g = ops.advect_adjoint(s, F, mask, g, dx, dt, gF)
+        return g, gF, None, None, None, None, None, None
 
 
-def advect(phi, F, mask, mask_phi, dx, dt, nsub):
+def advect(phi, F, mask, mask_phi, dx, dt, nsub, recompute=False):
     """nsub upwind substeps of size dt with the mask re-pin (ops.advect), differentiable in phi
-    and F."""
-    return _Advect.apply(phi, F, mask, mask_phi, dx, dt, nsub)
+    and F. recompute=True keeps only the input level set for the backward pass and rebuilds the
+    substep states there (one grid instead of nsub; the substeps are deterministic, so the
+    gradient is the same bit for bit)."""
+    return _Advect.apply(phi, F, mask, mask_phi, dx, dt, nsub, recompute)
 
 
 class _Radiosity(torch.autograd.Function):

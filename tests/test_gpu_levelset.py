@@ -145,6 +145,15 @@ def test_advect_adjoint_matches_oracle():
                                ops.advect(tp.clone(), tF, tm, torch.from_numpy(mask_phi).to(d),
                                           dx, dt, nsub).cpu().numpy(), rtol=0, atol=0)
     (out * w).sum().backward()
+    # the same gradient when the substep states are rebuilt in the backward pass
+    x2 = tp.clone().requires_grad_()
+    f2 = tF.clone().requires_grad_()
+    out2 = adjoint.advect(x2, f2, tm, torch.from_numpy(mask_phi).to(d), dx, dt, nsub,
+                          recompute=True)
+    (out2 * w).sum().backward()
+    assert torch.equal(out2, out)
+    assert torch.equal(x2.grad, x.grad)
+    np.testing.assert_allclose(f2.grad.cpu().numpy(), f.grad.cpu().numpy(), rtol=0, atol=0)
 
     def loss(p, fs):
         return float((go * ol.advect_pinned(p, fs, mask, mask_phi.astype(np.float64), dx, dt,
