@@ -441,6 +441,20 @@ struct NeutralRR {
 // segment's first emission (o0, e0, w0, b0 kept in registers) and phase 1 replays the segment
 // hit for hit, adding -beta (T - P_i) / (1 - S_i) to gbare[f_i]. One-hit segments owe nothing
 // and are not replayed. T and P are float64, so T - P_i keeps its digits when S_i is close to 1.
+// Most segments are short (the weight falls below the roulette threshold within a few tens of
+// hits at most), so phase 0 also keeps the face ids of the segment's first kSegBuf hits in LDS;
+// a segment that fits is scattered from there when it ends (its weights recomputed from the
+// segment's first weight with the same w - w S step, its S and gfl read again), and only
+// longer ones are traced again. Face ids alone keep the buffer at 16 KB per block.
+constexpr int kSegBuf = 16;
+
+// This lane's segment buffer: entry k of lane t at [k * kBS + t] (consecutive lanes, consecutive
+// banks).
+__device__ __forceinline__ int* seg_buf() {
+  __shared__ int buf_f[kSegBuf * kBS];
+  return buf_f + threadIdx.x;
+}
+
 struct NeutralRRAdjoint {
   static constexpr bool kReplay = true;
   BvhView bv;
@@ -455,6 +469,7 @@ struct NeutralRRAdjoint {
     float w, w0;
     int bounce, b0, bend;  // hit index now, at the segment's start, of the segment's last hit
     int phase;
+    int nb;                // hits of the segment kept in the LDS buffer, -1 once it overflowed
     unsigned ray;
     double T, P;
     Vec3 o0;
@@ -468,7 +483,22 @@ struct NeutralRRAdjoint {
     R.w0 = R.w;
     R.b0 = R.bounce;
     R.phase = 0;
+    R.nb = 0;
     R.T = 0.0;
+  }
+  // the gradients of a buffered segment (its last hit owes nothing)
+  __device__ __forceinline__ void scatter(const Ray& R) const {
+    const int* B = seg_buf();
+    double P = 0.0;
+    float w = R.w0;
+    for (int k = 0; k < R.nb - 1; ++k) {
+      const int f = B[k * kBS];
+      const float S = bare[f] * beta;
+      P += (double)(gfl[f] * w);
+      const double q = 1.0 - (double)S;
+      if (q > 0.0) deposit(gbare, f, (float)(-(double)beta * (R.T - P) / q));
+      w = w - w * S;
+    }
   }
   __device__ __forceinline__ static bool replay(Ray& R, int bend, Vec3& o, Emit& e) {
     R.bend = bend;
@@ -492,14 +522,24 @@ struct NeutralRRAdjoint {
     if (petch::dot(nn, d) > 0.f) nn = -nn;
     const int i = R.bounce;
     const float S = bare[h.face] * beta;
-    const double gw = (double)gfl[h.face] * (double)R.w;
+    const float gwf = gfl[h.face] * R.w;
+    const double gw = (double)gwf;
     float w2;
     bool reset;
     uint32_t u[4];
     const bool cont = rr_weight(R.w, S, seed, R.ray, R.bounce, w2, reset, u);
     if (R.phase == 0) {
       R.T += gw;
-      if ((!cont || reset) && i > R.b0) return replay(R, i, o, e);
+      if (R.nb >= 0 && R.nb < kSegBuf) {
+        seg_buf()[R.nb * kBS] = h.face;
+        ++R.nb;
+      } else {
+        R.nb = -1;
+      }
+      if ((!cont || reset) && i > R.b0) {
+        if (R.nb < 0) return replay(R, i, o, e);
+        scatter(R);
+      }
     } else {
       R.P += gw;
       const double q = 1.0 - (double)S;
@@ -513,9 +553,13 @@ struct NeutralRRAdjoint {
     return true;
   }
   // The ray left the domain after hit R.bounce - 1: a phase-0 segment of two or more hits is
-  // replayed; a replayed segment (phase 1) ends the ray.
+  // scattered from the buffer, or replayed if it did not fit; a replayed segment (phase 1)
+  // ends the ray.
   __device__ __forceinline__ bool miss(Ray& R, Vec3& o, Emit& e) const {
-    if (R.phase == 0 && R.bounce - 1 > R.b0) return replay(R, R.bounce - 1, o, e);
+    if (R.phase == 0 && R.bounce - 1 > R.b0) {
+      if (R.nb < 0) return replay(R, R.bounce - 1, o, e);
+      scatter(R);
+    }
     return false;
   }
 };
