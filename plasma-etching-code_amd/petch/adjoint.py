@@ -163,6 +163,38 @@ class _NeutralRR(torch.autograd.Function):
         return gbare, None, None, None, None, None, None
 
 
+class _NeutralRRPair(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, bare, bvh, box, src, beta_a, n_a, seed_a, beta_b, n_b, seed_b):
+        b = bare.detach().contiguous()
+        fa, fb = ops.trace_neutral_rr_pair(bvh, box, src, b, beta_a, n_a, seed_a, beta_b, n_b,
+                                           seed_b)
+        ctx.save_for_backward(b)
+        ctx.args = (bvh, box, src, float(beta_a), int(n_a), int(seed_a), float(beta_b), int(n_b),
+                    int(seed_b))
+        return fa, fb
+
+    @staticmethod
+    def backward(ctx, ga, gb):
+        (b,) = ctx.saved_tensors
+        bvh, box, src, beta_a, n_a, seed_a, beta_b, n_b, seed_b = ctx.args
+        g = None
+        for gfl, beta, n, seed in ((ga, beta_a, n_a, seed_a), (gb, beta_b, n_b, seed_b)):
+            if gfl is None:
+                continue
+            gs = ops.trace_neutral_rr_adjoint(bvh, box, src, b, beta, n, seed,
+                                              gfl.contiguous().float())
+            g = gs if g is None else g + gs
+        return g, None, None, None, None, None, None, None, None, None
+
+
+def neutral_rr_pair(bvh, box, src, bare, beta_a, n_a, seed_a, beta_b, n_b, seed_b):
+    """The F and O accumulators of one paired launch (ops.trace_neutral_rr_pair, the same
+    streams as two single launches), differentiable in bare (one path-replay adjoint per
+    species)."""
+    return _NeutralRRPair.apply(bare, bvh, box, src, beta_a, n_a, seed_a, beta_b, n_b, seed_b)
+
+
 def neutral_rr(bvh, box, src, bare, beta, n, seed):
     """Raw RR neutral accumulator (ops.trace_neutral_rr), differentiable in bare. The BVH must
     be unchanged between the forward and the backward pass."""
@@ -200,9 +232,8 @@ def finish(fl, mesh, scale, hi, n_smooth, alpha, blend_each, lo=0.0):
 
 def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, n_fp=None):
     """(bare, m_F, m_O) of the neutral rounds of flux.coupled_flux_steps for given ion terms
-    (m_i, cos_i, ion_yield): n_fp x {the F and O RR traces at bare (same seeds and streams as the
-    coupled flux; two single-species launches instead of the pair), finish, Belen update} from
-    bare = 1. Differentiable in m_i (without ion yields) or the ion yields, through every round:
+    (m_i, cos_i, ion_yield): n_fp x {the F and O RR traces at bare (one paired launch with the
+    seeds and streams of the coupled flux), finish, Belen update} from bare = 1. Differentiable in m_i (without ion yields) or the ion yields, through every round:
     the traces by their path-replay adjoint, finish by its transpose, Belen in closed form."""
     from .chemistry import chem_params
     from .flux import NEUTRAL_SOURCE, SEED_MASK, box_of
@@ -219,8 +250,8 @@ def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, 
     m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
     for it in range(n_fp):
         sF, sO = (seed * 9 + 2 + 2 * it) & SEED_MASK, (seed * 9 + 3 + 2 * it) & SEED_MASK
-        fF = neutral_rr(mesh.bvh, box, NEUTRAL_SOURCE, bare, betaE, n_neu, sF)
-        fO = neutral_rr(mesh.bvh, box, NEUTRAL_SOURCE, bare, betaO, n_neu, sO)
+        fF, fO = neutral_rr_pair(mesh.bvh, box, NEUTRAL_SOURCE, bare, betaE, n_neu, sF, betaO,
+                                 n_neu, sO)
         m_F = finish(fF, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
         m_O = finish(fO, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
         bare = belen(m_i, m_F, m_O, cos_i, ion_yield, chem)
