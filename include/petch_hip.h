@@ -217,6 +217,14 @@ int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box
 int petch_trace_neutral_rr_adjoint(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
                                    const petch_source* src, const float* bare, float beta, int n,
                                    uint64_t seed, const float* gfl, float* gbare);
+/* petch_trace_neutral_rr_adjoint for the two species of petch_trace_neutral_rr_pair in one
+ * launch: gbare (zeroed first) = the sum of both species' gradients (gfl_a for the n_a rays of
+ * (beta_a, seed_a), gfl_b for the n_b rays of (beta_b, seed_b)). No reference counterpart. */
+int petch_trace_neutral_rr_pair_adjoint(petch_ctx* ctx, const petch_bvh* bvh,
+                                        const petch_box* box, const petch_source* src,
+                                        const float* bare, float beta_a, int n_a, uint64_t seed_a,
+                                        const float* gfl_a, float beta_b, int n_b, uint64_t seed_b,
+                                        const float* gfl_b, float* gbare);
 /* The two _trace3d_cov_rr launches of one flux/coverage iteration (F then O, threed.py:876-880)
  * as one launch over the same surface and coverage: species a (beta_a, n_a rays, seed_a, fl_a)
  * and species b. Each ray keeps the stream it has in two petch_trace_neutral_rr calls, so the

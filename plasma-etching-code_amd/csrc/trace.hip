@@ -455,22 +455,27 @@ __device__ __forceinline__ int* seg_buf() {
   return buf_f + threadIdx.x;
 }
 
+// Like the trace, one launch can carry the two species of a fixed-point iteration: queue ids
+// [0, n0) are species 0, the rest species 1 with ids from n0, each with its own beta, stream
+// and gfl; both add into the one gbare.
 struct NeutralRRAdjoint {
   static constexpr bool kReplay = true;
   BvhView bv;
   Box b;
   Source src;
   const float* __restrict__ bare;
-  const float* __restrict__ gfl;
+  const float* gfl_sp[2];
   float* gbare;
-  float beta;
-  uint64_t seed;
+  float beta_sp[2];
+  uint64_t seed_sp[2];
+  unsigned n0;
   struct Ray {
     float w, w0;
     int bounce, b0, bend;  // hit index now, at the segment's start, of the segment's last hit
     int phase;
     int nb;                // hits of the segment kept in the LDS buffer, -1 once it overflowed
-    unsigned ray;
+    int sp;                // species
+    unsigned ray;          // id within the species' stream
     double T, P;
     Vec3 o0;
     Emit e0;
@@ -489,6 +494,8 @@ struct NeutralRRAdjoint {
   // the gradients of a buffered segment (its last hit owes nothing)
   __device__ __forceinline__ void scatter(const Ray& R) const {
     const int* B = seg_buf();
+    const float beta = beta_sp[R.sp];
+    const float* __restrict__ gfl = gfl_sp[R.sp];
     double P = 0.0;
     float w = R.w0;
     for (int k = 0; k < R.nb - 1; ++k) {
@@ -511,8 +518,9 @@ struct NeutralRRAdjoint {
     return true;
   }
   __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
-    R.ray = ray;
-    neutral_source(seed, ray, b, src, o, e);
+    R.sp = ray >= n0;
+    R.ray = R.sp ? ray - n0 : ray;
+    neutral_source(seed_sp[R.sp], R.ray, b, src, o, e);
     R.w = 1.f;
     R.bounce = 0;
     begin_segment(R, o, e);
@@ -521,13 +529,14 @@ struct NeutralRRAdjoint {
     Vec3 nn = hit_normal(bv, h.slot);
     if (petch::dot(nn, d) > 0.f) nn = -nn;
     const int i = R.bounce;
+    const float beta = beta_sp[R.sp];
     const float S = bare[h.face] * beta;
-    const float gwf = gfl[h.face] * R.w;
+    const float gwf = gfl_sp[R.sp][h.face] * R.w;
     const double gw = (double)gwf;
     float w2;
     bool reset;
     uint32_t u[4];
-    const bool cont = rr_weight(R.w, S, seed, R.ray, R.bounce, w2, reset, u);
+    const bool cont = rr_weight(R.w, S, seed_sp[R.sp], R.ray, R.bounce, w2, reset, u);
     if (R.phase == 0) {
       R.T += gw;
       if (R.nb >= 0 && R.nb < kSegBuf) {
@@ -1056,6 +1065,21 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   return launch_neutral(ctx, pol, n);
 }
 
+namespace {
+
+int launch_neutral_adjoint(petch_ctx* ctx, const NeutralRRAdjoint& pol, int n) {
+  RayQueue q;
+  int rc = make_queue(ctx, n, q);
+  if (rc) return rc;
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr_adjoint, n);
+  hipLaunchKernelGGL(k_trace_neutral_rr_adjoint, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
+                     (unsigned long long*)nullptr);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+}  // namespace
+
 extern "C" int petch_trace_neutral_rr_adjoint(petch_ctx* ctx, const petch_bvh* bvh,
                                               const petch_box* box, const petch_source* src,
                                               const float* bare, float beta, int n, uint64_t seed,
@@ -1068,16 +1092,31 @@ extern "C" int petch_trace_neutral_rr_adjoint(petch_ctx* ctx, const petch_bvh* b
   int rc = zero(ctx, gbare, bvh->nf);
   if (rc) return rc;
   if (n == 0 || bvh->nf == 0) return PETCH_OK;
-  NeutralRRAdjoint pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, gfl, gbare,
-                       beta, seed};
-  RayQueue q;
-  rc = make_queue(ctx, n, q);
+  NeutralRRAdjoint pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, {gfl, gfl},
+                       gbare, {beta, beta}, {seed, seed}, (unsigned)n};
+  return launch_neutral_adjoint(ctx, pol, n);
+}
+
+extern "C" int petch_trace_neutral_rr_pair_adjoint(petch_ctx* ctx, const petch_bvh* bvh,
+                                                   const petch_box* box, const petch_source* src,
+                                                   const float* bare, float beta_a, int n_a,
+                                                   uint64_t seed_a, const float* gfl_a,
+                                                   float beta_b, int n_b, uint64_t seed_b,
+                                                   const float* gfl_b, float* gbare) {
+  PETCH_ARG(ctx, ctx && bvh && box && src && bare && gfl_a && gfl_b && gbare,
+            "trace_neutral_rr_pair_adjoint: null pointer");
+  PETCH_ARG(ctx, n_a >= 0 && n_b >= 0, "trace_neutral_rr_pair_adjoint: negative ray count");
+  PETCH_ARG(ctx, (long long)n_a + n_b <= 0x7fffffffLL,
+            "trace_neutral_rr_pair_adjoint: n_a + n_b > 2^31-1");
+  PETCH_ARG(ctx, gbare != bare && gbare != gfl_a && gbare != gfl_b,
+            "trace_neutral_rr_pair_adjoint: gbare must not alias bare or the gfl inputs");
+  int rc = zero(ctx, gbare, bvh->nf);
   if (rc) return rc;
-  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr_adjoint, n);
-  hipLaunchKernelGGL(k_trace_neutral_rr_adjoint, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
-                     (unsigned long long*)nullptr);
-  PETCH_LAUNCH_CHECK(ctx);
-  return PETCH_OK;
+  const int n = n_a + n_b;
+  if (n == 0 || bvh->nf == 0) return PETCH_OK;
+  NeutralRRAdjoint pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, {gfl_a, gfl_b},
+                       gbare, {beta_a, beta_b}, {seed_a, seed_b}, (unsigned)n_a};
+  return launch_neutral_adjoint(ctx, pol, n);
 }
 
 extern "C" int petch_trace_neutral_rr_pair(petch_ctx* ctx, const petch_bvh* bvh,

@@ -136,6 +136,8 @@ SIGNATURES = {
                                P],
     "petch_trace_neutral_rr_adjoint": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int,
                                        c_uint64, P, P],
+    "petch_trace_neutral_rr_pair_adjoint": [P, P, POINTER(Box), POINTER(Source), P, c_float,
+                                            c_int, c_uint64, P, c_float, c_int, c_uint64, P, P],
     "petch_trace_neutral_rr_pair": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int,
                                     c_uint64, P, c_float, c_int, c_uint64, P],
     "petch_trace_neutral_rr_batch": [P, POINTER(Source), c_int, P],

@@ -178,20 +178,17 @@ class _NeutralRRPair(torch.autograd.Function):
     def backward(ctx, ga, gb):
         (b,) = ctx.saved_tensors
         bvh, box, src, beta_a, n_a, seed_a, beta_b, n_b, seed_b = ctx.args
-        g = None
-        for gfl, beta, n, seed in ((ga, beta_a, n_a, seed_a), (gb, beta_b, n_b, seed_b)):
-            if gfl is None:
-                continue
-            gs = ops.trace_neutral_rr_adjoint(bvh, box, src, b, beta, n, seed,
-                                              gfl.contiguous().float())
-            g = gs if g is None else g + gs
+        ga = torch.zeros_like(b) if ga is None else ga.contiguous().float()
+        gb = torch.zeros_like(b) if gb is None else gb.contiguous().float()
+        g = ops.trace_neutral_rr_pair_adjoint(bvh, box, src, b, beta_a, n_a, seed_a, ga, beta_b,
+                                              n_b, seed_b, gb)
         return g, None, None, None, None, None, None, None, None, None
 
 
 def neutral_rr_pair(bvh, box, src, bare, beta_a, n_a, seed_a, beta_b, n_b, seed_b):
     """The F and O accumulators of one paired launch (ops.trace_neutral_rr_pair, the same
-    streams as two single launches), differentiable in bare (one path-replay adjoint per
-    species)."""
+    streams as two single launches), differentiable in bare (both species' path-replay adjoints
+    in one launch)."""
     return _NeutralRRPair.apply(bare, bvh, box, src, beta_a, n_a, seed_a, beta_b, n_b, seed_b)
 
 

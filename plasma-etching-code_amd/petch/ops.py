@@ -305,6 +305,24 @@ def trace_neutral_rr_adjoint(bvh, box, src, bare, beta, n, seed, gfl):
     return gbare
 
 
+def trace_neutral_rr_pair_adjoint(bvh, box, src, bare, beta_a, n_a, seed_a, gfl_a, beta_b, n_b,
+                                  seed_b, gfl_b):
+    """d loss / d bare (F,) of trace_neutral_rr_pair for the gradients of its two outputs, in one
+    launch (petch_trace_neutral_rr_pair_adjoint)."""
+    import ctypes
+    for t, name in ((bare, "bare"), (gfl_a, "gfl_a"), (gfl_b, "gfl_b")):
+        if t.numel() < bvh.nf:
+            raise ValueError(f"trace_neutral_rr_pair_adjoint: {name} holds {t.numel()} values, "
+                             f"needs {bvh.nf}")
+    gbare = torch.empty((bvh.nf,), dtype=f32, device=bvh.device)
+    bvh.ctx.call("petch_trace_neutral_rr_pair_adjoint", bvh.h, ctypes.byref(box),
+                 ctypes.byref(src), ptr(bare, f32, "bare"), float(beta_a), int(n_a),
+                 int(seed_a) & 0xFFFFFFFFFFFFFFFF, ptr(gfl_a, f32, "gfl_a"), float(beta_b),
+                 int(n_b), int(seed_b) & 0xFFFFFFFFFFFFFFFF, ptr(gfl_b, f32, "gfl_b"),
+                 ptr(gbare, f32))
+    return gbare
+
+
 def trace_neutral_rr_pair(bvh, box, src, bare, beta_a, n_a, seed_a, beta_b, n_b, seed_b):
     """Two Russian-roulette neutral species over the same surface and coverage in one launch;
     each output equals trace_neutral_rr with the same (beta, n, seed)."""

@@ -358,3 +358,22 @@ def test_ion_yield_energy_tangent_matches_oracle_differences(name):
     out = adjoint.ion_yield(bvh, box, src, ion, n, 4321, E)
     (out[:3].double() * torch.from_numpy(wts).to(dev)).sum().backward()
     assert abs(float(E.grad) - a) <= 1e-4 * np.abs(wts * t).sum()
+
+
+def test_neutral_rr_pair_adjoint_equals_two_single_adjoints():
+    """The paired adjoint launch carries each ray on its single-launch stream, so its gradient
+    equals the sum of the two single-species adjoints up to the order of the float atomics."""
+    import torch
+    from petch import ops
+    g, v, f, bvh, box, obox, obvh, dev = _setup("trench")
+    rng = np.random.default_rng(6)
+    F = len(f)
+    tb = torch.from_numpy(rng.uniform(0.2, 1.0, F).astype(np.float32)).to(dev)
+    ga = torch.from_numpy(rng.normal(size=F).astype(np.float32)).to(dev)
+    gb = torch.from_numpy(rng.normal(size=F).astype(np.float32)).to(dev)
+    src = ops.make_source(0)
+    pair = ops.trace_neutral_rr_pair_adjoint(bvh, box, src, tb, 0.7, 30000, 11, ga, 1.0, 20000,
+                                             12, gb).cpu().numpy()
+    single = (ops.trace_neutral_rr_adjoint(bvh, box, src, tb, 0.7, 30000, 11, ga) +
+              ops.trace_neutral_rr_adjoint(bvh, box, src, tb, 1.0, 20000, 12, gb)).cpu().numpy()
+    np.testing.assert_allclose(pair, single, rtol=1e-4, atol=1e-4 * np.abs(single).max())
