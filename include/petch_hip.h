@@ -209,11 +209,13 @@ int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box
                            uint64_t seed, float* fl);
 /* Path-replay adjoint of petch_trace_neutral_rr with the same (bare, beta, n, seed): for
  * gfl = d loss / d fl, gbare (nf, zeroed first) = d loss / d bare with every ray's path and its
- * roulette / termination decisions held fixed (the rays redraw their streams). Each ray traces
- * every roulette segment of two or more hits twice: once to total gfl[f_k] w_k over the
- * segment, once to add -beta (segment total - running total) / (1 - S_i) to gbare[f_i].
- * gbare must not alias bare or gfl. No reference counterpart (the reference does not
- * differentiate _trace3d_cov_rr, threed.py:549-596). */
+ * roulette / termination decisions held fixed (the rays redraw their streams) and a roulette
+ * survivor's weight read as w / p with its survival probability p held fixed, so that the
+ * expectation of the result is the derivative of the expected flux. A ray totals gfl[f_k] w_k
+ * over its path, then adds -beta (total - running total) / (1 - S_i) to gbare[f_i] at each hit,
+ * from a buffer of its first 16 hits or by tracing the path again. gbare must not alias bare or
+ * gfl. No reference counterpart (the reference does not differentiate _trace3d_cov_rr,
+ * threed.py:549-596). */
 int petch_trace_neutral_rr_adjoint(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
                                    const petch_source* src, const float* bare, float beta, int n,
                                    uint64_t seed, const float* gfl, float* gbare);

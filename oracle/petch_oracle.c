@@ -688,9 +688,11 @@ void oracle_trace_neutral_rr(void* h, const oc_box* b, const oc_source* s, const
 
 /* oracle_trace_neutral_rr with the decisions frozen (test instrument, not in the reference): the
  * paths, roulettes and terminations are those of the sticking bare_dec, while the deposited
- * weights follow bare_w (w_{k+1} = w_k (1 - bare_w beta), 0.3 where the bare_dec ray survived a
- * roulette). At bare_w = bare_dec it is oracle_trace_neutral_rr; its central differences in
- * bare_w are the derivative the path adjoint computes, with no decision flipping. */
+ * weights follow bare_w (w_{k+1} = w_k (1 - bare_w beta); a roulette survivor's weight is
+ * divided by its survival probability p = w_dec / 0.3, p taken from the bare_dec ray, which is
+ * the trace's 0.3 when bare_w = bare_dec). At bare_w = bare_dec it is oracle_trace_neutral_rr;
+ * its central differences in bare_w are the derivative the path adjoint computes, with no
+ * decision flipping. */
 void oracle_trace_neutral_rr_frozen(void* h, const oc_box* b, const oc_source* s,
                                     const float* bare_dec, const float* bare_w, float beta, int n,
                                     uint64_t seed, double* fl) {
@@ -720,8 +722,8 @@ void oracle_trace_neutral_rr_frozen(void* h, const oc_box* b, const oc_source* s
         oc_block(seed, (uint32_t)p, (uint32_t)bounce + 1u, 1u, u);
         if (w < 0.1f) {
           if (oc_u24(u[0]) < (1.f - w / 0.3f)) break;
+          ww = ww * (0.3 / (double)w); /* survivors carry w / p with p = w / 0.3 frozen */
           w = 0.3f;
-          ww = 0.3;
         }
         f3 hit = add3(o, scl(t, d));
         d = oc_cosine_u(nn, oc_u24(u[1]), oc_u24(u[2]));
@@ -736,12 +738,14 @@ void oracle_trace_neutral_rr_frozen(void* h, const oc_box* b, const oc_source* s
 
 /* Path adjoint of oracle_trace_neutral_rr (not in the reference): gbare = d loss / d bare for
  * gfl = d loss / d fl, with every path's geometry and its roulette / termination decisions held
- * fixed. Each path is traced once and stored (face, weight, sticking, whether its next weight
- * was reset to 0.3 by a survived roulette); then, from the last hit back,
- *   Q_k = 0 where the next weight does not depend on w_k (reset at k, or the last hit), else
- *   Q_k = gfl[f_{k+1}] + (1 - S_{k+1}) Q_{k+1} with Q_{k+1} as defined for hit k+1,
- * is the derivative of the path's later deposits by w_{k+1}, and d loss / d S_k = -w_k Q_k
- * (w_{k+1} = w_k (1 - S_k)), so gbare[f_k] += -beta w_k Q_k. Per-face sums in double. */
+ * fixed and a roulette survivor's weight read as w / p with its survival probability
+ * p = w / 0.3 held fixed (the value is the trace's 0.3; the derivative then carries on through
+ * the roulette, and its expectation is the derivative of the expected flux, since E[1{u < p} w / p]
+ * = w). Each path is traced once and stored (face, arrival weight, sticking); with
+ * w_{k+1} = c_k w_k (1 - S_k) (c_k the roulette factor or 1), d w_j / d S_k = -w_j / (1 - S_k)
+ * for every later hit j, so from the last hit back
+ *   Q_k = sum over j > k of gfl[f_j] w_j = gfl[f_{k+1}] w_{k+1} + Q_{k+1},
+ * and gbare[f_k] += -beta Q_k / (1 - S_k). Per-face sums in double. */
 void oracle_trace_neutral_rr_adjoint(void* h, const oc_box* b, const oc_source* s,
                                      const float* bare, float beta, int n, uint64_t seed,
                                      const double* gfl, double* gbare) {
@@ -754,7 +758,6 @@ void oracle_trace_neutral_rr_adjoint(void* h, const oc_box* b, const oc_source* 
     int* pf = (int*)malloc(sizeof(int) * 1024);
     float* pw = (float*)malloc(sizeof(float) * 1024);
     float* pS = (float*)malloc(sizeof(float) * 1024);
-    char* prs = (char*)malloc(1024);
 #pragma omp for schedule(dynamic, 256)
     for (int p = 0; p < n; ++p) {
       oc_rng r;
@@ -771,7 +774,6 @@ void oracle_trace_neutral_rr_adjoint(void* h, const oc_box* b, const oc_source* 
         pf[K] = f;
         pw[K] = w;
         pS[K] = S;
-        prs[K] = 0;
         ++K;
         w = w - w * S;
         if (w <= 1.0e-6f) break;
@@ -780,18 +782,16 @@ void oracle_trace_neutral_rr_adjoint(void* h, const oc_box* b, const oc_source* 
         if (w < 0.1f) {
           if (oc_u24(u[0]) < (1.f - w / 0.3f)) break;
           w = 0.3f;
-          prs[K - 1] = 1;
         }
         f3 hit = add3(o, scl(t, d));
         d = oc_cosine_u(nn, oc_u24(u[1]), oc_u24(u[2]));
         o = add3(hit, scl(1.0e-4f, nn));
       }
-      double Qn = 0.0; /* Q of the hit after k */
+      double Q = 0.0; /* sum of gfl w over the hits after k */
       for (int k = K - 1; k >= 0; --k) {
-        double Q = 0.0;
-        if (k + 1 < K && !prs[k]) Q = gfl[pf[k + 1]] + (1.0 - (double)pS[k + 1]) * Qn;
-        if (!prs[k]) acc[pf[k]] += -(double)beta * (double)pw[k] * Q;
-        Qn = Q;
+        if (k + 1 < K) Q += gfl[pf[k + 1]] * (double)pw[k + 1];
+        const double q = 1.0 - (double)pS[k];
+        if (k + 1 < K && q > 0.0) acc[pf[k]] += -(double)beta * Q / q;
       }
     }
 #pragma omp critical
@@ -800,7 +800,6 @@ void oracle_trace_neutral_rr_adjoint(void* h, const oc_box* b, const oc_source* 
     free(pf);
     free(pw);
     free(pS);
-    free(prs);
   }
 }
 

@@ -428,24 +428,27 @@ struct NeutralRR {
   }
 };
 
-// Path-replay adjoint of the single-species trace (no reference counterpart): for gfl = d loss /
-// d fl it accumulates d loss / d bare into gbare, with every ray following the path the trace
-// drew (same Philox stream, same decisions; the roulette and termination decisions and the path
-// geometry are held fixed). Along a ray the deposits are w_k at hit k, w_{k+1} = w_k (1 - S_k)
-// with S_k = bare[f_k] beta, except that a survived roulette sets the next weight to 0.3. A
-// segment is a run of hits between such resets (or the ray's start or end); within one,
-//   d loss / d S_i = -(sum over later hits k of the segment of gfl[f_k] w_k) / (1 - S_i)
-//                  = -(T - P_i) / (1 - S_i),
-// with T the segment's total of gfl[f_k] w_k and P_i its running sum up to hit i. A ray
-// therefore traces each segment twice: phase 0 accumulates T, then it restarts from the
-// segment's first emission (o0, e0, w0, b0 kept in registers) and phase 1 replays the segment
-// hit for hit, adding -beta (T - P_i) / (1 - S_i) to gbare[f_i]. One-hit segments owe nothing
-// and are not replayed. T and P are float64, so T - P_i keeps its digits when S_i is close to 1.
-// Most segments are short (the weight falls below the roulette threshold within a few tens of
-// hits at most), so phase 0 also keeps the face ids of the segment's first kSegBuf hits in LDS;
-// a segment that fits is scattered from there when it ends (its weights recomputed from the
-// segment's first weight with the same w - w S step, its S and gfl read again), and only
-// longer ones are traced again. Face ids alone keep the buffer at 16 KB per block.
+// Path-replay adjoint of the trace (no reference counterpart): for gfl = d loss / d fl it
+// accumulates d loss / d bare into gbare, with every ray following the path the trace drew (same
+// Philox stream, same roulette and termination decisions, same geometry). Along a ray the
+// deposits are w_k at hit k and w_{k+1} = c_k w_k (1 - S_k) with S_k = bare[f_k] beta, where
+// c_k = 1, or 1 / p for a ray that survived a roulette with probability p = w / 0.3 (the trace
+// gives survivors 0.3 = w / p). Holding p fixed while w varies makes the derivative carry on
+// through the roulette, and its expectation is the derivative of the expected flux
+// (E[1{u < p} w / p] = w); cutting it at the roulette instead would miss that term (about a
+// fifth of the gradient on a trench, measured against common-random-number differences). So for
+// every later hit j, d w_j / d S_i = -w_j / (1 - S_i), and
+//   d loss / d S_i = -(sum over later hits k of gfl[f_k] w_k) / (1 - S_i) = -(T - P_i) / (1 - S_i),
+// with T the path's total of gfl[f_k] w_k and P_i its running sum up to hit i. T is only known
+// when the ray ends, so phase 0 accumulates it, then the ray restarts from its source emission
+// (origin, lobe, weight and bounce index kept in registers; the Philox stream is keyed by
+// (ray, bounce), so the replay redraws the same numbers) and phase 1 replays the path hit for
+// hit, adding -beta (T - P_i) / (1 - S_i) to gbare[f_i]. One-hit paths owe nothing and are not
+// replayed. T and P are float64, so T - P_i keeps its digits when S_i is close to 1. Most paths
+// are short, so phase 0 also keeps the face ids of the first kSegBuf hits in LDS; a path that
+// fits is scattered from there when it ends (its weights recomputed from the first one with the
+// same steps, its S and gfl read again), and only longer ones are traced again. Face ids alone
+// keep the buffer at 16 KB per block.
 constexpr int kSegBuf = 16;
 
 // This lane's segment buffer: entry k of lane t at [k * kBS + t] (consecutive lanes, consecutive
@@ -471,9 +474,9 @@ struct NeutralRRAdjoint {
   unsigned n0;
   struct Ray {
     float w, w0;
-    int bounce, b0, bend;  // hit index now, at the segment's start, of the segment's last hit
+    int bounce, b0, bend;  // hit index now, at the path's start, of the path's last hit
     int phase;
-    int nb;                // hits of the segment kept in the LDS buffer, -1 once it overflowed
+    int nb;                // hits of the path kept in the LDS buffer, -1 once it overflowed
     int sp;                // species
     unsigned ray;          // id within the species' stream
     double T, P;
@@ -482,7 +485,7 @@ struct NeutralRRAdjoint {
   };
   __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
   __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
-  __device__ __forceinline__ static void begin_segment(Ray& R, const Vec3& o, const Emit& e) {
+  __device__ __forceinline__ static void begin_path(Ray& R, const Vec3& o, const Emit& e) {
     R.o0 = o;
     R.e0 = e;
     R.w0 = R.w;
@@ -491,7 +494,8 @@ struct NeutralRRAdjoint {
     R.nb = 0;
     R.T = 0.0;
   }
-  // the gradients of a buffered segment (its last hit owes nothing)
+  // the gradients of a buffered path (its last hit owes nothing); every hit but the last
+  // continued the ray, so a weight that fell below 0.1 there survived its roulette (0.3)
   __device__ __forceinline__ void scatter(const Ray& R) const {
     const int* B = seg_buf();
     const float beta = beta_sp[R.sp];
@@ -504,7 +508,9 @@ struct NeutralRRAdjoint {
       P += (double)(gfl[f] * w);
       const double q = 1.0 - (double)S;
       if (q > 0.0) deposit(gbare, f, (float)(-(double)beta * (R.T - P) / q));
-      w = w - w * S;
+      float w2 = w - w * S;
+      if (w2 < 0.1f) w2 = 0.3f;
+      w = w2;
     }
   }
   __device__ __forceinline__ static bool replay(Ray& R, int bend, Vec3& o, Emit& e) {
@@ -523,7 +529,7 @@ struct NeutralRRAdjoint {
     neutral_source(seed_sp[R.sp], R.ray, b, src, o, e);
     R.w = 1.f;
     R.bounce = 0;
-    begin_segment(R, o, e);
+    begin_path(R, o, e);
   }
   __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
     Vec3 nn = hit_normal(bv, h.slot);
@@ -545,7 +551,7 @@ struct NeutralRRAdjoint {
       } else {
         R.nb = -1;
       }
-      if ((!cont || reset) && i > R.b0) {
+      if (!cont && i > R.b0) {
         if (R.nb < 0) return replay(R, i, o, e);
         scatter(R);
       }
@@ -558,12 +564,10 @@ struct NeutralRRAdjoint {
     if (!cont) return false;
     R.w = w2;
     rr_emit(h, nn, u, o, d, e);
-    if (reset) begin_segment(R, o, e);
     return true;
   }
-  // The ray left the domain after hit R.bounce - 1: a phase-0 segment of two or more hits is
-  // scattered from the buffer, or replayed if it did not fit; a replayed segment (phase 1)
-  // ends the ray.
+  // The ray left the domain after hit R.bounce - 1: a phase-0 path of two or more hits is
+  // scattered from the buffer, or replayed if it did not fit; a replayed path (phase 1) ends.
   __device__ __forceinline__ bool miss(Ray& R, Vec3& o, Emit& e) const {
     if (R.phase == 0 && R.bounce - 1 > R.b0) {
       if (R.nb < 0) return replay(R, R.bounce - 1, o, e);

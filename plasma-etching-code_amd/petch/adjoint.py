@@ -26,8 +26,10 @@ adjoint unrolled over the n_fp rounds.
 The Russian-roulette neutral trace (_trace3d_cov_rr, threed.py:549-596) carries a path-replay
 adjoint (petch_trace_neutral_rr_adjoint): `neutral_rr(bvh, box, src, bare, beta, n, seed)`
 returns the raw per-face accumulator and back-propagates to bare with every ray's path and its
-roulette decisions held fixed (common random numbers: the backward pass redraws the same Philox
-streams, so nothing is stored per ray). `coupled_coverage` chains it with the clip-normalisation
+roulette decisions held fixed and a roulette survivor's weight read as w / p (p, its survival
+probability, held fixed), which makes the gradient's expectation the derivative of the expected
+flux: it matches common-random-number differences to a fraction of a percent. The backward pass
+redraws the same Philox streams, so nothing is stored per ray. `coupled_coverage` chains it with the clip-normalisation
 and smoothing of the fluxes (petch_normalize_adjoint, petch_smooth_adjoint) and the Belen update
 into the neutral rounds of the MC coupled flux (threed.py:855-904), differentiable in the ion
 terms.

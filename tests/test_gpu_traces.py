@@ -198,9 +198,9 @@ def test_flat_open_field_unity():
 
 @pytest.mark.parametrize("name", ["hole", "trench"])
 def test_neutral_rr_path_adjoint_matches_oracle(name):
-    """The path-replay adjoint (each roulette segment traced twice: its total, then the
-    per-hit gradients) against the oracle's stored-path backward recursion on the same Philox
-    streams: same-seed L1 within 3 %, independent-seed z-scores, and directional derivatives
+    """The path-replay adjoint (a path's total, then the per-hit gradients from the LDS buffer or
+    a second trace of the path) against the oracle's stored-path backward sums on the same
+    Philox streams: same-seed L1 within 3 %, independent-seed z-scores, and directional derivatives
     against central differences of the oracle trace with frozen decisions. The autograd
     Function gives the same gradient."""
     import torch
