@@ -451,7 +451,7 @@ struct NeutralRR {
 // keep the buffer at 16 KB per block.
 constexpr int kSegBuf = 16;
 
-// This lane's segment buffer: entry k of lane t at [k * kBS + t] (consecutive lanes, consecutive
+// This lane's path buffer: entry k of lane t at [k * kBS + t] (consecutive lanes, consecutive
 // banks).
 __device__ __forceinline__ int* seg_buf() {
   __shared__ int buf_f[kSegBuf * kBS];
