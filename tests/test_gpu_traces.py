@@ -377,3 +377,31 @@ def test_neutral_rr_pair_adjoint_equals_two_single_adjoints():
     single = (ops.trace_neutral_rr_adjoint(bvh, box, src, tb, 0.7, 30000, 11, ga) +
               ops.trace_neutral_rr_adjoint(bvh, box, src, tb, 1.0, 20000, 12, gb)).cpu().numpy()
     np.testing.assert_allclose(pair, single, rtol=1e-4, atol=1e-4 * np.abs(single).max())
+
+
+def test_neutral_rr_adjoint_matches_common_random_number_differences():
+    """The acceptance check for the gradient of the expected flux, on the GPU's own trace: the
+    path-replay adjoint against central differences of the trace with common random numbers
+    (the same seed at bare +- h u, decisions free to flip), 4M rays, positive loss weights and
+    directions so that the flips' noise is small next to the signal: within 2 %."""
+    import torch
+    from petch import ops
+    g, v, f, bvh, box, obox, obvh, dev = _setup("trench")
+    rng = np.random.default_rng(13)
+    F = len(f)
+    bare = torch.from_numpy(rng.uniform(0.2, 1.0, F).astype(np.float32)).to(dev)
+    w = torch.from_numpy(rng.uniform(0.5, 1.5, F).astype(np.float32)).to(dev)
+    src = ops.make_source(0)
+    n, beta, h = 4000000, 0.7, 0.05
+    gb = ops.trace_neutral_rr_adjoint(bvh, box, src, bare, beta, n, 21, w).double()
+    for k in range(2):
+        u = torch.from_numpy(rng.uniform(0.0, 1.0, F).astype(np.float32)).to(dev)
+        lp = float((ops.trace_neutral_rr(bvh, box, src, bare + h * u, beta, n, 21).double() *
+                    w.double()).sum())
+        lm = float((ops.trace_neutral_rr(bvh, box, src, bare - h * u, beta, n, 21).double() *
+                    w.double()).sum())
+        fd = (lp - lm) / (2 * h)
+        an = float((gb * u.double()).sum())
+        print(f"direction {k}: CRN fd {fd:.1f} adjoint {an:.1f} rel {abs(an - fd) / abs(fd):.4f}",
+              flush=True)
+        assert abs(an - fd) <= 0.02 * abs(fd)
