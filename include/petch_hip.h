@@ -429,6 +429,7 @@ int petch_radiosity_solve(petch_ctx* ctx, const int* row_ptr, const int* col, co
  * under the same n_iter / tol rule as the solve; Aff^T is read straight from hitface (the
  * petch_trace_ff output the CSR was assembled from). gD (nf, float64) = lambda = d loss / dD;
  * gbare (nf, float32) = -beta [0 <= bare*beta <= 1] G (Aff^T lambda) = d loss / d bare.
+ * gD must not alias G (it holds the sweeps' iterates).
  * No reference counterpart (the reference does not differentiate its radiosity solve,
  * threed.py:965-985); it is the implicit-function adjoint, exact for a converged solve. */
 int petch_radiosity_adjoint(petch_ctx* ctx, const int* hitface, int nf, int n_ff, const double* G,

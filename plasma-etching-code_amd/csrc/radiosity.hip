@@ -281,6 +281,8 @@ extern "C" int petch_radiosity_adjoint(petch_ctx* ctx, const int* hitface, int n
             "radiosity_adjoint: null pointer");
   PETCH_ARG(ctx, nf >= 0 && n_ff > 0, "radiosity_adjoint: bad size");
   PETCH_ARG(ctx, (long long)nf * n_ff < (1LL << 31), "radiosity_adjoint: nf*n_ff too large");
+  PETCH_ARG(ctx, (const void*)gD != (const void*)G,
+            "radiosity_adjoint: gD is a sweep buffer and must not alias G");
   if (iters_host) *iters_host = 0;
   if (nf == 0) return PETCH_OK;
   double* w = (double*)petch_scratch(ctx, SCR_RAD_C, (size_t)nf * 24 + 64);
