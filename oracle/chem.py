@@ -170,15 +170,16 @@ def belen_bare_adjoint(m_i, m_F, m_O, cos_i, par, flags, gbare, ion_yield=None):
 
 def surface_rate_adjoint(m_i, m_F, m_O, cos_i, is_mask, par, flags, gV, ion_yield=None):
     """d loss / d inputs of surface_rate for gV = d loss / dV, by complex-step differentiation
-    (each face's V depends on that face's inputs only): dict(m_F, m_O, and m_i or sp, ie, ip);
-    cos_i constant; mask faces pass nothing."""
+    (each face's V depends on that face's inputs only): dict(m_F, m_O, m_i, and sp, ie, ip when
+    yields are given; an input the rate does not read gets zeros); cos_i constant; mask faces
+    pass nothing."""
     h = 1e-30
     base = dict(m_i=np.asarray(m_i, np.float64), m_F=np.asarray(m_F, np.float64),
                 m_O=np.asarray(m_O, np.float64))
     if ion_yield is not None:
         base.update(sp=np.asarray(ion_yield[0], np.float64), ie=np.asarray(ion_yield[1], np.float64),
                     ip=np.asarray(ion_yield[2], np.float64))
-    keys = ("m_F", "m_O") + (("sp", "ie", "ip") if ion_yield is not None else ("m_i",))
+    keys = ("m_F", "m_O", "m_i") + (("sp", "ie", "ip") if ion_yield is not None else ())
     g = np.asarray(gV, np.float64)
     out = {}
     for k in keys:

@@ -141,6 +141,17 @@ class OracleBvh:
                                       ctypes.c_uint64(seed), _p(out))
         return out
 
+    def trace_neutral_rr_paths(self, box, src, bare, beta, n, seed):
+        """trace_neutral_rr plus each ray's hit count (int32) and deposited weight (float64)."""
+        bare = np.ascontiguousarray(bare, np.float32)
+        out = np.zeros(self.nf, np.float64)
+        nhit = np.zeros(n, np.int32)
+        wsum = np.zeros(n, np.float64)
+        lib().oracle_trace_neutral_rr_paths(self.h, ctypes.byref(box), ctypes.byref(src),
+                                            _p(bare), ctypes.c_float(beta), ctypes.c_int(n),
+                                            ctypes.c_uint64(seed), _p(out), _p(nhit), _p(wsum))
+        return out, nhit, wsum
+
     def trace_neutral_rr_adjoint(self, box, src, bare, beta, n, seed, gfl):
         """d loss / d bare of trace_neutral_rr for gfl = d loss / d fl, paths and decisions fixed."""
         bare = np.ascontiguousarray(bare, np.float32)

@@ -644,6 +644,42 @@ static f3 oc_cosine(oc_rng* r, f3 n) {
   return oc_cosine_u(n, r1, r2);
 }
 
+/* One ray of _trace3d_cov_rr (threed.py:549-596): deposits into acc (per-face, double) and
+ * reports the ray's hit count and deposited weight when nhit / wsum are given. */
+static void oc_rr_ray(const oc_bvh* B, const oc_box* b, const oc_source* s, const float* bare,
+                      float beta, uint64_t seed, int p, double* acc, int* nhit, double* wsum) {
+  oc_rng r;
+  oc_rng_init(&r, seed, (uint32_t)p);
+  f3 o, d, nn;
+  oc_source_ray(&r, b, s, &o, &d);
+  float w = 1.f, t;
+  int k = 0;
+  double tot = 0.0;
+  for (int bounce = 0; bounce < 1024; ++bounce) {
+    int f = oc_query_bc(B, b, &o, d, &t, &nn);
+    if (f < 0) break;
+    if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
+    acc[f] += w;
+    ++k;
+    tot += w;
+    float S = bare[f] * beta;
+    w = w - w * S;
+    if (w <= 1.0e-6f) break;
+    /* the bounce's words: roulette, then the lobe's two uniforms */
+    uint32_t u[4];
+    oc_block(seed, (uint32_t)p, (uint32_t)bounce + 1u, 1u, u);
+    if (w < 0.1f) {
+      if (oc_u24(u[0]) < (1.f - w / 0.3f)) break;
+      w = 0.3f;
+    }
+    f3 hit = add3(o, scl(t, d));
+    d = oc_cosine_u(nn, oc_u24(u[1]), oc_u24(u[2]));
+    o = add3(hit, scl(1.0e-4f, nn));
+  }
+  if (nhit) *nhit = k;
+  if (wsum) *wsum = tot;
+}
+
 /* _trace3d_cov_rr (threed.py:549-596); per-face sums in double. */
 void oracle_trace_neutral_rr(void* h, const oc_box* b, const oc_source* s, const float* bare,
                              float beta, int n, uint64_t seed, double* fl) {
@@ -654,32 +690,26 @@ void oracle_trace_neutral_rr(void* h, const oc_box* b, const oc_source* s, const
   {
     double* acc = (double*)calloc(F > 0 ? F : 1, sizeof(double));
 #pragma omp for schedule(dynamic, 256)
-    for (int p = 0; p < n; ++p) {
-      oc_rng r;
-      oc_rng_init(&r, seed, (uint32_t)p);
-      f3 o, d, nn;
-      oc_source_ray(&r, b, s, &o, &d);
-      float w = 1.f, t;
-      for (int bounce = 0; bounce < 1024; ++bounce) {
-        int f = oc_query_bc(B, b, &o, d, &t, &nn);
-        if (f < 0) break;
-        if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
-        acc[f] += w;
-        float S = bare[f] * beta;
-        w = w - w * S;
-        if (w <= 1.0e-6f) break;
-        /* the bounce's words: roulette, then the lobe's two uniforms */
-        uint32_t u[4];
-        oc_block(seed, (uint32_t)p, (uint32_t)bounce + 1u, 1u, u);
-        if (w < 0.1f) {
-          if (oc_u24(u[0]) < (1.f - w / 0.3f)) break;
-          w = 0.3f;
-        }
-        f3 hit = add3(o, scl(t, d));
-        d = oc_cosine_u(nn, oc_u24(u[1]), oc_u24(u[2]));
-        o = add3(hit, scl(1.0e-4f, nn));
-      }
-    }
+    for (int p = 0; p < n; ++p) oc_rr_ray(B, b, s, bare, beta, seed, p, acc, NULL, NULL);
+#pragma omp critical
+    for (int f = 0; f < F; ++f) fl[f] += acc[f];
+    free(acc);
+  }
+}
+
+/* oracle_trace_neutral_rr with each ray's hit count and deposited weight (test instrument, not
+ * in the reference; the same per-ray code). */
+void oracle_trace_neutral_rr_paths(void* h, const oc_box* b, const oc_source* s,
+                                   const float* bare, float beta, int n, uint64_t seed,
+                                   double* fl, int* nhit, double* wsum) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  int F = B->nf;
+  memset(fl, 0, sizeof(double) * F);
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc(F > 0 ? F : 1, sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+    for (int p = 0; p < n; ++p) oc_rr_ray(B, b, s, bare, beta, seed, p, acc, nhit + p, wsum + p);
 #pragma omp critical
     for (int f = 0; f < F; ++f) fl[f] += acc[f];
     free(acc);

@@ -147,6 +147,18 @@ def radiosity_coverage(m_i, cos_i, ion_yield, D, hitface, row_ptr, col, n_ff, ch
     return bare, m_F, m_O
 
 
+
+def _check_bvh(bvh, what):
+    """Raises if the BVH's traversal guard tripped. The flag is cleared only by a rebuild, so one
+    read after a backward pass covers its launches and those of the forward pass on the same
+    BVH; a tripped guard means dropped subtrees and an incomplete gradient, which the etch loop
+    treats the same way (threed.EtchLoop.end_finish)."""
+    err = bvh.status()
+    if err:
+        raise RuntimeError(f"{what}: BVH traversal guard tripped (code {err}: 1 stack overflow, "
+                           "2 corrupt tree, 3 tree too deep); the gradient is incomplete")
+
+
 class _NeutralRR(torch.autograd.Function):
     @staticmethod
     def forward(ctx, bare, bvh, box, src, beta, n, seed):
@@ -162,6 +174,7 @@ class _NeutralRR(torch.autograd.Function):
         bvh, box, src, beta, n, seed = ctx.args
         gbare = ops.trace_neutral_rr_adjoint(bvh, box, src, b, beta, n, seed,
                                              gfl.contiguous().float())
+        _check_bvh(bvh, "RR neutral trace adjoint")
         return gbare, None, None, None, None, None, None
 
 
@@ -184,6 +197,7 @@ class _NeutralRRPair(torch.autograd.Function):
         gb = torch.zeros_like(b) if gb is None else gb.contiguous().float()
         g = ops.trace_neutral_rr_pair_adjoint(bvh, box, src, b, beta_a, n_a, seed_a, ga, beta_b,
                                               n_b, seed_b, gb)
+        _check_bvh(bvh, "paired RR neutral trace adjoint")
         return g, None, None, None, None, None, None, None, None, None
 
 
@@ -309,8 +323,9 @@ class _Extend(torch.autograd.Function):
     def backward(ctx, gFs):
         (phi,) = ctx.saved_tensors
         bvh, dx, band = ctx.args
-        return ops.extend_adjoint(bvh, phi, dx, band, gFs.contiguous().float()), None, None, \
-            None, None
+        g = ops.extend_adjoint(bvh, phi, dx, band, gFs.contiguous().float())
+        _check_bvh(bvh, "extension adjoint")
+        return g, None, None, None, None
 
 
 def extend(bvh, V, phi, dx, band):
@@ -350,6 +365,7 @@ class _IonYield(torch.autograd.Function):
     def backward(ctx, gacc):
         bvh, box, src, ion, n, seed, dev, dt = ctx.args
         t = ops.trace_ion_yield_tangent(bvh, box, src, ion, n, seed)
+        _check_bvh(bvh, "ion trace tangent")
         gE = (gacc[:3].double() * t.double()).sum()
         return gE.to(device=dev, dtype=dt), None, None, None, None, None, None
 

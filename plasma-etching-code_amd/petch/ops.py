@@ -525,8 +525,11 @@ def belen_coverage_adjoint(m_i, m_F, m_O, cos_i, ion_yield, chem, gbare):
 def surface_rate_adjoint(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, gV,
                          is_mask=None, per_face_yields=False):
     """Adjoint of surface_rate (petch_surface_rate_adjoint): (gm_F, gm_O, gm_i, g_sp, g_ie,
-    g_ip) for gV = d loss / dV; gm_i is None with ion yields, the g_* are None without. With
-    per_face_yields, a seventh value: (3, F) per-face shares of d loss / d (Yie, Ysp, Yp)."""
+    g_ip) for gV = d loss / dV. With ion yields and the Belen chemistry the rate reads the yields
+    and not m_i, so gm_i is None; without yields the g_* are None. The Langmuir rate reads m_i
+    even when yields are passed (belen.py:34 reads them only for Belen), so there gm_i is the
+    gradient and the g_* are zeros. With per_face_yields, a seventh value: (3, F) per-face
+    shares of d loss / d (Yie, Ysp, Yp)."""
     F = int(m_F.shape[0])
     for t, name in ((m_i, "m_i"), (m_O, "m_O"), (cos_i, "cos_i"), (gV, "gV")):
         if t.numel() < F:
@@ -534,10 +537,12 @@ def surface_rate_adjoint(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, 
     sp, ie, ip = ion_yield if ion_yield is not None else (None, None, None)
     gm_F, gm_O = torch.empty_like(m_F), torch.empty_like(m_F)
     gm_i = g_sp = g_ie = g_ip = None
-    if ion_yield is not None:
+    if ion_yield is not None and chem.chemistry_belen:
         g_sp, g_ie, g_ip = (torch.empty_like(m_F) for _ in range(3))
     else:
         gm_i = torch.empty_like(m_F)
+        if ion_yield is not None:
+            g_sp, g_ie, g_ip = (torch.zeros_like(m_F) for _ in range(3))
     gY = torch.zeros((3, F), dtype=f32, device=m_F.device) if per_face_yields else None
     if F:
         _ctx(m_F).call("petch_surface_rate_adjoint", ptr(m_i, f32), ptr(m_F, f32), ptr(m_O, f32),

@@ -55,3 +55,38 @@ def test_oracle_radiosity_adjoint_is_linear_in_the_output_gradient():
     gD2, gb2 = ofl.radiosity_adjoint(D, Aff, G, bare, 0.7, 2.0 * gm)
     np.testing.assert_allclose(gD2, 2.0 * gD, rtol=1e-12)
     np.testing.assert_allclose(gb2, 2.0 * gb, rtol=1e-12)
+
+
+@pytest.mark.parametrize("model", ["langmuir", "belen"])
+def test_surface_rate_adjoint_with_yields_follows_the_chemistry(monkeypatch, model):
+    """The Langmuir rate reads m_i even when ion yields are passed (only the Belen rate reads the
+    yields, belen.py:34), so its adjoint must hand the library an m_i gradient buffer and return
+    zero yield gradients; with Belen the yields get the gradient and m_i none. The library call
+    is intercepted: this checks which buffers the wrapper passes and returns."""
+    from petch.chemistry import chem_params
+    from petch.params import PAR, Flags
+    calls = []
+
+    class FakeCtx:
+        def call(self, name, *args):
+            calls.append((name, args))
+
+    monkeypatch.setattr(ops, "_ctx", lambda t: FakeCtx())
+    monkeypatch.setattr(ops, "ptr", lambda t, *a, **k: t)
+    F = 7
+    z = torch.zeros(F)
+    chem = chem_params(dict(PAR), Flags(chemistry=model))
+    iy = (torch.ones(F), torch.ones(F), torch.ones(F))
+    gm_F, gm_O, gm_i, g_sp, g_ie, g_ip = ops.surface_rate_adjoint(
+        z, z, z, z, iy, torch.zeros(F, 3), ops.MaskGeom(), chem, torch.ones(F))
+    (name, args), = calls
+    assert name == "petch_surface_rate_adjoint"
+    passed_gm_i, passed_g = args[15], args[16:19]
+    if model == "langmuir":
+        assert gm_i is not None and passed_gm_i is gm_i
+        for g in (g_sp, g_ie, g_ip):
+            assert g is not None and float(g.abs().sum()) == 0.0
+        assert all(p is None or float(p.abs().sum()) == 0.0 for p in passed_g)
+    else:
+        assert gm_i is None and passed_gm_i is None
+        assert all(p is g for p, g in zip(passed_g, (g_sp, g_ie, g_ip)))

@@ -34,17 +34,33 @@ def _setup(name):
     return g, v, f, bvh, box, obox, on.OracleBvh(v, f), dev
 
 
-def _check(gpu_same, ora_same, gpu_b, ora_b, name):
+# Same seed, both sides follow the same Philox streams: per-face sums agree to float rounding
+# (a path that splits on a transcendental's last bit is rare). Measured 1e-8 to 1e-6.
+SAME_SEED_BAR = 1e-5
+N_BATCH = 12
+
+
+def _check(gpu_same, ora_same, gpu_b, ora_b, name, bar=SAME_SEED_BAR):
+    """Same-seed per-face L1 within bar; with independent seeds (N_BATCH batches a side) the
+    per-face z-scores look like Student t with 2 (N_BATCH - 1) degrees of freedom: mean within
+    0.1, fewer than 1.5 % beyond 3 (0.66 % expected), and the total over all faces agrees within
+    3 combined standard errors, which catches a systematic bias far below 1 %."""
     tot = np.abs(ora_same).sum()
     rel = np.abs(gpu_same - ora_same).sum() / max(tot, 1e-30)
     print(f"{name}: same-seed L1 rel diff {rel:.2e}", flush=True)
-    assert rel < 0.03
+    assert rel < bar
     z, ok = zscores(gpu_b, ora_b)
-    frac = np.mean(np.abs(z) > 4.0)
-    print(f"{name}: faces {ok.sum()} |z|>4 frac {frac:.4f} mean z {z.mean():.3f}", flush=True)
+    frac = np.mean(np.abs(z) > 3.0)
+    ga = np.asarray(gpu_b, np.float64).sum(axis=1)
+    oa = np.asarray(ora_b, np.float64).sum(axis=1)
+    se = np.sqrt(ga.var(ddof=1) / len(ga) + oa.var(ddof=1) / len(oa))
+    zt = (ga.mean() - oa.mean()) / max(se, 1e-30)
+    print(f"{name}: faces {ok.sum()} |z|>3 frac {frac:.4f} mean z {z.mean():.3f}; total "
+          f"rel diff {ga.mean() / oa.mean() - 1:+.2e} (z {zt:+.2f})", flush=True)
     assert ok.sum() > 50
-    assert frac < 0.01
-    assert abs(z.mean()) < 0.15
+    assert frac < 0.015
+    assert abs(z.mean()) < 0.1
+    assert abs(zt) < 3.0
 
 
 @pytest.mark.parametrize("kind,sigma,expo", [(0, 0.0, 0.0), (1, np.deg2rad(2.5), 0.0), (2, 0.0, 1000.0)])
@@ -72,8 +88,8 @@ def test_neutral_rr_parity(name):
     print(f"neutral_rr {name} nf={len(f)}", flush=True)
     gs = ops.trace_neutral_rr(bvh, box, src, tb, 0.7, n, 1234).cpu().numpy()
     os_ = obvh.trace_neutral_rr(obox, osrc, bare, 0.7, n, 1234)
-    gb = [ops.trace_neutral_rr(bvh, box, src, tb, 0.7, n, 500 + k).cpu().numpy() for k in range(6)]
-    ob = [obvh.trace_neutral_rr(obox, osrc, bare, 0.7, n, 900 + k) for k in range(6)]
+    gb = [ops.trace_neutral_rr(bvh, box, src, tb, 0.7, n, 500 + k).cpu().numpy() for k in range(N_BATCH)]
+    ob = [obvh.trace_neutral_rr(obox, osrc, bare, 0.7, n, 900 + k) for k in range(N_BATCH)]
     assert bvh.status() == 0
     _check(gs, os_, gb, ob, f"neutral_rr/{name}")
 
@@ -99,7 +115,7 @@ def test_neutral_rr_pair_equals_two_launches(name):
     ob = obvh.trace_neutral_rr(obox, on.Source(0, 0.0, 0.0), bare, 1.0, n_b, 12)
     rel = np.abs(fb - ob).sum() / np.abs(ob).sum()
     print(f"{name}: pair species b vs oracle same-seed L1 rel {rel:.2e}", flush=True)
-    assert rel < 0.03
+    assert rel < SAME_SEED_BAR
     with pytest.raises(Exception):
         ops.trace_neutral_rr_pair(bvh, box, src, tb, 0.7, -1, 11, 1.0, n_b, 12)
 
@@ -116,8 +132,8 @@ def test_ion_yield_parity(name):
     print(f"ion_yield {name} nf={len(f)}", flush=True)
     gs = ops.trace_ion_yield(bvh, box, src, ion, n, 4321).cpu().numpy()
     os_ = obvh.trace_ion_yield(obox, osrc, oion, n, 4321)
-    gb = np.stack([ops.trace_ion_yield(bvh, box, src, ion, n, 600 + k).cpu().numpy() for k in range(6)])
-    ob = np.stack([obvh.trace_ion_yield(obox, osrc, oion, n, 800 + k) for k in range(6)])
+    gb = np.stack([ops.trace_ion_yield(bvh, box, src, ion, n, 600 + k).cpu().numpy() for k in range(N_BATCH)])
+    ob = np.stack([obvh.trace_ion_yield(obox, osrc, oion, n, 800 + k) for k in range(N_BATCH)])
     for ch, nm in enumerate(["sp", "ie", "ip", "arr", "ang"]):
         _check(gs[ch], os_[ch], gb[:, ch], ob[:, ch], f"ion/{name}/{nm}")
 
@@ -135,9 +151,9 @@ def test_legacy_parity(specular):
     gs = ops.trace_legacy(bvh, box, src, st, nre, specular, 0.34, 0.8, n, 11).cpu().numpy()
     os_ = obvh.trace_legacy(obox, osrc, st, nre, specular, 0.34, 0.8, n, 11)
     gb = np.stack([ops.trace_legacy(bvh, box, src, st, nre, specular, 0.34, 0.8, n, 20 + k).cpu().numpy()
-                   for k in range(6)])
+                   for k in range(N_BATCH)])
     ob = np.stack([obvh.trace_legacy(obox, osrc, st, nre, specular, 0.34, 0.8, n, 40 + k)
-                   for k in range(6)])
+                   for k in range(N_BATCH)])
     for ch in range(2):
         _check(gs[ch], os_[ch], gb[:, ch], ob[:, ch], f"legacy{specular}/{ch}")
 
@@ -161,11 +177,13 @@ def test_form_factor_rays(name):
     assert same > 0.98
     # escape fraction per face: independent seeds agree statistically
     esc_g = [(ops.trace_ff(bvh, box, torch.from_numpy(cen32).to(dev), torch.from_numpy(gn).to(dev),
-                           n_ff, 300 + k).cpu().numpy() < 0).reshape(-1, n_ff).mean(1) for k in range(6)]
+                           n_ff, 300 + k).cpu().numpy() < 0).reshape(-1, n_ff).mean(1) for k in range(N_BATCH)]
     esc_o = [(obvh.trace_ff(obox, cen32, gn, n_ff, 700 + k) < 0).reshape(-1, n_ff).mean(1)
-             for k in range(6)]
+             for k in range(N_BATCH)]
     z, okz = zscores(esc_g, esc_o)
-    assert np.mean(np.abs(z) > 4) < 0.01 and abs(z.mean()) < 0.15
+    print(f"ff {name}: escape fraction |z|>3 frac {np.mean(np.abs(z) > 3):.4f} mean z "
+          f"{z.mean():.3f}", flush=True)
+    assert np.mean(np.abs(z) > 3) < 0.015 and abs(z.mean()) < 0.1
 
 
 def test_flat_open_field_unity():
@@ -217,9 +235,9 @@ def test_neutral_rr_path_adjoint_matches_oracle(name):
     gs = ops.trace_neutral_rr_adjoint(bvh, box, src, tb, beta, n, 1234, tg).cpu().numpy()
     os_ = obvh.trace_neutral_rr_adjoint(obox, osrc, bare, beta, n, 1234, gfl.astype(np.float64))
     gb = [ops.trace_neutral_rr_adjoint(bvh, box, src, tb, beta, n, 500 + k, tg).cpu().numpy()
-          for k in range(6)]
+          for k in range(N_BATCH)]
     ob = [obvh.trace_neutral_rr_adjoint(obox, osrc, bare, beta, n, 900 + k,
-                                        gfl.astype(np.float64)) for k in range(6)]
+                                        gfl.astype(np.float64)) for k in range(N_BATCH)]
     assert bvh.status() == 0
     _check(gs, os_, gb, ob, f"neutral_rr_adjoint/{name}")
     h = 1e-2
@@ -230,8 +248,9 @@ def test_neutral_rr_path_adjoint_matches_oracle(name):
         fd = (lp - lm) / (2 * h)
         an = float(gs.astype(np.float64) @ u)
         print(f"{name} direction {k}: frozen-decision fd {fd:.4f} gpu adjoint {an:.4f}", flush=True)
-        # same bound as the same-seed L1 check, on the terms of the dot product
-        assert abs(fd - an) <= 0.03 * float(np.abs(gs * u).sum())
+        # the frozen trace is a polynomial in bare along each path: central differences are
+        # exact to O(h^2); the GPU adjoint is within float rounding of the oracle's
+        assert abs(fd - an) <= 3e-3 * float(np.abs(gs * u).sum())
     x = tb.clone().requires_grad_(True)
     fl = adjoint.neutral_rr(bvh, box, src, x, beta, n, 1234)
     np.testing.assert_allclose(fl.detach().cpu().numpy(),

@@ -111,9 +111,10 @@ def test_mc_all_256_configurations():
                     if (f8[a] > 0) != (f8[b] > 0)}
         assert set(tri.ravel().tolist()) == crossing
         assert len(tri) <= 10
-        # complementary case (solid <-> gas) has the same number of triangles
+        # the complementary case (solid <-> gas) cuts the same edges; its triangle count may
+        # differ, since an ambiguous face joins the other phase
         tri_c = on.mc_cell(-f8)
-        assert len(tri_c) == len(tri) or True
+        assert set(tri_c.ravel().tolist()) == crossing
         seen.add(len(tri))
     assert 0 in seen and max(seen) >= 4
 
