@@ -48,7 +48,8 @@ struct BvhView {
   const BvhWide* wide;
   const float4* tris;
   int nf;
-  int* err;  // device flag: set to 1 on traversal stack overflow (may be null)
+  int* err;           // device flag: set to 1 on traversal stack overflow (may be null)
+  const float* root;  // bounds of all triangles (lo xyz, hi xyz); null below two faces
 };
 
 struct Vec3 {
@@ -176,6 +177,19 @@ __device__ __forceinline__ float slab_entry(const RayPre& r, const Slabs& s, flo
                          fminf(__builtin_fmaf(fz, r.inv.z, s.fz), tbest));
   // tf widened by 2 ulp so rounding never culls a box the ray touches (conservative)
   return tn <= tf * kBoxSlack ? tn : kMiss;
+}
+
+// Whether a ray with 0 < t < 1e6 can meet the box of all the triangles at all: the fused slab
+// test above on the root bounds, as conservative as the tests of the root's children (their
+// union is the same box). A ray it rejects misses every triangle, so a query for it would visit
+// the root node and end in a miss.
+__device__ __forceinline__ bool root_may_hit(const BvhView& b, const RayPre& r) {
+  if (!b.root) return true;
+  const float* q = b.root;
+  const Slabs s = ray_slabs(r);
+  const bool px = r.inv.x >= 0.f, py = r.inv.y >= 0.f, pz = r.inv.z >= 0.f;
+  return slab_entry(r, s, px ? q[0] : q[3], py ? q[1] : q[4], pz ? q[2] : q[5],
+                    px ? q[3] : q[0], py ? q[4] : q[1], pz ? q[5] : q[2], 1.0e6f) < kHitLim;
 }
 
 struct Hit {

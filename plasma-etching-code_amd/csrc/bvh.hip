@@ -675,6 +675,7 @@ BvhView petch_bvh_view(const petch_bvh* b) {
   v.tris = (const float4*)b->tris;
   v.nf = b->nf;
   v.err = b->work ? carve(b->work, b->cap_f).err : nullptr;
+  v.root = (b->work && b->nf > 1) ? carve(b->work, b->cap_f).root : nullptr;
   return v;
 }
 

@@ -160,11 +160,17 @@ constexpr int kMaxWraps = 256;
 // _apply_bc + mesh_query_ray: with periodic lateral faces a ray that misses the surface and
 // leaves through x or y before the open top re-enters at the opposite face (<= 256 wraps, then
 // one last query whose result stands).
+// A query whose ray cannot meet the box of all triangles is skipped (it would miss).
 __device__ __forceinline__ Hit trace_bc(const BvhView& bv, const Box& b, Vec3& o, Vec3 d) {
   if (!b.periodic) return petch::bvh_closest_hit(bv, o, d, 1.0e6f);
   for (int w = 0; w < kMaxWraps; ++w) {
-    Hit h = petch::bvh_closest_hit(bv, o, d, 1.0e6f);
-    if (h.face >= 0 || !wrap_ray(b, o, d)) return h;
+    const petch::RayPre r = petch::ray_pre(o, d);
+    if (petch::root_may_hit(bv, r)) {
+      petch::ScratchStack stk;
+      Hit h = petch::bvh_closest_hit(bv, r, 1.0e6f, stk);
+      if (h.face >= 0) return h;
+    }
+    if (!wrap_ray(b, o, d)) return Hit{-1, -1, 1.0e6f};
   }
   return petch::bvh_closest_hit(bv, o, d, 1.0e6f);
 }
@@ -254,6 +260,27 @@ constexpr int kLdsStack = 8;  // traversal stack entries per lane held in LDS (1
 // (P::miss) before the lane goes idle.
 enum LaneState { kIdle = 0, kTrav = 1, kHit = 2, kMiss = 3 };
 
+// Starts the lane's query for the ray (o, d) with constants rp, returning the lane's state.
+// While the ray cannot meet the box of all triangles (it passes above or beside the surface:
+// an upward re-emission over the mask, a shallow ray crossing the cell), the query could only
+// miss, so the wraps that the miss would cause are applied on the spot without visiting a
+// node, and a ray that escapes ends here: the outcome (hit, wraps, escape) is the one the
+// node-by-node queries give, in one loop iteration instead of one per wrap.
+template <class P>
+__device__ __forceinline__ int begin_query(const P& pol, const typename P::Ray& R, petch::Trav& T,
+                                           petch::RayPre& rp, Vec3& o, Vec3 d, int& wraps) {
+  const petch::BvhView& bv = pol.view(R);
+  while (!petch::root_may_hit(bv, rp)) {
+    const Box& b = pol.box(R);
+    if (!(b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d, rp.inv)))
+      return P::kReplay ? kMiss : kIdle;
+    petch::ray_move(rp, o);
+    ++wraps;
+  }
+  petch::trav_begin(bv, T, rp, 1.0e6f);
+  return kTrav;
+}
+
 template <class P>
 __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q,
                                                  unsigned long long* __restrict__ hits) {
@@ -273,32 +300,37 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
   unsigned pool = 0, pool_end = 0;  // this wavefront's unclaimed queue ids [pool, pool_end)
   while (true) {
     if (__popcll(__ballot(st == kTrav)) <= kRefillAt) {
-      Emit e;
-      bool emit = false;
-      if (st == kHit) {
-        ++nh;
-        emit = pol.hit(R, T.h, o, d, e);
-        st = kIdle;
-      }
-      if constexpr (P::kReplay) {
-        if (st == kMiss) {
-          emit = pol.miss(R, o, e);
+      // a second pass refills the lanes whose new or re-emitted ray ended in begin_query
+#pragma unroll 1
+      for (int pass = 0; pass < 2; ++pass) {
+        Emit e;
+        bool emit = false;
+        if (st == kHit) {
+          ++nh;
+          emit = pol.hit(R, T.h, o, d, e);
           st = kIdle;
         }
+        if constexpr (P::kReplay) {
+          if (st == kMiss) {
+            emit = pol.miss(R, o, e);
+            st = kIdle;
+          }
+        }
+        unsigned ray;
+        if (claim_ray(q, emit || st == kTrav, exhausted, pool, pool_end, ray)) {
+          pol.start(R, ray, o, e);
+          emit = true;
+        }
+        if (emit) {
+          d = emit_dir(e);
+          rp = petch::ray_pre(o, d);
+          wraps = 0;
+          st = begin_query(pol, R, T, rp, o, d, wraps);
+        }
+        if (exhausted || !__ballot(st == kIdle)) break;
       }
-      unsigned ray;
-      if (claim_ray(q, emit || st == kTrav, exhausted, pool, pool_end, ray)) {
-        pol.start(R, ray, o, e);
-        emit = true;
-      }
-      if (emit) {
-        d = emit_dir(e);
-        rp = petch::ray_pre(o, d);
-        wraps = 0;
-        petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
-        st = kTrav;
-      }
-      if (!__ballot(st == kTrav)) break;
+      // every lane is idle and the queue is spent (a lane in kHit or kMiss still has work)
+      if (!__ballot(st != kIdle) && exhausted) break;
     }
     if (st == kTrav && petch::trav_step(pol.view(R), rp, T, stk)) {
       const Box& b = pol.box(R);
@@ -307,7 +339,7 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
       } else if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d, rp.inv)) {
         petch::ray_move(rp, o);
         ++wraps;
-        petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
+        st = begin_query(pol, R, T, rp, o, d, wraps);
       } else {
         st = P::kReplay ? kMiss : kIdle;
       }

@@ -151,3 +151,30 @@ def run_etch_3d_batch(features, t_end=2.0, n_steps=20, reinit_method="fsm", time
         geo['phi'] = _np(lp.phi)
         geo['timings'] = lp.timings
     return geos
+
+
+def seeded_features(n_feat, seed=1234):
+    """The seeded features of the batched workload (run_etch_3d_batch keyword dicts): holes and
+    trenches alternating, diameter U(2,10) um, mask U(1,3) um, dx 0.25, square lateral extent
+    U(12,40) um (48^2..160^2 cells), height U(20,50) um (80..200 cells), ion energy
+    U(50,200) eV, O/F flux ratio U(0.1,1.0); 400k rays per feature and step split 1:1:1
+    (ion : F : O). Feature f draws from the generator seeded seed + f and runs with
+    seed_offset f."""
+    import numpy as np
+    from .api import SF6O2
+    from .params import Flags
+    out = []
+    for f in range(n_feat):
+        r = np.random.default_rng(seed + f)
+        d, mask = r.uniform(2, 10), r.uniform(1, 3)
+        ext = max(round(r.uniform(12, 40) * 4) / 4, round((d + 4) * 4) / 4)
+        Lz = round(r.uniform(20, 50) * 4) / 4
+        m = SF6O2(meanE=float(r.uniform(50, 200)))
+        par = dict(m.par)
+        par['Oflux'] = float(r.uniform(0.1, 1.0)) * par['Fflux']
+        hole = f % 2 == 0
+        par['periodic_y'] = 0 if hole else 1
+        out.append(dict(Lx=ext, Ly=ext, Lz=Lz, dx=0.25, trench_width=d, mask_th=mask,
+                        sub_top=Lz - mask - 2.0, hole=hole, par=par, flags=Flags(**m.flags),
+                        n_ion=133334, n_neu=133333, seed_offset=f))
+    return out

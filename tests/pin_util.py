@@ -57,20 +57,21 @@ def rr_hit_law(S):
     return m1, c, q ** m1 / 0.3, q ** c, T0, Tc
 
 
-def wall_and_floor(theta, Lx, Ly):
+def wall_and_floor(theta, Lx, Ly, n=12):
     """A steep plane whose up-facing normal is theta from the vertical, covering the source
     footprint [0, Lx] x [0, Ly] (vertical ions hit it at incidence theta), and a floor at z = 0
-    from its foot outwards that catches every reflected ion."""
+    from its foot outwards that catches every reflected ion; n x n/2 quads on the wall, n x n on
+    the floor."""
     xb = Lx + 0.05
     cot = np.cos(theta) / np.sin(theta)
     H = (Lx + 0.1) / cot
     foot = (xb, -0.6, 0.0)
-    wall = grid_patch(foot, (-H * cot, 0.0, H), (0.0, Ly + 1.2, 0.0), 12, 6)
+    wall = grid_patch(foot, (-H * cot, 0.0, H), (0.0, Ly + 1.2, 0.0), n, n // 2)
     reach = H * np.tan(np.deg2rad(50)) + 1.0
     side = H * np.tan(np.deg2rad(20)) + 1.0
-    floor = grid_patch((xb, -side, 0.0), (reach, 0.0, 0.0), (0.0, Ly + 2 * side, 0.0), 12, 12)
+    floor = grid_patch((xb, -side, 0.0), (reach, 0.0, 0.0), (0.0, Ly + 2 * side, 0.0), n, n)
     v, f = join(wall, floor)
-    return v, f, 2 * 12 * 6, H
+    return v, f, 2 * n * (n // 2), H
 
 
 def reflected_energy_expectation(g, E, Eref, minE):

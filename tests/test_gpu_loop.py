@@ -61,10 +61,10 @@ def test_coupled_flux_matches_oracle(name, flags_kw):
         for k in ("m_i", "m_F", "m_O", "cos_i", "bare"):
             r = _rel_l1(g[k].cpu().numpy(), o[k])
             print(f"{name} {flags_kw} warm={bare_init is not None} {k}: rel L1 {r:.2e}", flush=True)
-            assert r < 2e-3, k
+            assert r < 5e-4, k
         if o['ion_yield'] is not None:
             for a, b in zip(g['ion_yield'], o['ion_yield']):
-                assert _rel_l1(a.cpu().numpy(), b) < 2e-3
+                assert _rel_l1(a.cpu().numpy(), b) < 5e-4
 
 
 def test_radiosity_flux_matches_oracle():
@@ -79,7 +79,7 @@ def test_radiosity_flux_matches_oracle():
     for k in ("m_i", "m_F", "m_O", "cos_i", "bare"):
         r = _rel_l1(g[k].cpu().numpy(), o[k])
         print(f"radiosity {k}: rel L1 {r:.2e}", flush=True)
-        assert r < 2e-3, k
+        assert r < 1e-5, k
 
 
 def test_uncoupled_flux_matches_oracle():
@@ -93,7 +93,9 @@ def test_uncoupled_flux_matches_oracle():
         g = flux.uncoupled_flux(mesh, geo, par, 30000, 30000, 4, refl)
         o = ofl.uncoupled_flux(M, geo, par, 30000, 30000, 4, refl)
         for k in ("m_i", "m_F", "m_O", "cos_i"):
-            assert _rel_l1(g[k].cpu().numpy(), o[k]) < 2e-3, k
+            r = _rel_l1(g[k].cpu().numpy(), o[k])
+            print(f"uncoupled reflection={refl} {k}: rel L1 {r:.2e}", flush=True)
+            assert r < 5e-4, k
 
 
 SMOKE = dict(Lx=3.0, Ly=0.9, Lz=6.5, dx=0.15, trench_width=0.9, mask_th=0.3, sub_top=6.0,

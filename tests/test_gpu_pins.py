@@ -2,8 +2,14 @@
 closed cavity (E[deposit per ray] = 1 / S), the faithful ion reflected off a steep wall onto a
 floor (the published sticking ramp and reflected-energy law by quadrature), and the RR mean in
 a deep hole against the radiosity solution of the same surface, built from the HIP form-factor
-rays and solved with scipy. Same seeds as the oracle pins, so each case also checks the HIP
-trace against the oracle's per-face sums."""
+rays and solved with scipy. Each case also checks the HIP trace against the oracle's per-face
+sums on the same streams.
+
+The per-face accumulators are float32 sums of float atomic adds, as in the reference
+(threed.py:579,650-654); a sum of k adds carries a relative rounding error of up to about
+k * 2^-24 (ulp(sum) / 2 per add), and with hundreds of thousands of adds on one face that
+reaches 1e-3 and more. The meshes and ray counts here keep each face to a few thousand adds, and
+the bars include that rounding bound."""
 import numpy as np
 import pytest
 from scipy import integrate, stats
@@ -37,30 +43,32 @@ def _rel_l1(a, b):
 def test_rr_closed_cavity_unbiased(S):
     from petch import ops
     Lx, Ly = 2.0, 1.5
-    v, f = closed_box((-0.3, -0.3, 0.0), (Lx + 0.3, Ly + 0.3, 2.8))
+    v, f = closed_box((-0.3, -0.3, 0.0), (Lx + 0.3, Ly + 0.3, 2.8), n=24)
     bvh = ops.Bvh(_dev()).build(_t(v), _t(f))
     box = ops.make_box(Lx, Ly, 3.0, 2.5, 0)
     beta = 0.5
     bare = np.full(len(f), S / beta, np.float32)
-    n = 4_000_000
-    print(f"closed cavity S={S}: {n} rays", flush=True)
+    S32 = float(np.float32(S / beta) * np.float32(beta))     # the sticking the trace applies
+    n = 200_000
+    print(f"closed cavity S={S}: {len(f)} faces, {n} rays", flush=True)
     fl = ops.trace_neutral_rr(bvh, box, ops.make_source(0), _t(bare), beta, n, 77).cpu().numpy()
     assert bvh.status() == 0
-    m1, c, p1, p2, T0, Tc = rr_hit_law(float(np.float32(S / beta) * np.float32(beta)))
+    m1, c, p1, p2, T0, Tc = rr_hit_law(S32)
     EJ = p1 / (1 - p2)
     EJ2 = p1 * (1 + p2) / (1 - p2) ** 2
     mean = T0 + Tc * EJ
-    var = Tc ** 2 * (EJ2 - EJ ** 2)
-    se = np.sqrt(var / n)
+    se = np.sqrt(Tc ** 2 * (EJ2 - EJ ** 2) / n)
+    hits = n * (m1 + c * EJ)                                # expected adds over all faces
+    rounding = hits / len(f) * 2.0 ** -24 * mean            # per-ray bound of the float32 sums
     got = fl.astype(np.float64).sum() / n
-    print(f"  deposit per ray {got:.6f} closed form {mean:.6f} = 1/S {1 / S:.6f} (se {se:.1e})",
-          flush=True)
-    assert mean == pytest.approx(1 / S, rel=1e-9)
-    assert abs(got - mean) < 4 * se + 1e-6 * mean
-    # the same streams as the oracle
+    print(f"  deposit per ray {got:.6f} closed form {mean:.6f} = 1/S {1 / S32:.6f} (se {se:.1e}, "
+          f"float32 rounding bound {rounding:.1e})", flush=True)
+    assert mean == pytest.approx(1 / S32, rel=1e-9)
+    assert abs(got - mean) < 4 * se + rounding
+    # the same streams as the oracle (fewer rays: about 50 adds per face)
     ob = on.OracleBvh(v, f).trace_neutral_rr(on.Box(Lx, Ly, 3.0, 2.5, 0), on.Source(0, 0, 0),
-                                             bare, beta, 200000, 77)
-    gs = ops.trace_neutral_rr(bvh, box, ops.make_source(0), _t(bare), beta, 200000,
+                                             bare, beta, 20000, 77)
+    gs = ops.trace_neutral_rr(bvh, box, ops.make_source(0), _t(bare), beta, 20000,
                               77).cpu().numpy()
     r = _rel_l1(gs, ob)
     print(f"  same-seed L1 vs oracle {r:.2e}", flush=True)
@@ -72,11 +80,11 @@ def test_ion_grazing_reflection_closed_form(deg):
     from petch import ops
     theta = np.deg2rad(deg)
     Lx, Ly = 1.0, 1.0
-    v, f, n_wall, H = wall_and_floor(theta, Lx, Ly)
+    v, f, n_wall, H = wall_and_floor(theta, Lx, Ly, n=48)
     bvh = ops.Bvh(_dev()).build(_t(v), _t(f))
     box = ops.make_box(Lx, Ly, H + 1.0, H + 0.5, 0)
     ip = oc.ion_params(dict(PAR))
-    n = 2_000_000
+    n = 1_000_000
     print(f"grazing ion {deg} deg: {n} ions", flush=True)
     out = ops.trace_ion_yield(bvh, box, ops.make_source(1, 0.0), ops.IonParams(**ip), n,
                               101).cpu().numpy().astype(np.float64)
@@ -99,8 +107,10 @@ def test_ion_grazing_reflection_closed_form(deg):
     def over_E(fn):
         return integrate.quad(lambda E: fn(E) * pdfE(E), *lim, limit=200)[0]
 
-    assert wall[3] == pytest.approx(1.0, rel=1e-6)
-    assert wall[4] == pytest.approx(c, rel=1e-5)
+    # every ion hits the wall once: n / n_wall float32 adds per wall face
+    rounding = n / n_wall * 2.0 ** -24
+    assert wall[3] == pytest.approx(1.0, rel=rounding)
+    assert wall[4] == pytest.approx(c, rel=1e-5 + rounding)
     for ch, (A, Eth, fa) in enumerate([(ip['A_sp'], ip['Eth_sp'], f_sp),
                                       (ip['A_ie'], ip['Eth_ie'], f_ie),
                                       (ip['A_p'], ip['Eth_p'], f_ie)]):
