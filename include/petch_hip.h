@@ -120,6 +120,16 @@ int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const float* F, const
 int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
                      double band, int n_iter);
 
+/* petch_reinit_fsm without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops that forbid hipMalloc). No counterpart in the
+ * reference. The same kernels and results as petch_reinit_fsm; the workspace (U, U2 and the
+ * frozen flags, about 9 bytes per node) is a fixed 64 MiB device array per GPU, resolved when
+ * the context is created, so grids are limited to about 7.4M nodes (PETCH_E_CAPACITY beyond).
+ * Every call on one GPU shares that GPU's workspace: calls that overlap on one GPU (two
+ * streams, two host threads) race and must be serialised by the caller. */
+int petch_reinit_fsm_noalloc(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
+                             double band, int n_iter);
+
 
 /* ---------------------------------------------------------------- a2: marching cubes ----- */
 /* extract_mesh_3d / extract_mesh_3d_gpu (threed.py:62-100): iso-0 surface of phi, vertices in

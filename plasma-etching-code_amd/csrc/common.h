@@ -47,6 +47,10 @@ struct petch_ctx {
   size_t ring_cap[8];
   hipEvent_t ring_ev[8];
   int ring_next;
+  // this device's copy of the fixed workspace of petch_reinit_fsm_noalloc (its device address,
+  // from hipGetSymbolAddress at context creation), or NULL with the error that prevented it
+  void* noalloc_scratch;
+  hipError_t noalloc_err;
 };
 
 struct petch_bvh {
@@ -66,6 +70,10 @@ struct petch_bvh {
 };
 
 int petch_set_error(petch_ctx* ctx, int code, const char* fmt, ...);
+
+// Records in ctx the device address of the current device's fixed no-allocation workspace
+// (levelset.hip); called by petch_ctx_create with the context's device current.
+void petch_noalloc_setup(petch_ctx* ctx);
 
 // Copies `bytes` of host data into the SCR_TABLE device buffer on ctx->stream (stream-ordered:
 // kernels launched afterwards on the stream see it; a later upload waits for them in stream

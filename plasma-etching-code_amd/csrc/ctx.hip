@@ -106,6 +106,7 @@ extern "C" int petch_ctx_create(int device, petch_ctx** out) {
   if (!c) return PETCH_E_HIP;
   c->device = device;
   c->stream = 0;
+  petch_noalloc_setup(c);
   *out = c;
   return PETCH_OK;
 }

@@ -9,6 +9,8 @@
 #include "common.h"
 #include <math.h>
 
+#include <mutex>
+
 namespace {
 
 constexpr int kBS = 256;
@@ -331,18 +333,16 @@ extern "C" int petch_advect_step(petch_ctx* ctx, const float* phi, const float* 
   return PETCH_OK;
 }
 
-extern "C" int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
-                                double band, int n_iter) {
-  PETCH_ARG(ctx, ctx && phi, "reinit_fsm: null pointer");
-  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "reinit_fsm: bad shape %d,%d,%d", nx, ny, nz);
-  PETCH_ARG(ctx, dx > 0 && band > 0, "reinit_fsm: dx and band must be > 0");
-  long long N = (long long)nx * ny * nz;
-  double big = band + 4.0 * dx;
+namespace {
+
+// The reinit's launches over a given workspace: U and U2 (N floats each, ping-pong) and the
+// frozen flags (N bytes). k_eik_init writes every node of U and of the flags before any read,
+// so the workspace needs no clearing.
+int reinit_launch(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx, double band,
+                  int n_iter, float* U, float* U2, uint8_t* fr) {
+  const long long N = (long long)nx * ny * nz;
+  const double big = band + 4.0 * dx;
   if (n_iter < 0) n_iter = (int)ceil(band / dx) + 4;
-  float* U = (float*)petch_scratch(ctx, SCR_LS_A, N * sizeof(float));
-  float* U2 = (float*)petch_scratch(ctx, SCR_LS_B, N * sizeof(float));
-  uint8_t* fr = (uint8_t*)petch_scratch(ctx, SCR_LS_FROZEN, N);
-  if (!U || !U2 || !fr) return PETCH_E_HIP;
   dim3 g(petch_blocks(N, kBS)), b(kBS);
   hipLaunchKernelGGL(k_eik_init, g, b, 0, ctx->stream, phi, U, fr, nx, ny, nz, dx, (float)big);
   PETCH_LAUNCH_CHECK(ctx);
@@ -354,6 +354,92 @@ extern "C" int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int 
   hipLaunchKernelGGL(k_eik_final, g, b, 0, ctx->stream, phi, U, N, (float)(band + dx));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
+}
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of the reinit workspace for N nodes: U, U2 and the flags, each 256-byte aligned.
+size_t reinit_workspace_bytes(long long N) {
+  return 2 * round256((size_t)N * sizeof(float)) + round256((size_t)N);
+}
+
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+struct NoAllocSlot {
+  int state;        // 0 not resolved, 1 resolved, -1 failed
+  void* ptr;
+  hipError_t err;
+};
+NoAllocSlot g_noalloc[kMaxDevices];
+std::mutex g_noalloc_mu;
+
+}  // namespace
+
+// Fixed workspace of petch_reinit_fsm_noalloc: one copy per GPU, allocated when the library's
+// code object is loaded on that GPU and held until the process ends. Host code never uses this
+// name as an address: its device address comes from hipGetSymbolAddress, per device, at context
+// creation (petch_noalloc_setup) and is kept in the context.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Resolves the device address of this GPU's copy of the fixed workspace, once per device, with
+// that device current (petch_ctx_create calls it right after hipSetDevice). A failure is kept
+// and reported by petch_reinit_fsm_noalloc; the other entry points do not need the workspace.
+void petch_noalloc_setup(petch_ctx* ctx) {
+  std::lock_guard<std::mutex> guard(g_noalloc_mu);
+  ctx->noalloc_scratch = nullptr;
+  ctx->noalloc_err = hipErrorInvalidDevice;
+  if (ctx->device >= 0 && ctx->device < kMaxDevices) {
+    NoAllocSlot& s = g_noalloc[ctx->device];
+    if (s.state == 0) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      s.state = (e == hipSuccess && p) ? 1 : -1;
+      s.ptr = (e == hipSuccess) ? p : nullptr;
+      s.err = (e == hipSuccess && !p) ? hipErrorInvalidSymbol : e;
+    }
+    ctx->noalloc_scratch = s.ptr;
+    ctx->noalloc_err = s.err;
+  }
+}
+
+extern "C" int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
+                                double band, int n_iter) {
+  PETCH_ARG(ctx, ctx && phi, "reinit_fsm: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "reinit_fsm: bad shape %d,%d,%d", nx, ny, nz);
+  PETCH_ARG(ctx, dx > 0 && band > 0, "reinit_fsm: dx and band must be > 0");
+  long long N = (long long)nx * ny * nz;
+  float* U = (float*)petch_scratch(ctx, SCR_LS_A, N * sizeof(float));
+  float* U2 = (float*)petch_scratch(ctx, SCR_LS_B, N * sizeof(float));
+  uint8_t* fr = (uint8_t*)petch_scratch(ctx, SCR_LS_FROZEN, N);
+  if (!U || !U2 || !fr) return PETCH_E_HIP;
+  return reinit_launch(ctx, phi, nx, ny, nz, dx, band, n_iter, U, U2, fr);
+}
+
+extern "C" int petch_reinit_fsm_noalloc(petch_ctx* ctx, float* phi, int nx, int ny, int nz,
+                                        double dx, double band, int n_iter) {
+  PETCH_ARG(ctx, ctx && phi, "reinit_fsm_noalloc: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "reinit_fsm_noalloc: bad shape %d,%d,%d", nx, ny,
+            nz);
+  PETCH_ARG(ctx, dx > 0 && band > 0, "reinit_fsm_noalloc: dx and band must be > 0");
+  const long long N = (long long)nx * ny * nz;
+  const size_t need = reinit_workspace_bytes(N);
+  if (need > kNoAllocScratchBytes)
+    return petch_set_error(ctx, PETCH_E_CAPACITY,
+                           "reinit_fsm_noalloc: a %dx%dx%d grid needs %zu workspace bytes; the "
+                           "fixed workspace holds %zu (grids up to about %lld nodes); use "
+                           "petch_reinit_fsm",
+                           nx, ny, nz, need, kNoAllocScratchBytes,
+                           (long long)(kNoAllocScratchBytes / 9));
+  if (!ctx->noalloc_scratch)
+    return petch_set_error(ctx, PETCH_E_HIP,
+                           "reinit_fsm_noalloc: the fixed workspace of device %d is unavailable "
+                           "(hipGetSymbolAddress: %s)",
+                           ctx->device, hipGetErrorString(ctx->noalloc_err));
+  unsigned char* base = (unsigned char*)ctx->noalloc_scratch;
+  float* U = (float*)base;
+  float* U2 = (float*)(base + round256((size_t)N * sizeof(float)));
+  uint8_t* fr = base + 2 * round256((size_t)N * sizeof(float));
+  return reinit_launch(ctx, phi, nx, ny, nz, dx, band, n_iter, U, U2, fr);
 }
 
 extern "C" int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const float* F,

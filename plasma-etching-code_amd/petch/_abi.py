@@ -117,6 +117,7 @@ SIGNATURES = {
                        c_double, c_double, c_int],
     "petch_advect": [P, P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, c_int],
     "petch_reinit_fsm": [P, P, c_int, c_int, c_int, c_double, c_double, c_int],
+    "petch_reinit_fsm_noalloc": [P, P, c_int, c_int, c_int, c_double, c_double, c_int],
     "petch_marching_cubes": [P, P, c_int, c_int, c_int, c_double, P, P, c_int, c_int,
                              POINTER(c_int), POINTER(c_int)],
     "petch_mesh_attrs": [P, P, P, c_int, P, P, P],
