@@ -78,6 +78,15 @@ int petch_ctx_destroy(petch_ctx* ctx);
 /* `stream` is a hipStream_t (torch.cuda.current_stream().cuda_stream); 0 = null stream. */
 int petch_set_stream(petch_ctx* ctx, void* stream);
 int petch_synchronize(petch_ctx* ctx);
+/* Deterministic mode of the context's ray traces (on = 1): the per-face accumulators of
+ * petch_trace_neutral_rr[_pair|_batch] and petch_trace_ion_yield[_batch] are summed as 64-bit
+ * integers in units of 2^-32 instead of float atomics, then converted to float, so a trace's
+ * output no longer depends on the order in which the GPU schedules its rays: the same inputs
+ * give bit-identical outputs, and with them a whole etch run repeats exactly. The sums are the
+ * exact sums of the deposits rounded to 2^-32 (the float path rounds every add to float). No
+ * reference counterpart (the reference accumulates with float atomics, threed.py:579,650-654).
+ * Default off. */
+int petch_set_deterministic(petch_ctx* ctx, int on);
 const char* petch_last_error(petch_ctx* ctx);
 int petch_version(void);
 
@@ -113,12 +122,49 @@ int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const float* F, const
                          const float* gout, int nx, int ny, int nz, double dx, double dt,
                          float* gphi, float* gF);
 
+/* The substeps of petch_advect for many grids at once (the features of a batched step, no
+ * reference counterpart: the reference advances one feature per run_etch_3d): one launch per
+ * substep index for all grids (a grid with fewer substeps sits out), then one copy-back launch
+ * for the grids whose last substep ended in `work`. Each item's result equals petch_advect's. */
+typedef struct petch_advect_item {
+  float* phi;
+  float* work;               /* a grid-sized buffer, distinct from phi */
+  const float* F;
+  const uint8_t* mask;
+  const float* mask_phi;
+  int nx, ny, nz;
+  double dx, dt;             /* dt is the substep's step */
+  int nsub;
+} petch_advect_item;
+int petch_advect_batch(petch_ctx* ctx, int n, const petch_advect_item* items_host);
+
 /* ---------------------------------------------------------------- a20: FSM reinit -------- */
 /* reinit_fsm (threed.py:1515-1537) with _eik_init / _eik_jacobi / _eik_solve
  * (threed.py:1421-1512): big = band+4dx, n_iter = ceil(band/dx)+4 when n_iter < 0, output
  * sign(phi0)*min(U, band+dx) with sign(0) = +1. phi is replaced in place. */
 int petch_reinit_fsm(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
                      double band, int n_iter);
+
+/* petch_reinit_fsm for many grids at once (the features of a batched step; no reference
+ * counterpart): one launch per stage (init, each Jacobi sweep, final) for all grids, each grid
+ * with its own dx, band and sweep count (n_iter < 0: ceil(band/dx)+4). Each item's result
+ * equals petch_reinit_fsm's. */
+typedef struct petch_reinit_item {
+  float* phi;
+  int nx, ny, nz;
+  double dx, band;
+  int n_iter;
+} petch_reinit_item;
+int petch_reinit_fsm_batch(petch_ctx* ctx, int n, const petch_reinit_item* items_host);
+
+/* Adjoint of petch_reinit_fsm (no reference counterpart: the reference's etch loop is not
+ * differentiated): gphi = d loss / d phi0 for gout = d loss / d (reinit output), phi0 the input
+ * level set. The exact adjoint of the piecewise-smooth map the reinit computes, every discrete
+ * choice (interface flags, the smaller neighbour per axis, the active Godunov terms, whether a
+ * sweep lowers a node, the clamp at band + dx) held at the forward's; the sweep states are
+ * rebuilt with the forward's kernels (workspace (n_iter + 1) * 4 + 25 bytes per node). */
+int petch_reinit_fsm_adjoint(petch_ctx* ctx, const float* phi0, int nx, int ny, int nz,
+                             double dx, double band, int n_iter, const float* gout, float* gphi);
 
 /* petch_reinit_fsm without any allocation, for callers that cannot allocate device memory
  * (code under HIP graph capture, serving loops that forbid hipMalloc). No counterpart in the
@@ -181,6 +227,13 @@ int petch_bvh_create(petch_ctx* ctx, petch_bvh** out);
 int petch_bvh_destroy(petch_ctx* ctx, petch_bvh* bvh);
 int petch_bvh_build(petch_ctx* ctx, petch_bvh* bvh, const float* verts, const int* faces, int nv,
                     int nf);
+/* petch_bvh_build of many meshes (the features of a batched step; no reference counterpart):
+ * each tree's parallel stages are launched per tree, the bottom-up refits of all trees run in
+ * one launch (a refit is a chain of dependent accesses, tens of microseconds for any tree).
+ * Host arrays of n entries; each BVH equals what petch_bvh_build gives. */
+int petch_bvh_build_batch(petch_ctx* ctx, int n, petch_bvh* const* bvhs_host,
+                          const float* const* verts_host, const int* const* faces_host,
+                          const int* nv_host, const int* nf_host);
 /* Build quality of the next petch_bvh_build calls: 0 Morton tree collapsed to 4-wide nodes by a
  * fixed rule (fastest build), 1 Morton tree with an area-driven collapse, 2 clustered tree
  * (PLOC) with the area-driven collapse (slowest build, fastest traces), -1 (default) 2 for
@@ -445,6 +498,19 @@ int petch_radiosity_solve(petch_ctx* ctx, const int* row_ptr, const int* col, co
 int petch_radiosity_adjoint(petch_ctx* ctx, const int* hitface, int nf, int n_ff, const double* G,
                             const float* bare, double beta, const float* gm, int n_iter,
                             double tol, double* gD, float* gbare, int* iters_host);
+
+/* ---------------------------------------------------------------- profile r(z) ------------- */
+/* Smoothed gas area of every z slice of phi: area[k] = dx^2 * sum over the slice's nodes of
+ * H_eps(-phi), H_eps(x) = 0 below -eps, 1 above eps, (1 + x/eps + sin(pi x/eps)/pi)/2 between;
+ * float64 (nz), a fixed-order reduction (deterministic). The equivalent radius of a hole is
+ * sqrt(area / pi), the profile r(z) a profile loss compares with a target. No counterpart in the
+ * reference, whose depth metrics (threed.py:1798-1869) are whole-cell column floors. */
+int petch_profile_area(petch_ctx* ctx, const float* phi, int nx, int ny, int nz, double dx,
+                       double eps, double* area);
+/* Adjoint of petch_profile_area: gphi = -garea[k] dx^2 H_eps'(-phi) (float32, phi's shape) for
+ * garea = d loss / d area (float64, nz). gphi must not alias phi. */
+int petch_profile_area_adjoint(petch_ctx* ctx, const float* phi, int nx, int ny, int nz,
+                               double dx, double eps, const double* garea, float* gphi);
 
 #ifdef __cplusplus
 }

@@ -9,6 +9,7 @@ Restates, from the reference src/petch/threed.py:
   eik_init      _eik_init                 threed.py:1455-1479
   eik_jacobi    _eik_jacobi               threed.py:1482-1512
   reinit_fsm    reinit_fsm                threed.py:1515-1537
+  reinit_fsm_adjoint (no reference counterpart: its adjoint, pinned by central differences)
   depth3d / max_depth_3d                  threed.py:1798-1869
 """
 import numpy as np
@@ -213,6 +214,97 @@ def reinit_fsm(phi, dx, band, n_iter=None):
     return sgn * np.minimum(U, band + dx)
 
 
+def _eik_weights(a, U, dx):
+    """Godunov root x of sum_d (x - a_d)_+^2 = dx^2 at the three axis minima a (3, ...) and its
+    derivative in each: dx/da_d = (x - a_d) / sum_active (x - a_e) on the active terms (the
+    implicit function theorem on the active equation), 0 on the others."""
+    x = eik_solve(a[0], a[1], a[2], dx)
+    act = a < x[None]
+    r = np.where(act, x[None] - a, 0.0)
+    den = r.sum(axis=0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        w = np.where(den[None] > 0.0, r / den[None], 0.0)
+    return x, w
+
+
+def reinit_fsm_adjoint(phi, dx, band, gout, n_iter=None):
+    """d loss / d phi of reinit_fsm(phi, dx, band) for d loss / d out = gout: the exact adjoint
+    of the piecewise-smooth map the sweeps compute, with every discrete choice (interface flags,
+    the smaller neighbour per axis, the active Godunov terms, whether a sweep lowers a node,
+    the clamp at band + dx) held at the forward's. Each sweep's state is recomputed and the
+    sweeps are transposed in reverse: a node lowered by sweep k took eik_solve of its upwind
+    neighbours' values at k, whose derivative is the Godunov weight; the frozen seeds
+    |phi|/max(|grad phi|_central, 0.5) pass their derivative to the node and its six
+    neighbours. Not in the reference, which differentiates nothing in its etch loop; pinned by
+    central differences of reinit_fsm."""
+    c = np.asarray(phi, np.float64)
+    big = float(band + 4.0 * dx)
+    if n_iter is None:
+        n_iter = int(np.ceil(band / dx)) + 4
+    U, frozen = eik_init(c, dx, big)
+    states = [U]
+    for _ in range(n_iter):
+        U = eik_jacobi(U, frozen, dx, big)
+        states.append(U)
+    sgn = np.where(np.sign(c) == 0, 1.0, np.sign(c))
+    G = np.where(states[-1] < band + dx, np.asarray(gout, np.float64) * sgn, 0.0)
+    for k in range(n_iter - 1, -1, -1):
+        Uk = states[k]
+        lo = [_nbr(Uk, ax, -1, big) for ax in range(3)]
+        hi = [_nbr(Uk, ax, +1, big) for ax in range(3)]
+        a = np.stack([np.minimum(lo[ax], hi[ax]) for ax in range(3)])
+        x, w = _eik_weights(a, Uk, dx)
+        took = ~frozen & (x < Uk)
+        Gt = np.where(took, G, 0.0)
+        Gn = np.where(took, 0.0, G)
+        for ax in range(3):
+            # the smaller neighbour (the lower side on a tie) receives w * G
+            use_lo = lo[ax] <= hi[ax]
+            contrib = w[ax] * Gt
+            to_lo = np.where(use_lo, contrib, 0.0)
+            to_hi = np.where(use_lo, 0.0, contrib)
+            src = [slice(None)] * 3
+            dst = [slice(None)] * 3
+            src[ax], dst[ax] = slice(1, None), slice(0, -1)     # node n sends to n - 1
+            Gn[tuple(dst)] += to_lo[tuple(src)]
+            src[ax], dst[ax] = slice(0, -1), slice(1, None)     # node n sends to n + 1
+            Gn[tuple(dst)] += to_hi[tuple(src)]
+        G = Gn
+    # the frozen seeds D = min(|c| / max(|g|, 0.5), 1.8 dx), g the central gradient with the
+    # index clamped at the grid edge (a clamped neighbour is the node itself)
+    o = [(_clamped(c, ax, -1), _clamped(c, ax, +1)) for ax in range(3)]
+    g = [(p - m) * 0.5 / dx for m, p in o]
+    gn = np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2)
+    grad0 = np.maximum(gn, 0.5)
+    D = np.abs(c) / grad0
+    live = frozen & (D < 1.8 * dx)
+    gD = np.where(live, G, 0.0)
+    gc = gD * np.sign(c) / grad0
+    # through |g|: dD/dg_ax = -|c| g_ax / (grad0^2 |g|) where |g| > 0.5
+    steep = live & (gn > 0.5)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        coef = np.where(steep, -gD * np.abs(c) / (grad0 * grad0 * gn), 0.0)
+    for ax in range(3):
+        t = coef * g[ax] * 0.5 / dx          # d/dc_{n+1}: +t, d/dc_{n-1}: -t
+        n_ax = c.shape[ax]
+        idx = np.arange(n_ax)
+        shp = [1, 1, 1]
+        shp[ax] = n_ax
+        at_hi = (idx == n_ax - 1).reshape(shp)
+        at_lo = (idx == 0).reshape(shp)
+        # the +1 neighbour (the node itself at the upper edge)
+        gc += np.where(at_hi, t, 0.0)
+        src = [slice(None)] * 3
+        dst = [slice(None)] * 3
+        src[ax], dst[ax] = slice(0, -1), slice(1, None)
+        gc[tuple(dst)] += np.where(at_hi, 0.0, t)[tuple(src)]
+        # the -1 neighbour (the node itself at the lower edge)
+        gc -= np.where(at_lo, t, 0.0)
+        src[ax], dst[ax] = slice(1, None), slice(0, -1)
+        gc[tuple(dst)] -= np.where(at_lo, 0.0, t)[tuple(src)]
+    return gc
+
+
 # ----------------------------------------------------------------------------- depth metrics
 def _floor_z(col, zs):
     """Lowest z of the gas run that reaches the top (threed.py:1824-1831)."""
@@ -269,3 +361,30 @@ def max_depth_3d(geo):
     if not floors:
         return 0.0
     return float(geo['sub_top'] - np.percentile(floors, 5))
+
+
+# ----------------------------------------------------------------------------- profile r(z)
+def smooth_heaviside(x, eps):
+    """0 below -eps, 1 above eps, (1 + x/eps + sin(pi x/eps)/pi)/2 between (C1-smooth)."""
+    x = np.asarray(x, np.float64)
+    mid = 0.5 * (1.0 + x / eps + np.sin(np.pi * x / eps) / np.pi)
+    return np.where(x <= -eps, 0.0, np.where(x >= eps, 1.0, mid))
+
+
+def slice_gas_area(phi, dx, eps):
+    """A(z) = dx^2 * sum over each z slice of H_eps(-phi): the smoothed gas cross-section area.
+    Not in the reference (whose depth metrics are whole-cell column floors); the profile the
+    differentiable configuration's loss is written on, r(z) = sqrt(A(z) / pi) for a hole."""
+    return smooth_heaviside(-np.asarray(phi, np.float64), eps).sum(axis=(0, 1)) * dx * dx
+
+
+def slice_gas_area_adjoint(phi, dx, eps, garea):
+    """d loss / d phi of slice_gas_area for garea = d loss / dA (one value per z slice)."""
+    x = -np.asarray(phi, np.float64)
+    dH = np.where(np.abs(x) < eps, 0.5 * (1.0 + np.cos(np.pi * x / eps)) / eps, 0.0)
+    return -np.asarray(garea, np.float64)[None, None, :] * dx * dx * dH
+
+
+def profile_rz(phi, dx, eps):
+    """Equivalent radius sqrt(A(z) / pi) of every z slice."""
+    return np.sqrt(np.maximum(slice_gas_area(phi, dx, eps), 0.0) / np.pi)

@@ -12,6 +12,8 @@
 //            grandchildren (one parallel pass)
 // plus the two query entry points used by the tests (the trace kernels inline the traversal).
 #include "common.h"
+
+#include <stdlib.h>
 #include "bvh.h"
 #include "scan.h"
 
@@ -207,11 +209,12 @@ __device__ __forceinline__ void put_child_box(BvhNode* nd, bool left, const floa
   }
 }
 
-__global__ void k_bvh_refit(const float4* __restrict__ tris, BvhNode* nodes,
-                            const int* __restrict__ pint, const int* __restrict__ pleaf,
-                            int* flags, int n, float* root) {
-  int s = blockIdx.x * kBS + threadIdx.x;
-  if (s >= n) return;
+// Leaf s climbs towards the root; at each node the second arriving thread (agent-scope acq_rel
+// flag) writes the node's box from both children and goes on, the first one stops.
+__device__ __forceinline__ void refit_leaf(const float4* __restrict__ tris, BvhNode* nodes,
+                                           const int* __restrict__ pint,
+                                           const int* __restrict__ pleaf, int* flags, int n,
+                                           float* root, int s) {
   float4 a = tris[3 * s], b = tris[3 * s + 1], c = tris[3 * s + 2];
   float lo[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z))};
   float hi[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z))};
@@ -235,6 +238,39 @@ __global__ void k_bvh_refit(const float4* __restrict__ tris, BvhNode* nodes,
     root[k] = lo[k];
     root[3 + k] = hi[k];
   }
+}
+
+__global__ void k_bvh_refit(const float4* __restrict__ tris, BvhNode* nodes,
+                            const int* __restrict__ pint, const int* __restrict__ pleaf,
+                            int* flags, int n, float* root) {
+  int s = blockIdx.x * kBS + threadIdx.x;
+  if (s >= n) return;
+  refit_leaf(tris, nodes, pint, pleaf, flags, n, root, s);
+}
+
+// The refits of many BVHs in one launch: the climbs are chains of dependent global accesses
+// (tens of microseconds for one tree however small), so the trees of a batched step refit side
+// by side instead of one after the other. Tree t owns the workgroups [blk0[t], blk0[t+1]).
+struct RefitItem {
+  const float4* tris;
+  BvhNode* nodes;
+  const int *pint, *pleaf;
+  int* flags;
+  float* root;
+  int n;
+  int blk0;
+};
+
+__global__ void k_bvh_refit_many(const RefitItem* __restrict__ items, int n_items) {
+  int lo = 0, hi = n_items - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].blk0 <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const RefitItem& it = items[lo];
+  const int s = ((int)blockIdx.x - it.blk0) * kBS + threadIdx.x;
+  if (s >= it.n) return;
+  refit_leaf(it.tris, it.nodes, it.pint, it.pleaf, it.flags, it.n, it.root, s);
 }
 
 __global__ void k_bvh_wide(const BvhNode* __restrict__ nodes, const int* __restrict__ pint, int n,
@@ -759,13 +795,18 @@ int ploc_build(petch_ctx* ctx, petch_bvh* b, int nf) {
 
 }  // namespace
 
-extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts, const int* faces,
-                               int nv, int nf) {
+namespace {
+
+// Everything of a build up to the refit: buffers, centroids, Morton sort, leaves, Karras nodes
+// and the cleared refit flags. Returns the refit's item (n = nf; nothing to refit when nf < 2).
+int build_prepare(petch_ctx* ctx, petch_bvh* b, const float* verts, const int* faces, int nv,
+                  int nf, RefitItem* refit) {
   PETCH_ARG(ctx, ctx && b, "bvh_build: null pointer");
   PETCH_ARG(ctx, nf >= 0 && nv >= 0, "bvh_build: negative size");
   PETCH_ARG(ctx, nf == 0 || (verts && faces), "bvh_build: null mesh");
   b->nv = nv;
   b->nf = 0;
+  refit->n = 0;
   if (nf > b->cap_f) {
     PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (b->nodes) (void)hipFree(b->nodes);
@@ -810,9 +851,17 @@ extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts,
                        w.keys, nf, (BvhNode*)b->nodes, w.pint, w.pleaf);
     PETCH_LAUNCH_CHECK(ctx);
     PETCH_HIP(ctx, hipMemsetAsync(w.flags, 0, (size_t)nf * sizeof(int), ctx->stream));
-    hipLaunchKernelGGL(k_bvh_refit, dim3(g), dim3(kBS), 0, ctx->stream, (const float4*)b->tris,
-                       (BvhNode*)b->nodes, w.pint, w.pleaf, w.flags, nf, w.root);
-    PETCH_LAUNCH_CHECK(ctx);
+    *refit = RefitItem{(const float4*)b->tris, (BvhNode*)b->nodes, w.pint, w.pleaf, w.flags,
+                       w.root, nf, 0};
+  }
+  return PETCH_OK;
+}
+
+// The rest of a build after the refit: the clustered tree where the quality asks for it, then
+// the 4-wide nodes the ray queries walk.
+int build_finish(petch_ctx* ctx, petch_bvh* b, int nf) {
+  if (nf > 1) {
+    Work w = carve(b->work, b->cap_f);
     const int q = b->quality >= 0 ? b->quality : (nf >= kWideSahMinFaces ? 2 : 0);
     if (q >= 2) {
       int rc2 = ploc_build(ctx, b, nf);
@@ -829,6 +878,68 @@ extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts,
     PETCH_LAUNCH_CHECK(ctx);
   }
   b->nf = nf;
+  return PETCH_OK;
+}
+
+}  // namespace
+
+extern "C" int petch_bvh_build(petch_ctx* ctx, petch_bvh* b, const float* verts, const int* faces,
+                               int nv, int nf) {
+  RefitItem r;
+  int rc = build_prepare(ctx, b, verts, faces, nv, nf, &r);
+  if (rc != PETCH_OK) return rc;
+  if (r.n > 1) {
+    hipLaunchKernelGGL(k_bvh_refit, dim3(petch_blocks(r.n, kBS)), dim3(kBS), 0, ctx->stream,
+                       r.tris, r.nodes, r.pint, r.pleaf, r.flags, r.n, r.root);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  return build_finish(ctx, b, nf);
+}
+
+extern "C" int petch_bvh_build_batch(petch_ctx* ctx, int n, petch_bvh* const* bvhs,
+                                     const float* const* verts, const int* const* faces,
+                                     const int* nv, const int* nf) {
+  PETCH_ARG(ctx, ctx && (n == 0 || (bvhs && verts && faces && nv && nf)),
+            "bvh_build_batch: null pointer");
+  PETCH_ARG(ctx, n >= 0, "bvh_build_batch: n < 0");
+  if (n == 0) return PETCH_OK;
+  RefitItem* items = (RefitItem*)calloc((size_t)n, sizeof(RefitItem));
+  if (!items) return petch_set_error(ctx, PETCH_E_HIP, "bvh_build_batch: out of host memory");
+  int m = 0;
+  long long blocks = 0;
+  for (int t = 0; t < n; ++t) {
+    RefitItem r;
+    const int rc = build_prepare(ctx, bvhs[t], verts[t], faces[t], nv[t], nf[t], &r);
+    if (rc != PETCH_OK) {
+      free(items);
+      return rc;
+    }
+    if (r.n > 1) {
+      r.blk0 = (int)blocks;
+      blocks += petch_blocks(r.n, kBS);
+      items[m++] = r;
+    }
+  }
+  if (m > 0) {
+    const RefitItem* d = (const RefitItem*)petch_upload_table(ctx, items,
+                                                              (size_t)m * sizeof(RefitItem));
+    if (!d) {
+      free(items);
+      return PETCH_E_HIP;
+    }
+    hipLaunchKernelGGL(k_bvh_refit_many, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d, m);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      free(items);
+      return petch_set_error(ctx, PETCH_E_HIP, "bvh_build_batch: refit launch: %s",
+                             hipGetErrorString(e));
+    }
+  }
+  free(items);
+  for (int t = 0; t < n; ++t) {
+    const int rc = build_finish(ctx, bvhs[t], nf[t]);
+    if (rc != PETCH_OK) return rc;
+  }
   return PETCH_OK;
 }
 

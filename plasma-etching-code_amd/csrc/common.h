@@ -30,6 +30,10 @@ enum PetchScratch {
   SCR_RAD_C,
   SCR_SMOOTH,        // flux smoothing ping-pong
   SCR_TABLE,         // per-launch descriptor tables (batched traces)
+  SCR_DET,           // int64 fixed-point accumulators of the deterministic mode
+  SCR_EIK_STATES,    // reinit adjoint: every sweep's Eikonal state
+  SCR_EIK_W,         // reinit adjoint: per-node Godunov weights of one sweep
+  SCR_EIK_G,         // reinit adjoint: adjoint ping-pong
   SCR_COUNT
 };
 
@@ -51,6 +55,7 @@ struct petch_ctx {
   // from hipGetSymbolAddress at context creation), or NULL with the error that prevented it
   void* noalloc_scratch;
   hipError_t noalloc_err;
+  int deterministic;       // traces accumulate in int64 fixed point (petch_set_deterministic)
 };
 
 struct petch_bvh {

@@ -132,6 +132,12 @@ extern "C" int petch_set_stream(petch_ctx* ctx, void* stream) {
   return PETCH_OK;
 }
 
+extern "C" int petch_set_deterministic(petch_ctx* ctx, int on) {
+  if (!ctx) return PETCH_E_ARG;
+  ctx->deterministic = on ? 1 : 0;
+  return PETCH_OK;
+}
+
 extern "C" int petch_synchronize(petch_ctx* ctx) {
   if (!ctx) return PETCH_E_ARG;
   PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -8,6 +8,7 @@
 // and the three neighbour planes are re-read from L2/MALL, not HBM.
 #include "common.h"
 #include <math.h>
+#include <stdlib.h>
 
 #include <mutex>
 
@@ -70,13 +71,13 @@ __global__ void k_init_sdf(float* __restrict__ phi, uint8_t* __restrict__ mask, 
 }
 
 // ------------------------------------------------------------------------------ advection
-__global__ void k_advect(const float* __restrict__ phi, const float* __restrict__ F,
-                         const uint8_t* __restrict__ mask, const float* __restrict__ mask_phi,
-                         float* __restrict__ out, int nx, int ny, int nz, float inv_dx,
-                         float dt) {
-  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long N = (long long)nx * ny * nz;
-  if (n >= N) return;
+// One upwind substep at node n of an (nx, ny, nz) grid.
+__device__ __forceinline__ void advect_node(const float* __restrict__ phi,
+                                            const float* __restrict__ F,
+                                            const uint8_t* __restrict__ mask,
+                                            const float* __restrict__ mask_phi,
+                                            float* __restrict__ out, int nx, int ny, int nz,
+                                            float inv_dx, float dt, long long n) {
   if (mask[n]) {
     out[n] = mask_phi[n];
     return;
@@ -97,6 +98,16 @@ __global__ void k_advect(const float* __restrict__ phi, const float* __restrict_
   dp = (k <= nz - 2) ? (phi[n + 1] - c) * inv_dx : 0.f;
   g += fmaxf(dm, 0.f) * fmaxf(dm, 0.f) + fminf(dp, 0.f) * fminf(dp, 0.f);
   out[n] = c - dt * F[n] * sqrtf(g);
+}
+
+__global__ void k_advect(const float* __restrict__ phi, const float* __restrict__ F,
+                         const uint8_t* __restrict__ mask, const float* __restrict__ mask_phi,
+                         float* __restrict__ out, int nx, int ny, int nz, float inv_dx,
+                         float dt) {
+  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long N = (long long)nx * ny * nz;
+  if (n >= N) return;
+  advect_node(phi, F, mask, mask_phi, out, nx, ny, nz, inv_dx, dt, n);
 }
 
 // ------------------------------------------------------------------------------ advection adjoint
@@ -218,12 +229,10 @@ __device__ __forceinline__ float eik_solve(float f1, float f2, float f3, double 
 // Frozen interface seed: nodes with a sign change to a face neighbour get the sub-cell distance
 // |phi0| / max(|grad phi0|_central, 0.5), capped at 1.8 dx; every other node starts at `big`.
 // Evaluated in float64 from the float32 inputs, rounded once.
-__global__ void k_eik_init(const float* __restrict__ phi0, float* __restrict__ U,
-                           uint8_t* __restrict__ frozen, int nx, int ny, int nz, double dx,
-                           float big) {
-  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long N = (long long)nx * ny * nz;
-  if (n >= N) return;
+__device__ __forceinline__ void eik_init_node(const float* __restrict__ phi0,
+                                              float* __restrict__ U,
+                                              uint8_t* __restrict__ frozen, int nx, int ny,
+                                              int nz, double dx, float big, long long n) {
   int k = (int)(n % nz);
   int j = (int)((n / nz) % ny);
   int i = (int)(n / ((long long)nz * ny));
@@ -248,12 +257,19 @@ __global__ void k_eik_init(const float* __restrict__ phi0, float* __restrict__ U
   }
 }
 
-__global__ void k_eik_jacobi(const float* __restrict__ U, const uint8_t* __restrict__ frozen,
-                             float* __restrict__ out, int nx, int ny, int nz, double dx,
-                             float big) {
+__global__ void k_eik_init(const float* __restrict__ phi0, float* __restrict__ U,
+                           uint8_t* __restrict__ frozen, int nx, int ny, int nz, double dx,
+                           float big) {
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long N = (long long)nx * ny * nz;
   if (n >= N) return;
+  eik_init_node(phi0, U, frozen, nx, ny, nz, dx, big, n);
+}
+
+__device__ __forceinline__ void eik_jacobi_node(const float* __restrict__ U,
+                                                const uint8_t* __restrict__ frozen,
+                                                float* __restrict__ out, int nx, int ny, int nz,
+                                                double dx, float big, long long n) {
   float u = U[n];
   if (frozen[n]) {
     out[n] = u;
@@ -269,15 +285,105 @@ __global__ void k_eik_jacobi(const float* __restrict__ U, const uint8_t* __restr
   out[n] = fminf(u, eik_solve(ax, ay, az, dx));
 }
 
-// phi <- sign(phi0) * min(U, clampv), sign(0) = +1, NaN propagates.
-__global__ void k_eik_final(float* __restrict__ phi, const float* __restrict__ U, long long N,
-                            float clampv) {
+__global__ void k_eik_jacobi(const float* __restrict__ U, const uint8_t* __restrict__ frozen,
+                             float* __restrict__ out, int nx, int ny, int nz, double dx,
+                             float big) {
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long N = (long long)nx * ny * nz;
   if (n >= N) return;
+  eik_jacobi_node(U, frozen, out, nx, ny, nz, dx, big, n);
+}
+
+// phi <- sign(phi0) * min(U, clampv), sign(0) = +1, NaN propagates.
+__device__ __forceinline__ void eik_final_node(float* __restrict__ phi,
+                                               const float* __restrict__ U, float clampv,
+                                               long long n) {
   float p = phi[n];
   float m = fminf(U[n], clampv);
   float s = (p < 0.f) ? -1.f : 1.f;
   phi[n] = (p != p) ? p : s * m;
+}
+
+__global__ void k_eik_final(float* __restrict__ phi, const float* __restrict__ U, long long N,
+                            float clampv) {
+  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  eik_final_node(phi, U, clampv, n);
+}
+
+// ------------------------------------------------------------------------------ many grids
+// One launch for the same stage of many grids (the features of a batched step): grid f owns the
+// workgroups [blk0[f], blk0[f+1]) of the launch, found by a binary search that is uniform across
+// the workgroup, and its node n is the workgroup's offset in that range times kBS plus the lane.
+// A stage that a grid does not take part in (fewer substeps or sweeps) exits at once.
+struct LsGrid {
+  float* phi;
+  float* work;               // advection ping-pong partner of phi
+  const float* F;
+  const uint8_t* mask;
+  const float* mask_phi;
+  float *U, *U2;             // reinit workspace slices
+  uint8_t* fr;
+  int nx, ny, nz;
+  int count;                 // advection substeps or Jacobi sweeps
+  double dx;
+  float inv_dx, dt, big, clampv;
+  long long blk0;            // first workgroup of this grid
+};
+
+__device__ __forceinline__ int grid_of_block(const LsGrid* __restrict__ g, int n) {
+  int lo = 0, hi = n - 1;
+  const long long b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// substep s of every grid with more than s substeps: phi -> work on even s, work -> phi on odd
+__global__ void k_advect_grids(const LsGrid* __restrict__ grids, int n_grids, int s) {
+  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
+  if (s >= G.count) return;
+  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
+  if (n >= (long long)G.nx * G.ny * G.nz) return;
+  const float* src = (s & 1) ? G.work : G.phi;
+  float* dst = (s & 1) ? G.phi : G.work;
+  advect_node(src, G.F, G.mask, G.mask_phi, dst, G.nx, G.ny, G.nz, G.inv_dx, G.dt, n);
+}
+
+// a grid with an odd substep count ends in work: copy it back to phi
+__global__ void k_advect_grids_copyback(const LsGrid* __restrict__ grids, int n_grids) {
+  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
+  if (!(G.count & 1)) return;
+  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
+  if (n >= (long long)G.nx * G.ny * G.nz) return;
+  G.phi[n] = G.work[n];
+}
+
+__global__ void k_eik_init_grids(const LsGrid* __restrict__ grids, int n_grids) {
+  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
+  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
+  if (n >= (long long)G.nx * G.ny * G.nz) return;
+  eik_init_node(G.phi, G.U, G.fr, G.nx, G.ny, G.nz, G.dx, G.big, n);
+}
+
+// sweep it of every grid with more than it sweeps: U -> U2 on even it, U2 -> U on odd
+__global__ void k_eik_jacobi_grids(const LsGrid* __restrict__ grids, int n_grids, int it) {
+  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
+  if (it >= G.count) return;
+  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
+  if (n >= (long long)G.nx * G.ny * G.nz) return;
+  const float* src = (it & 1) ? G.U2 : G.U;
+  float* dst = (it & 1) ? G.U : G.U2;
+  eik_jacobi_node(src, G.fr, dst, G.nx, G.ny, G.nz, G.dx, G.big, n);
+}
+
+__global__ void k_eik_final_grids(const LsGrid* __restrict__ grids, int n_grids) {
+  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
+  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
+  if (n >= (long long)G.nx * G.ny * G.nz) return;
+  eik_final_node(G.phi, (G.count & 1) ? G.U2 : G.U, G.clampv, n);
 }
 
 }  // namespace
@@ -442,6 +548,106 @@ extern "C" int petch_reinit_fsm_noalloc(petch_ctx* ctx, float* phi, int nx, int 
   return reinit_launch(ctx, phi, nx, ny, nz, dx, band, n_iter, U, U2, fr);
 }
 
+namespace {
+
+// Fills the per-grid blk0 prefix and returns the launch's workgroup count (0 on overflow).
+long long assign_blocks(LsGrid* g, int n) {
+  long long b = 0;
+  for (int f = 0; f < n; ++f) {
+    g[f].blk0 = b;
+    b += (long long)petch_blocks((long long)g[f].nx * g[f].ny * g[f].nz, kBS);
+  }
+  return b <= 0x7fffffffLL ? b : 0;
+}
+
+}  // namespace
+
+extern "C" int petch_advect_batch(petch_ctx* ctx, int n, const petch_advect_item* items) {
+  PETCH_ARG(ctx, ctx && (n == 0 || items), "advect_batch: null pointer");
+  PETCH_ARG(ctx, n >= 0, "advect_batch: n < 0");
+  if (n == 0) return PETCH_OK;
+  LsGrid* g = (LsGrid*)calloc((size_t)n, sizeof(LsGrid));
+  if (!g) return petch_set_error(ctx, PETCH_E_HIP, "advect_batch: out of host memory");
+  int max_sub = 0;
+  for (int f = 0; f < n; ++f) {
+    const petch_advect_item& it = items[f];
+    if (!(it.phi && it.work && it.F && it.mask && it.mask_phi) || it.nx <= 0 || it.ny <= 0 ||
+        it.nz <= 0 || it.nsub < 0 || !(it.dx > 0)) {
+      free(g);
+      return petch_set_error(ctx, PETCH_E_ARG, "advect_batch: item %d is invalid", f);
+    }
+    g[f] = LsGrid{it.phi, it.work, it.F, it.mask, it.mask_phi, nullptr, nullptr, nullptr,
+                  it.nx, it.ny, it.nz, it.nsub, it.dx, (float)(1.0 / it.dx), (float)it.dt,
+                  0.f, 0.f, 0};
+    if (it.nsub > max_sub) max_sub = it.nsub;
+  }
+  const long long blocks = assign_blocks(g, n);
+  if (!blocks) {
+    free(g);
+    return petch_set_error(ctx, PETCH_E_ARG, "advect_batch: too many nodes for one launch");
+  }
+  const LsGrid* dg = (const LsGrid*)petch_upload_table(ctx, g, (size_t)n * sizeof(LsGrid));
+  free(g);
+  if (!dg) return PETCH_E_HIP;
+  for (int sub = 0; sub < max_sub; ++sub) {
+    hipLaunchKernelGGL(k_advect_grids, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, dg, n,
+                       sub);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_advect_grids_copyback, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream,
+                     dg, n);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_reinit_fsm_batch(petch_ctx* ctx, int n, const petch_reinit_item* items) {
+  PETCH_ARG(ctx, ctx && (n == 0 || items), "reinit_fsm_batch: null pointer");
+  PETCH_ARG(ctx, n >= 0, "reinit_fsm_batch: n < 0");
+  if (n == 0) return PETCH_OK;
+  long long total = 0;
+  for (int f = 0; f < n; ++f) {
+    const petch_reinit_item& it = items[f];
+    PETCH_ARG(ctx, it.phi && it.nx > 0 && it.ny > 0 && it.nz > 0 && it.dx > 0 && it.band > 0,
+              "reinit_fsm_batch: item %d is invalid", f);
+    total += (long long)it.nx * it.ny * it.nz;
+  }
+  float* U = (float*)petch_scratch(ctx, SCR_LS_A, (size_t)total * sizeof(float));
+  float* U2 = (float*)petch_scratch(ctx, SCR_LS_B, (size_t)total * sizeof(float));
+  uint8_t* fr = (uint8_t*)petch_scratch(ctx, SCR_LS_FROZEN, (size_t)total);
+  if (!U || !U2 || !fr) return PETCH_E_HIP;
+  LsGrid* g = (LsGrid*)calloc((size_t)n, sizeof(LsGrid));
+  if (!g) return petch_set_error(ctx, PETCH_E_HIP, "reinit_fsm_batch: out of host memory");
+  long long off = 0;
+  int max_it = 0;
+  for (int f = 0; f < n; ++f) {
+    const petch_reinit_item& it = items[f];
+    const int n_iter = it.n_iter < 0 ? (int)ceil(it.band / it.dx) + 4 : it.n_iter;
+    g[f] = LsGrid{it.phi, nullptr, nullptr, nullptr, nullptr, U + off, U2 + off, fr + off,
+                  it.nx, it.ny, it.nz, n_iter, it.dx, 0.f, 0.f, (float)(it.band + 4.0 * it.dx),
+                  (float)(it.band + it.dx), 0};
+    off += (long long)it.nx * it.ny * it.nz;
+    if (n_iter > max_it) max_it = n_iter;
+  }
+  const long long blocks = assign_blocks(g, n);
+  if (!blocks) {
+    free(g);
+    return petch_set_error(ctx, PETCH_E_ARG, "reinit_fsm_batch: too many nodes for one launch");
+  }
+  const LsGrid* dg = (const LsGrid*)petch_upload_table(ctx, g, (size_t)n * sizeof(LsGrid));
+  free(g);
+  if (!dg) return PETCH_E_HIP;
+  hipLaunchKernelGGL(k_eik_init_grids, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, dg, n);
+  PETCH_LAUNCH_CHECK(ctx);
+  for (int it = 0; it < max_it; ++it) {
+    hipLaunchKernelGGL(k_eik_jacobi_grids, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, dg,
+                       n, it);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_eik_final_grids, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, dg, n);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
 extern "C" int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const float* F,
                                     const uint8_t* mask, const float* gout, int nx, int ny, int nz,
                                     double dx, double dt, float* gphi, float* gF) {
@@ -458,6 +664,228 @@ extern "C" int petch_advect_adjoint(petch_ctx* ctx, const float* phi, const floa
                      nx, ny, nz, inv, (float)dt);
   PETCH_LAUNCH_CHECK(ctx);
   hipLaunchKernelGGL(k_advect_adjoint_gather, g, b, 0, ctx->stream, phi, w, gphi, nx, ny, nz, inv);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+// ------------------------------------------------------------------------------ reinit adjoint
+// The adjoint of the FSM reinit as the piecewise-smooth map its launches compute: every discrete
+// choice (interface flags, the smaller neighbour per axis, the active Godunov terms, whether a
+// sweep lowers a node, the clamp at band + dx) is the forward's, recomputed from the sweep
+// states, which the adjoint rebuilds with the forward's own kernels. A node that sweep k lowers
+// took eik_solve of its three axis minima at state k; the root x of sum_d (x - a_d)^2 = dx^2 over
+// the active terms moves with a_d by (x - a_d) / sum_e (x - a_e). The frozen seeds
+// |phi| / max(|grad phi|_central, 0.5) pass their derivative to the node and its six neighbours.
+
+// Godunov root of the three axis minima in float64 and its weight on each axis (0 off the
+// active set); the same staging as eik_solve, so the active set is the forward's.
+__device__ __forceinline__ void eik_weights(float f1, float f2, float f3, double dx, double* w) {
+  double a[3] = {f1, f2, f3};
+  double s[3] = {f1, f2, f3}, t;
+  if (s[0] > s[1]) { t = s[0]; s[0] = s[1]; s[1] = t; }
+  if (s[1] > s[2]) { t = s[1]; s[1] = s[2]; s[2] = t; }
+  if (s[0] > s[1]) { t = s[0]; s[0] = s[1]; s[1] = t; }
+  double x = s[0] + dx;
+  if (x > s[1]) {
+    double b2 = s[1] - s[0], b3 = s[2] - s[0];
+    double y = 0.5 * (b2 + sqrt(fmax(2.0 * dx * dx - b2 * b2, 0.0)));
+    if (s[0] + y <= s[2]) {
+      x = s[0] + y;
+    } else {
+      double s1 = b2 + b3;
+      x = s[0] + (s1 + sqrt(fmax(s1 * s1 - 3.0 * (b2 * b2 + b3 * b3 - dx * dx), 0.0))) / 3.0;
+    }
+  }
+  double den = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    w[d] = a[d] < x ? x - a[d] : 0.0;
+    den += w[d];
+  }
+  for (int d = 0; d < 3; ++d) w[d] = den > 0.0 ? w[d] / den : 0.0;
+}
+
+// Per node of sweep k's state U: whether the sweep lowered it, and the weights it puts on the
+// smaller neighbour of each axis (W.w = bit 0 took, bits 1..3 "the upper neighbour" per axis).
+__global__ void k_eik_adj_weights(const float* __restrict__ U, const uint8_t* __restrict__ frozen,
+                                  float4* __restrict__ W, int nx, int ny, int nz, double dx,
+                                  float big) {
+  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long N = (long long)nx * ny * nz;
+  if (n >= N) return;
+  float4 out = make_float4(0.f, 0.f, 0.f, __int_as_float(0));
+  if (!frozen[n]) {
+    int k = (int)(n % nz);
+    int j = (int)((n / nz) % ny);
+    int i = (int)(n / ((long long)nz * ny));
+    long long sx = (long long)ny * nz, sy = nz;
+    float lo[3] = {i >= 1 ? U[n - sx] : big, j >= 1 ? U[n - sy] : big, k >= 1 ? U[n - 1] : big};
+    float hi[3] = {i <= nx - 2 ? U[n + sx] : big, j <= ny - 2 ? U[n + sy] : big,
+                   k <= nz - 2 ? U[n + 1] : big};
+    float a[3];
+    int bits = 0;
+    for (int d = 0; d < 3; ++d) {
+      a[d] = fminf(lo[d], hi[d]);
+      if (!(lo[d] <= hi[d])) bits |= 2 << d;     // the lower side on a tie
+    }
+    float cand = eik_solve(a[0], a[1], a[2], dx);
+    if (cand < U[n]) {
+      double w[3];
+      eik_weights(a[0], a[1], a[2], dx, w);
+      out = make_float4((float)w[0], (float)w[1], (float)w[2], __int_as_float(bits | 1));
+    }
+  }
+  W[n] = out;
+}
+
+// G_k = transpose of sweep k applied to G_{k+1}: a node keeps its own adjoint unless the sweep
+// lowered it, and gathers w * G from every neighbour that took it as its axis minimum.
+__global__ void k_eik_adj_sweep(const float4* __restrict__ W, const float* __restrict__ Gin,
+                                float* __restrict__ Gout, int nx, int ny, int nz) {
+  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long N = (long long)nx * ny * nz;
+  if (n >= N) return;
+  int k = (int)(n % nz);
+  int j = (int)((n / nz) % ny);
+  int i = (int)(n / ((long long)nz * ny));
+  const long long st[3] = {(long long)ny * nz, nz, 1};
+  const int idx[3] = {i, j, k}, ext[3] = {nx, ny, nz};
+  float4 me = W[n];
+  float g = (__float_as_int(me.w) & 1) ? 0.f : Gin[n];
+  for (int d = 0; d < 3; ++d) {
+    if (idx[d] + 1 < ext[d]) {               // the upper neighbour took n if its choice is "lower"
+      const long long m = n + st[d];
+      float4 q = W[m];
+      int b = __float_as_int(q.w);
+      float wd = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+      if ((b & 1) && !(b & (2 << d))) g += wd * Gin[m];
+    }
+    if (idx[d] >= 1) {                        // the lower neighbour took n if its choice is "upper"
+      const long long m = n - st[d];
+      float4 q = W[m];
+      int b = __float_as_int(q.w);
+      float wd = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+      if ((b & 1) && (b & (2 << d))) g += wd * Gin[m];
+    }
+  }
+  Gout[n] = g;
+}
+
+// The clamp and the sign of the output: G = gout * sign(phi0) where U < band + dx, else 0.
+__global__ void k_eik_adj_final(const float* __restrict__ phi0, const float* __restrict__ U,
+                                const float* __restrict__ gout, float* __restrict__ G,
+                                long long N, float clampv) {
+  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float p = phi0[n];
+  float s = (p < 0.f) ? -1.f : 1.f;
+  G[n] = (p == p && U[n] < clampv) ? gout[n] * s : 0.f;
+}
+
+// The seed terms of frozen node m at G0 = G[m]: d D / d phi0[m] and the coefficient c_d of the
+// central difference on axis d (d D / d phi0[m +- e_d] = +- c_d).
+__device__ __forceinline__ void eik_seed_terms(const float* __restrict__ phi0, long long m, int i,
+                                               int j, int k, int nx, int ny, int nz, double dx,
+                                               float G0, double* self, double* cd) {
+  long long sx = (long long)ny * nz, sy = nz;
+  double c = phi0[m];
+  double oxm = phi0[i > 0 ? m - sx : m], oxp = phi0[i < nx - 1 ? m + sx : m];
+  double oym = phi0[j > 0 ? m - sy : m], oyp = phi0[j < ny - 1 ? m + sy : m];
+  double ozm = phi0[k > 0 ? m - 1 : m], ozp = phi0[k < nz - 1 ? m + 1 : m];
+  double h = 0.5 / dx;
+  double g[3] = {(oxp - oxm) * h, (oyp - oym) * h, (ozp - ozm) * h};
+  double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  double grad0 = fmax(gn, 0.5);
+  *self = 0.0;
+  cd[0] = cd[1] = cd[2] = 0.0;
+  if (!(fabs(c) / grad0 < 1.8 * dx)) return;          // capped seed: constant
+  double sg = c > 0.0 ? 1.0 : (c < 0.0 ? -1.0 : 0.0);
+  *self = G0 * sg / grad0;
+  if (gn > 0.5) {
+    double coef = -G0 * fabs(c) / (grad0 * grad0 * gn);
+    for (int d = 0; d < 3; ++d) cd[d] = coef * g[d] * h;
+  }
+}
+
+// d loss / d phi0 from G0 (the adjoint of the initial state): only frozen nodes depend on phi0.
+__global__ void k_eik_adj_init(const float* __restrict__ phi0, const uint8_t* __restrict__ frozen,
+                               const float* __restrict__ G, float* __restrict__ gphi, int nx,
+                               int ny, int nz, double dx) {
+  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long N = (long long)nx * ny * nz;
+  if (n >= N) return;
+  int k = (int)(n % nz);
+  int j = (int)((n / nz) % ny);
+  int i = (int)(n / ((long long)nz * ny));
+  const long long st[3] = {(long long)ny * nz, nz, 1};
+  const int ext[3] = {nx, ny, nz};
+  int idx[3] = {i, j, k};
+  double acc = 0.0, self, cd[3];
+  if (frozen[n]) {
+    eik_seed_terms(phi0, n, i, j, k, nx, ny, nz, dx, G[n], &self, cd);
+    acc += self;
+    for (int d = 0; d < 3; ++d) {            // a clamped neighbour at the edge is n itself
+      if (idx[d] == ext[d] - 1) acc += cd[d];
+      if (idx[d] == 0) acc -= cd[d];
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    for (int side = -1; side <= 1; side += 2) {
+      int q = idx[d] + side;
+      if (q < 0 || q >= ext[d]) continue;
+      const long long m = n + side * st[d];
+      if (!frozen[m]) continue;
+      int mi[3] = {i, j, k};
+      mi[d] = q;
+      eik_seed_terms(phi0, m, mi[0], mi[1], mi[2], nx, ny, nz, dx, G[m], &self, cd);
+      // n is m's +1 neighbour when m sits below it (side = -1), its -1 neighbour otherwise
+      acc += side < 0 ? cd[d] : -cd[d];
+    }
+  }
+  gphi[n] = (float)acc;
+}
+
+extern "C" int petch_reinit_fsm_adjoint(petch_ctx* ctx, const float* phi0, int nx, int ny, int nz,
+                                        double dx, double band, int n_iter, const float* gout,
+                                        float* gphi) {
+  PETCH_ARG(ctx, ctx && phi0 && gout && gphi, "reinit_fsm_adjoint: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "reinit_fsm_adjoint: bad shape %d,%d,%d", nx, ny,
+            nz);
+  PETCH_ARG(ctx, dx > 0 && band > 0, "reinit_fsm_adjoint: dx and band must be > 0");
+  PETCH_ARG(ctx, gphi != gout && gphi != phi0, "reinit_fsm_adjoint: gphi must not alias");
+  const long long N = (long long)nx * ny * nz;
+  const double big = band + 4.0 * dx;
+  if (n_iter < 0) n_iter = (int)ceil(band / dx) + 4;
+  const size_t slab = round256((size_t)N * sizeof(float));
+  char* S = (char*)petch_scratch(ctx, SCR_EIK_STATES, slab * (size_t)(n_iter + 1));
+  float4* W = (float4*)petch_scratch(ctx, SCR_EIK_W, (size_t)N * sizeof(float4));
+  char* G = (char*)petch_scratch(ctx, SCR_EIK_G, 2 * slab);
+  uint8_t* fr = (uint8_t*)petch_scratch(ctx, SCR_LS_FROZEN, N);
+  if (!S || !W || !G || !fr) return PETCH_E_HIP;
+  auto state = [&](int k) { return (float*)(S + slab * (size_t)k); };
+  float* Ga = (float*)G;
+  float* Gb = (float*)(G + slab);
+  dim3 g(petch_blocks(N, kBS)), b(kBS);
+  // the forward's states, with the forward's kernels
+  hipLaunchKernelGGL(k_eik_init, g, b, 0, ctx->stream, phi0, state(0), fr, nx, ny, nz, dx,
+                     (float)big);
+  PETCH_LAUNCH_CHECK(ctx);
+  for (int it = 0; it < n_iter; ++it) {
+    hipLaunchKernelGGL(k_eik_jacobi, g, b, 0, ctx->stream, state(it), fr, state(it + 1), nx, ny,
+                       nz, dx, (float)big);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_eik_adj_final, g, b, 0, ctx->stream, phi0, state(n_iter), gout, Ga, N,
+                     (float)(band + dx));
+  PETCH_LAUNCH_CHECK(ctx);
+  for (int it = n_iter - 1; it >= 0; --it) {
+    hipLaunchKernelGGL(k_eik_adj_weights, g, b, 0, ctx->stream, state(it), fr, W, nx, ny, nz, dx,
+                       (float)big);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_eik_adj_sweep, g, b, 0, ctx->stream, W, Ga, Gb, nx, ny, nz);
+    PETCH_LAUNCH_CHECK(ctx);
+    float* t = Ga; Ga = Gb; Gb = t;
+  }
+  hipLaunchKernelGGL(k_eik_adj_init, g, b, 0, ctx->stream, phi0, fr, Ga, gphi, nx, ny, nz, dx);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

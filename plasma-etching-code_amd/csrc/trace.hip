@@ -181,6 +181,18 @@ __device__ __forceinline__ Vec3 hit_normal(const BvhView& bv, int slot) {
   return petch::normalize(petch::cross(v3(q1.x, q1.y, q1.z) - a, v3(q2.x, q2.y, q2.z) - a));
 }
 
+// Per-face accumulation: a float atomic add, or with `fixed` (the deterministic mode) a 64-bit
+// integer add of v in units of 2^-32 into an int64 accumulator at the same address. Integer sums
+// do not depend on the order of the adds, so the result is the same whatever the schedule; it is
+// the exact sum of the deposits rounded to 2^-32 (|sum| < 2^31 per face).
+constexpr double kFixScale = 4294967296.0;
+__device__ __forceinline__ void deposit(float* __restrict__ acc, int face, float v, int fixed) {
+  if (fixed)
+    atomicAdd((unsigned long long*)acc + face,
+              (unsigned long long)__double2ll_rn((double)v * kFixScale));
+  else
+    unsafeAtomicAdd(acc + face, v);
+}
 __device__ __forceinline__ void deposit(float* __restrict__ acc, int face, float v) {
   unsafeAtomicAdd(acc + face, v);
 }
@@ -260,27 +272,6 @@ constexpr int kLdsStack = 8;  // traversal stack entries per lane held in LDS (1
 // (P::miss) before the lane goes idle.
 enum LaneState { kIdle = 0, kTrav = 1, kHit = 2, kMiss = 3 };
 
-// Starts the lane's query for the ray (o, d) with constants rp, returning the lane's state.
-// While the ray cannot meet the box of all triangles (it passes above or beside the surface:
-// an upward re-emission over the mask, a shallow ray crossing the cell), the query could only
-// miss, so the wraps that the miss would cause are applied on the spot without visiting a
-// node, and a ray that escapes ends here: the outcome (hit, wraps, escape) is the one the
-// node-by-node queries give, in one loop iteration instead of one per wrap.
-template <class P>
-__device__ __forceinline__ int begin_query(const P& pol, const typename P::Ray& R, petch::Trav& T,
-                                           petch::RayPre& rp, Vec3& o, Vec3 d, int& wraps) {
-  const petch::BvhView& bv = pol.view(R);
-  while (!petch::root_may_hit(bv, rp)) {
-    const Box& b = pol.box(R);
-    if (!(b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d, rp.inv)))
-      return P::kReplay ? kMiss : kIdle;
-    petch::ray_move(rp, o);
-    ++wraps;
-  }
-  petch::trav_begin(bv, T, rp, 1.0e6f);
-  return kTrav;
-}
-
 template <class P>
 __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q,
                                                  unsigned long long* __restrict__ hits) {
@@ -300,37 +291,32 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
   unsigned pool = 0, pool_end = 0;  // this wavefront's unclaimed queue ids [pool, pool_end)
   while (true) {
     if (__popcll(__ballot(st == kTrav)) <= kRefillAt) {
-      // a second pass refills the lanes whose new or re-emitted ray ended in begin_query
-#pragma unroll 1
-      for (int pass = 0; pass < 2; ++pass) {
-        Emit e;
-        bool emit = false;
-        if (st == kHit) {
-          ++nh;
-          emit = pol.hit(R, T.h, o, d, e);
+      Emit e;
+      bool emit = false;
+      if (st == kHit) {
+        ++nh;
+        emit = pol.hit(R, T.h, o, d, e);
+        st = kIdle;
+      }
+      if constexpr (P::kReplay) {
+        if (st == kMiss) {
+          emit = pol.miss(R, o, e);
           st = kIdle;
         }
-        if constexpr (P::kReplay) {
-          if (st == kMiss) {
-            emit = pol.miss(R, o, e);
-            st = kIdle;
-          }
-        }
-        unsigned ray;
-        if (claim_ray(q, emit || st == kTrav, exhausted, pool, pool_end, ray)) {
-          pol.start(R, ray, o, e);
-          emit = true;
-        }
-        if (emit) {
-          d = emit_dir(e);
-          rp = petch::ray_pre(o, d);
-          wraps = 0;
-          st = begin_query(pol, R, T, rp, o, d, wraps);
-        }
-        if (exhausted || !__ballot(st == kIdle)) break;
       }
-      // every lane is idle and the queue is spent (a lane in kHit or kMiss still has work)
-      if (!__ballot(st != kIdle) && exhausted) break;
+      unsigned ray;
+      if (claim_ray(q, emit || st == kTrav, exhausted, pool, pool_end, ray)) {
+        pol.start(R, ray, o, e);
+        emit = true;
+      }
+      if (emit) {
+        d = emit_dir(e);
+        rp = petch::ray_pre(o, d);
+        wraps = 0;
+        petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
+        st = kTrav;
+      }
+      if (!__ballot(st == kTrav)) break;
     }
     if (st == kTrav && petch::trav_step(pol.view(R), rp, T, stk)) {
       const Box& b = pol.box(R);
@@ -339,7 +325,7 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
       } else if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d, rp.inv)) {
         petch::ray_move(rp, o);
         ++wraps;
-        st = begin_query(pol, R, T, rp, o, d, wraps);
+        petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
       } else {
         st = P::kReplay ? kMiss : kIdle;
       }
@@ -396,10 +382,10 @@ __device__ __forceinline__ void rr_emit(const Hit& h, Vec3 nn, const uint32_t u[
 
 __device__ __forceinline__ bool rr_bounce(float& w, int& bounce, uint64_t seed, unsigned ray,
                                           const BvhView& bv, const Hit& h, float* __restrict__ fl,
-                                          float S, Vec3& o, Vec3 d, Emit& e) {
+                                          int fixed, float S, Vec3& o, Vec3 d, Emit& e) {
   Vec3 nn = hit_normal(bv, h.slot);
   if (petch::dot(nn, d) > 0.f) nn = -nn;
-  deposit(fl, h.face, w);
+  deposit(fl, h.face, w, fixed);
   float w2;
   bool reset;
   uint32_t u[4];
@@ -438,6 +424,7 @@ struct NeutralRR {
   float* fl[2];
   uint64_t seed[2];
   unsigned n0;
+  int fixed;  // fl are int64 fixed-point accumulators (deterministic mode)
   struct Ray {
     float w;
     int bounce;
@@ -456,7 +443,7 @@ struct NeutralRR {
   __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
     const float S = bare[h.face] * (R.sp ? beta[1] : beta[0]);
     return rr_bounce(R.w, R.bounce, R.sp ? seed[1] : seed[0], R.ray, bv, h,
-                     R.sp ? fl[1] : fl[0], S, o, d, e);
+                     R.sp ? fl[1] : fl[0], fixed, S, o, d, e);
   }
 };
 
@@ -647,11 +634,11 @@ __device__ __forceinline__ bool ion_bounce(Rng& r, float& Ein, float& win, int& 
     deposit(F.out[1], h.face, sqrtE > F.sE[1] ? P.A_ie * f_ie * s : 0.f);
     deposit(F.out[2], h.face, sqrtE > F.sE[2] ? P.A_p * f_ie * s : 0.f);
   } else {
-    deposit(F.out[0], h.face, P.A_sp * fmaxf(sqrtE - F.sE[0], 0.f) * f_sp * w);
-    deposit(F.out[1], h.face, P.A_ie * fmaxf(sqrtE - F.sE[1], 0.f) * f_ie * w);
-    deposit(F.out[2], h.face, P.A_p * fmaxf(sqrtE - F.sE[2], 0.f) * f_ie * w);
-    deposit(F.out[3], h.face, w);
-    deposit(F.out[4], h.face, cosT * w);
+    deposit(F.out[0], h.face, P.A_sp * fmaxf(sqrtE - F.sE[0], 0.f) * f_sp * w, F.fixed);
+    deposit(F.out[1], h.face, P.A_ie * fmaxf(sqrtE - F.sE[1], 0.f) * f_ie * w, F.fixed);
+    deposit(F.out[2], h.face, P.A_p * fmaxf(sqrtE - F.sE[2], 0.f) * f_ie * w, F.fixed);
+    deposit(F.out[3], h.face, w, F.fixed);
+    deposit(F.out[4], h.face, cosT * w, F.fixed);
   }
   float stick = 1.f;
   if (incAng > P.thrmin) {
@@ -718,6 +705,7 @@ struct IonYield {
   float sE[3];
   float* out[5];
   uint64_t seed;
+  int fixed;  // out are int64 fixed-point accumulators (deterministic mode)
   struct Ray {
     float E, w;
     int bounce;
@@ -793,6 +781,8 @@ struct NeutralFeature {
   uint64_t seed[2];
   unsigned n0;   // rays of species 0; the feature's remaining rays are species 1
   unsigned end;  // exclusive end of the feature's ids in the batch queue
+  int fixed;     // fl are int64 fixed-point accumulators (deterministic mode)
+  float* dst[2]; // with fixed: the float outputs the accumulators are converted into
 };
 
 struct IonFeature {
@@ -805,6 +795,8 @@ struct IonFeature {
   float* out[5];
   uint64_t seed;
   unsigned end;
+  int fixed;      // out are int64 fixed-point accumulators (deterministic mode)
+  float* dst[5];  // with fixed: the float outputs the accumulators are converted into
 };
 
 // first feature whose id range ends after `ray`
@@ -851,7 +843,7 @@ struct NeutralRRBatch {
     const NeutralFeature& F = feat[R.f];
     const float S = F.bare[h.face] * (R.sp ? F.beta[1] : F.beta[0]);
     return rr_bounce(R.w, R.bounce, R.sp ? F.seed[1] : F.seed[0], R.ray, R.bv, h,
-                     R.sp ? F.fl[1] : F.fl[0], S, o, d, e);
+                     R.sp ? F.fl[1] : F.fl[0], F.fixed, S, o, d, e);
   }
 };
 
@@ -889,12 +881,15 @@ struct IonYieldBatch {
   }
 };
 
-// zero the per-face outputs of every feature of a batch (blockIdx.y = feature)
+// zero the per-face outputs (float, or int64 with fixed) of every feature of a batch
+// (blockIdx.y = feature)
 __global__ void k_zero_neutral_batch(const NeutralFeature* __restrict__ feat) {
   const NeutralFeature& F = feat[blockIdx.y];
   for (int i = blockIdx.x * kBS + threadIdx.x; i < F.bv.nf; i += gridDim.x * kBS) {
-    F.fl[0][i] = 0.f;
-    F.fl[1][i] = 0.f;
+    for (int k = 0; k < 2; ++k) {
+      if (F.fixed) ((unsigned long long*)F.fl[k])[i] = 0ull;
+      else F.fl[k][i] = 0.f;
+    }
   }
 }
 
@@ -902,8 +897,36 @@ __global__ void k_zero_ion_batch(const IonFeature* __restrict__ feat) {
   const IonFeature& F = feat[blockIdx.y];
   for (int i = blockIdx.x * kBS + threadIdx.x; i < F.bv.nf; i += gridDim.x * kBS) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) F.out[k][i] = 0.f;
+    for (int k = 0; k < 5; ++k) {
+      if (F.fixed) ((unsigned long long*)F.out[k])[i] = 0ull;
+      else F.out[k][i] = 0.f;
+    }
   }
+}
+
+__device__ __forceinline__ float fix_to_float(unsigned long long q) {
+  return (float)((double)(long long)q * (1.0 / kFixScale));
+}
+
+// the deterministic mode's last pass: int64 fixed-point sums -> the float outputs
+__global__ void k_fix_finish(const unsigned long long* __restrict__ q, float* __restrict__ out,
+                             int n) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n) out[i] = fix_to_float(q[i]);
+}
+
+__global__ void k_fix_finish_neutral_batch(const NeutralFeature* __restrict__ feat) {
+  const NeutralFeature& F = feat[blockIdx.y];
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < F.bv.nf; i += gridDim.x * kBS)
+    for (int k = 0; k < 2; ++k) F.dst[k][i] = fix_to_float(((const unsigned long long*)F.fl[k])[i]);
+}
+
+__global__ void k_fix_finish_ion_batch(const IonFeature* __restrict__ feat) {
+  const IonFeature& F = feat[blockIdx.y];
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < F.bv.nf; i += gridDim.x * kBS)
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      F.dst[k][i] = fix_to_float(((const unsigned long long*)F.out[k])[i]);
 }
 
 // The neutral loop fits 96 registers without spills, which gives 5 waves per SIMD; the ion
@@ -1060,6 +1083,26 @@ int zero(petch_ctx* ctx, float* p, int n) {
   return PETCH_OK;
 }
 
+// Deterministic mode: k zeroed int64 fixed-point accumulators of n faces each (null on error).
+unsigned long long* det_accumulators(petch_ctx* ctx, int k, long long n) {
+  const size_t bytes = (size_t)k * (size_t)(n > 0 ? n : 1) * sizeof(unsigned long long);
+  auto* q = (unsigned long long*)petch_scratch(ctx, SCR_DET, bytes);
+  if (!q) return nullptr;
+  if (hipMemsetAsync(q, 0, bytes, ctx->stream) != hipSuccess) {
+    petch_set_error(ctx, PETCH_E_HIP, "deterministic mode: clearing the accumulators failed");
+    return nullptr;
+  }
+  return q;
+}
+
+int det_finish(petch_ctx* ctx, const unsigned long long* q, float* out, int n) {
+  if (n <= 0) return PETCH_OK;
+  hipLaunchKernelGGL(k_fix_finish, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream, q, out,
+                     n);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
 }  // namespace
 
 extern "C" int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petch_source* src,
@@ -1098,7 +1141,14 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   if (n == 0 || bvh->nf == 0) return PETCH_OK;
   NeutralRR pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, {beta, beta},
                 {fl, fl}, {seed, seed}, (unsigned)n};
-  return launch_neutral(ctx, pol, n);
+  if (!ctx->deterministic) return launch_neutral(ctx, pol, n);
+  unsigned long long* q = det_accumulators(ctx, 1, bvh->nf);
+  if (!q) return PETCH_E_HIP;
+  pol.fl[0] = pol.fl[1] = (float*)q;
+  pol.fixed = 1;
+  rc = launch_neutral(ctx, pol, n);
+  if (rc) return rc;
+  return det_finish(ctx, q, fl, bvh->nf);
 }
 
 namespace {
@@ -1174,7 +1224,17 @@ extern "C" int petch_trace_neutral_rr_pair(petch_ctx* ctx, const petch_bvh* bvh,
   if (n == 0 || bvh->nf == 0) return PETCH_OK;
   NeutralRR pol{petch_bvh_view(bvh), make_box(box), make_source(src), bare, {beta_a, beta_b},
                 {fl_a, fl_b}, {seed_a, seed_b}, (unsigned)n_a};
-  return launch_neutral(ctx, pol, n);
+  if (!ctx->deterministic) return launch_neutral(ctx, pol, n);
+  const int nf = bvh->nf;
+  unsigned long long* q = det_accumulators(ctx, 2, nf);
+  if (!q) return PETCH_E_HIP;
+  pol.fl[0] = (float*)q;
+  pol.fl[1] = (float*)(q + nf);
+  pol.fixed = 1;
+  rc = launch_neutral(ctx, pol, n);
+  if (!rc) rc = det_finish(ctx, q, fl_a, nf);
+  if (!rc) rc = det_finish(ctx, q + nf, fl_b, nf);
+  return rc;
 }
 
 extern "C" int petch_trace_ion_yield_tangent(petch_ctx* ctx, const petch_bvh* bvh,
@@ -1227,10 +1287,23 @@ extern "C" int petch_trace_ion_yield(petch_ctx* ctx, const petch_bvh* bvh, const
   IonYield pol{petch_bvh_view(bvh), make_box(box), make_source(src), P,
                {sqrtf(P.Eth_sp), sqrtf(P.Eth_ie), sqrtf(P.Eth_p)}, {sp, ie, ipass, arr, ang},
                seed};
+  const int nf = bvh->nf;
+  unsigned long long* acc = nullptr;
+  if (ctx->deterministic) {
+    acc = det_accumulators(ctx, 5, nf);
+    if (!acc) return PETCH_E_HIP;
+    for (int k = 0; k < 5; ++k) pol.out[k] = (float*)(acc + (size_t)k * nf);
+    pol.fixed = 1;
+  }
   const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_ion_yield, n);
   hipLaunchKernelGGL(k_trace_ion_yield, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
                      petch_hit_counter(ctx, 1));
   PETCH_LAUNCH_CHECK(ctx);
+  if (acc)
+    for (int k = 0; k < 5; ++k) {
+      const int rc3 = det_finish(ctx, acc + (size_t)k * nf, outs[k], nf);
+      if (rc3) return rc3;
+    }
   return PETCH_OK;
 }
 
@@ -1261,6 +1334,9 @@ extern "C" int petch_trace_neutral_rr_batch(petch_ctx* ctx, const petch_source* 
     d.fl[1] = a.fl_b;
     d.seed[0] = a.seed_a;
     d.seed[1] = a.seed_b;
+    d.fixed = 0;
+    d.dst[0] = a.fl_a;
+    d.dst[1] = a.fl_b;
     const long long n = a.bvh->nf > 0 ? (long long)a.n_a + a.n_b : 0;
     d.n0 = a.bvh->nf > 0 ? (unsigned)a.n_a : 0u;
     end += n;
@@ -1268,23 +1344,44 @@ extern "C" int petch_trace_neutral_rr_batch(petch_ctx* ctx, const petch_source* 
     d.end = (unsigned)end;
     if (a.bvh->nf > max_nf) max_nf = a.bvh->nf;
   }
+  if (ctx->deterministic) {
+    long long total = 0;
+    for (int f = 0; f < nfeat; ++f) total += 2LL * feats[f].bvh->nf;
+    unsigned long long* acc = det_accumulators(ctx, 1, total);
+    if (!acc) return PETCH_E_HIP;
+    long long off = 0;
+    for (int f = 0; f < nfeat; ++f) {
+      const int nf = feats[f].bvh->nf;
+      t[f].fl[0] = (float*)(acc + off);
+      t[f].fl[1] = (float*)(acc + off + nf);
+      t[f].fixed = 1;
+      off += 2LL * nf;
+    }
+  }
   const NeutralFeature* dev =
       (const NeutralFeature*)petch_upload_table(ctx, t.data(), t.size() * sizeof(NeutralFeature));
   if (!dev) return PETCH_E_HIP;
-  if (max_nf > 0) {
-    const unsigned gx = petch_blocks(max_nf, kBS) < 64 ? petch_blocks(max_nf, kBS) : 64;
+  const unsigned gx = petch_blocks(max_nf, kBS) < 64 ? petch_blocks(max_nf, kBS) : 64;
+  if (max_nf > 0 && !ctx->deterministic) {
     hipLaunchKernelGGL(k_zero_neutral_batch, dim3(gx, nfeat), dim3(kBS), 0, ctx->stream, dev);
     PETCH_LAUNCH_CHECK(ctx);
   }
-  if (end == 0) return PETCH_OK;
-  RayQueue q;
-  int rc = make_queue(ctx, (int)end, q);
-  if (rc) return rc;
-  NeutralRRBatch pol{dev, nfeat, make_source(src)};
-  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_rr_batch, (int)end);
-  hipLaunchKernelGGL(k_trace_neutral_rr_batch, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
-                     petch_hit_counter(ctx, 0));
-  PETCH_LAUNCH_CHECK(ctx);
+  if (end > 0) {
+    RayQueue q;
+    int rc = make_queue(ctx, (int)end, q);
+    if (rc) return rc;
+    NeutralRRBatch pol{dev, nfeat, make_source(src)};
+    const unsigned grid =
+        persistent_blocks(ctx, (const void*)k_trace_neutral_rr_batch, (int)end);
+    hipLaunchKernelGGL(k_trace_neutral_rr_batch, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
+                       petch_hit_counter(ctx, 0));
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  if (max_nf > 0 && ctx->deterministic) {
+    hipLaunchKernelGGL(k_fix_finish_neutral_batch, dim3(gx, nfeat), dim3(kBS), 0, ctx->stream,
+                       dev);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
   return PETCH_OK;
 }
 
@@ -1318,29 +1415,50 @@ extern "C" int petch_trace_ion_yield_batch(petch_ctx* ctx, int nfeat,
     d.out[2] = a.ipass;
     d.out[3] = a.arr;
     d.out[4] = a.ang;
+    for (int k = 0; k < 5; ++k) d.dst[k] = d.out[k];
+    d.fixed = 0;
     d.seed = a.seed;
     end += a.bvh->nf > 0 ? a.n : 0;
     PETCH_ARG(ctx, end <= 0x7fffffffLL, "trace_ion_yield_batch: more than 2^31-1 rays");
     d.end = (unsigned)end;
     if (a.bvh->nf > max_nf) max_nf = a.bvh->nf;
   }
+  if (ctx->deterministic) {
+    long long total = 0;
+    for (int f = 0; f < nfeat; ++f) total += 5LL * feats[f].bvh->nf;
+    unsigned long long* acc = det_accumulators(ctx, 1, total);
+    if (!acc) return PETCH_E_HIP;
+    long long off = 0;
+    for (int f = 0; f < nfeat; ++f) {
+      const int nf = feats[f].bvh->nf;
+      for (int k = 0; k < 5; ++k) t[f].out[k] = (float*)(acc + off + (long long)k * nf);
+      t[f].fixed = 1;
+      off += 5LL * nf;
+    }
+  }
   const IonFeature* dev =
       (const IonFeature*)petch_upload_table(ctx, t.data(), t.size() * sizeof(IonFeature));
   if (!dev) return PETCH_E_HIP;
-  if (max_nf > 0) {
-    const unsigned gx = petch_blocks(max_nf, kBS) < 64 ? petch_blocks(max_nf, kBS) : 64;
+  const unsigned gx = petch_blocks(max_nf, kBS) < 64 ? petch_blocks(max_nf, kBS) : 64;
+  if (max_nf > 0 && !ctx->deterministic) {
     hipLaunchKernelGGL(k_zero_ion_batch, dim3(gx, nfeat), dim3(kBS), 0, ctx->stream, dev);
     PETCH_LAUNCH_CHECK(ctx);
   }
-  if (end == 0) return PETCH_OK;
-  RayQueue q;
-  int rc = make_queue(ctx, (int)end, q);
-  if (rc) return rc;
-  IonYieldBatch pol{dev, nfeat};
-  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_ion_yield_batch, (int)end);
-  hipLaunchKernelGGL(k_trace_ion_yield_batch, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
-                     petch_hit_counter(ctx, 1));
-  PETCH_LAUNCH_CHECK(ctx);
+  if (end > 0) {
+    RayQueue q;
+    int rc = make_queue(ctx, (int)end, q);
+    if (rc) return rc;
+    IonYieldBatch pol{dev, nfeat};
+    const unsigned grid =
+        persistent_blocks(ctx, (const void*)k_trace_ion_yield_batch, (int)end);
+    hipLaunchKernelGGL(k_trace_ion_yield_batch, dim3(grid), dim3(kBS), 0, ctx->stream, pol, q,
+                       petch_hit_counter(ctx, 1));
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  if (max_nf > 0 && ctx->deterministic) {
+    hipLaunchKernelGGL(k_fix_finish_ion_batch, dim3(gx, nfeat), dim3(kBS), 0, ctx->stream, dev);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
   return PETCH_OK;
 }
 

@@ -87,6 +87,19 @@ class IonFeature(ctypes.Structure):
                 ("ipass", c_void_p), ("arr", c_void_p), ("ang", c_void_p)]
 
 
+class AdvectItem(ctypes.Structure):
+    """petch_advect_item: one grid of a batched advection."""
+    _fields_ = [("phi", c_void_p), ("work", c_void_p), ("F", c_void_p), ("mask", c_void_p),
+                ("mask_phi", c_void_p), ("nx", c_int), ("ny", c_int), ("nz", c_int),
+                ("dx", c_double), ("dt", c_double), ("nsub", c_int)]
+
+
+class ReinitItem(ctypes.Structure):
+    """petch_reinit_item: one grid of a batched reinit."""
+    _fields_ = [("phi", c_void_p), ("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_double),
+                ("band", c_double), ("n_iter", c_int)]
+
+
 class McItem(ctypes.Structure):
     """petch_mc_item: one grid of a batched marching cubes."""
     _fields_ = [("phi", c_void_p), ("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_double),
@@ -112,12 +125,19 @@ SIGNATURES = {
     "petch_ctx_destroy": [P],
     "petch_set_stream": [P, P],
     "petch_synchronize": [P],
+    "petch_set_deterministic": [P, c_int],
     "petch_version": [],
     "petch_init_sdf": [P, P, P, c_int, c_int, c_int, c_double, c_double, c_double, c_double,
                        c_double, c_double, c_int],
     "petch_advect": [P, P, P, P, P, P, c_int, c_int, c_int, c_double, c_double, c_int],
     "petch_reinit_fsm": [P, P, c_int, c_int, c_int, c_double, c_double, c_int],
     "petch_reinit_fsm_noalloc": [P, P, c_int, c_int, c_int, c_double, c_double, c_int],
+    "petch_profile_area": [P, P, c_int, c_int, c_int, c_double, c_double, P],
+    "petch_advect_batch": [P, c_int, P],
+    "petch_bvh_build_batch": [P, c_int, P, P, P, P, P],
+    "petch_reinit_fsm_batch": [P, c_int, P],
+    "petch_reinit_fsm_adjoint": [P, P, c_int, c_int, c_int, c_double, c_double, c_int, P, P],
+    "petch_profile_area_adjoint": [P, P, c_int, c_int, c_int, c_double, c_double, P, P],
     "petch_marching_cubes": [P, P, c_int, c_int, c_int, c_double, P, P, c_int, c_int,
                              POINTER(c_int), POINTER(c_int)],
     "petch_mesh_attrs": [P, P, P, c_int, P, P, P],

@@ -39,7 +39,16 @@ The etch rate and the velocity extension carry their adjoints too: `surface_rate
 `extend(bvh, V, phi, dx, band)` (petch_extend_adjoint: each band node's gradient goes back to its
 nearest face; the nearest-face assignment is the geometry and stays fixed). With advect they
 take a loss on the advanced level set back to the fluxes of a step.
+
+The FSM reinit (threed.py:1515-1537) carries the exact adjoint of the piecewise-smooth map its
+sweeps compute (petch_reinit_fsm_adjoint): `reinit(a, dx, band)`. Every discrete choice
+(interface flags, the smaller neighbour per axis, the active Godunov terms, which sweep lowers a
+node, the clamp) is held at the forward's; a lowered node moves with its upwind neighbours by
+the Godunov weights (x - a_d) / sum_e (x - a_e), and the interface seeds |phi| / |grad phi|
+pass their derivative to the node and its neighbours.
 """
+import math
+
 import torch
 
 from . import ops
@@ -77,6 +86,33 @@ def advect(phi, F, mask, mask_phi, dx, dt, nsub, recompute=False):
     substep states there (one grid instead of nsub; the substeps are deterministic, so the
     gradient is the same bit for bit)."""
     return _Advect.apply(phi, F, mask, mask_phi, dx, dt, nsub, recompute)
+
+
+class _Reinit(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, a, dx, band, n_iter):
+        a = a.detach().contiguous()
+        r = a.clone()
+        ops.reinit_fsm(r, dx, band, n_iter)
+        ctx.save_for_backward(a)
+        ctx.args = (float(dx), float(band), n_iter)
+        return r
+
+    @staticmethod
+    def backward(ctx, gout):
+        (a,) = ctx.saved_tensors
+        dx, band, n_iter = ctx.args
+        return ops.reinit_fsm_adjoint(a, dx, band, gout.contiguous().float(), n_iter), None, \
+            None, None
+
+
+def reinit(a, dx, band, n_iter=None):
+    """The FSM reinit of a (ops.reinit_fsm, out of place), differentiable in a: the exact
+    adjoint of the map the sweeps compute, with the interface flags, the neighbour and
+    active-term choices of every Godunov update, the sweeps that lower each node and the clamp
+    held at the forward's (petch_reinit_fsm_adjoint). Keeps only a for the backward pass, which
+    rebuilds the sweep states."""
+    return _Reinit.apply(a, dx, band, n_iter)
 
 
 class _Radiosity(torch.autograd.Function):
@@ -243,11 +279,17 @@ def finish(fl, mesh, scale, hi, n_smooth, alpha, blend_each, lo=0.0):
                          blend_each)
 
 
-def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, n_fp=None):
+def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, n_fp=None,
+                     bare_init=None, flux_scale=None):
     """(bare, m_F, m_O) of the neutral rounds of flux.coupled_flux_steps for given ion terms
     (m_i, cos_i, ion_yield): n_fp x {the F and O RR traces at bare (one paired launch with the
-    seeds and streams of the coupled flux), finish, Belen update} from bare = 1. Differentiable in m_i (without ion yields) or the ion yields, through every round:
-    the traces by their path-replay adjoint, finish by its transpose, Belen in closed form."""
+    seeds and streams of the coupled flux), finish, Belen update} from bare = 1, or from
+    clamp(bare_init, 0, 1) (a warm start, itself differentiable). Differentiable in m_i (without
+    ion yields) or the ion yields, through every round: the traces by their path-replay adjoint,
+    finish by its transpose, Belen in closed form. flux_scale = (sF, sO), 0-d tensors, multiply
+    m_F and m_O where the Belen update reads them: the F and O fluxes enter the chemistry only as
+    Fflux * m_F and Oflux * m_O (belen.py:41-47), so sF = Fflux / par['Fflux'] is a change of the
+    F flux, differentiable too. The returned m_F, m_O are the unscaled normalised fluxes."""
     from .chemistry import chem_params
     from .flux import NEUTRAL_SOURCE, SEED_MASK, box_of
     F = mesh.nf
@@ -259,7 +301,10 @@ def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, 
     box = box_of(geo, par)
     chem = chem_params(par, flags)
     n_fp = int(par.get('n_fp', 4)) if n_fp is None else int(n_fp)
-    bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
+    if bare_init is None:
+        bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
+    else:
+        bare = torch.clamp(bare_init.to(torch.float32), 0.0, 1.0)
     m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
     for it in range(n_fp):
         sF, sO = (seed * 9 + 2 + 2 * it) & SEED_MASK, (seed * 9 + 3 + 2 * it) & SEED_MASK
@@ -267,7 +312,11 @@ def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, 
                                  n_neu, sO)
         m_F = finish(fF, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
         m_O = finish(fO, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
-        bare = belen(m_i, m_F, m_O, cos_i, ion_yield, chem)
+        if flux_scale is None:
+            bare = belen(m_i, m_F, m_O, cos_i, ion_yield, chem)
+        else:
+            bare = belen(m_i, m_F * flux_scale[0].float(), m_O * flux_scale[1].float(), cos_i,
+                         ion_yield, chem)
     return bare, m_F, m_O
 
 
@@ -398,3 +447,44 @@ def ion_terms(mesh, geo, par, n_ion, seed, E):
     m_i = fin(acc[3])
     cos_i = ops.cos_ratio(acc[3].detach().contiguous(), acc[4].detach().contiguous())
     return m_i, cos_i, iy
+
+
+class _SliceArea(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, phi, dx, eps):
+        p = phi.detach().contiguous().float()
+        ctx.save_for_backward(p)
+        ctx.args = (float(dx), float(eps))
+        return ops.slice_gas_area(p, dx, eps)
+
+    @staticmethod
+    def backward(ctx, garea):
+        (p,) = ctx.saved_tensors
+        dx, eps = ctx.args
+        return ops.slice_gas_area_adjoint(p, dx, eps, garea.double()), None, None
+
+
+def slice_gas_area(phi, dx, eps):
+    """ops.slice_gas_area, differentiable in phi."""
+    return _SliceArea.apply(phi, dx, eps)
+
+
+def profile_rz(phi, dx, eps=None):
+    """Equivalent radius r(z) = sqrt(A(z) / pi) of every z slice (float64 (nz,)), with A the
+    smoothed gas area (H_eps(-phi), eps = 1.5 dx by default), differentiable in phi. A slice
+    with no gas has r = 0 and passes no gradient."""
+    eps = 1.5 * dx if eps is None else eps
+    A = slice_gas_area(phi, dx, eps)
+    return torch.sqrt(A.clamp_min(1e-30) / math.pi)
+
+
+def profile_loss(phi, dx, target, zsel=None, eps=None):
+    """sum over the selected z slices of (r(z) - target(z))^2, differentiable in phi; zsel is a
+    boolean mask or index tensor of slices (all slices by default)."""
+    r = profile_rz(phi, dx, eps)
+    t = torch.as_tensor(target, dtype=torch.float64, device=r.device)
+    d = r - t
+    if zsel is not None:
+        d = d[torch.as_tensor(zsel, device=r.device)]
+    return (d * d).sum()
+

@@ -34,11 +34,13 @@ class BatchLoop:
 
     def step(self, step):
         loops = self.loops
-        # 1. every feature's mesh (one host read for all marching-cubes counts) and BVH; the
-        #    warm-start guards of all features in one read
+        # 1. every feature's mesh (one host read for all marching-cubes counts) and BVH (the
+        #    refits of all trees in one launch); the warm-start guards of all features in one read
         with self.timer.region("mesh"):
             mcs = ops.marching_cubes_batch([lp.mesh_item(step) for lp in loops])
-        finite = [lp.begin_mesh(step, mc) for lp, mc in zip(loops, mcs)]
+        finite = [lp.begin_mesh(step, mc, bvh=False) for lp, mc in zip(loops, mcs)]
+        with self.timer.region("mesh"):
+            ops.bvh_build_batch([lp._mesh.bvh_item() for lp in loops])
         need = [i for i, f in enumerate(finite) if f is not None]
         if need:
             vals = torch.stack([finite[i] for i in need]).cpu().tolist()
@@ -97,16 +99,22 @@ class BatchLoop:
             for r, i in enumerate(live):
                 vmx, nbad = ops.decode_rate_reduction(host[r, :2])
                 reads[i] = (vmx, nbad, int(host[r, 2]))
-        # 4. the velocity extension of all features in one pass, then CFL, advection and reinit
-        #    of every feature
+        # 4. the velocity extension of all features in one pass, every feature's CFL, then the
+        #    advection substeps and the reinit sweeps of all features, one launch each
         ext = [lp.extend_item() if h and not rd[2] else None
                for lp, h, rd in zip(loops, has, reads)]
         items = [e for e in ext if e is not None]
         if items:
             with self.timer.region("extend"):
                 ops.extend_batch(items)
-        return [lp.end_finish(step, *rd, extended=e is not None)
-                for lp, rd, e in zip(loops, reads, ext)]
+        deferred = dict(advect=[], reinit=[])
+        infos = [lp.end_finish(step, *rd, extended=e is not None, deferred=deferred)
+                 for lp, rd, e in zip(loops, reads, ext)]
+        with self.timer.region("advect"):
+            ops.advect_batch(deferred['advect'])
+        with self.timer.region("reinit"):
+            ops.reinit_fsm_batch(deferred['reinit'])
+        return infos
 
 
 _GEO_KEYS = ("Lx", "Ly", "Lz", "dx", "trench_width", "mask_th", "sub_top", "hole")

@@ -37,9 +37,11 @@ class SurfaceMesh:
     def build(self, phi, dx):
         return self.build_from(*ops.marching_cubes(phi, dx, self.bufs))
 
-    def build_from(self, v, f):
+    def build_from(self, v, f, bvh=True):
         """Everything after marching cubes, from its (verts, faces) (the batched loop meshes
-        all features' grids with one host read, then builds each feature's surface)."""
+        all features' grids with one host read, then builds each feature's surface). bvh=False
+        leaves the BVH to the caller (the batched loop builds all features' trees together with
+        ops.bvh_build_batch over bvh_item())."""
         self.verts, self.faces = v, f
         self.nv, self.nf = int(v.shape[0]), int(f.shape[0])
         self.centroids, self.areas, self.normals = ops.mesh_attrs(v, f)
@@ -49,8 +51,13 @@ class SurfaceMesh:
             if q != self._quality:
                 self.bvh.set_quality(q)
                 self._quality = q
-        self.bvh.build(v, f)
+        if bvh:
+            self.bvh.build(v, f)
         return self
+
+    def bvh_item(self):
+        """(bvh, verts, faces) of this surface for ops.bvh_build_batch."""
+        return (self.bvh, self.verts, self.faces)
 
 
 def ion_params(par):

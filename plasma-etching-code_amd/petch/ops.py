@@ -21,6 +21,14 @@ def _shape3(phi):
     return [int(s) for s in phi.shape]
 
 
+def set_deterministic(on=True, device=None):
+    """Deterministic mode of the ray traces on `device` (petch_set_deterministic): per-face sums
+    in 64-bit fixed point, so the same inputs give bit-identical fluxes and a whole etch run
+    repeats exactly. Off by default (float atomics, as the reference)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    Context.get(dev).call("petch_set_deterministic", int(bool(on)))
+
+
 def init_sdf(nx, ny, nz, dx, Lx, Ly, trench_width, mask_th, sub_top, hole, device=None):
     """Initial phi (float32) and mask (uint8) of make_trench_3d (threed.py:32-59)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
@@ -76,6 +84,77 @@ def reinit_fsm(phi, dx, band, n_iter=None):
     _ctx(phi).call("petch_reinit_fsm", ptr(phi, f32, "phi"), nx, ny, nz, float(dx), float(band),
                    -1 if n_iter is None else int(n_iter))
     return phi
+
+
+def reinit_fsm_adjoint(phi0, dx, band, gout, n_iter=None):
+    """d loss / d phi0 of reinit_fsm(phi0, dx, band) for gout = d loss / d(output), every
+    discrete choice held at the forward's (petch_reinit_fsm_adjoint); phi0 is not modified."""
+    nx, ny, nz = _shape3(phi0)
+    if tuple(gout.shape) != (nx, ny, nz):
+        raise ValueError(f"reinit_fsm_adjoint: gout shape {tuple(gout.shape)} != {(nx, ny, nz)}")
+    gphi = torch.empty_like(phi0)
+    _ctx(phi0).call("petch_reinit_fsm_adjoint", ptr(phi0, f32, "phi0"), nx, ny, nz, float(dx),
+                    float(band), -1 if n_iter is None else int(n_iter), ptr(gout, f32, "gout"),
+                    ptr(gphi, f32))
+    return gphi
+
+
+def advect_batch(items):
+    """advect for many grids in one launch per substep (petch_advect_batch). items: (phi, work,
+    F, mask, mask_phi, dx, dt, nsub) tuples, dt the substep's step; each phi is updated in
+    place as advect would."""
+    import ctypes
+    from ._abi import AdvectItem
+    if not items:
+        return
+    arr = (AdvectItem * len(items))()
+    for k, (phi, work, F, mask, mask_phi, dx, dt, nsub) in enumerate(items):
+        nx, ny, nz = _shape3(phi)
+        for t, n in ((work, "work"), (F, "F"), (mask, "mask"), (mask_phi, "mask_phi")):
+            if tuple(t.shape) != (nx, ny, nz):
+                raise ValueError(f"advect_batch: item {k} {n} shape {tuple(t.shape)} != phi "
+                                 f"shape {(nx, ny, nz)}")
+        arr[k] = AdvectItem(ptr(phi, f32, "phi"), ptr(work, f32, "work"), ptr(F, f32, "F"),
+                            ptr(mask, u8, "mask"), ptr(mask_phi, f32, "mask_phi"), nx, ny, nz,
+                            float(dx), float(dt), int(nsub))
+    _ctx(items[0][0]).call("petch_advect_batch", len(items), ctypes.cast(arr, ctypes.c_void_p))
+
+
+def reinit_fsm_batch(items):
+    """reinit_fsm for many grids in one launch per stage (petch_reinit_fsm_batch). items: (phi,
+    dx, band) or (phi, dx, band, n_iter) tuples; each phi is replaced in place."""
+    import ctypes
+    from ._abi import ReinitItem
+    if not items:
+        return
+    arr = (ReinitItem * len(items))()
+    for k, it in enumerate(items):
+        phi, dx, band = it[:3]
+        n_iter = it[3] if len(it) > 3 and it[3] is not None else -1
+        nx, ny, nz = _shape3(phi)
+        arr[k] = ReinitItem(ptr(phi, f32, "phi"), nx, ny, nz, float(dx), float(band), int(n_iter))
+    _ctx(items[0][0]).call("petch_reinit_fsm_batch", len(items), ctypes.cast(arr, ctypes.c_void_p))
+
+
+def slice_gas_area(phi, dx, eps):
+    """Smoothed gas area of every z slice (float64 (nz,)): dx^2 * sum of H_eps(-phi) over the
+    slice (petch_profile_area). The equivalent radius of a hole is sqrt(area / pi)."""
+    nx, ny, nz = _shape3(phi)
+    area = torch.empty((nz,), dtype=torch.float64, device=phi.device)
+    _ctx(phi).call("petch_profile_area", ptr(phi, f32, "phi"), nx, ny, nz, float(dx), float(eps),
+                   ptr(area))
+    return area
+
+
+def slice_gas_area_adjoint(phi, dx, eps, garea):
+    """d loss / d phi (float32, phi's shape) of slice_gas_area for garea = d loss / d area."""
+    nx, ny, nz = _shape3(phi)
+    if garea.numel() != nz:
+        raise ValueError(f"slice_gas_area_adjoint: garea holds {garea.numel()} values, needs {nz}")
+    gphi = torch.empty_like(phi)
+    _ctx(phi).call("petch_profile_area_adjoint", ptr(phi, f32, "phi"), nx, ny, nz, float(dx),
+                   float(eps), ptr(garea.contiguous(), torch.float64, "garea"), ptr(gphi))
+    return gphi
 
 
 # ------------------------------------------------------------------ surface mesh
@@ -634,6 +713,29 @@ def nearest_face_value_batch(items, miss=1.0):
         items[0][0].ctx.call("petch_nearest_face_value_batch", len(items),
                              ctypes.cast(arr, ctypes.c_void_p))
     return outs
+
+
+def bvh_build_batch(items):
+    """Bvh.build for many meshes with their refits in one launch (petch_bvh_build_batch).
+    items: (Bvh, verts, faces) tuples."""
+    import ctypes
+    if not items:
+        return
+    n = len(items)
+    hs = (ctypes.c_void_p * n)()
+    vs = (ctypes.c_void_p * n)()
+    fs = (ctypes.c_void_p * n)()
+    nv = (ctypes.c_int * n)()
+    nf = (ctypes.c_int * n)()
+    for k, (bvh, v, f) in enumerate(items):
+        hs[k] = bvh.h.value
+        vs[k] = ptr(v, f32, "verts").value
+        fs[k] = ptr(f, i32, "faces").value
+        nv[k], nf[k] = int(v.shape[0]), int(f.shape[0])
+        bvh.nf = nf[k]
+    items[0][0].ctx.call("petch_bvh_build_batch", n, ctypes.cast(hs, ctypes.c_void_p),
+                         ctypes.cast(vs, ctypes.c_void_p), ctypes.cast(fs, ctypes.c_void_p),
+                         ctypes.cast(nv, ctypes.c_void_p), ctypes.cast(nf, ctypes.c_void_p))
 
 
 def nearest_face_value(bvh, src, pts, miss=1.0):

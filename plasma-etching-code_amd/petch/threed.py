@@ -263,10 +263,11 @@ def max_depth_3d(geo):
 _REINIT_WARNED = set()
 
 
-def _reinit(phi_d, dx, band, method):
+def _reinit_args(phi_d, dx, band, method):
+    """(band, n_iter) of the GPU Eikonal reinit that stands in for `method` (n_iter None: the
+    default ceil(band/dx)+4 sweeps)."""
     if method == "fsm":
-        ops.reinit_fsm(phi_d, dx, band)
-        return
+        return band, None
     if method in ("skfmm", "skfmm_full"):
         # scikit-fmm (CPU fast marching) is not part of the GPU path: the narrow-band FSM computes
         # the same distance within the band; 'skfmm_full' sweeps until the band spans the grid.
@@ -275,13 +276,15 @@ def _reinit(phi_d, dx, band, method):
                           RuntimeWarning)
             _REINIT_WARNED.add(method)
         if method == "skfmm_full":
-            full = dx * (max(phi_d.shape) + 4)
-            ops.reinit_fsm(phi_d, dx, full, int(max(phi_d.shape)) + 4)
-        else:
-            ops.reinit_fsm(phi_d, dx, band)
-        return
+            return dx * (max(phi_d.shape) + 4), int(max(phi_d.shape)) + 4
+        return band, None
     raise NotImplementedError(f"reinit_method={method!r} is not part of the HIP etch loop "
                               "(supported: 'fsm', 'skfmm', 'skfmm_full')")
+
+
+def _reinit(phi_d, dx, band, method):
+    b, n_iter = _reinit_args(phi_d, dx, band, method)
+    ops.reinit_fsm(phi_d, dx, b, n_iter)
 
 
 class _FluxSteps:
@@ -374,18 +377,19 @@ class EtchLoop:
         """(phi, dx, buffers) of this step's marching cubes for ops.marching_cubes_batch."""
         return (self.phi, self.dx, self.meshes[step % 2].bufs)
 
-    def begin_mesh(self, step, mc=None):
+    def begin_mesh(self, step, mc=None, bvh=True):
         """Mesh + BVH of the step. Returns the device flag 'all centroids finite' that gates the
         warm start (threed.py:1699), or None when the step needs no warm start. mc: this step's
         marching-cubes (verts, faces) when the caller already extracted them (the batched loop
-        meshes all features with one host read)."""
+        meshes all features with one host read); bvh=False leaves the BVH to the caller (the
+        batched loop builds all features' trees together)."""
         T, dx = self.timer, self.dx
         tm = time.time()
         with T.region("mesh"):
             if mc is None:
                 mesh = self.meshes[step % 2].build(self.phi, dx)
             else:
-                mesh = self.meshes[step % 2].build_from(*mc)
+                mesh = self.meshes[step % 2].build_from(*mc, bvh=bvh)
         self.timings['mesh'] += time.time() - tm
         self._mesh = mesh
         if self.rays_per_point is not None:
@@ -475,10 +479,13 @@ class EtchLoop:
             return None
         return (self._mesh.bvh, self._V, self.phi, self.dx, self.band, self.Fs)
 
-    def end_finish(self, step, vmx, nbad, bvh_err, extended=False):
+    def end_finish(self, step, vmx, nbad, bvh_err, extended=False, deferred=None):
         """Extension, CFL, advection and reinit, given the host reads of the rate reduction
         (max V, non-finite count) and of the BVH guard flag. extended: Fs already holds this
-        step's extension (the batched loop extends all features in one pass)."""
+        step's extension (the batched loop extends all features in one pass). deferred: a dict
+        of 'advect' and 'reinit' lists that receive this step's advection and reinit as items of
+        ops.advect_batch / ops.reinit_fsm_batch instead of running them (the batched loop runs
+        every feature's in one launch per substep and sweep)."""
         T, dx, dev = self.timer, self.dx, self.dev
         tim = self.timings
         mesh = self._mesh
@@ -510,16 +517,25 @@ class EtchLoop:
         tim['extend'] += time.time() - te
         nsub, Vmax = _flux.cfl_substeps(vmx, self.dt, dx)
         tim['nsub_max'] = max(tim['nsub_max'], nsub)
-        ta = time.time()
-        with T.region("advect"):
-            ops.advect(self.phi, self.Fs, self.mask, self.mask_phi, dx, self.dt / nsub, nsub,
-                       work=self.work)
-        tim['advect'] += time.time() - ta
-        if (step + 1) % self.reinit_every == 0 or step == self.n_steps - 1:
-            tr = time.time()
-            with T.region("reinit"):
-                _reinit(self.phi, dx, self.band + 2.0 * dx, self.reinit_method)
-            tim['reinit'] += time.time() - tr
+        reinit = (step + 1) % self.reinit_every == 0 or step == self.n_steps - 1
+        if deferred is not None:
+            deferred['advect'].append((self.phi, self.work, self.Fs, self.mask, self.mask_phi,
+                                       dx, self.dt / nsub, nsub))
+            if reinit:
+                deferred['reinit'].append(
+                    (self.phi, dx) + _reinit_args(self.phi, dx, self.band + 2.0 * dx,
+                                                  self.reinit_method))
+        else:
+            ta = time.time()
+            with T.region("advect"):
+                ops.advect(self.phi, self.Fs, self.mask, self.mask_phi, dx, self.dt / nsub, nsub,
+                           work=self.work)
+            tim['advect'] += time.time() - ta
+            if reinit:
+                tr = time.time()
+                with T.region("reinit"):
+                    _reinit(self.phi, dx, self.band + 2.0 * dx, self.reinit_method)
+                tim['reinit'] += time.time() - tr
         self.last = dict(nf=mesh.nf, nv=mesh.nv, Vmax=Vmax, nsub=nsub, n_ion=self._rays[0],
                          n_neu=self._rays[1])
         return self.last

@@ -16,8 +16,10 @@ class FakeLoop:
     def mesh_item(self, step):
         return ("mc", self.k)
 
-    def begin_mesh(self, step, mc=None):
-        self.mc = mc
+    def begin_mesh(self, step, mc=None, bvh=True):
+        self.mc, self.bvh_here = mc, bvh
+        self._mesh = types.SimpleNamespace(bvh_item=lambda: ("bvh", self.k),
+                                           bvh=types.SimpleNamespace(status_into=lambda t: None))
         return None
 
     def warm_start_item(self, finite):
@@ -31,14 +33,15 @@ class FakeLoop:
         self._out = out
         self.red = torch.zeros(2, dtype=torch.int32)
         self.dev = torch.device("cpu")
-        self._mesh = types.SimpleNamespace(bvh=types.SimpleNamespace(status_into=lambda t: None))
         return out is not None
 
     def extend_item(self):
         return ("ext", self.k)
 
-    def end_finish(self, step, vmx, nbad, err, extended=False):
+    def end_finish(self, step, vmx, nbad, err, extended=False, deferred=None):
         self.extended = extended
+        deferred['advect'].append(("adv", self.k))
+        deferred['reinit'].append(("re", self.k))
         return self.end_step(step, self._out)
 
     def begin_step(self, step):
@@ -80,21 +83,37 @@ def test_lockstep_batches_each_kind(monkeypatch):
     def ext_batch(items):
         calls.append(("ext", list(items)))
 
+    def bvh_batch(items):
+        calls.append(("bvh", list(items)))
+
+    def adv_batch(items):
+        calls.append(("adv", list(items)))
+
+    def re_batch(items):
+        calls.append(("re", list(items)))
+
     monkeypatch.setattr(B.ops, "trace_ion_yield_batch", ion_batch)
     monkeypatch.setattr(B.ops, "trace_neutral_rr_batch", neutral_batch)
     monkeypatch.setattr(B.ops, "marching_cubes_batch", mc_batch)
     monkeypatch.setattr(B.ops, "nearest_face_value_batch", ws_batch)
     monkeypatch.setattr(B.ops, "extend_batch", ext_batch)
+    monkeypatch.setattr(B.ops, "bvh_build_batch", bvh_batch)
+    monkeypatch.setattr(B.ops, "advect_batch", adv_batch)
+    monkeypatch.setattr(B.ops, "reinit_fsm_batch", re_batch)
     monkeypatch.setattr(B.ops, "decode_rate_reduction", lambda r: (1.0, 0))
     loops = [FakeLoop(0, 1), FakeLoop(1, 4), FakeLoop(99, 0), FakeLoop(2, 1)]
     outs = B.BatchLoop(loops).step(0)
-    # one marching-cubes pass for all features, one warm-start launch for the features that have
-    # a lookup, one launch per trace kind and fixed-point round, one extension pass for every
-    # feature with a surface rate
-    assert calls == [("mc", [("mc", 0), ("mc", 1), ("mc", 99), ("mc", 2)]),
+    # one marching-cubes pass and one BVH build call for all features, one warm-start launch for
+    # the features that have a lookup, one launch per trace kind and fixed-point round, one
+    # extension pass for every feature with a surface rate, then the advection and the reinit of
+    # every feature, one call each
+    every = lambda tag: [(tag, k) for k in (0, 1, 99, 2)]
+    assert calls == [("mc", every("mc")), ("bvh", every("bvh")),
                      ("ws", [("ws", 1)]), ("ion", [0, 1, 2]), ("neutral", [0, 1, 2]),
                      ("neutral", [1]), ("neutral", [1]), ("neutral", [1]),
-                     ("ext", [("ext", 0), ("ext", 1), ("ext", 2)])]
+                     ("ext", [("ext", 0), ("ext", 1), ("ext", 2)]),
+                     ("adv", every("adv")), ("re", every("re"))]
+    assert not any(lp.bvh_here for lp in loops)
     assert [lp.mc for lp in loops] == [("mesh", 0), ("mesh", 1), ("mesh", 99), ("mesh", 2)]
     assert [lp.bare_init for lp in loops] == [None, ("bare", 1), None, None]
     assert [lp.extended for lp in loops] == [True, True, False, True]

@@ -110,3 +110,41 @@ def test_tiny_meshes():
     assert face.cpu().tolist() == [0, 1, -1]
     pf, pd = b.query_point(torch.tensor([[0.1, 0.1, 0.5]], device=dev))
     assert pf.cpu().tolist() == [0] and pd.cpu().numpy()[0] == pytest.approx(0.5)
+
+
+def test_batched_build_equals_single_builds():
+    """petch_bvh_build_batch (parallel stages per tree, the refits of all trees in one launch)
+    gives trees that answer ray and point queries exactly as single builds do, with the same
+    root bounds, including a tree of one face, an empty mesh and a clustered build."""
+    import torch
+    from petch import ops
+    dev = _dev()
+    meshes = [_mesh("sphere")[:2], _mesh("hole56")[:2], _mesh("trench160")[:2]]
+    one = (np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.int32))
+    empty = (np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32))
+    meshes += [one, empty]
+    tv = [(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev)) for v, f in meshes]
+    single = [ops.Bvh(dev).build(v, f) for v, f in tv]
+    batched = [ops.Bvh(dev) for _ in tv]
+    batched[2].set_quality(2)
+    single[2].set_quality(2).build(*tv[2])
+    ops.bvh_build_batch([(b, v, f) for b, (v, f) in zip(batched, tv)])
+    rng = np.random.default_rng(5)
+    for (v, f), a, b in zip(tv, single, batched):
+        assert a.status() == 0 and b.status() == 0
+        if int(f.shape[0]) > 1:
+            np.testing.assert_array_equal(a.root_bounds(), b.root_bounds())
+        if int(f.shape[0]) == 0:
+            continue
+        lo = v.min(0).values.cpu().numpy()
+        hi = v.max(0).values.cpu().numpy()
+        o = rng.uniform(lo - 0.5, hi + 0.5, (3000, 3)).astype(np.float32)
+        d = rng.normal(size=(3000, 3))
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        to, td = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+        fa, ta = a.query_ray(to, td)
+        fb, tb = b.query_ray(to, td)
+        assert torch.equal(fa, fb) and torch.equal(ta, tb)
+        pa, da = a.query_point(to)
+        pb, db = b.query_point(to)
+        assert torch.equal(pa, pb) and torch.equal(da, db)

@@ -171,3 +171,116 @@ def test_advect_adjoint_matches_oracle():
         an = float((f.grad.cpu().numpy().astype(np.float64) * u).sum())
         print(f"F direction {k}: fd {fd:.6f} adjoint {an:.6f}", flush=True)
         assert abs(fd - an) <= 2e-3 * abs(fd) + 1e-3
+
+
+def test_profile_area_and_its_adjoint_match_the_oracle():
+    """petch_profile_area (smoothed gas area per z slice, fixed-order float64 sums) and its
+    adjoint against the oracle on the small hole's initial level set, and the autograd profile
+    loss (sum over the hole's slices of (r(z) - r*(z))^2) against the oracle's chain."""
+    import torch
+    from petch import adjoint, ops
+    dx = 0.25
+    phi = ol.init_sdf(14.0, 14.0, 24.0, dx, 6.0, 2.0, 18.0, True).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    tphi = torch.from_numpy(phi).to(dev)
+    eps = 1.5 * dx
+    A = ops.slice_gas_area(tphi, dx, eps).cpu().numpy()
+    Ao = ol.slice_gas_area(phi.astype(np.float64), dx, eps)
+    np.testing.assert_allclose(A, Ao, rtol=1e-12, atol=1e-12)
+    rng = np.random.default_rng(8)
+    garea = rng.normal(size=A.shape)
+    g = ops.slice_gas_area_adjoint(tphi, dx, eps, torch.from_numpy(garea).to(dev)).cpu().numpy()
+    go = ol.slice_gas_area_adjoint(phi.astype(np.float64), dx, eps, garea)
+    np.testing.assert_allclose(g, go, rtol=1e-6, atol=1e-7 * np.abs(go).max())
+    # the profile loss over the hole's slices against a target 0.3 um narrower
+    nz = phi.shape[2]
+    zsel = (np.arange(nz) * dx >= 10.0) & (np.arange(nz) * dx < 18.0)
+    target = np.full(nz, 2.7)
+    x = tphi.clone().requires_grad_(True)
+    loss = adjoint.profile_loss(x, dx, torch.from_numpy(target), torch.from_numpy(zsel), eps)
+    loss.backward()
+    ro = ol.profile_rz(phi.astype(np.float64), dx, eps)
+    lo = float(((ro - target)[zsel] ** 2).sum())
+    assert float(loss) == pytest.approx(lo, rel=1e-10)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        gr = np.where(zsel & (ro > 0), 2 * (ro - target) / (2 * np.pi * ro), 0.0)
+    gref = ol.slice_gas_area_adjoint(phi.astype(np.float64), dx, eps, gr)
+    got = x.grad.cpu().numpy()
+    np.testing.assert_allclose(got, gref, rtol=1e-5, atol=1e-7 * np.abs(gref).max())
+    assert np.abs(got).max() > 0
+
+
+def test_batched_advection_and_reinit_equal_single_calls():
+    """petch_advect_batch and petch_reinit_fsm_batch (one launch per substep / sweep for many
+    grids) give exactly what petch_advect and petch_reinit_fsm give grid by grid: grids of
+    different shapes, substep counts 0-3 (odd counts end in the work buffer and are copied
+    back) and sweep counts."""
+    import torch
+    from petch import ops
+    dev = _dev()
+    rng = np.random.default_rng(9)
+    shapes = [(30, 22, 41), (17, 33, 25), (40, 8, 12)]
+    grids = []
+    for k, (nx, ny, nz) in enumerate(shapes):
+        dx = 0.1 + 0.05 * k
+        phi = ol.init_sdf(nx * dx, ny * dx, nz * dx, dx, nx * dx / 3, 3 * dx, nz * dx / 2,
+                          k % 2 == 0).astype(np.float32)
+        phi += (0.05 * dx * rng.normal(size=phi.shape)).astype(np.float32)
+        F = np.where(np.abs(phi) < 4 * dx, rng.uniform(0.5, 1.5, phi.shape), 0.0).astype(np.float32)
+        mask = (rng.random(phi.shape) < 0.05).astype(np.uint8)
+        grids.append((dx, torch.from_numpy(phi).to(dev), torch.from_numpy(F).to(dev),
+                      torch.from_numpy(mask).to(dev)))
+    for nsubs in ((1, 2, 3), (0, 3, 2)):
+        ref, items = [], []
+        for (dx, phi, F, mask), nsub in zip(grids, nsubs):
+            a = phi.clone()
+            ops.advect(a, F, mask, phi, dx, 0.3 * dx, nsub)
+            ref.append(a)
+            b = phi.clone()
+            items.append((b, torch.empty_like(b), F, mask, phi, dx, 0.3 * dx, nsub))
+        ops.advect_batch(items)
+        for r, it in zip(ref, items):
+            assert torch.equal(r, it[0])
+    ref, items = [], []
+    for (dx, phi, F, mask), n_iter in zip(grids, (None, 7, 10)):
+        a = phi.clone()
+        ops.reinit_fsm(a, dx, 6 * dx, n_iter)
+        ref.append(a)
+        b = phi.clone()
+        items.append((b, dx, 6 * dx, n_iter))
+    ops.reinit_fsm_batch(items)
+    for r, it in zip(ref, items):
+        assert torch.equal(r, it[0])
+
+
+@pytest.mark.parametrize("g", [GEOMS[0], GEOMS[2]])
+def test_reinit_adjoint_matches_the_oracle(g):
+    """petch_reinit_fsm_adjoint against the oracle's exact adjoint (pinned by central
+    differences and the plane shift) on the perturbed hole and the ragged case, band of the
+    etch loop, a random cotangent in the band. The float32 sweep states can take a sweep's
+    'lowered' decision one sweep apart from the float64 oracle's where a node converges within
+    rounding; the gradient then reaches the same neighbours."""
+    import torch
+    from petch import adjoint, ops
+    dx = g[3]
+    phi, _, _ = _perturbed_state(g, 3)
+    band = 4 * dx + 2.0 * dx
+    rng = np.random.default_rng(5)
+    go = (rng.standard_normal(phi.shape) * (np.abs(phi) < band + dx)).astype(np.float32)
+    print(f"reinit adjoint shape={phi.shape} band={band}", flush=True)
+    d = _dev()
+    tp = torch.from_numpy(phi).to(d)
+    got = ops.reinit_fsm_adjoint(tp, dx, band, torch.from_numpy(go).to(d)).cpu().numpy()
+    ref = ol.reinit_fsm_adjoint(phi.astype(np.float64), dx, band, go.astype(np.float64))
+    rel = np.abs(got - ref).sum() / np.abs(ref).sum()
+    print(f"L1 rel {rel:.2e}; max abs {np.abs(got - ref).max():.2e} of {np.abs(ref).max():.2e}",
+          flush=True)
+    assert rel < 1e-4
+    # the autograd wrapper: forward is ops.reinit_fsm out of place, backward the same adjoint
+    a = tp.clone().requires_grad_(True)
+    r = adjoint.reinit(a, dx, band)
+    ref_fwd = tp.clone()
+    ops.reinit_fsm(ref_fwd, dx, band)
+    assert torch.equal(r.detach(), ref_fwd) and torch.equal(a.detach(), tp)
+    r.backward(torch.from_numpy(go).to(d))
+    assert torch.equal(a.grad, torch.from_numpy(got).to(d))
