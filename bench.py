@@ -403,6 +403,52 @@ def step_adjoint_cost(reps=2, nsub=3):
             **res, "ratio": res["fwd_adj_ms"] / res["fwd_ms"]}
 
 
+def gradient_cost(reps=2, n_rays=200_000, checkpoint_every=5):
+    """Cost of the differentiable 20-step hole (petch.diffetch, deterministic mode): the small
+    hole at dx 0.2 um, 3.0 s over 20 steps, 200k ions and 200k F + 200k O neutrals per round;
+    fwd = the profile loss of the final level set, fwd_adj = the loss and its gradient in
+    (Emean, Fflux, Oflux, rate_scale), ckpt = the same with segments of 5 steps recomputed in the
+    backward pass. Each is run once untimed first."""
+    import torch
+    from petch import diffetch as de, ops
+    from petch.api import SF6O2
+    from petch.params import Flags
+    ops.set_deterministic(True)
+    try:
+        m = SF6O2()
+        case = dict(Lx=14.0, Ly=14.0, Lz=24.0, dx=0.2, trench_width=6.0, mask_th=2.0,
+                    sub_top=18.0, hole=True, t_end=3.0, n_steps=20)
+        dd = de.DiffEtch(**case, par=dict(m.par), flags=Flags(**m.flags), n_ion=n_rays,
+                         n_neu=n_rays)
+        tg = de.DiffEtch(**case, par=dict(m.par), flags=Flags(**m.flags), n_ion=n_rays,
+                         n_neu=n_rays, seed_offset=1234)
+        target = de.profile_target(tg, tg.theta0 * torch.tensor([1.1, 0.9, 1.1, 1.1],
+                                                                dtype=torch.float64))
+        zsel = de.zsel_feature(dd, below=8.0)
+        runs = {"fwd_s": lambda: de.loss_only(dd, dd.theta0, target, zsel),
+                "fwd_adj_s": lambda: de.loss_and_gradient(dd, dd.theta0, target, zsel),
+                "ckpt_s": lambda: de.loss_and_gradient(dd, dd.theta0, target, zsel,
+                                                       checkpoint_every=checkpoint_every)}
+        res = {}
+        for key, fn in runs.items():
+            fn()
+            torch.cuda.synchronize()
+            t = time.time()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            res[key] = (time.time() - t) / reps
+        _, g = de.loss_and_gradient(dd, dd.theta0, target, zsel)
+    finally:
+        ops.set_deterministic(False)
+    return {"scope": "20-step differentiable hole (dx 0.2 um, 70x70x120, 200k ions + 2 x 200k "
+                     "neutrals per round), profile loss, gradient in Emean, Fflux, Oflux, "
+                     "rate_scale; deterministic mode",
+            **res, "ratio": res["fwd_adj_s"] / res["fwd_s"],
+            "ratio_checkpointed": res["ckpt_s"] / res["fwd_s"],
+            "gradient": [float(x) for x in g]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -419,7 +465,8 @@ def main(argv=None):
     if a.adjoint_only:
         print(json.dumps({"adjoint": adjoint_cost(), "adjoint_radiosity": radiosity_adjoint_cost(),
                           "adjoint_neutral": neutral_adjoint_cost(),
-                          "adjoint_step": step_adjoint_cost()}), flush=True)
+                          "adjoint_step": step_adjoint_cost(),
+                          "gradient": gradient_cost()}), flush=True)
         return
     if a.copy_probe:
         import torch
@@ -510,6 +557,7 @@ def main(argv=None):
         res["adjoint_radiosity"] = radiosity_adjoint_cost()
         res["adjoint_neutral"] = neutral_adjoint_cost()
         res["adjoint_step"] = step_adjoint_cost()
+        res["gradient"] = gradient_cost()
     if not a.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline()

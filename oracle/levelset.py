@@ -227,7 +227,7 @@ def _eik_weights(a, U, dx):
     return x, w
 
 
-def reinit_fsm_adjoint(phi, dx, band, gout, n_iter=None):
+def reinit_fsm_adjoint(phi, dx, band, gout, n_iter=None, state_dtype=np.float64):
     """d loss / d phi of reinit_fsm(phi, dx, band) for d loss / d out = gout: the exact adjoint
     of the piecewise-smooth map the sweeps compute, with every discrete choice (interface flags,
     the smaller neighbour per axis, the active Godunov terms, whether a sweep lowers a node,
@@ -236,15 +236,20 @@ def reinit_fsm_adjoint(phi, dx, band, gout, n_iter=None):
     neighbours' values at k, whose derivative is the Godunov weight; the frozen seeds
     |phi|/max(|grad phi|_central, 0.5) pass their derivative to the node and its six
     neighbours. Not in the reference, which differentiates nothing in its etch loop; pinned by
-    central differences of reinit_fsm."""
+    central differences of reinit_fsm.
+
+    state_dtype=np.float32 rounds every sweep state (and each Godunov root) to float32 before
+    the next sweep reads it, as a device that stores the states in float32 does; its decisions
+    (which sweep lowers a node) then follow that rounding."""
     c = np.asarray(phi, np.float64)
     big = float(band + 4.0 * dx)
     if n_iter is None:
         n_iter = int(np.ceil(band / dx)) + 4
     U, frozen = eik_init(c, dx, big)
+    U = U.astype(state_dtype).astype(np.float64)
     states = [U]
     for _ in range(n_iter):
-        U = eik_jacobi(U, frozen, dx, big)
+        U = eik_jacobi(U, frozen, dx, big).astype(state_dtype).astype(np.float64)
         states.append(U)
     sgn = np.where(np.sign(c) == 0, 1.0, np.sign(c))
     G = np.where(states[-1] < band + dx, np.asarray(gout, np.float64) * sgn, 0.0)
@@ -254,7 +259,7 @@ def reinit_fsm_adjoint(phi, dx, band, gout, n_iter=None):
         hi = [_nbr(Uk, ax, +1, big) for ax in range(3)]
         a = np.stack([np.minimum(lo[ax], hi[ax]) for ax in range(3)])
         x, w = _eik_weights(a, Uk, dx)
-        took = ~frozen & (x < Uk)
+        took = ~frozen & (x.astype(state_dtype).astype(np.float64) < Uk)
         Gt = np.where(took, G, 0.0)
         Gn = np.where(took, 0.0, G)
         for ax in range(3):

@@ -202,7 +202,7 @@ def loss_and_gradient(dd, theta, target, zsel, eps=None, checkpoint_every=None, 
     phi = dd.run(th, checkpoint_every=checkpoint_every, record=record)
     loss = adjoint.profile_loss(phi, dd.dx, target, zsel, eps)
     loss.backward()
-    return float(loss), th.grad.detach().clone()
+    return float(loss.detach()), th.grad.detach().clone()
 
 
 def loss_only(dd, theta, target, zsel, eps=None, frozen=None):

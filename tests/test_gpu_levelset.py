@@ -257,9 +257,11 @@ def test_batched_advection_and_reinit_equal_single_calls():
 def test_reinit_adjoint_matches_the_oracle(g):
     """petch_reinit_fsm_adjoint against the oracle's exact adjoint (pinned by central
     differences and the plane shift) on the perturbed hole and the ragged case, band of the
-    etch loop, a random cotangent in the band. The float32 sweep states can take a sweep's
-    'lowered' decision one sweep apart from the float64 oracle's where a node converges within
-    rounding; the gradient then reaches the same neighbours."""
+    etch loop, a random cotangent in the band. The device stores the sweep states in float32,
+    so where a node converges within rounding its last lowering comes a sweep earlier than in
+    float64, and the gradient reaches the same seeds by another path: against the float64
+    oracle that moves 1e-4 to 8e-4 of the L1 mass. The oracle with float32 states takes the
+    device's decisions and is the tight comparison."""
     import torch
     from petch import adjoint, ops
     dx = g[3]
@@ -271,11 +273,15 @@ def test_reinit_adjoint_matches_the_oracle(g):
     d = _dev()
     tp = torch.from_numpy(phi).to(d)
     got = ops.reinit_fsm_adjoint(tp, dx, band, torch.from_numpy(go).to(d)).cpu().numpy()
-    ref = ol.reinit_fsm_adjoint(phi.astype(np.float64), dx, band, go.astype(np.float64))
+    ref = ol.reinit_fsm_adjoint(phi.astype(np.float64), dx, band, go.astype(np.float64),
+                                state_dtype=np.float32)
+    ref64 = ol.reinit_fsm_adjoint(phi.astype(np.float64), dx, band, go.astype(np.float64))
     rel = np.abs(got - ref).sum() / np.abs(ref).sum()
-    print(f"L1 rel {rel:.2e}; max abs {np.abs(got - ref).max():.2e} of {np.abs(ref).max():.2e}",
-          flush=True)
-    assert rel < 1e-4
+    rel64 = np.abs(got - ref64).sum() / np.abs(ref64).sum()
+    print(f"L1 rel {rel:.2e} (float64-state oracle {rel64:.2e}); max abs "
+          f"{np.abs(got - ref).max():.2e} of {np.abs(ref).max():.2e}", flush=True)
+    assert rel < 1e-5
+    assert rel64 < 2e-3
     # the autograd wrapper: forward is ops.reinit_fsm out of place, backward the same adjoint
     a = tp.clone().requires_grad_(True)
     r = adjoint.reinit(a, dx, band)

@@ -291,6 +291,7 @@ def test_reinit_adjoint_of_a_shifted_plane():
             g = rng.standard_normal(shape)
             r = ol.reinit_fsm(phi, dx, band)
             assert np.abs(r - np.clip(phi, -(band + dx), band + dx)).max() < 1e-12
-            gc = ol.reinit_fsm_adjoint(phi, dx, band, g)
             live = np.abs(r) < band + dx
-            assert abs(gc.sum() - g[live].sum()) < 1e-9 * np.abs(g).sum(), (ax, sgn)
+            for dt in (np.float64, np.float32):
+                gc = ol.reinit_fsm_adjoint(phi, dx, band, g, state_dtype=dt)
+                assert abs(gc.sum() - g[live].sum()) < 1e-9 * np.abs(g).sum(), (ax, sgn, dt)
