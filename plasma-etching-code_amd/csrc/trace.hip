@@ -157,6 +157,17 @@ __device__ __forceinline__ bool wrap_ray(const Box& b, Vec3& o, Vec3 d) {
 
 constexpr int kMaxWraps = 256;
 
+// A ray below all triangles and not rising (or above all and not falling) can meet no triangle
+// again: a wrap moves it only in x or y. _apply_bc would wrap it until it escapes through the
+// top or runs out of wraps, with every query a miss, so ending it at once gives the same result.
+// On a trench whose mesh stops one cell short of the periodic faces (the grid spans
+// [0, (n-1) dx], the period n dx), the rays that slip through that strip fall below the floor and
+// would spend the 256 wraps there: over half of all queries of a neutral trace.
+__device__ __forceinline__ bool never_returns(const BvhView& bv, Vec3 o, Vec3 d) {
+  if (!bv.root) return false;
+  return (o.z < bv.root[2] && d.z <= 0.f) || (o.z > bv.root[5] && d.z >= 0.f);
+}
+
 // _apply_bc + mesh_query_ray: with periodic lateral faces a ray that misses the surface and
 // leaves through x or y before the open top re-enters at the opposite face (<= 256 wraps, then
 // one last query whose result stands).
@@ -170,7 +181,7 @@ __device__ __forceinline__ Hit trace_bc(const BvhView& bv, const Box& b, Vec3& o
       Hit h = petch::bvh_closest_hit(bv, r, 1.0e6f, stk);
       if (h.face >= 0) return h;
     }
-    if (!wrap_ray(b, o, d)) return Hit{-1, -1, 1.0e6f};
+    if (never_returns(bv, o, d) || !wrap_ray(b, o, d)) return Hit{-1, -1, 1.0e6f};
   }
   return petch::bvh_closest_hit(bv, o, d, 1.0e6f);
 }
@@ -322,7 +333,8 @@ __device__ __forceinline__ void persistent_trace(const P& pol, const RayQueue& q
       const Box& b = pol.box(R);
       if (T.h.face >= 0) {
         st = kHit;
-      } else if (b.periodic && wraps < kMaxWraps && wrap_ray(b, o, d, rp.inv)) {
+      } else if (b.periodic && wraps < kMaxWraps && !never_returns(pol.view(R), o, d) &&
+                 wrap_ray(b, o, d, rp.inv)) {
         petch::ray_move(rp, o);
         ++wraps;
         petch::trav_begin(pol.view(R), T, rp, 1.0e6f);
