@@ -97,6 +97,20 @@ def measured_traffic():
         return None, None
 
 
+def _cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name} ({os.cpu_count()} logical CPUs visible)"
+
+
 def cpu_baseline(scale=1):
     """Oracle (C/OpenMP traces + FP64 numpy level set) on one step of the same workload; with
     scale > 1 the ray counts are divided by `scale` and the ray-proportional time scaled back."""
@@ -134,7 +148,7 @@ def cpu_baseline(scale=1):
     t_rest = time.time() - t0
     step_s = t_mesh + t_rest + t_flux * scale
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=1.0 / step_s, unit="steps/s", cores=cores, kind="port",
+    return dict(value=1.0 / step_s, unit="steps/s", cores=cores, kind="port", cpu=_cpu_model(),
                 sample=(f"one etch step of the same trench state on the oracle: marching cubes, "
                         f"rate, extension, {nsub} advection substeps and FSM reinit at full size "
                         f"({t_mesh + t_rest:.1f} s) plus the coupled flux with "

@@ -209,8 +209,12 @@ __device__ __forceinline__ void put_child_box(BvhNode* nd, bool left, const floa
   }
 }
 
-// Leaf s climbs towards the root; at each node the second arriving thread (agent-scope acq_rel
-// flag) writes the node's box from both children and goes on, the first one stops.
+// Leaf s climbs towards the root; at each node the second arriving thread (acq_rel flag at
+// memory scope kScope) writes the node's box from both children and goes on, the first one
+// stops. Threads of the whole grid need agent scope; within one workgroup (all threads on one
+// CU, one L2) workgroup scope suffices and spares the L2 write-back an agent-scope release
+// costs on a chip whose XCDs keep separate L2s.
+template <int kScope = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ void refit_leaf(const float4* __restrict__ tris, BvhNode* nodes,
                                            const int* __restrict__ pint,
                                            const int* __restrict__ pleaf, int* flags, int n,
@@ -224,7 +228,7 @@ __device__ __forceinline__ void refit_leaf(const float4* __restrict__ tris, BvhN
     BvhNode* nd = nodes + p;
     bool isleft = nd->d.x == child;
     put_child_box(nd, isleft, lo, hi);
-    int old = __hip_atomic_fetch_add(flags + p, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    int old = __hip_atomic_fetch_add(flags + p, 1, __ATOMIC_ACQ_REL, kScope);
     if (old == 0) return;  // the sibling's thread finishes this node
     const float* q = (const float*)nd;
     for (int k = 0; k < 3; ++k) {
@@ -250,7 +254,7 @@ __global__ void k_bvh_refit(const float4* __restrict__ tris, BvhNode* nodes,
 
 // The refits of many BVHs in one launch: the climbs are chains of dependent global accesses
 // (tens of microseconds for one tree however small), so the trees of a batched step refit side
-// by side instead of one after the other. Tree t owns the workgroups [blk0[t], blk0[t+1]).
+// by side instead of one after the other.
 struct RefitItem {
   const float4* tris;
   BvhNode* nodes;
@@ -258,19 +262,16 @@ struct RefitItem {
   int* flags;
   float* root;
   int n;
-  int blk0;
 };
 
-__global__ void k_bvh_refit_many(const RefitItem* __restrict__ items, int n_items) {
-  int lo = 0, hi = n_items - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].blk0 <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
-  }
-  const RefitItem& it = items[lo];
-  const int s = ((int)blockIdx.x - it.blk0) * kBS + threadIdx.x;
-  if (s >= it.n) return;
-  refit_leaf(it.tris, it.nodes, it.pint, it.pleaf, it.flags, it.n, it.root, s);
+// One workgroup per tree (the trees of a batched build): the workgroup's threads climb from
+// all of the tree's leaves in turn with workgroup-scope flags, and the trees refit side by side.
+constexpr int kRefitWG = 1024;
+__global__ void __launch_bounds__(kRefitWG) k_bvh_refit_wg(const RefitItem* __restrict__ items) {
+  const RefitItem& it = items[blockIdx.x];
+  for (int s = threadIdx.x; s < it.n; s += kRefitWG)
+    refit_leaf<__HIP_MEMORY_SCOPE_WORKGROUP>(it.tris, it.nodes, it.pint, it.pleaf, it.flags, it.n,
+                                             it.root, s);
 }
 
 __global__ void k_bvh_wide(const BvhNode* __restrict__ nodes, const int* __restrict__ pint, int n,
@@ -852,7 +853,7 @@ int build_prepare(petch_ctx* ctx, petch_bvh* b, const float* verts, const int* f
     PETCH_LAUNCH_CHECK(ctx);
     PETCH_HIP(ctx, hipMemsetAsync(w.flags, 0, (size_t)nf * sizeof(int), ctx->stream));
     *refit = RefitItem{(const float4*)b->tris, (BvhNode*)b->nodes, w.pint, w.pleaf, w.flags,
-                       w.root, nf, 0};
+                       w.root, nf};
   }
   return PETCH_OK;
 }
@@ -906,7 +907,6 @@ extern "C" int petch_bvh_build_batch(petch_ctx* ctx, int n, petch_bvh* const* bv
   RefitItem* items = (RefitItem*)calloc((size_t)n, sizeof(RefitItem));
   if (!items) return petch_set_error(ctx, PETCH_E_HIP, "bvh_build_batch: out of host memory");
   int m = 0;
-  long long blocks = 0;
   for (int t = 0; t < n; ++t) {
     RefitItem r;
     const int rc = build_prepare(ctx, bvhs[t], verts[t], faces[t], nv[t], nf[t], &r);
@@ -914,11 +914,7 @@ extern "C" int petch_bvh_build_batch(petch_ctx* ctx, int n, petch_bvh* const* bv
       free(items);
       return rc;
     }
-    if (r.n > 1) {
-      r.blk0 = (int)blocks;
-      blocks += petch_blocks(r.n, kBS);
-      items[m++] = r;
-    }
+    if (r.n > 1) items[m++] = r;
   }
   if (m > 0) {
     const RefitItem* d = (const RefitItem*)petch_upload_table(ctx, items,
@@ -927,7 +923,7 @@ extern "C" int petch_bvh_build_batch(petch_ctx* ctx, int n, petch_bvh* const* bv
       free(items);
       return PETCH_E_HIP;
     }
-    hipLaunchKernelGGL(k_bvh_refit_many, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d, m);
+    hipLaunchKernelGGL(k_bvh_refit_wg, dim3((unsigned)m), dim3(kRefitWG), 0, ctx->stream, d);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       free(items);
