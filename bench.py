@@ -195,10 +195,11 @@ def small_hole():
             "speedup_vs_reference": SMALL_HOLE_REF_WALL_S / w}
 
 
-def batched(n_feat=64, n_steps=40):
+def batched(n_feat=64, n_steps=40, sequential=True):
     """The batched configuration, whole runs (40 steps over 3.0 s): all features in lockstep with
     one launch per trace (petch.batch) against the same features run one after the other through
-    run_etch_3d. Same streams, so the depths must agree."""
+    run_etch_3d. Same streams, so the depths must agree. sequential=False times the batched run
+    only (for profiling it)."""
     import torch
     from petch import batch as pb
     from petch import threed as t3
@@ -210,6 +211,11 @@ def batched(n_feat=64, n_steps=40):
     gb = pb.run_etch_3d_batch(feats, t_end=t_end, n_steps=n_steps)
     torch.cuda.synchronize()
     wb = time.time() - t0
+    if not sequential:
+        lb = gb[0]['timings']['total']
+        return {"workload": f"{n_feat} seeded holes/trenches, {n_steps} steps over {t_end:g} s",
+                "batched_steps_s": lb, "features_per_s_batched": n_feat / lb,
+                "batched_wall_s": wb}
     t0 = time.time()
     gs = [t3.run_etch_3d(**ft, t_end=t_end, n_steps=n_steps, reinit_method="fsm", verbose=False)
           for ft in feats]
@@ -473,9 +479,15 @@ def main(argv=None):
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run plain device copies of 256 MiB (reference for PMC byte counts)")
+    ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
+                    help="only run the batched 64-feature etch for STEPS steps (one JSON line; "
+                         "for profiling)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the differentiable pieces (one JSON line; for profiling)")
     a = ap.parse_args(argv)
+    if a.batch_only:
+        print(json.dumps({"batch": batched(n_steps=a.batch_only, sequential=False)}), flush=True)
+        return
     if a.adjoint_only:
         print(json.dumps({"adjoint": adjoint_cost(), "adjoint_radiosity": radiosity_adjoint_cost(),
                           "adjoint_neutral": neutral_adjoint_cost(),

@@ -34,6 +34,8 @@ enum PetchScratch {
   SCR_EIK_STATES,    // reinit adjoint: every sweep's Eikonal state
   SCR_EIK_W,         // reinit adjoint: per-node Godunov weights of one sweep
   SCR_EIK_G,         // reinit adjoint: adjoint ping-pong
+  SCR_LS_BRICK,      // batched level-set stages: per-brick activity flags and list counters
+  SCR_LS_LIST,       // batched level-set stages: lists of the bricks a launch processes
   SCR_COUNT
 };
 

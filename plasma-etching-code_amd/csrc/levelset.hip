@@ -312,10 +312,13 @@ __global__ void k_eik_final(float* __restrict__ phi, const float* __restrict__ U
 }
 
 // ------------------------------------------------------------------------------ many grids
-// One launch for the same stage of many grids (the features of a batched step): grid f owns the
-// workgroups [blk0[f], blk0[f+1]) of the launch, found by a binary search that is uniform across
-// the workgroup, and its node n is the workgroup's offset in that range times kBS plus the lane.
-// A stage that a grid does not take part in (fewer substeps or sweeps) exits at once.
+// One launch for the same stage of many grids (the features of a batched step). Full passes map
+// workgroups to nodes: grid f owns the workgroups [nblk0[f], nblk0[f+1]) of the launch, found by
+// a binary search that is uniform across the workgroup, and its node n is the workgroup's
+// offset in that range times kBS plus the lane. Stages that leave most of a grid unchanged
+// (advection substeps, Jacobi sweeps) work on bricks instead: a list of (grid, brick) entries is
+// built on the device for the launch, and a fixed number of workgroups loops over it, so the
+// bricks left out cost neither a workgroup dispatch nor a memory access.
 struct LsGrid {
   float* phi;
   float* work;               // advection ping-pong partner of phi
@@ -324,64 +327,222 @@ struct LsGrid {
   const float* mask_phi;
   float *U, *U2;             // reinit workspace slices
   uint8_t* fr;
+  uint8_t* bfl;              // per-brick activity flags (one plane, or count + 1 for the reinit)
   int nx, ny, nz;
+  int bnx, bny, bnz;         // bricks per axis
   int count;                 // advection substeps or Jacobi sweeps
   double dx;
   float inv_dx, dt, big, clampv;
-  long long blk0;            // first workgroup of this grid
+  long long nblk0;           // first workgroup of this grid in a full pass
+  long long bblk0;           // first brick of this grid in the launch's brick numbering
 };
 
-__device__ __forceinline__ int grid_of_block(const LsGrid* __restrict__ g, int n) {
+// Bricks are 4 x 4 x 16 nodes (z fastest inside, bk fastest across); in a brick launch lane t
+// takes node (4 bi + t / 64, 4 bj + (t / 16) % 4, 16 bk + t % 16), so each 16-lane group reads a
+// 64-byte run of one z column. A node's six neighbours lie in its brick or a face neighbour.
+constexpr int kBX = 4, kBY = 4, kBZ = 16;
+static_assert(kBX * kBY * kBZ == kBS, "one brick per workgroup pass");
+constexpr int kListWG = 2048;   // workgroups of a brick-list launch (8 per CU)
+constexpr int kHalo = 0x40000000;  // list entry flag: copy the brick, do not update it
+
+__device__ __forceinline__ int grid_of_nodes(const LsGrid* __restrict__ g, int n) {
   int lo = 0, hi = n - 1;
   const long long b = blockIdx.x;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (g[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+    if (g[mid].nblk0 <= b) lo = mid; else hi = mid - 1;
   }
   return lo;
 }
 
+__device__ __forceinline__ int grid_of_brick(const LsGrid* __restrict__ g, int n, long long gb) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].bblk0 <= gb) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ long long nbricks(const LsGrid& G) {
+  return (long long)G.bnx * G.bny * G.bnz;
+}
+
+__device__ __forceinline__ long long brick_of_node(const LsGrid& G, long long n) {
+  const int k = (int)(n % G.nz);
+  const int j = (int)((n / G.nz) % G.ny);
+  const int i = (int)(n / ((long long)G.nz * G.ny));
+  return ((long long)(i / kBX) * G.bny + j / kBY) * G.bnz + k / kBZ;
+}
+
+__device__ __forceinline__ long long brick_node(const LsGrid& G, long long b, int t, bool& valid) {
+  const int bk = (int)(b % G.bnz);
+  const long long r = b / G.bnz;
+  const int bj = (int)(r % G.bny), bi = (int)(r / G.bny);
+  const int i = bi * kBX + t / (kBY * kBZ), j = bj * kBY + (t / kBZ) % kBY, k = bk * kBZ + t % kBZ;
+  valid = i < G.nx && j < G.ny && k < G.nz;
+  return ((long long)i * G.ny + j) * G.nz + k;
+}
+
+// any of the flags of brick b and of its face neighbours in the flag plane f
+__device__ __forceinline__ bool brick_or_nbr(const LsGrid& G, const uint8_t* __restrict__ f,
+                                             long long b) {
+  const int bk = (int)(b % G.bnz);
+  const long long r = b / G.bnz;
+  const int bj = (int)(r % G.bny), bi = (int)(r / G.bny);
+  const long long sx = (long long)G.bny * G.bnz, sy = G.bnz;
+  return f[b] || (bk > 0 && f[b - 1]) || (bk < G.bnz - 1 && f[b + 1]) ||
+         (bj > 0 && f[b - sy]) || (bj < G.bny - 1 && f[b + sy]) || (bi > 0 && f[b - sx]) ||
+         (bi < G.bnx - 1 && f[b + sx]);
+}
+
+__global__ void k_zero_bytes(uint8_t* __restrict__ p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// Appends (f, b | tag) to the list with one counter atomic per wavefront.
+__device__ __forceinline__ void list_append(bool want, int f, int bt, int2* __restrict__ list,
+                                            int* __restrict__ cnt) {
+  const unsigned long long m = __ballot(want);
+  if (!m) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, leader);
+  if (want) list[base + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(f, bt);
+}
+
+// Batched advection. Before the substeps, a brick is marked when one of its nodes has a nonzero
+// speed or is a mask node not at its pinned value; no other node changes in any substep
+// (phi - dt 0 |grad phi| = phi for a finite level set, and the pin is a no-op). The substeps
+// update the marked bricks, copy their unmarked face neighbours from the source buffer to the
+// destination (so every value a marked brick's stencils read is current in both ping-pong
+// buffers) and leave the rest, whose phi is already final.
+__global__ void k_advect_mark(const LsGrid* __restrict__ grids, int n_grids) {
+  const LsGrid& G = grids[grid_of_nodes(grids, n_grids)];
+  const long long n = (blockIdx.x - G.nblk0) * (long long)kBS + threadIdx.x;
+  if (n >= (long long)G.nx * G.ny * G.nz) return;
+  if (G.F[n] != 0.f || (G.mask[n] && !(G.phi[n] == G.mask_phi[n])))
+    G.bfl[brick_of_node(G, n)] = 1;
+}
+
+__global__ void k_advect_list(const LsGrid* __restrict__ grids, int n_grids, long long total,
+                              int2* __restrict__ list, int* __restrict__ cnt) {
+  const long long gb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool want = false;
+  int f = 0, bt = 0;
+  if (gb < total) {
+    f = grid_of_brick(grids, n_grids, gb);
+    const LsGrid& G = grids[f];
+    const long long b = gb - G.bblk0;
+    if (G.count > 0) {
+      if (G.bfl[b]) {
+        want = true;
+        bt = (int)b;
+      } else if (brick_or_nbr(G, G.bfl, b)) {
+        want = true;
+        bt = (int)b | kHalo;
+      }
+    }
+  }
+  list_append(want, f, bt, list, cnt);
+}
+
 // substep s of every grid with more than s substeps: phi -> work on even s, work -> phi on odd
-__global__ void k_advect_grids(const LsGrid* __restrict__ grids, int n_grids, int s) {
-  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
-  if (s >= G.count) return;
-  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
-  if (n >= (long long)G.nx * G.ny * G.nz) return;
-  const float* src = (s & 1) ? G.work : G.phi;
-  float* dst = (s & 1) ? G.phi : G.work;
-  advect_node(src, G.F, G.mask, G.mask_phi, dst, G.nx, G.ny, G.nz, G.inv_dx, G.dt, n);
+__global__ void k_advect_bricks(const LsGrid* __restrict__ grids, const int2* __restrict__ list,
+                                const int* __restrict__ cnt, int s) {
+  const int m = *cnt;
+  for (int e = blockIdx.x; e < m; e += gridDim.x) {
+    const int2 q = list[e];
+    const LsGrid& G = grids[q.x];
+    if (s >= G.count) continue;
+    bool valid;
+    const long long n = brick_node(G, q.y & (kHalo - 1), threadIdx.x, valid);
+    if (!valid) continue;
+    const float* src = (s & 1) ? G.work : G.phi;
+    float* dst = (s & 1) ? G.phi : G.work;
+    if (q.y & kHalo)
+      dst[n] = src[n];
+    else
+      advect_node(src, G.F, G.mask, G.mask_phi, dst, G.nx, G.ny, G.nz, G.inv_dx, G.dt, n);
+  }
 }
 
-// a grid with an odd substep count ends in work: copy it back to phi
-__global__ void k_advect_grids_copyback(const LsGrid* __restrict__ grids, int n_grids) {
-  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
-  if (!(G.count & 1)) return;
-  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
-  if (n >= (long long)G.nx * G.ny * G.nz) return;
-  G.phi[n] = G.work[n];
+// a grid with an odd substep count ends in work: copy its listed bricks back to phi
+__global__ void k_advect_bricks_copyback(const LsGrid* __restrict__ grids,
+                                         const int2* __restrict__ list,
+                                         const int* __restrict__ cnt) {
+  const int m = *cnt;
+  for (int e = blockIdx.x; e < m; e += gridDim.x) {
+    const int2 q = list[e];
+    const LsGrid& G = grids[q.x];
+    if (!(G.count & 1)) continue;
+    bool valid;
+    const long long n = brick_node(G, q.y & (kHalo - 1), threadIdx.x, valid);
+    if (valid) G.phi[n] = G.work[n];
+  }
 }
 
+// Batched reinit. Flag plane k (k = 0..count) of a grid marks the bricks whose values changed in
+// sweep k (plane 0: the bricks holding interface seeds). Both ping-pong buffers start at the
+// initial state. A Jacobi update of a node none of whose neighbours changed since the previous
+// sweep returns its current value, so sweep k lists only the bricks that changed in sweep k - 1
+// or touch one that did; a brick left out keeps the previous sweep's values, which its
+// destination buffer already holds (by induction: it did not change in sweep k - 1 either). The
+// result equals the full sweeps' to the bit.
 __global__ void k_eik_init_grids(const LsGrid* __restrict__ grids, int n_grids) {
-  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
-  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
+  const LsGrid& G = grids[grid_of_nodes(grids, n_grids)];
+  const long long n = (blockIdx.x - G.nblk0) * (long long)kBS + threadIdx.x;
   if (n >= (long long)G.nx * G.ny * G.nz) return;
   eik_init_node(G.phi, G.U, G.fr, G.nx, G.ny, G.nz, G.dx, G.big, n);
+  G.U2[n] = G.U[n];
+  if (G.fr[n]) G.bfl[brick_of_node(G, n)] = 1;
 }
 
-// sweep it of every grid with more than it sweeps: U -> U2 on even it, U2 -> U on odd
-__global__ void k_eik_jacobi_grids(const LsGrid* __restrict__ grids, int n_grids, int it) {
-  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
-  if (it >= G.count) return;
-  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
-  if (n >= (long long)G.nx * G.ny * G.nz) return;
-  const float* src = (it & 1) ? G.U2 : G.U;
-  float* dst = (it & 1) ? G.U : G.U2;
-  eik_jacobi_node(src, G.fr, dst, G.nx, G.ny, G.nz, G.dx, G.big, n);
+__global__ void k_eik_list(const LsGrid* __restrict__ grids, int n_grids, long long total, int it,
+                           int2* __restrict__ list, int* __restrict__ cnt) {
+  const long long gb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool want = false;
+  int f = 0, bt = 0;
+  if (gb < total) {
+    f = grid_of_brick(grids, n_grids, gb);
+    const LsGrid& G = grids[f];
+    const long long b = gb - G.bblk0;
+    bt = (int)b;
+    want = it < G.count && brick_or_nbr(G, G.bfl + (long long)it * nbricks(G), b);
+  }
+  list_append(want, f, bt, list, cnt);
+}
+
+// sweep it (k = it + 1) over the listed bricks: U -> U2 on even it, U2 -> U on odd
+__global__ void k_eik_jacobi_bricks(const LsGrid* __restrict__ grids,
+                                    const int2* __restrict__ list, const int* __restrict__ cnt,
+                                    int it) {
+  const int m = *cnt;
+  for (int e = blockIdx.x; e < m; e += gridDim.x) {
+    const int2 q = list[e];
+    const LsGrid& G = grids[q.x];
+    bool valid;
+    const long long n = brick_node(G, q.y, threadIdx.x, valid);
+    int ch = 0;
+    if (valid) {
+      const float* src = (it & 1) ? G.U2 : G.U;
+      float* dst = (it & 1) ? G.U : G.U2;
+      const float u = src[n];
+      eik_jacobi_node(src, G.fr, dst, G.nx, G.ny, G.nz, G.dx, G.big, n);
+      ch = dst[n] < u;
+    }
+    ch = __syncthreads_or(ch);
+    if (threadIdx.x == 0 && ch) G.bfl[(long long)(it + 1) * nbricks(G) + q.y] = 1;
+  }
 }
 
 __global__ void k_eik_final_grids(const LsGrid* __restrict__ grids, int n_grids) {
-  const LsGrid& G = grids[grid_of_block(grids, n_grids)];
-  const long long n = (blockIdx.x - G.blk0) * (long long)kBS + threadIdx.x;
+  const LsGrid& G = grids[grid_of_nodes(grids, n_grids)];
+  const long long n = (blockIdx.x - G.nblk0) * (long long)kBS + threadIdx.x;
   if (n >= (long long)G.nx * G.ny * G.nz) return;
   eik_final_node(G.phi, (G.count & 1) ? G.U2 : G.U, G.clampv, n);
 }
@@ -551,13 +712,21 @@ extern "C" int petch_reinit_fsm_noalloc(petch_ctx* ctx, float* phi, int nx, int 
 namespace {
 
 // Fills the per-grid blk0 prefix and returns the launch's workgroup count (0 on overflow).
-long long assign_blocks(LsGrid* g, int n) {
-  long long b = 0;
+// Sets each grid's brick counts, first workgroup of a full pass and first brick; returns the
+// total brick count, or 0 if a full pass or a brick list does not fit one launch.
+long long assign_grids(LsGrid* g, int n, long long* node_blocks) {
+  long long b = 0, nb = 0;
   for (int f = 0; f < n; ++f) {
-    g[f].blk0 = b;
-    b += (long long)petch_blocks((long long)g[f].nx * g[f].ny * g[f].nz, kBS);
+    g[f].bnx = (g[f].nx + kBX - 1) / kBX;
+    g[f].bny = (g[f].ny + kBY - 1) / kBY;
+    g[f].bnz = (g[f].nz + kBZ - 1) / kBZ;
+    g[f].nblk0 = nb;
+    nb += (long long)petch_blocks((long long)g[f].nx * g[f].ny * g[f].nz, kBS);
+    g[f].bblk0 = b;
+    b += (long long)g[f].bnx * g[f].bny * g[f].bnz;
   }
-  return b <= 0x7fffffffLL ? b : 0;
+  *node_blocks = nb;
+  return (nb <= 0x7fffffffLL && b < kHalo) ? b : 0;
 }
 
 }  // namespace
@@ -576,26 +745,56 @@ extern "C" int petch_advect_batch(petch_ctx* ctx, int n, const petch_advect_item
       free(g);
       return petch_set_error(ctx, PETCH_E_ARG, "advect_batch: item %d is invalid", f);
     }
-    g[f] = LsGrid{it.phi, it.work, it.F, it.mask, it.mask_phi, nullptr, nullptr, nullptr,
-                  it.nx, it.ny, it.nz, it.nsub, it.dx, (float)(1.0 / it.dx), (float)it.dt,
-                  0.f, 0.f, 0};
+    LsGrid q = {};
+    q.phi = it.phi;
+    q.work = it.work;
+    q.F = it.F;
+    q.mask = it.mask;
+    q.mask_phi = it.mask_phi;
+    q.nx = it.nx;
+    q.ny = it.ny;
+    q.nz = it.nz;
+    q.count = it.nsub;
+    q.dx = it.dx;
+    q.inv_dx = (float)(1.0 / it.dx);
+    q.dt = (float)it.dt;
+    g[f] = q;
     if (it.nsub > max_sub) max_sub = it.nsub;
   }
-  const long long blocks = assign_blocks(g, n);
-  if (!blocks) {
+  long long nodeblocks = 0;
+  const long long bricks = assign_grids(g, n, &nodeblocks);
+  if (!bricks) {
     free(g);
     return petch_set_error(ctx, PETCH_E_ARG, "advect_batch: too many nodes for one launch");
   }
+  // one flag per brick, then the list counter (int, 4-byte aligned)
+  const long long fbytes = (bricks + 3) & ~3LL;
+  uint8_t* bfl = (uint8_t*)petch_scratch(ctx, SCR_LS_BRICK, (size_t)fbytes + sizeof(int));
+  int2* list = (int2*)petch_scratch(ctx, SCR_LS_LIST, (size_t)bricks * sizeof(int2));
+  if (!bfl || !list) {
+    free(g);
+    return PETCH_E_HIP;
+  }
+  int* cnt = (int*)(bfl + fbytes);
+  for (int f = 0; f < n; ++f) g[f].bfl = bfl + g[f].bblk0;
   const LsGrid* dg = (const LsGrid*)petch_upload_table(ctx, g, (size_t)n * sizeof(LsGrid));
   free(g);
   if (!dg) return PETCH_E_HIP;
+  const unsigned zb = (unsigned)std::min<long long>(petch_blocks(fbytes + 4, kBS), 1024);
+  hipLaunchKernelGGL(k_zero_bytes, dim3(zb), dim3(kBS), 0, ctx->stream, bfl, fbytes + 4);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_advect_mark, dim3((unsigned)nodeblocks), dim3(kBS), 0, ctx->stream, dg, n);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_advect_list, dim3(petch_blocks(bricks, kBS)), dim3(kBS), 0, ctx->stream,
+                     dg, n, bricks, list, cnt);
+  PETCH_LAUNCH_CHECK(ctx);
+  const unsigned lw = (unsigned)std::min<long long>(bricks, kListWG);
   for (int sub = 0; sub < max_sub; ++sub) {
-    hipLaunchKernelGGL(k_advect_grids, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, dg, n,
-                       sub);
+    hipLaunchKernelGGL(k_advect_bricks, dim3(lw), dim3(kBS), 0, ctx->stream, dg, list, cnt, sub);
     PETCH_LAUNCH_CHECK(ctx);
   }
-  hipLaunchKernelGGL(k_advect_grids_copyback, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream,
-                     dg, n);
+  hipLaunchKernelGGL(k_advect_bricks_copyback, dim3(lw), dim3(kBS), 0, ctx->stream, dg, list,
+                     cnt);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
@@ -622,28 +821,67 @@ extern "C" int petch_reinit_fsm_batch(petch_ctx* ctx, int n, const petch_reinit_
   for (int f = 0; f < n; ++f) {
     const petch_reinit_item& it = items[f];
     const int n_iter = it.n_iter < 0 ? (int)ceil(it.band / it.dx) + 4 : it.n_iter;
-    g[f] = LsGrid{it.phi, nullptr, nullptr, nullptr, nullptr, U + off, U2 + off, fr + off,
-                  it.nx, it.ny, it.nz, n_iter, it.dx, 0.f, 0.f, (float)(it.band + 4.0 * it.dx),
-                  (float)(it.band + it.dx), 0};
+    LsGrid q = {};
+    q.phi = it.phi;
+    q.U = U + off;
+    q.U2 = U2 + off;
+    q.fr = fr + off;
+    q.nx = it.nx;
+    q.ny = it.ny;
+    q.nz = it.nz;
+    q.count = n_iter;
+    q.dx = it.dx;
+    q.big = (float)(it.band + 4.0 * it.dx);
+    q.clampv = (float)(it.band + it.dx);
+    g[f] = q;
     off += (long long)it.nx * it.ny * it.nz;
     if (n_iter > max_it) max_it = n_iter;
   }
-  const long long blocks = assign_blocks(g, n);
-  if (!blocks) {
+  long long nodeblocks = 0;
+  const long long bricks = assign_grids(g, n, &nodeblocks);
+  if (!bricks) {
     free(g);
     return petch_set_error(ctx, PETCH_E_ARG, "reinit_fsm_batch: too many nodes for one launch");
+  }
+  // flag planes 0..count of every grid (grid f's after those of the grids before it), then one
+  // list counter per sweep
+  long long nflags = 0;
+  for (int f = 0; f < n; ++f)
+    nflags += (long long)(g[f].count + 1) * g[f].bnx * g[f].bny * g[f].bnz;
+  const long long fbytes = (nflags + 3) & ~3LL;
+  const long long zbytes = fbytes + (long long)(max_it + 1) * (long long)sizeof(int);
+  uint8_t* bfl = (uint8_t*)petch_scratch(ctx, SCR_LS_BRICK, (size_t)zbytes);
+  int2* list = (int2*)petch_scratch(ctx, SCR_LS_LIST, (size_t)bricks * sizeof(int2));
+  if (!bfl || !list) {
+    free(g);
+    return PETCH_E_HIP;
+  }
+  int* cnt = (int*)(bfl + fbytes);
+  long long foff = 0;
+  for (int f = 0; f < n; ++f) {
+    g[f].bfl = bfl + foff;
+    foff += (long long)(g[f].count + 1) * g[f].bnx * g[f].bny * g[f].bnz;
   }
   const LsGrid* dg = (const LsGrid*)petch_upload_table(ctx, g, (size_t)n * sizeof(LsGrid));
   free(g);
   if (!dg) return PETCH_E_HIP;
-  hipLaunchKernelGGL(k_eik_init_grids, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, dg, n);
+  const unsigned zb = (unsigned)std::min<long long>(petch_blocks(zbytes, kBS), 1024);
+  hipLaunchKernelGGL(k_zero_bytes, dim3(zb), dim3(kBS), 0, ctx->stream, bfl, zbytes);
   PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_eik_init_grids, dim3((unsigned)nodeblocks), dim3(kBS), 0, ctx->stream, dg,
+                     n);
+  PETCH_LAUNCH_CHECK(ctx);
+  const unsigned lw = (unsigned)std::min<long long>(bricks, kListWG);
   for (int it = 0; it < max_it; ++it) {
-    hipLaunchKernelGGL(k_eik_jacobi_grids, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, dg,
-                       n, it);
+    hipLaunchKernelGGL(k_eik_list, dim3(petch_blocks(bricks, kBS)), dim3(kBS), 0, ctx->stream, dg,
+                       n, bricks, it, list, cnt + it);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_eik_jacobi_bricks, dim3(lw), dim3(kBS), 0, ctx->stream, dg, list,
+                       cnt + it, it);
     PETCH_LAUNCH_CHECK(ctx);
   }
-  hipLaunchKernelGGL(k_eik_final_grids, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, dg, n);
+  hipLaunchKernelGGL(k_eik_final_grids, dim3((unsigned)nodeblocks), dim3(kBS), 0, ctx->stream, dg,
+                     n);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

@@ -212,37 +212,40 @@ def test_profile_area_and_its_adjoint_match_the_oracle():
 
 def test_batched_advection_and_reinit_equal_single_calls():
     """petch_advect_batch and petch_reinit_fsm_batch (one launch per substep / sweep for many
-    grids) give exactly what petch_advect and petch_reinit_fsm give grid by grid: grids of
-    different shapes, substep counts 0-3 (odd counts end in the work buffer and are copied
-    back) and sweep counts."""
+    grids, bricks that provably do not change skipped) give exactly what petch_advect and
+    petch_reinit_fsm give grid by grid: grids of different shapes (one large enough that most
+    bricks lie far from the interface), substep counts 0-3 (odd counts end in the work buffer and
+    are copied back), sweep counts, mask nodes off and at their pinned values, and a work buffer
+    full of NaN (a skipped brick's stale work values must never be read)."""
     import torch
     from petch import ops
     dev = _dev()
     rng = np.random.default_rng(9)
-    shapes = [(30, 22, 41), (17, 33, 25), (40, 8, 12)]
+    shapes = [(30, 22, 41), (17, 33, 25), (40, 8, 12), (72, 56, 100)]
     grids = []
     for k, (nx, ny, nz) in enumerate(shapes):
-        dx = 0.1 + 0.05 * k
+        dx = 0.1 + 0.05 * (k % 3)
         phi = ol.init_sdf(nx * dx, ny * dx, nz * dx, dx, nx * dx / 3, 3 * dx, nz * dx / 2,
                           k % 2 == 0).astype(np.float32)
         phi += (0.05 * dx * rng.normal(size=phi.shape)).astype(np.float32)
         F = np.where(np.abs(phi) < 4 * dx, rng.uniform(0.5, 1.5, phi.shape), 0.0).astype(np.float32)
         mask = (rng.random(phi.shape) < 0.05).astype(np.uint8)
+        mp = phi + np.where(mask & (rng.random(phi.shape) < 0.3), 0.2 * dx, 0.0).astype(np.float32)
         grids.append((dx, torch.from_numpy(phi).to(dev), torch.from_numpy(F).to(dev),
-                      torch.from_numpy(mask).to(dev)))
-    for nsubs in ((1, 2, 3), (0, 3, 2)):
+                      torch.from_numpy(mask).to(dev), torch.from_numpy(mp).to(dev)))
+    for nsubs in ((1, 2, 3, 3), (0, 3, 2, 2)):
         ref, items = [], []
-        for (dx, phi, F, mask), nsub in zip(grids, nsubs):
+        for (dx, phi, F, mask, mp), nsub in zip(grids, nsubs):
             a = phi.clone()
-            ops.advect(a, F, mask, phi, dx, 0.3 * dx, nsub)
+            ops.advect(a, F, mask, mp, dx, 0.3 * dx, nsub)
             ref.append(a)
             b = phi.clone()
-            items.append((b, torch.empty_like(b), F, mask, phi, dx, 0.3 * dx, nsub))
+            items.append((b, torch.full_like(b, float("nan")), F, mask, mp, dx, 0.3 * dx, nsub))
         ops.advect_batch(items)
         for r, it in zip(ref, items):
             assert torch.equal(r, it[0])
     ref, items = [], []
-    for (dx, phi, F, mask), n_iter in zip(grids, (None, 7, 10)):
+    for (dx, phi, F, mask, mp), n_iter in zip(grids, (None, 7, 10, None)):
         a = phi.clone()
         ops.reinit_fsm(a, dx, 6 * dx, n_iter)
         ref.append(a)
