@@ -473,7 +473,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    # two warm-up steps: the first runs from full coverage, the second is the first with the
+    # coverage warm start, whose first torch ops load their kernels (stalls of ~0.3 s)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small-hole", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
