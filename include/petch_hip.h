@@ -399,6 +399,23 @@ int petch_belen_coverage(petch_ctx* ctx, const float* m_i, const float* m_F, con
                          const float* cos_i, const float* iy_ie, const float* iy_ip,
                          const petch_chem_params* cp, int F, float* bare, float* thF,
                          float* thO);
+
+/* The end of one neutral fixed-point round of many features in one launch (no reference
+ * counterpart: the reference finishes one feature at a time, threed.py:881-904): per feature,
+ * m_F and m_O = the F and O accumulators flF and flO normalised by petch_normalize's clip
+ * (lo 0, hi) and, with smooth = 1, smoothed by one petch_smooth pass with blend alpha; then
+ * bare = petch_belen_coverage of (m_i, m_F, m_O, cos_i, iy_ie, iy_ip). The same arithmetic as
+ * those three calls, so the outputs equal theirs. */
+typedef struct petch_round_item {
+  const float *flF, *flO, *area, *normals;
+  const int* nbr;
+  const float *m_i, *cos_i, *iy_ie, *iy_ip;
+  float *m_F, *m_O, *bare;
+  petch_chem_params chem;
+  int nf, smooth;
+  float scale, hi, alpha;
+} petch_round_item;
+int petch_finish_round_batch(petch_ctx* ctx, int n, const petch_round_item* items_host);
 /* Closed-form adjoint of petch_belen_coverage for gbare = d loss / d bare (zero where the clip
  * of 1 - thF - thO is active): gm_F, gm_O = d loss / d m_F, d m_O; with iy_ie/iy_ip also
  * g_ie, g_ip = d loss / d iy_ie, d iy_ip, else gm_i = d loss / d m_i (cos_i held constant).

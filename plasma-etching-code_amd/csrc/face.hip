@@ -26,12 +26,17 @@ namespace {
 
 constexpr int kBS = 256;
 
+__device__ __forceinline__ float norm_value(float fl, float area, float scale, float lo,
+                                            float hi) {
+  const float A = fmaxf(area, 1.0e-9f);
+  return fminf(fmaxf((fl / A) * scale, lo), hi);
+}
+
 __global__ void k_normalize(const float* __restrict__ fl, const float* __restrict__ area, int F,
                             float scale, float lo, float hi, float* __restrict__ out) {
   int f = blockIdx.x * kBS + threadIdx.x;
   if (f >= F) return;
-  float A = fmaxf(area[f], 1.0e-9f);
-  out[f] = fminf(fmaxf((fl[f] / A) * scale, lo), hi);
+  out[f] = norm_value(fl[f], area[f], scale, lo, hi);
 }
 
 __global__ void k_cos_ratio(const float* __restrict__ fl, const float* __restrict__ ang, int F,
@@ -44,23 +49,33 @@ __global__ void k_cos_ratio(const float* __restrict__ fl, const float* __restric
 
 // One smoothing pass: out_i = (x_i + sum_j w_ij x_j) / (1 + sum_j w_ij), w_ij = max(0, n_i.n_j),
 // j over the (up to 3) edge neighbours; blended out = (1-alpha)*raw + alpha*out when alpha < 1.
-__global__ void k_smooth(const float* __restrict__ x, const int* __restrict__ nbr,
-                         const float* __restrict__ nrm, int F, float alpha,
-                         const float* __restrict__ raw, float* __restrict__ out) {
-  int f = blockIdx.x * kBS + threadIdx.x;
-  if (f >= F) return;
+// The smoothing pass at face f over the values val(g) of f and its neighbours (a functor, so a
+// fused caller can compute them on the fly).
+template <class V>
+__device__ __forceinline__ float smooth_value(const V& val, const int* __restrict__ nbr,
+                                              const float* __restrict__ nrm, int F, float alpha,
+                                              float raw, int f) {
   float nx = nrm[3 * f], ny = nrm[3 * f + 1], nz = nrm[3 * f + 2];
-  float num = x[f], den = 1.f;
+  float num = val(f), den = 1.f;
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
     int g = nbr[3 * f + e];
     if (g < 0 || g >= F) continue;
     float w = fmaxf(0.f, nx * nrm[3 * g] + ny * nrm[3 * g + 1] + nz * nrm[3 * g + 2]);
-    num += x[g] * w;
+    num += val(g) * w;
     den += w;
   }
   float s = num / fmaxf(den, 1.0e-12f);
-  out[f] = (alpha < 1.f) ? (1.f - alpha) * raw[f] + alpha * s : s;
+  return (alpha < 1.f) ? (1.f - alpha) * raw + alpha * s : s;
+}
+
+__global__ void k_smooth(const float* __restrict__ x, const int* __restrict__ nbr,
+                         const float* __restrict__ nrm, int F, float alpha,
+                         const float* __restrict__ raw, float* __restrict__ out) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  auto val = [x](int g) { return x[g]; };
+  out[f] = smooth_value(val, nbr, nrm, F, alpha, raw[f], f);
 }
 
 // Adjoint of k_normalize: gfl = gout * scale / A where the clamp is inactive (lo <= v <= hi).
@@ -133,13 +148,12 @@ __device__ __forceinline__ void angular(double cos_i, const Chem& c, double& f_i
   }
 }
 
-__global__ void k_belen_coverage(const float* __restrict__ m_i, const float* __restrict__ m_F,
-                                 const float* __restrict__ m_O, const float* __restrict__ cos_i,
-                                 const float* __restrict__ iy_ie, const float* __restrict__ iy_ip,
-                                 Chem c, int F, float* __restrict__ bare,
-                                 float* __restrict__ thF_out, float* __restrict__ thO_out) {
-  int f = blockIdx.x * kBS + threadIdx.x;
-  if (f >= F) return;
+// Belen coverages of face f for the normalised fluxes mF, mO: bare = clip(1 - thF - thO, 0, 1).
+__device__ __forceinline__ float belen_value(const float* __restrict__ m_i, float mF, float mO,
+                                             const float* __restrict__ cos_i,
+                                             const float* __restrict__ iy_ie,
+                                             const float* __restrict__ iy_ip, const Chem& c, int f,
+                                             double& thF, double& thO) {
   const double eps = 1e-9;
   double GY_ie, GY_p;
   if (iy_ie) {
@@ -152,16 +166,69 @@ __global__ void k_belen_coverage(const float* __restrict__ m_i, const float* __r
     GY_ie = c.Yie * f_ie * Fi;
     GY_p = c.Yp * f_ie * Fi;
   }
-  double GbE = c.Fflux * (double)m_F[f] + eps;
-  double GbP = c.Oflux * (double)m_O[f] + eps;
+  double GbE = c.Fflux * (double)mF + eps;
+  double GbP = c.Oflux * (double)mO + eps;
   double a = (c.k_sigma + 2.0 * GY_ie) / GbE;
   double b = (c.beta_sigma + GY_p) / GbP;
-  double thF = 1.0 / (1.0 + a * (1.0 + 1.0 / (b + eps)));
-  double thO = 1.0 / (1.0 + b * (1.0 + 1.0 / (a + eps)));
+  thF = 1.0 / (1.0 + a * (1.0 + 1.0 / (b + eps)));
+  thO = 1.0 / (1.0 + b * (1.0 + 1.0 / (a + eps)));
   double br = 1.0 - thF - thO;
-  bare[f] = (float)fmin(fmax(br, 0.0), 1.0);
+  return (float)fmin(fmax(br, 0.0), 1.0);
+}
+
+__global__ void k_belen_coverage(const float* __restrict__ m_i, const float* __restrict__ m_F,
+                                 const float* __restrict__ m_O, const float* __restrict__ cos_i,
+                                 const float* __restrict__ iy_ie, const float* __restrict__ iy_ip,
+                                 Chem c, int F, float* __restrict__ bare,
+                                 float* __restrict__ thF_out, float* __restrict__ thO_out) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f >= F) return;
+  double thF, thO;
+  bare[f] = belen_value(m_i, m_F[f], m_O[f], cos_i, iy_ie, iy_ip, c, f, thF, thO);
   if (thF_out) thF_out[f] = (float)thF;
   if (thO_out) thO_out[f] = (float)thO;
+}
+
+// The end of one neutral fixed-point round of many features (petch.batch): per face, both
+// species' accumulators normalised and smoothed once (normalised neighbour values computed on
+// the fly), then the Belen update; the arithmetic of k_normalize, k_smooth and
+// k_belen_coverage, shared through the same device functions. Feature t owns the workgroups
+// [blk0[t], blk0[t+1]).
+struct RoundItem {
+  const float *flF, *flO, *area, *nrm;
+  const int* nbr;
+  const float *m_i, *cos_i, *iy_ie, *iy_ip;
+  float *m_F, *m_O, *bare;
+  Chem c;
+  int nf, smooth;
+  float scale, hi, alpha;
+  long long blk0;
+};
+
+__global__ void k_finish_round(const RoundItem* __restrict__ items, int n) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].blk0 <= (long long)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const RoundItem& it = items[lo];
+  const int f = (int)(blockIdx.x - it.blk0) * kBS + threadIdx.x;
+  if (f >= it.nf) return;
+  const float* area = it.area;
+  const float scale = it.scale, hv = it.hi;
+  const float* flF = it.flF;
+  const float* flO = it.flO;
+  auto vF = [=](int g) { return norm_value(flF[g], area[g], scale, 0.f, hv); };
+  auto vO = [=](int g) { return norm_value(flO[g], area[g], scale, 0.f, hv); };
+  float mF = vF(f), mO = vO(f);
+  if (it.smooth) {
+    mF = smooth_value(vF, it.nbr, it.nrm, it.nf, it.alpha, mF, f);
+    mO = smooth_value(vO, it.nbr, it.nrm, it.nf, it.alpha, mO, f);
+  }
+  it.m_F[f] = mF;
+  it.m_O[f] = mO;
+  double thF, thO;
+  it.bare[f] = belen_value(it.m_i, mF, mO, it.cos_i, it.iy_ie, it.iy_ip, it.c, f, thF, thO);
 }
 
 // Closed-form adjoint of k_belen_coverage for gbare = d loss / d bare (the clip passes the
@@ -615,6 +682,45 @@ extern "C" int petch_belen_coverage(petch_ctx* ctx, const float* m_i, const floa
   if (F <= 0) return PETCH_OK;
   hipLaunchKernelGGL(k_belen_coverage, dim3(petch_blocks(F, kBS)), dim3(kBS), 0, ctx->stream, m_i,
                      m_F, m_O, cos_i, iy_ie, iy_ip, make_chem(cp), F, bare, thF, thO);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_finish_round_batch(petch_ctx* ctx, int n, const petch_round_item* items) {
+  PETCH_ARG(ctx, ctx && (n == 0 || items), "finish_round_batch: null pointer");
+  PETCH_ARG(ctx, n >= 0, "finish_round_batch: n < 0");
+  if (n == 0) return PETCH_OK;
+  std::vector<RoundItem> t;
+  t.reserve((size_t)n);
+  long long blocks = 0;
+  for (int k = 0; k < n; ++k) {
+    const petch_round_item& q = items[k];
+    PETCH_ARG(ctx, q.nf >= 0 && q.flF && q.flO && q.area && q.m_F && q.m_O && q.bare,
+              "finish_round_batch: item %d is invalid", k);
+    PETCH_ARG(ctx, !q.smooth || (q.nbr && q.normals), "finish_round_batch: item %d needs the "
+              "adjacency and normals to smooth", k);
+    PETCH_ARG(ctx, (q.iy_ie && q.iy_ip) || (q.m_i && q.cos_i),
+              "finish_round_batch: item %d needs ion yields or m_i/cos_i", k);
+    if (q.nf == 0) continue;
+    RoundItem r;
+    r.flF = q.flF; r.flO = q.flO; r.area = q.area; r.nrm = q.normals; r.nbr = q.nbr;
+    r.m_i = q.m_i; r.cos_i = q.cos_i; r.iy_ie = q.iy_ie; r.iy_ip = q.iy_ip;
+    r.m_F = q.m_F; r.m_O = q.m_O; r.bare = q.bare;
+    r.c = make_chem(&q.chem);
+    r.nf = q.nf;
+    r.smooth = q.smooth && q.alpha > 0.f;
+    r.scale = q.scale; r.hi = q.hi; r.alpha = q.alpha;
+    r.blk0 = blocks;
+    blocks += petch_blocks(q.nf, kBS);
+    t.push_back(r);
+  }
+  if (t.empty()) return PETCH_OK;
+  PETCH_ARG(ctx, blocks <= 0x7fffffffLL, "finish_round_batch: too many faces for one launch");
+  const RoundItem* d =
+      (const RoundItem*)petch_upload_table(ctx, t.data(), t.size() * sizeof(RoundItem));
+  if (!d) return PETCH_E_HIP;
+  hipLaunchKernelGGL(k_finish_round, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d,
+                     (int)t.size());
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

@@ -100,6 +100,15 @@ class ReinitItem(ctypes.Structure):
                 ("band", c_double), ("n_iter", c_int)]
 
 
+class RoundItem(ctypes.Structure):
+    """petch_round_item: one feature of a batched neutral-round finish."""
+    _fields_ = [("flF", c_void_p), ("flO", c_void_p), ("area", c_void_p), ("normals", c_void_p),
+                ("nbr", c_void_p), ("m_i", c_void_p), ("cos_i", c_void_p), ("iy_ie", c_void_p),
+                ("iy_ip", c_void_p), ("m_F", c_void_p), ("m_O", c_void_p), ("bare", c_void_p),
+                ("chem", ChemParams), ("nf", c_int), ("smooth", c_int), ("scale", c_float),
+                ("hi", c_float), ("alpha", c_float)]
+
+
 class McItem(ctypes.Structure):
     """petch_mc_item: one grid of a batched marching cubes."""
     _fields_ = [("phi", c_void_p), ("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_double),
@@ -177,6 +186,7 @@ SIGNATURES = {
     "petch_normalize_adjoint": [P, P, P, c_int, c_float, c_float, c_float, P, P],
     "petch_smooth_adjoint": [P, P, P, c_int, c_int, c_float, c_int, P, P],
     "petch_belen_coverage": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P],
+    "petch_finish_round_batch": [P, c_int, P],
     "petch_belen_coverage_adjoint": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P,
                                      P, P, P],
     "petch_surface_rate": [P, P, P, P, P, P, P, P, P, P, POINTER(MaskGeom),

@@ -56,8 +56,9 @@ class BatchLoop:
         warm = dict(zip(ws_idx, ops.nearest_face_value_batch(ws_items, 1.0)))
         outs = [None] * len(loops)
         pending = {}
+        rounds = []     # the neutral rounds' ends of all features, run as one launch
         for i, lp in enumerate(loops):
-            job = lp.begin_flux(step, finite[i], bare_init=warm.get(i))
+            job = lp.begin_flux(step, finite[i], bare_init=warm.get(i), rounds=rounds)
             if isinstance(job, _FluxSteps):
                 try:
                     pending[i] = (job.steps, next(job.steps))
@@ -86,6 +87,9 @@ class BatchLoop:
                         except StopIteration as stop:
                             outs[i] = stop.value
                             del pending[i]
+                    if rounds:
+                        ops.finish_round_batch(rounds)
+                        rounds.clear()
         # 3. every feature's rate; the rate reductions and BVH guards of all features in one read
         has = [lp.end_rate(step, out) for lp, out in zip(loops, outs)]
         reads = [(0.0, 0, 0)] * len(loops)

@@ -106,14 +106,20 @@ def _check_unsupported(par, flags):
 NEUTRAL_SOURCE = ops.make_source(ops.SRC_NEUTRAL)
 
 
-def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None):
+def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, rounds=None):
     """mc_flux_3d_coupled (threed.py:740-905) as a generator over its ray traces.
 
     It yields ("ion", request) and ("neutral", request) dicts and is sent the trace outputs
     back, so one driver can run it alone (run_traces) and another can advance many features in
     lockstep with one batched launch per trace (petch.batch). Its return value (StopIteration)
     is dict(m_i, m_F, m_O, cos_i, bare, ion_yield) of (F,) float32 device tensors; ion_yield =
-    (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None."""
+    (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None.
+
+    rounds: a list that, when given, receives the end of each neutral round (normalise and
+    smooth both species, Belen update) as an ops.finish_round_batch item with its output tensors
+    allocated, instead of the generator launching it; the driver must run the collected items
+    (ops.finish_round_batch) before the next trace it launches and before it reads the result.
+    Used when smoothing takes at most one pass (the default flux_smooth = 1)."""
     _check_unsupported(par, flags)
     F = mesh.nf
     Lx, Ly = geo['Lx'], geo['Ly']
@@ -168,6 +174,13 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
         sF, sO = (seed * 9 + 2 + 2 * it) & SEED_MASK, (seed * 9 + 3 + 2 * it) & SEED_MASK
         fF, fO = yield ("neutral", dict(bvh=mesh.bvh, box=box, bare=bare, beta_a=betaE,
                                         n_a=n_neu, seed_a=sF, beta_b=betaO, n_b=n_neu, seed_b=sO))
+        if rounds is not None and n_smooth <= 1:
+            m_F, m_O, bare = (torch.empty_like(fF) for _ in range(3))
+            rounds.append(dict(flF=fF, flO=fO, areas=A, nbr=mesh.nbr, normals=mesh.normals,
+                               scale=A_src / n_neu, hi=8.0, n_smooth=max(n_smooth, 0),
+                               alpha=alpha, m_i=m_i, cos_i=cos_i, ion_yield=ion_yield, chem=chem,
+                               m_F=m_F, m_O=m_O, bare=bare))
+            continue
         m_F, m_O = finish(fF, A_src / n_neu, 8.0), finish(fO, A_src / n_neu, 8.0)
         bare = ops.belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem)
     return dict(m_i=m_i, m_F=m_F, m_O=m_O, cos_i=cos_i, bare=bare, ion_yield=ion_yield)

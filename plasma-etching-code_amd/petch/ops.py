@@ -99,6 +99,36 @@ def reinit_fsm_adjoint(phi0, dx, band, gout, n_iter=None):
     return gphi
 
 
+def finish_round_batch(items):
+    """The end of one neutral fixed-point round of many features in one launch
+    (petch_finish_round_batch). items: dicts with flF, flO (raw accumulators), areas, nbr,
+    normals, scale, hi, n_smooth (0 or 1), alpha, m_i, cos_i, ion_yield (sp, ie, ip) or None,
+    chem, and the outputs m_F, m_O, bare (float32 (F,) tensors the call fills). Equals, per item,
+    m = smooth(normalize(fl, areas, scale, 0, hi), nbr, normals, n_smooth, alpha) for both
+    species and bare = belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem)."""
+    from ._abi import RoundItem
+    if not items:
+        return
+    arr = (RoundItem * len(items))()
+    for k, it in enumerate(items):
+        F = int(it['flF'].shape[0])
+        if int(it['n_smooth']) not in (0, 1):
+            raise ValueError(f"finish_round_batch: item {k} n_smooth {it['n_smooth']} is not 0 or 1")
+        for name in ("flO", "areas", "m_F", "m_O", "bare"):
+            if int(it[name].shape[0]) < F:
+                raise ValueError(f"finish_round_batch: item {k} {name} holds fewer than {F} values")
+        iy = it.get('ion_yield')
+        arr[k] = RoundItem(ptr(it['flF'], f32, "flF"), ptr(it['flO'], f32, "flO"),
+                           ptr(it['areas'], f32, "areas"), ptr(it['normals'], f32, "normals"),
+                           ptr(it['nbr'], i32, "nbr"), ptr(it['m_i']), ptr(it['cos_i']),
+                           ptr(iy[1] if iy is not None else None),
+                           ptr(iy[2] if iy is not None else None), ptr(it['m_F'], f32, "m_F"),
+                           ptr(it['m_O'], f32, "m_O"), ptr(it['bare'], f32, "bare"), it['chem'],
+                           F, int(it['n_smooth']), float(it['scale']), float(it['hi']),
+                           float(it['alpha']))
+    _ctx(items[0]['flF']).call("petch_finish_round_batch", len(items), arr)
+
+
 def advect_batch(items):
     """advect for many grids in one launch per substep (petch_advect_batch). items: (phi, work,
     F, mask, mask_phi, dx, dt, nsub) tuples, dt the substep's step; each phi is updated in

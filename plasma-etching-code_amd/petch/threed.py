@@ -411,10 +411,12 @@ class EtchLoop:
             return None
         return (self.prev_mesh.bvh, self.prev_bare, self._mesh.centroids)
 
-    def begin_flux(self, step, finite, bare_init=None):
+    def begin_flux(self, step, finite, bare_init=None, rounds=None):
         """The flux of the step begun by begin_mesh; `finite` is that flag read on the host.
         bare_init: the warm-start values when the caller already looked them up (the batched
-        loop does all features' lookups in one launch)."""
+        loop does all features' lookups in one launch). rounds: passed to
+        flux.coupled_flux_steps (the batched loop ends all features' neutral rounds in one
+        launch)."""
         T, par, flags, dev = self.timer, self.par, self.flags, self.dev
         mesh = self._mesh
         n_ion_s, n_neu_s = self._rays
@@ -441,7 +443,7 @@ class EtchLoop:
                                                         mesh.centroids, 1.0)
                     return _FluxSteps(_flux.coupled_flux_steps(mesh, self.geo, par, n_ion_s,
                                                                n_neu_s, seed, flags,
-                                                               bare_init=bi))
+                                                               bare_init=bi, rounds=rounds))
                 return _flux.uncoupled_flux(mesh, self.geo, self.mc_par, n_ion_s, n_neu_s, seed,
                                             getattr(flags, "ion_reflection", False))
         finally:

@@ -25,8 +25,9 @@ class FakeLoop:
     def warm_start_item(self, finite):
         return ("ws", self.k) if self.k == 1 else None
 
-    def begin_flux(self, step, finite, bare_init=None):
+    def begin_flux(self, step, finite, bare_init=None, rounds=None):
         self.bare_init = bare_init
+        self.rounds = rounds
         return self.begin_step(step)
 
     def end_rate(self, step, out):
@@ -51,6 +52,7 @@ class FakeLoop:
             for it in range(self.n_fp):
                 r = yield ("neutral", dict(k=self.k, it=it))
                 assert r == ("neutral", self.k, it)
+                self.rounds.append(("round", self.k, it))      # the round's end, deferred
             return dict(k=self.k, n=self.n_fp)
         if self.k == 99:
             return None                      # e.g. an empty surface: no flux job
@@ -92,6 +94,9 @@ def test_lockstep_batches_each_kind(monkeypatch):
     def re_batch(items):
         calls.append(("re", list(items)))
 
+    def round_batch(items):
+        calls.append(("round", list(items)))
+
     monkeypatch.setattr(B.ops, "trace_ion_yield_batch", ion_batch)
     monkeypatch.setattr(B.ops, "trace_neutral_rr_batch", neutral_batch)
     monkeypatch.setattr(B.ops, "marching_cubes_batch", mc_batch)
@@ -100,17 +105,22 @@ def test_lockstep_batches_each_kind(monkeypatch):
     monkeypatch.setattr(B.ops, "bvh_build_batch", bvh_batch)
     monkeypatch.setattr(B.ops, "advect_batch", adv_batch)
     monkeypatch.setattr(B.ops, "reinit_fsm_batch", re_batch)
+    monkeypatch.setattr(B.ops, "finish_round_batch", round_batch)
     monkeypatch.setattr(B.ops, "decode_rate_reduction", lambda r: (1.0, 0))
     loops = [FakeLoop(0, 1), FakeLoop(1, 4), FakeLoop(99, 0), FakeLoop(2, 1)]
     outs = B.BatchLoop(loops).step(0)
     # one marching-cubes pass and one BVH build call for all features, one warm-start launch for
-    # the features that have a lookup, one launch per trace kind and fixed-point round, one
-    # extension pass for every feature with a surface rate, then the advection and the reinit of
-    # every feature, one call each
+    # the features that have a lookup, one launch per trace kind and fixed-point round, each
+    # neutral round ended for all its features in one call before the next trace, one extension
+    # pass for every feature with a surface rate, then the advection and the reinit of every
+    # feature, one call each
     every = lambda tag: [(tag, k) for k in (0, 1, 99, 2)]
     assert calls == [("mc", every("mc")), ("bvh", every("bvh")),
                      ("ws", [("ws", 1)]), ("ion", [0, 1, 2]), ("neutral", [0, 1, 2]),
-                     ("neutral", [1]), ("neutral", [1]), ("neutral", [1]),
+                     ("round", [("round", 0, 0), ("round", 1, 0), ("round", 2, 0)]),
+                     ("neutral", [1]), ("round", [("round", 1, 1)]),
+                     ("neutral", [1]), ("round", [("round", 1, 2)]),
+                     ("neutral", [1]), ("round", [("round", 1, 3)]),
                      ("ext", [("ext", 0), ("ext", 1), ("ext", 2)]),
                      ("adv", every("adv")), ("re", every("re"))]
     assert not any(lp.bvh_here for lp in loops)

@@ -487,3 +487,47 @@ def test_rate_gradient_in_ion_energy(chem):
     fd = (loss(par['Emean'] + h) - loss(par['Emean'] - h)) / (2 * h)
     print(f"{chem}: d loss / dE fd {fd:.6e} adjoint {float(E.grad):.6e}", flush=True)
     assert abs(float(E.grad) - fd) <= 1e-4 * abs(fd)
+
+
+def test_batched_round_finish_equals_the_separate_calls():
+    """petch_finish_round_batch (one launch for the end of a neutral round of many features:
+    normalise and smooth both species, Belen update) gives exactly what petch_normalize,
+    petch_smooth and petch_belen_coverage give feature by feature: three surfaces of different
+    sizes, with and without ion yields, one smoothing pass blended or not, and none."""
+    import torch
+    from petch import ops
+    from petch.chemistry import chem_params
+    meshes = []
+    for k, (w, d) in enumerate(((3.0, 8.0), (2.0, 6.0), (4.5, 9.0))):
+        phi = ol.init_sdf(8.0, 8.0, 14.0, 0.25, w, d, 4.0, k != 1).astype(np.float32)
+        v, f = on.marching_cubes(phi, 0.25)
+        tv, tf = _t(v), _t(f)
+        cen, area, nrm = ops.mesh_attrs(tv, tf)
+        meshes.append((len(f), area, nrm, ops.face_adjacency(tf, len(v))))
+    rng = np.random.default_rng(11)
+    cases = [(True, 1, 1.0), (False, 1, 0.6), (True, 0, 1.0)]
+    items, refs = [], []
+    for k, ((F, area, nrm, nbr), (faithful, n_smooth, alpha)) in enumerate(zip(meshes, cases)):
+        m_i, _, _, cos_i, iy = _fluxes(F, 20 + k)
+        flF = _t(rng.exponential(2.0, F).astype(np.float32))
+        flO = _t(rng.exponential(1.0, F).astype(np.float32))
+        par = dict(PAR)
+        par['Fflux'] *= 1.0 + 0.2 * k
+        chem = chem_params(par, Flags(chemistry="belen"))
+        ion_yield = tuple(_t(a) for a in iy) if faithful else None
+        scale = 0.05 + 0.01 * k
+        mF = ops.normalize(flF, area, scale, 0.0, 8.0)
+        mO = ops.normalize(flO, area, scale, 0.0, 8.0)
+        if n_smooth:
+            mF = ops.smooth(mF, nbr, nrm, n_smooth, alpha, True)
+            mO = ops.smooth(mO, nbr, nrm, n_smooth, alpha, True)
+        bare = ops.belen_coverage(_t(m_i), mF, mO, _t(cos_i), ion_yield, chem)
+        refs.append((mF, mO, bare))
+        items.append(dict(flF=flF, flO=flO, areas=area, nbr=nbr, normals=nrm, scale=scale,
+                          hi=8.0, n_smooth=n_smooth, alpha=alpha, m_i=_t(m_i), cos_i=_t(cos_i),
+                          ion_yield=ion_yield, chem=chem, m_F=torch.empty_like(flF),
+                          m_O=torch.empty_like(flF), bare=torch.empty_like(flF)))
+    ops.finish_round_batch(items)
+    for it, (mF, mO, bare) in zip(items, refs):
+        assert torch.equal(it['m_F'], mF) and torch.equal(it['m_O'], mO)
+        assert torch.equal(it['bare'], bare)
