@@ -52,7 +52,7 @@ def workload_par():
     return par, m.flags
 
 
-def build_loop(n_steps, timer=None):
+def build_loop(n_steps, timer=None, autotune=True):
     import torch
     from petch import Flags, ops
     from petch.threed import EtchLoop, grid_shape
@@ -69,8 +69,9 @@ def build_loop(n_steps, timer=None):
                trench_width=W['width'], hole=False)
     par, fl = workload_par()
     flags = Flags(**fl)
+    # the first (untimed) step picks the BVH build and the trace occupancy by measurement
     loop = EtchLoop(phi, mask, geo, W['mask_th'], DT * n_steps, n_steps, par, flags, N_ION, N_NEU,
-                    reinit_method="fsm", timer=timer)
+                    reinit_method="fsm", timer=timer, autotune=autotune)
     return loop, (nx, ny, nz)
 
 
@@ -577,6 +578,13 @@ def main(argv=None):
                      "hits_per_launch": hits_n / nt_cnt,
                      "bytes_per_hit": NEUTRAL_BYTES_PER_HIT},
     }
+    if loop.tuner is not None and loop.tuned is not None:
+        res["autotune"] = {"bvh_quality": loop.tuned[0], "trace_blocks_per_cu": loop.tuned[1],
+                           "candidates_ms": {f"q{q}/occ{o}": [round(b, 3), round(t, 3)]
+                                             for (q, o), (b, t) in loop.tuner.last.items()},
+                           "note": "build ms, paired neutral trace ms (occupancy 0 = the "
+                                   "kernel's limit)"}
+    ops.set_trace_occupancy(0)
     if not a.no_small_hole:
         res["small_hole"] = small_hole()
     if not a.no_batch:

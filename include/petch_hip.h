@@ -359,6 +359,14 @@ int petch_trace_neutral_rr_batch(petch_ctx* ctx, const petch_source* src, int nf
 /* petch_trace_ion_yield for every feature of the batch, in one launch. */
 int petch_trace_ion_yield_batch(petch_ctx* ctx, int nfeat, const petch_ion_feature* feats);
 
+/* Launch configuration of the persistent trace kernels (no reference counterpart: Warp picks
+ * its own launch shapes): their grid is CUs x min(blocks_per_cu, the kernel's occupancy limit)
+ * workgroups; 0 (the default) means the occupancy limit. The fluxes do not depend on it (a ray's
+ * stream is keyed by its id, not by the wavefront that takes it). petch_trace_occupancy_limit
+ * reports the limit of the neutral trace kernel. Used by petch.autotune. */
+int petch_set_trace_occupancy(petch_ctx* ctx, int blocks_per_cu);
+int petch_trace_occupancy_limit(petch_ctx* ctx, int* blocks_per_cu);
+
 /* Ray-hit counters accumulated by petch_trace_neutral_rr[_pair|_batch] (counts_host[0]) and
  * petch_trace_ion_yield[_batch] (counts_host[1]) on this context since the last reset (work accounting
  * for throughput reports; no reference counterpart). counts_host may be NULL; reset != 0 zeroes

@@ -58,6 +58,7 @@ struct petch_ctx {
   void* noalloc_scratch;
   hipError_t noalloc_err;
   int deterministic;       // traces accumulate in int64 fixed point (petch_set_deterministic)
+  int trace_blocks_per_cu; // persistent trace grids: workgroups per CU (0: the occupancy limit)
 };
 
 struct petch_bvh {

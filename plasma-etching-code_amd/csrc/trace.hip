@@ -1076,6 +1076,8 @@ unsigned persistent_blocks(petch_ctx* ctx, const void* kernel, int n) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBS, 0) != hipSuccess ||
       per_cu <= 0)
     per_cu = 4;
+  if (ctx->trace_blocks_per_cu > 0 && ctx->trace_blocks_per_cu < per_cu)
+    per_cu = ctx->trace_blocks_per_cu;
   long long want = (long long)cus * per_cu;
   long long need = ((long long)n + kBS - 1) / kBS;
   return (unsigned)(need < want ? need : want);
@@ -1504,6 +1506,23 @@ extern "C" int petch_trace_ff(petch_ctx* ctx, const petch_bvh* bvh, const petch_
                      petch_bvh_view(bvh), make_box(box), centroids, gas_normals, nf, n_ff, seed,
                      hitface);
   PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_set_trace_occupancy(petch_ctx* ctx, int blocks_per_cu) {
+  PETCH_ARG(ctx, ctx, "set_trace_occupancy: null context");
+  PETCH_ARG(ctx, blocks_per_cu >= 0, "set_trace_occupancy: blocks_per_cu < 0");
+  ctx->trace_blocks_per_cu = blocks_per_cu;
+  return PETCH_OK;
+}
+
+extern "C" int petch_trace_occupancy_limit(petch_ctx* ctx, int* blocks_per_cu) {
+  PETCH_ARG(ctx, ctx && blocks_per_cu, "trace_occupancy_limit: null pointer");
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace_neutral_rr, kBS,
+                                                   0) != hipSuccess)
+    return petch_set_error(ctx, PETCH_E_HIP, "trace_occupancy_limit: occupancy query failed");
+  *blocks_per_cu = per_cu;
   return PETCH_OK;
 }
 

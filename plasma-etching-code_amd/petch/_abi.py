@@ -156,6 +156,8 @@ SIGNATURES = {
     "petch_bvh_destroy": [P, P],
     "petch_bvh_build": [P, P, P, P, c_int, c_int],
     "petch_bvh_set_quality": [P, P, c_int],
+    "petch_set_trace_occupancy": [P, c_int],
+    "petch_trace_occupancy_limit": [P, POINTER(c_int)],
     "petch_bvh_query_ray": [P, P, P, P, c_int, c_float, P, P],
     "petch_bvh_query_point": [P, P, P, c_int, c_float, P, P],
     "petch_bvh_status": [P, P, POINTER(c_int)],

@@ -30,6 +30,7 @@ class SurfaceMesh:
         self.bufs = ops.MeshBuffers(device)
         self.bvh = ops.Bvh(device)
         self.rays_per_step = rays_per_step
+        self.quality = None    # a fixed BVH build quality (petch.autotune's choice), or None
         self._quality = None
         self.nf = 0
         self.nv = 0
@@ -46,8 +47,10 @@ class SurfaceMesh:
         self.nv, self.nf = int(v.shape[0]), int(f.shape[0])
         self.centroids, self.areas, self.normals = ops.mesh_attrs(v, f)
         self.nbr = ops.face_adjacency(f, self.nv)
-        if self.rays_per_step is not None:
-            q = 2 if self.rays_per_step >= 1000 * max(self.nf, 1) else 0
+        if self.quality is not None or self.rays_per_step is not None:
+            q = self.quality
+            if q is None:
+                q = 2 if self.rays_per_step >= 1000 * max(self.nf, 1) else 0
             if q != self._quality:
                 self.bvh.set_quality(q)
                 self._quality = q

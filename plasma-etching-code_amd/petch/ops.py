@@ -29,6 +29,22 @@ def set_deterministic(on=True, device=None):
     Context.get(dev).call("petch_set_deterministic", int(bool(on)))
 
 
+def set_trace_occupancy(blocks_per_cu, device=None):
+    """Workgroups per CU of the persistent trace grids on `device` (petch_set_trace_occupancy);
+    0 restores the kernels' occupancy limit. The fluxes do not depend on it."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    Context.get(dev).call("petch_set_trace_occupancy", int(blocks_per_cu))
+
+
+def trace_occupancy_limit(device=None):
+    """The neutral trace kernel's workgroups per CU at full occupancy (petch_trace_occupancy_limit)."""
+    import ctypes
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    out = ctypes.c_int(0)
+    Context.get(dev).call("petch_trace_occupancy_limit", ctypes.byref(out))
+    return int(out.value)
+
+
 def init_sdf(nx, ny, nz, dx, Lx, Ly, trench_width, mask_th, sub_top, hole, device=None):
     """Initial phi (float32) and mask (uint8) of make_trench_3d (threed.py:32-59)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)

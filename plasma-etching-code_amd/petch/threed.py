@@ -302,7 +302,7 @@ class EtchLoop:
 
     def __init__(self, phi_d, mask_d, geo, mask_th, t_end, n_steps, par, flags, n_ion, n_neu,
                  reinit_every=1, extend="gpu", reinit_method="fsm", seed_offset=0,
-                 rays_per_point=None, etch_stop_z=None, timer=None):
+                 rays_per_point=None, etch_stop_z=None, timer=None, autotune=None):
         from .profiling import NULL
         from .chemistry import chem_params
         if par is None:
@@ -347,6 +347,13 @@ class EtchLoop:
         if rays_per_point is None:
             rays = int(n_ion) + 2 * int(n_neu) * max(1, int(par.get('n_fp', 1)))
         self.meshes = [_flux.SurfaceMesh(self.dev, rays), _flux.SurfaceMesh(self.dev, rays)]
+        # autotune: a petch.autotune.TraceTuner (or True for a new one) that picks the BVH build
+        # and the trace occupancy on the first surface with a Monte-Carlo neutral trace
+        if autotune is True:
+            from .autotune import TraceTuner
+            autotune = TraceTuner()
+        self.tuner = autotune or None
+        self.tuned = None
         self.prev_mesh = self.prev_bare = self.prev_cen = None
         self.pt_bvh = None if self.gpu_ws else ops.Bvh(self.dev)
         self.red = torch.zeros((2,), dtype=torch.int32, device=self.dev)
@@ -397,6 +404,15 @@ class EtchLoop:
             self._rays = (n, n)
         else:
             self._rays = (self.n_ion, self.n_neu)
+        if self.tuner is not None and self.tuned is None and bvh and mesh.nf > 1 and \
+                self.nt != "radiosity" and getattr(self.flags, "coverage_sticking", False):
+            from .flux import box_of
+            bare = torch.ones((mesh.nf,), dtype=torch.float32, device=self.dev)
+            self.tuned = self.tuner.tune(mesh, box_of(self.geo, self.par), bare,
+                                         self.par.get('betaE', 0.7), self.par.get('betaO', 1.0),
+                                         self._rays[1])
+            for m in self.meshes:
+                m.quality = self.tuned[0]
         if mesh.nf and self.nt != "radiosity" and getattr(self.flags, "coverage_sticking", False) \
                 and self.warm and self.prev_bare is not None:
             return torch.isfinite(mesh.centroids).all()
