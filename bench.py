@@ -470,6 +470,49 @@ def gradient_cost(reps=2, n_rays=200_000, checkpoint_every=5):
             "gradient": [float(x) for x in g]}
 
 
+def config4_cost(n_variants=8, n_rays=1_000_000):
+    """The batched differentiable configuration (petch.diffetch.CONFIG4: a 60 x 10 x 240 trench,
+    30 steps over 6 s, 1M rays per species and step, five-parameter gradient of each variant's
+    profile loss, checkpoints every 6 steps), run variant after variant in deterministic mode;
+    seconds per variant for its target run plus its checkpointed forward and adjoint, and the
+    same variant's plain forward for the cost ratio."""
+    import torch
+    from petch import diffetch as de, ops
+    from petch.params import Flags
+    ops.set_deterministic(True)
+    try:
+        de.variant_gradients(1, n_rays=n_rays,
+                             case=dict(de.CONFIG4, n_steps=2, t_end=de.CONFIG4['t_end'] / 15))
+        def progress(done, secs):
+            if done % 16 == 0:
+                print(f"bench: config4 {done}/{n_variants} variants, {secs:.1f} s",
+                      file=sys.stderr, flush=True)
+        res, secs = de.variant_gradients(n_variants, n_rays=n_rays, progress=progress)
+        (par, flags), = de.variant_recipes(1)
+        dd = de.DiffEtch(**de.CONFIG4, par=par, flags=Flags(**flags), n_ion=n_rays, n_neu=n_rays,
+                         params=de.PARAMS5)
+        target = de.profile_target(dd, dd.theta0)
+        zsel = de.zsel_feature(dd)
+        torch.cuda.synchronize()
+        t = time.time()
+        de.loss_only(dd, dd.theta0, target, zsel)
+        torch.cuda.synchronize()
+        fwd = time.time() - t
+        t = time.time()
+        de.loss_and_gradient(dd, dd.theta0, target, zsel, checkpoint_every=6)
+        torch.cuda.synchronize()
+        ckpt = time.time() - t
+    finally:
+        ops.set_deterministic(False)
+    return {"scope": f"{n_variants} of the 256 variants of the 60x10x240 trench, 30 steps, "
+                     f"{n_rays} rays per species and step, 5-parameter gradient, checkpoints "
+                     f"every 6 steps, variants one after the other, deterministic mode",
+            "variants": n_variants, "seconds": secs, "seconds_per_variant": secs / n_variants,
+            "fwd_s": fwd, "fwd_adj_checkpointed_s": ckpt, "ratio_checkpointed": ckpt / fwd,
+            "losses": [float(l) for l, _ in res],
+            "gradient_first": [float(x) for x in res[0][1]]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -485,9 +528,15 @@ def main(argv=None):
     ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
                     help="only run the batched 64-feature etch for STEPS steps (one JSON line; "
                          "for profiling)")
+    ap.add_argument("--config4", type=int, default=0, metavar="N",
+                    help="only run N variants of the batched differentiable trench (one JSON "
+                         "line)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the differentiable pieces (one JSON line; for profiling)")
     a = ap.parse_args(argv)
+    if a.config4:
+        print(json.dumps({"config4": config4_cost(a.config4)}), flush=True)
+        return
     if a.batch_only:
         print(json.dumps({"batch": batched(n_steps=a.batch_only, sequential=False)}), flush=True)
         return
@@ -594,6 +643,7 @@ def main(argv=None):
         res["adjoint_neutral"] = neutral_adjoint_cost()
         res["adjoint_step"] = step_adjoint_cost()
         res["gradient"] = gradient_cost()
+        res["config4"] = config4_cost()
     if not a.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline()

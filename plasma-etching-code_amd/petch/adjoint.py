@@ -244,6 +244,41 @@ def neutral_rr_pair(bvh, box, src, bare, beta_a, n_a, seed_a, beta_b, n_b, seed_
     return _NeutralRRPair.apply(bare, bvh, box, src, beta_a, n_a, seed_a, beta_b, n_b, seed_b)
 
 
+class _NeutralRRPairBeta(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, bare, beta_a, beta_b, bvh, box, src, n_a, seed_a, n_b, seed_b):
+        b = bare.detach().contiguous()
+        ba, bb = float(beta_a), float(beta_b)
+        fa, fb = ops.trace_neutral_rr_pair(bvh, box, src, b, ba, n_a, seed_a, bb, n_b, seed_b)
+        ctx.save_for_backward(b, beta_a, beta_b)
+        ctx.args = (bvh, box, src, ba, int(n_a), int(seed_a), bb, int(n_b), int(seed_b))
+        return fa, fb
+
+    @staticmethod
+    def backward(ctx, ga, gb):
+        b, ta, tb = ctx.saved_tensors
+        bvh, box, src, ba, n_a, seed_a, bb, n_b, seed_b = ctx.args
+        ga = torch.zeros_like(b) if ga is None else ga.contiguous().float()
+        gb = torch.zeros_like(b) if gb is None else gb.contiguous().float()
+        # each species' own bare gradient (the pair's streams are those of two single launches)
+        g_a = ops.trace_neutral_rr_adjoint(bvh, box, src, b, ba, n_a, seed_a, ga)
+        g_b = ops.trace_neutral_rr_adjoint(bvh, box, src, b, bb, n_b, seed_b, gb)
+        _check_bvh(bvh, "paired RR neutral trace adjoint")
+        bd = b.double()
+        gba = ((bd * g_a.double()).sum() / ba).to(device=ta.device, dtype=ta.dtype)
+        gbb = ((bd * g_b.double()).sum() / bb).to(device=tb.device, dtype=tb.dtype)
+        return g_a + g_b, gba, gbb, None, None, None, None, None, None, None
+
+
+def neutral_rr_pair_beta(bvh, box, src, bare, beta_a, n_a, seed_a, beta_b, n_b, seed_b):
+    """neutral_rr_pair, differentiable also in the species' sticking factors beta_a and beta_b
+    (0-d tensors): the trace sticks with S = bare * beta, and the path-replay adjoint's bare
+    gradient is beta dL/dS, so dL/dbeta = sum_f bare_f g_f / beta with g the species' own bare
+    gradient (two single-species adjoint launches instead of the paired one)."""
+    return _NeutralRRPairBeta.apply(bare, beta_a, beta_b, bvh, box, src, n_a, seed_a, n_b,
+                                    seed_b)
+
+
 def neutral_rr(bvh, box, src, bare, beta, n, seed):
     """Raw RR neutral accumulator (ops.trace_neutral_rr), differentiable in bare. The BVH must
     be unchanged between the forward and the backward pass."""
@@ -280,7 +315,7 @@ def finish(fl, mesh, scale, hi, n_smooth, alpha, blend_each, lo=0.0):
 
 
 def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, n_fp=None,
-                     bare_init=None, flux_scale=None):
+                     bare_init=None, flux_scale=None, betas=None):
     """(bare, m_F, m_O) of the neutral rounds of flux.coupled_flux_steps for given ion terms
     (m_i, cos_i, ion_yield): n_fp x {the F and O RR traces at bare (one paired launch with the
     seeds and streams of the coupled flux), finish, Belen update} from bare = 1, or from
@@ -289,7 +324,9 @@ def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, 
     finish by its transpose, Belen in closed form. flux_scale = (sF, sO), 0-d tensors, multiply
     m_F and m_O where the Belen update reads them: the F and O fluxes enter the chemistry only as
     Fflux * m_F and Oflux * m_O (belen.py:41-47), so sF = Fflux / par['Fflux'] is a change of the
-    F flux, differentiable too. The returned m_F, m_O are the unscaled normalised fluxes."""
+    F flux, differentiable too. betas = (betaE, betaO), 0-d tensors, replace par's sticking
+    factors of the traces and are differentiable (neutral_rr_pair_beta). The returned m_F, m_O
+    are the unscaled normalised fluxes."""
     from .chemistry import chem_params
     from .flux import NEUTRAL_SOURCE, SEED_MASK, box_of
     F = mesh.nf
@@ -308,8 +345,12 @@ def coupled_coverage(mesh, geo, par, flags, m_i, cos_i, ion_yield, n_neu, seed, 
     m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
     for it in range(n_fp):
         sF, sO = (seed * 9 + 2 + 2 * it) & SEED_MASK, (seed * 9 + 3 + 2 * it) & SEED_MASK
-        fF, fO = neutral_rr_pair(mesh.bvh, box, NEUTRAL_SOURCE, bare, betaE, n_neu, sF, betaO,
-                                 n_neu, sO)
+        if betas is None:
+            fF, fO = neutral_rr_pair(mesh.bvh, box, NEUTRAL_SOURCE, bare, betaE, n_neu, sF, betaO,
+                                     n_neu, sO)
+        else:
+            fF, fO = neutral_rr_pair_beta(mesh.bvh, box, NEUTRAL_SOURCE, bare, betas[0], n_neu,
+                                          sF, betas[1], n_neu, sO)
         m_F = finish(fF, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
         m_O = finish(fO, mesh, A_src / n_neu, 8.0, n_smooth, alpha, blend_each)
         if flux_scale is None:

@@ -42,6 +42,7 @@ from . import flux as _flux
 from .params import PAR
 
 PARAMS = ("Emean", "Fflux", "Oflux", "rate_scale")
+PARAMS5 = PARAMS + ("betaE",)       # with the F sticking factor of the neutral traces
 
 
 class DiffEtch:
@@ -49,7 +50,7 @@ class DiffEtch:
     as a function of theta = (Emean, Fflux, Oflux, rate_scale), a float64 (4,) tensor."""
 
     def __init__(self, Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole, t_end, n_steps,
-                 par=None, flags=None, n_ion=20000, n_neu=20000, seed_offset=0):
+                 par=None, flags=None, n_ion=20000, n_neu=20000, seed_offset=0, params=PARAMS):
         from .chemistry import chem_params
         from .threed import _dev, _trench_geo
         par = dict(PAR if par is None else par)
@@ -71,7 +72,10 @@ class DiffEtch:
         self.mg = ops.MaskGeom(float(sub_top), float(mask_th), float(Lx), float(Ly),
                                float(trench_width), int(bool(hole)))
         self.chem = chem_params(par, flags)
-        self.theta0 = torch.tensor([float(par[k]) for k in PARAMS], dtype=torch.float64)
+        if tuple(params) not in (PARAMS, PARAMS5):
+            raise ValueError(f"DiffEtch params must be {PARAMS} or {PARAMS5}")
+        self.params = tuple(params)
+        self.theta0 = torch.tensor([float(par[k]) for k in self.params], dtype=torch.float64)
         self.rays = int(n_ion) + 2 * int(n_neu) * max(1, int(par.get('n_fp', 1)))
         self.warm = getattr(flags, "warm_start_coverage", False)
         self.nsub = {}      # step -> CFL substeps of the last forward (a recompute reuses them)
@@ -100,7 +104,11 @@ class DiffEtch:
             return phi, None
         th = theta.to(dev)
         E = th[0]
-        sF, sO, sR = (th[1:] / self.theta0[1:].to(dev)).unbind()
+        sF, sO, sR = (th[1:4] / self.theta0[1:4].to(dev)).unbind()
+        betas = None
+        if len(self.params) > 4:
+            betas = (th[4], torch.tensor(float(self.par.get('betaO', 1.0)), dtype=torch.float64,
+                                         device=dev))
         seed = step + self.seed_offset
         m_i, cos_i, iy = adjoint.ion_terms(mesh, self.geo, self.par, self.n_ion, seed, E)
         bare_init, n_fp = None, int(self.par.get('n_fp', 4))
@@ -119,7 +127,8 @@ class DiffEtch:
             n_fp = int(self.par.get('n_fp', 1))
         bare, m_F, m_O = adjoint.coupled_coverage(mesh, self.geo, self.par, self.flags, m_i,
                                                   cos_i, iy, self.n_neu, seed, n_fp,
-                                                  bare_init=bare_init, flux_scale=(sF, sO))
+                                                  bare_init=bare_init, flux_scale=(sF, sO),
+                                                  betas=betas)
         V = adjoint.surface_rate(m_i, m_F * sF.float(), m_O * sO.float(), cos_i, iy,
                                  mesh.centroids, self.mg, self.chem)
         V = V * sR.float()
@@ -220,3 +229,54 @@ def zsel_feature(dd, below=None):
     top = dd.geo['sub_top'] + float(dd.mg.mask_th)
     lo = dd.geo['sub_top'] - (below if below is not None else dd.geo['sub_top'])
     return torch.as_tensor((zs >= lo) & (zs < top))
+
+
+# The batched differentiable configuration: 256 recipe variants of one deep trench (width
+# 1.5 um, 20 um of substrate under a 2 um mask, dx 0.1 um: 60 x 10 x 240 nodes), 30 steps over
+# 6 s, 1M rays per species and step, the profile loss of each variant against its own target,
+# gradients in five parameters (with the F sticking factor betaE), checkpoints every 6 steps.
+CONFIG4 = dict(Lx=6.0, Ly=1.0, Lz=24.0, dx=0.1, trench_width=1.5, mask_th=2.0, sub_top=20.0,
+               hole=False, t_end=6.0, n_steps=30)
+PERTURB5 = (1.10, 0.90, 1.10, 1.10, 0.90)
+
+
+def variant_recipes(n, seed=4321):
+    """The variants' recipes (SF6O2 defaults, lateral periodic): mean ion energy U(50, 200) eV
+    and O/F flux ratio U(0.1, 1.0), seeded (as the batched features draw theirs)."""
+    import numpy as np
+    from .api import SF6O2
+    r = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m = SF6O2(meanE=float(r.uniform(50, 200)))
+        par = dict(m.par)
+        par['Oflux'] = float(r.uniform(0.1, 1.0)) * par['Fflux']
+        par['periodic_y'] = 1
+        out.append((par, m.flags))
+    return out
+
+
+def variant_gradients(n_variants, n_rays=1_000_000, checkpoint_every=6, seed=4321, case=None,
+                      target_seed=1234, progress=None):
+    """Loss and five-parameter gradient of every variant (one after the other; each variant's
+    run is its own DiffEtch): (list of (loss, gradient), seconds). The target of variant v is
+    the profile of its own run with seed offset `target_seed` at the parameters PERTURB5 x its
+    recipe's. progress(done, seconds) is called after each variant."""
+    import time
+    from .params import Flags
+    case = dict(CONFIG4 if case is None else case)
+    res = []
+    t0 = time.time()
+    for par, flags in variant_recipes(n_variants, seed):
+        dd = DiffEtch(**case, par=par, flags=Flags(**flags), n_ion=n_rays, n_neu=n_rays,
+                      params=PARAMS5)
+        tg = DiffEtch(**case, par=par, flags=Flags(**flags), n_ion=n_rays, n_neu=n_rays,
+                      seed_offset=target_seed, params=PARAMS5)
+        target = profile_target(tg, tg.theta0 * torch.tensor(PERTURB5, dtype=torch.float64))
+        zsel = zsel_feature(dd)
+        res.append(loss_and_gradient(dd, dd.theta0, target, zsel,
+                                     checkpoint_every=checkpoint_every))
+        if progress is not None:
+            progress(len(res), time.time() - t0)
+    torch.cuda.synchronize()
+    return res, time.time() - t0
