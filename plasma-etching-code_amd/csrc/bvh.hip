@@ -12,6 +12,7 @@
 //            grandchildren (one parallel pass)
 // plus the two query entry points used by the tests (the trace kernels inline the traversal).
 #include "common.h"
+#include <vector>
 
 #include <stdlib.h>
 #include "bvh.h"
@@ -73,11 +74,12 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-__global__ void __launch_bounds__(kBS) k_bvh_prep(const float* __restrict__ verts,
-                                                  const int* __restrict__ faces, int nf,
-                                                  float4* __restrict__ cen,
-                                                  float* __restrict__ part) {
-  int f = blockIdx.x * kBS + threadIdx.x;
+// Centroids of block lb's faces and the block's centroid bounds (6 floats at part[6 lb]).
+__device__ __forceinline__ void bvh_prep_block(const float* __restrict__ verts,
+                                               const int* __restrict__ faces, int nf,
+                                               float4* __restrict__ cen, float* __restrict__ part,
+                                               int lb) {
+  int f = lb * kBS + threadIdx.x;
   float c[3] = {3.0e38f, 3.0e38f, 3.0e38f}, cmax[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
   if (f < nf) {
     int a = faces[3 * f], b = faces[3 * f + 1], d = faces[3 * f + 2];
@@ -103,12 +105,20 @@ __global__ void __launch_bounds__(kBS) k_bvh_prep(const float* __restrict__ vert
     int k = threadIdx.x;
     float v = s[0][k];
     for (int q = 1; q < kBS / 64; ++q) v = (k < 3) ? fminf(v, s[q][k]) : fmaxf(v, s[q][k]);
-    part[blockIdx.x * 6 + k] = v;
+    part[lb * 6 + k] = v;
   }
 }
 
-__global__ void __launch_bounds__(kBS) k_bvh_bounds(const float* __restrict__ part, int nb,
-                                                    float* __restrict__ scene) {
+__global__ void __launch_bounds__(kBS) k_bvh_prep(const float* __restrict__ verts,
+                                                  const int* __restrict__ faces, int nf,
+                                                  float4* __restrict__ cen,
+                                                  float* __restrict__ part) {
+  bvh_prep_block(verts, faces, nf, cen, part, blockIdx.x);
+}
+
+// The scene bounds from nb block partials (one workgroup).
+__device__ __forceinline__ void bvh_bounds_wg(const float* __restrict__ part, int nb,
+                                              float* __restrict__ scene) {
   __shared__ float s[6][kBS];
   for (int k = 0; k < 6; ++k) {
     float v = (k < 3) ? 3.0e38f : -3.0e38f;
@@ -125,6 +135,11 @@ __global__ void __launch_bounds__(kBS) k_bvh_bounds(const float* __restrict__ pa
   }
 }
 
+__global__ void __launch_bounds__(kBS) k_bvh_bounds(const float* __restrict__ part, int nb,
+                                                    float* __restrict__ scene) {
+  bvh_bounds_wg(part, nb, scene);
+}
+
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   v = (v * 0x00010001u) & 0xFF0000FFu;
   v = (v * 0x00000101u) & 0x0F00F00Fu;
@@ -133,11 +148,10 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   return v;
 }
 
-__global__ void k_bvh_morton(const float4* __restrict__ cen, int nf,
-                             const float* __restrict__ scene, uint32_t* __restrict__ keys,
-                             uint32_t* __restrict__ vals) {
-  int f = blockIdx.x * kBS + threadIdx.x;
-  if (f >= nf) return;
+__device__ __forceinline__ void bvh_morton_face(const float4* __restrict__ cen,
+                                                const float* __restrict__ scene,
+                                                uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals, int f) {
   float4 c = cen[f];
   float q[3] = {c.x, c.y, c.z};
   uint32_t u[3];
@@ -151,16 +165,29 @@ __global__ void k_bvh_morton(const float4* __restrict__ cen, int nf,
   vals[f] = (uint32_t)f;
 }
 
-__global__ void k_bvh_leaves(const float* __restrict__ verts, const int* __restrict__ faces,
-                             const uint32_t* __restrict__ vals, int nf,
-                             float4* __restrict__ tris) {
-  int s = blockIdx.x * kBS + threadIdx.x;
-  if (s >= nf) return;
+__global__ void k_bvh_morton(const float4* __restrict__ cen, int nf,
+                             const float* __restrict__ scene, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f < nf) bvh_morton_face(cen, scene, keys, vals, f);
+}
+
+__device__ __forceinline__ void bvh_leaf(const float* __restrict__ verts,
+                                         const int* __restrict__ faces,
+                                         const uint32_t* __restrict__ vals,
+                                         float4* __restrict__ tris, int s) {
   int f = (int)vals[s];
   int a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
   tris[3 * s] = make_float4(verts[3 * a], verts[3 * a + 1], verts[3 * a + 2], __int_as_float(f));
   tris[3 * s + 1] = make_float4(verts[3 * b], verts[3 * b + 1], verts[3 * b + 2], 0.f);
   tris[3 * s + 2] = make_float4(verts[3 * c], verts[3 * c + 1], verts[3 * c + 2], 0.f);
+}
+
+__global__ void k_bvh_leaves(const float* __restrict__ verts, const int* __restrict__ faces,
+                             const uint32_t* __restrict__ vals, int nf,
+                             float4* __restrict__ tris) {
+  int s = blockIdx.x * kBS + threadIdx.x;
+  if (s < nf) bvh_leaf(verts, faces, vals, tris, s);
 }
 
 __device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int n, int i, int j) {
@@ -170,10 +197,9 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ k, int n, int 
   return __clz(a ^ b);
 }
 
-__global__ void k_bvh_karras(const uint32_t* __restrict__ keys, int n, BvhNode* __restrict__ nodes,
-                             int* __restrict__ pint, int* __restrict__ pleaf) {
-  int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n - 1) return;
+__device__ __forceinline__ void bvh_karras_node(const uint32_t* __restrict__ keys, int n,
+                                                BvhNode* __restrict__ nodes, int* __restrict__ pint,
+                                                int* __restrict__ pleaf, int i) {
   int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
   int dmin = delta(keys, n, i, i - d);
   int lmax = 2;
@@ -197,6 +223,12 @@ __global__ void k_bvh_karras(const uint32_t* __restrict__ keys, int n, BvhNode* 
   if (left < 0) pleaf[~left] = i; else pint[left] = i;
   if (right < 0) pleaf[~right] = i; else pint[right] = i;
   if (i == 0) pint[0] = -1;
+}
+
+__global__ void k_bvh_karras(const uint32_t* __restrict__ keys, int n, BvhNode* __restrict__ nodes,
+                             int* __restrict__ pint, int* __restrict__ pleaf) {
+  int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n - 1) bvh_karras_node(keys, n, nodes, pint, pleaf, i);
 }
 
 __device__ __forceinline__ void put_child_box(BvhNode* nd, bool left, const float lo[3],
@@ -274,10 +306,9 @@ __global__ void __launch_bounds__(kRefitWG) k_bvh_refit_wg(const RefitItem* __re
                                              it.root, s);
 }
 
-__global__ void k_bvh_wide(const BvhNode* __restrict__ nodes, const int* __restrict__ pint, int n,
-                           BvhWide* __restrict__ wide) {
-  int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= n - 1) return;
+__device__ __forceinline__ void bvh_wide_node(const BvhNode* __restrict__ nodes,
+                                              const int* __restrict__ pint, int n,
+                                              BvhWide* __restrict__ wide, int i) {
   int depth = 0;
   int p = pint[i];
   for (int guard = 0; p >= 0 && p < n - 1 && guard < n; ++guard) {
@@ -328,6 +359,76 @@ __global__ void k_bvh_wide(const BvhNode* __restrict__ nodes, const int* __restr
   w.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
   w.pad = make_int4(0, 0, 0, 0);
   wide[i] = w;
+}
+
+__global__ void k_bvh_wide(const BvhNode* __restrict__ nodes, const int* __restrict__ pint, int n,
+                           BvhWide* __restrict__ wide) {
+  int i = blockIdx.x * kBS + threadIdx.x;
+  if (i < n - 1) bvh_wide_node(nodes, pint, n, wide, i);
+}
+
+// The Morton-tree builds of many meshes (the features of a batched step) side by side: one
+// launch per stage for all of them. Item t owns the workgroups [fblk0[t], fblk0[t+1]) of the
+// launches over faces and [nblk0[t], nblk0[t+1]) of those over internal nodes; every stage runs
+// the same device code as the single build, so the trees are the same.
+struct BuildItem {
+  const float* verts;
+  const int* faces;
+  int nf, nfb;               // faces, and workgroups over them
+  Work w;
+  BvhNode* nodes;
+  BvhWide* wide;
+  float4* tris;
+  long long fblk0, nblk0;
+};
+
+__device__ __forceinline__ int build_item_of(const BuildItem* __restrict__ it, int n, bool nodes) {
+  int lo = 0, hi = n - 1;
+  const long long b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((nodes ? it[mid].nblk0 : it[mid].fblk0) <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kBS) k_bvh_prep_many(const BuildItem* __restrict__ items, int n) {
+  const BuildItem& it = items[build_item_of(items, n, false)];
+  bvh_prep_block(it.verts, it.faces, it.nf, it.w.cen, it.w.part, (int)(blockIdx.x - it.fblk0));
+}
+
+// one workgroup per item: the scene bounds, and the traversal error flag cleared
+__global__ void __launch_bounds__(kBS) k_bvh_bounds_many(const BuildItem* __restrict__ items) {
+  const BuildItem& it = items[blockIdx.x];
+  bvh_bounds_wg(it.w.part, it.nfb, it.w.scene);
+  if (threadIdx.x == 0) *it.w.err = 0;
+}
+
+__global__ void k_bvh_morton_many(const BuildItem* __restrict__ items, int n) {
+  const BuildItem& it = items[build_item_of(items, n, false)];
+  const int f = (int)(blockIdx.x - it.fblk0) * kBS + threadIdx.x;
+  if (f < it.nf) bvh_morton_face(it.w.cen, it.w.scene, it.w.keys, it.w.vals, f);
+}
+
+// the leaves in Morton order, and the refit flags cleared
+__global__ void k_bvh_leaves_many(const BuildItem* __restrict__ items, int n) {
+  const BuildItem& it = items[build_item_of(items, n, false)];
+  const int f = (int)(blockIdx.x - it.fblk0) * kBS + threadIdx.x;
+  if (f >= it.nf) return;
+  bvh_leaf(it.verts, it.faces, it.w.vals, it.tris, f);
+  it.w.flags[f] = 0;
+}
+
+__global__ void k_bvh_karras_many(const BuildItem* __restrict__ items, int n) {
+  const BuildItem& it = items[build_item_of(items, n, true)];
+  const int i = (int)(blockIdx.x - it.nblk0) * kBS + threadIdx.x;
+  if (i < it.nf - 1) bvh_karras_node(it.w.keys, it.nf, it.nodes, it.w.pint, it.w.pleaf, i);
+}
+
+__global__ void k_bvh_wide_many(const BuildItem* __restrict__ items, int n) {
+  const BuildItem& it = items[build_item_of(items, n, true)];
+  const int i = (int)(blockIdx.x - it.nblk0) * kBS + threadIdx.x;
+  if (i < it.nf - 1) bvh_wide_node(it.nodes, it.w.pint, it.nf, it.wide, i);
 }
 
 // Surface area of the box lo..hi (0 for an inverted box).
@@ -798,16 +899,9 @@ int ploc_build(petch_ctx* ctx, petch_bvh* b, int nf) {
 
 namespace {
 
-// Everything of a build up to the refit: buffers, centroids, Morton sort, leaves, Karras nodes
-// and the cleared refit flags. Returns the refit's item (n = nf; nothing to refit when nf < 2).
-int build_prepare(petch_ctx* ctx, petch_bvh* b, const float* verts, const int* faces, int nv,
-                  int nf, RefitItem* refit) {
-  PETCH_ARG(ctx, ctx && b, "bvh_build: null pointer");
-  PETCH_ARG(ctx, nf >= 0 && nv >= 0, "bvh_build: negative size");
-  PETCH_ARG(ctx, nf == 0 || (verts && faces), "bvh_build: null mesh");
-  b->nv = nv;
-  b->nf = 0;
-  refit->n = 0;
+// Grows the BVH's device buffers to hold nf faces (they are never shrunk); a BVH that was never
+// built gets a minimal set, so its error flag is valid.
+int ensure_capacity(petch_ctx* ctx, petch_bvh* b, int nf) {
   if (nf > b->cap_f) {
     PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (b->nodes) (void)hipFree(b->nodes);
@@ -830,6 +924,21 @@ int build_prepare(petch_ctx* ctx, petch_bvh* b, const float* verts, const int* f
     PETCH_HIP(ctx, hipMalloc(&b->work, work_bytes(cap)));
     b->cap_f = cap;
   }
+  return PETCH_OK;
+}
+
+// Everything of a build up to the refit: buffers, centroids, Morton sort, leaves, Karras nodes
+// and the cleared refit flags. Returns the refit's item (n = nf; nothing to refit when nf < 2).
+int build_prepare(petch_ctx* ctx, petch_bvh* b, const float* verts, const int* faces, int nv,
+                  int nf, RefitItem* refit) {
+  PETCH_ARG(ctx, ctx && b, "bvh_build: null pointer");
+  PETCH_ARG(ctx, nf >= 0 && nv >= 0, "bvh_build: negative size");
+  PETCH_ARG(ctx, nf == 0 || (verts && faces), "bvh_build: null mesh");
+  b->nv = nv;
+  b->nf = 0;
+  refit->n = 0;
+  const int rc0 = ensure_capacity(ctx, b, nf);
+  if (rc0 != PETCH_OK) return rc0;
   Work w = carve(b->work, b->cap_f);
   PETCH_HIP(ctx, hipMemsetAsync(w.err, 0, sizeof(int), ctx->stream));
   if (nf == 0) return PETCH_OK;
@@ -904,35 +1013,123 @@ extern "C" int petch_bvh_build_batch(petch_ctx* ctx, int n, petch_bvh* const* bv
             "bvh_build_batch: null pointer");
   PETCH_ARG(ctx, n >= 0, "bvh_build_batch: n < 0");
   if (n == 0) return PETCH_OK;
-  RefitItem* items = (RefitItem*)calloc((size_t)n, sizeof(RefitItem));
-  if (!items) return petch_set_error(ctx, PETCH_E_HIP, "bvh_build_batch: out of host memory");
-  int m = 0;
   for (int t = 0; t < n; ++t) {
-    RefitItem r;
-    const int rc = build_prepare(ctx, bvhs[t], verts[t], faces[t], nv[t], nf[t], &r);
-    if (rc != PETCH_OK) {
-      free(items);
-      return rc;
-    }
-    if (r.n > 1) items[m++] = r;
+    PETCH_ARG(ctx, bvhs[t], "bvh_build_batch: item %d has a null BVH", t);
+    PETCH_ARG(ctx, nf[t] >= 0 && nv[t] >= 0, "bvh_build_batch: item %d has a negative size", t);
+    PETCH_ARG(ctx, nf[t] == 0 || (verts[t] && faces[t]), "bvh_build_batch: item %d has a null "
+              "mesh", t);
   }
+  // Morton trees of two or more faces take the batched stages; the others (clustered or
+  // area-collapsed trees, meshes under two faces) are prepared one by one
+  std::vector<RefitItem> refit;
+  std::vector<BuildItem> fast;
+  std::vector<petch_sort_item> sorts;
+  std::vector<int> fast_t;
+  long long fb = 0, nb = 0, tiles = 0, tab = 0;
+  for (int t = 0; t < n; ++t) {
+    petch_bvh* b = bvhs[t];
+    const int q = b->quality >= 0 ? b->quality : (nf[t] >= kWideSahMinFaces ? 2 : 0);
+    if (q != 0 || nf[t] < 2) {
+      RefitItem r;
+      const int rc = build_prepare(ctx, b, verts[t], faces[t], nv[t], nf[t], &r);
+      if (rc != PETCH_OK) return rc;
+      if (r.n > 1) refit.push_back(r);
+      continue;
+    }
+    b->nv = nv[t];
+    b->nf = 0;
+    const int rc = ensure_capacity(ctx, b, nf[t]);
+    if (rc != PETCH_OK) return rc;
+    BuildItem it;
+    it.verts = verts[t];
+    it.faces = faces[t];
+    it.nf = nf[t];
+    it.nfb = (int)petch_blocks(nf[t], kBS);
+    it.w = carve(b->work, b->cap_f);
+    it.nodes = (BvhNode*)b->nodes;
+    it.wide = (BvhWide*)b->wide;
+    it.tris = (float4*)b->tris;
+    it.fblk0 = fb;
+    it.nblk0 = nb;
+    fb += it.nfb;
+    nb += petch_blocks(nf[t] - 1, kBS);
+    petch_sort_item si;
+    si.keys = it.w.keys;
+    si.vals = it.w.vals;
+    si.ktmp = it.w.ktmp;
+    si.vtmp = it.w.vtmp;
+    si.n = nf[t];
+    si.nblocks = petch_sort_tiles(nf[t]);
+    si.tblk0 = tiles;
+    si.hist = si.offs = nullptr;
+    tiles += si.nblocks;
+    tab += 2LL * 256 * si.nblocks;
+    fast.push_back(it);
+    sorts.push_back(si);
+    fast_t.push_back(t);
+    refit.push_back(RefitItem{it.tris, it.nodes, it.w.pint, it.w.pleaf, it.w.flags, it.w.root,
+                              it.nf});
+  }
+  PETCH_ARG(ctx, fb <= 0x7fffffffLL && tiles <= 0x7fffffffLL,
+            "bvh_build_batch: too many faces for one launch");
+  const int m = (int)fast.size();
   if (m > 0) {
-    const RefitItem* d = (const RefitItem*)petch_upload_table(ctx, items,
-                                                              (size_t)m * sizeof(RefitItem));
-    if (!d) {
-      free(items);
-      return PETCH_E_HIP;
-    }
-    hipLaunchKernelGGL(k_bvh_refit_wg, dim3((unsigned)m), dim3(kRefitWG), 0, ctx->stream, d);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-      free(items);
-      return petch_set_error(ctx, PETCH_E_HIP, "bvh_build_batch: refit launch: %s",
-                             hipGetErrorString(e));
+    int* tabs = (int*)petch_scratch(ctx, SCR_SORT_MANY, (size_t)tab * sizeof(int));
+    if (!tabs) return PETCH_E_HIP;
+    long long off = 0;
+    for (auto& si : sorts) {
+      si.hist = tabs + off;
+      si.offs = tabs + off + 256LL * si.nblocks;
+      off += 2LL * 256 * si.nblocks;
     }
   }
-  free(items);
+  // one table upload for every stage: build items, sort items, refit items
+  const size_t sz_b = fast.size() * sizeof(BuildItem), sz_s = sorts.size() * sizeof(petch_sort_item);
+  const size_t off_s = (sz_b + 255) & ~(size_t)255, off_r = off_s + ((sz_s + 255) & ~(size_t)255);
+  const size_t total = off_r + refit.size() * sizeof(RefitItem);
+  if (refit.empty()) {
+    for (int t = 0; t < n; ++t) {
+      const int rc = build_finish(ctx, bvhs[t], nf[t]);
+      if (rc != PETCH_OK) return rc;
+    }
+    return PETCH_OK;
+  }
+  std::vector<char> blob(total);
+  if (sz_b) memcpy(blob.data(), fast.data(), sz_b);
+  if (sz_s) memcpy(blob.data() + off_s, sorts.data(), sz_s);
+  memcpy(blob.data() + off_r, refit.data(), refit.size() * sizeof(RefitItem));
+  char* d = (char*)petch_upload_table(ctx, blob.data(), total);
+  if (!d) return PETCH_E_HIP;
+  const BuildItem* db = (const BuildItem*)d;
+  const petch_sort_item* ds = (const petch_sort_item*)(d + off_s);
+  const RefitItem* dr = (const RefitItem*)(d + off_r);
+  if (m > 0) {
+    hipLaunchKernelGGL(k_bvh_prep_many, dim3((unsigned)fb), dim3(kBS), 0, ctx->stream, db, m);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_bvh_bounds_many, dim3((unsigned)m), dim3(kBS), 0, ctx->stream, db);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_bvh_morton_many, dim3((unsigned)fb), dim3(kBS), 0, ctx->stream, db, m);
+    PETCH_LAUNCH_CHECK(ctx);
+    const int rc = petch_radix_sort_pairs_many(ctx, ds, m, tiles);
+    if (rc != PETCH_OK) return rc;
+    hipLaunchKernelGGL(k_bvh_leaves_many, dim3((unsigned)fb), dim3(kBS), 0, ctx->stream, db, m);
+    PETCH_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_bvh_karras_many, dim3((unsigned)nb), dim3(kBS), 0, ctx->stream, db, m);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_bvh_refit_wg, dim3((unsigned)refit.size()), dim3(kRefitWG), 0, ctx->stream,
+                     dr);
+  PETCH_LAUNCH_CHECK(ctx);
+  if (m > 0) {
+    hipLaunchKernelGGL(k_bvh_wide_many, dim3((unsigned)nb), dim3(kBS), 0, ctx->stream, db, m);
+    PETCH_LAUNCH_CHECK(ctx);
+    for (int k = 0; k < m; ++k) bvhs[fast_t[k]]->nf = nf[fast_t[k]];
+  }
+  // the others: their clustered tree / area-driven collapse, one by one
+  std::vector<char> done((size_t)n, 0);
+  for (int k = 0; k < m; ++k) done[(size_t)fast_t[k]] = 1;
   for (int t = 0; t < n; ++t) {
+    if (done[(size_t)t]) continue;
     const int rc = build_finish(ctx, bvhs[t], nf[t]);
     if (rc != PETCH_OK) return rc;
   }

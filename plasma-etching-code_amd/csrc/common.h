@@ -36,6 +36,7 @@ enum PetchScratch {
   SCR_EIK_G,         // reinit adjoint: adjoint ping-pong
   SCR_LS_BRICK,      // batched level-set stages: per-brick activity flags and list counters
   SCR_LS_LIST,       // batched level-set stages: lists of the bricks a launch processes
+  SCR_SORT_MANY,     // batched radix sort: per-array digit tables
   SCR_COUNT
 };
 
@@ -97,6 +98,19 @@ void* petch_pinned(petch_ctx* ctx, size_t bytes);
 // Exclusive scan of n int32 counts into out (distinct from in); the total goes to *total_dev.
 // Uses the SCR_RED scratch slot.
 int petch_exclusive_scan(petch_ctx* ctx, const int* in, int* out, long long n, int* total_dev);
+
+// One array of a batched radix sort (petch_radix_sort_pairs_many): keys/vals sorted by their 32
+// key bits with ktmp/vtmp as n-element scratch; hist and offs hold 256 x nblocks ints each;
+// tblk0 is the array's first workgroup in the batched launches.
+struct petch_sort_item {
+  uint32_t *keys, *vals, *ktmp, *vtmp;
+  int *hist, *offs;
+  int n, nblocks;
+  long long tblk0;
+};
+int petch_radix_sort_pairs_many(petch_ctx* ctx, const petch_sort_item* items_dev, int n_items,
+                                long long tiles);
+int petch_sort_tiles(int n);   // workgroups (tiles) of one array in the sort launches
 
 #define PETCH_HIP(ctx, call)                                                              \
   do {                                                                                    \
