@@ -424,6 +424,21 @@ typedef struct petch_round_item {
   float scale, hi, alpha;
 } petch_round_item;
 int petch_finish_round_batch(petch_ctx* ctx, int n, const petch_round_item* items_host);
+
+/* The end of the faithful-ion trace of many features in one launch (no reference counterpart:
+ * the reference finishes one feature at a time, threed.py:812-821): per feature, acc is the
+ * trace's (5, nf) accumulator (sp, ie, ip, arrivals, angle sum); out (5, nf) receives the three
+ * yields and m_i (petch_normalize without a clip, then with smooth = 1 one petch_smooth pass
+ * with blend alpha) and cos_i (petch_cos_ratio of arrivals and angle sum). The same arithmetic as
+ * those calls, so the outputs equal theirs. */
+typedef struct petch_ion_finish_item {
+  const float *acc, *area, *normals;
+  const int* nbr;
+  float* out;
+  int nf, smooth;
+  float scale, alpha;
+} petch_ion_finish_item;
+int petch_finish_ion_batch(petch_ctx* ctx, int n, const petch_ion_finish_item* items_host);
 /* Closed-form adjoint of petch_belen_coverage for gbare = d loss / d bare (zero where the clip
  * of 1 - thF - thO is active): gm_F, gm_O = d loss / d m_F, d m_O; with iy_ie/iy_ip also
  * g_ie, g_ip = d loss / d iy_ie, d iy_ip, else gm_i = d loss / d m_i (cos_i held constant).

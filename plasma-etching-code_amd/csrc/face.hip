@@ -231,6 +231,44 @@ __global__ void k_finish_round(const RoundItem* __restrict__ items, int n) {
   it.bare[f] = belen_value(it.m_i, mF, mO, it.cos_i, it.iy_ie, it.iy_ip, it.c, f, thF, thO);
 }
 
+// The end of the faithful-ion trace of many features (petch.batch): the three yield channels and
+// the arrival count normalised without a clip and smoothed once (k_normalize, k_smooth), and
+// cos_i = ang / arrivals (k_cos_ratio), through the same device functions. acc is the trace's
+// (5, nf) accumulator (sp, ie, ip, arrivals, angle sum); out receives (sp, ie, ip, m_i, cos_i)
+// as five rows of nf.
+struct IonItem {
+  const float *acc, *area, *nrm;
+  const int* nbr;
+  float* out;
+  int nf, smooth;
+  float scale, alpha;
+  long long blk0;
+};
+
+__global__ void k_finish_ion(const IonItem* __restrict__ items, int n) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].blk0 <= (long long)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const IonItem& it = items[lo];
+  const int f = (int)(blockIdx.x - it.blk0) * kBS + threadIdx.x;
+  const int F = it.nf;
+  if (f >= F) return;
+  const float* area = it.area;
+  const float scale = it.scale;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float* a = it.acc + (long long)k * F;
+    auto v = [=](int g) { return norm_value(a[g], area[g], scale, -INFINITY, INFINITY); };
+    float m = v(f);
+    if (it.smooth) m = smooth_value(v, it.nbr, it.nrm, F, it.alpha, m, f);
+    it.out[(long long)k * F + f] = m;
+  }
+  const float x = it.acc[3LL * F + f];
+  it.out[4LL * F + f] = x > 0.f ? it.acc[4LL * F + f] / x : 0.f;
+}
+
 // Closed-form adjoint of k_belen_coverage for gbare = d loss / d bare (the clip passes the
 // gradient where 0 <= 1 - thF - thO <= 1). With a = (k_sigma + 2 GY_ie) / GbE and
 // b = (beta_sigma + GY_p) / GbP:
@@ -720,6 +758,38 @@ extern "C" int petch_finish_round_batch(petch_ctx* ctx, int n, const petch_round
       (const RoundItem*)petch_upload_table(ctx, t.data(), t.size() * sizeof(RoundItem));
   if (!d) return PETCH_E_HIP;
   hipLaunchKernelGGL(k_finish_round, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d,
+                     (int)t.size());
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_finish_ion_batch(petch_ctx* ctx, int n, const petch_ion_finish_item* items) {
+  PETCH_ARG(ctx, ctx && (n == 0 || items), "finish_ion_batch: null pointer");
+  PETCH_ARG(ctx, n >= 0, "finish_ion_batch: n < 0");
+  std::vector<IonItem> t;
+  t.reserve((size_t)n);
+  long long blocks = 0;
+  for (int k = 0; k < n; ++k) {
+    const petch_ion_finish_item& q = items[k];
+    PETCH_ARG(ctx, q.nf >= 0 && q.acc && q.area && q.out, "finish_ion_batch: item %d is invalid",
+              k);
+    PETCH_ARG(ctx, !q.smooth || (q.nbr && q.normals), "finish_ion_batch: item %d needs the "
+              "adjacency and normals to smooth", k);
+    if (q.nf == 0) continue;
+    IonItem r;
+    r.acc = q.acc; r.area = q.area; r.nrm = q.normals; r.nbr = q.nbr; r.out = q.out;
+    r.nf = q.nf;
+    r.smooth = q.smooth && q.alpha > 0.f;
+    r.scale = q.scale; r.alpha = q.alpha;
+    r.blk0 = blocks;
+    blocks += petch_blocks(q.nf, kBS);
+    t.push_back(r);
+  }
+  if (t.empty()) return PETCH_OK;
+  PETCH_ARG(ctx, blocks <= 0x7fffffffLL, "finish_ion_batch: too many faces for one launch");
+  const IonItem* d = (const IonItem*)petch_upload_table(ctx, t.data(), t.size() * sizeof(IonItem));
+  if (!d) return PETCH_E_HIP;
+  hipLaunchKernelGGL(k_finish_ion, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d,
                      (int)t.size());
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;

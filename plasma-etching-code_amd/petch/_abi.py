@@ -109,6 +109,13 @@ class RoundItem(ctypes.Structure):
                 ("hi", c_float), ("alpha", c_float)]
 
 
+class IonFinishItem(ctypes.Structure):
+    """petch_ion_finish_item: one feature of a batched faithful-ion finish."""
+    _fields_ = [("acc", c_void_p), ("area", c_void_p), ("normals", c_void_p), ("nbr", c_void_p),
+                ("out", c_void_p), ("nf", c_int), ("smooth", c_int), ("scale", c_float),
+                ("alpha", c_float)]
+
+
 class McItem(ctypes.Structure):
     """petch_mc_item: one grid of a batched marching cubes."""
     _fields_ = [("phi", c_void_p), ("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_double),
@@ -189,6 +196,7 @@ SIGNATURES = {
     "petch_smooth_adjoint": [P, P, P, c_int, c_int, c_float, c_int, P, P],
     "petch_belen_coverage": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P],
     "petch_finish_round_batch": [P, c_int, P],
+    "petch_finish_ion_batch": [P, c_int, P],
     "petch_belen_coverage_adjoint": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P,
                                      P, P, P],
     "petch_surface_rate": [P, P, P, P, P, P, P, P, P, P, POINTER(MaskGeom),

@@ -88,7 +88,8 @@ class BatchLoop:
                             outs[i] = stop.value
                             del pending[i]
                     if rounds:
-                        ops.finish_round_batch(rounds)
+                        ops.finish_ion_batch([r for r in rounds if r['kind'] == "ion"])
+                        ops.finish_round_batch([r for r in rounds if r['kind'] == "neutral"])
                         rounds.clear()
         # 3. every feature's rate; the rate reductions and BVH guards of all features in one read
         has = [lp.end_rate(step, out) for lp, out in zip(loops, outs)]

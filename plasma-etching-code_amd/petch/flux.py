@@ -118,11 +118,13 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
     is dict(m_i, m_F, m_O, cos_i, bare, ion_yield) of (F,) float32 device tensors; ion_yield =
     (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None.
 
-    rounds: a list that, when given, receives the end of each neutral round (normalise and
-    smooth both species, Belen update) as an ops.finish_round_batch item with its output tensors
-    allocated, instead of the generator launching it; the driver must run the collected items
-    (ops.finish_round_batch) before the next trace it launches and before it reads the result.
-    Used when smoothing takes at most one pass (the default flux_smooth = 1)."""
+    rounds: a list that, when given, receives the end of the faithful-ion trace (normalise and
+    smooth the yields and arrivals, the cosine ratio; kind "ion", an ops.finish_ion_batch item)
+    and of each neutral round (normalise and smooth both species, Belen update; kind "neutral",
+    an ops.finish_round_batch item) with their output tensors allocated, instead of the generator
+    launching them; the driver must run the collected items before the next trace it launches and
+    before it reads the result. Used when smoothing takes at most one pass (the default
+    flux_smooth = 1)."""
     _check_unsupported(par, flags)
     F = mesh.nf
     Lx, Ly = geo['Lx'], geo['Ly']
@@ -150,10 +152,16 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
         acc = yield ("ion", dict(bvh=mesh.bvh, box=box, src=ion_source(par), ion=ion_params(par),
                                  n=n_ion, seed=ion_seed))
         norm = A_src / n_ion
-        ion_yield = tuple(host_smooth(ops.normalize(acc[k], A, norm, -ops.INF, ops.INF))
-                          for k in range(3))
-        m_i = host_smooth(ops.normalize(acc[3], A, norm, -ops.INF, ops.INF))
-        cos_i = ops.cos_ratio(acc[3], acc[4])
+        if rounds is not None and n_smooth <= 1:
+            out = torch.empty_like(acc)
+            rounds.append(dict(kind="ion", acc=acc, areas=A, nbr=mesh.nbr, normals=mesh.normals,
+                               scale=norm, n_smooth=max(n_smooth, 0), alpha=alpha, out=out))
+            ion_yield, m_i, cos_i = (out[0], out[1], out[2]), out[3], out[4]
+        else:
+            ion_yield = tuple(host_smooth(ops.normalize(acc[k], A, norm, -ops.INF, ops.INF))
+                              for k in range(3))
+            m_i = host_smooth(ops.normalize(acc[3], A, norm, -ops.INF, ops.INF))
+            cos_i = ops.cos_ratio(acc[3], acc[4])
     else:
         acc = ops.trace_legacy(mesh.bvh, box, ion_source(par), 1.0, 0, 0, 0.34, 0.8, n_ion,
                                ion_seed)
@@ -179,7 +187,8 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
                                         n_a=n_neu, seed_a=sF, beta_b=betaO, n_b=n_neu, seed_b=sO))
         if rounds is not None and n_smooth <= 1:
             m_F, m_O, bare = (torch.empty_like(fF) for _ in range(3))
-            rounds.append(dict(flF=fF, flO=fO, areas=A, nbr=mesh.nbr, normals=mesh.normals,
+            rounds.append(dict(kind="neutral", flF=fF, flO=fO, areas=A, nbr=mesh.nbr,
+                               normals=mesh.normals,
                                scale=A_src / n_neu, hi=8.0, n_smooth=max(n_smooth, 0),
                                alpha=alpha, m_i=m_i, cos_i=cos_i, ion_yield=ion_yield, chem=chem,
                                m_F=m_F, m_O=m_O, bare=bare))

@@ -145,6 +145,30 @@ def finish_round_batch(items):
     _ctx(items[0]['flF']).call("petch_finish_round_batch", len(items), arr)
 
 
+def finish_ion_batch(items):
+    """The end of the faithful-ion trace of many features in one launch (petch_finish_ion_batch).
+    items: dicts with acc (the trace's (5, F) accumulator), areas, nbr, normals, scale,
+    n_smooth (0 or 1), alpha and out, a (5, F) float32 tensor the call fills with (sp, ie, ip,
+    m_i, cos_i): normalize(acc[k], areas, scale, -INF, INF) smoothed (one pass, the host
+    semantics) for k < 4, and cos_ratio(acc[3], acc[4])."""
+    from ._abi import IonFinishItem
+    if not items:
+        return
+    arr = (IonFinishItem * len(items))()
+    for k, it in enumerate(items):
+        acc, out = it['acc'], it['out']
+        F = int(acc.shape[1])
+        if tuple(acc.shape) != (5, F) or tuple(out.shape) != (5, F):
+            raise ValueError(f"finish_ion_batch: item {k} acc/out must be (5, F)")
+        if int(it['n_smooth']) not in (0, 1):
+            raise ValueError(f"finish_ion_batch: item {k} n_smooth {it['n_smooth']} is not 0 or 1")
+        arr[k] = IonFinishItem(ptr(acc, f32, "acc"), ptr(it['areas'], f32, "areas"),
+                               ptr(it['normals'], f32, "normals"), ptr(it['nbr'], i32, "nbr"),
+                               ptr(out, f32, "out"), F, int(it['n_smooth']), float(it['scale']),
+                               float(it['alpha']))
+    _ctx(items[0]['acc']).call("petch_finish_ion_batch", len(items), arr)
+
+
 def advect_batch(items):
     """advect for many grids in one launch per substep (petch_advect_batch). items: (phi, work,
     F, mask, mask_phi, dx, dt, nsub) tuples, dt the substep's step; each phi is updated in

@@ -49,10 +49,11 @@ class FakeLoop:
         def steps():
             acc = yield ("ion", dict(k=self.k, step=step))
             assert acc == ("ion", self.k)
+            self.rounds.append(dict(kind="ion", tag=("ion_finish", self.k)))
             for it in range(self.n_fp):
                 r = yield ("neutral", dict(k=self.k, it=it))
                 assert r == ("neutral", self.k, it)
-                self.rounds.append(("round", self.k, it))      # the round's end, deferred
+                self.rounds.append(dict(kind="neutral", tag=("round", self.k, it)))
             return dict(k=self.k, n=self.n_fp)
         if self.k == 99:
             return None                      # e.g. an empty surface: no flux job
@@ -95,7 +96,11 @@ def test_lockstep_batches_each_kind(monkeypatch):
         calls.append(("re", list(items)))
 
     def round_batch(items):
-        calls.append(("round", list(items)))
+        calls.append(("round", [it['tag'] for it in items]))
+
+    def ion_finish_batch(items):
+        if items:
+            calls.append(("ion_finish", [it['tag'] for it in items]))
 
     monkeypatch.setattr(B.ops, "trace_ion_yield_batch", ion_batch)
     monkeypatch.setattr(B.ops, "trace_neutral_rr_batch", neutral_batch)
@@ -106,6 +111,7 @@ def test_lockstep_batches_each_kind(monkeypatch):
     monkeypatch.setattr(B.ops, "advect_batch", adv_batch)
     monkeypatch.setattr(B.ops, "reinit_fsm_batch", re_batch)
     monkeypatch.setattr(B.ops, "finish_round_batch", round_batch)
+    monkeypatch.setattr(B.ops, "finish_ion_batch", ion_finish_batch)
     monkeypatch.setattr(B.ops, "decode_rate_reduction", lambda r: (1.0, 0))
     loops = [FakeLoop(0, 1), FakeLoop(1, 4), FakeLoop(99, 0), FakeLoop(2, 1)]
     outs = B.BatchLoop(loops).step(0)
@@ -116,7 +122,9 @@ def test_lockstep_batches_each_kind(monkeypatch):
     # feature, one call each
     every = lambda tag: [(tag, k) for k in (0, 1, 99, 2)]
     assert calls == [("mc", every("mc")), ("bvh", every("bvh")),
-                     ("ws", [("ws", 1)]), ("ion", [0, 1, 2]), ("neutral", [0, 1, 2]),
+                     ("ws", [("ws", 1)]), ("ion", [0, 1, 2]),
+                     ("ion_finish", [("ion_finish", 0), ("ion_finish", 1), ("ion_finish", 2)]),
+                     ("round", []), ("neutral", [0, 1, 2]),
                      ("round", [("round", 0, 0), ("round", 1, 0), ("round", 2, 0)]),
                      ("neutral", [1]), ("round", [("round", 1, 1)]),
                      ("neutral", [1]), ("round", [("round", 1, 2)]),

@@ -531,3 +531,36 @@ def test_batched_round_finish_equals_the_separate_calls():
     for it, (mF, mO, bare) in zip(items, refs):
         assert torch.equal(it['m_F'], mF) and torch.equal(it['m_O'], mO)
         assert torch.equal(it['bare'], bare)
+
+
+def test_batched_ion_finish_equals_the_separate_calls():
+    """petch_finish_ion_batch (the end of the faithful-ion trace of many features in one launch)
+    gives exactly what petch_normalize (no clip), petch_smooth (one pass, host semantics) and
+    petch_cos_ratio give feature by feature, smoothed or not, blended or not."""
+    import torch
+    from petch import ops
+    rng = np.random.default_rng(12)
+    items, refs = [], []
+    for k, (w, d, n_smooth, alpha) in enumerate(((3.0, 8.0, 1, 1.0), (2.0, 6.0, 1, 0.6),
+                                                 (4.5, 9.0, 0, 1.0))):
+        phi = ol.init_sdf(8.0, 8.0, 14.0, 0.25, w, d, 4.0, k != 1).astype(np.float32)
+        v, f = on.marching_cubes(phi, 0.25)
+        tv, tf = _t(v), _t(f)
+        cen, area, nrm = ops.mesh_attrs(tv, tf)
+        nbr = ops.face_adjacency(tf, len(v))
+        F = len(f)
+        acc = rng.exponential(1.0, (5, F)).astype(np.float32)
+        acc[3, :40] = 0.0                                     # faces no ion reached
+        acc = _t(acc)
+        scale = 0.03 + 0.01 * k
+        ref = [ops.normalize(acc[j], area, scale, -ops.INF, ops.INF) for j in range(4)]
+        if n_smooth:
+            ref = [ops.smooth(x, nbr, nrm, 1, alpha, False) for x in ref]
+        ref.append(ops.cos_ratio(acc[3], acc[4]))
+        refs.append(ref)
+        items.append(dict(acc=acc, areas=area, nbr=nbr, normals=nrm, scale=scale,
+                          n_smooth=n_smooth, alpha=alpha, out=torch.empty_like(acc)))
+    ops.finish_ion_batch(items)
+    for it, ref in zip(items, refs):
+        for j in range(5):
+            assert torch.equal(it['out'][j], ref[j]), j
