@@ -214,6 +214,18 @@ int petch_mesh_attrs(petch_ctx* ctx, const float* verts, const int* faces, int n
  * edge e of face f (e: v0v1, v1v2, v2v0), or -1 on an open boundary. */
 int petch_face_adjacency(petch_ctx* ctx, const int* faces, int nf, int nv, int* nbr);
 
+/* petch_mesh_attrs and petch_face_adjacency for many meshes at once (the features of a batched
+ * step; no reference counterpart): one launch per stage for all meshes, the outputs those two
+ * calls give (centroids, areas, unit normals, and the 3-per-face edge neighbours). */
+typedef struct petch_surface_item {
+  const float* verts;
+  const int* faces;
+  int nf, nv;
+  float *centroids, *areas, *normals;
+  int* nbr;
+} petch_surface_item;
+int petch_surface_build_batch(petch_ctx* ctx, int n, const petch_surface_item* items_host);
+
 /* _gas_normals (threed.py:1553-1568): unit normals flipped to agree with -grad(phi) (numpy
  * gradient) at the grid node nearest each centroid. */
 int petch_gas_normals(petch_ctx* ctx, const float* phi, int nx, int ny, int nz, double dx,

@@ -7,16 +7,18 @@
 //   petch_gas_normals   face normals oriented into the gas by the sign of -grad(phi) at the
 //                       centroid's nearest node (_gas_normals, threed.py:1553-1568)
 #include "common.h"
+#include "scan.h"
+
+#include <vector>
 
 namespace {
 
 constexpr int kBS = 256;
 
-__global__ void k_mesh_attrs(const float* __restrict__ verts, const int* __restrict__ faces, int nf,
-                             float* __restrict__ cen, float* __restrict__ area,
-                             float* __restrict__ nrm) {
-  int f = blockIdx.x * kBS + threadIdx.x;
-  if (f >= nf) return;
+__device__ __forceinline__ void mesh_attrs_face(const float* __restrict__ verts,
+                                                const int* __restrict__ faces,
+                                                float* __restrict__ cen, float* __restrict__ area,
+                                                float* __restrict__ nrm, int f) {
   int a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
   float ax = verts[3 * a], ay = verts[3 * a + 1], az = verts[3 * a + 2];
   float bx = verts[3 * b], by = verts[3 * b + 1], bz = verts[3 * b + 2];
@@ -40,6 +42,13 @@ __global__ void k_mesh_attrs(const float* __restrict__ verts, const int* __restr
   }
 }
 
+__global__ void k_mesh_attrs(const float* __restrict__ verts, const int* __restrict__ faces, int nf,
+                             float* __restrict__ cen, float* __restrict__ area,
+                             float* __restrict__ nrm) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  if (f < nf) mesh_attrs_face(verts, faces, cen, area, nrm, f);
+}
+
 __global__ void k_vert_count(const int* __restrict__ faces, int nf, int nv,
                              int* __restrict__ cnt) {
   int t = blockIdx.x * kBS + threadIdx.x;
@@ -48,22 +57,28 @@ __global__ void k_vert_count(const int* __restrict__ faces, int nf, int nv,
   if (v >= 0 && v < nv) atomicAdd(&cnt[v], 1);
 }
 
-__global__ void k_vert_fill(const int* __restrict__ faces, int nf, int nv,
-                            const int* __restrict__ off, int* __restrict__ cur,
-                            int* __restrict__ inc) {
-  int t = blockIdx.x * kBS + threadIdx.x;
-  if (t >= 3 * nf) return;
+__device__ __forceinline__ void vert_fill_corner(const int* __restrict__ faces, int nv,
+                                                 const int* __restrict__ off,
+                                                 int* __restrict__ cur, int* __restrict__ inc,
+                                                 int t) {
   int v = faces[t];
   if (v < 0 || v >= nv) return;
   int p = atomicAdd(&cur[v], 1);
   inc[off[v] + p] = t / 3;
 }
 
-// Neighbour across edge (a,b) of face f: the smallest other face incident to a that also uses b.
-__global__ void k_face_nbr(const int* __restrict__ faces, int nf, int nv, const int* __restrict__ off,
-                           const int* __restrict__ inc, int* __restrict__ nbr) {
+__global__ void k_vert_fill(const int* __restrict__ faces, int nf, int nv,
+                            const int* __restrict__ off, int* __restrict__ cur,
+                            int* __restrict__ inc) {
   int t = blockIdx.x * kBS + threadIdx.x;
-  if (t >= 3 * nf) return;
+  if (t < 3 * nf) vert_fill_corner(faces, nv, off, cur, inc, t);
+}
+
+// Neighbour across edge (a,b) of face f: the smallest other face incident to a that also uses b.
+__device__ __forceinline__ void face_nbr_edge(const int* __restrict__ faces, int nv,
+                                              const int* __restrict__ off,
+                                              const int* __restrict__ inc, int* __restrict__ nbr,
+                                              int t) {
   int f = t / 3, e = t - 3 * f;
   int a = faces[3 * f + e], b = faces[3 * f + (e + 1) % 3];
   int best = -1;
@@ -76,6 +91,83 @@ __global__ void k_face_nbr(const int* __restrict__ faces, int nf, int nv, const 
     }
   }
   nbr[t] = best;
+}
+
+__global__ void k_face_nbr(const int* __restrict__ faces, int nf, int nv, const int* __restrict__ off,
+                           const int* __restrict__ inc, int* __restrict__ nbr) {
+  int t = blockIdx.x * kBS + threadIdx.x;
+  if (t < 3 * nf) face_nbr_edge(faces, nv, off, inc, nbr, t);
+}
+
+// The attributes and adjacency of many meshes (the features of a batched step), one launch per
+// stage for all: item t owns the workgroups [blk0[t], blk0[t+1]) of the launches over its 3 nf
+// face corners, and one workgroup per item scans its vertex counts. Every stage runs the device
+// code of the single calls, so the outputs are theirs (the incidence lists fill in any order;
+// the neighbour chosen is the smallest face id, whatever that order).
+struct SurfItem {
+  const float* verts;
+  const int* faces;
+  int nf, nv;
+  float *cen, *area, *nrm;
+  int* nbr;
+  int *cnt, *cur, *off, *inc;   // workspace: nv + 1, nv + 1, nv + 1, 3 nf
+  long long blk0;
+};
+
+__device__ __forceinline__ int surf_item_of(const SurfItem* __restrict__ it, int n) {
+  int lo = 0, hi = n - 1;
+  const long long b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (it[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// one workgroup per item: clear the vertex counters
+__global__ void k_surf_clear(const SurfItem* __restrict__ items) {
+  const SurfItem& it = items[blockIdx.x];
+  for (int v = threadIdx.x; v <= it.nv; v += kBS) {
+    it.cnt[v] = 0;
+    it.cur[v] = 0;
+  }
+}
+
+// corner t: its vertex's count, and (first corner of a face) the face's attributes
+__global__ void k_surf_count(const SurfItem* __restrict__ items, int n) {
+  const SurfItem& it = items[surf_item_of(items, n)];
+  const int t = (int)(blockIdx.x - it.blk0) * kBS + threadIdx.x;
+  if (t >= 3 * it.nf) return;
+  const int v = it.faces[t];
+  if (v >= 0 && v < it.nv) atomicAdd(&it.cnt[v], 1);
+  if (t % 3 == 0) mesh_attrs_face(it.verts, it.faces, it.cen, it.area, it.nrm, t / 3);
+}
+
+// one workgroup per item: exclusive scan of its nv + 1 counts
+__global__ void k_surf_scan(const SurfItem* __restrict__ items) {
+  const SurfItem& it = items[blockIdx.x];
+  const int m = it.nv + 1;
+  int carry = 0;
+  for (int c = 0; c < m; c += kBS) {
+    const int i = c + threadIdx.x;
+    const int v = i < m ? it.cnt[i] : 0;
+    int tot;
+    const int ex = petch::block_excl_scan<kBS>(v, tot);
+    if (i < m) it.off[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ void k_surf_fill(const SurfItem* __restrict__ items, int n) {
+  const SurfItem& it = items[surf_item_of(items, n)];
+  const int t = (int)(blockIdx.x - it.blk0) * kBS + threadIdx.x;
+  if (t < 3 * it.nf) vert_fill_corner(it.faces, it.nv, it.off, it.cur, it.inc, t);
+}
+
+__global__ void k_surf_nbr(const SurfItem* __restrict__ items, int n) {
+  const SurfItem& it = items[surf_item_of(items, n)];
+  const int t = (int)(blockIdx.x - it.blk0) * kBS + threadIdx.x;
+  if (t < 3 * it.nf) face_nbr_edge(it.faces, it.nv, it.off, it.inc, it.nbr, t);
 }
 
 __global__ void k_gas_normals(const float* __restrict__ phi, int nx, int ny, int nz, float dx,
@@ -150,6 +242,55 @@ extern "C" int petch_face_adjacency(petch_ctx* ctx, const int* faces, int nf, in
   PETCH_LAUNCH_CHECK(ctx);
   hipLaunchKernelGGL(k_face_nbr, dim3(g3), dim3(kBS), 0, ctx->stream, faces, nf, nv, off, inc,
                      nbr);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_surface_build_batch(petch_ctx* ctx, int n, const petch_surface_item* items) {
+  PETCH_ARG(ctx, ctx && (n == 0 || items), "surface_build_batch: null pointer");
+  PETCH_ARG(ctx, n >= 0, "surface_build_batch: n < 0");
+  std::vector<SurfItem> t;
+  t.reserve((size_t)n);
+  long long ws = 0, blocks = 0;
+  for (int k = 0; k < n; ++k) {
+    const petch_surface_item& q = items[k];
+    PETCH_ARG(ctx, q.nf >= 0 && q.nv >= 0, "surface_build_batch: item %d has a negative size", k);
+    if (q.nf == 0) continue;
+    PETCH_ARG(ctx, q.verts && q.faces && q.centroids && q.areas && q.normals && q.nbr,
+              "surface_build_batch: item %d has a null pointer", k);
+    ws += 3LL * (q.nv + 1) + 3LL * q.nf;
+  }
+  int* w = ws ? (int*)petch_scratch(ctx, SCR_ADJ, (size_t)ws * sizeof(int)) : nullptr;
+  if (ws && !w) return PETCH_E_HIP;
+  long long off = 0;
+  for (int k = 0; k < n; ++k) {
+    const petch_surface_item& q = items[k];
+    if (q.nf == 0) continue;
+    SurfItem r;
+    r.verts = q.verts; r.faces = q.faces; r.nf = q.nf; r.nv = q.nv;
+    r.cen = q.centroids; r.area = q.areas; r.nrm = q.normals; r.nbr = q.nbr;
+    r.cnt = w + off; off += q.nv + 1;
+    r.cur = w + off; off += q.nv + 1;
+    r.off = w + off; off += q.nv + 1;
+    r.inc = w + off; off += 3LL * q.nf;
+    r.blk0 = blocks;
+    blocks += petch_blocks(3LL * q.nf, kBS);
+    t.push_back(r);
+  }
+  if (t.empty()) return PETCH_OK;
+  PETCH_ARG(ctx, blocks <= 0x7fffffffLL, "surface_build_batch: too many faces for one launch");
+  const SurfItem* d = (const SurfItem*)petch_upload_table(ctx, t.data(), t.size() * sizeof(SurfItem));
+  if (!d) return PETCH_E_HIP;
+  const int m = (int)t.size();
+  hipLaunchKernelGGL(k_surf_clear, dim3((unsigned)m), dim3(kBS), 0, ctx->stream, d);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_surf_count, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d, m);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_surf_scan, dim3((unsigned)m), dim3(kBS), 0, ctx->stream, d);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_surf_fill, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d, m);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_surf_nbr, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d, m);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

@@ -116,6 +116,13 @@ class IonFinishItem(ctypes.Structure):
                 ("alpha", c_float)]
 
 
+class SurfaceItem(ctypes.Structure):
+    """petch_surface_item: one mesh of a batched attribute + adjacency build."""
+    _fields_ = [("verts", c_void_p), ("faces", c_void_p), ("nf", c_int), ("nv", c_int),
+                ("centroids", c_void_p), ("areas", c_void_p), ("normals", c_void_p),
+                ("nbr", c_void_p)]
+
+
 class McItem(ctypes.Structure):
     """petch_mc_item: one grid of a batched marching cubes."""
     _fields_ = [("phi", c_void_p), ("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_double),
@@ -197,6 +204,7 @@ SIGNATURES = {
     "petch_belen_coverage": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P],
     "petch_finish_round_batch": [P, c_int, P],
     "petch_finish_ion_batch": [P, c_int, P],
+    "petch_surface_build_batch": [P, c_int, P],
     "petch_belen_coverage_adjoint": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P,
                                      P, P, P],
     "petch_surface_rate": [P, P, P, P, P, P, P, P, P, P, POINTER(MaskGeom),

@@ -38,7 +38,12 @@ class BatchLoop:
         #    refits of all trees in one launch); the warm-start guards of all features in one read
         with self.timer.region("mesh"):
             mcs = ops.marching_cubes_batch([lp.mesh_item(step) for lp in loops])
-        finite = [lp.begin_mesh(step, mc, bvh=False) for lp, mc in zip(loops, mcs)]
+        surf = []
+        for lp, mc in zip(loops, mcs):
+            lp.stage_mesh(step, mc, surf)
+        with self.timer.region("mesh"):
+            ops.surface_build_batch(surf)
+        finite = [lp.begin_mesh(step, bvh=False, staged=True) for lp in loops]
         with self.timer.region("mesh"):
             ops.bvh_build_batch([lp._mesh.bvh_item() for lp in loops])
         need = [i for i, f in enumerate(finite) if f is not None]

@@ -38,15 +38,24 @@ class SurfaceMesh:
     def build(self, phi, dx):
         return self.build_from(*ops.marching_cubes(phi, dx, self.bufs))
 
-    def build_from(self, v, f, bvh=True):
+    def build_from(self, v, f, bvh=True, surf=None):
         """Everything after marching cubes, from its (verts, faces) (the batched loop meshes
         all features' grids with one host read, then builds each feature's surface). bvh=False
         leaves the BVH to the caller (the batched loop builds all features' trees together with
-        ops.bvh_build_batch over bvh_item())."""
+        ops.bvh_build_batch over bvh_item()); with a list `surf` the face attributes and
+        adjacency are only allocated and their item appended, for ops.surface_build_batch."""
         self.verts, self.faces = v, f
         self.nv, self.nf = int(v.shape[0]), int(f.shape[0])
-        self.centroids, self.areas, self.normals = ops.mesh_attrs(v, f)
-        self.nbr = ops.face_adjacency(f, self.nv)
+        if surf is None:
+            self.centroids, self.areas, self.normals = ops.mesh_attrs(v, f)
+            self.nbr = ops.face_adjacency(f, self.nv)
+        else:
+            nf, dev = self.nf, v.device
+            self.centroids = torch.empty((nf, 3), dtype=torch.float32, device=dev)
+            self.areas = torch.empty((nf,), dtype=torch.float32, device=dev)
+            self.normals = torch.empty((nf, 3), dtype=torch.float32, device=dev)
+            self.nbr = torch.empty((nf, 3), dtype=torch.int32, device=dev)
+            surf.append((v, f, self.centroids, self.areas, self.normals, self.nbr))
         if self.quality is not None or self.rays_per_step is not None:
             q = self.quality
             if q is None:

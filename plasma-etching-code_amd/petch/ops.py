@@ -323,6 +323,26 @@ def face_adjacency(faces, nv):
     return nbr
 
 
+def surface_build_batch(items):
+    """mesh_attrs and face_adjacency of many meshes in one launch per stage
+    (petch_surface_build_batch). items: (verts, faces, centroids, areas, normals, nbr) with the
+    outputs allocated as those two calls return them; they are filled in place."""
+    from ._abi import SurfaceItem
+    items = [it for it in items if int(it[1].shape[0])]
+    if not items:
+        return
+    arr = (SurfaceItem * len(items))()
+    for k, (v, f, cen, area, nrm, nbr) in enumerate(items):
+        nf, nv = int(f.shape[0]), int(v.shape[0])
+        if tuple(cen.shape) != (nf, 3) or tuple(area.shape) != (nf,) or \
+                tuple(nrm.shape) != (nf, 3) or tuple(nbr.shape) != (nf, 3):
+            raise ValueError(f"surface_build_batch: item {k} outputs do not match {nf} faces")
+        arr[k] = SurfaceItem(ptr(v, f32, "verts"), ptr(f, i32, "faces"), nf, nv,
+                                  ptr(cen, f32, "centroids"), ptr(area, f32, "areas"),
+                                  ptr(nrm, f32, "normals"), ptr(nbr, i32, "nbr"))
+    _ctx(items[0][0]).call("petch_surface_build_batch", len(items), arr)
+
+
 def gas_normals(phi, dx, centroids, normals):
     """Unit normals oriented into the gas (threed.py:1553-1568)."""
     nx, ny, nz = _shape3(phi)

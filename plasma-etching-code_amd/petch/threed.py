@@ -384,16 +384,28 @@ class EtchLoop:
         """(phi, dx, buffers) of this step's marching cubes for ops.marching_cubes_batch."""
         return (self.phi, self.dx, self.meshes[step % 2].bufs)
 
-    def begin_mesh(self, step, mc=None, bvh=True):
+    def stage_mesh(self, step, mc, surf):
+        """The batched loop's first half of begin_mesh: this step's surface from its
+        marching-cubes (verts, faces), its face attributes and adjacency left to one
+        ops.surface_build_batch over `surf` (the list its item is appended to)."""
+        tm = time.time()
+        with self.timer.region("mesh"):
+            self.meshes[step % 2].build_from(*mc, bvh=False, surf=surf)
+        self.timings['mesh'] += time.time() - tm
+
+    def begin_mesh(self, step, mc=None, bvh=True, staged=False):
         """Mesh + BVH of the step. Returns the device flag 'all centroids finite' that gates the
         warm start (threed.py:1699), or None when the step needs no warm start. mc: this step's
         marching-cubes (verts, faces) when the caller already extracted them (the batched loop
         meshes all features with one host read); bvh=False leaves the BVH to the caller (the
-        batched loop builds all features' trees together)."""
+        batched loop builds all features' trees together); staged: the surface was built by
+        stage_mesh and ops.surface_build_batch."""
         T, dx = self.timer, self.dx
         tm = time.time()
         with T.region("mesh"):
-            if mc is None:
+            if staged:
+                mesh = self.meshes[step % 2]
+            elif mc is None:
                 mesh = self.meshes[step % 2].build(self.phi, dx)
             else:
                 mesh = self.meshes[step % 2].build_from(*mc, bvh=bvh)

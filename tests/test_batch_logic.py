@@ -16,8 +16,13 @@ class FakeLoop:
     def mesh_item(self, step):
         return ("mc", self.k)
 
-    def begin_mesh(self, step, mc=None, bvh=True):
-        self.mc, self.bvh_here = mc, bvh
+    def stage_mesh(self, step, mc, surf):
+        self.mc = mc
+        surf.append(("surf", self.k))
+
+    def begin_mesh(self, step, mc=None, bvh=True, staged=False):
+        assert staged and mc is None
+        self.bvh_here = bvh
         self._mesh = types.SimpleNamespace(bvh_item=lambda: ("bvh", self.k),
                                            bvh=types.SimpleNamespace(status_into=lambda t: None))
         return None
@@ -108,6 +113,8 @@ def test_lockstep_batches_each_kind(monkeypatch):
     monkeypatch.setattr(B.ops, "nearest_face_value_batch", ws_batch)
     monkeypatch.setattr(B.ops, "extend_batch", ext_batch)
     monkeypatch.setattr(B.ops, "bvh_build_batch", bvh_batch)
+    monkeypatch.setattr(B.ops, "surface_build_batch",
+                        lambda items: calls.append(("surf", list(items))))
     monkeypatch.setattr(B.ops, "advect_batch", adv_batch)
     monkeypatch.setattr(B.ops, "reinit_fsm_batch", re_batch)
     monkeypatch.setattr(B.ops, "finish_round_batch", round_batch)
@@ -115,13 +122,14 @@ def test_lockstep_batches_each_kind(monkeypatch):
     monkeypatch.setattr(B.ops, "decode_rate_reduction", lambda r: (1.0, 0))
     loops = [FakeLoop(0, 1), FakeLoop(1, 4), FakeLoop(99, 0), FakeLoop(2, 1)]
     outs = B.BatchLoop(loops).step(0)
-    # one marching-cubes pass and one BVH build call for all features, one warm-start launch for
+    # one marching-cubes pass, one face attribute/adjacency build and one BVH build call for
+    # all features, one warm-start launch for
     # the features that have a lookup, one launch per trace kind and fixed-point round, each
     # neutral round ended for all its features in one call before the next trace, one extension
     # pass for every feature with a surface rate, then the advection and the reinit of every
     # feature, one call each
     every = lambda tag: [(tag, k) for k in (0, 1, 99, 2)]
-    assert calls == [("mc", every("mc")), ("bvh", every("bvh")),
+    assert calls == [("mc", every("mc")), ("surf", every("surf")), ("bvh", every("bvh")),
                      ("ws", [("ws", 1)]), ("ion", [0, 1, 2]),
                      ("ion_finish", [("ion_finish", 0), ("ion_finish", 1), ("ion_finish", 2)]),
                      ("round", []), ("neutral", [0, 1, 2]),

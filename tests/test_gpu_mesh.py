@@ -104,3 +104,32 @@ def test_batched_marching_cubes_equals_single_calls():
         np.testing.assert_array_equal(v.cpu().numpy(), rv)
         np.testing.assert_array_equal(f.cpu().numpy(), rf)
     assert len(singles[2][1]) == 0 and len(outs[2][1]) == 0
+
+
+def test_batched_surface_build_equals_single_calls():
+    """surface_build_batch over meshes of different sizes (closed, open and empty) fills each
+    mesh's centroids, areas, normals and neighbour table exactly as mesh_attrs and
+    face_adjacency do, and a second batch reuses the workspace."""
+    import torch
+    from petch import ops
+    d = _dev()
+    meshes = []
+    for name, phi, dx in FIELDS:
+        v, f = ops.marching_cubes(torch.from_numpy(np.ascontiguousarray(phi, np.float32)).to(d), dx)
+        meshes.append((name, v, f))
+    print("surface batch", [(n, int(f.shape[0])) for n, _, f in meshes], flush=True)
+    for order in (meshes, meshes[::-1][:4]):
+        items, singles = [], []
+        for _, v, f in order:
+            nf = int(f.shape[0])
+            singles.append(ops.mesh_attrs(v, f) + (ops.face_adjacency(f, v.shape[0]),))
+            outs = (torch.full((nf, 3), -7.0, device=d), torch.full((nf,), -7.0, device=d),
+                    torch.full((nf, 3), -7.0, device=d),
+                    torch.full((nf, 3), -7, dtype=torch.int32, device=d))
+            items.append((v, f) + outs)
+        ops.surface_build_batch(items)
+        torch.cuda.synchronize()
+        for (name, _, _), it, ref in zip(order, items, singles):
+            for got, want in zip(it[2:], ref):
+                assert torch.equal(got, want), name
+    assert any(int(f.shape[0]) == 0 for _, _, f in meshes)
