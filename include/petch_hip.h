@@ -216,13 +216,16 @@ int petch_face_adjacency(petch_ctx* ctx, const int* faces, int nf, int nv, int* 
 
 /* petch_mesh_attrs and petch_face_adjacency for many meshes at once (the features of a batched
  * step; no reference counterpart): one launch per stage for all meshes, the outputs those two
- * calls give (centroids, areas, unit normals, and the 3-per-face edge neighbours). */
+ * calls give (centroids, areas, unit normals, and the 3-per-face edge neighbours). finite (may
+ * be NULL) receives 1 if every centroid is finite, else 0: the warm-start guard of
+ * threed.py:1699, read for all features at once. */
 typedef struct petch_surface_item {
   const float* verts;
   const int* faces;
   int nf, nv;
   float *centroids, *areas, *normals;
   int* nbr;
+  int* finite;
 } petch_surface_item;
 int petch_surface_build_batch(petch_ctx* ctx, int n, const petch_surface_item* items_host);
 
@@ -471,6 +474,23 @@ int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float* m_F, const
                        const float* iy_ip, const float* centroids, const uint8_t* is_mask,
                        const petch_mask_geom* mg, const petch_chem_params* cp, int F, float* V,
                        unsigned* red);
+
+/* petch_surface_rate for many surfaces in one launch (the features of a batched step; no
+ * reference counterpart): each item's V and reduction pair red[0..1] as that call gives them.
+ * With a BVH, the item's traversal guard flag (what petch_bvh_status reads) is copied to
+ * *bvh_err in the same pass, so one host read returns every feature's rate reduction and guard. */
+typedef struct petch_rate_item {
+  const float *m_i, *m_F, *m_O, *cos_i, *iy_sp, *iy_ie, *iy_ip, *centroids;
+  const uint8_t* is_mask;
+  petch_mask_geom mg;
+  petch_chem_params chem;
+  int F;
+  float* V;
+  unsigned* red;
+  const petch_bvh* bvh;
+  int* bvh_err;
+} petch_rate_item;
+int petch_surface_rate_batch(petch_ctx* ctx, int n, const petch_rate_item* items_host);
 /* Closed-form adjoint of petch_surface_rate for gV = d loss / dV: gm_F, gm_O = d loss / d m_F,
  * d m_O; with ion yields (belen) g_sp, g_ie, g_ip = d loss / d iy_sp, iy_ie, iy_ip, else
  * gm_i = d loss / d m_i. Mask faces and non-finite V pass no gradient; cos_i is held constant.

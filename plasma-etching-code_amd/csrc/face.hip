@@ -343,16 +343,18 @@ __device__ __forceinline__ bool face_masked(const uint8_t* __restrict__ is_mask,
   return in_band && !opening;
 }
 
-__global__ void k_surface_rate(const float* __restrict__ m_i, const float* __restrict__ m_F,
-                               const float* __restrict__ m_O, const float* __restrict__ cos_i,
-                               const float* __restrict__ iy_sp, const float* __restrict__ iy_ie,
-                               const float* __restrict__ iy_ip,
-                               const float* __restrict__ centroids,
-                               const uint8_t* __restrict__ is_mask, MaskGeom mg, Chem c, int F,
-                               float* __restrict__ V, unsigned* __restrict__ red) {
+__device__ __forceinline__ void rate_face(const float* __restrict__ m_i,
+                                          const float* __restrict__ m_F,
+                                          const float* __restrict__ m_O,
+                                          const float* __restrict__ cos_i,
+                                          const float* __restrict__ iy_sp,
+                                          const float* __restrict__ iy_ie,
+                                          const float* __restrict__ iy_ip,
+                                          const float* __restrict__ centroids,
+                                          const uint8_t* __restrict__ is_mask, const MaskGeom& mg,
+                                          const Chem& c, int f, float* __restrict__ V,
+                                          unsigned& vbits, unsigned& bad) {
 #pragma clang fp contract(off)
-  int f = blockIdx.x * kBS + threadIdx.x;
-  if (f >= F) return;
   const double eps = 1e-9;
   double f_ie, f_sp;
   angular((double)cos_i[f], c, f_ie, f_sp);
@@ -391,11 +393,81 @@ __global__ void k_surface_rate(const float* __restrict__ m_i, const float* __res
   float vf = (float)v;
   if (!(fabsf(vf) <= 3.402823466e38f)) {  // NaN or inf
     vf = 0.f;
-    atomicAdd(red + 1, 1u);
+    bad = 1u;
   }
   V[f] = vf;
   // V >= 0 here, so the float bit pattern orders like an unsigned integer
-  atomicMax(red, __float_as_uint(vf > 0.f ? vf : 0.f));
+  vbits = __float_as_uint(vf > 0.f ? vf : 0.f);
+}
+
+// red[0] = max(red[0], the wavefront's largest V bits), red[1] += its non-finite count: one
+// atomic each per wavefront (all its lanes belong to one surface; lanes without a face pass 0).
+// Max and count do not depend on the order, so the pair is the same as with per-face atomics.
+__device__ __forceinline__ void rate_reduce(unsigned* __restrict__ red, unsigned vbits,
+                                            unsigned bad) {
+  unsigned m = vbits;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, d, 64));
+  const unsigned long long b = __ballot(bad != 0u);
+  if ((threadIdx.x & 63) == 0) {
+    if (m) atomicMax(red, m);
+    if (b) atomicAdd(red + 1, (unsigned)__popcll(b));
+  }
+}
+
+__global__ void k_surface_rate(const float* __restrict__ m_i, const float* __restrict__ m_F,
+                               const float* __restrict__ m_O, const float* __restrict__ cos_i,
+                               const float* __restrict__ iy_sp, const float* __restrict__ iy_ie,
+                               const float* __restrict__ iy_ip,
+                               const float* __restrict__ centroids,
+                               const uint8_t* __restrict__ is_mask, MaskGeom mg, Chem c, int F,
+                               float* __restrict__ V, unsigned* __restrict__ red) {
+  int f = blockIdx.x * kBS + threadIdx.x;
+  unsigned vbits = 0u, bad = 0u;
+  if (f < F)
+    rate_face(m_i, m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask, mg, c, f, V, vbits,
+              bad);
+  rate_reduce(red, vbits, bad);
+}
+
+// The surface rates of many surfaces in one launch: item t owns the workgroups
+// [blk0, blk0 + ceil(F / kBS)) (none for an empty surface), and its reduction pair is cleared,
+// and its BVH guard flag copied, by k_rate_clear before.
+struct RateItem {
+  const float *m_i, *m_F, *m_O, *cos_i, *iy_sp, *iy_ie, *iy_ip, *centroids;
+  const uint8_t* is_mask;
+  MaskGeom mg;
+  Chem c;
+  int F;
+  float* V;
+  unsigned* red;
+  const int* err_src;
+  int* err_dst;
+  long long blk0;
+};
+
+__global__ void k_rate_clear(const RateItem* __restrict__ it, int n) {
+  const int t = blockIdx.x * kBS + threadIdx.x;
+  if (t >= n) return;
+  it[t].red[0] = 0u;
+  it[t].red[1] = 0u;
+  if (it[t].err_dst) *it[t].err_dst = it[t].err_src ? *it[t].err_src : 0;
+}
+
+__global__ void k_rate_many(const RateItem* __restrict__ items, int n) {
+  int lo = 0, hi = n - 1;
+  const long long b = blockIdx.x;
+  while (lo < hi) {                // the last item starting at or before block b owns it
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  const RateItem& it = items[lo];
+  const int f = (int)(b - it.blk0) * kBS + threadIdx.x;
+  unsigned vbits = 0u, bad = 0u;
+  if (f < it.F)
+    rate_face(it.m_i, it.m_F, it.m_O, it.cos_i, it.iy_sp, it.iy_ie, it.iy_ip, it.centroids,
+              it.is_mask, it.mg, it.c, f, it.V, vbits, bad);
+  rate_reduce(it.red, vbits, bad);
 }
 
 // Closed-form adjoint of k_surface_rate for gV = d loss / dV (zero on mask faces and where V
@@ -853,6 +925,49 @@ extern "C" int petch_surface_rate(petch_ctx* ctx, const float* m_i, const float*
                      m_F, m_O, cos_i, iy_sp, iy_ie, iy_ip, centroids, is_mask, g, make_chem(cp), F,
                      V, red);
   PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
+}
+
+extern "C" int petch_surface_rate_batch(petch_ctx* ctx, int n, const petch_rate_item* items) {
+  PETCH_ARG(ctx, ctx && n >= 0 && (n == 0 || items), "surface_rate_batch: null pointer or n < 0");
+  if (n == 0) return PETCH_OK;
+  std::vector<RateItem> t((size_t)n);
+  long long blocks = 0;
+  for (int k = 0; k < n; ++k) {
+    const petch_rate_item& a = items[k];
+    PETCH_ARG(ctx, a.F >= 0 && a.red, "surface_rate_batch: item %d: F < 0 or no reduction pair", k);
+    PETCH_ARG(ctx, a.F == 0 || (a.m_i && a.m_F && a.m_O && a.cos_i && a.V),
+              "surface_rate_batch: item %d has a null pointer", k);
+    PETCH_ARG(ctx, a.F == 0 || a.centroids || a.is_mask,
+              "surface_rate_batch: item %d needs centroids or an explicit mask", k);
+    PETCH_ARG(ctx, !a.iy_ie || (a.iy_sp && a.iy_ip),
+              "surface_rate_batch: item %d: ion yields need all three channels", k);
+    const petch_mask_geom& mg = a.mg;
+    RateItem& r = t[k];
+    r.m_i = a.m_i; r.m_F = a.m_F; r.m_O = a.m_O; r.cos_i = a.cos_i;
+    r.iy_sp = a.iy_sp; r.iy_ie = a.iy_ie; r.iy_ip = a.iy_ip; r.centroids = a.centroids;
+    r.is_mask = a.is_mask;
+    r.mg = MaskGeom{mg.sub_top, mg.sub_top + mg.mask_th, mg.Lx / 2.0, mg.Ly / 2.0,
+                    mg.trench_width / 2.0, (mg.trench_width / 2.0) * (mg.trench_width / 2.0),
+                    mg.hole};
+    r.c = make_chem(&a.chem);
+    r.F = a.F;
+    r.V = a.V;
+    r.red = a.red;
+    r.err_src = a.bvh ? petch_bvh_view(a.bvh).err : nullptr;
+    r.err_dst = a.bvh_err;
+    r.blk0 = blocks;
+    blocks += petch_blocks(a.F, kBS);
+  }
+  PETCH_ARG(ctx, blocks <= 0x7fffffffLL, "surface_rate_batch: too many faces for one launch");
+  const RateItem* d = (const RateItem*)petch_upload_table(ctx, t.data(), t.size() * sizeof(RateItem));
+  if (!d) return PETCH_E_HIP;
+  hipLaunchKernelGGL(k_rate_clear, dim3(petch_blocks(n, kBS)), dim3(kBS), 0, ctx->stream, d, n);
+  PETCH_LAUNCH_CHECK(ctx);
+  if (blocks > 0) {
+    hipLaunchKernelGGL(k_rate_many, dim3((unsigned)blocks), dim3(kBS), 0, ctx->stream, d, n);
+    PETCH_LAUNCH_CHECK(ctx);
+  }
   return PETCH_OK;
 }
 

@@ -168,10 +168,12 @@ __device__ __forceinline__ void load_cell(const float* __restrict__ phi, const G
   for (int c = 0; c < 8; ++c) f[c] = phi[n + corner_off(g, c)];
 }
 
-__global__ void __launch_bounds__(kMcBS) k_mc_count(const float* __restrict__ phi, Grid g,
-                                                    long long N, int* __restrict__ bv,
-                                                    int* __restrict__ bt) {
-  long long n = (long long)blockIdx.x * kMcBS + threadIdx.x;
+// The three passes over the nodes of block `blk` of one grid (blk = blockIdx.x for a single
+// grid; the batched launches pass the block's index within its grid).
+__device__ __forceinline__ void mc_count_block(const float* __restrict__ phi, const Grid& g,
+                                               long long N, long long blk, int* __restrict__ bv,
+                                               int* __restrict__ bt) {
+  long long n = blk * kMcBS + threadIdx.x;
   int vc = 0, tc = 0;
   if (n < N) {
     int i, j, k;
@@ -186,26 +188,32 @@ __global__ void __launch_bounds__(kMcBS) k_mc_count(const float* __restrict__ ph
   int sv = petch::block_sum<kMcBS>(vc);
   int st = petch::block_sum<kMcBS>(tc);
   if (threadIdx.x == 0) {
-    bv[blockIdx.x] = sv;
-    bt[blockIdx.x] = st;
+    bv[blk] = sv;
+    bt[blk] = st;
   }
 }
 
-__global__ void __launch_bounds__(kMcBS) k_mc_verts(const float* __restrict__ phi, Grid g,
-                                                    long long N, double dx,
-                                                    const int* __restrict__ bv,
-                                                    int* __restrict__ vbase,
-                                                    uint8_t* __restrict__ vflg,
-                                                    float* __restrict__ verts, int cap_v) {
+__global__ void __launch_bounds__(kMcBS) k_mc_count(const float* __restrict__ phi, Grid g,
+                                                    long long N, int* __restrict__ bv,
+                                                    int* __restrict__ bt) {
+  mc_count_block(phi, g, N, blockIdx.x, bv, bt);
+}
+
+__device__ __forceinline__ void mc_verts_block(const float* __restrict__ phi, const Grid& g,
+                                               long long N, long long blk, double dx,
+                                               const int* __restrict__ bv,
+                                               int* __restrict__ vbase,
+                                               uint8_t* __restrict__ vflg,
+                                               float* __restrict__ verts, int cap_v) {
 #pragma clang fp contract(off)
-  long long n = (long long)blockIdx.x * kMcBS + threadIdx.x;
+  long long n = blk * kMcBS + threadIdx.x;
   int fl = 0, i = 0, j = 0, k = 0;
   if (n < N) {
     node_ijk(g, n, i, j, k);
     fl = vertex_flags(phi, g, n, i, j, k);
   }
   int tot;
-  int vb = bv[blockIdx.x] + petch::block_excl_scan<kMcBS>(__popc(fl), tot);
+  int vb = bv[blk] + petch::block_excl_scan<kMcBS>(__popc(fl), tot);
   if (n >= N) return;
   vbase[n] = vb;
   vflg[n] = (uint8_t)fl;
@@ -229,6 +237,15 @@ __global__ void __launch_bounds__(kMcBS) k_mc_verts(const float* __restrict__ ph
   }
 }
 
+__global__ void __launch_bounds__(kMcBS) k_mc_verts(const float* __restrict__ phi, Grid g,
+                                                    long long N, double dx,
+                                                    const int* __restrict__ bv,
+                                                    int* __restrict__ vbase,
+                                                    uint8_t* __restrict__ vflg,
+                                                    float* __restrict__ verts, int cap_v) {
+  mc_verts_block(phi, g, N, blockIdx.x, dx, bv, vbase, vflg, verts, cap_v);
+}
+
 // The two cube faces an edge lies on, as bits (2*axis + side) of a 6-bit mask.
 __device__ __forceinline__ int edge_faces(int e) {
   int a = e >> 2, lo = edge_base(e), m = 0;
@@ -245,12 +262,13 @@ __device__ __forceinline__ int edge_vertex(const Grid& g, long long n, int ce,
   return vbase[base] + __popc(vflg[base] & ((1 << (ce >> 2)) - 1));
 }
 
-__global__ void __launch_bounds__(kMcBS) k_mc_faces(const float* __restrict__ phi, Grid g,
-                                                    long long N, const int* __restrict__ bt,
-                                                    const int* __restrict__ vbase,
-                                                    const uint8_t* __restrict__ vflg,
-                                                    int* __restrict__ faces, int cap_f) {
-  long long n = (long long)blockIdx.x * kMcBS + threadIdx.x;
+__device__ __forceinline__ void mc_faces_block(const float* __restrict__ phi, const Grid& g,
+                                               long long N, long long blk,
+                                               const int* __restrict__ bt,
+                                               const int* __restrict__ vbase,
+                                               const uint8_t* __restrict__ vflg,
+                                               int* __restrict__ faces, int cap_f) {
+  long long n = blk * kMcBS + threadIdx.x;
   float f[8];
   int tc = 0;
   if (n < N) {
@@ -262,7 +280,7 @@ __global__ void __launch_bounds__(kMcBS) k_mc_faces(const float* __restrict__ ph
     }
   }
   int tot;
-  int tb = bt[blockIdx.x] + petch::block_excl_scan<kMcBS>(tc, tot);
+  int tb = bt[blk] + petch::block_excl_scan<kMcBS>(tc, tot);
   if (tc == 0) return;
   int ne;
   unsigned long long nxt = cell_links(f, ne);
@@ -310,6 +328,80 @@ __global__ void __launch_bounds__(kMcBS) k_mc_faces(const float* __restrict__ ph
       --len;
     }
   }
+}
+
+__global__ void __launch_bounds__(kMcBS) k_mc_faces(const float* __restrict__ phi, Grid g,
+                                                    long long N, const int* __restrict__ bt,
+                                                    const int* __restrict__ vbase,
+                                                    const uint8_t* __restrict__ vflg,
+                                                    int* __restrict__ faces, int cap_f) {
+  mc_faces_block(phi, g, N, blockIdx.x, bt, vbase, vflg, faces, cap_f);
+}
+
+// The passes of many grids (the features of a batched step), one launch each: grid t owns the
+// workgroups [blk0, blk0 + nb) of the node launches; one workgroup per grid scans its block
+// sums. Each grid's blocks see exactly the nodes, partial sums and scans of the single call, so
+// the vertices and faces are bit-identical to its output.
+struct McGrid {
+  const float* phi;
+  Grid g;
+  long long N, nb, blk0;
+  double dx;
+  int *bv, *bt, *totals;
+  int* vbase;
+  uint8_t* vflg;
+  float* verts;
+  int* faces;
+  int cap_v, cap_f;
+};
+
+__device__ __forceinline__ int mc_grid_of(const McGrid* __restrict__ t, int n) {
+  int lo = 0, hi = n - 1;
+  const long long b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kMcBS) k_mc_count_many(const McGrid* __restrict__ t, int n) {
+  const McGrid& m = t[mc_grid_of(t, n)];
+  mc_count_block(m.phi, m.g, m.N, blockIdx.x - m.blk0, m.bv, m.bt);
+}
+
+__global__ void __launch_bounds__(1024) k_mc_scan_many(const McGrid* __restrict__ t) {
+  const McGrid& m = t[blockIdx.x];
+  int ca = 0, cb = 0;
+  for (long long c0 = 0; c0 < m.nb; c0 += 1024) {
+    const long long i = c0 + threadIdx.x;
+    const int va = (i < m.nb) ? m.bv[i] : 0;
+    const int vb = (i < m.nb) ? m.bt[i] : 0;
+    int ta, tb;
+    const int ea = petch::block_excl_scan<1024>(va, ta);
+    const int eb = petch::block_excl_scan<1024>(vb, tb);
+    if (i < m.nb) {
+      m.bv[i] = ca + ea;
+      m.bt[i] = cb + eb;
+    }
+    ca += ta;
+    cb += tb;
+  }
+  if (threadIdx.x == 0) {
+    m.totals[0] = ca;
+    m.totals[1] = cb;
+  }
+}
+
+__global__ void __launch_bounds__(kMcBS) k_mc_verts_many(const McGrid* __restrict__ t, int n) {
+  const McGrid& m = t[mc_grid_of(t, n)];
+  mc_verts_block(m.phi, m.g, m.N, blockIdx.x - m.blk0, m.dx, m.bv, m.vbase, m.vflg, m.verts,
+                 m.cap_v);
+}
+
+__global__ void __launch_bounds__(kMcBS) k_mc_faces_many(const McGrid* __restrict__ t, int n) {
+  const McGrid& m = t[mc_grid_of(t, n)];
+  mc_faces_block(m.phi, m.g, m.N, blockIdx.x - m.blk0, m.bt, m.vbase, m.vflg, m.faces, m.cap_f);
 }
 
 }  // namespace
@@ -366,8 +458,10 @@ extern "C" int petch_marching_cubes_batch(petch_ctx* ctx, int n, const petch_mc_
   PETCH_ARG(ctx, ctx && n >= 0 && (n == 0 || items), "marching_cubes_batch: null pointer or n < 0");
   if (n == 0) return PETCH_OK;
   // block partial sums of every grid side by side, then the totals of every grid
-  std::vector<long long> off((size_t)n + 1, 0), cells((size_t)n, 0);
-  long long maxN = 0;
+  std::vector<McGrid> t;
+  std::vector<int> which;
+  t.reserve((size_t)n);
+  long long parts = 0, blocks = 0;
   for (int f = 0; f < n; ++f) {
     const petch_mc_item& a = items[f];
     PETCH_ARG(ctx, a.phi && a.nv_out && a.nf_out, "marching_cubes_batch: item %d has a null pointer", f);
@@ -375,67 +469,89 @@ extern "C" int petch_marching_cubes_batch(petch_ctx* ctx, int n, const petch_mc_
     PETCH_ARG(ctx, a.cap_v >= 0 && a.cap_f >= 0, "marching_cubes_batch: item %d has a negative capacity", f);
     *a.nv_out = 0;
     *a.nf_out = 0;
-    const bool empty = a.nx < 2 || a.ny < 2 || a.nz < 2;
-    const long long N = empty ? 0 : (long long)a.nx * a.ny * a.nz;
-    cells[f] = N;
-    if (N > maxN) maxN = N;
-    off[f + 1] = off[f] + 2 * ((N + kMcBS - 1) / kMcBS);
+    if (a.nx < 2 || a.ny < 2 || a.nz < 2) continue;       // no cells, no surface
+    McGrid m{};
+    m.phi = a.phi;
+    m.g = Grid{a.nx, a.ny, a.nz, (long long)a.ny * a.nz, (long long)a.nz};
+    m.N = (long long)a.nx * a.ny * a.nz;
+    m.nb = (m.N + kMcBS - 1) / kMcBS;
+    m.blk0 = blocks;
+    m.dx = a.dx;
+    m.verts = a.verts;
+    m.faces = a.faces;
+    m.cap_v = a.cap_v;
+    m.cap_f = a.cap_f;
+    m.bv = (int*)(intptr_t)parts;                           // offsets until the buffer exists
+    parts += 2 * m.nb;
+    blocks += m.nb;
+    t.push_back(m);
+    which.push_back(f);
   }
-  int* blk = (int*)petch_scratch(ctx, SCR_MC_BLOCK, (size_t)(off[n] + 2 * n) * sizeof(int));
-  int* h = (int*)petch_pinned(ctx, (size_t)(2 * n) * sizeof(int));
+  const int m_ = (int)t.size();
+  if (m_ == 0) return PETCH_OK;
+  PETCH_ARG(ctx, blocks <= 0x7fffffffLL, "marching_cubes_batch: too many nodes for one launch");
+  int* blk = (int*)petch_scratch(ctx, SCR_MC_BLOCK, (size_t)(parts + 2 * m_) * sizeof(int));
+  int* h = (int*)petch_pinned(ctx, (size_t)(2 * m_) * sizeof(int));
   if (!blk || !h) return PETCH_E_HIP;
-  int* totals = blk + off[n];
-  PETCH_HIP(ctx, hipMemsetAsync(totals, 0, (size_t)(2 * n) * sizeof(int), ctx->stream));
-  for (int f = 0; f < n; ++f) {
-    if (cells[f] == 0) continue;
-    const petch_mc_item& a = items[f];
-    const long long N = cells[f], nb = (N + kMcBS - 1) / kMcBS;
-    Grid g{a.nx, a.ny, a.nz, (long long)a.ny * a.nz, (long long)a.nz};
-    int* bv = blk + off[f];
-    int* bt = bv + nb;
-    hipLaunchKernelGGL(k_mc_count, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, a.phi, g, N,
-                       bv, bt);
-    PETCH_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(petch::k_scan_partials2, dim3(1), dim3(1024), 0, ctx->stream, bv, bt,
-                       (int)nb, totals + 2 * f);
-    PETCH_LAUNCH_CHECK(ctx);
+  for (int k = 0; k < m_; ++k) {
+    McGrid& m = t[k];
+    m.bv = blk + (intptr_t)m.bv;
+    m.bt = m.bv + m.nb;
+    m.totals = blk + parts + 2 * k;
   }
+  const McGrid* d = (const McGrid*)petch_upload_table(ctx, t.data(), t.size() * sizeof(McGrid));
+  if (!d) return PETCH_E_HIP;
+  hipLaunchKernelGGL(k_mc_count_many, dim3((unsigned)blocks), dim3(kMcBS), 0, ctx->stream, d, m_);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_mc_scan_many, dim3((unsigned)m_), dim3(1024), 0, ctx->stream, d);
+  PETCH_LAUNCH_CHECK(ctx);
   // the one host read of the batch: every grid's vertex and face count
-  PETCH_HIP(ctx, hipMemcpyAsync(h, totals, (size_t)(2 * n) * sizeof(int), hipMemcpyDeviceToHost,
-                                ctx->stream));
+  PETCH_HIP(ctx, hipMemcpyAsync(h, blk + parts, (size_t)(2 * m_) * sizeof(int),
+                                hipMemcpyDeviceToHost, ctx->stream));
   PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   int short_of = -1;
-  for (int f = 0; f < n; ++f) {
-    const petch_mc_item& a = items[f];
-    *a.nv_out = h[2 * f];
-    *a.nf_out = h[2 * f + 1];
-    if (h[2 * f] > a.cap_v || h[2 * f + 1] > a.cap_f || (h[2 * f] > 0 && !a.verts) ||
-        (h[2 * f + 1] > 0 && !a.faces))
-      short_of = f;
+  for (int k = 0; k < m_; ++k) {
+    const petch_mc_item& a = items[which[k]];
+    *a.nv_out = h[2 * k];
+    *a.nf_out = h[2 * k + 1];
+    if (h[2 * k] > a.cap_v || h[2 * k + 1] > a.cap_f || (h[2 * k] > 0 && !a.verts) ||
+        (h[2 * k + 1] > 0 && !a.faces))
+      short_of = k;
   }
   if (short_of >= 0)
     return petch_set_error(ctx, PETCH_E_CAPACITY,
                            "marching_cubes_batch: item %d needs %d vertices / %d faces, capacity "
-                           "%d / %d (every item's counts are in its nv_out / nf_out)", short_of,
-                           h[2 * short_of], h[2 * short_of + 1], items[short_of].cap_v,
-                           items[short_of].cap_f);
-  unsigned char* cell = (unsigned char*)petch_scratch(ctx, SCR_MC_CELL, (size_t)maxN * 5);
-  if (maxN > 0 && !cell) return PETCH_E_HIP;
-  for (int f = 0; f < n; ++f) {
-    const petch_mc_item& a = items[f];
-    if (cells[f] == 0 || (h[2 * f] == 0 && h[2 * f + 1] == 0)) continue;
-    const long long N = cells[f], nb = (N + kMcBS - 1) / kMcBS;
-    Grid g{a.nx, a.ny, a.nz, (long long)a.ny * a.nz, (long long)a.nz};
-    int* bv = blk + off[f];
-    int* bt = bv + nb;
-    int* vbase = (int*)cell;
-    uint8_t* vflg = cell + 4 * N;
-    hipLaunchKernelGGL(k_mc_verts, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, a.phi, g, N,
-                       a.dx, bv, vbase, vflg, a.verts, a.cap_v);
-    PETCH_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_mc_faces, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, a.phi, g, N,
-                       bt, vbase, vflg, a.faces, a.cap_f);
-    PETCH_LAUNCH_CHECK(ctx);
+                           "%d / %d (every item's counts are in its nv_out / nf_out)",
+                           which[short_of], h[2 * short_of], h[2 * short_of + 1],
+                           items[which[short_of]].cap_v, items[which[short_of]].cap_f);
+  // the emit passes over the grids with a surface; each gets its own per-node vertex tables
+  std::vector<McGrid> e;
+  e.reserve(t.size());
+  long long cells = 0;
+  blocks = 0;
+  for (int k = 0; k < m_; ++k) {
+    if (h[2 * k] == 0 && h[2 * k + 1] == 0) continue;
+    McGrid m = t[k];
+    m.blk0 = blocks;
+    blocks += m.nb;
+    m.vbase = (int*)(intptr_t)cells;
+    cells += 5 * ((m.N + 63) / 64 * 64);
+    e.push_back(m);
   }
+  if (e.empty()) return PETCH_OK;
+  unsigned char* cell = (unsigned char*)petch_scratch(ctx, SCR_MC_CELL, (size_t)cells);
+  if (!cell) return PETCH_E_HIP;
+  for (McGrid& m : e) {
+    unsigned char* base = cell + (intptr_t)m.vbase;
+    m.vbase = (int*)base;
+    m.vflg = base + 4 * ((m.N + 63) / 64 * 64);
+  }
+  d = (const McGrid*)petch_upload_table(ctx, e.data(), e.size() * sizeof(McGrid));
+  if (!d) return PETCH_E_HIP;
+  const int ne = (int)e.size();
+  hipLaunchKernelGGL(k_mc_verts_many, dim3((unsigned)blocks), dim3(kMcBS), 0, ctx->stream, d, ne);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_mc_faces_many, dim3((unsigned)blocks), dim3(kMcBS), 0, ctx->stream, d, ne);
+  PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }

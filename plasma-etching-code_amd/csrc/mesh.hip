@@ -15,7 +15,8 @@ namespace {
 
 constexpr int kBS = 256;
 
-__device__ __forceinline__ void mesh_attrs_face(const float* __restrict__ verts,
+// Writes face f's centroid, area and unit normal; returns whether the centroid is finite.
+__device__ __forceinline__ bool mesh_attrs_face(const float* __restrict__ verts,
                                                 const int* __restrict__ faces,
                                                 float* __restrict__ cen, float* __restrict__ area,
                                                 float* __restrict__ nrm, int f) {
@@ -23,10 +24,11 @@ __device__ __forceinline__ void mesh_attrs_face(const float* __restrict__ verts,
   float ax = verts[3 * a], ay = verts[3 * a + 1], az = verts[3 * a + 2];
   float bx = verts[3 * b], by = verts[3 * b + 1], bz = verts[3 * b + 2];
   float cx = verts[3 * c], cy = verts[3 * c + 1], cz = verts[3 * c + 2];
+  const float gx = (ax + bx + cx) / 3.f, gy = (ay + by + cy) / 3.f, gz = (az + bz + cz) / 3.f;
   if (cen) {
-    cen[3 * f + 0] = (ax + bx + cx) / 3.f;
-    cen[3 * f + 1] = (ay + by + cy) / 3.f;
-    cen[3 * f + 2] = (az + bz + cz) / 3.f;
+    cen[3 * f + 0] = gx;
+    cen[3 * f + 1] = gy;
+    cen[3 * f + 2] = gz;
   }
   // cross product in float64: slivers (vertices near a grid node) keep an accurate normal
   double ux = (double)bx - ax, uy = (double)by - ay, uz = (double)bz - az;
@@ -40,6 +42,7 @@ __device__ __forceinline__ void mesh_attrs_face(const float* __restrict__ verts,
     nrm[3 * f + 1] = (float)(ny * inv);
     nrm[3 * f + 2] = (float)(nz * inv);
   }
+  return isfinite(gx) && isfinite(gy) && isfinite(gz);
 }
 
 __global__ void k_mesh_attrs(const float* __restrict__ verts, const int* __restrict__ faces, int nf,
@@ -111,6 +114,7 @@ struct SurfItem {
   float *cen, *area, *nrm;
   int* nbr;
   int *cnt, *cur, *off, *inc;   // workspace: nv + 1, nv + 1, nv + 1, 3 nf
+  int* finite;                  // optional: 1 if every centroid is finite, else 0
   long long blk0;
 };
 
@@ -124,9 +128,10 @@ __device__ __forceinline__ int surf_item_of(const SurfItem* __restrict__ it, int
   return lo;
 }
 
-// one workgroup per item: clear the vertex counters
+// one workgroup per item: clear the vertex counters and set the finite flag
 __global__ void k_surf_clear(const SurfItem* __restrict__ items) {
   const SurfItem& it = items[blockIdx.x];
+  if (threadIdx.x == 0 && it.finite) *it.finite = 1;
   for (int v = threadIdx.x; v <= it.nv; v += kBS) {
     it.cnt[v] = 0;
     it.cur[v] = 0;
@@ -140,7 +145,9 @@ __global__ void k_surf_count(const SurfItem* __restrict__ items, int n) {
   if (t >= 3 * it.nf) return;
   const int v = it.faces[t];
   if (v >= 0 && v < it.nv) atomicAdd(&it.cnt[v], 1);
-  if (t % 3 == 0) mesh_attrs_face(it.verts, it.faces, it.cen, it.area, it.nrm, t / 3);
+  if (t % 3 == 0 && !mesh_attrs_face(it.verts, it.faces, it.cen, it.area, it.nrm, t / 3) &&
+      it.finite)
+    *it.finite = 0;
 }
 
 // one workgroup per item: exclusive scan of its nv + 1 counts
@@ -273,6 +280,7 @@ extern "C" int petch_surface_build_batch(petch_ctx* ctx, int n, const petch_surf
     r.cur = w + off; off += q.nv + 1;
     r.off = w + off; off += q.nv + 1;
     r.inc = w + off; off += 3LL * q.nf;
+    r.finite = q.finite;
     r.blk0 = blocks;
     blocks += petch_blocks(3LL * q.nf, kBS);
     t.push_back(r);

@@ -120,7 +120,15 @@ class SurfaceItem(ctypes.Structure):
     """petch_surface_item: one mesh of a batched attribute + adjacency build."""
     _fields_ = [("verts", c_void_p), ("faces", c_void_p), ("nf", c_int), ("nv", c_int),
                 ("centroids", c_void_p), ("areas", c_void_p), ("normals", c_void_p),
-                ("nbr", c_void_p)]
+                ("nbr", c_void_p), ("finite", c_void_p)]
+
+
+class RateItem(ctypes.Structure):
+    """petch_rate_item: one surface of a batched surface-rate launch."""
+    _fields_ = [(k, c_void_p) for k in ("m_i", "m_F", "m_O", "cos_i", "iy_sp", "iy_ie", "iy_ip",
+                                        "centroids", "is_mask")] + \
+               [("mg", MaskGeom), ("chem", ChemParams), ("F", c_int), ("V", c_void_p),
+                ("red", c_void_p), ("bvh", c_void_p), ("bvh_err", c_void_p)]
 
 
 class McItem(ctypes.Structure):
@@ -205,6 +213,7 @@ SIGNATURES = {
     "petch_finish_round_batch": [P, c_int, P],
     "petch_finish_ion_batch": [P, c_int, P],
     "petch_surface_build_batch": [P, c_int, P],
+    "petch_surface_rate_batch": [P, c_int, P],
     "petch_belen_coverage_adjoint": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P,
                                      P, P, P],
     "petch_surface_rate": [P, P, P, P, P, P, P, P, P, P, POINTER(MaskGeom),

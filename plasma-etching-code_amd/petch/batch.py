@@ -97,14 +97,14 @@ class BatchLoop:
                         ops.finish_round_batch([r for r in rounds if r['kind'] == "neutral"])
                         rounds.clear()
         # 3. every feature's rate; the rate reductions and BVH guards of all features in one read
-        has = [lp.end_rate(step, out) for lp, out in zip(loops, outs)]
+        rates = []
+        has = [lp.end_rate(step, out, rates=rates) for lp, out in zip(loops, outs)]
         reads = [(0.0, 0, 0)] * len(loops)
         live = [i for i, h in enumerate(has) if h]
         if live:
-            buf = torch.zeros((len(live), 3), dtype=torch.int32, device=loops[live[0]].dev)
-            buf[:, :2] = torch.stack([loops[i].red for i in live])
-            for r, i in enumerate(live):
-                loops[i]._mesh.bvh.status_into(buf[r, 2:3])
+            buf = torch.empty((len(live), 3), dtype=torch.int32, device=loops[live[0]].dev)
+            with self.timer.region("rate"):
+                ops.surface_rate_batch(rates, buf)
             host = buf.cpu().numpy()
             for r, i in enumerate(live):
                 vmx, nbad = ops.decode_rate_reduction(host[r, :2])

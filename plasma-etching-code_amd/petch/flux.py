@@ -55,7 +55,8 @@ class SurfaceMesh:
             self.areas = torch.empty((nf,), dtype=torch.float32, device=dev)
             self.normals = torch.empty((nf, 3), dtype=torch.float32, device=dev)
             self.nbr = torch.empty((nf, 3), dtype=torch.int32, device=dev)
-            surf.append((v, f, self.centroids, self.areas, self.normals, self.nbr))
+            self.finite = torch.empty((), dtype=torch.int32, device=dev)
+            surf.append((v, f, self.centroids, self.areas, self.normals, self.nbr, self.finite))
         if self.quality is not None or self.rays_per_step is not None:
             q = self.quality
             if q is None:
@@ -187,7 +188,8 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
         bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
     else:
         bare = torch.clamp(bare_init.to(torch.float32), 0.0, 1.0)
-    m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
+    if n_fp <= 0:
+        m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
     for it in range(n_fp):
         # the F and O traces of one iteration (threed.py:876-880), seeds seed*9+2+2it and
         # seed*9+3+2it, as one launch

@@ -325,21 +325,27 @@ def face_adjacency(faces, nv):
 
 def surface_build_batch(items):
     """mesh_attrs and face_adjacency of many meshes in one launch per stage
-    (petch_surface_build_batch). items: (verts, faces, centroids, areas, normals, nbr) with the
-    outputs allocated as those two calls return them; they are filled in place."""
+    (petch_surface_build_batch). items: (verts, faces, centroids, areas, normals, nbr[, finite])
+    with the outputs allocated as those two calls return them; they are filled in place. finite:
+    an int32 device scalar set to 1 if every centroid is finite, else 0 (not written for a
+    mesh without faces)."""
     from ._abi import SurfaceItem
     items = [it for it in items if int(it[1].shape[0])]
     if not items:
         return
     arr = (SurfaceItem * len(items))()
-    for k, (v, f, cen, area, nrm, nbr) in enumerate(items):
+    for k, (v, f, cen, area, nrm, nbr, *fin) in enumerate(items):
+        fin = fin[0] if fin else None
+        if fin is not None and (fin.dtype != i32 or fin.numel() != 1):
+            raise ValueError(f"surface_build_batch: item {k} finite flag must be one int32")
         nf, nv = int(f.shape[0]), int(v.shape[0])
         if tuple(cen.shape) != (nf, 3) or tuple(area.shape) != (nf,) or \
                 tuple(nrm.shape) != (nf, 3) or tuple(nbr.shape) != (nf, 3):
             raise ValueError(f"surface_build_batch: item {k} outputs do not match {nf} faces")
         arr[k] = SurfaceItem(ptr(v, f32, "verts"), ptr(f, i32, "faces"), nf, nv,
                                   ptr(cen, f32, "centroids"), ptr(area, f32, "areas"),
-                                  ptr(nrm, f32, "normals"), ptr(nbr, i32, "nbr"))
+                                  ptr(nrm, f32, "normals"), ptr(nbr, i32, "nbr"),
+                                  ptr(fin))
     _ctx(items[0][0]).call("petch_surface_build_batch", len(items), arr)
 
 
@@ -735,6 +741,34 @@ def surface_rate(m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem, is
                    ptr(cos_i, f32), ptr(sp), ptr(ie), ptr(ip), ptr(centroids),
                    ptr(is_mask), ctypes_ref(mask_geom), ctypes_ref(chem), F, ptr(V), ptr(red))
     return V, red
+
+
+def surface_rate_batch(items, red):
+    """surface_rate of many surfaces in one launch (petch_surface_rate_batch). items: dicts with
+    the surface_rate arguments (m_i, m_F, m_O, cos_i, ion_yield, centroids, mask_geom, chem,
+    optional is_mask), the output V (F,) and optional bvh (a Bvh). red: (len(items), 3) int32
+    device tensor; row r receives item r's reduction pair and (column 2) its BVH guard flag."""
+    from ._abi import RateItem
+    n = len(items)
+    if tuple(red.shape) != (n, 3) or red.dtype != i32 or not red.is_contiguous():
+        raise ValueError(f"surface_rate_batch: red must be a contiguous ({n}, 3) int32 tensor")
+    if not n:
+        return
+    arr = (RateItem * n)()
+    base = red.data_ptr()
+    for k, it in enumerate(items):
+        F = int(it['m_F'].shape[0])
+        if tuple(it['V'].shape) != (F,):
+            raise ValueError(f"surface_rate_batch: item {k} V does not have {F} entries")
+        sp, ie, ip = it['ion_yield'] if it.get('ion_yield') is not None else (None, None, None)
+        bvh = it.get('bvh')
+        arr[k] = RateItem(ptr(it['m_i'], f32), ptr(it['m_F'], f32), ptr(it['m_O'], f32),
+                          ptr(it['cos_i'], f32), ptr(sp), ptr(ie), ptr(ip),
+                          ptr(it.get('centroids')), ptr(it.get('is_mask')), it['mask_geom'],
+                          it['chem'], F, ptr(it['V'], f32), base + 12 * k,
+                          bvh.h if bvh is not None else None,
+                          base + 12 * k + 8 if bvh is not None else None)
+    _ctx(red).call("petch_surface_rate_batch", n, arr)
 
 
 def read_rate_reduction(red):

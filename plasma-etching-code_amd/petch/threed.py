@@ -427,7 +427,8 @@ class EtchLoop:
                 m.quality = self.tuned[0]
         if mesh.nf and self.nt != "radiosity" and getattr(self.flags, "coverage_sticking", False) \
                 and self.warm and self.prev_bare is not None:
-            return torch.isfinite(mesh.centroids).all()
+            # staged: the flag the batched surface build computed (1 iff all centroids finite)
+            return mesh.finite if staged else torch.isfinite(mesh.centroids).all()
         return None
 
     def warm_start_item(self, finite):
@@ -484,8 +485,10 @@ class EtchLoop:
             return self.end_finish(step, vmx, nbad, self._mesh.bvh.status())
         return self.end_finish(step, 0.0, 0, 0)
 
-    def end_rate(self, step, out):
-        """Surface rate of the step (no host read). Returns False when there is no surface."""
+    def end_rate(self, step, out, rates=None):
+        """Surface rate of the step (no host read). Returns False when there is no surface.
+        rates: a list that receives this step's rate as an item of ops.surface_rate_batch
+        instead of running it (the batched loop rates every feature in one launch)."""
         T, flags = self.timer, self.flags
         mesh = self._mesh
         if out is not None and getattr(flags, "coverage_sticking", False) and \
@@ -497,8 +500,15 @@ class EtchLoop:
         trt = time.time()
         with T.region("rate"):
             iy = out['ion_yield'] if getattr(flags, "chemistry", "") == "belen" else None
-            self._V, _ = ops.surface_rate(out['m_i'], out['m_F'], out['m_O'], out['cos_i'], iy,
-                                          mesh.centroids, self.mg, self.chem, red=self.red)
+            if rates is not None:
+                self._V = torch.empty_like(out['m_F'])
+                rates.append(dict(m_i=out['m_i'], m_F=out['m_F'], m_O=out['m_O'],
+                                  cos_i=out['cos_i'], ion_yield=iy, centroids=mesh.centroids,
+                                  mask_geom=self.mg, chem=self.chem, V=self._V, bvh=mesh.bvh))
+            else:
+                self._V, _ = ops.surface_rate(out['m_i'], out['m_F'], out['m_O'], out['cos_i'],
+                                              iy, mesh.centroids, self.mg, self.chem,
+                                              red=self.red)
         self.timings['rate'] = self.timings.get('rate', 0.0) + time.time() - trt
         return True
 

@@ -35,10 +35,11 @@ class FakeLoop:
         self.rounds = rounds
         return self.begin_step(step)
 
-    def end_rate(self, step, out):
+    def end_rate(self, step, out, rates=None):
         self._out = out
-        self.red = torch.zeros(2, dtype=torch.int32)
         self.dev = torch.device("cpu")
+        if out is not None:
+            rates.append(("rate", self.k))
         return out is not None
 
     def extend_item(self):
@@ -120,14 +121,18 @@ def test_lockstep_batches_each_kind(monkeypatch):
     monkeypatch.setattr(B.ops, "finish_round_batch", round_batch)
     monkeypatch.setattr(B.ops, "finish_ion_batch", ion_finish_batch)
     monkeypatch.setattr(B.ops, "decode_rate_reduction", lambda r: (1.0, 0))
+
+    def rate_batch(items, red):
+        calls.append(("rate", list(items)))
+        red.zero_()
+    monkeypatch.setattr(B.ops, "surface_rate_batch", rate_batch)
     loops = [FakeLoop(0, 1), FakeLoop(1, 4), FakeLoop(99, 0), FakeLoop(2, 1)]
     outs = B.BatchLoop(loops).step(0)
     # one marching-cubes pass, one face attribute/adjacency build and one BVH build call for
-    # all features, one warm-start launch for
-    # the features that have a lookup, one launch per trace kind and fixed-point round, each
-    # neutral round ended for all its features in one call before the next trace, one extension
-    # pass for every feature with a surface rate, then the advection and the reinit of every
-    # feature, one call each
+    # all features, one warm-start launch for the features that have a lookup, one launch per
+    # trace kind and fixed-point round, each neutral round ended for all its features in one
+    # call before the next trace, one surface-rate launch and one extension pass for every
+    # feature with a surface, then the advection and the reinit of every feature, one call each
     every = lambda tag: [(tag, k) for k in (0, 1, 99, 2)]
     assert calls == [("mc", every("mc")), ("surf", every("surf")), ("bvh", every("bvh")),
                      ("ws", [("ws", 1)]), ("ion", [0, 1, 2]),
@@ -137,6 +142,7 @@ def test_lockstep_batches_each_kind(monkeypatch):
                      ("neutral", [1]), ("round", [("round", 1, 1)]),
                      ("neutral", [1]), ("round", [("round", 1, 2)]),
                      ("neutral", [1]), ("round", [("round", 1, 3)]),
+                     ("rate", [("rate", 0), ("rate", 1), ("rate", 2)]),
                      ("ext", [("ext", 0), ("ext", 1), ("ext", 2)]),
                      ("adv", every("adv")), ("re", every("re"))]
     assert not any(lp.bvh_here for lp in loops)

@@ -148,6 +148,56 @@ def test_surface_rate_nonfinite_guard():
     assert vmax == pytest.approx(float(v.max()))
 
 
+def test_batched_surface_rate_equals_single_calls():
+    """surface_rate_batch over surfaces of different chemistries, masks and sizes (one empty,
+    one with non-finite fluxes) gives each its single call's V and reduction pair, and copies
+    each BVH's guard flag beside them."""
+    import torch
+    from petch import ops
+    from petch.chemistry import chem_params
+    rng = np.random.default_rng(11)
+    cases = []
+    for k, (model, faithful, hole, F) in enumerate([("belen", True, True, 6000),
+                                                    ("belen", False, False, 300),
+                                                    ("langmuir", False, True, 0),
+                                                    ("langmuir", False, False, 2500),
+                                                    ("belen", True, False, 777)]):
+        m_i, m_F, m_O, cos_i, iy = _fluxes(F, 20 + k)
+        if k == 4:
+            m_F[5] = np.nan
+            m_O[7] = np.inf
+        cen = np.stack([rng.uniform(0, 14, F), rng.uniform(0, 14, F),
+                        rng.uniform(15, 23, F)], 1).astype(np.float32)
+        par = dict(PAR)
+        par['rate_scale'] = 0.01 * (k + 1)
+        par['fnorm_E'] = 0.7
+        cases.append(dict(m_i=_t(m_i), m_F=_t(m_F), m_O=_t(m_O), cos_i=_t(cos_i),
+                          ion_yield=tuple(_t(a) for a in iy) if faithful else None,
+                          centroids=_t(cen), mask_geom=ops.MaskGeom(18.0, 2.0, 14.0, 14.0, 6.0,
+                                                                    int(hole)),
+                          chem=chem_params(par, Flags(chemistry=model, yield_angular="viennaps"))))
+    singles = [ops.surface_rate(c['m_i'], c['m_F'], c['m_O'], c['cos_i'], c['ion_yield'],
+                                c['centroids'], c['mask_geom'], c['chem'])
+               if c['m_F'].numel() else (c['m_F'], torch.zeros(2, dtype=torch.int32))
+               for c in cases]
+    _, v, f = _mesh()
+    bvh = ops.Bvh(_dev())
+    bvh.build(_t(v), _t(f, torch.int32))
+    for c in cases:
+        c['V'] = torch.full_like(c['m_F'], -5.0)
+    cases[1]['bvh'] = bvh
+    red = torch.full((len(cases), 3), 77, dtype=torch.int32, device=_dev())
+    ops.surface_rate_batch(cases, red)
+    host = red.cpu().numpy()
+    for k, (c, (V, r)) in enumerate(zip(cases, singles)):
+        assert torch.equal(c['V'], V), k
+        np.testing.assert_array_equal(host[k, :2], r.cpu().numpy())
+    assert host[1, 2] == bvh.status() == 0
+    assert (host[[0, 2, 3, 4], 2] == 77).all()            # no BVH: column left alone
+    assert ops.decode_rate_reduction(host[4, :2])[1] >= 1
+    assert ops.decode_rate_reduction(host[2, :2]) == (0.0, 0)
+
+
 def test_extend_and_warm_start_nearest_face():
     import torch
     from petch import ops

@@ -92,7 +92,9 @@ def test_batched_marching_cubes_equals_single_calls():
     d = torch.device("cuda", 0)
     grids = [(ol.init_sdf(8.0, 8.0, 14.0, 0.25, 3.0, 8.0, 4.0, True), 0.25),
              (ol.init_sdf(6.0, 3.0, 10.0, 0.2, 2.0, 1.0, 6.0, False), 0.2),
-             (np.ones((4, 4, 4), np.float64), 0.5)]                   # no surface
+             (np.ones((4, 4, 4), np.float64), 0.5),                   # no surface
+             (np.random.default_rng(5).normal(size=(1, 6, 6)), 0.5),  # no cells
+             (FIELDS[3][1], FIELDS[3][2]), (FIELDS[7][1], FIELDS[7][2])]
     items, singles = [], []
     for phi, dx in grids:
         tp = torch.from_numpy(phi.astype(np.float32)).to(d)
@@ -104,6 +106,12 @@ def test_batched_marching_cubes_equals_single_calls():
         np.testing.assert_array_equal(v.cpu().numpy(), rv)
         np.testing.assert_array_equal(f.cpu().numpy(), rf)
     assert len(singles[2][1]) == 0 and len(outs[2][1]) == 0
+    assert len(singles[3][1]) == 0 and len(outs[3][1]) == 0
+    # again with the buffers now large enough (no retry), in another order
+    outs = ops.marching_cubes_batch(items[::-1])
+    for (v, f), (rv, rf) in zip(outs, singles[::-1]):
+        np.testing.assert_array_equal(v.cpu().numpy(), rv)
+        np.testing.assert_array_equal(f.cpu().numpy(), rf)
 
 
 def test_batched_surface_build_equals_single_calls():
@@ -118,6 +126,11 @@ def test_batched_surface_build_equals_single_calls():
         v, f = ops.marching_cubes(torch.from_numpy(np.ascontiguousarray(phi, np.float32)).to(d), dx)
         meshes.append((name, v, f))
     print("surface batch", [(n, int(f.shape[0])) for n, _, f in meshes], flush=True)
+    # a mesh with a non-finite vertex: its faces' attributes are NaN in both paths, and its
+    # finite flag is 0
+    v_bad = meshes[0][1].clone()
+    v_bad[int(meshes[0][2][3, 1])] = float("nan")
+    meshes.append(("nan_vertex", v_bad, meshes[0][2]))
     for order in (meshes, meshes[::-1][:4]):
         items, singles = [], []
         for _, v, f in order:
@@ -125,11 +138,18 @@ def test_batched_surface_build_equals_single_calls():
             singles.append(ops.mesh_attrs(v, f) + (ops.face_adjacency(f, v.shape[0]),))
             outs = (torch.full((nf, 3), -7.0, device=d), torch.full((nf,), -7.0, device=d),
                     torch.full((nf, 3), -7.0, device=d),
-                    torch.full((nf, 3), -7, dtype=torch.int32, device=d))
+                    torch.full((nf, 3), -7, dtype=torch.int32, device=d),
+                    torch.full((), 5, dtype=torch.int32, device=d))
             items.append((v, f) + outs)
         ops.surface_build_batch(items)
         torch.cuda.synchronize()
-        for (name, _, _), it, ref in zip(order, items, singles):
-            for got, want in zip(it[2:], ref):
-                assert torch.equal(got, want), name
+        for (name, _, f), it, ref in zip(order, items, singles):
+            for got, want in zip(it[2:5], ref[:3]):
+                torch.testing.assert_close(got, want, rtol=0, atol=0, equal_nan=True, msg=name)
+            assert torch.equal(it[5], ref[3]), name
+            if int(f.shape[0]):
+                assert int(it[6]) == int(torch.isfinite(ref[0]).all()), name
+            else:
+                assert int(it[6]) == 5, name                  # no faces: flag untouched
     assert any(int(f.shape[0]) == 0 for _, _, f in meshes)
+    assert int(items[0][6]) == 0                              # the NaN-vertex mesh, first now
