@@ -17,6 +17,8 @@
 // rounded to float32 once.
 #include "common.h"
 #include "bvh.h"
+#include "fastdiv.h"
+#include "scan.h"
 
 #include <vector>
 
@@ -583,9 +585,20 @@ struct ExtendItem {
   const float* phi;
   float* Fs;
   int ny, nz;
+  petch::FastDiv dy, dz;
   double dx, band;
   long long end;  // exclusive end of this grid's nodes in the global node numbering
 };
+
+// grid coordinates of node m of an item (m < 2^31: the batch's node numbering is int32)
+__device__ __forceinline__ petch::Vec3 item_node_pos(const ExtendItem& e, long long m) {
+  const uint32_t u = (uint32_t)m;
+  const uint32_t c = petch::fastdiv(u, e.dz);
+  const uint32_t k = u - c * (uint32_t)e.nz;
+  const uint32_t i = petch::fastdiv(c, e.dy);
+  const uint32_t j = c - i * (uint32_t)e.ny;
+  return petch::v3((float)((double)i * e.dx), (float)((double)j * e.dx), (float)((double)k * e.dx));
+}
 
 __device__ __forceinline__ int locate_item(const ExtendItem* __restrict__ it, int n, long long g,
                                            long long& local) {
@@ -598,36 +611,58 @@ __device__ __forceinline__ int locate_item(const ExtendItem* __restrict__ it, in
   return lo;
 }
 
-__global__ void k_band_flags(const ExtendItem* __restrict__ it, int n, long long total,
-                             int* __restrict__ flag, int zero_fs) {
-  const long long g = (long long)blockIdx.x * kBS + threadIdx.x;
-  if (g >= total) return;
+// The band |phi| < band of every item as the ascending list of its global node ids, by a
+// two-pass compaction over tiles of kBandTile consecutive nodes: k_band_count counts each
+// tile's band nodes (and, with zero_fs, sets Fs = 0 on every node, which the extension then
+// fills on the band), one workgroup scans the tile counts into offsets (the band's size goes
+// after the list), and k_band_emit writes each tile's band nodes in order at its offset.
+// Ascending order keeps neighbouring queries on neighbouring nodes of one surface.
+constexpr int kBandRounds = 8;
+constexpr long long kBandTile = (long long)kBandRounds * kBS;
+
+__device__ __forceinline__ bool band_node(const ExtendItem* __restrict__ it, int n, long long g,
+                                          long long total, int zero_fs) {
+  if (g >= total) return false;
   long long m;
   const ExtendItem& e = it[locate_item(it, n, g, m)];
-  flag[g] = fabs((double)e.phi[m]) < e.band ? 1 : 0;
   if (zero_fs) e.Fs[m] = 0.f;
+  return fabs((double)e.phi[m]) < e.band;
 }
 
-__global__ void k_band_compact(const int* __restrict__ flag, const int* __restrict__ pos,
-                               long long total, int* __restrict__ idx) {
-  const long long g = (long long)blockIdx.x * kBS + threadIdx.x;
-  if (g >= total) return;
-  if (flag[g]) idx[pos[g]] = (int)g;
+__global__ void k_band_count(const ExtendItem* __restrict__ it, int n, long long total,
+                             int* __restrict__ tile_cnt, int zero_fs) {
+  const long long g0 = (long long)blockIdx.x * kBandTile + threadIdx.x;
+  int c = 0;
+#pragma unroll
+  for (int r = 0; r < kBandRounds; ++r) c += band_node(it, n, g0 + (long long)r * kBS, total, zero_fs);
+  const int s = petch::block_sum<kBS>(c);
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s;
 }
 
+__global__ void k_band_emit(const ExtendItem* __restrict__ it, int n, long long total,
+                            const int* __restrict__ tile_off, int* __restrict__ list) {
+  const long long g0 = (long long)blockIdx.x * kBandTile + threadIdx.x;
+  int base = tile_off[blockIdx.x];
+  for (int r = 0; r < kBandRounds; ++r) {
+    const long long g = g0 + (long long)r * kBS;
+    const bool in = band_node(it, n, g, total, 0);
+    int tot;
+    const int ex = petch::block_excl_scan<kBS>(in ? 1 : 0, tot);
+    if (in) list[base + ex] = (int)g;
+    base += tot;
+  }
+}
+
+// one nearest-face query per band node, over the list in grid-stride order
 __global__ void k_extend(const ExtendItem* __restrict__ it, int n, const int* __restrict__ idx,
                          const int* __restrict__ count) {
-  const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= *count) return;
-  long long m;
-  const ExtendItem& e = it[locate_item(it, n, idx[i], m)];
-  const int k = (int)(m % e.nz);
-  const int j = (int)((m / e.nz) % e.ny);
-  const int ii = (int)(m / ((long long)e.nz * e.ny));
-  const petch::Vec3 p = petch::v3((float)((double)ii * e.dx), (float)((double)j * e.dx),
-                                  (float)((double)k * e.dx));
-  const petch::PointHit h = petch::bvh_closest_point(e.bv, p, 1.0e6f);
-  e.Fs[m] = h.face >= 0 ? e.V[h.face] : 0.f;
+  const int cnt = *count;
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < cnt; i += gridDim.x * kBS) {
+    long long m;
+    const ExtendItem& e = it[locate_item(it, n, idx[i], m)];
+    const petch::PointHit h = petch::bvh_closest_point(e.bv, item_node_pos(e, m), 1.0e6f);
+    e.Fs[m] = h.face >= 0 ? e.V[h.face] : 0.f;
+  }
 }
 
 // Adjoint of k_extend (one grid): the extension copies V[nearest face] to each band node, so
@@ -635,17 +670,13 @@ __global__ void k_extend(const ExtendItem* __restrict__ it, int n, const int* __
 __global__ void k_extend_adjoint(const ExtendItem* __restrict__ it, int n,
                                  const int* __restrict__ idx, const int* __restrict__ count,
                                  float* __restrict__ gV) {
-  const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i >= *count) return;
-  long long m;
-  const ExtendItem& e = it[locate_item(it, n, idx[i], m)];
-  const int k = (int)(m % e.nz);
-  const int j = (int)((m / e.nz) % e.ny);
-  const int ii = (int)(m / ((long long)e.nz * e.ny));
-  const petch::Vec3 p = petch::v3((float)((double)ii * e.dx), (float)((double)j * e.dx),
-                                  (float)((double)k * e.dx));
-  const petch::PointHit h = petch::bvh_closest_point(e.bv, p, 1.0e6f);
-  if (h.face >= 0) atomicAdd(gV + h.face, e.Fs[m]);
+  const int cnt = *count;
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < cnt; i += gridDim.x * kBS) {
+    long long m;
+    const ExtendItem& e = it[locate_item(it, n, idx[i], m)];
+    const petch::PointHit h = petch::bvh_closest_point(e.bv, item_node_pos(e, m), 1.0e6f);
+    if (h.face >= 0) atomicAdd(gV + h.face, e.Fs[m]);
+  }
 }
 
 // Nearest-face values of many point sets, each on its own surface, in one launch.
@@ -975,6 +1006,8 @@ extern "C" int petch_surface_rate_batch(petch_ctx* ctx, int n, const petch_rate_
 // band list of the items, then the extension queries; with gV_adjoint (one item) the
 // transposed queries instead, which read the item's Fs as gFs (left untouched) and add it into
 // gV_adjoint.
+constexpr unsigned kExtendBlocks = 8192;
+
 static int extend_run(petch_ctx* ctx, int n, const petch_extend_item* items, float* gV_adjoint) {
   PETCH_ARG(ctx, ctx && n >= 0 && (n == 0 || items), "extend_batch: null pointer or n < 0");
   if (n == 0) return PETCH_OK;
@@ -985,30 +1018,37 @@ static int extend_run(petch_ctx* ctx, int n, const petch_extend_item* items, flo
     PETCH_ARG(ctx, a.bvh && a.V && a.phi && a.Fs, "extend_batch: item %d has a null pointer", f);
     PETCH_ARG(ctx, a.nx > 0 && a.ny > 0 && a.nz > 0, "extend_batch: item %d has a bad shape", f);
     total += (long long)a.nx * a.ny * a.nz;
-    t[f] = ExtendItem{petch_bvh_view(a.bvh), a.V, a.phi, a.Fs, a.ny, a.nz, a.dx, a.band, total};
+    t[f] = ExtendItem{petch_bvh_view(a.bvh), a.V, a.phi, a.Fs, a.ny, a.nz,
+                      petch::make_fastdiv((uint32_t)a.ny), petch::make_fastdiv((uint32_t)a.nz),
+                      a.dx, a.band, total};
   }
   PETCH_ARG(ctx, total < 0x7fffffffLL, "extend_batch: %lld nodes exceed the int32 node index",
             total);
   const ExtendItem* dev = (const ExtendItem*)petch_upload_table(ctx, t.data(), t.size() * sizeof(ExtendItem));
-  // flags, their scan, and the node list with the band's size after it (scratch slots of the
+  // the band's node list with its length after it, and the tile counts (scratch slots of the
   // level-set stages, which run after the extension on the same stream)
-  int* flag = (int*)petch_scratch(ctx, SCR_LS_A, (size_t)total * sizeof(int));
-  int* pos = (int*)petch_scratch(ctx, SCR_LS_B, (size_t)total * sizeof(int));
+  const long long tiles = (total + kBandTile - 1) / kBandTile;
   int* idx = (int*)petch_scratch(ctx, SCR_LS_FROZEN, (size_t)(total + 1) * sizeof(int));
-  if (!dev || !flag || !pos || !idx) return PETCH_E_HIP;
+  int* tcnt = (int*)petch_scratch(ctx, SCR_LS_A, (size_t)tiles * sizeof(int));
+  if (!dev || !idx || !tcnt) return PETCH_E_HIP;
+  hipLaunchKernelGGL(k_band_count, dim3((unsigned)tiles), dim3(kBS), 0, ctx->stream, dev, n,
+                     total, tcnt, gV_adjoint ? 0 : 1);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(petch::k_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, tcnt,
+                     (int)tiles, idx + total);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_band_emit, dim3((unsigned)tiles), dim3(kBS), 0, ctx->stream, dev, n, total,
+                     tcnt, idx);
+  PETCH_LAUNCH_CHECK(ctx);
+  // the queries loop over the list (its length is on the device): enough workgroups to fill
+  // every CU several times over, fewer when the whole grid has fewer nodes
   const unsigned g = petch_blocks(total, kBS);
-  hipLaunchKernelGGL(k_band_flags, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, total, flag,
-                     gV_adjoint ? 0 : 1);
-  PETCH_LAUNCH_CHECK(ctx);
-  int rc = petch_exclusive_scan(ctx, flag, pos, total, idx + total);
-  if (rc != PETCH_OK) return rc;
-  hipLaunchKernelGGL(k_band_compact, dim3(g), dim3(kBS), 0, ctx->stream, flag, pos, total, idx);
-  PETCH_LAUNCH_CHECK(ctx);
+  const unsigned gq = g < kExtendBlocks ? g : kExtendBlocks;
   if (gV_adjoint) {
-    hipLaunchKernelGGL(k_extend_adjoint, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, idx,
+    hipLaunchKernelGGL(k_extend_adjoint, dim3(gq), dim3(kBS), 0, ctx->stream, dev, n, idx,
                        idx + total, gV_adjoint);
   } else {
-    hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBS), 0, ctx->stream, dev, n, idx, idx + total);
+    hipLaunchKernelGGL(k_extend, dim3(gq), dim3(kBS), 0, ctx->stream, dev, n, idx, idx + total);
   }
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;

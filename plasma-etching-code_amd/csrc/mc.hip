@@ -23,6 +23,7 @@
 
 #include <vector>
 #include "scan.h"
+#include "fastdiv.h"
 
 namespace {
 
@@ -140,12 +141,20 @@ __device__ __forceinline__ int cell_tri_count(const float f[8]) {
 struct Grid {
   int nx, ny, nz;
   long long sx, sy;  // node strides
+  petch::FastDiv dy, dz;  // division by ny and nz (node indices of one grid fit 32 bits)
 };
 
+__host__ inline Grid make_grid(int nx, int ny, int nz) {
+  return Grid{nx, ny, nz, (long long)ny * nz, (long long)nz, petch::make_fastdiv((uint32_t)ny),
+              petch::make_fastdiv((uint32_t)nz)};
+}
+
 __device__ __forceinline__ void node_ijk(const Grid& g, long long n, int& i, int& j, int& k) {
-  k = (int)(n % g.nz);
-  j = (int)((n / g.nz) % g.ny);
-  i = (int)(n / ((long long)g.nz * g.ny));
+  const uint32_t u = (uint32_t)n;
+  const uint32_t c = petch::fastdiv(u, g.dz);   // column (i, j)
+  k = (int)(u - c * (uint32_t)g.nz);
+  i = (int)petch::fastdiv(c, g.dy);
+  j = (int)(c - (uint32_t)i * (uint32_t)g.ny);
 }
 
 __device__ __forceinline__ long long corner_off(const Grid& g, int c) {
@@ -416,7 +425,8 @@ extern "C" int petch_marching_cubes(petch_ctx* ctx, const float* phi, int nx, in
   *nf_host = 0;
   if (nx < 2 || ny < 2 || nz < 2) return PETCH_OK;  // no cells, no surface
   long long N = (long long)nx * ny * nz;
-  Grid g{nx, ny, nz, (long long)ny * nz, (long long)nz};
+  PETCH_ARG(ctx, N <= 0xffffffffLL, "marching_cubes: %lld nodes exceed 32-bit node indices", N);
+  Grid g = make_grid(nx, ny, nz);
   long long nb = (N + kMcBS - 1) / kMcBS;
   int* blk = (int*)petch_scratch(ctx, SCR_MC_BLOCK, (size_t)(2 * nb + 2) * sizeof(int));
   if (!blk) return PETCH_E_HIP;
@@ -472,8 +482,10 @@ extern "C" int petch_marching_cubes_batch(petch_ctx* ctx, int n, const petch_mc_
     if (a.nx < 2 || a.ny < 2 || a.nz < 2) continue;       // no cells, no surface
     McGrid m{};
     m.phi = a.phi;
-    m.g = Grid{a.nx, a.ny, a.nz, (long long)a.ny * a.nz, (long long)a.nz};
+    m.g = make_grid(a.nx, a.ny, a.nz);
     m.N = (long long)a.nx * a.ny * a.nz;
+    PETCH_ARG(ctx, m.N <= 0xffffffffLL,
+              "marching_cubes_batch: item %d: %lld nodes exceed 32-bit node indices", f, m.N);
     m.nb = (m.N + kMcBS - 1) / kMcBS;
     m.blk0 = blocks;
     m.dx = a.dx;
