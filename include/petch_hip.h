@@ -544,6 +544,48 @@ typedef struct petch_nearest_item {
 } petch_nearest_item;
 int petch_nearest_face_value_batch(petch_ctx* ctx, int n, const petch_nearest_item* items);
 
+/* ------------------------------------------- deterministic neutral transports (f4) ------- */
+/* knudsen_face_flux (knudsen.py:185-241 with conductance_profile 93-151), the neutral tail of
+ * mc_flux_3d_knudsen (threed.py:1149-1208), for two sticking coefficients at once (the F and O
+ * species of one coverage round): per-slice gas area and perimeter of phi (z = k dx), the floor
+ * faces' (into-gas normal z > 0.7, below sub_top, inside the footprint) sink 0.25 clip(s) A per
+ * slice with s = bare beta (or min(bare, mean field bare) with sink_field, threed.py:1193-1197),
+ * the conductance chain's tridiagonal solve, and each face's slice concentration (faces at or
+ * above sub_top read 1), clipped to [0, 8], into m_a (beta_a) and m_b (beta_b; may be NULL). */
+typedef struct petch_knudsen_params {
+  double dx, sub_top, Ly;            /* Ly: trench length (a trench's sink is per unit length) */
+  int trench;                        /* 1 trench (area = mean open x-width), 0 hole */
+  double wall_loss_scale;            /* knudsen_wall_loss_scale */
+  int front_loss;                    /* flags.knudsen_front_loss: per-slice loss scale */
+  double band_W, pass_frac, ar_pass; /* its knudsen_front_band_W / _passive_frac / _front_ar_pass */
+  int footprint;                     /* restrict the sink to |x - cx| (trench) or the radius
+                                        (hole) <= half_width + 2 dx; W = 2 half_width */
+  double cx, cy, half_width;
+  int sink_field;                    /* knudsen_sink = 'field' */
+  double beta_a, beta_b;
+} petch_knudsen_params;
+int petch_knudsen_face_flux(petch_ctx* ctx, const float* phi, int nx, int ny, int nz,
+                            const float* centroids, const float* gas_normals, const float* areas,
+                            const float* bare, int F, const petch_knudsen_params* p, float* m_a,
+                            float* m_b);
+
+/* dda_neutral_flux (dda.py:138-220), the neutrals of mc_flux_3d_dda (threed.py:1211-1244), for
+ * two sticking coefficients at once: the march of the _dda_gather_kernel (dda.py:26-95) from
+ * every centroid along the 2 n_dir fib_sphere directions, cosine-weighted by the into-gas
+ * normal, first from the sky (1 at z >= z_top = (nz - 1.5) dx, 0 at walls), then n_reemit
+ * passes with the walls' re-emission field (area-weighted per solid cell, dilated into unseeded
+ * solid neighbours), Aitken on the last three iterates, clipped to [0, 8]. Faces' areas are
+ * floored at 1e-9. periodic_y wraps the march along y. m_b may be NULL. */
+typedef struct petch_dda_params {
+  double dx;
+  int n_dir, n_reemit, periodic_y;
+  double beta_a, beta_b;
+} petch_dda_params;
+int petch_dda_neutral_flux(petch_ctx* ctx, const float* phi, int nx, int ny, int nz,
+                           const float* centroids, const float* gas_normals, const float* areas,
+                           const float* bare, int F, const petch_dda_params* p, float* m_a,
+                           float* m_b);
+
 /* ---------------------------------------------------------------- a21: radiosity -------- */
 /* Form-factor assembly of mc_flux_3d_radiosity (threed.py:956-960) from the petch_trace_ff
  * output: D[j] = escapes of face j / n_ff (float64), and the rays j -> i as a CSR of rows i

@@ -131,6 +131,21 @@ class RateItem(ctypes.Structure):
                 ("red", c_void_p), ("bvh", c_void_p), ("bvh_err", c_void_p)]
 
 
+class KnudsenParams(ctypes.Structure):
+    """petch_knudsen_params."""
+    _fields_ = [("dx", c_double), ("sub_top", c_double), ("Ly", c_double), ("trench", c_int),
+                ("wall_loss_scale", c_double), ("front_loss", c_int), ("band_W", c_double),
+                ("pass_frac", c_double), ("ar_pass", c_double), ("footprint", c_int),
+                ("cx", c_double), ("cy", c_double), ("half_width", c_double),
+                ("sink_field", c_int), ("beta_a", c_double), ("beta_b", c_double)]
+
+
+class DdaParams(ctypes.Structure):
+    """petch_dda_params."""
+    _fields_ = [("dx", c_double), ("n_dir", c_int), ("n_reemit", c_int), ("periodic_y", c_int),
+                ("beta_a", c_double), ("beta_b", c_double)]
+
+
 class McItem(ctypes.Structure):
     """petch_mc_item: one grid of a batched marching cubes."""
     _fields_ = [("phi", c_void_p), ("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_double),
@@ -214,6 +229,10 @@ SIGNATURES = {
     "petch_finish_ion_batch": [P, c_int, P],
     "petch_surface_build_batch": [P, c_int, P],
     "petch_surface_rate_batch": [P, c_int, P],
+    "petch_knudsen_face_flux": [P, P, c_int, c_int, c_int, P, P, P, P, c_int,
+                                POINTER(KnudsenParams), P, P],
+    "petch_dda_neutral_flux": [P, P, c_int, c_int, c_int, P, P, P, P, c_int,
+                               POINTER(DdaParams), P, P],
     "petch_belen_coverage_adjoint": [P, P, P, P, P, P, P, POINTER(ChemParams), c_int, P, P, P,
                                      P, P, P],
     "petch_surface_rate": [P, P, P, P, P, P, P, P, P, P, POINTER(MaskGeom),

@@ -256,6 +256,67 @@ def deterministic_ions(mesh, geo, par, n_ion, seed, flags):
     return ops.normalize(acc[0], A, A_src / n_ion, 0.0, 1.5), ops.cos_ratio(acc[0], acc[1]), None
 
 
+def _deterministic_neutrals(mesh, phi, geo, par, n_ion, seed, flags, n_fp, neutrals):
+    """The coverage loop shared by the deterministic-neutral transports (threed.py:1149-1244):
+    the deterministic ion launch, then n_fp rounds of {m_F, m_O = neutrals(bare); Belen
+    coverages}. neutrals(bare) returns the two species' multipliers."""
+    F = mesh.nf
+    m_i, cos_i, ion_yield = deterministic_ions(mesh, geo, par, n_ion, seed * 9 + 1, flags)
+    chem = chem_params(par, flags)
+    bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
+    m_F = m_O = None
+    for _ in range(n_fp):
+        m_F, m_O = neutrals(bare)
+        bare = ops.belen_coverage(m_i, m_F, m_O, cos_i, ion_yield, chem)
+    if m_F is None:
+        m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
+    return dict(m_i=m_i, m_F=m_F, m_O=m_O, cos_i=cos_i, bare=bare, ion_yield=ion_yield)
+
+
+def knudsen_flux(mesh, phi, geo, par, n_ion, seed, flags, n_fp=4):
+    """mc_flux_3d_knudsen (threed.py:1149-1208): Monte-Carlo ions, neutrals from the 1-D
+    Knudsen conductance tail down the feature (knudsen.py) with the floor sink at the local
+    coverage, or clamped at the open field's mean coverage (par['knudsen_sink'] = 'field')."""
+    if getattr(flags, "surface_charging", False) in ("hg", "petch") and _faithful(flags):
+        raise NotImplementedError("the Hwang-Giapis charging hook (threed.py:1046-1147) is outside "
+                                  "the HIP etch-loop path")
+    dx = geo['dx']
+    gn = ops.gas_normals(phi, dx, mesh.centroids, mesh.normals)
+    betaE, betaO = par.get('betaE', 0.7), par.get('betaO', 1.0)
+    sink = par.get('knudsen_sink', 'local')
+    if sink not in ('local', 'field'):
+        raise ValueError(f"knudsen_sink must be 'local' or 'field', got {sink!r}")
+    kw = dict(Ly=geo['Ly'], wall_loss_scale=float(par.get('knudsen_wall_loss_scale', 1.85)),
+              front_loss=bool(getattr(flags, 'knudsen_front_loss', False)),
+              band_W=float(par.get('knudsen_front_band_W', 8.0)),
+              pass_frac=float(par.get('knudsen_passive_frac', 0.5)),
+              ar_pass=float(par.get('knudsen_front_ar_pass', 15.0)),
+              center=(0.5 * geo['Lx'], 0.5 * geo['Ly']),
+              half_width=0.5 * float(geo.get('trench_width', geo['Lx'])),
+              sink_field=sink == 'field')
+    trench = not geo.get('hole', False)
+
+    def neutrals(bare):
+        return ops.knudsen_face_flux(phi, dx, geo['sub_top'], trench, mesh.centroids, gn,
+                                     mesh.areas, bare, (betaE, betaO), **kw)
+    return _deterministic_neutrals(mesh, phi, geo, par, n_ion, seed, flags, n_fp, neutrals)
+
+
+def dda_flux(mesh, phi, geo, par, n_ion, seed, flags, n_fp=4):
+    """mc_flux_3d_dda (threed.py:1211-1244): Monte-Carlo ions, neutrals from the deterministic
+    discrete-ordinates march with diffuse re-emission (dda.py)."""
+    dx = geo['dx']
+    gn = ops.gas_normals(phi, dx, mesh.centroids, mesh.normals)
+    betaE, betaO = par.get('betaE', 0.7), par.get('betaO', 1.0)
+    n_dir, n_re = int(par.get('dda_n_dir', 64)), int(par.get('dda_n_reemit', 12))
+    py = int(par.get('periodic_y', 0))
+
+    def neutrals(bare):
+        return ops.dda_neutral_flux(phi, dx, mesh.centroids, gn, mesh.areas, bare,
+                                    (betaE, betaO), n_dir, n_re, py)
+    return _deterministic_neutrals(mesh, phi, geo, par, n_ion, seed, flags, n_fp, neutrals)
+
+
 def radiosity_flux(mesh, phi, geo, par, n_ion, seed, flags, n_ff=64, n_fp=4):
     """mc_flux_3d_radiosity (threed.py:923-991): form factors by n_ff first-hit cosine rays per
     face, then n_fp x {G = D + Aff diag(1-s) G solved for F and O, Belen coverages}."""

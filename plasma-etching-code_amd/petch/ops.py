@@ -929,3 +929,53 @@ def trace_hit_counts(device=None, reset=True):
     Context.get(dev).call("petch_trace_hit_counts", int(bool(reset)),
                           out.ctypes.data_as(ctypes.c_void_p))
     return int(out[0]), int(out[1])
+
+
+# ------------------------------------------------- deterministic neutral transports ---------
+
+def knudsen_face_flux(phi, dx, sub_top, trench, centroids, gas_normals, areas, bare, betas,
+                      Ly, wall_loss_scale=1.85, front_loss=False, band_W=8.0, pass_frac=0.5,
+                      ar_pass=15.0, center=None, half_width=None, sink_field=False):
+    """knudsen_face_flux (knudsen.py:185-241) for the sticking coefficients bare * beta of the
+    one or two betas (petch_knudsen_face_flux): one (F,) float32 multiplier per beta."""
+    from ._abi import KnudsenParams
+    nx, ny, nz = _shape3(phi)
+    F = int(centroids.shape[0])
+    betas = list(betas)
+    if not 1 <= len(betas) <= 2:
+        raise ValueError("knudsen_face_flux: one or two betas")
+    outs = [torch.empty((F,), dtype=f32, device=phi.device) for _ in betas]
+    fp = center is not None and half_width is not None
+    cx, cy = (float(center[0]), float(center[1])) if fp else (0.0, 0.0)
+    p = KnudsenParams(float(dx), float(sub_top), float(Ly), int(bool(trench)),
+                      float(wall_loss_scale), int(bool(front_loss)), float(band_W),
+                      float(pass_frac), float(ar_pass), int(fp), cx, cy,
+                      float(half_width) if fp else 0.0, int(bool(sink_field)), float(betas[0]),
+                      float(betas[-1]))
+    if F:
+        _ctx(phi).call("petch_knudsen_face_flux", ptr(phi, f32, "phi"), nx, ny, nz,
+                       ptr(centroids, f32, "centroids"), ptr(gas_normals, f32, "gas_normals"),
+                       ptr(areas, f32, "areas"), ptr(bare, f32, "bare"), F, ctypes_ref(p),
+                       ptr(outs[0]), ptr(outs[1]) if len(outs) > 1 else None)
+    return outs
+
+
+def dda_neutral_flux(phi, dx, centroids, gas_normals, areas, bare, betas, n_dir=64, n_reemit=12,
+                     periodic_y=0):
+    """dda_neutral_flux (dda.py:138-220) for the sticking coefficients bare * beta of the one or
+    two betas (petch_dda_neutral_flux): one (F,) float32 multiplier per beta."""
+    from ._abi import DdaParams
+    nx, ny, nz = _shape3(phi)
+    F = int(centroids.shape[0])
+    betas = list(betas)
+    if not 1 <= len(betas) <= 2:
+        raise ValueError("dda_neutral_flux: one or two betas")
+    outs = [torch.empty((F,), dtype=f32, device=phi.device) for _ in betas]
+    p = DdaParams(float(dx), int(n_dir), int(n_reemit), int(bool(periodic_y)), float(betas[0]),
+                  float(betas[-1]))
+    if F:
+        _ctx(phi).call("petch_dda_neutral_flux", ptr(phi, f32, "phi"), nx, ny, nz,
+                       ptr(centroids, f32, "centroids"), ptr(gas_normals, f32, "gas_normals"),
+                       ptr(areas, f32, "areas"), ptr(bare, f32, "bare"), F, ctypes_ref(p),
+                       ptr(outs[0]), ptr(outs[1]) if len(outs) > 1 else None)
+    return outs

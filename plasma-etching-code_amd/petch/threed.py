@@ -64,6 +64,10 @@ def _trench_geo(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole):
                 _phi_d=phi_d, _mask_d=mask_d)
 
 
+# neutral transports without Monte-Carlo neutrals (and so without the coverage warm start)
+_DET_NT = ("radiosity", "knudsen", "dda")
+
+
 class Mesh:
     """Triangle mesh + LBVH on the device (the wp.Mesh of threed.py:1674-1675)."""
 
@@ -160,6 +164,32 @@ def mc_flux_3d_radiosity(mesh, verts, faces, centroids, areas, geo, par, n_ion=2
     else:
         par.pop('_ion_yield', None)
     return tuple(_np(out[k]) for k in ("m_i", "m_F", "m_O", "cos_i"))
+
+
+def _det_flux(fn, mesh, verts, faces, geo, par, seed, flags, **kw):
+    M = _mesh_of(mesh, verts, faces)
+    phi = geo.get('_phi_d')
+    phi = _t(geo['phi']) if phi is None else phi
+    out = fn(M, phi, geo, par, seed=seed, flags=flags, **kw)
+    if out['ion_yield'] is not None:
+        par['_ion_yield'] = tuple(_np(a) for a in out['ion_yield'])
+    else:
+        par.pop('_ion_yield', None)
+    return tuple(_np(out[k]) for k in ("m_i", "m_F", "m_O", "cos_i"))
+
+
+def mc_flux_3d_knudsen(mesh, verts, faces, centroids, areas, geo, par, n_ion=20000, seed=0,
+                       flags=None, n_fp=4):
+    """mc_flux_3d_knudsen (threed.py:1149-1208): returns numpy (m_i, m_F, m_O, cos_i)."""
+    return _det_flux(_flux.knudsen_flux, mesh, verts, faces, geo, par, seed, flags, n_ion=n_ion,
+                     n_fp=n_fp)
+
+
+def mc_flux_3d_dda(mesh, verts, faces, centroids, areas, geo, par, n_ion=20000, seed=0,
+                   flags=None, n_fp=4):
+    """mc_flux_3d_dda (threed.py:1211-1244): returns numpy (m_i, m_F, m_O, cos_i)."""
+    return _det_flux(_flux.dda_flux, mesh, verts, faces, geo, par, seed, flags, n_ion=n_ion,
+                     n_fp=n_fp)
 
 
 def mc_flux_3d(mesh, verts, faces, areas, geo, par, n_ion=20000, n_neu=20000, seed=0,
@@ -318,9 +348,6 @@ class EtchLoop:
         if getattr(flags, "redeposition", False):
             raise NotImplementedError("redeposition (threed.py:1730-1743) is outside the HIP etch loop")
         self.nt = getattr(flags, "neutral_transport", "mc")
-        if self.nt in ("knudsen", "dda"):
-            raise NotImplementedError(f"neutral_transport={self.nt!r} is outside the HIP etch loop "
-                                      "(supported: 'mc', 'radiosity')")
         self.dev = phi_d.device
         self.phi = phi_d
         self.mask = mask_d
@@ -417,7 +444,7 @@ class EtchLoop:
         else:
             self._rays = (self.n_ion, self.n_neu)
         if self.tuner is not None and self.tuned is None and bvh and mesh.nf > 1 and \
-                self.nt != "radiosity" and getattr(self.flags, "coverage_sticking", False):
+                self.nt not in _DET_NT and getattr(self.flags, "coverage_sticking", False):
             from .flux import box_of
             bare = torch.ones((mesh.nf,), dtype=torch.float32, device=self.dev)
             self.tuned = self.tuner.tune(mesh, box_of(self.geo, self.par), bare,
@@ -425,7 +452,7 @@ class EtchLoop:
                                          self._rays[1])
             for m in self.meshes:
                 m.quality = self.tuned[0]
-        if mesh.nf and self.nt != "radiosity" and getattr(self.flags, "coverage_sticking", False) \
+        if mesh.nf and self.nt not in _DET_NT and getattr(self.flags, "coverage_sticking", False) \
                 and self.warm and self.prev_bare is not None:
             # staged: the flag the batched surface build computed (1 iff all centroids finite)
             return mesh.finite if staged else torch.isfinite(mesh.centroids).all()
@@ -435,7 +462,7 @@ class EtchLoop:
         """(bvh, src, points) of this step's warm-start lookup (nearest previous face of every
         new centroid, threed.py:1700-1701) for ops.nearest_face_value_batch, or None when the
         step has no such lookup."""
-        if not finite or not self.gpu_ws or self._mesh.nf == 0 or self.nt == "radiosity" or \
+        if not finite or not self.gpu_ws or self._mesh.nf == 0 or self.nt in _DET_NT or \
                 not getattr(self.flags, "coverage_sticking", False):
             return None
         return (self.prev_mesh.bvh, self.prev_bare, self._mesh.centroids)
@@ -455,8 +482,10 @@ class EtchLoop:
         tf = time.time()
         try:
             with T.region("flux"):
-                if self.nt == "radiosity":
-                    return _flux.radiosity_flux(mesh, self.phi, self.geo, par, n_ion_s, seed, flags)
+                if self.nt in _DET_NT:              # threed.py:1688-1696
+                    fn = dict(radiosity=_flux.radiosity_flux, knudsen=_flux.knudsen_flux,
+                              dda=_flux.dda_flux)[self.nt]
+                    return fn(mesh, self.phi, self.geo, par, n_ion_s, seed, flags)
                 if getattr(flags, "coverage_sticking", False):
                     bi = None
                     if finite and bare_init is not None:
@@ -492,7 +521,7 @@ class EtchLoop:
         T, flags = self.timer, self.flags
         mesh = self._mesh
         if out is not None and getattr(flags, "coverage_sticking", False) and \
-                self.nt != "radiosity":
+                self.nt not in _DET_NT:
             self.prev_mesh, self.prev_bare, self.prev_cen = mesh, out['bare'], mesh.centroids
         self._V = None
         if out is None:
