@@ -201,7 +201,7 @@ def test_profile_area_and_its_adjoint_match_the_oracle():
     loss.backward()
     ro = ol.profile_rz(phi.astype(np.float64), dx, eps)
     lo = float(((ro - target)[zsel] ** 2).sum())
-    assert float(loss) == pytest.approx(lo, rel=1e-10)
+    assert float(loss.detach()) == pytest.approx(lo, rel=1e-10)
     with np.errstate(divide="ignore", invalid="ignore"):
         gr = np.where(zsel & (ro > 0), 2 * (ro - target) / (2 * np.pi * ro), 0.0)
     gref = ol.slice_gas_area_adjoint(phi.astype(np.float64), dx, eps, gr)
