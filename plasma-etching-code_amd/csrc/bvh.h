@@ -526,9 +526,95 @@ __device__ __forceinline__ void point_pass(const BvhView& b, Vec3 p, float lim, 
   }
 }
 
+// Squared tie limit of a minimum squared distance m: faces within tie_band(p) of sqrt(m).
+__device__ __forceinline__ float tie_lim(float m, float tb, float maxd2) {
+  const float r = sqrtf(m) + tb;
+  return fminf(r * r, maxd2);
+}
+
+// The two passes of the query in one traversal: the running minimum m of the squared distances
+// below max_dist^2, and the faces with d2 <= tie_lim(m) in kTieSlots slots. Boxes are visited
+// while their distance is <= tie_lim(m); a face enters while d2 <= tie_lim(m) and its slot is
+// reused once d2 > tie_lim(m) for a smaller m. Every face within tie_lim(final m) is in a box
+// visited at a larger limit and keeps its slot, so the slots end holding exactly the faces the
+// second pass would select. Returns false if a face found no slot (the caller then runs the
+// second pass).
+constexpr int kTieSlots = 8;
+
+__device__ __forceinline__ bool point_pass_ties(const BvhView& b, Vec3 p, float maxd2, float tb,
+                                                float& m, int (&cf)[kTieSlots],
+                                                float (&cd)[kTieSlots]) {
+  bool ok = true;
+  float lim = maxd2;
+  int stack[kBvhStack];
+  int sp = 0;
+  int node = 0;
+  const int max_steps = 2 * b.nf + 64;
+  for (int steps = 0;; ++steps) {
+    if (steps > max_steps || node < 0 || node >= b.nf - 1) {
+      if (b.err) *b.err = 2;
+      break;
+    }
+    const BvhNode* N = b.nodes + node;
+    float4 A = N->a, B = N->b, C = N->c;
+    int4 D = N->d;
+    float dl = box_dist2(p, A.x, A.y, A.z, A.w, B.x, B.y);
+    float dr = box_dist2(p, B.z, B.w, C.x, C.y, C.z, C.w);
+    bool hl = dl <= lim, hr = dr <= lim;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int c = side ? D.y : D.x;
+      bool& hit = side ? hr : hl;
+      if (!hit || c >= 0) continue;
+      hit = false;
+      const int slot = ~c;
+      if (slot < 0 || slot >= b.nf) continue;
+      int face;
+      const float d2 = point_face_d2(b, p, slot, face);
+      if (d2 < m) {
+        m = d2;
+        lim = tie_lim(m, tb, maxd2);
+      }
+      if (d2 <= lim) {
+        bool placed = false;
+#pragma unroll
+        for (int j = 0; j < kTieSlots; ++j) {
+          if (!placed && !(cd[j] <= lim)) {
+            cf[j] = face;
+            cd[j] = d2;
+            placed = true;
+          }
+        }
+        ok = ok && placed;
+      }
+    }
+    if (hl && !(dl <= lim)) hl = false;
+    if (hr && !(dr <= lim)) hr = false;
+    if (hl && hr) {
+      int nearc = (dl <= dr) ? D.x : D.y;
+      int farc = (dl <= dr) ? D.y : D.x;
+      if (sp < kBvhStack) {
+        stack[sp++] = farc;
+      } else if (b.err) {
+        *b.err = 1;
+      }
+      node = nearc;
+    } else if (hl) {
+      node = D.x;
+    } else if (hr) {
+      node = D.y;
+    } else {
+      if (sp == 0) break;
+      node = stack[--sp];
+    }
+  }
+  return ok;
+}
+
 // Closest face within max_dist (distance squared < max_dist^2), ties to the lowest face id.
 __device__ __forceinline__ PointHit bvh_closest_point(const BvhView& b, Vec3 p, float max_dist) {
-  PointHit h{-1, max_dist * max_dist};
+  const float maxd2 = max_dist * max_dist;
+  PointHit h{-1, maxd2};
   if (b.nf <= 0) return h;
   if (b.nf == 1) {
     int face;
@@ -536,13 +622,31 @@ __device__ __forceinline__ PointHit bvh_closest_point(const BvhView& b, Vec3 p, 
     if (d2 < h.d2) h = PointHit{face, d2};
     return h;
   }
-  point_pass(b, p, -1.f, h);
-  if (h.face < 0) return h;
-  const float r = sqrtf(h.d2) + tie_band(p);
-  const float lim = fminf(r * r, max_dist * max_dist);
-  PointHit t{-1, h.d2};
-  point_pass(b, p, lim, t);
-  return t.face >= 0 ? t : h;
+  const float tb = tie_band(p);
+  float m = maxd2;
+  int cf[kTieSlots];
+  float cd[kTieSlots];
+#pragma unroll
+  for (int j = 0; j < kTieSlots; ++j) {
+    cf[j] = -1;
+    cd[j] = INFINITY;
+  }
+  const bool ok = point_pass_ties(b, p, maxd2, tb, m, cf, cd);
+  if (!(m < maxd2)) return h;                 // no face below max_dist
+  const float lim = tie_lim(m, tb, maxd2);
+  PointHit t{-1, m};
+  if (ok) {
+#pragma unroll
+    for (int j = 0; j < kTieSlots; ++j) {
+      if (cd[j] <= lim && (t.face < 0 || cf[j] < t.face)) {
+        t.face = cf[j];
+        t.d2 = cd[j];
+      }
+    }
+  } else {
+    point_pass(b, p, lim, t);
+  }
+  return t;
 }
 
 }  // namespace petch

@@ -28,6 +28,12 @@
 namespace {
 
 constexpr int kMcBS = 256;
+// A workgroup takes a tile of kMcRounds x kMcBS consecutive nodes, one round of kMcBS at a
+// time: fewer, longer workgroups (one per 256 nodes made the passes bound by workgroup
+// dispatch). Vertex and face ids are prefix sums over the nodes in order, so the tile size does
+// not change the output.
+constexpr int kMcRounds = 8;
+constexpr long long kMcTile = (long long)kMcRounds * kMcBS;
 
 __device__ __forceinline__ int edge_base(int e) {
   int a = e >> 2, q = e & 3, lo = q & 1, hi = q >> 1;
@@ -182,16 +188,18 @@ __device__ __forceinline__ void load_cell(const float* __restrict__ phi, const G
 __device__ __forceinline__ void mc_count_block(const float* __restrict__ phi, const Grid& g,
                                                long long N, long long blk, int* __restrict__ bv,
                                                int* __restrict__ bt) {
-  long long n = blk * kMcBS + threadIdx.x;
   int vc = 0, tc = 0;
-  if (n < N) {
-    int i, j, k;
-    node_ijk(g, n, i, j, k);
-    vc = __popc(vertex_flags(phi, g, n, i, j, k));
-    if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
-      float f[8];
-      load_cell(phi, g, n, f);
-      tc = cell_tri_count(f);
+  for (int r = 0; r < kMcRounds; ++r) {
+    const long long n = blk * kMcTile + r * kMcBS + threadIdx.x;
+    if (n < N) {
+      int i, j, k;
+      node_ijk(g, n, i, j, k);
+      vc += __popc(vertex_flags(phi, g, n, i, j, k));
+      if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
+        float f[8];
+        load_cell(phi, g, n, f);
+        tc += cell_tri_count(f);
+      }
     }
   }
   int sv = petch::block_sum<kMcBS>(vc);
@@ -208,22 +216,22 @@ __global__ void __launch_bounds__(kMcBS) k_mc_count(const float* __restrict__ ph
   mc_count_block(phi, g, N, blockIdx.x, bv, bt);
 }
 
-__device__ __forceinline__ void mc_verts_block(const float* __restrict__ phi, const Grid& g,
-                                               long long N, long long blk, double dx,
-                                               const int* __restrict__ bv,
-                                               int* __restrict__ vbase,
-                                               uint8_t* __restrict__ vflg,
-                                               float* __restrict__ verts, int cap_v) {
+// The emit passes skip a block that holds no vertex (no face): its per-node vertex tables are
+// never read, since a face looks up only the start nodes of its crossing edges, which hold that
+// edge's vertex. nb: the grid's blocks; total: the pass's grand total (the end of the last
+// block's range).
+__device__ __forceinline__ bool mc_block_empty(const int* __restrict__ part, long long blk,
+                                               long long nb, const int* __restrict__ total) {
+  return (blk + 1 < nb ? part[blk + 1] : *total) == part[blk];
+}
+
+// One node's vertex-table entries and the vertices on its crossing edges (ids from vb on).
+__device__ __forceinline__ void mc_node_verts(const float* __restrict__ phi, const Grid& g,
+                                              long long n, int i, int j, int k, int fl, int vb,
+                                              double dx, int* __restrict__ vbase,
+                                              uint8_t* __restrict__ vflg,
+                                              float* __restrict__ verts, int cap_v) {
 #pragma clang fp contract(off)
-  long long n = blk * kMcBS + threadIdx.x;
-  int fl = 0, i = 0, j = 0, k = 0;
-  if (n < N) {
-    node_ijk(g, n, i, j, k);
-    fl = vertex_flags(phi, g, n, i, j, k);
-  }
-  int tot;
-  int vb = bv[blk] + petch::block_excl_scan<kMcBS>(__popc(fl), tot);
-  if (n >= N) return;
   vbase[n] = vb;
   vflg[n] = (uint8_t)fl;
   if (!fl) return;
@@ -246,13 +254,38 @@ __device__ __forceinline__ void mc_verts_block(const float* __restrict__ phi, co
   }
 }
 
+__device__ __forceinline__ void mc_verts_block(const float* __restrict__ phi, const Grid& g,
+                                               long long N, long long blk, double dx,
+                                               const int* __restrict__ bv,
+                                               int* __restrict__ vbase,
+                                               uint8_t* __restrict__ vflg,
+                                               float* __restrict__ verts, int cap_v,
+                                               long long nb, const int* __restrict__ total) {
+#pragma clang fp contract(off)
+  if (mc_block_empty(bv, blk, nb, total)) return;
+  int base = bv[blk];
+  for (int r = 0; r < kMcRounds; ++r) {
+    const long long n = blk * kMcTile + r * kMcBS + threadIdx.x;
+    int fl = 0, i = 0, j = 0, k = 0;
+    if (n < N) {
+      node_ijk(g, n, i, j, k);
+      fl = vertex_flags(phi, g, n, i, j, k);
+    }
+    int tot;
+    const int vb = base + petch::block_excl_scan<kMcBS>(__popc(fl), tot);
+    base += tot;
+    if (n < N) mc_node_verts(phi, g, n, i, j, k, fl, vb, dx, vbase, vflg, verts, cap_v);
+  }
+}
+
 __global__ void __launch_bounds__(kMcBS) k_mc_verts(const float* __restrict__ phi, Grid g,
                                                     long long N, double dx,
                                                     const int* __restrict__ bv,
                                                     int* __restrict__ vbase,
                                                     uint8_t* __restrict__ vflg,
-                                                    float* __restrict__ verts, int cap_v) {
-  mc_verts_block(phi, g, N, blockIdx.x, dx, bv, vbase, vflg, verts, cap_v);
+                                                    float* __restrict__ verts, int cap_v,
+                                                    long long nb, const int* __restrict__ total) {
+  mc_verts_block(phi, g, N, blockIdx.x, dx, bv, vbase, vflg, verts, cap_v, nb, total);
 }
 
 // The two cube faces an edge lies on, as bits (2*axis + side) of a 6-bit mask.
@@ -271,26 +304,11 @@ __device__ __forceinline__ int edge_vertex(const Grid& g, long long n, int ce,
   return vbase[base] + __popc(vflg[base] & ((1 << (ce >> 2)) - 1));
 }
 
-__device__ __forceinline__ void mc_faces_block(const float* __restrict__ phi, const Grid& g,
-                                               long long N, long long blk,
-                                               const int* __restrict__ bt,
-                                               const int* __restrict__ vbase,
-                                               const uint8_t* __restrict__ vflg,
-                                               int* __restrict__ faces, int cap_f) {
-  long long n = blk * kMcBS + threadIdx.x;
-  float f[8];
-  int tc = 0;
-  if (n < N) {
-    int i, j, k;
-    node_ijk(g, n, i, j, k);
-    if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
-      load_cell(phi, g, n, f);
-      tc = cell_tri_count(f);
-    }
-  }
-  int tot;
-  int tb = bt[blk] + petch::block_excl_scan<kMcBS>(tc, tot);
-  if (tc == 0) return;
+// The triangles of the cell at node n (f: its corner values; ids from tb on).
+__device__ __forceinline__ void mc_cell_faces(const Grid& g, long long n, const float f[8], int tb,
+                                              const int* __restrict__ vbase,
+                                              const uint8_t* __restrict__ vflg,
+                                              int* __restrict__ faces, int cap_f) {
   int ne;
   unsigned long long nxt = cell_links(f, ne);
   int visited = 0, q = 0;
@@ -339,12 +357,41 @@ __device__ __forceinline__ void mc_faces_block(const float* __restrict__ phi, co
   }
 }
 
+__device__ __forceinline__ void mc_faces_block(const float* __restrict__ phi, const Grid& g,
+                                               long long N, long long blk,
+                                               const int* __restrict__ bt,
+                                               const int* __restrict__ vbase,
+                                               const uint8_t* __restrict__ vflg,
+                                               int* __restrict__ faces, int cap_f, long long nb,
+                                               const int* __restrict__ total) {
+  if (mc_block_empty(bt, blk, nb, total)) return;
+  int base = bt[blk];
+  for (int r = 0; r < kMcRounds; ++r) {
+    const long long n = blk * kMcTile + r * kMcBS + threadIdx.x;
+    float f[8];
+    int tc = 0;
+    if (n < N) {
+      int i, j, k;
+      node_ijk(g, n, i, j, k);
+      if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
+        load_cell(phi, g, n, f);
+        tc = cell_tri_count(f);
+      }
+    }
+    int tot;
+    const int tb = base + petch::block_excl_scan<kMcBS>(tc, tot);
+    base += tot;
+    if (tc) mc_cell_faces(g, n, f, tb, vbase, vflg, faces, cap_f);
+  }
+}
+
 __global__ void __launch_bounds__(kMcBS) k_mc_faces(const float* __restrict__ phi, Grid g,
                                                     long long N, const int* __restrict__ bt,
                                                     const int* __restrict__ vbase,
                                                     const uint8_t* __restrict__ vflg,
-                                                    int* __restrict__ faces, int cap_f) {
-  mc_faces_block(phi, g, N, blockIdx.x, bt, vbase, vflg, faces, cap_f);
+                                                    int* __restrict__ faces, int cap_f,
+                                                    long long nb, const int* __restrict__ total) {
+  mc_faces_block(phi, g, N, blockIdx.x, bt, vbase, vflg, faces, cap_f, nb, total);
 }
 
 // The passes of many grids (the features of a batched step), one launch each: grid t owns the
@@ -405,12 +452,13 @@ __global__ void __launch_bounds__(1024) k_mc_scan_many(const McGrid* __restrict_
 __global__ void __launch_bounds__(kMcBS) k_mc_verts_many(const McGrid* __restrict__ t, int n) {
   const McGrid& m = t[mc_grid_of(t, n)];
   mc_verts_block(m.phi, m.g, m.N, blockIdx.x - m.blk0, m.dx, m.bv, m.vbase, m.vflg, m.verts,
-                 m.cap_v);
+                 m.cap_v, m.nb, m.totals);
 }
 
 __global__ void __launch_bounds__(kMcBS) k_mc_faces_many(const McGrid* __restrict__ t, int n) {
   const McGrid& m = t[mc_grid_of(t, n)];
-  mc_faces_block(m.phi, m.g, m.N, blockIdx.x - m.blk0, m.bt, m.vbase, m.vflg, m.faces, m.cap_f);
+  mc_faces_block(m.phi, m.g, m.N, blockIdx.x - m.blk0, m.bt, m.vbase, m.vflg, m.faces, m.cap_f,
+                 m.nb, m.totals + 1);
 }
 
 }  // namespace
@@ -427,7 +475,7 @@ extern "C" int petch_marching_cubes(petch_ctx* ctx, const float* phi, int nx, in
   long long N = (long long)nx * ny * nz;
   PETCH_ARG(ctx, N <= 0xffffffffLL, "marching_cubes: %lld nodes exceed 32-bit node indices", N);
   Grid g = make_grid(nx, ny, nz);
-  long long nb = (N + kMcBS - 1) / kMcBS;
+  long long nb = (N + kMcTile - 1) / kMcTile;
   int* blk = (int*)petch_scratch(ctx, SCR_MC_BLOCK, (size_t)(2 * nb + 2) * sizeof(int));
   if (!blk) return PETCH_E_HIP;
   int* bv = blk;
@@ -456,10 +504,10 @@ extern "C" int petch_marching_cubes(petch_ctx* ctx, const float* phi, int nx, in
   int* vbase = (int*)cell;
   uint8_t* vflg = cell + 4 * N;
   hipLaunchKernelGGL(k_mc_verts, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, phi, g, N, dx,
-                     bv, vbase, vflg, verts, cap_v);
+                     bv, vbase, vflg, verts, cap_v, nb, totals);
   PETCH_LAUNCH_CHECK(ctx);
   hipLaunchKernelGGL(k_mc_faces, dim3((unsigned)nb), dim3(kMcBS), 0, ctx->stream, phi, g, N, bt,
-                     vbase, vflg, faces, cap_f);
+                     vbase, vflg, faces, cap_f, nb, totals + 1);
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
@@ -486,7 +534,7 @@ extern "C" int petch_marching_cubes_batch(petch_ctx* ctx, int n, const petch_mc_
     m.N = (long long)a.nx * a.ny * a.nz;
     PETCH_ARG(ctx, m.N <= 0xffffffffLL,
               "marching_cubes_batch: item %d: %lld nodes exceed 32-bit node indices", f, m.N);
-    m.nb = (m.N + kMcBS - 1) / kMcBS;
+    m.nb = (m.N + kMcTile - 1) / kMcTile;
     m.blk0 = blocks;
     m.dx = a.dx;
     m.verts = a.verts;
