@@ -7,6 +7,7 @@
 // z-nodes and the +-x / +-y neighbours are 64 consecutive words too: every load is coalesced
 // and the three neighbour planes are re-read from L2/MALL, not HBM.
 #include "common.h"
+#include "fastdiv.h"
 #include <math.h>
 #include <stdlib.h>
 
@@ -70,21 +71,30 @@ __global__ void k_init_sdf(float* __restrict__ phi, uint8_t* __restrict__ mask, 
   phi[n] = (float)(solid ? d : -d);
 }
 
+// Node n of an (nx, ny, nz) grid as (i, j, k), by 32-bit division by the invariant ny and nz
+// (node indices of one grid fit 32 bits; the entry points check).
+__device__ __forceinline__ void split_node(long long n, const petch::FastDiv& fy,
+                                           const petch::FastDiv& fz, int& i, int& j, int& k) {
+  const uint32_t u = (uint32_t)n;
+  const uint32_t c = petch::fastdiv(u, fz);
+  k = (int)(u - c * fz.d);
+  i = (int)petch::fastdiv(c, fy);
+  j = (int)(c - (uint32_t)i * fy.d);
+}
+
 // ------------------------------------------------------------------------------ advection
-// One upwind substep at node n of an (nx, ny, nz) grid.
+// One upwind substep at node n = (i, j, k) of an (nx, ny, nz) grid.
 __device__ __forceinline__ void advect_node(const float* __restrict__ phi,
                                             const float* __restrict__ F,
                                             const uint8_t* __restrict__ mask,
                                             const float* __restrict__ mask_phi,
                                             float* __restrict__ out, int nx, int ny, int nz,
-                                            float inv_dx, float dt, long long n) {
+                                            float inv_dx, float dt, long long n, int i, int j,
+                                            int k) {
   if (mask[n]) {
     out[n] = mask_phi[n];
     return;
   }
-  int k = (int)(n % nz);
-  int j = (int)((n / nz) % ny);
-  int i = (int)(n / ((long long)nz * ny));
   long long sx = (long long)ny * nz, sy = nz;
   float c = phi[n];
   float g = 0.f, dm, dp;
@@ -103,11 +113,13 @@ __device__ __forceinline__ void advect_node(const float* __restrict__ phi,
 __global__ void k_advect(const float* __restrict__ phi, const float* __restrict__ F,
                          const uint8_t* __restrict__ mask, const float* __restrict__ mask_phi,
                          float* __restrict__ out, int nx, int ny, int nz, float inv_dx,
-                         float dt) {
+                         float dt, petch::FastDiv fy, petch::FastDiv fz) {
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long N = (long long)nx * ny * nz;
   if (n >= N) return;
-  advect_node(phi, F, mask, mask_phi, out, nx, ny, nz, inv_dx, dt, n);
+  int i, j, k;
+  split_node(n, fy, fz, i, j, k);
+  advect_node(phi, F, mask, mask_phi, out, nx, ny, nz, inv_dx, dt, n, i, j, k);
 }
 
 // ------------------------------------------------------------------------------ advection adjoint
@@ -232,10 +244,8 @@ __device__ __forceinline__ float eik_solve(float f1, float f2, float f3, double 
 __device__ __forceinline__ void eik_init_node(const float* __restrict__ phi0,
                                               float* __restrict__ U,
                                               uint8_t* __restrict__ frozen, int nx, int ny,
-                                              int nz, double dx, float big, long long n) {
-  int k = (int)(n % nz);
-  int j = (int)((n / nz) % ny);
-  int i = (int)(n / ((long long)nz * ny));
+                                              int nz, double dx, float big, long long n, int i,
+                                              int j, int k) {
   long long sx = (long long)ny * nz, sy = nz;
   float c = phi0[n];
   float oxm = phi0[i > 0 ? n - sx : n], oxp = phi0[i < nx - 1 ? n + sx : n];
@@ -259,25 +269,25 @@ __device__ __forceinline__ void eik_init_node(const float* __restrict__ phi0,
 
 __global__ void k_eik_init(const float* __restrict__ phi0, float* __restrict__ U,
                            uint8_t* __restrict__ frozen, int nx, int ny, int nz, double dx,
-                           float big) {
+                           float big, petch::FastDiv fy, petch::FastDiv fz) {
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long N = (long long)nx * ny * nz;
   if (n >= N) return;
-  eik_init_node(phi0, U, frozen, nx, ny, nz, dx, big, n);
+  int i, j, k;
+  split_node(n, fy, fz, i, j, k);
+  eik_init_node(phi0, U, frozen, nx, ny, nz, dx, big, n, i, j, k);
 }
 
 __device__ __forceinline__ void eik_jacobi_node(const float* __restrict__ U,
                                                 const uint8_t* __restrict__ frozen,
                                                 float* __restrict__ out, int nx, int ny, int nz,
-                                                double dx, float big, long long n) {
+                                                double dx, float big, long long n, int i,
+                                                int j, int k) {
   float u = U[n];
   if (frozen[n]) {
     out[n] = u;
     return;
   }
-  int k = (int)(n % nz);
-  int j = (int)((n / nz) % ny);
-  int i = (int)(n / ((long long)nz * ny));
   long long sx = (long long)ny * nz, sy = nz;
   float ax = fminf(i >= 1 ? U[n - sx] : big, i <= nx - 2 ? U[n + sx] : big);
   float ay = fminf(j >= 1 ? U[n - sy] : big, j <= ny - 2 ? U[n + sy] : big);
@@ -287,11 +297,13 @@ __device__ __forceinline__ void eik_jacobi_node(const float* __restrict__ U,
 
 __global__ void k_eik_jacobi(const float* __restrict__ U, const uint8_t* __restrict__ frozen,
                              float* __restrict__ out, int nx, int ny, int nz, double dx,
-                             float big) {
+                             float big, petch::FastDiv fy, petch::FastDiv fz) {
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long N = (long long)nx * ny * nz;
   if (n >= N) return;
-  eik_jacobi_node(U, frozen, out, nx, ny, nz, dx, big, n);
+  int i, j, k;
+  split_node(n, fy, fz, i, j, k);
+  eik_jacobi_node(U, frozen, out, nx, ny, nz, dx, big, n, i, j, k);
 }
 
 // phi <- sign(phi0) * min(U, clampv), sign(0) = +1, NaN propagates.
@@ -330,6 +342,7 @@ struct LsGrid {
   uint8_t* bfl;              // per-brick activity flags (one plane, or count + 1 for the reinit)
   int nx, ny, nz;
   int bnx, bny, bnz;         // bricks per axis
+  petch::FastDiv fy, fz, fby, fbz;   // division by ny, nz, bny, bnz
   int count;                 // advection substeps or Jacobi sweeps
   double dx;
   float inv_dx, dt, big, clampv;
@@ -369,17 +382,25 @@ __device__ __forceinline__ long long nbricks(const LsGrid& G) {
 }
 
 __device__ __forceinline__ long long brick_of_node(const LsGrid& G, long long n) {
-  const int k = (int)(n % G.nz);
-  const int j = (int)((n / G.nz) % G.ny);
-  const int i = (int)(n / ((long long)G.nz * G.ny));
+  int i, j, k;
+  split_node(n, G.fy, G.fz, i, j, k);
   return ((long long)(i / kBX) * G.bny + j / kBY) * G.bnz + k / kBZ;
 }
 
-__device__ __forceinline__ long long brick_node(const LsGrid& G, long long b, int t, bool& valid) {
-  const int bk = (int)(b % G.bnz);
-  const long long r = b / G.bnz;
-  const int bj = (int)(r % G.bny), bi = (int)(r / G.bny);
-  const int i = bi * kBX + t / (kBY * kBZ), j = bj * kBY + (t / kBZ) % kBY, k = bk * kBZ + t % kBZ;
+// brick b as (bi, bj, bk)
+__device__ __forceinline__ void split_brick(const LsGrid& G, long long b, int& bi, int& bj,
+                                            int& bk) {
+  split_node(b, G.fby, G.fbz, bi, bj, bk);
+}
+
+// node of lane t in brick b, with its (i, j, k)
+__device__ __forceinline__ long long brick_node(const LsGrid& G, long long b, int t, bool& valid,
+                                                int& i, int& j, int& k) {
+  int bi, bj, bk;
+  split_brick(G, b, bi, bj, bk);
+  i = bi * kBX + t / (kBY * kBZ);
+  j = bj * kBY + (t / kBZ) % kBY;
+  k = bk * kBZ + t % kBZ;
   valid = i < G.nx && j < G.ny && k < G.nz;
   return ((long long)i * G.ny + j) * G.nz + k;
 }
@@ -387,9 +408,8 @@ __device__ __forceinline__ long long brick_node(const LsGrid& G, long long b, in
 // any of the flags of brick b and of its face neighbours in the flag plane f
 __device__ __forceinline__ bool brick_or_nbr(const LsGrid& G, const uint8_t* __restrict__ f,
                                              long long b) {
-  const int bk = (int)(b % G.bnz);
-  const long long r = b / G.bnz;
-  const int bj = (int)(r % G.bny), bi = (int)(r / G.bny);
+  int bi, bj, bk;
+  split_brick(G, b, bi, bj, bk);
   const long long sx = (long long)G.bny * G.bnz, sy = G.bnz;
   return f[b] || (bk > 0 && f[b - 1]) || (bk < G.bnz - 1 && f[b + 1]) ||
          (bj > 0 && f[b - sy]) || (bj < G.bny - 1 && f[b + sy]) || (bi > 0 && f[b - sx]) ||
@@ -460,14 +480,16 @@ __global__ void k_advect_bricks(const LsGrid* __restrict__ grids, const int2* __
     const LsGrid& G = grids[q.x];
     if (s >= G.count) continue;
     bool valid;
-    const long long n = brick_node(G, q.y & (kHalo - 1), threadIdx.x, valid);
+    int i, j, k;
+    const long long n = brick_node(G, q.y & (kHalo - 1), threadIdx.x, valid, i, j, k);
     if (!valid) continue;
     const float* src = (s & 1) ? G.work : G.phi;
     float* dst = (s & 1) ? G.phi : G.work;
     if (q.y & kHalo)
       dst[n] = src[n];
     else
-      advect_node(src, G.F, G.mask, G.mask_phi, dst, G.nx, G.ny, G.nz, G.inv_dx, G.dt, n);
+      advect_node(src, G.F, G.mask, G.mask_phi, dst, G.nx, G.ny, G.nz, G.inv_dx, G.dt, n, i, j,
+                  k);
   }
 }
 
@@ -481,7 +503,8 @@ __global__ void k_advect_bricks_copyback(const LsGrid* __restrict__ grids,
     const LsGrid& G = grids[q.x];
     if (!(G.count & 1)) continue;
     bool valid;
-    const long long n = brick_node(G, q.y & (kHalo - 1), threadIdx.x, valid);
+    int i, j, k;
+    const long long n = brick_node(G, q.y & (kHalo - 1), threadIdx.x, valid, i, j, k);
     if (valid) G.phi[n] = G.work[n];
   }
 }
@@ -497,9 +520,11 @@ __global__ void k_eik_init_grids(const LsGrid* __restrict__ grids, int n_grids) 
   const LsGrid& G = grids[grid_of_nodes(grids, n_grids)];
   const long long n = (blockIdx.x - G.nblk0) * (long long)kBS + threadIdx.x;
   if (n >= (long long)G.nx * G.ny * G.nz) return;
-  eik_init_node(G.phi, G.U, G.fr, G.nx, G.ny, G.nz, G.dx, G.big, n);
+  int i, j, k;
+  split_node(n, G.fy, G.fz, i, j, k);
+  eik_init_node(G.phi, G.U, G.fr, G.nx, G.ny, G.nz, G.dx, G.big, n, i, j, k);
   G.U2[n] = G.U[n];
-  if (G.fr[n]) G.bfl[brick_of_node(G, n)] = 1;
+  if (G.fr[n]) G.bfl[((long long)(i / kBX) * G.bny + j / kBY) * G.bnz + k / kBZ] = 1;
 }
 
 __global__ void k_eik_list(const LsGrid* __restrict__ grids, int n_grids, long long total, int it,
@@ -526,13 +551,14 @@ __global__ void k_eik_jacobi_bricks(const LsGrid* __restrict__ grids,
     const int2 q = list[e];
     const LsGrid& G = grids[q.x];
     bool valid;
-    const long long n = brick_node(G, q.y, threadIdx.x, valid);
+    int i, j, k;
+    const long long n = brick_node(G, q.y, threadIdx.x, valid, i, j, k);
     int ch = 0;
     if (valid) {
       const float* src = (it & 1) ? G.U2 : G.U;
       float* dst = (it & 1) ? G.U : G.U2;
       const float u = src[n];
-      eik_jacobi_node(src, G.fr, dst, G.nx, G.ny, G.nz, G.dx, G.big, n);
+      eik_jacobi_node(src, G.fr, dst, G.nx, G.ny, G.nz, G.dx, G.big, n, i, j, k);
       ch = dst[n] < u;
     }
     ch = __syncthreads_or(ch);
@@ -572,13 +598,14 @@ extern "C" int petch_advect(petch_ctx* ctx, float* phi, float* work, const float
   PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "advect: bad shape %d,%d,%d", nx, ny, nz);
   PETCH_ARG(ctx, nsub >= 0, "advect: nsub must be >= 0");
   long long N = (long long)nx * ny * nz;
+  PETCH_ARG(ctx, N <= 0xffffffffLL, "advect: %lld nodes exceed 32-bit node indices", N);
   float inv = (float)(1.0 / dx);
   float fdt = (float)dt;
   float* a = phi;
   float* b = work;
   for (int s = 0; s < nsub; ++s) {
     hipLaunchKernelGGL(k_advect, dim3(petch_blocks(N, kBS)), dim3(kBS), 0, ctx->stream, a, F,
-                       mask, mask_phi, b, nx, ny, nz, inv, fdt);
+                       mask, mask_phi, b, nx, ny, nz, inv, fdt, petch::make_fastdiv((uint32_t)ny), petch::make_fastdiv((uint32_t)nz));
     PETCH_LAUNCH_CHECK(ctx);
     float* tmp = a; a = b; b = tmp;
   }
@@ -594,8 +621,9 @@ extern "C" int petch_advect_step(petch_ctx* ctx, const float* phi, const float* 
   PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "advect_step: bad shape %d,%d,%d", nx, ny, nz);
   PETCH_ARG(ctx, out != phi, "advect_step: out must not alias phi");
   const long long N = (long long)nx * ny * nz;
+  PETCH_ARG(ctx, N <= 0xffffffffLL, "advect_step: %lld nodes exceed 32-bit node indices", N);
   hipLaunchKernelGGL(k_advect, dim3(petch_blocks(N, kBS)), dim3(kBS), 0, ctx->stream, phi, F,
-                     mask, mask_phi, out, nx, ny, nz, (float)(1.0 / dx), (float)dt);
+                     mask, mask_phi, out, nx, ny, nz, (float)(1.0 / dx), (float)dt, petch::make_fastdiv((uint32_t)ny), petch::make_fastdiv((uint32_t)nz));
   PETCH_LAUNCH_CHECK(ctx);
   return PETCH_OK;
 }
@@ -611,10 +639,15 @@ int reinit_launch(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
   const double big = band + 4.0 * dx;
   if (n_iter < 0) n_iter = (int)ceil(band / dx) + 4;
   dim3 g(petch_blocks(N, kBS)), b(kBS);
-  hipLaunchKernelGGL(k_eik_init, g, b, 0, ctx->stream, phi, U, fr, nx, ny, nz, dx, (float)big);
+  if (N > 0xffffffffLL)
+    return petch_set_error(ctx, PETCH_E_ARG, "reinit: %lld nodes exceed 32-bit node indices", N);
+  const petch::FastDiv fy = petch::make_fastdiv((uint32_t)ny), fz = petch::make_fastdiv((uint32_t)nz);
+  hipLaunchKernelGGL(k_eik_init, g, b, 0, ctx->stream, phi, U, fr, nx, ny, nz, dx, (float)big,
+                     fy, fz);
   PETCH_LAUNCH_CHECK(ctx);
   for (int it = 0; it < n_iter; ++it) {
-    hipLaunchKernelGGL(k_eik_jacobi, g, b, 0, ctx->stream, U, fr, U2, nx, ny, nz, dx, (float)big);
+    hipLaunchKernelGGL(k_eik_jacobi, g, b, 0, ctx->stream, U, fr, U2, nx, ny, nz, dx, (float)big,
+                       fy, fz);
     PETCH_LAUNCH_CHECK(ctx);
     float* t = U; U = U2; U2 = t;
   }
@@ -720,6 +753,11 @@ long long assign_grids(LsGrid* g, int n, long long* node_blocks) {
     g[f].bnx = (g[f].nx + kBX - 1) / kBX;
     g[f].bny = (g[f].ny + kBY - 1) / kBY;
     g[f].bnz = (g[f].nz + kBZ - 1) / kBZ;
+    g[f].fy = petch::make_fastdiv((uint32_t)g[f].ny);
+    g[f].fz = petch::make_fastdiv((uint32_t)g[f].nz);
+    g[f].fby = petch::make_fastdiv((uint32_t)g[f].bny);
+    g[f].fbz = petch::make_fastdiv((uint32_t)g[f].bnz);
+    if ((long long)g[f].nx * g[f].ny * g[f].nz > 0xffffffffLL) return 0;
     g[f].nblk0 = nb;
     nb += (long long)petch_blocks((long long)g[f].nx * g[f].ny * g[f].nz, kBS);
     g[f].bblk0 = b;
@@ -1104,12 +1142,14 @@ extern "C" int petch_reinit_fsm_adjoint(petch_ctx* ctx, const float* phi0, int n
   float* Gb = (float*)(G + slab);
   dim3 g(petch_blocks(N, kBS)), b(kBS);
   // the forward's states, with the forward's kernels
+  PETCH_ARG(ctx, N <= 0xffffffffLL, "reinit_fsm_adjoint: %lld nodes exceed 32-bit node indices", N);
+  const petch::FastDiv fy = petch::make_fastdiv((uint32_t)ny), fz = petch::make_fastdiv((uint32_t)nz);
   hipLaunchKernelGGL(k_eik_init, g, b, 0, ctx->stream, phi0, state(0), fr, nx, ny, nz, dx,
-                     (float)big);
+                     (float)big, fy, fz);
   PETCH_LAUNCH_CHECK(ctx);
   for (int it = 0; it < n_iter; ++it) {
     hipLaunchKernelGGL(k_eik_jacobi, g, b, 0, ctx->stream, state(it), fr, state(it + 1), nx, ny,
-                       nz, dx, (float)big);
+                       nz, dx, (float)big, fy, fz);
     PETCH_LAUNCH_CHECK(ctx);
   }
   hipLaunchKernelGGL(k_eik_adj_final, g, b, 0, ctx->stream, phi0, state(n_iter), gout, Ga, N,
