@@ -11,6 +11,7 @@ import math
 
 import torch
 
+from . import charging as _charging
 from . import ops
 from .chemistry import chem_params
 from .profiling import NULL as _NULL_TIMER
@@ -107,19 +108,11 @@ def _faithful(flags):
             and getattr(flags, "chemistry", "langmuir") == "belen")
 
 
-def _check_unsupported(par, flags):
-    if flags is not None and getattr(flags, "surface_charging", False) and \
-            float(par.get('charge_alpha', 0.0)) > 0.0:
-        raise NotImplementedError("surface charging (threed.py:847-868) is outside the HIP "
-                                  "etch-loop path")
-    if flags is not None and getattr(flags, "floor_charge_throttle", False):
-        raise NotImplementedError("floor_charge_throttle is outside the HIP etch-loop path")
-
-
 NEUTRAL_SOURCE = ops.make_source(ops.SRC_NEUTRAL)
 
 
-def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, rounds=None):
+def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, rounds=None,
+                       phi=None):
     """mc_flux_3d_coupled (threed.py:740-905) as a generator over its ray traces.
 
     It yields ("ion", request) and ("neutral", request) dicts and is sent the trace outputs
@@ -134,8 +127,9 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
     an ops.finish_round_batch item) with their output tensors allocated, instead of the generator
     launching them; the driver must run the collected items before the next trace it launches and
     before it reads the result. Used when smoothing takes at most one pass (the default
-    flux_smooth = 1)."""
-    _check_unsupported(par, flags)
+    flux_smooth = 1) and no charging hook needs the ion arrival at once.
+
+    phi: the level set on the device, needed by the floor-charge throttle (its gas normals)."""
     F = mesh.nf
     Lx, Ly = geo['Lx'], geo['Ly']
     A_src = Lx * Ly
@@ -162,7 +156,7 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
         acc = yield ("ion", dict(bvh=mesh.bvh, box=box, src=ion_source(par), ion=ion_params(par),
                                  n=n_ion, seed=ion_seed))
         norm = A_src / n_ion
-        if rounds is not None and n_smooth <= 1:
+        if rounds is not None and n_smooth <= 1 and not _charging.active(par, flags):
             out = torch.empty_like(acc)
             rounds.append(dict(kind="ion", acc=acc, areas=A, nbr=mesh.nbr, normals=mesh.normals,
                                scale=norm, n_smooth=max(n_smooth, 0), alpha=alpha, out=out))
@@ -178,6 +172,18 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
         ion_yield = None
         cos_i = ops.cos_ratio(acc[0], acc[1])
         m_i = finish(acc[0], A_src / n_ion, 1.5)
+
+    # surface charging, alpha model (threed.py:842-868)
+    if flags is not None and getattr(flags, "surface_charging", False) and \
+            float(par.get('charge_alpha', 0.0)) > 0.0:
+        m_i = m_i * _charging.electron_factor(mesh, box, A_src, par, m_i, n_neu, seed,
+                                              host_smooth)
+    # the deep-AR floor-charge throttle (threed.py:893-894)
+    if flags is not None and getattr(flags, "floor_charge_throttle", False):
+        if phi is None:
+            raise ValueError("floor_charge_throttle needs the level set (phi) for the gas normals")
+        gn = ops.gas_normals(phi, geo['dx'], mesh.centroids, mesh.normals)
+        m_i, ion_yield = _charging.floor_throttle(mesh, gn, geo, m_i, ion_yield)
 
     if par.get('cov_transport', 'rr') != 'rr':
         raise NotImplementedError("cov_transport='cap' (_trace3d_cov) is not part of the HIP path")
@@ -229,13 +235,15 @@ def run_traces(steps, timer=_NULL_TIMER):
         return stop.value
 
 
-def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, timer=_NULL_TIMER):
+def coupled_flux(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, timer=_NULL_TIMER,
+                 phi=None):
     """mc_flux_3d_coupled (threed.py:740-905) on the device, one feature.
 
     Returns dict(m_i, m_F, m_O, cos_i, bare, ion_yield) of (F,) float32 device tensors;
-    ion_yield = (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None."""
-    return run_traces(coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init),
-                      timer)
+    ion_yield = (sp, ie, ip) normalised (the par['_ion_yield'] side channel) or None. phi: the
+    level set on the device (the floor-charge throttle needs it)."""
+    return run_traces(coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init,
+                                         phi=phi), timer)
 
 
 def deterministic_ions(mesh, geo, par, n_ion, seed, flags):
@@ -256,12 +264,16 @@ def deterministic_ions(mesh, geo, par, n_ion, seed, flags):
     return ops.normalize(acc[0], A, A_src / n_ion, 0.0, 1.5), ops.cos_ratio(acc[0], acc[1]), None
 
 
-def _deterministic_neutrals(mesh, phi, geo, par, n_ion, seed, flags, n_fp, neutrals):
+def _deterministic_neutrals(mesh, phi, geo, par, n_ion, seed, flags, n_fp, neutrals,
+                            ions=None):
     """The coverage loop shared by the deterministic-neutral transports (threed.py:1149-1244):
-    the deterministic ion launch, then n_fp rounds of {m_F, m_O = neutrals(bare); Belen
-    coverages}. neutrals(bare) returns the two species' multipliers."""
+    the deterministic ion launch (then ions(m_i, ion_yield), when given), then n_fp rounds of
+    {m_F, m_O = neutrals(bare); Belen coverages}. neutrals(bare) returns the two species'
+    multipliers."""
     F = mesh.nf
     m_i, cos_i, ion_yield = deterministic_ions(mesh, geo, par, n_ion, seed * 9 + 1, flags)
+    if ions is not None:
+        m_i, ion_yield = ions(m_i, ion_yield)
     chem = chem_params(par, flags)
     bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
     m_F = m_O = None
@@ -277,9 +289,6 @@ def knudsen_flux(mesh, phi, geo, par, n_ion, seed, flags, n_fp=4):
     """mc_flux_3d_knudsen (threed.py:1149-1208): Monte-Carlo ions, neutrals from the 1-D
     Knudsen conductance tail down the feature (knudsen.py) with the floor sink at the local
     coverage, or clamped at the open field's mean coverage (par['knudsen_sink'] = 'field')."""
-    if getattr(flags, "surface_charging", False) in ("hg", "petch") and _faithful(flags):
-        raise NotImplementedError("the Hwang-Giapis charging hook (threed.py:1046-1147) is outside "
-                                  "the HIP etch-loop path")
     dx = geo['dx']
     gn = ops.gas_normals(phi, dx, mesh.centroids, mesh.normals)
     betaE, betaO = par.get('betaE', 0.7), par.get('betaO', 1.0)
@@ -299,7 +308,11 @@ def knudsen_flux(mesh, phi, geo, par, n_ion, seed, flags, n_fp=4):
     def neutrals(bare):
         return ops.knudsen_face_flux(phi, dx, geo['sub_top'], trench, mesh.centroids, gn,
                                      mesh.areas, bare, (betaE, betaO), **kw)
-    return _deterministic_neutrals(mesh, phi, geo, par, n_ion, seed, flags, n_fp, neutrals)
+
+    def ions(m_i, ion_yield):                       # threed.py:1182 (surface_charging='hg')
+        return _charging.hg_closure(mesh, gn, geo, par, flags, m_i, ion_yield)
+    return _deterministic_neutrals(mesh, phi, geo, par, n_ion, seed, flags, n_fp, neutrals,
+                                   ions)
 
 
 def dda_flux(mesh, phi, geo, par, n_ion, seed, flags, n_fp=4):
@@ -320,7 +333,6 @@ def dda_flux(mesh, phi, geo, par, n_ion, seed, flags, n_fp=4):
 def radiosity_flux(mesh, phi, geo, par, n_ion, seed, flags, n_ff=64, n_fp=4):
     """mc_flux_3d_radiosity (threed.py:923-991): form factors by n_ff first-hit cosine rays per
     face, then n_fp x {G = D + Aff diag(1-s) G solved for F and O, Belen coverages}."""
-    _check_unsupported(par, flags)
     F = mesh.nf
     gn = ops.gas_normals(phi, geo['dx'], mesh.centroids, mesh.normals)
     m_i, cos_i, ion_yield = deterministic_ions(mesh, geo, par, n_ion, seed * 9 + 1, flags)

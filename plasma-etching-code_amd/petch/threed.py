@@ -144,7 +144,10 @@ def mc_flux_3d_coupled(mesh, verts, faces, areas, geo, par, n_ion=20000, n_neu=2
     the device (the reference's gpu_source path); `sampling` is accepted and unused."""
     M = _mesh_of(mesh, verts, faces)
     bi = None if bare_init is None else _t(np.asarray(bare_init, np.float64))
-    out = _flux.coupled_flux(M, geo, par, n_ion, n_neu, seed, flags, bare_init=bi)
+    phi = geo.get('_phi_d')
+    if phi is None and 'phi' in geo:
+        phi = _t(geo['phi'])
+    out = _flux.coupled_flux(M, geo, par, n_ion, n_neu, seed, flags, bare_init=bi, phi=phi)
     if out['ion_yield'] is not None:
         par['_ion_yield'] = tuple(_np(a) for a in out['ion_yield'])
     else:
@@ -501,7 +504,8 @@ class EtchLoop:
                                                         mesh.centroids, 1.0)
                     return _FluxSteps(_flux.coupled_flux_steps(mesh, self.geo, par, n_ion_s,
                                                                n_neu_s, seed, flags,
-                                                               bare_init=bi, rounds=rounds))
+                                                               bare_init=bi, rounds=rounds,
+                                                               phi=self.phi))
                 return _flux.uncoupled_flux(mesh, self.geo, self.mc_par, n_ion_s, n_neu_s, seed,
                                             getattr(flags, "ion_reflection", False))
         finally:

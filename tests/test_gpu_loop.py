@@ -206,7 +206,6 @@ def test_api_hole_runs_and_saves(tmp_path):
 @pytest.mark.parametrize("kw,flag_kw", [
     (dict(surf_smooth=0.5), {}),
     ({}, dict(redeposition=True)),
-    ({}, dict(neutral_transport="knudsen", surface_charging="hg")),
     (dict(reinit_method="gpu"), {}),
 ])
 def test_out_of_path_options_raise(kw, flag_kw):
