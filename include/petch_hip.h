@@ -139,6 +139,14 @@ typedef struct petch_advect_item {
 int petch_advect_batch(petch_ctx* ctx, int n, const petch_advect_item* items_host);
 
 /* ---------------------------------------------------------------- a20: FSM reinit -------- */
+/* gaussian_filter(phi, sigma) of run_etch_3d's surf_smooth (threed.py:1779-1782, scipy.ndimage,
+ * mode 'reflect'), in place: three axis passes (x, y, z) of the 2 radius + 1 weights (host
+ * array, the normalised exp(-x^2 / 2 sigma^2) as scipy forms them), each in float64 stored as
+ * float32; then nodes with mask[n] != 0 take mask_phi[n] (mask may be NULL). */
+int petch_gaussian_smooth(petch_ctx* ctx, float* phi, int nx, int ny, int nz,
+                          const double* weights_host, int radius, const uint8_t* mask,
+                          const float* mask_phi);
+
 /* reinit_fsm (threed.py:1515-1537) with _eik_init / _eik_jacobi / _eik_solve
  * (threed.py:1421-1512): big = band+4dx, n_iter = ceil(band/dx)+4 when n_iter < 0, output
  * sign(phi0)*min(U, band+dx) with sign(0) = +1. phi is replaced in place. */

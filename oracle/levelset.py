@@ -393,3 +393,33 @@ def slice_gas_area_adjoint(phi, dx, eps, garea):
 def profile_rz(phi, dx, eps):
     """Equivalent radius sqrt(A(z) / pi) of every z slice."""
     return np.sqrt(np.maximum(slice_gas_area(phi, dx, eps), 0.0) / np.pi)
+
+
+def gaussian_smooth(phi, sigma, truncate=4.0):
+    """scipy.ndimage.gaussian_filter(phi, sigma) as run_etch_3d's surf_smooth calls it
+    (threed.py:1779-1782): weights exp(-x^2 / (2 sigma^2)) over |x| <= int(truncate sigma + 0.5),
+    normalised; one axis after the other (x, y, z), each pass in float64 and stored in the
+    input's dtype; boundaries 'reflect' (d c b a | a b c d | d c b a); the symmetric taps summed
+    as w0 x0 + sum over j = r..1 of wj (x[-j] + x[+j]). Pinned bit for bit against scipy
+    (tests/test_oracle_levelset.py)."""
+    a = np.array(phi)
+    r = int(truncate * float(sigma) + 0.5)
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 / (float(sigma) ** 2) * x ** 2)
+    w = w / w.sum()
+    for ax in range(a.ndim):
+        n = a.shape[ax]
+        src = np.moveaxis(a, ax, -1).astype(np.float64)
+
+        def at(k):
+            k = np.arange(n) + k
+            while True:                            # reflect until every index is inside
+                lo, hi = k < 0, k >= n
+                if not (lo.any() or hi.any()):
+                    return src[..., k]
+                k = np.where(lo, -k - 1, np.where(hi, 2 * n - k - 1, k))
+        acc = w[r] * src
+        for j in range(r, 0, -1):
+            acc = acc + (at(-j) + at(j)) * w[r + j]
+        a = np.moveaxis(acc.astype(a.dtype), -1, ax)
+    return a

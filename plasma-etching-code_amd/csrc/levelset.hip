@@ -323,6 +323,38 @@ __global__ void k_eik_final(float* __restrict__ phi, const float* __restrict__ U
   eik_final_node(phi, U, clampv, n);
 }
 
+// One axis pass of the separable Gaussian of surf_smooth: out = sum over |j| <= r of
+// w[r + j] in[reflect(i + j)] along `axis`, in float64 as w[r] x0 + sum_{j=r..1} w[r+j]
+// (x[-j] + x[+j]), stored as float32; 'reflect' boundaries (d c b a | a b c d | d c b a).
+// With mask, nodes with mask[n] take mask_phi[n] (the last pass re-pins the mask).
+__global__ void k_gauss_axis(const float* __restrict__ in, float* __restrict__ out, int nx,
+                             int ny, int nz, int axis, const double* __restrict__ w, int r,
+                             petch::FastDiv fy, petch::FastDiv fz, const uint8_t* __restrict__ mask,
+                             const float* __restrict__ mask_phi) {
+#pragma clang fp contract(off)
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long N = (long long)nx * ny * nz;
+  if (n >= N) return;
+  if (mask && mask[n]) {
+    out[n] = mask_phi[n];
+    return;
+  }
+  int i, j, k;
+  split_node(n, fy, fz, i, j, k);
+  const int len = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  const int pos = axis == 0 ? i : (axis == 1 ? j : k);
+  const long long stride = axis == 0 ? (long long)ny * nz : (axis == 1 ? (long long)nz : 1LL);
+  const long long base = n - (long long)pos * stride;
+  double acc = w[r] * (double)in[n];
+  for (int q = r; q >= 1; --q) {
+    int a = pos - q, b = pos + q;
+    while (a < 0 || a >= len) a = a < 0 ? -a - 1 : 2 * len - a - 1;
+    while (b < 0 || b >= len) b = b < 0 ? -b - 1 : 2 * len - b - 1;
+    acc = acc + ((double)in[base + (long long)a * stride] + (double)in[base + (long long)b * stride]) * w[r + q];
+  }
+  out[n] = (float)acc;
+}
+
 // ------------------------------------------------------------------------------ many grids
 // One launch for the same stage of many grids (the features of a batched step). Full passes map
 // workgroups to nodes: grid f owns the workgroups [nblk0[f], nblk0[f+1]) of the launch, found by
@@ -740,6 +772,35 @@ extern "C" int petch_reinit_fsm_noalloc(petch_ctx* ctx, float* phi, int nx, int 
   float* U2 = (float*)(base + round256((size_t)N * sizeof(float)));
   uint8_t* fr = base + 2 * round256((size_t)N * sizeof(float));
   return reinit_launch(ctx, phi, nx, ny, nz, dx, band, n_iter, U, U2, fr);
+}
+
+extern "C" int petch_gaussian_smooth(petch_ctx* ctx, float* phi, int nx, int ny, int nz,
+                                     const double* weights, int radius, const uint8_t* mask,
+                                     const float* mask_phi) {
+  PETCH_ARG(ctx, ctx && phi && weights, "gaussian_smooth: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "gaussian_smooth: bad shape %d,%d,%d", nx, ny, nz);
+  PETCH_ARG(ctx, radius >= 0 && radius <= 4096, "gaussian_smooth: radius %d out of range", radius);
+  PETCH_ARG(ctx, !mask || mask_phi, "gaussian_smooth: a mask needs its pinned values");
+  const long long N = (long long)nx * ny * nz;
+  PETCH_ARG(ctx, N <= 0xffffffffLL, "gaussian_smooth: %lld nodes exceed 32-bit node indices", N);
+  float* a = (float*)petch_scratch(ctx, SCR_LS_A, (size_t)N * sizeof(float));
+  float* b = (float*)petch_scratch(ctx, SCR_LS_B, (size_t)N * sizeof(float));
+  if (!a || !b) return PETCH_E_HIP;
+  const double* w = (const double*)petch_upload_table(ctx, weights,
+                                                      (size_t)(2 * radius + 1) * sizeof(double));
+  if (!w) return PETCH_E_HIP;
+  const petch::FastDiv fy = petch::make_fastdiv((uint32_t)ny), fz = petch::make_fastdiv((uint32_t)nz);
+  const dim3 g(petch_blocks(N, kBS)), bs(kBS);
+  hipLaunchKernelGGL(k_gauss_axis, g, bs, 0, ctx->stream, phi, a, nx, ny, nz, 0, w, radius, fy,
+                     fz, (const uint8_t*)nullptr, (const float*)nullptr);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_gauss_axis, g, bs, 0, ctx->stream, a, b, nx, ny, nz, 1, w, radius, fy, fz,
+                     (const uint8_t*)nullptr, (const float*)nullptr);
+  PETCH_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_gauss_axis, g, bs, 0, ctx->stream, b, phi, nx, ny, nz, 2, w, radius, fy,
+                     fz, mask, mask_phi);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
 }
 
 namespace {

@@ -229,6 +229,7 @@ SIGNATURES = {
     "petch_finish_ion_batch": [P, c_int, P],
     "petch_surface_build_batch": [P, c_int, P],
     "petch_surface_rate_batch": [P, c_int, P],
+    "petch_gaussian_smooth": [P, P, c_int, c_int, c_int, P, c_int, P, P],
     "petch_knudsen_face_flux": [P, P, c_int, c_int, c_int, P, P, P, P, c_int,
                                 POINTER(KnudsenParams), P, P],
     "petch_dda_neutral_flux": [P, P, c_int, c_int, c_int, P, P, P, P, c_int,

@@ -124,6 +124,8 @@ class BatchLoop:
             ops.advect_batch(deferred['advect'])
         with self.timer.region("reinit"):
             ops.reinit_fsm_batch(deferred['reinit'])
+        for smooth in deferred.get('smooth', ()):        # surf_smooth, after the reinit
+            smooth()
         return infos
 
 
@@ -135,14 +137,14 @@ def run_etch_3d_batch(features, t_end=2.0, n_steps=20, reinit_method="fsm", time
 
     features: dicts of run_etch_3d keyword arguments (geometry Lx, Ly, Lz, dx, trench_width,
     mask_th, sub_top, hole; par, flags, n_ion, n_neu, seed_offset, reinit_every, extend,
-    rays_per_point, etch_stop_z); t_end, n_steps and reinit_method are shared. Returns one
+    rays_per_point, etch_stop_z, surf_smooth); t_end, n_steps and reinit_method are shared. Returns one
     geo dict per feature, as run_etch_3d does."""
     _dev()
     loops, geos = [], []
     for ft in features:
         unknown = set(ft) - set(_GEO_KEYS) - {"par", "flags", "n_ion", "n_neu", "seed_offset",
                                              "reinit_every", "extend", "rays_per_point",
-                                             "etch_stop_z"}
+                                             "etch_stop_z", "surf_smooth"}
         if unknown:
             raise TypeError(f"run_etch_3d_batch: unsupported feature keys {sorted(unknown)}")
         g = {k: ft[k] for k in _GEO_KEYS if k in ft}
@@ -156,7 +158,8 @@ def run_etch_3d_batch(features, t_end=2.0, n_steps=20, reinit_method="fsm", time
                               extend=ft.get("extend", "gpu"), reinit_method=reinit_method,
                               seed_offset=ft.get("seed_offset", 0),
                               rays_per_point=ft.get("rays_per_point"),
-                              etch_stop_z=ft.get("etch_stop_z"), timer=timer))
+                              etch_stop_z=ft.get("etch_stop_z"), timer=timer,
+                              surf_smooth=ft.get("surf_smooth", 0.0)))
         geos.append(geo)
     batch = BatchLoop(loops, timer)
     t0 = time.time()

@@ -979,3 +979,18 @@ def dda_neutral_flux(phi, dx, centroids, gas_normals, areas, bare, betas, n_dir=
                        ptr(areas, f32, "areas"), ptr(bare, f32, "bare"), F, ctypes_ref(p),
                        ptr(outs[0]), ptr(outs[1]) if len(outs) > 1 else None)
     return outs
+
+
+def gaussian_smooth(phi, sigma, mask=None, mask_phi=None, truncate=4.0):
+    """In place: phi = gaussian_filter(phi, sigma) (scipy.ndimage, mode 'reflect') then the mask
+    nodes re-pinned to mask_phi (threed.py:1779-1782; petch_gaussian_smooth). The weights are
+    formed here exactly as scipy forms them."""
+    import numpy as np
+    nx, ny, nz = _shape3(phi)
+    r = int(truncate * float(sigma) + 0.5)
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 / (float(sigma) ** 2) * x ** 2)
+    w = np.ascontiguousarray(w / w.sum())
+    _ctx(phi).call("petch_gaussian_smooth", ptr(phi, f32, "phi"), nx, ny, nz,
+                   w.ctypes.data, r, ptr(mask), ptr(mask_phi))
+    return phi

@@ -335,7 +335,8 @@ class EtchLoop:
 
     def __init__(self, phi_d, mask_d, geo, mask_th, t_end, n_steps, par, flags, n_ion, n_neu,
                  reinit_every=1, extend="gpu", reinit_method="fsm", seed_offset=0,
-                 rays_per_point=None, etch_stop_z=None, timer=None, autotune=None):
+                 rays_per_point=None, etch_stop_z=None, timer=None, autotune=None,
+                 surf_smooth=0.0):
         from .profiling import NULL
         from .chemistry import chem_params
         if par is None:
@@ -364,6 +365,7 @@ class EtchLoop:
         self.band = 4 * self.dx
         self.n_ion, self.n_neu = n_ion, n_neu
         self.reinit_every, self.extend_mode, self.reinit_method = reinit_every, extend, reinit_method
+        self.surf_smooth = float(surf_smooth or 0.0)
         self.seed_offset, self.rays_per_point, self.etch_stop_z = seed_offset, rays_per_point, etch_stop_z
         self.timer = NULL if timer is None else timer
         self.warm = getattr(flags, "warm_start_coverage", False)
@@ -598,6 +600,8 @@ class EtchLoop:
                 deferred['reinit'].append(
                     (self.phi, dx) + _reinit_args(self.phi, dx, self.band + 2.0 * dx,
                                                   self.reinit_method))
+            if self.surf_smooth > 0.0:
+                deferred.setdefault('smooth', []).append(self.smooth)
         else:
             ta = time.time()
             with T.region("advect"):
@@ -609,9 +613,17 @@ class EtchLoop:
                 with T.region("reinit"):
                     _reinit(self.phi, dx, self.band + 2.0 * dx, self.reinit_method)
                 tim['reinit'] += time.time() - tr
+            if self.surf_smooth > 0.0:
+                self.smooth()
         self.last = dict(nf=mesh.nf, nv=mesh.nv, Vmax=Vmax, nsub=nsub, n_ion=self._rays[0],
                          n_neu=self._rays[1])
         return self.last
+
+    def smooth(self):
+        """surf_smooth (threed.py:1779-1782): phi = gaussian_filter(phi, surf_smooth), then the
+        mask re-pinned, every step after the reinit."""
+        with self.timer.region("smooth"):
+            ops.gaussian_smooth(self.phi, self.surf_smooth, self.mask, self.mask_phi)
 
 
 def run_etch_3d(Lx=10.0, Ly=4.0, Lz=14.0, dx=0.4, trench_width=4.0, mask_th=2.0,
@@ -621,14 +633,13 @@ def run_etch_3d(Lx=10.0, Ly=4.0, Lz=14.0, dx=0.4, trench_width=4.0, mask_th=2.0,
                 rays_per_point=None, record_frames=False, surf_smooth=0.0,
                 etch_stop_z=None):
     """The per-step etch loop (threed.py:1643-1795) with every stage on the GPU."""
-    if surf_smooth and surf_smooth > 0.0:
-        raise NotImplementedError("surf_smooth (a scipy gaussian_filter) is outside the HIP etch loop")
     _dev()
     geo = _trench_geo(Lx, Ly, Lz, dx, trench_width, mask_th, sub_top, hole)
     phi_d, mask_d = geo.pop('_phi_d'), geo.pop('_mask_d')
     loop = EtchLoop(phi_d, mask_d, dict(geo), mask_th, t_end, n_steps, par, flags, n_ion, n_neu,
                     reinit_every=reinit_every, extend=extend, reinit_method=reinit_method,
-                    seed_offset=seed_offset, rays_per_point=rays_per_point, etch_stop_z=etch_stop_z)
+                    seed_offset=seed_offset, rays_per_point=rays_per_point, etch_stop_z=etch_stop_z,
+                    surf_smooth=surf_smooth)
     t0 = time.time()
     for step in range(n_steps):
         info = loop.step(step)

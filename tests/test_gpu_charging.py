@@ -90,7 +90,7 @@ def test_loop_runs_with_charging(flag_kw, par_kw):
                ion_reflection=True, **flag_kw)
     print(f"loop charging {flag_kw} {par_kw}", flush=True)
     geo = t3.run_etch_3d(Lx=3.0, Ly=0.9, Lz=6.5, dx=0.15, trench_width=0.9, mask_th=0.3,
-                         sub_top=6.0, t_end=1.0, n_steps=4, par=dict(PAR, **par_kw), flags=fl,
+                         sub_top=6.0, t_end=0.3, n_steps=4, par=dict(PAR, **par_kw), flags=fl,
                          n_ion=4000, n_neu=2000, verbose=False, reinit_method="fsm")
     assert np.isfinite(geo['phi']).all() and t3.max_depth_3d(geo) > 0.0
 

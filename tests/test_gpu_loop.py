@@ -204,7 +204,6 @@ def test_api_hole_runs_and_saves(tmp_path):
 
 
 @pytest.mark.parametrize("kw,flag_kw", [
-    (dict(surf_smooth=0.5), {}),
     ({}, dict(redeposition=True)),
     (dict(reinit_method="gpu"), {}),
 ])
@@ -370,3 +369,29 @@ def test_mc_step_energy_gradient_matches_common_random_number_differences():
     print(f"MC step d loss / dE: CRN fd {fd:.6e} adjoint {an:.6e} rel {abs(an - fd) / abs(fd):.4f}",
           flush=True)
     assert abs(an - fd) <= 0.02 * abs(fd)
+
+
+def test_surf_smooth_runs_and_keeps_the_mask_pinned():
+    """run_etch_3d(surf_smooth=...) smooths phi after every reinit (threed.py:1779-1782): the
+    run completes, phi differs from the unsmoothed run, and the mask nodes hold their initial
+    values; the batched driver runs the same option."""
+    from petch import batch as B
+    from petch import threed as t3
+    kw = dict(SMOKE, n_steps=3, t_end=SMOKE['t_end'] * 3 / SMOKE['n_steps'])  # the smoke's dt
+    print("surf_smooth loop", flush=True)
+    a = t3.run_etch_3d(**kw, par=dict(PAR), flags=FAITHFUL, n_ion=3000, n_neu=3000,
+                       verbose=False, reinit_method="fsm", surf_smooth=0.6)
+    b = t3.run_etch_3d(**kw, par=dict(PAR), flags=FAITHFUL, n_ion=3000, n_neu=3000,
+                       verbose=False, reinit_method="fsm")
+    init = t3.make_trench_3d(kw['Lx'], kw['Ly'], kw['Lz'], kw['dx'], kw['trench_width'],
+                             kw['mask_th'], kw['sub_top'])
+    m = init['mask']
+    assert np.isfinite(a['phi']).all() and np.isfinite(b['phi']).all()
+    assert np.abs(a['phi'] - b['phi']).max() > 1e-3
+    np.testing.assert_array_equal(a['phi'][m], init['phi'][m])
+    geo_keys = {k: kw[k] for k in ("Lx", "Ly", "Lz", "dx", "trench_width", "mask_th", "sub_top")}
+    outs = B.run_etch_3d_batch([dict(geo_keys, par=dict(PAR), flags=FAITHFUL, n_ion=3000,
+                                     n_neu=3000, surf_smooth=0.6)],
+                               t_end=kw['t_end'], n_steps=3)
+    assert np.isfinite(outs[0]['phi']).all()
+    np.testing.assert_array_equal(outs[0]['phi'][m], init['phi'][m])

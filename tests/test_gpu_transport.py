@@ -122,7 +122,7 @@ def test_etch_loop_runs_deterministic_transports(nt):
                ion_reflection=True, neutral_transport=nt)
     print(f"loop {nt}", flush=True)
     geo = t3.run_etch_3d(Lx=3.0, Ly=0.9, Lz=6.5, dx=0.15, trench_width=0.9, mask_th=0.3,
-                         sub_top=6.0, t_end=1.0, n_steps=4, par=par, flags=fl, n_ion=4000,
+                         sub_top=6.0, t_end=0.3, n_steps=4, par=par, flags=fl, n_ion=4000,
                          n_neu=1000, verbose=False, reinit_method="fsm")
     d = t3.max_depth_3d(geo)
     assert np.isfinite(geo['phi']).all() and d > 0.0

@@ -295,3 +295,18 @@ def test_reinit_adjoint_of_a_shifted_plane():
             for dt in (np.float64, np.float32):
                 gc = ol.reinit_fsm_adjoint(phi, dx, band, g, state_dtype=dt)
                 assert abs(gc.sum() - g[live].sum()) < 1e-9 * np.abs(g).sum(), (ax, sgn, dt)
+
+
+@pytest.mark.parametrize("sigma", [0.5, 0.8, 1.3, 2.6])
+def test_gaussian_smooth_equals_scipy(sigma):
+    """The surf_smooth restatement against scipy.ndimage.gaussian_filter itself (the reference
+    calls it at threed.py:1781): float32 grids, small axes where the reflection wraps more
+    than once, bit for bit."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(int(sigma * 10))
+    for shape in ((9, 7, 11), (3, 2, 20), (16, 12, 5)):
+        a = rng.normal(size=shape).astype(np.float32)
+        got = ol.gaussian_smooth(a, sigma)
+        ref = gaussian_filter(a, sigma)
+        assert got.dtype == ref.dtype == np.float32
+        np.testing.assert_array_equal(got, ref)
