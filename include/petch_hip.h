@@ -139,6 +139,11 @@ typedef struct petch_advect_item {
 int petch_advect_batch(petch_ctx* ctx, int n, const petch_advect_item* items_host);
 
 /* ---------------------------------------------------------------- a20: FSM reinit -------- */
+/* reinit_gpu (threed.py:1399-1418), reinit_method='gpu': n_iter forward-Euler steps
+ * (dtau = 0.3 dx) of phi_tau + sgn(phi0)(|grad phi| - 1) = 0 in float32 with the Russo-Smereka
+ * interface fix (the kernel _reinit_iter, threed.py:1349-1398), in place; the reference runs 24. */
+int petch_reinit_pde(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx, int n_iter);
+
 /* gaussian_filter(phi, sigma) of run_etch_3d's surf_smooth (threed.py:1779-1782, scipy.ndimage,
  * mode 'reflect'), in place: three axis passes (x, y, z) of the 2 radius + 1 weights (host
  * array, the normalised exp(-x^2 / 2 sigma^2) as scipy forms them), each in float64 stored as

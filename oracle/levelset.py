@@ -423,3 +423,47 @@ def gaussian_smooth(phi, sigma, truncate=4.0):
             acc = acc + (at(-j) + at(j)) * w[r + j]
         a = np.moveaxis(acc.astype(a.dtype), -1, ax)
     return a
+
+
+def reinit_pde(phi, dx, n_iter=24):
+    """reinit_gpu (threed.py:1349-1418): n_iter forward-Euler steps (dtau = 0.3 dx) of
+    phi_tau + sgn(phi0)(|grad phi| - 1) = 0 in float32, one operation one rounding, neighbours
+    clamped at the boundary. Interface nodes (a sign change of phi0 to a face neighbour) relax
+    toward D = clip(phi0 / max(|grad phi0|_central, 0.5), +-1.8 dx) (the Russo-Smereka fix);
+    the others take the Godunov upwind |grad phi| with the smoothed sign
+    phi0 / sqrt(phi0^2 + (dx/2)^2)."""
+    f = np.float32
+    p0 = np.asarray(phi, np.float32)
+    a = p0.copy()
+    inv = f(1.0) / f(dx)
+    dtau = f(f(0.3) * f(dx))
+    nx, ny, nz = p0.shape
+
+    def sh(x, ax, d):                          # x at index i + d along ax, clamped
+        idx = np.clip(np.arange(x.shape[ax]) + d, 0, x.shape[ax] - 1)
+        return np.take(x, idx, axis=ax)
+    o = [sh(p0, ax, d) for ax in range(3) for d in (-1, 1)]
+    iface = np.zeros(p0.shape, bool)
+    for q in o:
+        iface |= (p0 * q) < f(0.0)
+    half = f(0.5)
+    g0 = [((o[2 * ax + 1] - o[2 * ax]) * half) * inv for ax in range(3)]
+    grad0 = np.maximum(np.sqrt(g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2]), f(0.5))
+    dxl = f(1.0) / inv
+    lim = f(f(1.8) * dxl)
+    D = np.clip(p0 / grad0, -lim, lim)
+    sgn0 = np.where(p0 < f(0.0), f(-1.0), f(1.0))
+    sgn = p0 / np.sqrt(p0 * p0 + f(0.25) / (inv * inv))
+    for _ in range(int(n_iter)):
+        c = a
+        out_if = c - (dtau * inv) * (sgn0 * np.abs(c) - D)
+        Dm = [(c - sh(c, ax, -1)) * inv for ax in range(3)]
+        Dp = [(sh(c, ax, 1) - c) * inv for ax in range(3)]
+        z = f(0.0)
+        gp = [np.maximum(np.maximum(Dm[ax], z), -np.minimum(Dp[ax], z)) for ax in range(3)]
+        gn = [np.maximum(-np.minimum(Dm[ax], z), np.maximum(Dp[ax], z)) for ax in range(3)]
+        g = [np.where(sgn > z, gp[ax], gn[ax]) for ax in range(3)]
+        G = np.sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2])
+        out = c - dtau * sgn * (G - f(1.0))
+        a = np.where(iface, out_if, out).astype(np.float32)
+    return a

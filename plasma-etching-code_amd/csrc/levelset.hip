@@ -323,6 +323,59 @@ __global__ void k_eik_final(float* __restrict__ phi, const float* __restrict__ U
   eik_final_node(phi, U, clampv, n);
 }
 
+// One step of the PDE reinit of reinit_method='gpu' (threed.py:1349-1398), float32 in the
+// reference's order: interface nodes (phi0 changes sign to a face neighbour) relax toward
+// D = clip(phi0 / max(|grad phi0|, 0.5), +-1.8 dx); the others take the Godunov upwind
+// |grad phi| with the smoothed sign phi0 / sqrt(phi0^2 + (dx/2)^2). Neighbours clamp at the
+// boundary.
+__global__ void k_reinit_pde(const float* __restrict__ phi, const float* __restrict__ phi0,
+                             float* __restrict__ out, int nx, int ny, int nz, float inv_dx,
+                             float dtau, petch::FastDiv fy, petch::FastDiv fz) {
+#pragma clang fp contract(off)
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long long)nx * ny * nz) return;
+  int i, j, k;
+  split_node(n, fy, fz, i, j, k);
+  const long long sx = (long long)ny * nz, sy = nz;
+  const long long xm = i > 0 ? n - sx : n, xp = i < nx - 1 ? n + sx : n;
+  const long long ym = j > 0 ? n - sy : n, yp = j < ny - 1 ? n + sy : n;
+  const long long zm = k > 0 ? n - 1 : n, zp = k < nz - 1 ? n + 1 : n;
+  const float c = phi[n];
+  const float s0 = phi0[n];
+  const float o_xm = phi0[xm], o_xp = phi0[xp], o_ym = phi0[ym], o_yp = phi0[yp];
+  const float o_zm = phi0[zm], o_zp = phi0[zp];
+  const bool iface = (s0 * o_xm < 0.f) || (s0 * o_xp < 0.f) || (s0 * o_ym < 0.f) ||
+                     (s0 * o_yp < 0.f) || (s0 * o_zm < 0.f) || (s0 * o_zp < 0.f);
+  if (iface) {
+    const float gx0 = (o_xp - o_xm) * 0.5f * inv_dx;
+    const float gy0 = (o_yp - o_ym) * 0.5f * inv_dx;
+    const float gz0 = (o_zp - o_zm) * 0.5f * inv_dx;
+    const float grad0 = fmaxf(sqrtf(gx0 * gx0 + gy0 * gy0 + gz0 * gz0), 0.5f);
+    const float dxl = 1.0f / inv_dx;
+    const float lim = 1.8f * dxl;
+    const float D = fminf(fmaxf(s0 / grad0, -lim), lim);
+    const float sgn0 = s0 < 0.f ? -1.f : 1.f;
+    out[n] = c - dtau * inv_dx * (sgn0 * fabsf(c) - D);
+    return;
+  }
+  const float Dxm = (c - phi[xm]) * inv_dx, Dxp = (phi[xp] - c) * inv_dx;
+  const float Dym = (c - phi[ym]) * inv_dx, Dyp = (phi[yp] - c) * inv_dx;
+  const float Dzm = (c - phi[zm]) * inv_dx, Dzp = (phi[zp] - c) * inv_dx;
+  const float sgn = s0 / sqrtf(s0 * s0 + 0.25f / (inv_dx * inv_dx));
+  float gx, gy, gz;
+  if (sgn > 0.f) {
+    gx = fmaxf(fmaxf(Dxm, 0.f), -fminf(Dxp, 0.f));
+    gy = fmaxf(fmaxf(Dym, 0.f), -fminf(Dyp, 0.f));
+    gz = fmaxf(fmaxf(Dzm, 0.f), -fminf(Dzp, 0.f));
+  } else {
+    gx = fmaxf(-fminf(Dxm, 0.f), fmaxf(Dxp, 0.f));
+    gy = fmaxf(-fminf(Dym, 0.f), fmaxf(Dyp, 0.f));
+    gz = fmaxf(-fminf(Dzm, 0.f), fmaxf(Dzp, 0.f));
+  }
+  const float G = sqrtf(gx * gx + gy * gy + gz * gz);
+  out[n] = c - dtau * sgn * (G - 1.f);
+}
+
 // One axis pass of the separable Gaussian of surf_smooth: out = sum over |j| <= r of
 // w[r + j] in[reflect(i + j)] along `axis`, in float64 as w[r] x0 + sum_{j=r..1} w[r+j]
 // (x[-j] + x[+j]), stored as float32; 'reflect' boundaries (d c b a | a b c d | d c b a).
@@ -772,6 +825,38 @@ extern "C" int petch_reinit_fsm_noalloc(petch_ctx* ctx, float* phi, int nx, int 
   float* U2 = (float*)(base + round256((size_t)N * sizeof(float)));
   uint8_t* fr = base + 2 * round256((size_t)N * sizeof(float));
   return reinit_launch(ctx, phi, nx, ny, nz, dx, band, n_iter, U, U2, fr);
+}
+
+extern "C" int petch_reinit_pde(petch_ctx* ctx, float* phi, int nx, int ny, int nz, double dx,
+                                int n_iter) {
+  PETCH_ARG(ctx, ctx && phi, "reinit_pde: null pointer");
+  PETCH_ARG(ctx, nx > 0 && ny > 0 && nz > 0, "reinit_pde: bad shape %d,%d,%d", nx, ny, nz);
+  PETCH_ARG(ctx, dx > 0.0 && n_iter >= 0, "reinit_pde: dx <= 0 or n_iter < 0");
+  const long long N = (long long)nx * ny * nz;
+  PETCH_ARG(ctx, N <= 0xffffffffLL, "reinit_pde: %lld nodes exceed 32-bit node indices", N);
+  if (n_iter == 0) return PETCH_OK;
+  float* p0 = (float*)petch_scratch(ctx, SCR_LS_A, (size_t)N * sizeof(float));
+  float* b = (float*)petch_scratch(ctx, SCR_LS_B, (size_t)N * sizeof(float));
+  if (!p0 || !b) return PETCH_E_HIP;
+  PETCH_HIP(ctx, hipMemcpyAsync(p0, phi, (size_t)N * sizeof(float), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  const float dx32 = (float)dx;
+  const float inv = 1.0f / dx32;
+  const float dtau = 0.3f * dx32;
+  const petch::FastDiv fy = petch::make_fastdiv((uint32_t)ny), fz = petch::make_fastdiv((uint32_t)nz);
+  const dim3 g(petch_blocks(N, kBS)), bs(kBS);
+  float* a = phi;
+  float* c = b;
+  for (int it = 0; it < n_iter; ++it) {
+    hipLaunchKernelGGL(k_reinit_pde, g, bs, 0, ctx->stream, a, p0, c, nx, ny, nz, inv, dtau, fy,
+                       fz);
+    PETCH_LAUNCH_CHECK(ctx);
+    float* t = a; a = c; c = t;
+  }
+  if (a != phi)
+    PETCH_HIP(ctx, hipMemcpyAsync(phi, a, (size_t)N * sizeof(float), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+  return PETCH_OK;
 }
 
 extern "C" int petch_gaussian_smooth(petch_ctx* ctx, float* phi, int nx, int ny, int nz,

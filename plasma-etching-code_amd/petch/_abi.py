@@ -230,6 +230,7 @@ SIGNATURES = {
     "petch_surface_build_batch": [P, c_int, P],
     "petch_surface_rate_batch": [P, c_int, P],
     "petch_gaussian_smooth": [P, P, c_int, c_int, c_int, P, c_int, P, P],
+    "petch_reinit_pde": [P, P, c_int, c_int, c_int, c_double, c_int],
     "petch_knudsen_face_flux": [P, P, c_int, c_int, c_int, P, P, P, P, c_int,
                                 POINTER(KnudsenParams), P, P],
     "petch_dda_neutral_flux": [P, P, c_int, c_int, c_int, P, P, P, P, c_int,

@@ -124,8 +124,8 @@ class BatchLoop:
             ops.advect_batch(deferred['advect'])
         with self.timer.region("reinit"):
             ops.reinit_fsm_batch(deferred['reinit'])
-        for smooth in deferred.get('smooth', ()):        # surf_smooth, after the reinit
-            smooth()
+        for post in deferred.get('post', ()):     # per-feature work after the batched reinit:
+            post()                                # the PDE reinit, then surf_smooth
         return infos
 
 

@@ -994,3 +994,10 @@ def gaussian_smooth(phi, sigma, mask=None, mask_phi=None, truncate=4.0):
     _ctx(phi).call("petch_gaussian_smooth", ptr(phi, f32, "phi"), nx, ny, nz,
                    w.ctypes.data, r, ptr(mask), ptr(mask_phi))
     return phi
+
+
+def reinit_pde(phi, dx, n_iter=24):
+    """In place: reinit_gpu (threed.py:1399-1418; petch_reinit_pde), n_iter PDE steps."""
+    nx, ny, nz = _shape3(phi)
+    _ctx(phi).call("petch_reinit_pde", ptr(phi, f32, "phi"), nx, ny, nz, float(dx), int(n_iter))
+    return phi

@@ -298,7 +298,9 @@ _REINIT_WARNED = set()
 
 def _reinit_args(phi_d, dx, band, method):
     """(band, n_iter) of the GPU Eikonal reinit that stands in for `method` (n_iter None: the
-    default ceil(band/dx)+4 sweeps)."""
+    default ceil(band/dx)+4 sweeps); None for 'gpu', which is the PDE reinit."""
+    if method == "gpu":
+        return None
     if method == "fsm":
         return band, None
     if method in ("skfmm", "skfmm_full"):
@@ -312,12 +314,15 @@ def _reinit_args(phi_d, dx, band, method):
             return dx * (max(phi_d.shape) + 4), int(max(phi_d.shape)) + 4
         return band, None
     raise NotImplementedError(f"reinit_method={method!r} is not part of the HIP etch loop "
-                              "(supported: 'fsm', 'skfmm', 'skfmm_full')")
+                              "(supported: 'fsm', 'gpu', 'skfmm', 'skfmm_full')")
 
 
 def _reinit(phi_d, dx, band, method):
-    b, n_iter = _reinit_args(phi_d, dx, band, method)
-    ops.reinit_fsm(phi_d, dx, b, n_iter)
+    args = _reinit_args(phi_d, dx, band, method)
+    if args is None:                      # reinit_method='gpu' (threed.py:1772-1773)
+        ops.reinit_pde(phi_d, dx)
+        return
+    ops.reinit_fsm(phi_d, dx, *args)
 
 
 class _FluxSteps:
@@ -560,7 +565,9 @@ class EtchLoop:
         step's extension (the batched loop extends all features in one pass). deferred: a dict
         of 'advect' and 'reinit' lists that receive this step's advection and reinit as items of
         ops.advect_batch / ops.reinit_fsm_batch instead of running them (the batched loop runs
-        every feature's in one launch per substep and sweep)."""
+        every feature's in one launch per substep and sweep); work that must follow the reinit
+        (the PDE reinit of reinit_method='gpu', surf_smooth) goes to its 'post' list of
+        callables."""
         T, dx, dev = self.timer, self.dx, self.dev
         tim = self.timings
         mesh = self._mesh
@@ -597,11 +604,13 @@ class EtchLoop:
             deferred['advect'].append((self.phi, self.work, self.Fs, self.mask, self.mask_phi,
                                        dx, self.dt / nsub, nsub))
             if reinit:
-                deferred['reinit'].append(
-                    (self.phi, dx) + _reinit_args(self.phi, dx, self.band + 2.0 * dx,
-                                                  self.reinit_method))
+                args = _reinit_args(self.phi, dx, self.band + 2.0 * dx, self.reinit_method)
+                if args is None:              # the PDE reinit is not batched: run it after
+                    deferred.setdefault('post', []).insert(0, self._reinit_now)
+                else:
+                    deferred['reinit'].append((self.phi, dx) + args)
             if self.surf_smooth > 0.0:
-                deferred.setdefault('smooth', []).append(self.smooth)
+                deferred.setdefault('post', []).append(self.smooth)
         else:
             ta = time.time()
             with T.region("advect"):
@@ -618,6 +627,9 @@ class EtchLoop:
         self.last = dict(nf=mesh.nf, nv=mesh.nv, Vmax=Vmax, nsub=nsub, n_ion=self._rays[0],
                          n_neu=self._rays[1])
         return self.last
+
+    def _reinit_now(self):
+        _reinit(self.phi, self.dx, self.band + 2.0 * self.dx, self.reinit_method)
 
     def smooth(self):
         """surf_smooth (threed.py:1779-1782): phi = gaussian_filter(phi, surf_smooth), then the

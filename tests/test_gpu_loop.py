@@ -205,7 +205,6 @@ def test_api_hole_runs_and_saves(tmp_path):
 
 @pytest.mark.parametrize("kw,flag_kw", [
     ({}, dict(redeposition=True)),
-    (dict(reinit_method="gpu"), {}),
 ])
 def test_out_of_path_options_raise(kw, flag_kw):
     """Options outside the etch-loop hot path fail loudly instead of silently changing physics."""
@@ -369,6 +368,23 @@ def test_mc_step_energy_gradient_matches_common_random_number_differences():
     print(f"MC step d loss / dE: CRN fd {fd:.6e} adjoint {an:.6e} rel {abs(an - fd) / abs(fd):.4f}",
           flush=True)
     assert abs(an - fd) <= 0.02 * abs(fd)
+
+
+def test_pde_reinit_runs_in_both_loops():
+    """reinit_method='gpu' (the PDE reinit of threed.py:1399-1418) in run_etch_3d and in the
+    batched driver: both finish with a finite level set and etch."""
+    from petch import batch as B
+    from petch import threed as t3
+    kw = dict(SMOKE, n_steps=3, t_end=SMOKE['t_end'] * 3 / SMOKE['n_steps'])
+    print("pde reinit loop", flush=True)
+    a = t3.run_etch_3d(**kw, par=dict(PAR), flags=FAITHFUL, n_ion=3000, n_neu=3000,
+                       verbose=False, reinit_method="gpu")
+    assert np.isfinite(a['phi']).all() and t3.max_depth_3d(a) > 0.0
+    geo_keys = {k: kw[k] for k in ("Lx", "Ly", "Lz", "dx", "trench_width", "mask_th", "sub_top")}
+    outs = B.run_etch_3d_batch([dict(geo_keys, par=dict(PAR), flags=FAITHFUL, n_ion=3000,
+                                     n_neu=3000)], t_end=kw['t_end'], n_steps=3,
+                               reinit_method="gpu")
+    assert np.isfinite(outs[0]['phi']).all()
 
 
 def test_surf_smooth_runs_and_keeps_the_mask_pinned():

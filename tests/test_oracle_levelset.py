@@ -310,3 +310,32 @@ def test_gaussian_smooth_equals_scipy(sigma):
         ref = gaussian_filter(a, sigma)
         assert got.dtype == ref.dtype == np.float32
         np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_reinit_pde_keeps_an_axis_plane_distance(axis):
+    """reinit_gpu's PDE step (threed.py:1349-1398) leaves an exact plane distance unchanged,
+    boundaries included (the clamped one-sided differences of a plane along an axis stay exact),
+    and drives the interface nodes of a scaled plane to the true sub-cell distance."""
+    dx = 0.25
+    shape = (10, 8, 12)
+    x = np.indices(shape)[axis] * dx
+    d = (x - 1.13).astype(np.float32)
+    np.testing.assert_array_equal(ol.reinit_pde(d, dx, 24), d)
+    r = ol.reinit_pde(3.0 * d, dx, 300)
+    iface = np.abs(d) < dx
+    np.testing.assert_allclose(r[iface], d[iface], atol=2e-6)
+    assert np.abs(r - d)[np.abs(d) < 4 * dx].max() < 1e-4         # the band relaxes too
+
+
+def test_reinit_pde_signs_and_clip():
+    """One step at an interface node by hand: D = phi0 / |grad phi0|_central (clipped to
+    1.8 dx) and phi <- phi - 0.3 (sgn(phi0) |phi| - D)."""
+    dx = 1.0
+    p = np.full((3, 3, 4), 5.0, np.float32)
+    p[:, :, :2] = -5.0
+    p[1, 1, 1], p[1, 1, 2] = -0.1, 0.1
+    r = ol.reinit_pde(p, dx, 1)
+    # node (1,1,1): s0 = -0.1, o_zp = 0.1 -> interface; grad0 = max(|(0.1 - (-5)) / 2|, ...) = 2.55
+    D = max(min(-0.1 / 2.55, 1.8), -1.8)
+    np.testing.assert_allclose(r[1, 1, 1], -0.1 - 0.3 * (-1 * 0.1 - D), rtol=1e-6)
