@@ -28,12 +28,18 @@
 namespace {
 
 constexpr int kMcBS = 256;
-// A workgroup takes a tile of kMcRounds x kMcBS consecutive nodes, one round of kMcBS at a
-// time: fewer, longer workgroups (one per 256 nodes made the passes bound by workgroup
-// dispatch). Vertex and face ids are prefix sums over the nodes in order, so the tile size does
-// not change the output.
-constexpr int kMcRounds = 8;
-constexpr long long kMcTile = (long long)kMcRounds * kMcBS;
+// A workgroup takes a tile of `rounds` x kMcBS consecutive nodes, one round of kMcBS at a time.
+// Launches over very many nodes (the grids of a batched step) use tiles of up to 8 rounds, so
+// they need fewer, longer workgroups; a launch over one grid keeps 256-node tiles so that it
+// still fills the chip. Vertex and face ids are prefix sums over the nodes in order, so the tile
+// length does not change the output.
+constexpr int kMcMaxRounds = 8;
+
+// tile length (in rounds of kMcBS) of a launch over `nodes` nodes: about 16k workgroups or more
+int mc_rounds(long long nodes) {
+  const long long r = nodes / ((long long)kMcBS * 16384);
+  return (int)(r < 1 ? 1 : (r > kMcMaxRounds ? kMcMaxRounds : r));
+}
 
 __device__ __forceinline__ int edge_base(int e) {
   int a = e >> 2, q = e & 3, lo = q & 1, hi = q >> 1;
@@ -148,11 +154,12 @@ struct Grid {
   int nx, ny, nz;
   long long sx, sy;  // node strides
   petch::FastDiv dy, dz;  // division by ny and nz (node indices of one grid fit 32 bits)
+  int rounds;        // tile length of the launch, in rounds of kMcBS nodes
 };
 
-__host__ inline Grid make_grid(int nx, int ny, int nz) {
+__host__ inline Grid make_grid(int nx, int ny, int nz, int rounds) {
   return Grid{nx, ny, nz, (long long)ny * nz, (long long)nz, petch::make_fastdiv((uint32_t)ny),
-              petch::make_fastdiv((uint32_t)nz)};
+              petch::make_fastdiv((uint32_t)nz), rounds};
 }
 
 __device__ __forceinline__ void node_ijk(const Grid& g, long long n, int& i, int& j, int& k) {
@@ -189,8 +196,8 @@ __device__ __forceinline__ void mc_count_block(const float* __restrict__ phi, co
                                                long long N, long long blk, int* __restrict__ bv,
                                                int* __restrict__ bt) {
   int vc = 0, tc = 0;
-  for (int r = 0; r < kMcRounds; ++r) {
-    const long long n = blk * kMcTile + r * kMcBS + threadIdx.x;
+  for (int r = 0; r < g.rounds; ++r) {
+    const long long n = (blk * g.rounds + r) * kMcBS + threadIdx.x;
     if (n < N) {
       int i, j, k;
       node_ijk(g, n, i, j, k);
@@ -264,8 +271,8 @@ __device__ __forceinline__ void mc_verts_block(const float* __restrict__ phi, co
 #pragma clang fp contract(off)
   if (mc_block_empty(bv, blk, nb, total)) return;
   int base = bv[blk];
-  for (int r = 0; r < kMcRounds; ++r) {
-    const long long n = blk * kMcTile + r * kMcBS + threadIdx.x;
+  for (int r = 0; r < g.rounds; ++r) {
+    const long long n = (blk * g.rounds + r) * kMcBS + threadIdx.x;
     int fl = 0, i = 0, j = 0, k = 0;
     if (n < N) {
       node_ijk(g, n, i, j, k);
@@ -366,8 +373,8 @@ __device__ __forceinline__ void mc_faces_block(const float* __restrict__ phi, co
                                                const int* __restrict__ total) {
   if (mc_block_empty(bt, blk, nb, total)) return;
   int base = bt[blk];
-  for (int r = 0; r < kMcRounds; ++r) {
-    const long long n = blk * kMcTile + r * kMcBS + threadIdx.x;
+  for (int r = 0; r < g.rounds; ++r) {
+    const long long n = (blk * g.rounds + r) * kMcBS + threadIdx.x;
     float f[8];
     int tc = 0;
     if (n < N) {
@@ -474,8 +481,10 @@ extern "C" int petch_marching_cubes(petch_ctx* ctx, const float* phi, int nx, in
   if (nx < 2 || ny < 2 || nz < 2) return PETCH_OK;  // no cells, no surface
   long long N = (long long)nx * ny * nz;
   PETCH_ARG(ctx, N <= 0xffffffffLL, "marching_cubes: %lld nodes exceed 32-bit node indices", N);
-  Grid g = make_grid(nx, ny, nz);
-  long long nb = (N + kMcTile - 1) / kMcTile;
+  const int rounds = mc_rounds(N);
+  const long long tile = (long long)rounds * kMcBS;
+  Grid g = make_grid(nx, ny, nz, rounds);
+  long long nb = (N + tile - 1) / tile;
   int* blk = (int*)petch_scratch(ctx, SCR_MC_BLOCK, (size_t)(2 * nb + 2) * sizeof(int));
   if (!blk) return PETCH_E_HIP;
   int* bv = blk;
@@ -519,7 +528,7 @@ extern "C" int petch_marching_cubes_batch(petch_ctx* ctx, int n, const petch_mc_
   std::vector<McGrid> t;
   std::vector<int> which;
   t.reserve((size_t)n);
-  long long parts = 0, blocks = 0;
+  long long parts = 0, blocks = 0, nodes = 0;
   for (int f = 0; f < n; ++f) {
     const petch_mc_item& a = items[f];
     PETCH_ARG(ctx, a.phi && a.nv_out && a.nf_out, "marching_cubes_batch: item %d has a null pointer", f);
@@ -530,25 +539,32 @@ extern "C" int petch_marching_cubes_batch(petch_ctx* ctx, int n, const petch_mc_
     if (a.nx < 2 || a.ny < 2 || a.nz < 2) continue;       // no cells, no surface
     McGrid m{};
     m.phi = a.phi;
-    m.g = make_grid(a.nx, a.ny, a.nz);
+    m.g = make_grid(a.nx, a.ny, a.nz, 1);
     m.N = (long long)a.nx * a.ny * a.nz;
     PETCH_ARG(ctx, m.N <= 0xffffffffLL,
               "marching_cubes_batch: item %d: %lld nodes exceed 32-bit node indices", f, m.N);
-    m.nb = (m.N + kMcTile - 1) / kMcTile;
-    m.blk0 = blocks;
     m.dx = a.dx;
     m.verts = a.verts;
     m.faces = a.faces;
     m.cap_v = a.cap_v;
     m.cap_f = a.cap_f;
-    m.bv = (int*)(intptr_t)parts;                           // offsets until the buffer exists
-    parts += 2 * m.nb;
-    blocks += m.nb;
     t.push_back(m);
     which.push_back(f);
+    nodes += m.N;
   }
   const int m_ = (int)t.size();
   if (m_ == 0) return PETCH_OK;
+  // one tile length for the launch, from all its nodes; each grid's tiles and block-sum slots
+  const int rounds = mc_rounds(nodes);
+  const long long tile = (long long)rounds * kMcBS;
+  for (McGrid& m : t) {
+    m.g.rounds = rounds;
+    m.nb = (m.N + tile - 1) / tile;
+    m.blk0 = blocks;
+    m.bv = (int*)(intptr_t)parts;                           // offsets until the buffer exists
+    parts += 2 * m.nb;
+    blocks += m.nb;
+  }
   PETCH_ARG(ctx, blocks <= 0x7fffffffLL, "marching_cubes_batch: too many nodes for one launch");
   int* blk = (int*)petch_scratch(ctx, SCR_MC_BLOCK, (size_t)(parts + 2 * m_) * sizeof(int));
   int* h = (int*)petch_pinned(ctx, (size_t)(2 * m_) * sizeof(int));
