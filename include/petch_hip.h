@@ -298,6 +298,16 @@ int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petch_source* s
 int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
                            const petch_source* src, const float* bare, float beta, int n,
                            uint64_t seed, float* fl);
+/* mc_redep_3d (threed.py:1582-1605): n product rays leave face f with probability
+ * max(rate_f areas_f, 0) / total (non-finite products count as 0) from centroid + 1e-3 n_f,
+ * n_f = gas_normals[f] (unit, into the gas), in the cosine lobe about n_f, then follow the RR
+ * walk of petch_trace_neutral_rr with sticking s_redep on every face and no lateral wrap.
+ * out (nf,) = deposit * (total / n) / max(areas, 1e-9), the redeposited flux density (all 0 when
+ * total <= 0). centroids, gas_normals (nf, 3); areas, rate (nf,). Ray r's face and lobe come
+ * from block 0 of its stream, its bounces as in petch_trace_neutral_rr. */
+int petch_trace_redeposition(petch_ctx* ctx, const petch_bvh* bvh, const float* centroids,
+                             const float* gas_normals, const float* areas, const float* rate,
+                             float s_redep, int n, uint64_t seed, float* out);
 /* Path-replay adjoint of petch_trace_neutral_rr with the same (bare, beta, n, seed): for
  * gfl = d loss / d fl, gbare (nf, zeroed first) = d loss / d bare with every ray's path and its
  * roulette / termination decisions held fixed (the rays redraw their streams) and a roulette

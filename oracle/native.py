@@ -141,6 +141,17 @@ class OracleBvh:
                                       ctypes.c_uint64(seed), _p(out))
         return out
 
+    def trace_redeposition(self, cen, gn, area, rate, s_redep, n, seed):
+        """Redeposited flux density (float64 (F,)) of oracle_trace_redeposition."""
+        c = np.ascontiguousarray(cen, np.float32)
+        g = np.ascontiguousarray(gn, np.float32)
+        a = np.ascontiguousarray(area, np.float32)
+        r = np.ascontiguousarray(rate, np.float32)
+        out = np.zeros(max(self.nf, 1), np.float64)
+        lib().oracle_trace_redeposition(self.h, _p(c), _p(g), _p(a), _p(r), ctypes.c_float(s_redep),
+                                        ctypes.c_int(n), ctypes.c_uint64(seed), _p(out))
+        return out[:self.nf]
+
     def trace_neutral_rr_paths(self, box, src, bare, beta, n, seed):
         """trace_neutral_rr plus each ray's hit count (int32) and deposited weight (float64)."""
         bare = np.ascontiguousarray(bare, np.float32)
@@ -196,6 +207,16 @@ class OracleBvh:
         lib().oracle_trace_ff(self.h, ctypes.byref(box), _p(c), _p(g), ctypes.c_int(len(c)),
                               ctypes.c_int(n_ff), ctypes.c_uint64(seed), _p(out))
         return out
+
+
+def redep_faces(rate, area, n, seed):
+    """The emitting face of each of n redeposition rays (int32 (n,)), oracle_redep_faces."""
+    r = np.ascontiguousarray(rate, np.float32)
+    a = np.ascontiguousarray(area, np.float32)
+    out = np.zeros(n, np.int32)
+    lib().oracle_redep_faces(_p(r), _p(a), ctypes.c_int(len(r)), ctypes.c_int(n),
+                             ctypes.c_uint64(seed), _p(out))
+    return out
 
 
 def gen_source(box, src, n, seed):

@@ -716,6 +716,98 @@ void oracle_trace_neutral_rr_paths(void* h, const oc_box* b, const oc_source* s,
   }
 }
 
+/* Redeposition (mc_redep_3d, threed.py:1582-1605). The emission weight of face f is
+ * max(rate_f area_f, 0), a non-finite product counting as 0; cum holds its running sum in order
+ * (float64). Ray p draws its face from word 0 and 1 of its block 0 as a 53-bit uniform x: the
+ * first f with cum[f] > x * cum[nf - 1] (the last face when none is), rng.choice's rule. */
+static double oc_redep_weight(const float* rate, const float* area, int f) {
+  double w = (double)rate[f] * (double)area[f];
+  return (w > 0.0 && w <= 1.0e300) ? w : 0.0;
+}
+static int oc_redep_face(const double* cum, int nf, const uint32_t u[4]) {
+  double x = ((double)(u[0] >> 5) * 67108864.0 + (double)(u[1] >> 6)) *
+             (1.0 / 9007199254740992.0);
+  double target = x * cum[nf - 1];
+  int lo = 0, hi = nf - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (cum[mid] > target) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+/* The face each of n rays leaves (test instrument: the emission draw alone). */
+void oracle_redep_faces(const float* rate, const float* area, int nf, int n, uint64_t seed,
+                        int* face) {
+  double* cum = (double*)malloc(sizeof(double) * (nf > 0 ? nf : 1));
+  double c = 0.0;
+  for (int f = 0; f < nf; ++f) { c += oc_redep_weight(rate, area, f); cum[f] = c; }
+  for (int p = 0; p < n; ++p) {
+    uint32_t u[4];
+    oc_block(seed, (uint32_t)p, 0u, 0u, u);
+    face[p] = nf > 0 ? oc_redep_face(cum, nf, u) : -1;
+  }
+  free(cum);
+}
+
+/* n product rays: from face f (drawn as above) at centroid + 1e-3 n_f along the into-gas normal
+ * n_f, cosine lobe about n_f from words 2 and 3 of block 0, then the RR walk of
+ * _trace3d_cov_rr with S = s_redep on every face in an open box. out = deposit * (total / n) /
+ * max(area, 1e-9), all 0 when total <= 0. */
+void oracle_trace_redeposition(void* h, const float* cen, const float* gn, const float* area,
+                               const float* rate, float s_redep, int n, uint64_t seed,
+                               double* out) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  int F = B->nf;
+  memset(out, 0, sizeof(double) * (F > 0 ? F : 1));
+  if (F == 0 || n <= 0) return;
+  double* cum = (double*)malloc(sizeof(double) * F);
+  double c = 0.0;
+  for (int f = 0; f < F; ++f) { c += oc_redep_weight(rate, area, f); cum[f] = c; }
+  const double tot = cum[F - 1];
+  if (tot <= 0.0) { free(cum); return; }
+  const oc_box box = {1.f, 1.f, 1.f, 1.f, 0};
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc(F, sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+    for (int p = 0; p < n; ++p) {
+      uint32_t u[4];
+      oc_block(seed, (uint32_t)p, 0u, 0u, u);
+      int f0 = oc_redep_face(cum, F, u);
+      f3 nn0 = mk(gn[3 * f0], gn[3 * f0 + 1], gn[3 * f0 + 2]);
+      f3 o = add3(mk(cen[3 * f0], cen[3 * f0 + 1], cen[3 * f0 + 2]), scl(1.0e-3f, nn0));
+      f3 d = oc_cosine_u(nn0, oc_u24(u[2]), oc_u24(u[3])), nn;
+      float w = 1.f, t;
+      for (int bounce = 0; bounce < 1024; ++bounce) {
+        int f = oc_query_bc(B, &box, &o, d, &t, &nn);
+        if (f < 0) break;
+        if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
+        acc[f] += w;
+        w = w - w * s_redep;
+        if (w <= 1.0e-6f) break;
+        uint32_t v[4];
+        oc_block(seed, (uint32_t)p, (uint32_t)bounce + 1u, 1u, v);
+        if (w < 0.1f) {
+          if (oc_u24(v[0]) < (1.f - w / 0.3f)) break;
+          w = 0.3f;
+        }
+        f3 hit = add3(o, scl(t, d));
+        d = oc_cosine_u(nn, oc_u24(v[1]), oc_u24(v[2]));
+        o = add3(hit, scl(1.0e-4f, nn));
+      }
+    }
+#pragma omp critical
+    for (int f = 0; f < F; ++f) out[f] += acc[f];
+    free(acc);
+  }
+  for (int f = 0; f < F; ++f) {
+    double a = (double)area[f] > 1.0e-9 ? (double)area[f] : 1.0e-9;
+    out[f] = out[f] * (tot / (double)n) / a;
+  }
+  free(cum);
+}
+
 /* oracle_trace_neutral_rr with the decisions frozen (test instrument, not in the reference): the
  * paths, roulettes and terminations are those of the sticking bare_dec, while the deposited
  * weights follow bare_w (w_{k+1} = w_k (1 - bare_w beta); a roulette survivor's weight is

@@ -37,6 +37,7 @@ enum PetchScratch {
   SCR_LS_BRICK,      // batched level-set stages: per-brick activity flags and list counters
   SCR_LS_LIST,       // batched level-set stages: lists of the bricks a launch processes
   SCR_SORT_MANY,     // batched radix sort: per-array digit tables
+  SCR_REDEP,         // redeposition: float64 emission CDF and its workgroup partials
   SCR_COUNT
 };
 

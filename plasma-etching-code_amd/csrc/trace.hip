@@ -459,6 +459,63 @@ struct NeutralRR {
   }
 };
 
+// Etch-product redeposition (mc_redep_3d, threed.py:1582-1605): product leaves face f with
+// probability w_f / sum w, w_f = max(rate_f area_f, 0), from 1e-3 off its centroid along its
+// into-gas normal, in the cosine lobe about that normal, then follows the RR neutral walk with
+// sticking s_redep on every face (bare = 1) in an open box (no lateral wrap). cum holds the
+// inclusive float64 prefix sums of w; a ray draws its face by bisection for the first prefix
+// above u * total, u a 53-bit uniform from words 0 and 1 of its block 0, and its lobe from
+// words 2 and 3. With nothing to emit (total <= 0) every ray carries weight 0 and deposits
+// nothing; the result is then zero.
+struct Redep {
+  static constexpr bool kReplay = false;
+  BvhView bv;
+  Box b;
+  const float* __restrict__ cen;
+  const float* __restrict__ gn;
+  const double* __restrict__ cum;
+  int nf;
+  float S;
+  float* fl;
+  uint64_t seed;
+  int fixed;
+  struct Ray {
+    float w;
+    int bounce;
+    unsigned ray;
+  };
+  __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
+  __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
+  __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
+    uint32_t u[4];
+    petch::philox_block(seed, ray, 0u, 0u, u);
+    const double tot = cum[nf - 1];
+    const double x = ((double)(u[0] >> 5) * 67108864.0 + (double)(u[1] >> 6)) *
+                     (1.0 / 9007199254740992.0);
+    const double target = x * tot;
+    int lo = 0, hi = nf - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cum[mid] > target)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    const Vec3 n = v3(gn[3 * lo], gn[3 * lo + 1], gn[3 * lo + 2]);
+    o = v3(cen[3 * lo], cen[3 * lo + 1], cen[3 * lo + 2]) + 1.0e-3f * n;
+    e.n = n;
+    e.r1 = petch::u24(u[2]);
+    e.r2 = petch::u24(u[3]);
+    e.mode = kLobe;
+    R.w = tot > 0.0 ? 1.f : 0.f;
+    R.bounce = 0;
+    R.ray = ray;
+  }
+  __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
+    return rr_bounce(R.w, R.bounce, seed, R.ray, bv, h, fl, fixed, S, o, d, e);
+  }
+};
+
 // Path-replay adjoint of the trace (no reference counterpart): for gfl = d loss / d fl it
 // accumulates d loss / d bare into gbare, with every ray following the path the trace drew (same
 // Philox stream, same roulette and termination decisions, same geometry). Along a ray the
@@ -973,6 +1030,91 @@ k_trace_ion_yield_batch(IonYieldBatch pol, RayQueue q, unsigned long long* __res
   persistent_trace(pol, q, hits);
 }
 
+__global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5)))
+k_trace_redep(Redep pol, RayQueue q, unsigned long long* __restrict__ hits) {
+  persistent_trace(pol, q, hits);
+}
+
+// Emission weight of a face for redeposition: max(rate * area, 0), a non-finite product as 0.
+__device__ __forceinline__ double redep_weight(const float* __restrict__ rate,
+                                               const float* __restrict__ area, int f) {
+  const double w = (double)rate[f] * (double)area[f];
+  return (w > 0.0 && w <= 1.0e300) ? w : 0.0;
+}
+
+// Inclusive and exclusive prefix of v over a workgroup of BS threads (BS a multiple of 64) in
+// float64; the workgroup total goes to `total`.
+template <int BS>
+__device__ __forceinline__ double block_scan_f64(double v, double& excl, double& total) {
+  __shared__ double wsum[BS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  double ex = __shfl_up(inc, 1, 64);
+  if (lane == 0) ex = 0.0;
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  double base = 0.0, tot = 0.0;
+#pragma unroll
+  for (int i = 0; i < BS / 64; ++i) {
+    const double s = wsum[i];
+    base += (i < w) ? s : 0.0;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  excl = base + ex;
+  return base + inc;
+}
+
+// Redeposition CDF, pass 1: the total emission weight of every workgroup's kBS faces.
+__global__ void __launch_bounds__(kBS) k_redep_partials(const float* __restrict__ rate,
+                                                        const float* __restrict__ area, int nf,
+                                                        double* __restrict__ part) {
+  const int f = blockIdx.x * kBS + threadIdx.x;
+  double ex, tot;
+  block_scan_f64<kBS>(f < nf ? redep_weight(rate, area, f) : 0.0, ex, tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// Pass 2: exclusive prefix of the nb workgroup totals, in place, by one workgroup.
+__global__ void __launch_bounds__(1024) k_redep_scan_partials(double* __restrict__ part, int nb) {
+  double carry = 0.0;
+  for (int c0 = 0; c0 < nb; c0 += 1024) {
+    const int i = c0 + threadIdx.x;
+    double ex, tot;
+    block_scan_f64<1024>(i < nb ? part[i] : 0.0, ex, tot);
+    if (i < nb) part[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+// Pass 3: cum[f] = the workgroup's offset + the inclusive prefix within it.
+__global__ void __launch_bounds__(kBS) k_redep_cum(const float* __restrict__ rate,
+                                                   const float* __restrict__ area, int nf,
+                                                   const double* __restrict__ part,
+                                                   double* __restrict__ cum) {
+  const int f = blockIdx.x * kBS + threadIdx.x;
+  double ex, tot;
+  const double inc = block_scan_f64<kBS>(f < nf ? redep_weight(rate, area, f) : 0.0, ex, tot);
+  if (f < nf) cum[f] = part[blockIdx.x] + inc;
+}
+
+// The redeposited flux density (threed.py:1605): out = fl * (total / n) / max(area, 1e-9), or 0
+// when nothing was emitted; fl is the trace's per-face deposit, read from out.
+__global__ void k_redep_finish(float* __restrict__ out, const float* __restrict__ area,
+                               const double* __restrict__ cum, int nf, int n) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const double tot = cum[nf - 1];
+  const double a = fmax((double)area[f], 1.0e-9);
+  out[f] = tot > 0.0 ? (float)((double)out[f] * (tot / (double)n) / a) : 0.f;
+}
+
 __global__ void __launch_bounds__(kBS) k_trace_legacy(BvhView bv, Source src, Box b,
                                                       float sticking, int n_reemit, int specular,
                                                       float cos_thr, float eta, int n,
@@ -1163,6 +1305,54 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   rc = launch_neutral(ctx, pol, n);
   if (rc) return rc;
   return det_finish(ctx, q, fl, bvh->nf);
+}
+
+extern "C" int petch_trace_redeposition(petch_ctx* ctx, const petch_bvh* bvh,
+                                        const float* centroids, const float* gas_normals,
+                                        const float* areas, const float* rate, float s_redep,
+                                        int n, uint64_t seed, float* out) {
+  PETCH_ARG(ctx, ctx && bvh && centroids && gas_normals && areas && rate && out,
+            "trace_redeposition: null pointer");
+  PETCH_ARG(ctx, n >= 0, "trace_redeposition: n < 0");
+  const int nf = bvh->nf;
+  int rc = zero(ctx, out, nf);
+  if (rc) return rc;
+  if (n == 0 || nf == 0) return PETCH_OK;
+  const int nb = (int)petch_blocks(nf, kBS);
+  auto* cum = (double*)petch_scratch(ctx, SCR_REDEP, ((size_t)nf + (size_t)nb) * sizeof(double));
+  if (!cum) return PETCH_E_HIP;
+  double* part = cum + nf;
+  hipLaunchKernelGGL(k_redep_partials, dim3(nb), dim3(kBS), 0, ctx->stream, rate, areas, nf,
+                     part);
+  hipLaunchKernelGGL(k_redep_scan_partials, dim3(1), dim3(1024), 0, ctx->stream, part, nb);
+  hipLaunchKernelGGL(k_redep_cum, dim3(nb), dim3(kBS), 0, ctx->stream, rate, areas, nf, part,
+                     cum);
+  PETCH_LAUNCH_CHECK(ctx);
+  // open box: no lateral wrap (threed.py:1604 passes periodic = 0)
+  Redep pol{petch_bvh_view(bvh), Box{1.f, 1.f, 1.f, 1.f, 0}, centroids, gas_normals, cum, nf,
+            s_redep, out, seed, 0};
+  unsigned long long* q = nullptr;
+  if (ctx->deterministic) {
+    q = det_accumulators(ctx, 1, nf);
+    if (!q) return PETCH_E_HIP;
+    pol.fl = (float*)q;
+    pol.fixed = 1;
+  }
+  RayQueue rq;
+  rc = make_queue(ctx, n, rq);
+  if (rc) return rc;
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_redep, n);
+  hipLaunchKernelGGL(k_trace_redep, dim3(grid), dim3(kBS), 0, ctx->stream, pol, rq,
+                     (unsigned long long*)nullptr);
+  PETCH_LAUNCH_CHECK(ctx);
+  if (q) {
+    rc = det_finish(ctx, q, out, nf);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_redep_finish, dim3(petch_blocks(nf, kBS)), dim3(kBS), 0, ctx->stream, out,
+                     areas, cum, nf, n);
+  PETCH_LAUNCH_CHECK(ctx);
+  return PETCH_OK;
 }
 
 namespace {

@@ -105,6 +105,9 @@ class BatchLoop:
             buf = torch.empty((len(live), 3), dtype=torch.int32, device=loops[live[0]].dev)
             with self.timer.region("rate"):
                 ops.surface_rate_batch(rates, buf)
+            for r, i in enumerate(live):          # redeposition lowers V (and its max) first
+                if loops[i].redep:
+                    loops[i].redeposit(step, buf[r])
             host = buf.cpu().numpy()
             for r, i in enumerate(live):
                 vmx, nbad = ops.decode_rate_reduction(host[r, :2])

@@ -465,6 +465,24 @@ def trace_neutral_rr(bvh, box, src, bare, beta, n, seed, out=None):
     return fl
 
 
+def trace_redeposition(bvh, centroids, gas_normals, areas, rate, s_redep, n, seed):
+    """Redeposited flux density (F,) of mc_redep_3d (threed.py:1582-1605): n product rays
+    emitted from the faces in proportion to max(rate * area, 0), cosine about the into-gas
+    normals, re-sticking with probability s_redep (petch_trace_redeposition)."""
+    F = bvh.nf
+    for t, name, shape in ((centroids, "centroids", (F, 3)), (gas_normals, "gas_normals", (F, 3)),
+                           (areas, "areas", (F,)), (rate, "rate", (F,))):
+        if tuple(t.shape) != shape:
+            raise ValueError(f"trace_redeposition: {name} has shape {tuple(t.shape)}, "
+                             f"needs {shape}")
+    out = torch.empty((F,), dtype=f32, device=bvh.device)
+    bvh.ctx.call("petch_trace_redeposition", bvh.h, ptr(centroids, f32, "centroids"),
+                 ptr(gas_normals, f32, "gas_normals"), ptr(areas, f32, "areas"),
+                 ptr(rate, f32, "rate"), float(s_redep), int(n),
+                 int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(out, f32))
+    return out
+
+
 def trace_neutral_rr_adjoint(bvh, box, src, bare, beta, n, seed, gfl):
     """d loss / d bare (F,) of trace_neutral_rr(bvh, box, src, bare, beta, n, seed) for
     gfl = d loss / d fl, paths and roulette decisions held fixed (petch_trace_neutral_rr_adjoint)."""

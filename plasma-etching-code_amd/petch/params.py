@@ -2,8 +2,8 @@
 
 The keys, defaults and field names are petch's configuration interface (reference
 src/petch/params.py:13-76 for PAR, :80-127 for Flags) and are kept identical so existing
-scripts pass the same dicts. Keys that only feed subsystems outside the etch-loop hot path
-(Knudsen / DDA transport, charging, redeposition) are accepted and carried but not used here.
+scripts pass the same dicts. The opt-in physics keys (Knudsen / DDA transport, surface
+charging, redeposition) feed the same options of the device etch loop.
 """
 from dataclasses import dataclass, asdict
 import math
@@ -33,7 +33,7 @@ PAR = {
     "dda_n_dir": 64, "dda_n_reemit": 12,
     # ion energy distribution used by the analytic yields
     "ied_mode": "mean", "ied_dE": 40.0,
-    # redeposition and charging (outside the etch-loop hot path)
+    # redeposition and surface charging (opt-in)
     "s_redep": 0.5, "k_redep": 1.0,
     "charge_alpha": 0.0, "eFlux": 1.0, "e_ang_sigma": 0.5,
 }

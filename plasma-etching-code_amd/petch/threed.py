@@ -203,6 +203,32 @@ def mc_flux_3d(mesh, verts, faces, areas, geo, par, n_ion=20000, n_neu=20000, se
     return tuple(_np(out[k]) for k in ("m_i", "m_F", "m_O", "cos_i"))
 
 
+def _gas_normals(verts, faces, centroids, geo):
+    """Face normals oriented into the gas (threed.py:1553-1568): numpy (F, 3)."""
+    v = _t(np.asarray(verts, np.float32).reshape(-1, 3))
+    f = _t(np.asarray(faces, np.int32).reshape(-1, 3), torch.int32)
+    _, _, nrm = ops.mesh_attrs(v, f)
+    phi = geo.get('_phi_d')
+    phi = _t(geo['phi']) if phi is None else phi
+    return _np(ops.gas_normals(phi, geo['dx'], _t(np.asarray(centroids).reshape(-1, 3)), nrm))
+
+
+def mc_redep_3d(mesh, centroids, areas, normals, src_flux, n_redep=20000, s_redep=0.5, seed=0):
+    """mc_redep_3d (threed.py:1582-1605): the redeposited flux density per face (numpy float64)
+    of n_redep product rays emitted in proportion to max(src_flux * areas, 0) along the
+    into-gas normals and re-sticking with probability s_redep. mesh: a Mesh of the surface."""
+    if not isinstance(mesh, Mesh):
+        raise TypeError("mc_redep_3d: mesh must be a petch.threed.Mesh of the surface")
+    F = mesh.nf
+    c, a = _t(np.asarray(centroids).reshape(-1, 3)), _t(np.asarray(areas).reshape(-1))
+    n, v = _t(np.asarray(normals).reshape(-1, 3)), _t(np.asarray(src_flux).reshape(-1))
+    if F == 0:
+        return np.zeros(0)
+    out = ops.trace_redeposition(mesh.bvh, c, n, a, v, float(s_redep), int(n_redep),
+                                 (int(seed) * 7 + 5) & _flux.SEED_MASK)
+    return _np(out)
+
+
 def gpu_warmstart_bare(prev_mesh, prev_bare, centroids):
     """Coverage seed from the nearest face of the previous mesh (threed.py:1633-1639)."""
     return _np(ops.nearest_face_value(prev_mesh.bvh, _t(prev_bare), _t(np.asarray(centroids, np.float32)), 1.0))
@@ -354,8 +380,7 @@ class EtchLoop:
             self.mc_par = dict(par)
             self.mc_par['s_F'] = par['betaE']
             self.mc_par['s_O'] = par['betaO']
-        if getattr(flags, "redeposition", False):
-            raise NotImplementedError("redeposition (threed.py:1730-1743) is outside the HIP etch loop")
+        self.redep = bool(getattr(flags, "redeposition", False))
         self.nt = getattr(flags, "neutral_transport", "mc")
         self.dev = phi_d.device
         self.phi = phi_d
@@ -521,6 +546,8 @@ class EtchLoop:
     def end_step(self, step, out):
         """Rate, extension, CFL, advection and reinit of the step begun by begin_step."""
         if self.end_rate(step, out):
+            if self.redep:
+                self.redeposit(step, self.red)
             vmx, nbad = ops.read_rate_reduction(self.red)
             return self.end_finish(step, vmx, nbad, self._mesh.bvh.status())
         return self.end_finish(step, 0.0, 0, 0)
@@ -551,6 +578,28 @@ class EtchLoop:
                                               red=self.red)
         self.timings['rate'] = self.timings.get('rate', 0.0) + time.time() - trt
         return True
+
+    def redeposit(self, step, red):
+        """Etch-product redeposition (threed.py:1730-1743) on this step's rate V: n_neu product
+        rays emitted in proportion to V * area (stream step * 7 + 5) give the redeposited flux
+        Rf (zero on the sidewall-foot band when surface charging is on and an etch stop is set),
+        then V = max(V - k_redep Rf, 0) in place and red[0] = the bits of the new max V, so the
+        CFL sees the reduced rate. No host read."""
+        mesh, V = self._mesh, self._V
+        if V is None or mesh.nf == 0:
+            return
+        with self.timer.region("redep"):
+            gn = ops.gas_normals(self.phi, self.dx, mesh.centroids, mesh.normals)
+            Rf = ops.trace_redeposition(mesh.bvh, mesh.centroids, gn, mesh.areas, V,
+                                        float(self.par.get('s_redep', 0.5)), int(self.n_neu),
+                                        (step * 7 + 5) & _flux.SEED_MASK)
+            if getattr(self.flags, "surface_charging", False) and self.etch_stop_z is not None:
+                foot_h = max(0.3 * self.geo['trench_width'], 2.0 * self.dx)
+                foot = mesh.centroids[:, 2].double() <= float(self.etch_stop_z) + foot_h
+                Rf = torch.where(foot, torch.zeros_like(Rf), Rf)
+            k = float(self.par.get('k_redep', 1.0))
+            V.copy_((V.double() - k * Rf.double()).clamp_(min=0.0))
+            red[0] = V.max().reshape(1).view(torch.int32)[0]
 
     def extend_item(self):
         """(bvh, V, phi, dx, band, Fs) of this step's velocity extension for ops.extend_batch,

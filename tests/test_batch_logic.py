@@ -12,6 +12,7 @@ from petch.threed import _FluxSteps
 class FakeLoop:
     def __init__(self, k, n_fp):
         self.k, self.n_fp, self.log = k, n_fp, []
+        self.redep = k == 1                  # this feature has flags.redeposition
 
     def mesh_item(self, step):
         return ("mc", self.k)
@@ -41,6 +42,9 @@ class FakeLoop:
         if out is not None:
             rates.append(("rate", self.k))
         return out is not None
+
+    def redeposit(self, step, red):
+        red[0] = 100 + self.k                # stands for the max of the lowered rate
 
     def extend_item(self):
         return ("ext", self.k)
@@ -120,7 +124,9 @@ def test_lockstep_batches_each_kind(monkeypatch):
     monkeypatch.setattr(B.ops, "reinit_fsm_batch", re_batch)
     monkeypatch.setattr(B.ops, "finish_round_batch", round_batch)
     monkeypatch.setattr(B.ops, "finish_ion_batch", ion_finish_batch)
-    monkeypatch.setattr(B.ops, "decode_rate_reduction", lambda r: (1.0, 0))
+    decoded = []
+    monkeypatch.setattr(B.ops, "decode_rate_reduction",
+                        lambda r: decoded.append(int(r[0])) or (1.0, 0))
 
     def rate_batch(items, red):
         calls.append(("rate", list(items)))
@@ -149,6 +155,9 @@ def test_lockstep_batches_each_kind(monkeypatch):
     assert [lp.mc for lp in loops] == [("mesh", 0), ("mesh", 1), ("mesh", 99), ("mesh", 2)]
     assert [lp.bare_init for lp in loops] == [None, ("bare", 1), None, None]
     assert [lp.extended for lp in loops] == [True, True, False, True]
+    # the redepositing feature lowers its rate (and the max the CFL reads) after the batched
+    # rate launch, before the host read
+    assert decoded == [0, 101, 0]
     assert outs == [dict(k=0, n=1), dict(k=1, n=4), None, dict(k=2, n=1)]
 
 

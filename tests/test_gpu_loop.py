@@ -203,18 +203,6 @@ def test_api_hole_runs_and_saves(tmp_path):
     assert sum(1 for ln in open(p) if ln.startswith("f ")) == len(f)
 
 
-@pytest.mark.parametrize("kw,flag_kw", [
-    ({}, dict(redeposition=True)),
-])
-def test_out_of_path_options_raise(kw, flag_kw):
-    """Options outside the etch-loop hot path fail loudly instead of silently changing physics."""
-    from petch import threed as t3
-    fl = Flags(**{**FAITHFUL.to_dict(), **flag_kw})
-    with pytest.raises(NotImplementedError):
-        t3.run_etch_3d(**{**SMOKE, "n_steps": 1}, par=dict(PAR), flags=fl, n_ion=1000, n_neu=1000,
-                       verbose=False, **{"reinit_method": "fsm", **kw})
-
-
 def test_differentiable_step_matches_finite_differences():
     """A loss on the level set after a differentiable step on a fixed trench surface: the
     radiosity coverage loop (two rounds), the surface rate, the extension and two advection
