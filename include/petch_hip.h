@@ -298,6 +298,13 @@ int petch_gen_source(petch_ctx* ctx, const petch_box* box, const petch_source* s
 int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
                            const petch_source* src, const float* bare, float beta, int n,
                            uint64_t seed, float* fl);
+/* _trace3d_cov (threed.py:511-546), the capped trace of cov_transport = 'cap': every hit
+ * deposits 1 into fl; the ray sticks with probability bare[face]*beta (word 0 of its bounce
+ * block) and ends at the latest at its (n_reemit + 1)-th hit; cosine re-emission otherwise. No
+ * lateral wrap (the reference kernel applies no boundary condition): box->periodic is ignored. */
+int petch_trace_neutral_cap(petch_ctx* ctx, const petch_bvh* bvh, const petch_box* box,
+                            const petch_source* src, const float* bare, float beta, int n_reemit,
+                            int n, uint64_t seed, float* fl);
 /* mc_redep_3d (threed.py:1582-1605): n product rays leave face f with probability
  * max(rate_f areas_f, 0) / total (non-finite products count as 0) from centroid + 1e-3 n_f,
  * n_f = gas_normals[f] (unit, into the gas), in the cosine lobe about n_f, then follow the RR

@@ -141,6 +141,15 @@ class OracleBvh:
                                       ctypes.c_uint64(seed), _p(out))
         return out
 
+    def trace_neutral_cap(self, box, src, bare, beta, n_reemit, n, seed):
+        """Per-face arrivals (float64 (F,)) of oracle_trace_neutral_cap."""
+        bare = np.ascontiguousarray(bare, np.float32)
+        out = np.zeros(max(self.nf, 1), np.float64)
+        lib().oracle_trace_neutral_cap(self.h, ctypes.byref(box), ctypes.byref(src), _p(bare),
+                                       ctypes.c_float(beta), ctypes.c_int(n_reemit),
+                                       ctypes.c_int(n), ctypes.c_uint64(seed), _p(out))
+        return out[:self.nf]
+
     def trace_redeposition(self, cen, gn, area, rate, s_redep, n, seed):
         """Redeposited flux density (float64 (F,)) of oracle_trace_redeposition."""
         c = np.ascontiguousarray(cen, np.float32)

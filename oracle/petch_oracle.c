@@ -716,6 +716,45 @@ void oracle_trace_neutral_rr_paths(void* h, const oc_box* b, const oc_source* s,
   }
 }
 
+/* _trace3d_cov (threed.py:511-546), cov_transport = 'cap': 1 per hit, stick with probability
+ * bare * beta (word 0 of the bounce's block), at most n_reemit re-emissions, no boundary
+ * condition (the box's periodic flag is ignored); per-face sums in double. */
+void oracle_trace_neutral_cap(void* h, const oc_box* b, const oc_source* s, const float* bare,
+                              float beta, int n_reemit, int n, uint64_t seed, double* fl) {
+  const oc_bvh* B = (const oc_bvh*)h;
+  int F = B->nf;
+  oc_box open = *b;
+  open.periodic = 0;
+  memset(fl, 0, sizeof(double) * (F > 0 ? F : 1));
+#pragma omp parallel
+  {
+    double* acc = (double*)calloc(F > 0 ? F : 1, sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+    for (int p = 0; p < n; ++p) {
+      oc_rng r;
+      oc_rng_init(&r, seed, (uint32_t)p);
+      f3 o, d, nn;
+      oc_source_ray(&r, &open, s, &o, &d);
+      float t;
+      for (int bounce = 0; bounce <= n_reemit; ++bounce) {
+        int f = oc_query_bc(B, &open, &o, d, &t, &nn);
+        if (f < 0) break;
+        if (dot3(nn, d) > 0.f) nn = scl(-1.f, nn);
+        acc[f] += 1.0;
+        uint32_t u[4];
+        oc_block(seed, (uint32_t)p, (uint32_t)bounce + 1u, 1u, u);
+        if (oc_u24(u[0]) < bare[f] * beta || bounce == n_reemit) break;
+        f3 hit = add3(o, scl(t, d));
+        d = oc_cosine_u(nn, oc_u24(u[1]), oc_u24(u[2]));
+        o = add3(hit, scl(1.0e-4f, nn));
+      }
+    }
+#pragma omp critical
+    for (int f = 0; f < F; ++f) fl[f] += acc[f];
+    free(acc);
+  }
+}
+
 /* Redeposition (mc_redep_3d, threed.py:1582-1605). The emission weight of face f is
  * max(rate_f area_f, 0), a non-finite product counting as 0; cum holds its running sum in order
  * (float64). Ray p draws its face from word 0 and 1 of its block 0 as a 53-bit uniform x: the

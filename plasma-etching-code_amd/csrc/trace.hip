@@ -516,6 +516,46 @@ struct Redep {
   }
 };
 
+// _trace3d_cov (threed.py:511-546), the capped coverage trace of cov_transport = 'cap': every
+// hit deposits 1; the ray sticks with probability S = bare[face] * beta (word 0 of its bounce
+// block against S) and stops at the latest after n_reemit re-emissions; otherwise cosine
+// re-emission as in the RR walk. The reference launches it with no boundary handling, so the
+// box never wraps. Streams as NeutralRR's: source from block 0, bounce k from block (k, ray, 1).
+struct NeutralCap {
+  static constexpr bool kReplay = false;
+  BvhView bv;
+  Box b;
+  Source src;
+  const float* __restrict__ bare;
+  float beta;
+  int n_reemit;
+  float* fl;
+  uint64_t seed;
+  int fixed;
+  struct Ray {
+    int bounce;
+    unsigned ray;
+  };
+  __device__ __forceinline__ const BvhView& view(const Ray&) const { return bv; }
+  __device__ __forceinline__ const Box& box(const Ray&) const { return b; }
+  __device__ __forceinline__ void start(Ray& R, unsigned ray, Vec3& o, Emit& e) const {
+    neutral_source(seed, ray, b, src, o, e);
+    R.bounce = 0;
+    R.ray = ray;
+  }
+  __device__ __forceinline__ bool hit(Ray& R, const Hit& h, Vec3& o, Vec3 d, Emit& e) const {
+    Vec3 nn = hit_normal(bv, h.slot);
+    if (petch::dot(nn, d) > 0.f) nn = -nn;
+    deposit(fl, h.face, 1.f, fixed);
+    uint32_t u[4];
+    petch::philox_block(seed, R.ray, (uint32_t)R.bounce + 1u, 1u, u);
+    if (petch::u24(u[0]) < bare[h.face] * beta || R.bounce >= n_reemit) return false;
+    ++R.bounce;
+    rr_emit(h, nn, u, o, d, e);
+    return true;
+  }
+};
+
 // Path-replay adjoint of the trace (no reference counterpart): for gfl = d loss / d fl it
 // accumulates d loss / d bare into gbare, with every ray following the path the trace drew (same
 // Philox stream, same roulette and termination decisions, same geometry). Along a ray the
@@ -1035,6 +1075,11 @@ k_trace_redep(Redep pol, RayQueue q, unsigned long long* __restrict__ hits) {
   persistent_trace(pol, q, hits);
 }
 
+__global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(5)))
+k_trace_neutral_cap(NeutralCap pol, RayQueue q, unsigned long long* __restrict__ hits) {
+  persistent_trace(pol, q, hits);
+}
+
 // Emission weight of a face for redeposition: max(rate * area, 0), a non-finite product as 0.
 __device__ __forceinline__ double redep_weight(const float* __restrict__ rate,
                                                const float* __restrict__ area, int f) {
@@ -1305,6 +1350,36 @@ extern "C" int petch_trace_neutral_rr(petch_ctx* ctx, const petch_bvh* bvh, cons
   rc = launch_neutral(ctx, pol, n);
   if (rc) return rc;
   return det_finish(ctx, q, fl, bvh->nf);
+}
+
+extern "C" int petch_trace_neutral_cap(petch_ctx* ctx, const petch_bvh* bvh,
+                                       const petch_box* box, const petch_source* src,
+                                       const float* bare, float beta, int n_reemit, int n,
+                                       uint64_t seed, float* fl) {
+  PETCH_ARG(ctx, ctx && bvh && box && src && bare && fl, "trace_neutral_cap: null pointer");
+  PETCH_ARG(ctx, n >= 0, "trace_neutral_cap: n < 0");
+  PETCH_ARG(ctx, n_reemit >= 0, "trace_neutral_cap: n_reemit < 0");
+  int rc = zero(ctx, fl, bvh->nf);
+  if (rc) return rc;
+  if (n == 0 || bvh->nf == 0) return PETCH_OK;
+  Box b = make_box(box);
+  b.periodic = 0;  // _trace3d_cov applies no boundary condition
+  NeutralCap pol{petch_bvh_view(bvh), b, make_source(src), bare, beta, n_reemit, fl, seed, 0};
+  unsigned long long* q = nullptr;
+  if (ctx->deterministic) {
+    q = det_accumulators(ctx, 1, bvh->nf);
+    if (!q) return PETCH_E_HIP;
+    pol.fl = (float*)q;
+    pol.fixed = 1;
+  }
+  RayQueue rq;
+  rc = make_queue(ctx, n, rq);
+  if (rc) return rc;
+  const unsigned grid = persistent_blocks(ctx, (const void*)k_trace_neutral_cap, n);
+  hipLaunchKernelGGL(k_trace_neutral_cap, dim3(grid), dim3(kBS), 0, ctx->stream, pol, rq,
+                     petch_hit_counter(ctx, 0));
+  PETCH_LAUNCH_CHECK(ctx);
+  return q ? det_finish(ctx, q, fl, bvh->nf) : PETCH_OK;
 }
 
 extern "C" int petch_trace_redeposition(petch_ctx* ctx, const petch_bvh* bvh,

@@ -201,6 +201,8 @@ SIGNATURES = {
     "petch_bvh_status_async": [P, P, P],
     "petch_bvh_root_bounds": [P, P, P],
     "petch_gen_source": [P, POINTER(Box), POINTER(Source), c_int, c_uint64, P, P],
+    "petch_trace_neutral_cap": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int, c_int,
+                                c_uint64, P],
     "petch_trace_redeposition": [P, P, P, P, P, P, c_float, c_int, c_uint64, P],
     "petch_trace_neutral_rr": [P, P, POINTER(Box), POINTER(Source), P, c_float, c_int, c_uint64,
                                P],

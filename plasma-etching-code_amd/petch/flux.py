@@ -185,8 +185,7 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
         gn = ops.gas_normals(phi, geo['dx'], mesh.centroids, mesh.normals)
         m_i, ion_yield = _charging.floor_throttle(mesh, gn, geo, m_i, ion_yield)
 
-    if par.get('cov_transport', 'rr') != 'rr':
-        raise NotImplementedError("cov_transport='cap' (_trace3d_cov) is not part of the HIP path")
+    cap = par.get('cov_transport', 'rr') != 'rr'
     chem = chem_params(par, flags)
 
     n_fp = int(par.get('n_fp', 1 if bare_init is not None else 4))
@@ -200,8 +199,17 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
         # the F and O traces of one iteration (threed.py:876-880), seeds seed*9+2+2it and
         # seed*9+3+2it, as one launch
         sF, sO = (seed * 9 + 2 + 2 * it) & SEED_MASK, (seed * 9 + 3 + 2 * it) & SEED_MASK
-        fF, fO = yield ("neutral", dict(bvh=mesh.bvh, box=box, bare=bare, beta_a=betaE,
-                                        n_a=n_neu, seed_a=sF, beta_b=betaO, n_b=n_neu, seed_b=sO))
+        if cap:
+            # cov_transport='cap' (threed.py:881-885): the capped trace, one launch per species,
+            # re-emissions capped at n_reemit_cov or clip(8 / max(beta, 0.02), 24, 200)
+            fF, fO = (ops.trace_neutral_cap(
+                mesh.bvh, box, NEUTRAL_SOURCE, bare, b,
+                int(par.get('n_reemit_cov', min(max(8.0 / max(b, 0.02), 24.0), 200.0))), n_neu, s)
+                for b, s in ((betaE, sF), (betaO, sO)))
+        else:
+            fF, fO = yield ("neutral", dict(bvh=mesh.bvh, box=box, bare=bare, beta_a=betaE,
+                                            n_a=n_neu, seed_a=sF, beta_b=betaO, n_b=n_neu,
+                                            seed_b=sO))
         if rounds is not None and n_smooth <= 1:
             m_F, m_O, bare = (torch.empty_like(fF) for _ in range(3))
             rounds.append(dict(kind="neutral", flF=fF, flO=fO, areas=A, nbr=mesh.nbr,

@@ -465,6 +465,19 @@ def trace_neutral_rr(bvh, box, src, bare, beta, n, seed, out=None):
     return fl
 
 
+def trace_neutral_cap(bvh, box, src, bare, beta, n_reemit, n, seed):
+    """Capped coverage-trace arrivals (F,) of _trace3d_cov (threed.py:511-546): 1 per hit,
+    sticking with probability bare * beta, at most n_reemit re-emissions, no lateral wrap."""
+    import ctypes
+    if bare.numel() < bvh.nf:
+        raise ValueError(f"trace_neutral_cap: bare holds {bare.numel()} values, needs {bvh.nf}")
+    fl = torch.empty((bvh.nf,), dtype=f32, device=bvh.device)
+    bvh.ctx.call("petch_trace_neutral_cap", bvh.h, ctypes.byref(box), ctypes.byref(src),
+                 ptr(bare, f32, "bare"), float(beta), int(n_reemit), int(n),
+                 int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(fl, f32))
+    return fl
+
+
 def trace_redeposition(bvh, centroids, gas_normals, areas, rate, s_redep, n, seed):
     """Redeposited flux density (F,) of mc_redep_3d (threed.py:1582-1605): n product rays
     emitted from the faces in proportion to max(rate * area, 0), cosine about the into-gas

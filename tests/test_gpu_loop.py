@@ -203,6 +203,23 @@ def test_api_hole_runs_and_saves(tmp_path):
     assert sum(1 for ln in open(p) if ln.startswith("f ")) == len(f)
 
 
+def test_capped_coverage_trace_runs_in_both_loops():
+    """par['cov_transport'] = 'cap' (threed.py:881-885): the coupled flux's neutrals come from
+    the capped trace; the loop and the batched loop run, etch, and agree after one step."""
+    from petch import batch, threed as t3
+    par = dict(PAR, cov_transport="cap", n_reemit_cov=30)
+    kw = dict(SMOKE, n_steps=1, t_end=SMOKE['t_end'] / SMOKE['n_steps'])
+    print("loop cov_transport=cap", flush=True)
+    g = t3.run_etch_3d(**kw, par=par, flags=FAITHFUL, n_ion=4000, n_neu=4000, verbose=False,
+                       reinit_method="fsm")
+    assert np.isfinite(g['phi']).all() and t3.max_depth_3d(g) > 0.0
+    spec = {k: v for k, v in kw.items() if k not in ("t_end", "n_steps")}
+    gb, = batch.run_etch_3d_batch([dict(spec, par=par, flags=FAITHFUL, n_ion=4000, n_neu=4000)],
+                                  t_end=kw['t_end'], n_steps=1)
+    band = np.abs(g['phi']) < 3 * kw['dx']
+    assert np.abs(gb['phi'] - g['phi'])[band].max() <= 0.01 * kw['dx']
+
+
 def test_differentiable_step_matches_finite_differences():
     """A loss on the level set after a differentiable step on a fixed trench surface: the
     radiosity coverage loop (two rounds), the surface rate, the extension and two advection

@@ -95,6 +95,42 @@ def test_neutral_rr_parity(name):
 
 
 @pytest.mark.parametrize("name", ["hole", "trench"])
+def test_neutral_cap_parity(name):
+    """cov_transport='cap' (_trace3d_cov, threed.py:511-546) against the oracle."""
+    import torch
+    from petch import ops
+    g, v, f, bvh, box, obox, obvh, dev = _setup(name)
+    rng = np.random.default_rng(2)
+    bare = rng.uniform(0.3, 1.0, len(f)).astype(np.float32)
+    tb = torch.from_numpy(bare).to(dev)
+    src, osrc = ops.make_source(0), on.Source(0, 0, 0)
+    n, nre = 40000, 24
+    print(f"neutral_cap {name} nf={len(f)}", flush=True)
+    gs = ops.trace_neutral_cap(bvh, box, src, tb, 0.7, nre, n, 4321).cpu().numpy()
+    os_ = obvh.trace_neutral_cap(obox, osrc, bare, 0.7, nre, n, 4321)
+    gb = [ops.trace_neutral_cap(bvh, box, src, tb, 0.7, nre, n, 600 + k).cpu().numpy()
+          for k in range(N_BATCH)]
+    ob = [obvh.trace_neutral_cap(obox, osrc, bare, 0.7, nre, n, 1000 + k) for k in range(N_BATCH)]
+    assert bvh.status() == 0
+    _check(gs, os_, gb, ob, f"neutral_cap/{name}")
+
+
+def test_neutral_cap_closed_cavity_counts():
+    """In a closed cavity with S = 0 every ray deposits exactly n_reemit + 1 times."""
+    import torch
+    from petch import ops
+    from test_oracle_traces import _cube_cavity
+    v, f, c = _cube_cavity()
+    dev = torch.device("cuda", 0)
+    bvh = ops.Bvh(dev).build(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev))
+    box = ops.make_box(2.5, 2.5, 3.0, float(c), 1)
+    n, nre = 30000, 7
+    fl = ops.trace_neutral_cap(bvh, box, ops.make_source(0), torch.zeros(len(f), device=dev),
+                               0.7, nre, n, 5)
+    assert float(fl.double().sum()) == n * (nre + 1)
+
+
+@pytest.mark.parametrize("name", ["hole", "trench"])
 def test_neutral_rr_pair_equals_two_launches(name):
     """The paired F+O launch carries each ray on the stream it has in a single launch, so each
     output equals the single launch's up to the order of the float atomic adds; the oracle's

@@ -213,3 +213,51 @@ def test_rr_path_adjoint_matches_common_random_number_differences():
         lm = g @ bvh.trace_neutral_rr(box, src, (bare - h * u).astype(np.float32), beta, n, 7)
         fd = (lp - lm) / (2 * h)
         assert abs(gb @ u - fd) <= 0.02 * abs(fd), (fd, gb @ u)
+
+
+def _cube_cavity(R=1.5, dx=0.25):
+    """MC mesh of a gas-filled cube of half-width R in solid, shifted so that its x, y range is
+    [-0.25, 2R - 0.25]: a source plane [0, 2R - 0.5]^2 at its mid height lies inside it."""
+    n = int(np.ceil(2 * R / dx)) + 5
+    c = 0.5 * (n - 1) * dx
+    x = np.arange(n) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    phi = (np.maximum(np.maximum(np.abs(X - c), np.abs(Y - c)), np.abs(Z - c)) - R)
+    v, f = on.marching_cubes(phi.astype(np.float32), dx)
+    v = v.copy()
+    v[:, :2] -= c - R + 0.25
+    return v, f, c
+
+
+def test_capped_trace_closed_forms():
+    """_trace3d_cov (threed.py:511-546): in a closed cavity every ray deposits 1 per hit until it
+    sticks or reaches the cap, so with S = 0 exactly n_reemit + 1 per ray, with S = 1 exactly 1,
+    and with S = 1/2 on average (1 - 2^-(n_reemit + 1)) / (1/2); on a flat field wide enough to
+    catch every source ray each ray deposits exactly once (the re-emission escapes). The box's
+    periodic flag changes nothing (the reference kernel applies no boundary condition)."""
+    v, f, c = _cube_cavity()
+    bvh = on.OracleBvh(v, f)
+    box, src = on.Box(2.5, 2.5, 3.0, float(c), 0), on.Source(0, 0, 0)
+    n, nre = 20000, 5
+    ones = np.ones(len(f), np.float32)
+    assert bvh.trace_neutral_cap(box, src, np.zeros(len(f), np.float32), 0.7, nre, n, 3).sum() \
+        == n * (nre + 1)
+    assert bvh.trace_neutral_cap(box, src, ones, 1.0, nre, n, 3).sum() == n
+    half = bvh.trace_neutral_cap(box, src, ones, 0.5, nre, n, 3).sum() / n
+    mean = (1.0 - 0.5 ** (nre + 1)) / 0.5
+    assert abs(half - mean) < 5 * 1.0 / np.sqrt(n)                 # per-ray sd below 1
+    wrap = bvh.trace_neutral_cap(on.Box(2.5, 2.5, 3.0, float(c), 1), src, ones, 0.5, nre, n, 3)
+    np.testing.assert_array_equal(wrap, bvh.trace_neutral_cap(box, src, ones, 0.5, nre, n, 3))
+    m = 20.0
+    vf, ff = flat_mesh(4.0, 3.0, 1.0, margin=m)
+    fb = on.OracleBvh(vf, ff)
+    fbox = on.Box(4.0, 3.0, 6.0, 5.75, 0)
+    o, d = on.gen_source(fbox, src, 5000, 9)
+    t = (1.0 - o[:, 2]) / d[:, 2]                    # every source ray that lands on the plane
+    px, py = o[:, 0] + t * d[:, 0], o[:, 1] + t * d[:, 1]
+    lands = ((px > -m + 1e-3) & (px < 4.0 + m - 1e-3) & (py > -m + 1e-3) &
+             (py < 3.0 + m - 1e-3)).sum()
+    assert 4500 < lands < 5000
+    for beta in (0.0, 0.5):
+        fl = fb.trace_neutral_cap(fbox, src, np.ones(len(ff), np.float32), beta, 24, 5000, 9)
+        assert fl.sum() == lands
