@@ -620,21 +620,33 @@ __device__ __forceinline__ int locate_item(const ExtendItem* __restrict__ it, in
 constexpr int kBandRounds = 8;
 constexpr long long kBandTile = (long long)kBandRounds * kBS;
 
-__device__ __forceinline__ bool band_node(const ExtendItem* __restrict__ it, int n, long long g,
-                                          long long total, int zero_fs) {
+// A tile's nodes are found from the item of its first node, c0 (one search per workgroup, the
+// same for all its lanes): a node moves on to later items only past their ends. A search per
+// node (six dependent table loads each, eight nodes a lane) made the two passes latency-bound.
+__device__ __forceinline__ bool band_node(const ExtendItem* __restrict__ it, int n, int c0,
+                                          long long g, long long total, int zero_fs) {
   if (g >= total) return false;
-  long long m;
-  const ExtendItem& e = it[locate_item(it, n, g, m)];
+  int c = c0;
+  while (c < n - 1 && it[c].end <= g) ++c;
+  const long long m = g - (c ? it[c - 1].end : 0);
+  const ExtendItem& e = it[c];
   if (zero_fs) e.Fs[m] = 0.f;
   return fabs((double)e.phi[m]) < e.band;
+}
+
+__device__ __forceinline__ int tile_item(const ExtendItem* __restrict__ it, int n) {
+  long long m;
+  return locate_item(it, n, (long long)blockIdx.x * kBandTile, m);
 }
 
 __global__ void k_band_count(const ExtendItem* __restrict__ it, int n, long long total,
                              int* __restrict__ tile_cnt, int zero_fs) {
   const long long g0 = (long long)blockIdx.x * kBandTile + threadIdx.x;
+  const int c0 = tile_item(it, n);
   int c = 0;
 #pragma unroll
-  for (int r = 0; r < kBandRounds; ++r) c += band_node(it, n, g0 + (long long)r * kBS, total, zero_fs);
+  for (int r = 0; r < kBandRounds; ++r)
+    c += band_node(it, n, c0, g0 + (long long)r * kBS, total, zero_fs);
   const int s = petch::block_sum<kBS>(c);
   if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s;
 }
@@ -643,9 +655,10 @@ __global__ void k_band_emit(const ExtendItem* __restrict__ it, int n, long long 
                             const int* __restrict__ tile_off, int* __restrict__ list) {
   const long long g0 = (long long)blockIdx.x * kBandTile + threadIdx.x;
   int base = tile_off[blockIdx.x];
+  const int c0 = tile_item(it, n);
   for (int r = 0; r < kBandRounds; ++r) {
     const long long g = g0 + (long long)r * kBS;
-    const bool in = band_node(it, n, g, total, 0);
+    const bool in = band_node(it, n, c0, g, total, 0);
     int tot;
     const int ex = petch::block_excl_scan<kBS>(in ? 1 : 0, tot);
     if (in) list[base + ex] = (int)g;

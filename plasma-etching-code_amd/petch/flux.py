@@ -112,7 +112,7 @@ NEUTRAL_SOURCE = ops.make_source(ops.SRC_NEUTRAL)
 
 
 def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, rounds=None,
-                       phi=None):
+                       phi=None, bare_in_unit=False):
     """mc_flux_3d_coupled (threed.py:740-905) as a generator over its ray traces.
 
     It yields ("ion", request) and ("neutral", request) dicts and is sent the trace outputs
@@ -129,7 +129,9 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
     before it reads the result. Used when smoothing takes at most one pass (the default
     flux_smooth = 1) and no charging hook needs the ion arrival at once.
 
-    phi: the level set on the device, needed by the floor-charge throttle (its gas normals)."""
+    phi: the level set on the device, needed by the floor-charge throttle (its gas normals).
+    bare_in_unit: bare_init is float32 and already in [0, 1] (a warm start from the previous
+    step's coverage), so its clip to [0, 1] is the identity and is skipped."""
     F = mesh.nf
     Lx, Ly = geo['Lx'], geo['Ly']
     A_src = Lx * Ly
@@ -192,7 +194,7 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
     if bare_init is None:
         bare = torch.ones((F,), dtype=torch.float32, device=mesh.device)
     else:
-        bare = torch.clamp(bare_init.to(torch.float32), 0.0, 1.0)
+        bare = bare_init if bare_in_unit else torch.clamp(bare_init.to(torch.float32), 0.0, 1.0)
     if n_fp <= 0:
         m_F = m_O = torch.zeros((F,), dtype=torch.float32, device=mesh.device)
     for it in range(n_fp):

@@ -537,7 +537,8 @@ class EtchLoop:
                     return _FluxSteps(_flux.coupled_flux_steps(mesh, self.geo, par, n_ion_s,
                                                                n_neu_s, seed, flags,
                                                                bare_init=bi, rounds=rounds,
-                                                               phi=self.phi))
+                                                               phi=self.phi,
+                                                               bare_in_unit=True))
                 return _flux.uncoupled_flux(mesh, self.geo, self.mc_par, n_ion_s, n_neu_s, seed,
                                             getattr(flags, "ion_reflection", False))
         finally:
