@@ -900,7 +900,9 @@ int ploc_build(petch_ctx* ctx, petch_bvh* b, int nf) {
 namespace {
 
 // Grows the BVH's device buffers to hold nf faces (they are never shrunk); a BVH that was never
-// built gets a minimal set, so its error flag is valid.
+// built gets a minimal set, so its error flag is valid. A regrowth waits for the stream and
+// frees and allocates four buffers (hundreds of microseconds with the device idle), and an
+// etching surface keeps growing, so the new capacity is twice the request.
 int ensure_capacity(petch_ctx* ctx, petch_bvh* b, int nf) {
   if (nf > b->cap_f) {
     PETCH_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -909,7 +911,7 @@ int ensure_capacity(petch_ctx* ctx, petch_bvh* b, int nf) {
     if (b->tris) (void)hipFree(b->tris);
     if (b->work) (void)hipFree(b->work);
     b->nodes = b->wide = b->tris = b->work = nullptr;
-    int cap = nf + nf / 2 + 256;
+    int cap = nf > (0x7fffffff - 1024) / 2 ? nf : 2 * nf + 1024;
     PETCH_HIP(ctx, hipMalloc(&b->nodes, (size_t)cap * sizeof(BvhNode)));
     PETCH_HIP(ctx, hipMalloc(&b->wide, (size_t)cap * sizeof(BvhWide)));
     PETCH_HIP(ctx, hipMalloc(&b->tris, (size_t)cap * 48));
