@@ -190,24 +190,52 @@ __device__ __forceinline__ void load_cell(const float* __restrict__ phi, const G
   for (int c = 0; c < 8; ++c) f[c] = phi[n + corner_off(g, c)];
 }
 
+// Sign mask of the cell at node n (bit c: corner c solid), or 0 for a node that starts no cell.
+__device__ __forceinline__ int cell_signs(const float* __restrict__ phi, const Grid& g,
+                                          long long n, int i, int j, int k) {
+  if (i >= g.nx - 1 || j >= g.ny - 1 || k >= g.nz - 1) return 0;
+  int s = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s |= (phi[n + corner_off(g, c)] > 0.f ? 1 : 0) << c;
+  return s;
+}
+
+// The cells of a tile that the surface crosses (sign mask neither 0 nor 255), in node order:
+// every round's crossing cells are appended to the workgroup's list `cells` (node ids, which fit
+// 32 bits), and the triangle work then runs over the list with every lane on a crossing cell.
+// Without the list a wavefront ran the long per-cell loop walk whenever one of its 64 nodes was
+// a crossing cell, i.e. in most wavefronts, with one or two lanes active.
+__device__ __forceinline__ int mc_append_cell(bool live, long long n, uint32_t* cells, int na) {
+  int tot;
+  const int pos = petch::block_excl_scan<kMcBS>(live ? 1 : 0, tot);
+  if (live) cells[na + pos] = (uint32_t)n;
+  return na + tot;
+}
+
 // The three passes over the nodes of block `blk` of one grid (blk = blockIdx.x for a single
 // grid; the batched launches pass the block's index within its grid).
 __device__ __forceinline__ void mc_count_block(const float* __restrict__ phi, const Grid& g,
                                                long long N, long long blk, int* __restrict__ bv,
                                                int* __restrict__ bt) {
-  int vc = 0, tc = 0;
+  __shared__ uint32_t cells[kMcMaxRounds * kMcBS];
+  int vc = 0, na = 0;
   for (int r = 0; r < g.rounds; ++r) {
     const long long n = (blk * g.rounds + r) * kMcBS + threadIdx.x;
+    int s = 0;
     if (n < N) {
       int i, j, k;
       node_ijk(g, n, i, j, k);
       vc += __popc(vertex_flags(phi, g, n, i, j, k));
-      if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
-        float f[8];
-        load_cell(phi, g, n, f);
-        tc += cell_tri_count(f);
-      }
+      s = cell_signs(phi, g, n, i, j, k);
     }
+    na = mc_append_cell(s != 0 && s != 255, n, cells, na);
+  }
+  __syncthreads();
+  int tc = 0;
+  for (int e = threadIdx.x; e < na; e += kMcBS) {
+    float f[8];
+    load_cell(phi, g, (long long)cells[e], f);
+    tc += cell_tri_count(f);
   }
   int sv = petch::block_sum<kMcBS>(vc);
   int st = petch::block_sum<kMcBS>(tc);
@@ -372,18 +400,31 @@ __device__ __forceinline__ void mc_faces_block(const float* __restrict__ phi, co
                                                int* __restrict__ faces, int cap_f, long long nb,
                                                const int* __restrict__ total) {
   if (mc_block_empty(bt, blk, nb, total)) return;
-  int base = bt[blk];
+  __shared__ uint32_t cells[kMcMaxRounds * kMcBS];
+  int na = 0;
   for (int r = 0; r < g.rounds; ++r) {
     const long long n = (blk * g.rounds + r) * kMcBS + threadIdx.x;
-    float f[8];
-    int tc = 0;
+    int s = 0;
     if (n < N) {
       int i, j, k;
       node_ijk(g, n, i, j, k);
-      if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1) {
-        load_cell(phi, g, n, f);
-        tc = cell_tri_count(f);
-      }
+      s = cell_signs(phi, g, n, i, j, k);
+    }
+    na = mc_append_cell(s != 0 && s != 255, n, cells, na);
+  }
+  __syncthreads();
+  // the listed cells in node order, kMcBS at a time: each one's faces start at the block's
+  // offset plus the triangles of the listed cells before it (cells off the list have none)
+  int base = bt[blk];
+  for (int c0 = 0; c0 < na; c0 += kMcBS) {
+    const int e = c0 + threadIdx.x;
+    float f[8];
+    int tc = 0;
+    long long n = 0;
+    if (e < na) {
+      n = (long long)cells[e];
+      load_cell(phi, g, n, f);
+      tc = cell_tri_count(f);
     }
     int tot;
     const int tb = base + petch::block_excl_scan<kMcBS>(tc, tot);
