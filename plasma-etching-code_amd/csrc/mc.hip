@@ -267,9 +267,11 @@ __device__ __forceinline__ void mc_node_verts(const float* __restrict__ phi, con
                                               uint8_t* __restrict__ vflg,
                                               float* __restrict__ verts, int cap_v) {
 #pragma clang fp contract(off)
+  // only a node with a crossing edge is ever looked up (a face reads the start node of each of
+  // its crossing edges), so the others' entries are left unwritten
+  if (!fl) return;
   vbase[n] = vb;
   vflg[n] = (uint8_t)fl;
-  if (!fl) return;
   double p0 = (double)phi[n];
   long long st[3] = {g.sx, g.sy, 1};
   int r = 0;
