@@ -111,8 +111,15 @@ def _faithful(flags):
 NEUTRAL_SOURCE = ops.make_source(ops.SRC_NEUTRAL)
 
 
+def run_consts(geo, par, flags):
+    """The per-run constants of coupled_flux_steps: the tracer box, the ion source and
+    faithful-ion constants, and the chemistry constants."""
+    return dict(box=box_of(geo, par), src=ion_source(par), ion=ion_params(par),
+                chem=chem_params(par, flags))
+
+
 def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None, rounds=None,
-                       phi=None, bare_in_unit=False):
+                       phi=None, bare_in_unit=False, consts=None):
     """mc_flux_3d_coupled (threed.py:740-905) as a generator over its ray traces.
 
     It yields ("ion", request) and ("neutral", request) dicts and is sent the trace outputs
@@ -131,7 +138,9 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
 
     phi: the level set on the device, needed by the floor-charge throttle (its gas normals).
     bare_in_unit: bare_init is float32 and already in [0, 1] (a warm start from the previous
-    step's coverage), so its clip to [0, 1] is the identity and is skipped."""
+    step's coverage), so its clip to [0, 1] is the identity and is skipped.
+    consts: the run's constants (box, ion_source, ion_params, chem) computed once from the same
+    geo, par and flags (run_consts), so a loop does not rebuild them every step."""
     F = mesh.nf
     Lx, Ly = geo['Lx'], geo['Ly']
     A_src = Lx * Ly
@@ -141,7 +150,9 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
     alpha = float(par.get('flux_smooth_alpha', 1.0))
     # device-resident normalize+smooth of the reference's GPU path (threed.py:757-759, 783-790)
     dev = bool(par.get('device_flux', True)) and bool(par.get('flux_smooth_gpu', True))
-    box = box_of(geo, par)
+    if consts is None:
+        consts = run_consts(geo, par, flags)
+    box = consts['box']
     A = mesh.areas
 
     def host_smooth(x):                       # smooth_flux / smooth_flux_gpu semantics
@@ -155,7 +166,7 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
 
     ion_seed = (seed * 9 + 1) & SEED_MASK
     if _faithful(flags):
-        acc = yield ("ion", dict(bvh=mesh.bvh, box=box, src=ion_source(par), ion=ion_params(par),
+        acc = yield ("ion", dict(bvh=mesh.bvh, box=box, src=consts['src'], ion=consts['ion'],
                                  n=n_ion, seed=ion_seed))
         norm = A_src / n_ion
         if rounds is not None and n_smooth <= 1 and not _charging.active(par, flags):
@@ -188,7 +199,7 @@ def coupled_flux_steps(mesh, geo, par, n_ion, n_neu, seed, flags, bare_init=None
         m_i, ion_yield = _charging.floor_throttle(mesh, gn, geo, m_i, ion_yield)
 
     cap = par.get('cov_transport', 'rr') != 'rr'
-    chem = chem_params(par, flags)
+    chem = consts['chem']
 
     n_fp = int(par.get('n_fp', 1 if bare_init is not None else 4))
     if bare_init is None:

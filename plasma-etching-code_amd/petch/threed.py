@@ -403,6 +403,7 @@ class EtchLoop:
         self.mg = ops.MaskGeom(float(geo['sub_top']), float(mask_th), float(geo['Lx']),
                                float(geo['Ly']), float(geo['trench_width']), int(bool(geo['hole'])))
         self.chem = chem_params(par, flags)
+        self.consts = _flux.run_consts(geo, par, flags)   # the coupled flux's per-run constants
         # the ray budget of a step picks the BVH build (clustered trees pay off at ~1000 rays
         # per face): ions + F and O neutrals per fixed-point iteration
         rays = None
@@ -538,7 +539,8 @@ class EtchLoop:
                                                                n_neu_s, seed, flags,
                                                                bare_init=bi, rounds=rounds,
                                                                phi=self.phi,
-                                                               bare_in_unit=True))
+                                                               bare_in_unit=True,
+                                                               consts=self.consts))
                 return _flux.uncoupled_flux(mesh, self.geo, self.mc_par, n_ion_s, n_neu_s, seed,
                                             getattr(flags, "ion_reflection", False))
         finally:
